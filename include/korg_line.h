@@ -1,0 +1,181 @@
+/*
+ * korg_line.h -- C ABI of the MI355X (gfx950) line-opacity library.
+ *
+ * The library computes the line opacity of Korg.jl's `line_absorption!`
+ * (src/line_absorption.jl:33-129): for every line, per-layer Doppler and Lorentz widths and
+ * amplitudes, a window cut where the profile falls below cutoff_threshold * alpha_cntm
+ * (src/line_absorption.jl:108-119), and the sum of Hunger-approximation Voigt profiles
+ * (src/line_absorption.jl:245-310) added into alpha[layer, wavelength].
+ *
+ * Every entry point replaces the body of that one Julia function; the Julia side flattens its
+ * arguments (`linelist`, `λs::Wavelengths`, `temps`, `nₑ`, `n_densities`, `partition_fns`, `ξ`,
+ * `α_cntm`, `cutoff_threshold`) into the structs below.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Units: cm, s^-1, eV, K, cm^-3 (as in the reference).
+ *   - alpha is column-major [n_layers x n_wl] (Julia's Matrix): alpha[l + n_layers * j].
+ *     Lines are ADDED into alpha (src/line_absorption.jl:127); an empty line list leaves
+ *     alpha untouched (src/line_absorption.jl:35-37).
+ *   - Return value: KORG_OK (0) or a KORG_ERR_* code; korg_last_error() gives the message.
+ *     No C++ exception crosses the ABI.
+ *   - A context owns one HIP stream and its device scratch; it is not thread-safe (use one
+ *     context per host thread).  Results are deterministic: no atomics touch alpha and the
+ *     order in which lines are summed into an element is fixed.
+ */
+#ifndef KORG_LINE_H
+#define KORG_LINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KORG_LINE_ABI_VERSION 1
+
+enum korg_status {
+    KORG_OK = 0,
+    KORG_ERR_INVALID_ARGUMENT = 1,
+    KORG_ERR_HIP = 2,
+    KORG_ERR_OUT_OF_MEMORY = 3,
+    KORG_ERR_NO_DEVICE = 4,
+    KORG_ERR_UNSUPPORTED = 5,
+};
+
+/* Arithmetic of the profile evaluation. */
+enum korg_precision {
+    /* Float64 throughout, evaluating line_profile/voigt_hjerting with the reference's own
+     * expressions (src/line_absorption.jl:245-310): the device form of the Float64 oracle. */
+    KORG_PRECISION_FP64 = 0,
+    /* Float32 profile evaluation and accumulation with Float64 fix-ups: per-(line, layer)
+     * setup, window, branch selection and the wavelength offset lambda - lambda0 stay Float64. */
+    KORG_PRECISION_FP32_FIXUP = 1,
+};
+
+enum korg_dtype { KORG_DTYPE_F64 = 0, KORG_DTYPE_F32 = 1 };
+
+/* How alpha_cntm(lambda0) of each layer is supplied (src/line_absorption.jl:108). */
+enum korg_cntm_mode {
+    /* values[i * n_layers + l] = alpha_cntm of layer l at line i's wavelength */
+    KORG_CNTM_AT_LINES = 0,
+    /* values[l * n_knots + k] at knots[k] (cm, ascending); linear interpolation, the form of
+     * the interpolators synthesize builds (src/synthesize.jl:226-229) */
+    KORG_CNTM_GRID = 1,
+};
+
+/* The line list, struct of arrays (fields of Korg.Line, src/linelist.jl:5-14). */
+typedef struct {
+    int64_t n_lines;
+    const double* wl;          /* cm */
+    const double* log_gf;
+    const double* E_lower;     /* eV */
+    const double* gamma_rad;   /* s^-1 */
+    const double* gamma_stark; /* s^-1 per electron at 10^4 K */
+    const double* vdW1;        /* gamma_vdW at 10^4 K if vdW2 == -1, else ABO sigma */
+    const double* vdW2;        /* -1, or ABO alpha */
+    const int32_t* species;    /* index into korg_species */
+} korg_lines;
+
+/* Species of the lines: mass (get_mass, src/species.jl:175) and ismolecule (:168). */
+typedef struct {
+    int32_t n_species;
+    const double* mass;          /* g */
+    const uint8_t* is_molecule;  /* 1: Stark and vdW broadening are skipped */
+} korg_species;
+
+/* Atmospheric layers (the per-layer vectors of line_absorption!). */
+typedef struct {
+    int32_t n_layers;
+    const double* temps;    /* K */
+    const double* n_e;      /* cm^-3 */
+    /* [n_species * n_layers]: n_densities[s][l] / partition_fns[s](log(temps[l]))
+     * (src/line_absorption.jl:42-44) */
+    const double* n_div_U;
+    const double* n_HI;     /* perturbers for n_eff_vdW (src/line_absorption.jl:54-60) */
+    const double* n_HeI;
+    const double* n_H2;
+    const double* xi;       /* [n_layers] microturbulence, cm/s */
+} korg_layers;
+
+/* The wavelength grid: every point of a Korg.Wavelengths (its all_wls), cm, ascending. */
+typedef struct {
+    int64_t n_wl;
+    const double* all_wls;
+} korg_grid;
+
+typedef struct {
+    int32_t mode;           /* korg_cntm_mode */
+    int64_t n_knots;        /* KORG_CNTM_GRID only */
+    const double* knots;    /* KORG_CNTM_GRID only */
+    const double* values;
+} korg_continuum;
+
+/* Statistics of the last call on a context. */
+typedef struct {
+    double n_evals;          /* profile evaluations: sum over lines of n_layers*(ub-lb+1) */
+    int64_t n_lines_used;    /* lines with a non-empty window */
+    double setup_ms;         /* device time of the per-line setup kernels */
+    double bin_ms;           /* device time of the binning kernels */
+    double profile_ms;       /* device time of the profile-accumulation kernel */
+    double core_ms;          /* device time of the Voigt-core kernel (Float32 form only) */
+    double total_ms;         /* device time of the whole call (excludes host<->device copies) */
+    /* evaluations by Hunger branch (src/line_absorption.jl:288-309), filled when
+     * korg_tuning.count_branches is set: far wing a <= 0.2 (A), core a <= 0.2 (B),
+     * core 0.2 < a <= 1.4 (C), far wing / strongly damped (D) */
+    double n_evals_branch[4];
+} korg_stats;
+
+typedef struct korg_ctx korg_ctx;
+
+/* Create a context on HIP device `device` (its own stream, scratch grown on demand). */
+int korg_create(korg_ctx** out, int device);
+void korg_destroy(korg_ctx* ctx);
+/* Message of the last error on ctx (never NULL; "" when none). ctx may be NULL. */
+const char* korg_last_error(const korg_ctx* ctx);
+int korg_abi_version(void);
+
+/*
+ * Drop-in body of line_absorption! (src/line_absorption.jl:33-129) on HOST memory.
+ * alpha: host pointer, column-major [n_layers x n_wl], dtype korg_dtype; updated in place.
+ * All other pointers are host pointers.  Synchronous: returns after alpha is updated.
+ */
+int korg_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* grid,
+                         const korg_lines* lines, const korg_species* species,
+                         const korg_layers* layers, const korg_continuum* cntm,
+                         double cutoff_threshold, int precision);
+
+/*
+ * The same on DEVICE memory: every pointer in the structs and alpha are device pointers
+ * (HBM-resident inputs).  Runs on the context's stream; returns without synchronising when
+ * `synchronize` is 0 (use korg_synchronize).
+ */
+int korg_line_absorption_device(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                const korg_grid* grid, const korg_lines* lines,
+                                const korg_species* species, const korg_layers* layers,
+                                const korg_continuum* cntm, double cutoff_threshold,
+                                int precision, int synchronize);
+
+/* Use an existing HIP stream (hipStream_t passed as void*) instead of the context's own. */
+int korg_set_stream(korg_ctx* ctx, void* stream);
+int korg_synchronize(korg_ctx* ctx);
+int korg_get_stats(const korg_ctx* ctx, korg_stats* out);
+
+/*
+ * Per-line windows of the last call, 1-based as the reference's lb/ub
+ * (src/line_absorption.jl:114-115); lb > ub marks a skipped line.  Host arrays of n_lines.
+ */
+int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines);
+
+/* Kernel tuning (0 = library default / autotuned choice). */
+typedef struct {
+    int32_t points_per_lane;   /* wavelength points per lane: 1 or 2 */
+    int32_t waves_per_tile;    /* reserved, 0 */
+    int32_t waves_per_cu;      /* persistent waves launched per compute unit */
+    int32_t count_branches;    /* 1: fill korg_stats.n_evals_branch (extra setup cost) */
+} korg_tuning;
+int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KORG_LINE_H */

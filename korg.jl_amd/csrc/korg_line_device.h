@@ -1,0 +1,160 @@
+// Device-side physics of the line-opacity path (Float64), restating Korg.jl's
+// src/line_absorption.jl with the same expressions and operation order, plus the Float32
+// far-wing forms used by the mixed-precision kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace korg {
+
+// src/constants.jl:3-19
+constexpr double kboltz_cgs = 1.380649e-16;
+constexpr double c_cgs = 2.99792458e10;
+constexpr double electron_mass_cgs = 9.1093897e-28;
+constexpr double electron_charge_cgs = 4.80320425e-10;
+constexpr double amu_cgs = 1.6605402e-24;
+constexpr double kboltz_eV = 8.617333262145e-5;
+constexpr double hplanck_eV = 4.135667696e-15;
+constexpr double PI = 3.141592653589793;
+
+// Wavelength offsets in the Float32 kernel are measured in this unit (cm): 0.01 A.
+constexpr double DELTA_UNIT = 1e-10;
+
+// Per-line result of the setup pass.  Indices are 0-based global grid indices; lb > ub marks a
+// line whose window holds no grid point (skipped, src/line_absorption.jl:117-119).
+struct LineMeta {
+    double lam0;   // line centre, cm
+    int32_t lb;    // first grid index in the window
+    int32_t ub;    // last grid index in the window
+    int32_t clo;   // grid-index range that may hold a Voigt-core evaluation in some layer
+    int32_t chi;   //   (clo > chi: none)
+    int32_t jc;    // first grid index with lambda >= lam0 (unclamped; monotone in lam0)
+    int32_t cls;   // window-width class, or N_CLASSES when skipped
+};
+static_assert(sizeof(LineMeta) == 32, "LineMeta layout");
+
+// Window-width classes used to bin lines onto wavelength tiles: class c holds lines whose window
+// has at most CLASS_WMAX[c] points.
+constexpr int N_CLASSES = 6;
+__host__ __device__ inline int64_t class_wmax(int c) {
+    return c == 0 ? 64 : c == 1 ? 512 : c == 2 ? 4096 : c == 3 ? 32768 : c == 4 ? 262144
+                                                                                : (int64_t)1 << 40;
+}
+
+// Float32 far-wing forms, per (line, layer):
+//   a <= 0.2: phi = d1 R + d2 R^2 + d3 R^3,  R = 1/delta^2 (shared by all layers)
+//   a >  0.2: phi = sum_{k=1..5} c_k rho^k,  rho = 1/(delta^2/g2 + 1)
+
+// --------------------------------------------------------------------------------------------
+// Float64 reference expressions (device restatement of src/line_absorption.jl)
+// --------------------------------------------------------------------------------------------
+
+// src/line_absorption.jl:140-145
+__device__ inline double inverse_gaussian_density(double rho, double sigma) {
+    if (rho > 1.0 / (sqrt(2.0 * PI) * sigma)) return 0.0;
+    return sigma * sqrt(-2.0 * log(sqrt(2.0 * PI) * sigma * rho));
+}
+
+// src/line_absorption.jl:156-161
+__device__ inline double inverse_lorentz_density(double rho, double gamma) {
+    if (rho > 1.0 / (PI * gamma)) return 0.0;
+    return sqrt(gamma / (PI * rho) - gamma * gamma);
+}
+
+// src/line_absorption.jl:190
+__device__ inline double doppler_width(double lambda0, double T, double m, double xi) {
+    return lambda0 * sqrt(kboltz_cgs * T / m + (xi * xi) / 2.0) / c_cgs;
+}
+
+// src/line_absorption.jl:208-221, ABO branch (the simple branch uses a per-layer table)
+__device__ inline double scaled_vdW_abo(double sigma, double alpha, double m, double T) {
+    const double v0 = 1e6;
+    const double invmu = 1.0 / (1.008 * amu_cgs) + 1.0 / m;
+    const double vbar = sqrt(8.0 * kboltz_cgs * T / PI * invmu);
+    return 2.0 * pow(4.0 / PI, alpha / 2.0) * tgamma((4.0 - alpha) / 2.0) * v0 * sigma *
+           pow(vbar / v0, 1.0 - alpha);
+}
+
+// src/line_absorption.jl:229-237
+__device__ inline double sigma_line(double lambda) {
+    const double e = electron_charge_cgs, me = electron_mass_cgs, c = c_cgs;
+    return (PI * (e * e) / me / c) * ((lambda * lambda) / c);
+}
+
+// src/line_absorption.jl:251-265
+__device__ inline double harris_series64(double v, double& H1, double& H2) {
+    const double v2 = v * v;
+    const double H0 = exp(-v2);
+    if (v < 1.3)
+        H1 = -1.12470432 + (-0.15516677 + (3.288675912 + (-2.34357915 + 0.42139162 * v) * v) * v) * v;
+    else if (v < 2.4)
+        H1 = -4.48480194 + (9.39456063 + (-6.61487486 + (1.98919585 - 0.22041650 * v) * v) * v) * v;
+    else
+        H1 = ((0.554153432 +
+               (0.278711796 + (-0.1883256872 + (0.042991293 - 0.003278278 * v) * v) * v) * v) /
+              (v2 - 3.0 / 2.0));
+    H2 = (1.0 - 2.0 * v2) * H0;
+    return H0;
+}
+
+// src/line_absorption.jl:286-310
+__device__ inline double voigt_hjerting64(double a, double v) {
+    const double sqrt_pi = 1.7724538509055159;  // sqrt(pi) in Float64
+    const double v2 = v * v;
+    if (a <= 0.2 && v >= 5.0) {
+        const double invv2 = 1.0 / v2;
+        return (a / sqrt_pi * invv2) * (1.0 + 1.5 * invv2 + 3.75 * (invv2 * invv2));
+    } else if (a <= 0.2) {
+        double H1, H2;
+        const double H0 = harris_series64(v, H1, H2);
+        return H0 + (H1 + H2 * a) * a;
+    } else if (a <= 1.4 && a + v < 3.2) {
+        double H1, H2;
+        const double H0 = harris_series64(v, H1, H2);
+        const double M0 = H0;
+        const double M1 = H1 + 2.0 / sqrt_pi * M0;
+        const double M2 = H2 - M0 + 2.0 / sqrt_pi * M1;
+        const double M3 = 2.0 / (3.0 * sqrt_pi) * (1.0 - H2) - (2.0 / 3.0) * v2 * M1 + (2.0 / sqrt_pi) * M2;
+        const double M4 = 2.0 / 3.0 * v2 * v2 * M0 - 2.0 / (3.0 * sqrt_pi) * M1 + 2.0 / sqrt_pi * M3;
+        const double psi = 0.979895023 + (-0.962846325 + (0.532770573 - 0.122727278 * a) * a) * a;
+        return psi * (M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a);
+    } else {
+        const double r2 = v2 / (a * a);
+        const double a_invu = 0.7071067811865475 / ((r2 + 1.0) * a);  // 1/sqrt(2)/((r2+1)a)
+        const double a2_invu2 = a_invu * a_invu;
+        return 0.7978845608028654 * a_invu *  // sqrt(2/pi)
+               (1.0 + (3.0 * r2 - 1.0 + ((r2 - 2.0) * 15.0 * r2 + 2.0) * a2_invu2) * a2_invu2);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// Float32 Voigt core (branches B and C of voigt_hjerting), branch choices made by the caller
+// in Float64.  piece: 0 (v < 1.3), 1 (v < 2.4), 2 (otherwise), decided on the Float64 v.
+// --------------------------------------------------------------------------------------------
+__device__ inline float harris_h1_f32(float v, float v2, int piece) {
+    if (piece == 0)
+        return -1.12470432f + (-0.15516677f + (3.288675912f + (-2.34357915f + 0.42139162f * v) * v) * v) * v;
+    if (piece == 1)
+        return -4.48480194f + (9.39456063f + (-6.61487486f + (1.98919585f - 0.22041650f * v) * v) * v) * v;
+    return (0.554153432f + (0.278711796f + (-0.1883256872f + (0.042991293f - 0.003278278f * v) * v) * v) * v) *
+           __builtin_amdgcn_rcpf(v2 - 1.5f);
+}
+
+// Branch B (a <= 0.2, v < 5) or C (0.2 < a <= 1.4, a + v < 3.2) in Float32.
+__device__ inline float voigt_core_f32(float a, float v, int piece, bool branch_c) {
+    const float v2 = v * v;
+    const float H0 = __expf(-v2);
+    const float H1 = harris_h1_f32(v, v2, piece);
+    const float H2 = (1.0f - 2.0f * v2) * H0;
+    if (!branch_c) return H0 + (H1 + H2 * a) * a;
+    const float isp = 0.5641895835477563f;  // 1/sqrt(pi)
+    const float M0 = H0;
+    const float M1 = H1 + 2.0f * isp * M0;
+    const float M2 = H2 - M0 + 2.0f * isp * M1;
+    const float M3 = (2.0f / 3.0f) * isp * (1.0f - H2) - (2.0f / 3.0f) * v2 * M1 + 2.0f * isp * M2;
+    const float M4 = (2.0f / 3.0f) * v2 * v2 * M0 - (2.0f / 3.0f) * isp * M1 + 2.0f * isp * M3;
+    const float psi = 0.979895023f + (-0.962846325f + (0.532770573f - 0.122727278f * a) * a) * a;
+    return psi * (M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a);
+}
+
+}  // namespace korg

@@ -1,0 +1,122 @@
+"""ctypes binding of ``libkorgline.so`` (the C ABI declared in ``include/korg_line.h``).
+
+The library is built in-tree (``korg.jl_amd/libkorgline.so``).  Loading fails loudly when it is
+missing: there is no CPU fallback for the line-opacity path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "libkorgline.so")
+
+KORG_OK = 0
+KORG_ERR_INVALID_ARGUMENT = 1
+KORG_ERR_HIP = 2
+KORG_ERR_OUT_OF_MEMORY = 3
+KORG_ERR_NO_DEVICE = 4
+KORG_ERR_UNSUPPORTED = 5
+
+PRECISION_FP64 = 0
+PRECISION_FP32_FIXUP = 1
+DTYPE_F64 = 0
+DTYPE_F32 = 1
+CNTM_AT_LINES = 0
+CNTM_GRID = 1
+
+_c_dp = ctypes.c_void_p  # all array pointers are passed as raw addresses
+
+
+class KorgLines(ctypes.Structure):
+    _fields_ = [("n_lines", ctypes.c_int64), ("wl", _c_dp), ("log_gf", _c_dp),
+                ("E_lower", _c_dp), ("gamma_rad", _c_dp), ("gamma_stark", _c_dp),
+                ("vdW1", _c_dp), ("vdW2", _c_dp), ("species", _c_dp)]
+
+
+class KorgSpecies(ctypes.Structure):
+    _fields_ = [("n_species", ctypes.c_int32), ("mass", _c_dp), ("is_molecule", _c_dp)]
+
+
+class KorgLayers(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int32), ("temps", _c_dp), ("n_e", _c_dp),
+                ("n_div_U", _c_dp), ("n_HI", _c_dp), ("n_HeI", _c_dp), ("n_H2", _c_dp),
+                ("xi", _c_dp)]
+
+
+class KorgGrid(ctypes.Structure):
+    _fields_ = [("n_wl", ctypes.c_int64), ("all_wls", _c_dp)]
+
+
+class KorgContinuum(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("n_knots", ctypes.c_int64), ("knots", _c_dp),
+                ("values", _c_dp)]
+
+
+class KorgStats(ctypes.Structure):
+    _fields_ = [("n_evals", ctypes.c_double), ("n_lines_used", ctypes.c_int64),
+                ("setup_ms", ctypes.c_double), ("bin_ms", ctypes.c_double),
+                ("profile_ms", ctypes.c_double), ("core_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4)]
+
+
+class KorgTuning(ctypes.Structure):
+    _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
+
+
+EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
+            "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
+            "korg_synchronize", "korg_get_stats", "korg_get_windows", "korg_set_tuning"]
+
+_lib = None
+
+
+class KorgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"korg_line error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load the library (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                          f"`make -C {PKG_ROOT}`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    L.korg_create.argtypes = [ctypes.POINTER(vp), i32]
+    L.korg_create.restype = i32
+    L.korg_destroy.argtypes = [vp]
+    L.korg_destroy.restype = None
+    L.korg_last_error.argtypes = [vp]
+    L.korg_last_error.restype = ctypes.c_char_p
+    L.korg_abi_version.argtypes = []
+    L.korg_abi_version.restype = i32
+    common = [vp, vp, i32, ctypes.POINTER(KorgGrid), ctypes.POINTER(KorgLines),
+              ctypes.POINTER(KorgSpecies), ctypes.POINTER(KorgLayers),
+              ctypes.POINTER(KorgContinuum), d, i32]
+    L.korg_line_absorption.argtypes = common
+    L.korg_line_absorption.restype = i32
+    L.korg_line_absorption_device.argtypes = common + [i32]
+    L.korg_line_absorption_device.restype = i32
+    L.korg_set_stream.argtypes = [vp, vp]
+    L.korg_set_stream.restype = i32
+    L.korg_synchronize.argtypes = [vp]
+    L.korg_synchronize.restype = i32
+    L.korg_get_stats.argtypes = [vp, ctypes.POINTER(KorgStats)]
+    L.korg_get_stats.restype = i32
+    L.korg_get_windows.argtypes = [vp, vp, vp, i64]
+    L.korg_get_windows.restype = i32
+    L.korg_set_tuning.argtypes = [vp, ctypes.POINTER(KorgTuning)]
+    L.korg_set_tuning.restype = i32
+    _lib = L
+    return L
+
+
+def check(ctx, rc):
+    if rc != KORG_OK:
+        msg = load().korg_last_error(ctx)
+        raise KorgError(rc, msg.decode() if msg else "")

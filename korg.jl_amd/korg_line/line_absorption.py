@@ -1,0 +1,205 @@
+"""Host-side mirror of Korg.jl's ``line_absorption!`` over the device library.
+
+``line_absorption_(alpha, linelist, wls, temps, n_e, n_densities, partition_fns, xi, alpha_cntm;
+cutoff_threshold)`` has the argument meaning of ``src/line_absorption.jl:33-34``: it ADDS the
+line opacity [cm^-1] into ``alpha`` (shape (n_layers, n_wavelengths), Julia's column-major
+layout: pass a Fortran-ordered array, or the transposed view of a C-ordered one), leaves it
+untouched for an empty line list, and logs an error (continuing) if H I is in the line list.
+
+Device-resident use (inputs already in HBM, e.g. torch tensors on ``cuda``) goes through
+``Context.line_absorption_device`` with a ``DeviceProblem``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .problem import Problem
+
+PRECISIONS = {"fp64": _lib.PRECISION_FP64, "fp32": _lib.PRECISION_FP32_FIXUP,
+              "fp32_fixup": _lib.PRECISION_FP32_FIXUP}
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _host_structs(P):
+    ll = P.lines
+    keep = [P.all_wls, ll.wl, ll.log_gf, ll.E_lower, ll.gamma_rad, ll.gamma_stark, ll.vdW1,
+            ll.vdW2, ll.species_index, P.species_mass, P.species_is_mol, P.n_div_U, P.temps,
+            P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi, P.cntm_values, P.cntm_knots]
+    for a in keep:
+        assert a.flags.c_contiguous
+    g = _lib.KorgGrid(P.n_wl, P.all_wls.ctypes.data)
+    lines = _lib.KorgLines(len(ll), ll.wl.ctypes.data, ll.log_gf.ctypes.data,
+                           ll.E_lower.ctypes.data, ll.gamma_rad.ctypes.data,
+                           ll.gamma_stark.ctypes.data, ll.vdW1.ctypes.data, ll.vdW2.ctypes.data,
+                           ll.species_index.ctypes.data)
+    sp = _lib.KorgSpecies(P.species_mass.size, P.species_mass.ctypes.data,
+                          P.species_is_mol.ctypes.data)
+    ly = _lib.KorgLayers(P.n_layers, P.temps.ctypes.data, P.n_e.ctypes.data,
+                         P.n_div_U.ctypes.data, P.n_HI.ctypes.data, P.n_HeI.ctypes.data,
+                         P.n_H2.ctypes.data, P.xi.ctypes.data)
+    cn = _lib.KorgContinuum(P.cntm_mode, P.n_knots,
+                            P.cntm_knots.ctypes.data if P.n_knots else None,
+                            P.cntm_values.ctypes.data)
+    return (g, lines, sp, ly, cn), keep
+
+
+class DeviceProblem:
+    """A ``Problem`` uploaded once to device memory (torch tensors on ``device``)."""
+
+    def __init__(self, problem, device="cuda"):
+        import torch
+        P = problem
+        ll = P.lines
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+        self.problem = P
+        self.t = {k: up(v) for k, v in {
+            "all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
+            "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
+            "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
+            "is_mol": P.species_is_mol, "n_div_U": P.n_div_U, "temps": P.temps, "n_e": P.n_e,
+            "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
+            "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}.items()}
+        t = self.t
+        d = lambda k: t[k].data_ptr()
+        self.grid = _lib.KorgGrid(P.n_wl, d("all_wls"))
+        self.lines = _lib.KorgLines(len(ll), d("wl"), d("log_gf"), d("E_lower"), d("gamma_rad"),
+                                    d("gamma_stark"), d("vdW1"), d("vdW2"), d("species"))
+        self.species = _lib.KorgSpecies(P.species_mass.size, d("mass"), d("is_mol"))
+        self.layers = _lib.KorgLayers(P.n_layers, d("temps"), d("n_e"), d("n_div_U"), d("n_HI"),
+                                      d("n_HeI"), d("n_H2"), d("xi"))
+        self.cntm = _lib.KorgContinuum(P.cntm_mode, P.n_knots, d("knots") if P.n_knots else None,
+                                       d("cntm"))
+
+    def new_alpha(self, dtype="float32", fill=None):
+        """Device alpha in the library's layout: a C-ordered (n_wl, n_layers) tensor is the
+        column-major (n_layers, n_wl) matrix."""
+        import torch
+        P = self.problem
+        dt = torch.float32 if dtype in ("float32", "f32") else torch.float64
+        a = torch.zeros((P.n_wl, P.n_layers), dtype=dt, device=self.t["wl"].device)
+        if fill is not None:
+            a.copy_(torch.as_tensor(np.ascontiguousarray(np.asarray(fill).T), dtype=dt))
+        return a
+
+
+class Context:
+    """One library context (HIP stream + device scratch) on one device."""
+
+    def __init__(self, device=0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = L.korg_create(ctypes.byref(h), int(device))
+        if rc != _lib.KORG_OK:
+            raise _lib.KorgError(rc, "korg_create failed (no HIP device?)")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if self._h:
+            self._L.korg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tuning(self, points_per_lane=0, waves_per_cu=0, count_branches=0):
+        t = _lib.KorgTuning(points_per_lane, 0, waves_per_cu, count_branches)
+        _lib.check(self._h, self._L.korg_set_tuning(self._h, ctypes.byref(t)))
+
+    def set_stream(self, stream_handle):
+        _lib.check(self._h, self._L.korg_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def stats(self):
+        s = _lib.KorgStats()
+        _lib.check(self._h, self._L.korg_get_stats(self._h, ctypes.byref(s)))
+        out = {k: getattr(s, k) for k, _ in s._fields_}
+        out["n_evals_branch"] = list(s.n_evals_branch)
+        return out
+
+    def windows(self, n_lines):
+        lb = np.zeros(n_lines, dtype=np.int64)
+        ub = np.zeros(n_lines, dtype=np.int64)
+        _lib.check(self._h, self._L.korg_get_windows(self._h, _ptr(lb), _ptr(ub), n_lines))
+        return lb, ub
+
+    def line_absorption(self, alpha, problem, precision="fp64"):
+        """Host arrays: ``alpha`` float64/float32 (n_layers, n_wl), Fortran-ordered, updated."""
+        if not isinstance(problem, Problem):
+            raise TypeError("problem must be a korg_line.Problem")
+        if alpha.shape != (problem.n_layers, problem.n_wl):
+            raise ValueError(f"alpha has shape {alpha.shape}, expected "
+                             f"{(problem.n_layers, problem.n_wl)}")
+        if not alpha.flags.f_contiguous:
+            raise ValueError("alpha must be column-major (Fortran order), like Julia's Matrix")
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        if alpha.dtype not in dt:
+            raise TypeError("alpha must be float64 or float32")
+        structs, keep = _host_structs(problem)
+        g, lines, sp, ly, cn = structs
+        rc = self._L.korg_line_absorption(self._h, _ptr(alpha), dt[alpha.dtype], ctypes.byref(g),
+                                          ctypes.byref(lines), ctypes.byref(sp), ctypes.byref(ly),
+                                          ctypes.byref(cn), problem.cutoff_threshold,
+                                          PRECISIONS[precision])
+        del keep
+        _lib.check(self._h, rc)
+
+    def line_absorption_device(self, alpha_dev, dprob, precision="fp32", synchronize=True):
+        """Device arrays: ``alpha_dev`` a torch tensor from ``DeviceProblem.new_alpha``."""
+        import torch
+        dt = {torch.float64: _lib.DTYPE_F64, torch.float32: _lib.DTYPE_F32}[alpha_dev.dtype]
+        P = dprob.problem
+        if tuple(alpha_dev.shape) != (P.n_wl, P.n_layers) or not alpha_dev.is_contiguous():
+            raise ValueError("alpha_dev must be a contiguous (n_wl, n_layers) device tensor")
+        rc = self._L.korg_line_absorption_device(
+            self._h, ctypes.c_void_p(alpha_dev.data_ptr()), dt, ctypes.byref(dprob.grid),
+            ctypes.byref(dprob.lines), ctypes.byref(dprob.species), ctypes.byref(dprob.layers),
+            ctypes.byref(dprob.cntm), P.cutoff_threshold, PRECISIONS[precision],
+            1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
+    def synchronize(self):
+        _lib.check(self._h, self._L.korg_synchronize(self._h))
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def line_absorption_(alpha, linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                     alpha_cntm, cutoff_threshold=3e-4, precision="fp64", ctx=None):
+    """``line_absorption!`` (``src/line_absorption.jl:33-129``) on the GPU; alpha is updated.
+
+    ``precision``: "fp64" (the reference's Float64 arithmetic) or "fp32" (Float32 profiles
+    with Float64 fix-ups).  Unsorted line lists are sorted by wavelength first, as
+    ``synthesize`` does (``src/synthesize.jl:191-195``).
+    """
+    if len(linelist) == 0:
+        return None  # src/line_absorption.jl:35-37
+    if not linelist.is_sorted():
+        linelist = linelist.sorted()
+    P = Problem.from_korg_args(linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                               alpha_cntm, cutoff_threshold)
+    (ctx or default_context()).line_absorption(alpha, P, precision)
+    return None

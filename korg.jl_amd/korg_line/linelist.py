@@ -1,0 +1,265 @@
+"""Spectral lines and Korg's default broadening recipes.
+
+``Line`` restates the Korg ``Line`` constructor (``src/linelist.jl:56-100``): unit conversion of the
+wavelength, imputation of missing broadening parameters by ``approximate_radiative_gamma``
+(``src/linelist.jl:124-129``) and ``approximate_gammas`` (``src/linelist.jl:149-183``), and the
+decoding of the van der Waals parameter (log gamma / Unsoeld fudge factor / packed ABO).
+
+``LineList`` is the struct-of-arrays form handed to the C ABI; ``LineList.from_arrays`` applies the
+same recipes element-wise with numpy (same IEEE operation order as the scalar code), which is how
+the large synthetic lists are built.
+"""
+import math
+import os
+
+import numpy as np
+
+from .constants import (bohr_radius_cgs, c_cgs, electron_charge_cgs, electron_mass_cgs,
+                        hplanck_eV, kboltz_cgs, RydbergH_eV, Rydberg_eV)
+from .species import Species
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def _load_ionization_energies():
+    table = {}
+    with open(os.path.join(_DATA, "ionization_energies.tsv")) as f:
+        for row in f:
+            if row.startswith("#") or not row.strip():
+                continue
+            t = row.split()
+            table[int(t[0])] = (float(t[2]), float(t[3]), float(t[4]))
+    return table
+
+
+ionization_energies = _load_ionization_energies()
+
+_PI = math.pi
+_8PI2 = 8 * (_PI * _PI)  # Julia's 8π^2
+
+
+def approximate_radiative_gamma(wl, log_gf):
+    """Classical radiative damping, ``src/linelist.jl:124-129`` (wl in cm)."""
+    e, m, c = electron_charge_cgs, electron_mass_cgs, c_cgs
+    return _8PI2 * (e * e) / (m * c * (wl * wl)) * 10.0 ** log_gf
+
+
+def approximate_gammas(wl, species, E_lower, ionization_energies=ionization_energies):
+    """Cowley (1971) Stark and Unsoeld vdW at 10^4 K, ``src/linelist.jl:149-183``.
+
+    Returns ``(gamma_stark, log10(gamma_vdW))``; ``log10(gamma_vdW)`` is 0.0 for autoionizing lines
+    (E_upper > chi), and both are 0 for molecules and for ionization stages above III.
+    """
+    Z = species.charge + 1
+    if species.ismolecule() or Z > 3:
+        return 0.0, 0.0
+    chi = ionization_energies[species.get_atoms()[0]][Z - 1]
+    c, h, k = c_cgs, hplanck_eV, kboltz_cgs
+    E_upper = E_lower + (h * c / wl)
+    nstar4_upper = (float(Z * Z) * RydbergH_eV / (chi - E_upper)) ** 2
+    if Z == 1:
+        gamma_stark = 2.25910152e-7 * nstar4_upper
+    else:
+        gamma_stark = 5.42184365e-7 * nstar4_upper / float((Z + 1) * (Z + 1))
+    drbar2 = (5 / 2) * (Rydberg_eV * Rydberg_eV) * float(Z * Z) * \
+        (1 / (chi - E_upper) ** 2 - 1 / (chi - E_lower) ** 2)
+    if chi < E_upper:
+        log_gamma_vdW = 0.0
+    else:
+        log_gamma_vdW = 6.33 + 0.4 * math.log10(drbar2) + 0.3 * math.log10(10_000) + math.log10(k)
+    return gamma_stark, log_gamma_vdW
+
+
+def _is_missing(x):
+    return x is None or (isinstance(x, float) and math.isnan(x))
+
+
+class Line:
+    """One spectral line (``src/linelist.jl:5-100``).
+
+    ``wl`` in cm (converted from Angstrom when >= 1), ``log_gf``, ``species`` (a ``Species`` or a
+    code string), ``E_lower`` in eV, ``gamma_rad``/``gamma_stark`` in s^-1 (FWHM, per perturber
+    for Stark) and ``vdW`` as log10(gamma_vdW) (< 0), 0 (none), an Unsoeld fudge factor (0-20),
+    packed ABO (>= 20) or an ``(sigma, alpha)`` tuple.  Missing values (None/NaN) are imputed.
+    """
+    __slots__ = ("wl", "log_gf", "species", "E_lower", "gamma_rad", "gamma_stark", "vdW")
+
+    def __init__(self, wl, log_gf, species, E_lower, gamma_rad=None, gamma_stark=None, vdW=None):
+        if not isinstance(species, Species):
+            species = Species(species)
+        wl = float(wl)
+        if wl >= 1:
+            wl *= 1e-8
+        if _is_missing(gamma_stark) or _is_missing(vdW):
+            stark_approx, vdW_approx = approximate_gammas(wl, species, E_lower)
+            if _is_missing(gamma_stark):
+                gamma_stark = stark_approx
+            if _is_missing(vdW):
+                vdW = vdW_approx
+        if _is_missing(gamma_rad):
+            gamma_rad = approximate_radiative_gamma(wl, log_gf)
+        if not isinstance(vdW, tuple):
+            vdW = float(vdW)
+            if vdW < 0:
+                vdW = (10.0 ** vdW, -1.0)
+            elif vdW == 0:
+                vdW = (0.0, -1.0)
+            elif 0 < vdW < 20:
+                vdW = (vdW * 10.0 ** approximate_gammas(wl, species, E_lower)[1], -1.0)
+            else:
+                fl = math.floor(vdW)
+                vdW = (fl * bohr_radius_cgs * bohr_radius_cgs, vdW - fl)
+        self.wl = wl
+        self.log_gf = float(log_gf)
+        self.species = species
+        self.E_lower = float(E_lower)
+        self.gamma_rad = float(gamma_rad)
+        self.gamma_stark = float(gamma_stark)
+        self.vdW = (float(vdW[0]), float(vdW[1]))
+
+    def __repr__(self):
+        return (f"{self.species!r} {self.wl * 1e8:.6f} Å (log gf = {self.log_gf:.2f}, "
+                f"χ = {self.E_lower:.2f} eV)")
+
+
+class LineList:
+    """Struct-of-arrays line list: the layout the C ABI consumes.
+
+    Arrays (length ``n``): ``wl`` [cm], ``log_gf``, ``E_lower`` [eV], ``gamma_rad``,
+    ``gamma_stark``, ``vdW1``, ``vdW2`` (``vdW2 == -1`` marks a plain gamma_vdW at 10^4 K in
+    ``vdW1``; otherwise ``(vdW1, vdW2)`` are ABO (sigma, alpha)), ``species_index`` (int32 into
+    ``species``).
+    """
+
+    def __init__(self, wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2, species_index,
+                 species):
+        self.wl = np.ascontiguousarray(wl, dtype=np.float64)
+        self.log_gf = np.ascontiguousarray(log_gf, dtype=np.float64)
+        self.E_lower = np.ascontiguousarray(E_lower, dtype=np.float64)
+        self.gamma_rad = np.ascontiguousarray(gamma_rad, dtype=np.float64)
+        self.gamma_stark = np.ascontiguousarray(gamma_stark, dtype=np.float64)
+        self.vdW1 = np.ascontiguousarray(vdW1, dtype=np.float64)
+        self.vdW2 = np.ascontiguousarray(vdW2, dtype=np.float64)
+        self.species_index = np.ascontiguousarray(species_index, dtype=np.int32)
+        self.species = list(species)
+
+    def __len__(self):
+        return self.wl.size
+
+    @classmethod
+    def from_lines(cls, lines):
+        species, index = [], {}
+        idx = []
+        for l in lines:
+            if l.species not in index:
+                index[l.species] = len(species)
+                species.append(l.species)
+            idx.append(index[l.species])
+        return cls([l.wl for l in lines], [l.log_gf for l in lines], [l.E_lower for l in lines],
+                   [l.gamma_rad for l in lines], [l.gamma_stark for l in lines],
+                   [l.vdW[0] for l in lines], [l.vdW[1] for l in lines], idx, species)
+
+    def to_lines(self):
+        out = []
+        for i in range(len(self)):
+            l = Line.__new__(Line)
+            l.wl, l.log_gf = float(self.wl[i]), float(self.log_gf[i])
+            l.species = self.species[self.species_index[i]]
+            l.E_lower = float(self.E_lower[i])
+            l.gamma_rad, l.gamma_stark = float(self.gamma_rad[i]), float(self.gamma_stark[i])
+            l.vdW = (float(self.vdW1[i]), float(self.vdW2[i]))
+            out.append(l)
+        return out
+
+    def take(self, idx):
+        idx = np.asarray(idx)
+        return LineList(self.wl[idx], self.log_gf[idx], self.E_lower[idx], self.gamma_rad[idx],
+                        self.gamma_stark[idx], self.vdW1[idx], self.vdW2[idx],
+                        self.species_index[idx], self.species)
+
+    def is_sorted(self):
+        return bool(np.all(self.wl[1:] >= self.wl[:-1]))
+
+    def sorted(self):
+        """Stable sort by wavelength, as ``synthesize`` does (``src/synthesize.jl:191-195``)."""
+        return self.take(np.argsort(self.wl, kind="stable"))
+
+    @classmethod
+    def from_arrays(cls, wl, log_gf, species, species_index, E_lower, gamma_rad=None,
+                    gamma_stark=None, vdW=None):
+        """Vectorised ``Line`` construction; ``None``/NaN entries are imputed like ``Line``."""
+        wl = np.asarray(wl, dtype=np.float64).copy()
+        n = wl.size
+        wl = np.where(wl >= 1, wl * 1e-8, wl)
+        log_gf = np.asarray(log_gf, dtype=np.float64)
+        E_lower = np.asarray(E_lower, dtype=np.float64)
+        species_index = np.asarray(species_index, dtype=np.int32)
+
+        def _arr(x):
+            return np.full(n, np.nan) if x is None else np.asarray(x, dtype=np.float64).copy()
+
+        gamma_rad, gamma_stark, vdW = _arr(gamma_rad), _arr(gamma_stark), _arr(vdW)
+        need = np.isnan(gamma_stark) | np.isnan(vdW)
+        stark_a, vdw_a = _approximate_gammas_vec(wl, species, species_index, E_lower)
+        gamma_stark = np.where(need & np.isnan(gamma_stark), stark_a, gamma_stark)
+        vdW = np.where(need & np.isnan(vdW), vdw_a, vdW)
+        gamma_rad = np.where(np.isnan(gamma_rad), _approximate_radiative_gamma_vec(wl, log_gf),
+                             gamma_rad)
+        vdW1 = np.empty(n)
+        vdW2 = np.full(n, -1.0)
+        neg, zero = vdW < 0, vdW == 0
+        fudge = (vdW > 0) & (vdW < 20)
+        abo = vdW >= 20
+        vdW1[neg] = _pow10(vdW[neg])
+        vdW1[zero] = 0.0
+        vdW1[fudge] = vdW[fudge] * _pow10(vdw_a[fudge])
+        fl = np.floor(vdW[abo])
+        vdW1[abo] = fl * bohr_radius_cgs * bohr_radius_cgs
+        vdW2[abo] = vdW[abo] - fl
+        return cls(wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2, species_index, species)
+
+
+def _pow10(x):
+    """Element-wise 10.0 ** x with the same libm pow as the scalar ``Line`` path."""
+    return np.array([10.0 ** v for v in np.asarray(x, dtype=np.float64).tolist()])
+
+
+def _log10(x):
+    return np.array([math.log10(v) if v > 0 else (-math.inf if v == 0 else math.nan)
+                     for v in np.asarray(x, dtype=np.float64).tolist()])
+
+
+def _approximate_radiative_gamma_vec(wl, log_gf):
+    e, m, c = electron_charge_cgs, electron_mass_cgs, c_cgs
+    return _8PI2 * (e * e) / (m * c * (wl * wl)) * _pow10(log_gf)
+
+
+def _approximate_gammas_vec(wl, species, species_index, E_lower):
+    """Element-wise ``approximate_gammas`` with the scalar version's operation order."""
+    n = wl.size
+    stark = np.zeros(n)
+    logvdw = np.zeros(n)
+    c, h, k = c_cgs, hplanck_eV, kboltz_cgs
+    log10_k = math.log10(k)
+    c03 = 0.3 * math.log10(10_000)
+    for si, sp in enumerate(species):
+        sel = species_index == si
+        if not np.any(sel):
+            continue
+        Z = sp.charge + 1
+        if sp.ismolecule() or Z > 3:
+            continue
+        chi = ionization_energies[sp.get_atoms()[0]][Z - 1]
+        El = E_lower[sel]
+        Eu = El + (h * c / wl[sel])
+        nstar4 = (float(Z * Z) * RydbergH_eV / (chi - Eu)) ** 2
+        if Z == 1:
+            stark[sel] = 2.25910152e-7 * nstar4
+        else:
+            stark[sel] = 5.42184365e-7 * nstar4 / float((Z + 1) * (Z + 1))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            drbar2 = (5 / 2) * (Rydberg_eV * Rydberg_eV) * float(Z * Z) * \
+                (1 / (chi - Eu) ** 2 - 1 / (chi - El) ** 2)
+            lv = 6.33 + 0.4 * _log10(drbar2) + c03 + log10_k
+        logvdw[sel] = np.where(chi < Eu, 0.0, lv)
+    return stark, logvdw

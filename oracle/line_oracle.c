@@ -1,0 +1,382 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Float64 CPU restatement of Korg.jl's line-opacity path, used by the test-suite and by the
+ * benchmark's CPU-baseline leg as the checker for the device library.  Nothing in the product
+ * links or calls this file.
+ *
+ * It follows, expression by expression (same IEEE operation order, no FMA contraction):
+ *   src/line_absorption.jl:33-129   line_absorption!            -> oracle_line_absorption
+ *   src/line_absorption.jl:140-145  inverse_gaussian_density    -> oracle_inverse_gaussian_density
+ *   src/line_absorption.jl:156-161  inverse_lorentz_density     -> oracle_inverse_lorentz_density
+ *   src/line_absorption.jl:190      doppler_width               -> oracle_doppler_width
+ *   src/line_absorption.jl:195      scaled_stark                -> oracle_scaled_stark
+ *   src/line_absorption.jl:208-221  scaled_vdW                  -> oracle_scaled_vdW
+ *   src/line_absorption.jl:229-237  sigma_line                  -> oracle_sigma_line
+ *   src/line_absorption.jl:245-249  line_profile                -> oracle_line_profile
+ *   src/line_absorption.jl:251-265  harris_series               -> oracle_harris_series
+ *   src/line_absorption.jl:286-310  voigt_hjerting              -> oracle_voigt_hjerting
+ *   src/wavelengths.jl:188-210      searchsortedfirst/last      -> oracle_searchsorted_first/last
+ * with the constants of src/constants.jl:3-19 and the masses of src/atomic_data.jl:16-26.
+ *
+ * Threading: the reference spawns one task per line chunk, each with a private alpha that is
+ * summed afterwards (src/line_absorption.jl:62-69,127).  oracle_line_absorption offers that
+ * schedule (mode ORACLE_KORG_CHUNKS, used for timing) and a deterministic schedule
+ * (ORACLE_DETERMINISTIC) whose result equals the reference run on one thread: every alpha
+ * element receives its lines in list order into a zeroed accumulator, which is then added to
+ * alpha once.
+ *
+ * Transcendentals come from the C library; Julia's own exp/log/pow may differ in the last ulp.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* src/constants.jl:3-19 */
+static const double kboltz_cgs = 1.380649e-16;
+static const double c_cgs = 2.99792458e10;
+static const double electron_mass_cgs = 9.1093897e-28;
+static const double electron_charge_cgs = 4.80320425e-10;
+static const double amu_cgs = 1.6605402e-24;
+static const double kboltz_eV = 8.617333262145e-5;
+static const double hplanck_eV = 4.135667696e-15;
+
+enum { ORACLE_DETERMINISTIC = 0, ORACLE_KORG_CHUNKS = 1, ORACLE_WINDOWS_ONLY = 2 };
+enum { ORACLE_CNTM_AT_LINES = 0, ORACLE_CNTM_GRID = 1 };
+
+/* src/line_absorption.jl:140-145 */
+double oracle_inverse_gaussian_density(double rho, double sigma) {
+    if (rho > 1.0 / (sqrt(2.0 * M_PI) * sigma)) return 0.0;
+    return sigma * sqrt(-2.0 * log(sqrt(2.0 * M_PI) * sigma * rho));
+}
+
+/* src/line_absorption.jl:156-161 */
+double oracle_inverse_lorentz_density(double rho, double gamma) {
+    if (rho > 1.0 / (M_PI * gamma)) return 0.0;
+    return sqrt(gamma / (M_PI * rho) - gamma * gamma);
+}
+
+/* src/line_absorption.jl:190 */
+double oracle_doppler_width(double lambda0, double T, double m, double xi) {
+    return lambda0 * sqrt(kboltz_cgs * T / m + (xi * xi) / 2.0) / c_cgs;
+}
+
+/* src/line_absorption.jl:195 */
+double oracle_scaled_stark(double gamma_stark, double T) {
+    return gamma_stark * pow(T / 10000.0, 1.0 / 6.0);
+}
+
+/* src/line_absorption.jl:208-221 */
+double oracle_scaled_vdW(double vdW1, double vdW2, double m, double T) {
+    if (vdW2 == -1.0) return vdW1 * pow(T / 10000.0, 0.3);
+    const double v0 = 1e6;
+    const double sigma = vdW1, alpha = vdW2;
+    const double invmu = 1.0 / (1.008 * amu_cgs) + 1.0 / m;
+    const double vbar = sqrt(8.0 * kboltz_cgs * T / M_PI * invmu);
+    return 2.0 * pow(4.0 / M_PI, alpha / 2.0) * tgamma((4.0 - alpha) / 2.0) * v0 * sigma *
+           pow(vbar / v0, 1.0 - alpha);
+}
+
+/* src/line_absorption.jl:229-237 */
+double oracle_sigma_line(double lambda) {
+    const double e = electron_charge_cgs, me = electron_mass_cgs, c = c_cgs;
+    return (M_PI * (e * e) / me / c) * ((lambda * lambda) / c);
+}
+
+/* src/line_absorption.jl:251-265; returns H0, writes H1, H2 */
+double oracle_harris_series(double v, double* H1_out, double* H2_out) {
+    const double v2 = v * v;
+    const double H0 = exp(-v2);
+    double H1;
+    if (v < 1.3)
+        H1 = -1.12470432 + (-0.15516677 + (3.288675912 + (-2.34357915 + 0.42139162 * v) * v) * v) * v;
+    else if (v < 2.4)
+        H1 = -4.48480194 + (9.39456063 + (-6.61487486 + (1.98919585 - 0.22041650 * v) * v) * v) * v;
+    else
+        H1 = ((0.554153432 +
+               (0.278711796 + (-0.1883256872 + (0.042991293 - 0.003278278 * v) * v) * v) * v) /
+              (v2 - 3.0 / 2.0));
+    *H1_out = H1;
+    *H2_out = (1.0 - 2.0 * v2) * H0;
+    return H0;
+}
+
+/* src/line_absorption.jl:286-310 */
+double oracle_voigt_hjerting(double a, double v) {
+    const double sqrt_pi = sqrt(M_PI);
+    const double v2 = v * v;
+    if (a <= 0.2 && v >= 5.0) {
+        const double invv2 = 1.0 / v2;
+        return (a / sqrt_pi * invv2) * (1.0 + 1.5 * invv2 + 3.75 * (invv2 * invv2));
+    } else if (a <= 0.2) {
+        double H1, H2;
+        const double H0 = oracle_harris_series(v, &H1, &H2);
+        return H0 + (H1 + H2 * a) * a;
+    } else if (a <= 1.4 && a + v < 3.2) {
+        double H1, H2;
+        const double H0 = oracle_harris_series(v, &H1, &H2);
+        const double M0 = H0;
+        const double M1 = H1 + 2.0 / sqrt_pi * M0;
+        const double M2 = H2 - M0 + 2.0 / sqrt_pi * M1;
+        const double M3 = 2.0 / (3.0 * sqrt_pi) * (1.0 - H2) - (2.0 / 3.0) * v2 * M1 + (2.0 / sqrt_pi) * M2;
+        const double M4 = 2.0 / 3.0 * v2 * v2 * M0 - 2.0 / (3.0 * sqrt_pi) * M1 + 2.0 / sqrt_pi * M3;
+        const double psi = 0.979895023 + (-0.962846325 + (0.532770573 - 0.122727278 * a) * a) * a;
+        return psi * (M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a);
+    } else {
+        const double r2 = v2 / (a * a);
+        const double a_invu = 1.0 / sqrt(2.0) / ((r2 + 1.0) * a);
+        const double a2_invu2 = a_invu * a_invu;
+        return sqrt(2.0 / M_PI) * a_invu *
+               (1.0 + (3.0 * r2 - 1.0 + ((r2 - 2.0) * 15.0 * r2 + 2.0) * a2_invu2) * a2_invu2);
+    }
+}
+
+/* src/line_absorption.jl:245-249 */
+double oracle_line_profile(double lambda0, double sigma, double gamma, double amplitude,
+                           double lambda) {
+    const double inv_sigma_sqrt2 = 1.0 / (sigma * sqrt(2.0));
+    const double scaling = inv_sigma_sqrt2 / sqrt(M_PI) * amplitude;
+    return oracle_voigt_hjerting(gamma * inv_sigma_sqrt2, fabs(lambda - lambda0) * inv_sigma_sqrt2) *
+           scaling;
+}
+
+/* src/wavelengths.jl:188-198: 1-based index of the first grid point >= lam (n+1 if none).
+ * A value >= 1 is read as Angstrom.  NaN compares false everywhere, giving n+1. */
+int64_t oracle_searchsorted_first(const double* wls, int64_t n, double lam) {
+    if (lam >= 1.0) lam = lam * 1e-8;
+    if (!(lam <= wls[n - 1])) return n + 1;
+    int64_t lo = 0, hi = n - 1; /* first index with wls[i] >= lam lies in [lo, hi] */
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (wls[mid] < lam) lo = mid + 1; else hi = mid;
+    }
+    return lo + 1;
+}
+
+/* src/wavelengths.jl:200-210: 1-based index of the last grid point <= lam (0 if none). */
+int64_t oracle_searchsorted_last(const double* wls, int64_t n, double lam) {
+    if (lam >= 1.0) lam = lam * 1e-8;
+    if (!(lam <= wls[n - 1])) return n;
+    int64_t lo = 0, hi = n; /* count of points <= lam lies in [lo, hi] */
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (wls[mid] <= lam) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* Julia's maximum(): NaN if any element is NaN. */
+static double max_nan(const double* x, int64_t n) {
+    double m = x[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (isnan(x[i])) return x[i];
+        if (x[i] > m) m = x[i];
+    }
+    return m;
+}
+
+/* Linear interpolation of one layer's continuum on the coarse grid at x (knots ascending). */
+static double cntm_interp(const double* knots, const double* vals, int64_t nk, double x) {
+    int64_t lo = 0, hi = nk; /* count of knots <= x */
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (knots[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    int64_t i = lo - 1;
+    if (i < 0) i = 0;
+    if (i > nk - 2) i = nk - 2;
+    const double t = (x - knots[i]) / (knots[i + 1] - knots[i]);
+    return (1.0 - t) * vals[i] + t * vals[i + 1];
+}
+
+typedef struct {
+    int64_t L, nwl, nlines;
+    const double *wls;
+    const double *wl, *log_gf, *E_lower, *gamma_rad, *gamma_stark, *vdW1, *vdW2;
+    const int32_t* species_index;
+    const double* species_mass;
+    const uint8_t* species_is_mol;
+    const double* n_div_U;
+    const double *temps, *n_e, *xi;
+    int cntm_mode;
+    const double *cntm_values, *knots;
+    int64_t n_knots;
+    double cutoff;
+    double *beta, *n_eff;
+} problem_t;
+
+/* Per-line setup of src/line_absorption.jl:82-119 for one line: per-layer sigma, gamma,
+ * amplitude and the 1-based window [lb, ub].  Scratch arrays have length L. */
+static void line_setup(const problem_t* P, int64_t i, double* sigma, double* gamma, double* amp,
+                       double* rho, double* inv, int64_t* lb, int64_t* ub) {
+    const int64_t L = P->L;
+    const int32_t s = P->species_index[i];
+    const double m = P->species_mass[s];
+    const double wl = P->wl[i];
+    for (int64_t l = 0; l < L; ++l) sigma[l] = oracle_doppler_width(wl, P->temps[l], m, P->xi[l]);
+    for (int64_t l = 0; l < L; ++l) {
+        double G = P->gamma_rad[i];
+        if (!P->species_is_mol[s]) {
+            G = G + P->n_e[l] * oracle_scaled_stark(P->gamma_stark[i], P->temps[l]);
+            G = G + P->n_eff[l] * oracle_scaled_vdW(P->vdW1[i], P->vdW2[i], m, P->temps[l]);
+        }
+        gamma[l] = G * (wl * wl) / (c_cgs * (4.0 * M_PI));
+    }
+    const double E_upper = P->E_lower[i] + c_cgs * hplanck_eV / wl;
+    const double gf = pow(10.0, P->log_gf[i]);
+    const double sl = oracle_sigma_line(wl);
+    for (int64_t l = 0; l < L; ++l) {
+        const double levels = exp(-P->beta[l] * P->E_lower[i]) - exp(-P->beta[l] * E_upper);
+        amp[l] = gf * sl * levels * P->n_div_U[(int64_t)s * L + l];
+    }
+    for (int64_t l = 0; l < L; ++l) {
+        double c;
+        if (P->cntm_mode == ORACLE_CNTM_AT_LINES)
+            c = P->cntm_values[i * L + l];
+        else
+            c = cntm_interp(P->knots, P->cntm_values + l * P->n_knots, P->n_knots, wl);
+        rho[l] = c * P->cutoff / amp[l];
+    }
+    for (int64_t l = 0; l < L; ++l) inv[l] = oracle_inverse_gaussian_density(rho[l], sigma[l]);
+    const double doppler_window = max_nan(inv, L);
+    for (int64_t l = 0; l < L; ++l) inv[l] = oracle_inverse_lorentz_density(rho[l], gamma[l]);
+    const double lorentz_window = max_nan(inv, L);
+    const double window = sqrt(lorentz_window * lorentz_window + doppler_window * doppler_window);
+    *lb = oracle_searchsorted_first(P->wls, P->nwl, wl - window);
+    *ub = oracle_searchsorted_last(P->wls, P->nwl, wl + window);
+}
+
+/*
+ * alpha: column-major [L x nwl] (alpha[l + L*j]), accumulated into (src/line_absorption.jl:127).
+ * n_div_U: [n_species x L] row-major, n_species(l)/U_species(ln T_l) (src/line_absorption.jl:42-44).
+ * n_HI, n_HeI, n_H2: perturber densities for n_eff_vdW (src/line_absorption.jl:54-60).
+ * xi: [L] microturbulence in cm/s.
+ * cntm_mode 0: cntm_values[i*L + l] is alpha_cntm of layer l at line i's wavelength;
+ *           1: cntm_values[l*n_knots + k] on knots[k] (linear interpolation).
+ * lb_out/ub_out (optional): the 1-based windows, lb > ub for skipped lines.
+ * n_evals_out (optional): sum over lines of L*(ub-lb+1) for lb <= ub.
+ * Returns 0.
+ */
+int oracle_line_absorption(double* alpha, int64_t L, int64_t nwl, const double* wls,
+                           int64_t nlines, const double* wl, const double* log_gf,
+                           const double* E_lower, const double* gamma_rad,
+                           const double* gamma_stark, const double* vdW1, const double* vdW2,
+                           const int32_t* species_index, const double* species_mass,
+                           const uint8_t* species_is_mol, const double* n_div_U,
+                           const double* temps, const double* n_e, const double* n_HI,
+                           const double* n_HeI, const double* n_H2, const double* xi,
+                           int cntm_mode, const double* cntm_values, int64_t n_knots,
+                           const double* knots, double cutoff_threshold, int nthreads, int mode,
+                           int64_t* lb_out, int64_t* ub_out, double* n_evals_out) {
+    if (nlines == 0) return 0; /* src/line_absorption.jl:35-37 */
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    problem_t P = {L, nwl, nlines, wls, wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2,
+                   species_index, species_mass, species_is_mol, n_div_U, temps, n_e, xi,
+                   cntm_mode, cntm_values, knots, n_knots, cutoff_threshold, NULL, NULL};
+    double* beta = malloc(sizeof(double) * L);
+    double* n_eff = malloc(sizeof(double) * L);
+    /* src/line_absorption.jl:39, 54-60 */
+    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
+    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;
+    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
+    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+    for (int64_t l = 0; l < L; ++l) {
+        beta[l] = 1.0 / (kboltz_eV * temps[l]);
+        n_eff[l] = n_HI[l] + c_He * n_HeI[l] + c_H2 * n_H2[l];
+    }
+    P.beta = beta;
+    P.n_eff = n_eff;
+
+    /* per-line setup: sigma, gamma, amplitude per layer and the window */
+    double* sig = malloc(sizeof(double) * nlines * L);
+    double* gam = malloc(sizeof(double) * nlines * L);
+    double* amp = malloc(sizeof(double) * nlines * L);
+    int64_t* lb = malloc(sizeof(int64_t) * nlines);
+    int64_t* ub = malloc(sizeof(int64_t) * nlines);
+#pragma omp parallel num_threads(nthreads)
+    {
+        double* rho = malloc(sizeof(double) * L);
+        double* inv = malloc(sizeof(double) * L);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < nlines; ++i)
+            line_setup(&P, i, sig + i * L, gam + i * L, amp + i * L, rho, inv, lb + i, ub + i);
+        free(rho);
+        free(inv);
+    }
+    double nev = 0.0;
+    for (int64_t i = 0; i < nlines; ++i)
+        if (lb[i] <= ub[i]) nev += (double)L * (double)(ub[i] - lb[i] + 1);
+    if (n_evals_out) *n_evals_out = nev;
+    if (lb_out) memcpy(lb_out, lb, sizeof(int64_t) * nlines);
+    if (ub_out) memcpy(ub_out, ub, sizeof(int64_t) * nlines);
+
+    if (mode == ORACLE_WINDOWS_ONLY) {
+        /* windows and evaluation count only; alpha is left untouched */
+    } else if (mode == ORACLE_DETERMINISTIC) {
+        /* each thread owns a contiguous block of wavelengths (all layers) and adds the lines in
+         * list order into a zeroed accumulator: identical to the one-task reference run */
+        const int64_t chunk = 512;
+        const int64_t nchunks = (nwl + chunk - 1) / chunk;
+#pragma omp parallel num_threads(nthreads)
+        {
+            double* acc = malloc(sizeof(double) * L * chunk);
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t c = 0; c < nchunks; ++c) {
+                const int64_t j0 = c * chunk;
+                const int64_t j1 = (j0 + chunk < nwl) ? j0 + chunk : nwl;
+                memset(acc, 0, sizeof(double) * L * chunk);
+                for (int64_t i = 0; i < nlines; ++i) {
+                    if (lb[i] > ub[i]) continue;
+                    const int64_t a = (lb[i] - 1 > j0) ? lb[i] - 1 : j0;
+                    const int64_t b = (ub[i] < j1) ? ub[i] : j1; /* exclusive, 0-based */
+                    for (int64_t j = a; j < b; ++j)
+                        for (int64_t l = 0; l < L; ++l)
+                            acc[(j - j0) * L + l] +=
+                                oracle_line_profile(wl[i], sig[i * L + l], gam[i * L + l],
+                                                    amp[i * L + l], wls[j]);
+                }
+                for (int64_t j = j0; j < j1; ++j)
+                    for (int64_t l = 0; l < L; ++l) alpha[j * L + l] += acc[(j - j0) * L + l];
+            }
+            free(acc);
+        }
+    } else {
+        /* the reference's schedule: line chunks, private alpha per chunk, summed in chunk order */
+        const int64_t nch = nthreads;
+        const int64_t csize = nlines / nch + (nlines % nch > 0);
+        double** parts = malloc(sizeof(double*) * nch);
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+        for (int64_t c = 0; c < nch; ++c) {
+            double* a_task = calloc((size_t)(L * nwl), sizeof(double));
+            const int64_t i0 = c * csize;
+            const int64_t i1 = (i0 + csize < nlines) ? i0 + csize : nlines;
+            for (int64_t i = i0; i < i1; ++i) {
+                if (lb[i] > ub[i]) continue;
+                for (int64_t j = lb[i] - 1; j < ub[i]; ++j)
+                    for (int64_t l = 0; l < L; ++l)
+                        a_task[j * L + l] += oracle_line_profile(wl[i], sig[i * L + l],
+                                                                 gam[i * L + l], amp[i * L + l],
+                                                                 wls[j]);
+            }
+            parts[c] = a_task;
+        }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t k = 0; k < L * nwl; ++k) {
+            double s = parts[0][k];
+            for (int64_t c = 1; c < nch; ++c) s += parts[c][k];
+            alpha[k] += s;
+        }
+        for (int64_t c = 0; c < nch; ++c) free(parts[c]);
+        free(parts);
+    }
+    free(sig); free(gam); free(amp); free(lb); free(ub); free(beta); free(n_eff);
+    return 0;
+}

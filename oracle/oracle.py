@@ -1,0 +1,138 @@
+"""ORACLE -- test infrastructure only: ctypes access to the Float64 CPU restatement of Korg's
+``line_absorption!`` (``line_oracle.c``; every function there cites the reference file:line it
+follows).
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg use this module, as the
+checker of the device library; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+DETERMINISTIC = 0
+KORG_CHUNKS = 1
+WINDOWS_ONLY = 2
+CNTM_AT_LINES = 0
+CNTM_GRID = 1
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        d, i64 = ctypes.c_double, ctypes.c_int64
+        for name, args in [("oracle_voigt_hjerting", [d, d]),
+                           ("oracle_line_profile", [d, d, d, d, d]),
+                           ("oracle_inverse_gaussian_density", [d, d]),
+                           ("oracle_inverse_lorentz_density", [d, d]),
+                           ("oracle_doppler_width", [d, d, d, d]),
+                           ("oracle_scaled_stark", [d, d]),
+                           ("oracle_scaled_vdW", [d, d, d, d]),
+                           ("oracle_sigma_line", [d])]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = d
+        L.oracle_harris_series.argtypes = [d, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.oracle_harris_series.restype = d
+        for name in ("oracle_searchsorted_first", "oracle_searchsorted_last"):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.c_void_p, i64, d]
+            f.restype = i64
+        L.oracle_line_absorption.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def voigt_hjerting(a, v):
+    return lib().oracle_voigt_hjerting(float(a), float(v))
+
+
+def line_profile(lambda0, sigma, gamma, amplitude, lam):
+    return lib().oracle_line_profile(float(lambda0), float(sigma), float(gamma), float(amplitude),
+                                     float(lam))
+
+
+def harris_series(v):
+    h1, h2 = ctypes.c_double(), ctypes.c_double()
+    h0 = lib().oracle_harris_series(float(v), ctypes.byref(h1), ctypes.byref(h2))
+    return h0, h1.value, h2.value
+
+
+def inverse_gaussian_density(rho, sigma):
+    return lib().oracle_inverse_gaussian_density(float(rho), float(sigma))
+
+
+def inverse_lorentz_density(rho, gamma):
+    return lib().oracle_inverse_lorentz_density(float(rho), float(gamma))
+
+
+def doppler_width(lambda0, T, m, xi):
+    return lib().oracle_doppler_width(float(lambda0), float(T), float(m), float(xi))
+
+
+def scaled_stark(gamma_stark, T):
+    return lib().oracle_scaled_stark(float(gamma_stark), float(T))
+
+
+def scaled_vdW(vdW, m, T):
+    return lib().oracle_scaled_vdW(float(vdW[0]), float(vdW[1]), float(m), float(T))
+
+
+def sigma_line(lam):
+    return lib().oracle_sigma_line(float(lam))
+
+
+def searchsortedfirst(all_wls, lam):
+    a = np.ascontiguousarray(all_wls, dtype=np.float64)
+    return lib().oracle_searchsorted_first(_p(a), a.size, float(lam))
+
+
+def searchsortedlast(all_wls, lam):
+    a = np.ascontiguousarray(all_wls, dtype=np.float64)
+    return lib().oracle_searchsorted_last(_p(a), a.size, float(lam))
+
+
+def line_absorption(alpha, problem, nthreads=0, mode=DETERMINISTIC, return_windows=False):
+    """Accumulate the lines of ``problem`` (a ``korg_line.problem.Problem``) into ``alpha``
+    (float64, shape (L, n_wl), Fortran order == the reference's column-major layout).
+
+    Returns ``(lb, ub, n_evals)`` (1-based windows) when ``return_windows``.
+    """
+    assert alpha.dtype == np.float64 and alpha.flags.f_contiguous
+    P = problem
+    L, nwl = alpha.shape
+    n = len(P.lines)
+    lb = np.zeros(n, dtype=np.int64)
+    ub = np.zeros(n, dtype=np.int64)
+    nev = ctypes.c_double(0.0)
+    ll = P.lines
+    rc = lib().oracle_line_absorption(
+        _p(alpha), ctypes.c_int64(L), ctypes.c_int64(nwl), _p(P.all_wls), ctypes.c_int64(n),
+        _p(ll.wl), _p(ll.log_gf), _p(ll.E_lower), _p(ll.gamma_rad), _p(ll.gamma_stark),
+        _p(ll.vdW1), _p(ll.vdW2), _p(ll.species_index), _p(P.species_mass), _p(P.species_is_mol),
+        _p(P.n_div_U), _p(P.temps), _p(P.n_e), _p(P.n_HI), _p(P.n_HeI), _p(P.n_H2), _p(P.xi),
+        ctypes.c_int(P.cntm_mode), _p(P.cntm_values), ctypes.c_int64(P.n_knots),
+        _p(P.cntm_knots), ctypes.c_double(P.cutoff_threshold), ctypes.c_int(nthreads),
+        ctypes.c_int(mode), _p(lb), _p(ub), ctypes.byref(nev))
+    if rc != 0:
+        raise RuntimeError(f"oracle_line_absorption returned {rc}")
+    if return_windows:
+        return lb, ub, nev.value
+    return None

@@ -1,0 +1,259 @@
+"""Parity of the gfx950 library (through the C ABI) with the Float64 oracle.  Needs a GPU.
+
+Tolerances (stated by the north star): Float64 form <= 1e-12 relative to the oracle per alpha
+element; Float32-with-Float64-fix-ups form <= 1e-4 relative per line-opacity element above
+1e-8 of the maximum, and <= 1e-4 absolute in normalized flux.  Windows (index work) bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from korg_line import Context, DeviceProblem, Line, LineList, Problem, Species, Wavelengths
+from korg_line import _lib, rt, workloads
+from korg_line.atmosphere import Atmosphere, continuum_grid
+from korg_line.problem import ContinuumGrid
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-12
+FP32_RTOL = 1e-4
+FLUX_ATOL = 1e-4
+
+
+def oracle_alpha(P, base=None, windows=False):
+    a = np.zeros((P.n_layers, P.n_wl), order="F") if base is None else np.asfortranarray(base.copy())
+    lb, ub, nev = oracle.line_absorption(a, P, return_windows=True)
+    return (a, lb, ub, nev) if windows else a
+
+
+def gpu_alpha(ctx, P, precision, dtype=np.float64, base=None):
+    a = np.zeros((P.n_layers, P.n_wl), dtype=dtype, order="F") if base is None else \
+        np.asfortranarray(base.astype(dtype))
+    ctx.line_absorption(a, P, precision)
+    return a
+
+
+def rel_err(a, ref, floor_frac=1e-8):
+    """Max |a - ref| / |ref| over elements with |ref| >= floor_frac * max|ref|."""
+    ref = np.asarray(ref, dtype=np.float64)
+    a = np.asarray(a, dtype=np.float64)
+    m = np.abs(ref) >= floor_frac * np.abs(ref).max()
+    return float(np.max(np.abs(a[m] - ref[m]) / np.abs(ref[m])))
+
+
+def check_fp64(ctx, P):
+    ref, lb, ub, nev = oracle_alpha(P, windows=True)
+    got = gpu_alpha(ctx, P, "fp64")
+    glb, gub = ctx.windows(len(P.lines))
+    skip = lb > ub
+    assert np.array_equal(skip, glb > gub)
+    assert np.array_equal(lb[~skip], glb[~skip]) and np.array_equal(ub[~skip], gub[~skip])
+    assert ctx.stats()["n_evals"] == nev
+    np.testing.assert_allclose(got, ref, rtol=FP64_RTOL, atol=1e-300)
+    return ref
+
+
+def check_fp32(ctx, P, ref=None):
+    if ref is None:
+        ref = oracle_alpha(P)
+    got = gpu_alpha(ctx, P, "fp32")
+    err = rel_err(got, ref)
+    assert err <= FP32_RTOL, err
+    return got, err
+
+
+# ----------------------------------------------------------------------------- solar, c1
+def test_solar_small_fp64_and_fp32(ctx):
+    P, atm, wls = workloads.solar_small()
+    ref = check_fp64(ctx, P)
+    got, err = check_fp32(ctx, P, ref)
+    # normalized flux: continuum on the fine grid + lines, anchored at the 5000 A continuum
+    cntm = np.array([np.interp(wls.all_wls, P.cntm_knots, P.cntm_values[l])
+                     for l in range(P.n_layers)])
+    aref = np.interp(5000e-8, P.cntm_knots, P.cntm_values[0]) * 0 + \
+        np.array([np.interp(5000e-8, P.cntm_knots, P.cntm_values[l]) for l in range(P.n_layers)])
+    tau = 10.0 ** atm.lgTau5
+    f_ref = rt.normalized_flux(cntm + ref, cntm, aref, tau, atm.temps, wls.all_wls)
+    f_got = rt.normalized_flux(cntm + got, cntm, aref, tau, atm.temps, wls.all_wls)
+    assert np.max(np.abs(f_got - f_ref)) <= FLUX_ATOL
+    assert f_ref.min() < 0.5  # strong lines present: the comparison is not vacuous
+
+
+def test_solar_small_at_line_continuum(ctx):
+    """alpha_cntm given per line centre (callables), not as the coarse grid."""
+    P, _, _ = workloads.solar_small(n_lines=400, cntm="callables")
+    assert P.cntm_mode == _lib.CNTM_AT_LINES
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+def test_solar_wide_subrange(ctx):
+    """The 300,000-line wide workload's parameters, restricted to 5000-5020 A."""
+    P, _, _ = workloads.solar_wide_subset(sub_range=(5000, 5020))
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+# ----------------------------------------------------------------------------- semantics
+def test_accumulates_into_alpha(ctx):
+    P, _, _ = workloads.solar_small(n_lines=200)
+    base = np.random.default_rng(1).random((P.n_layers, P.n_wl)) * 1e-6
+    for prec in ("fp64", "fp32"):
+        z = gpu_alpha(ctx, P, prec)
+        a = gpu_alpha(ctx, P, prec, base=base)
+        assert np.array_equal(a, base + z)
+
+
+def test_empty_linelist_leaves_alpha(ctx):
+    P, _, _ = workloads.solar_small(n_lines=10)
+    E = Problem(P.all_wls, P.lines.take(np.zeros(0, dtype=int)), P.species_mass,
+                P.species_is_mol, P.n_div_U, P.temps, P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi,
+                P.cntm_mode, P.cntm_values, P.cntm_knots, P.cutoff_threshold)
+    base = np.asfortranarray(np.random.default_rng(2).random((P.n_layers, P.n_wl)))
+    a = base.copy(order="F")
+    ctx.line_absorption(a, E, "fp32")
+    assert np.array_equal(a, base)
+
+
+def test_deterministic_repeat(ctx):
+    P, _, _ = workloads.solar_small(n_lines=500)
+    for prec in ("fp64", "fp32"):
+        assert np.array_equal(gpu_alpha(ctx, P, prec), gpu_alpha(ctx, P, prec))
+
+
+def test_float32_alpha_storage(ctx):
+    P, _, _ = workloads.solar_small(n_lines=300)
+    ref = oracle_alpha(P)
+    got = gpu_alpha(ctx, P, "fp32", dtype=np.float32)
+    assert got.dtype == np.float32 and rel_err(got, ref) <= FP32_RTOL
+    got64 = gpu_alpha(ctx, P, "fp64", dtype=np.float32)
+    assert rel_err(got64, ref) <= 1e-6
+
+
+def test_unsorted_lines(ctx):
+    """Device entry point rejects an unsorted list and leaves alpha alone; the Python mirror sorts
+    first, as synthesize does."""
+    import torch
+    P, _, _ = workloads.solar_small(n_lines=100)
+    perm = np.random.default_rng(4).permutation(len(P.lines))
+    U = Problem(P.all_wls, P.lines.take(perm), P.species_mass, P.species_is_mol, P.n_div_U,
+                P.temps, P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi, P.cntm_mode, P.cntm_values,
+                P.cntm_knots, P.cutoff_threshold)
+    D = DeviceProblem(U)
+    a = D.new_alpha("float32")
+    torch.cuda.synchronize()
+    with pytest.raises(_lib.KorgError) as e:
+        ctx.line_absorption_device(a, D, "fp32")
+    assert e.value.code == _lib.KORG_ERR_INVALID_ARGUMENT
+    assert float(a.abs().max()) == 0.0
+
+
+# ----------------------------------------------------------------------------- shapes
+def _atm_subset(n_layers):
+    atm = Atmosphere.from_model("sun")
+    idx = np.linspace(0, atm.n_layers - 1, min(n_layers, atm.n_layers)).astype(int)
+    if n_layers > atm.n_layers:  # repeat layers to make a deeper stand-in
+        idx = np.resize(np.arange(atm.n_layers), n_layers)
+    return atm, idx
+
+
+@pytest.mark.parametrize("n_layers", [1, 5, 8, 17, 70])
+def test_layer_counts(ctx, n_layers):
+    atm, idx = _atm_subset(n_layers)
+    wls = Wavelengths((5000, 5020))
+    ll = workloads.synthetic_linelist(5, 150, 4995.0, 5025.0, (-4.0, 0.5), (0.0, 5.0))
+    knots, vals = continuum_grid(atm, float(wls.all_wls[0]), float(wls.all_wls[-1]))
+    nd = {k: v[idx] for k, v in atm.n_densities.items()}
+    P = Problem.from_korg_args(ll, wls, atm.temps[idx], atm.n_e[idx], nd, atm.partition_fns,
+                               1e5 * (1 + 0.1 * np.arange(len(idx))), ContinuumGrid(knots, vals[idx]))
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+def test_multi_range_grid(ctx):
+    atm = Atmosphere.from_model("sun")
+    wls = Wavelengths([(5000, 5005), (5006, 5010, 0.02), (5012.5, 5015, 0.005)])
+    ll = workloads.synthetic_linelist(6, 300, 4998.0, 5017.0, (-4.0, 0.5), (0.0, 5.0))
+    P = workloads.make_problem(atm, ll, wls)
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+def test_single_wavelength(ctx):
+    """The reference-wavelength call of synthesize: a one-point grid at 5000 A
+    (src/synthesize.jl:266-276)."""
+    atm = Atmosphere.from_model("sun")
+    wls = Wavelengths([5000.0])
+    ll = workloads.synthetic_linelist(7, 200, 4990.0, 5010.0, (-4.0, 0.5), (0.0, 5.0))
+    P = workloads.make_problem(atm, ll, wls)
+    assert P.n_wl == 1
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+def test_molecules_abo_and_missing_perturbers(ctx):
+    atm = Atmosphere.from_model("sun")
+    wls = Wavelengths((5000, 5010))
+    lines = [Line(5001.0, -1.0, "CO", 0.5, 1e8, 1e-4, -7.0),
+             Line(5002.0, -0.5, "Fe I", 1.0, 1e8, 1e-5, (1.3917417470792187e-14, 0.227)),
+             Line(5003.0, -0.5, "Fe I", 2.0, 1e8, 1e-5, 497.227),
+             Line(5004.0, 0.0, "Ti I", 0.3),
+             Line(5005.0, -2.0, "FeH", 0.2, 1e7, 0.0, 0.0)]
+    ll = LineList.from_lines(lines)
+    nd = dict(atm.n_densities)
+    L = atm.n_layers
+    nd[Species("CO")] = np.full(L, 1e12)
+    nd[Species("FeH")] = np.full(L, 1e9)
+    pf = dict(atm.partition_fns)
+    pf[Species("CO")] = lambda x: 10.0 + 0 * x
+    pf[Species("FeH")] = lambda x: 50.0 + 0 * x
+    knots, vals = continuum_grid(atm, float(wls.all_wls[0]), float(wls.all_wls[-1]))
+    P = Problem.from_korg_args(ll, wls, atm.temps, atm.n_e, nd, pf, 1e5, ContinuumGrid(knots, vals))
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
+# ----------------------------------------------------------------------------- full size
+def test_solar_wide_full_size_properties(ctx):
+    """300,000 lines x 300,001 points x 56 layers: windows identical to the oracle's window pass
+    (index work), the same evaluation count, and the two device forms within the Float32
+    tolerance of each other; the Float32 run is bitwise repeatable."""
+    import torch
+    P, _, _ = workloads.solar_wide()
+    a = np.zeros((1, 1), order="F")
+    lb, ub, nev = oracle.line_absorption(np.zeros((P.n_layers, P.n_wl), order="F"), P,
+                                         mode=oracle.WINDOWS_ONLY, return_windows=True)
+    D = DeviceProblem(P)
+    a64 = D.new_alpha("float64")
+    a32 = D.new_alpha("float32")
+    b32 = D.new_alpha("float32")
+    torch.cuda.synchronize()
+    ctx.line_absorption_device(a64, D, "fp64")
+    glb, gub = ctx.windows(len(P.lines))
+    skip = lb > ub
+    assert np.array_equal(skip, glb > gub)
+    assert np.array_equal(lb[~skip], glb[~skip]) and np.array_equal(ub[~skip], gub[~skip])
+    assert ctx.stats()["n_evals"] == nev
+    ctx.line_absorption_device(a32, D, "fp32")
+    ctx.line_absorption_device(b32, D, "fp32")
+    assert torch.equal(a32, b32)
+    err = rel_err(a32.double().cpu().numpy(), a64.cpu().numpy())
+    assert err <= FP32_RTOL, err
+    del a
+
+
+def test_invalid_species_index_rejected(ctx):
+    """A species index outside the species table is reported, and alpha is left alone (the
+    device checks the index before any read through it)."""
+    P, _, _ = workloads.solar_small(n_lines=50)
+    bad = P.lines.take(np.arange(len(P.lines)))
+    bad.species_index = bad.species_index.copy()
+    bad.species_index[7] = len(P.species_mass)
+    B = Problem(P.all_wls, bad, P.species_mass, P.species_is_mol, P.n_div_U, P.temps, P.n_e,
+                P.n_HI, P.n_HeI, P.n_H2, P.xi, P.cntm_mode, P.cntm_values, P.cntm_knots,
+                P.cutoff_threshold)
+    a = np.zeros((P.n_layers, P.n_wl), order="F")
+    with pytest.raises(_lib.KorgError) as e:
+        ctx.line_absorption(a, B, "fp32")
+    assert e.value.code == _lib.KORG_ERR_INVALID_ARGUMENT
+    assert not a.any()
