@@ -51,22 +51,22 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     from korg_line import Context, DeviceProblem, rt, workloads
+    from korg_line.hipmem import synchronize as device_sync
 
     t = time.time()
     P, atm, wls = workloads.solar_wide()
     log(f"workload built in {time.time() - t:.1f}s: {len(P.lines)} lines, {P.n_wl} points, "
         f"{P.n_layers} layers")
-    D = DeviceProblem(P, "cuda")
     ctx = Context(0)
+    D = DeviceProblem(P)
     a32 = D.new_alpha("float32")
-    torch.cuda.synchronize()
+    device_sync()
 
     def run(alpha, precision, steps, warmup):
         for _ in range(warmup):
             ctx.line_absorption_device(alpha, D, precision, synchronize=True)
-        torch.cuda.synchronize()
+        device_sync()
         ctx.synchronize()
         stats = []
         t0 = time.perf_counter()
@@ -74,7 +74,7 @@ def main():
             ctx.line_absorption_device(alpha, D, precision, synchronize=True)
             stats.append(ctx.stats())
         ctx.synchronize()
-        torch.cuda.synchronize()
+        device_sync()
         return (time.perf_counter() - t0) / steps, stats
 
     log("timing the Float32 form ...")
@@ -95,8 +95,8 @@ def main():
     ctx.set_tuning()
     nA, nB, nC, nD = sb["n_evals_branch"]
     E = sb["n_evals"]
-    lines64 = z64.cpu().numpy().T  # (layers, points)
-    lines32 = z32.double().cpu().numpy().T
+    lines64 = z64.numpy().T  # (layers, points)
+    lines32 = z32.numpy().astype(np.float64).T
     m = np.abs(lines64) >= 1e-8 * np.abs(lines64).max()
     alpha_err = float(np.max(np.abs(lines32[m] - lines64[m]) / np.abs(lines64[m])))
     flux_err = None

@@ -129,6 +129,10 @@ struct LayerTab {
     double beta, n_eff, t16, t03;
 };
 
+// Device counters of one call (reset by k_layers): [0] profile queue, [1] evaluations,
+// [2] lines used, [3] error flags, [4] core reach, [5] core queue, [6..9] evaluations by branch.
+constexpr int N_COUNTERS = 16;
+
 __global__ void k_layers(int L, const double* __restrict__ temps, const double* __restrict__ n_HI,
                          const double* __restrict__ n_HeI, const double* __restrict__ n_H2,
                          LayerTab* __restrict__ tab, unsigned long long* __restrict__ counters) {
@@ -977,7 +981,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         TRY(grow(ctx, ctx->dmask, sizeof(uint64_t) * N * n_lblocks));
     }
     TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * L));
-    TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * 8));
+    TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * N_COUNTERS));
     TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
     TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
     TRY(grow(ctx, ctx->cls_lines, sizeof(int) * N));
@@ -1112,6 +1116,7 @@ int finish_stats(korg_ctx* ctx) {
     HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
     HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
     unsigned long long c[10];
+    static_assert(sizeof c <= sizeof(unsigned long long) * N_COUNTERS, "counters");
     HIP_TRY(ctx, hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
     if (c[3] & 2)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,

@@ -6,8 +6,8 @@ line opacity [cm^-1] into ``alpha`` (shape (n_layers, n_wavelengths), Julia's co
 layout: pass a Fortran-ordered array, or the transposed view of a C-ordered one), leaves it
 untouched for an empty line list, and logs an error (continuing) if H I is in the line list.
 
-Device-resident use (inputs already in HBM, e.g. torch tensors on ``cuda``) goes through
-``Context.line_absorption_device`` with a ``DeviceProblem``.
+Device-resident use (inputs already in HBM) goes through ``Context.line_absorption_device`` with
+a ``DeviceProblem``; its buffers are ``hipmem.DeviceArray``s.
 """
 import ctypes
 
@@ -48,18 +48,14 @@ def _host_structs(P):
 
 
 class DeviceProblem:
-    """A ``Problem`` uploaded once to device memory (torch tensors on ``device``)."""
+    """A ``Problem`` uploaded once to device memory (HBM-resident inputs)."""
 
-    def __init__(self, problem, device="cuda"):
-        import torch
+    def __init__(self, problem):
+        from .hipmem import DeviceArray
         P = problem
         ll = P.lines
-
-        def up(a):
-            return torch.from_numpy(np.ascontiguousarray(a)).to(device)
-
         self.problem = P
-        self.t = {k: up(v) for k, v in {
+        self.t = {k: DeviceArray.from_numpy(v) for k, v in {
             "all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
             "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
             "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
@@ -67,7 +63,7 @@ class DeviceProblem:
             "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
             "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}.items()}
         t = self.t
-        d = lambda k: t[k].data_ptr()
+        d = lambda k: t[k].ptr
         self.grid = _lib.KorgGrid(P.n_wl, d("all_wls"))
         self.lines = _lib.KorgLines(len(ll), d("wl"), d("log_gf"), d("E_lower"), d("gamma_rad"),
                                     d("gamma_stark"), d("vdW1"), d("vdW2"), d("species"))
@@ -78,15 +74,14 @@ class DeviceProblem:
                                        d("cntm"))
 
     def new_alpha(self, dtype="float32", fill=None):
-        """Device alpha in the library's layout: a C-ordered (n_wl, n_layers) tensor is the
-        column-major (n_layers, n_wl) matrix."""
-        import torch
+        """Device alpha in the library's layout: a C-ordered (n_wl, n_layers) buffer is the
+        column-major (n_layers, n_wl) matrix.  ``fill``: optional (n_layers, n_wl) start value."""
+        from .hipmem import DeviceArray
         P = self.problem
-        dt = torch.float32 if dtype in ("float32", "f32") else torch.float64
-        a = torch.zeros((P.n_wl, P.n_layers), dtype=dt, device=self.t["wl"].device)
-        if fill is not None:
-            a.copy_(torch.as_tensor(np.ascontiguousarray(np.asarray(fill).T), dtype=dt))
-        return a
+        dt = np.float32 if dtype in ("float32", "f32") else np.float64
+        if fill is None:
+            return DeviceArray.zeros((P.n_wl, P.n_layers), dt)
+        return DeviceArray.from_numpy(np.asarray(fill, dtype=dt).T)
 
 
 class Context:
@@ -160,14 +155,13 @@ class Context:
         _lib.check(self._h, rc)
 
     def line_absorption_device(self, alpha_dev, dprob, precision="fp32", synchronize=True):
-        """Device arrays: ``alpha_dev`` a torch tensor from ``DeviceProblem.new_alpha``."""
-        import torch
-        dt = {torch.float64: _lib.DTYPE_F64, torch.float32: _lib.DTYPE_F32}[alpha_dev.dtype]
+        """Device arrays: ``alpha_dev`` a ``DeviceArray`` from ``DeviceProblem.new_alpha``."""
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
         P = dprob.problem
-        if tuple(alpha_dev.shape) != (P.n_wl, P.n_layers) or not alpha_dev.is_contiguous():
-            raise ValueError("alpha_dev must be a contiguous (n_wl, n_layers) device tensor")
+        if tuple(alpha_dev.shape) != (P.n_wl, P.n_layers) or alpha_dev.dtype not in dt:
+            raise ValueError("alpha_dev must be a (n_wl, n_layers) float32/float64 device array")
         rc = self._L.korg_line_absorption_device(
-            self._h, ctypes.c_void_p(alpha_dev.data_ptr()), dt, ctypes.byref(dprob.grid),
+            self._h, ctypes.c_void_p(alpha_dev.ptr), dt[alpha_dev.dtype], ctypes.byref(dprob.grid),
             ctypes.byref(dprob.lines), ctypes.byref(dprob.species), ctypes.byref(dprob.layers),
             ctypes.byref(dprob.cntm), P.cutoff_threshold, PRECISIONS[precision],
             1 if synchronize else 0)

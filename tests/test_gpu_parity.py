@@ -96,12 +96,16 @@ def test_solar_wide_subrange(ctx):
 
 # ----------------------------------------------------------------------------- semantics
 def test_accumulates_into_alpha(ctx):
+    """Lines are added into what alpha holds.  The Float64 form sums a wavelength point's lines
+    from zero and adds once (bitwise alpha_in + lines); the Float32 form adds its far-wing and
+    Voigt-core sums in two steps, equal to rounding."""
     P, _, _ = workloads.solar_small(n_lines=200)
     base = np.random.default_rng(1).random((P.n_layers, P.n_wl)) * 1e-6
-    for prec in ("fp64", "fp32"):
-        z = gpu_alpha(ctx, P, prec)
-        a = gpu_alpha(ctx, P, prec, base=base)
-        assert np.array_equal(a, base + z)
+    z = gpu_alpha(ctx, P, "fp64")
+    assert np.array_equal(gpu_alpha(ctx, P, "fp64", base=base), base + z)
+    z = gpu_alpha(ctx, P, "fp32")
+    np.testing.assert_allclose(gpu_alpha(ctx, P, "fp32", base=base), base + z, rtol=1e-15,
+                               atol=0)
 
 
 def test_empty_linelist_leaves_alpha(ctx):
@@ -133,7 +137,6 @@ def test_float32_alpha_storage(ctx):
 def test_unsorted_lines(ctx):
     """Device entry point rejects an unsorted list and leaves alpha alone; the Python mirror sorts
     first, as synthesize does."""
-    import torch
     P, _, _ = workloads.solar_small(n_lines=100)
     perm = np.random.default_rng(4).permutation(len(P.lines))
     U = Problem(P.all_wls, P.lines.take(perm), P.species_mass, P.species_is_mol, P.n_div_U,
@@ -141,11 +144,10 @@ def test_unsorted_lines(ctx):
                 P.cntm_knots, P.cutoff_threshold)
     D = DeviceProblem(U)
     a = D.new_alpha("float32")
-    torch.cuda.synchronize()
     with pytest.raises(_lib.KorgError) as e:
         ctx.line_absorption_device(a, D, "fp32")
     assert e.value.code == _lib.KORG_ERR_INVALID_ARGUMENT
-    assert float(a.abs().max()) == 0.0
+    assert not a.numpy().any()
 
 
 # ----------------------------------------------------------------------------- shapes
@@ -218,16 +220,13 @@ def test_solar_wide_full_size_properties(ctx):
     """300,000 lines x 300,001 points x 56 layers: windows identical to the oracle's window pass
     (index work), the same evaluation count, and the two device forms within the Float32
     tolerance of each other; the Float32 run is bitwise repeatable."""
-    import torch
     P, _, _ = workloads.solar_wide()
-    a = np.zeros((1, 1), order="F")
     lb, ub, nev = oracle.line_absorption(np.zeros((P.n_layers, P.n_wl), order="F"), P,
                                          mode=oracle.WINDOWS_ONLY, return_windows=True)
     D = DeviceProblem(P)
     a64 = D.new_alpha("float64")
     a32 = D.new_alpha("float32")
     b32 = D.new_alpha("float32")
-    torch.cuda.synchronize()
     ctx.line_absorption_device(a64, D, "fp64")
     glb, gub = ctx.windows(len(P.lines))
     skip = lb > ub
@@ -236,10 +235,9 @@ def test_solar_wide_full_size_properties(ctx):
     assert ctx.stats()["n_evals"] == nev
     ctx.line_absorption_device(a32, D, "fp32")
     ctx.line_absorption_device(b32, D, "fp32")
-    assert torch.equal(a32, b32)
-    err = rel_err(a32.double().cpu().numpy(), a64.cpu().numpy())
+    assert np.array_equal(a32.numpy(), b32.numpy())
+    err = rel_err(a32.numpy(), a64.numpy())
     assert err <= FP32_RTOL, err
-    del a
 
 
 def test_invalid_species_index_rejected(ctx):
