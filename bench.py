@@ -43,7 +43,12 @@ def main():
                     help="fraction of the lines timed on the CPU oracle (every k-th line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flux", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (PMC byte-counter scale)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the line-opacity "
               "path shards by wavelength tiles or by star, which this build does not run.",
@@ -166,6 +171,22 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def copy_calibration(mib, reps=5):
+    """Plain device-to-device copies of a known size: FETCH_SIZE/WRITE_SIZE per copy dispatch
+    against mib MiB give the counters' scale."""
+    import numpy as np
+    from korg_line.hipmem import D2D, DeviceArray, hip, synchronize
+    n = mib * (1 << 20) // 4
+    src = DeviceArray.from_numpy(np.arange(n, dtype=np.float32))
+    dst = DeviceArray.zeros((n,), np.float32)
+    import ctypes
+    for _ in range(reps):
+        rc = hip().hipMemcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), n * 4, D2D)
+        assert rc == 0
+    synchronize()
+    log(f"copied {mib} MiB x {reps}")
 
 
 def load_traffic():

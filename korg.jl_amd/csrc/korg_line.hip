@@ -1,15 +1,36 @@
 // MI355X (gfx950) line-opacity library: the body of Korg.jl's line_absorption!
 // (src/line_absorption.jl:33-129) as HIP kernels behind the C ABI of include/korg_line.h.
 //
-// Pipeline of one call (all on one stream, inputs resident in HBM):
-//   k_layers     per-layer tables: beta = 1/(kT), n_eff_vdW, (T/1e4)^(1/6), (T/1e4)^0.3
-//   k_setup      one wavefront per line, lanes = layers (Float64): widths, amplitudes, the window
-//                (max over layers), per-(line, layer) profile records, width class
+// Pipeline of one call (one stream, inputs resident in HBM):
+//   k_layers      per-layer tables: beta = 1/(kT), n_eff_vdW, (T/1e4)^(1/6), (T/1e4)^0.3; resets
+//                 the call's counters
+//   k_setup       one wavefront per line, lanes = layers (Float64): Doppler and Lorentz widths,
+//                 amplitudes, the window (max over layers) and the per-line records the profile
+//                 kernels read
 //   k_count / k_scan / k_scatter   stable partition of the lines into window-width classes
-//   k_ranges     per (class, wavelength tile): index range of candidate lines
-//   k_profile_*  persistent wavefronts take (tile, layer block) items from a queue; each owns its
-//                alpha block outright, adds the overlapping lines in a fixed order in registers,
-//                then adds the block into alpha once.  No atomics on alpha: deterministic.
+//   k_ranges      per (class, wavelength tile): index range of candidate lines
+//   Float64 form:
+//     k_profile_f64  line_profile/voigt_hjerting per evaluation (the reference's expressions)
+//   Float32 form with Float64 fix-ups:
+//     k_wing_mfma    far wing of every line as a rank-4 update on the matrix cores (FP32 MFMA)
+//     k_near         the "near zone" of each line (Voigt cores, and where the damped-wing series
+//                    is not yet converged): exact branch-by-branch evaluation
+// Each profile work item owns its block of alpha outright; lines are added in a fixed order, and
+// the block is added into alpha once per kernel.  No atomics touch alpha: results are
+// deterministic.
+//
+// The far-wing expansion.  With w = 1/v^2 the Hunger far-wing branch A (a <= 0.2, v >= 5) is
+//   H = (a/sqrt(pi)) (w + 1.5 w^2 + 3.75 w^3),
+// and branch D, H = (a/sqrt(pi)) sum_k C_k r^k with r = 1/(v^2 + a^2) = w/(1 + a^2 w) and
+// C = (1, 1.5, 3.75 - 2a^2, -15a^2, 11.75a^4), expands to
+//   H = (a/sqrt(pi)) (w + (1.5 - a^2) w^2 + (3.75 - 5a^2 + a^4) w^3
+//                     + (-26.25a^2 + 10.5a^4 - a^6) w^4 + e5 w^5 + ...),
+//   e5 = 94.25a^4 - 18a^6 + a^8.
+// Since v = s |delta| (delta the offset in DELTA_UNIT, s = DELTA_UNIT/(sigma sqrt 2)), w = R/s^2 with
+// R = 1/delta^2 shared by all layers, so phi = scaling H = sum_{k=1..4} c_k R^k with per-(line,
+// layer) coefficients c_k.  Summed over lines this is a matrix product: alpha[layer, point] +=
+// C[layer, (line,k)] * P[(line,k), point].  The 4-term truncation is exact for branch A and is
+// used for branch D only where |e5| w^4 <= 2e-8; nearer points belong to the near zone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +49,7 @@ using namespace korg;
 namespace {
 
 constexpr int WAVE = 64;
+constexpr double SERIES_EPS = 2e-8;  // relative truncation error allowed in the far-wing series
 
 // --------------------------------------------------------------------------------------------
 // wave helpers
@@ -57,9 +79,7 @@ __device__ inline const KCONST T* uniform_ptr(const T* p) {
     return (const KCONST T*)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ inline double wave_max_nan(double x) {
-    // Julia's maximum: NaN as soon as one element is NaN.
-    if (ballot(x != x)) return __builtin_nan("");
+__device__ inline double wave_max(double x) {
     for (int off = 32; off >= 1; off >>= 1) {
         double y = __shfl_xor(x, off, WAVE);
         x = y > x ? y : x;
@@ -102,52 +122,81 @@ __device__ inline int64_t wave_upper_bound(const T* __restrict__ a, int64_t n, T
     const bool le = p < hi && a[p] <= x;
     return lo + __popcll(ballot(le));
 }
-__device__ inline int64_t wave_lower_bound_i32(const int* a, int64_t n, int x, int lane) {
-    return wave_lower_bound<int>(a, n, x, lane);
-}
-__device__ inline int64_t wave_upper_bound_i32(const int* a, int64_t n, int x, int lane) {
-    return wave_upper_bound<int>(a, n, x, lane);
+
+// Per-lane search with a guess: on a (nearly) evenly spaced ascending array, the index estimated
+// from g0 + i * step is checked against its neighbours and moved at most 3 places.  upper: the
+// first index with a[i] > x, else the first with a[i] >= x.  Returns false when unresolved.
+__device__ inline bool lane_bound_guess(const double* __restrict__ a, int64_t n, double g0,
+                                        double ginv, double x, bool upper, int64_t& out) {
+    if (!(x == x)) return false;
+    double t = (x - g0) * ginv;
+    t = t < -1.0 ? -1.0 : (t > (double)n ? (double)n : t);
+    int64_t j = (int64_t)ceil(t);
+    j = j < 0 ? 0 : (j > n ? n : j);
+    for (int it = 0; it < 4; ++it) {
+        const bool left_ok = j == 0 || (upper ? a[j - 1] <= x : a[j - 1] < x);
+        const bool right_ok = j == n || (upper ? a[j] > x : a[j] >= x);
+        if (left_ok && right_ok) {
+            out = j;
+            return true;
+        }
+        j += left_ok ? 1 : -1;
+    }
+    return false;
 }
 
-// searchsortedfirst / searchsortedlast on a Wavelengths (src/wavelengths.jl:188-210), 0-based:
-// values >= 1 are read as Angstrom; NaN gives an empty window (first = n, last = n - 1).
-__device__ inline int64_t ss_first0(const double* wls, int64_t n, double x, int lane) {
-    if (x >= 1.0) x = x * 1e-8;
-    if (!(x <= wls[n - 1])) return n;
-    return wave_lower_bound<double>(wls, n, x, lane);
-}
-__device__ inline int64_t ss_last0(const double* wls, int64_t n, double x, int lane) {
-    if (x >= 1.0) x = x * 1e-8;
-    if (!(x <= wls[n - 1])) return n - 1;
-    return wave_upper_bound<double>(wls, n, x, lane) - 1;
+// Resolve, for every lane holding a query (active), lower/upper bounds of x on a: guesses first,
+// then the 64-way search for the lanes whose guess failed.  Whole wavefront must call.
+__device__ inline int64_t wave_bounds(const double* __restrict__ a, int64_t n, double g0,
+                                      double ginv, double x, bool upper, bool active, int lane) {
+    int64_t r = 0;
+    const bool ok = active ? lane_bound_guess(a, n, g0, ginv, x, upper, r) : true;
+    uint64_t bad = ballot(!ok);
+    while (bad) {
+        const int b = __ffsll((unsigned long long)bad) - 1;
+        bad &= bad - 1;
+        const double xb = readlane_f64(x, b);
+        const bool ub = __builtin_amdgcn_readlane((int)upper, b) != 0;
+        int64_t v;
+        if (!(xb == xb)) v = n;  // NaN: nothing compares; lower and upper bound are both n
+        else v = ub ? wave_upper_bound<double>(a, n, xb, lane) : wave_lower_bound<double>(a, n, xb, lane);
+        if (lane == b) r = v;
+    }
+    return r;
 }
 
 // --------------------------------------------------------------------------------------------
-// k_layers: per-layer tables (src/line_absorption.jl:39, 54-60, 195, 210) and queue reset
+// k_layers: per-layer tables (src/line_absorption.jl:39, 54-60, 195, 210) and counter reset
 // --------------------------------------------------------------------------------------------
 struct LayerTab {
     double beta, n_eff, t16, t03;
 };
 
 // Device counters of one call (reset by k_layers): [0] profile queue, [1] evaluations,
-// [2] lines used, [3] error flags, [4] core reach, [5] core queue, [6..9] evaluations by branch.
+// [2] lines used, [3] error flags, [4] near-zone reach in points, [5] near queue,
+// [6..9] evaluations by Hunger branch.
 constexpr int N_COUNTERS = 16;
+
+struct GridInfo {
+    double g0, ginv;  // first point and inverse mean spacing of the wavelength grid
+    double k0, kinv;  // the same for the continuum knots
+};
 
 __global__ void k_layers(int L, const double* __restrict__ temps, const double* __restrict__ n_HI,
                          const double* __restrict__ n_HeI, const double* __restrict__ n_H2,
-                         LayerTab* __restrict__ tab, unsigned long long* __restrict__ counters) {
+                         LayerTab* __restrict__ tab, unsigned long long* __restrict__ counters,
+                         const double* __restrict__ wls, int64_t n_wl,
+                         const double* __restrict__ knots, int64_t n_knots,
+                         GridInfo* __restrict__ gi) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l == 0) {
-        counters[0] = 0;  // profile work queue
-        counters[1] = 0;  // evaluation count
-        counters[2] = 0;  // lines used
-        counters[3] = 0;  // set when the line list is not sorted by wavelength
-        counters[4] = 0;  // max distance (grid points) from jc to a core point
-        counters[5] = 0;  // core work queue
-        counters[6] = 0;  // evaluations in branch A (far wing, a <= 0.2)
-        counters[7] = 0;  //                      B (core, a <= 0.2)
-        counters[8] = 0;  //                      C (core, 0.2 < a <= 1.4)
-        counters[9] = 0;  //                      D (far wing, a > 0.2)
+        for (int k = 0; k < N_COUNTERS; ++k) counters[k] = 0;
+        GridInfo g;
+        g.g0 = wls[0];
+        g.ginv = n_wl > 1 ? (double)(n_wl - 1) / (wls[n_wl - 1] - wls[0]) : 0.0;
+        g.k0 = n_knots > 1 ? knots[0] : 0.0;
+        g.kinv = n_knots > 1 ? (double)(n_knots - 1) / (knots[n_knots - 1] - knots[0]) : 0.0;
+        *gi = g;
     }
     if (l >= L) return;
     const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
@@ -169,7 +218,8 @@ __global__ void k_layers(int L, const double* __restrict__ temps, const double* 
 struct SetupArgs {
     int64_t n_lines, n_wl;
     int L;
-    int Ls;  // layers per line in the record arrays (L padded with zero records)
+    int Ls;          // rec64 stride per line (L padded to a whole number of Float64 groups)
+    int n_lblocks;   // 64-layer blocks of the Float32 records
     const double* wls;
     const double *wl, *log_gf, *E_lower, *gamma_rad, *gamma_stark, *vdW1, *vdW2;
     const int32_t* species;
@@ -179,19 +229,17 @@ struct SetupArgs {
     const double* n_div_U;
     const double *temps, *n_e, *xi;
     const LayerTab* tab;
+    const GridInfo* gi;
     int cntm_mode;
     int64_t n_knots;
     const double *knots, *cntm;
     double cutoff;
-    int want_rec32;
-    int n_lblocks;  // layer blocks (each <= 64 layers, aligned with 64-layer chunks)
+    int want_f32;
     int count_branches;
     LineMeta* meta;
     int* jc_all;
-    double4* rec64;  // [line * L + l]: (inv_sigma_sqrt2, a, scaling, 0)
-    float4* recA;    // [line * Ls + l]: (d1, d2, d3, 0) for a <= 0.2, (c1, c2, c3, c4) otherwise
-    float2* recD;    // [line * Ls + l]: (c5, 1/g2) for a > 0.2
-    uint64_t* dmask; // [line * n_lblocks + b]: bit k set when layer 64b+k has a > 0.2
+    double4* rec64;  // [line * Ls + l]: (1/(sigma sqrt 2), a, scaling, 0)
+    float4* recM;    // [(line * n_lblocks + b) * 64 + lane]: far-wing coefficients, MFMA A layout
     unsigned long long* counters;
 };
 
@@ -219,137 +267,147 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         }
         return;
     }
+    if (lane == 0 && i > 0 && A.wl[i - 1] > wl) atomicOr(&A.counters[3], 1ull);
+    const GridInfo gi = *A.gi;
     const double m = A.mass[s];
     const bool mol = A.is_mol[s] != 0;
     const double gamma_rad = A.gamma_rad[i], gamma_stark = A.gamma_stark[i];
     const double vdW1 = A.vdW1[i], vdW2 = A.vdW2[i];
     const double E_lower = A.E_lower[i];
-    if (lane == 0 && i > 0 && A.wl[i - 1] > wl) atomicOr(&A.counters[3], 1ull);
     const double E_upper = E_lower + c_cgs * hplanck_eV / wl;  // src/line_absorption.jl:101
     const double gf = pow(10.0, A.log_gf[i]);
     const double sl = sigma_line(wl);
     int64_t kidx = 0;
     if (A.cntm_mode == KORG_CNTM_GRID) {
         // knot interval holding wl: clamp(searchsortedlast(knots, wl), 1, n_knots - 1) - 1
-        kidx = wave_upper_bound<double>(A.knots, A.n_knots, wl, lane) - 1;
+        int64_t ub = wave_bounds(A.knots, A.n_knots, gi.k0, gi.kinv, wl, true, lane == 0, lane);
+        kidx = __builtin_amdgcn_readfirstlane((int)ub) - 1;
         kidx = kidx < 0 ? 0 : (kidx > A.n_knots - 2 ? A.n_knots - 2 : kidx);
     }
-    double dmax = 0.0, lmax = 0.0, hwmax = -1.0;
+    double dmax = 0.0, lmax = 0.0, rmax = -1.0;
     bool any_nan = false;
     double my_hw = -1.0, my_a = 0.0;  // layer `lane` (first 64 layers), for branch counting
-    for (int l = L + lane; l < A.Ls; l += WAVE) {
-        A.rec64[i * A.Ls + l] = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (A.want_rec32) {
-            A.recA[i * A.Ls + l] = make_float4(0.f, 0.f, 0.f, 0.f);
-            A.recD[i * A.Ls + l] = make_float2(0.f, 0.f);
-        }
-    }
-    for (int l = lane; l < L; l += WAVE) {
-        const LayerTab tb = A.tab[l];
-        const double T = A.temps[l];
-        // src/line_absorption.jl:86-106
-        const double sigma = doppler_width(wl, T, m, A.xi[l]);
-        double G = gamma_rad;
-        if (!mol) {
-            G = G + A.n_e[l] * (gamma_stark * tb.t16);
-            const double vdw = (vdW2 == -1.0) ? vdW1 * tb.t03 : scaled_vdW_abo(vdW1, vdW2, m, T);
-            G = G + tb.n_eff * vdw;
-        }
-        const double gamma = G * (wl * wl) / (c_cgs * (4.0 * PI));
-        const double levels = exp(-tb.beta * E_lower) - exp(-tb.beta * E_upper);
-        const double amp = gf * sl * levels * A.n_div_U[(int64_t)s * L + l];
-        // src/line_absorption.jl:108-112
-        const double c = (A.cntm_mode == KORG_CNTM_GRID)
-                              ? cntm_lerp(A.knots, A.cntm + (int64_t)l * A.n_knots, kidx, wl)
-                              : A.cntm[i * L + l];
-        const double rho = c * A.cutoff / amp;
-        const double dg = inverse_gaussian_density(rho, sigma);
-        const double dl = inverse_lorentz_density(rho, gamma);
-        any_nan |= (dg != dg) || (dl != dl);
-        dmax = dg > dmax ? dg : dmax;
-        lmax = dl > lmax ? dl : lmax;
-        // profile parameters of line_profile (src/line_absorption.jl:245-249)
-        const double inv = 1.0 / (sigma * 1.4142135623730951);
-        const double a = gamma * inv;
-        const double scaling = inv / 1.7724538509055159 * amp;
-        A.rec64[i * A.Ls + l] = make_double4(inv, a, scaling, 0.0);
-        // half-width (cm) of the Voigt-core region of this layer (branches B and C)
-        const double hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
-        hwmax = hw > hwmax ? hw : hwmax;
-        if (l == lane) {
-            my_hw = hw;
-            my_a = a;
-        }
-        if (A.want_rec32) {
-            // Float32 far-wing coefficients, formed in Float64 (offsets in DELTA_UNIT):
-            //   v = |delta| s, s = DELTA_UNIT * inv;  base = scaling * a / sqrt(pi)
-            //   A (a <= 0.2): H = (a/sqrt(pi)) w (1 + 1.5 w + 3.75 w^2), w = 1/v^2 = R/s^2
-            //   D (a > 0.2):  H = (a/sqrt(pi)) sum_k C_k r^k, r = 1/(v^2 + a^2) = rho'/a^2,
-            //                 C = (1, 1.5, 3.75 - 2a^2, -15 a^2, 11.75 a^4),
-            //                 rho' = 1/(delta^2/g2 + 1), g2 = (a/s)^2
-            const double sdel = DELTA_UNIT * inv;
-            const double base = scaling * (a / 1.7724538509055159);
-            const bool kind_d = !(a <= 0.2);
-            if (!kind_d) {
-                const double q = 1.0 / (sdel * sdel);
-                A.recA[i * A.Ls + l] = make_float4((float)(base * q), (float)(1.5 * base * q * q),
-                                                   (float)(3.75 * base * q * q * q), 0.f);
-                A.recD[i * A.Ls + l] = make_float2(0.f, 0.f);
-            } else {
-                const double ia2 = 1.0 / (a * a);
-                const double g2 = a / sdel * (a / sdel);
-                const double ia6 = ia2 * ia2 * ia2;
-                A.recA[i * A.Ls + l] =
-                    make_float4((float)(base * ia2), (float)(1.5 * base * ia2 * ia2),
-                                (float)((3.75 - 2.0 * a * a) * base * ia6), (float)(-15.0 * base * ia6));
-                A.recD[i * A.Ls + l] = make_float2((float)(11.75 * base * ia6), (float)(1.0 / g2));
+    for (int l = L + lane; l < A.Ls; l += WAVE) A.rec64[i * A.Ls + l] = make_double4(0, 0, 0, 0);
+    const int Lm = A.n_lblocks * WAVE;
+    for (int l = lane; l < Lm; l += WAVE) {
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+        if (l < L) {
+            const LayerTab tb = A.tab[l];
+            const double T = A.temps[l];
+            // src/line_absorption.jl:86-106
+            const double sigma = doppler_width(wl, T, m, A.xi[l]);
+            double G = gamma_rad;
+            if (!mol) {
+                G = G + A.n_e[l] * (gamma_stark * tb.t16);
+                const double vdw = (vdW2 == -1.0) ? vdW1 * tb.t03 : scaled_vdW_abo(vdW1, vdW2, m, T);
+                G = G + tb.n_eff * vdw;
             }
-            const uint64_t dm = ballot(kind_d);
-            if (lane == 0) A.dmask[i * A.n_lblocks + l / WAVE] = dm;
+            const double gamma = G * (wl * wl) / (c_cgs * (4.0 * PI));
+            const double levels = exp(-tb.beta * E_lower) - exp(-tb.beta * E_upper);
+            const double amp = gf * sl * levels * A.n_div_U[(int64_t)s * L + l];
+            // src/line_absorption.jl:108-112
+            const double cn = (A.cntm_mode == KORG_CNTM_GRID)
+                                  ? cntm_lerp(A.knots, A.cntm + (int64_t)l * A.n_knots, kidx, wl)
+                                  : A.cntm[i * L + l];
+            const double rho = cn * A.cutoff / amp;
+            const double dg = inverse_gaussian_density(rho, sigma);
+            const double dl = inverse_lorentz_density(rho, gamma);
+            any_nan |= (dg != dg) || (dl != dl);
+            dmax = dg > dmax ? dg : dmax;
+            lmax = dl > lmax ? dl : lmax;
+            // profile parameters of line_profile (src/line_absorption.jl:245-249)
+            const double inv = 1.0 / (sigma * 1.4142135623730951);
+            const double a = gamma * inv;
+            const double scaling = inv / 1.7724538509055159 * amp;
+            A.rec64[i * A.Ls + l] = make_double4(inv, a, scaling, 0.0);
+            // near zone of this layer, as v: the Voigt core (branches B and C) and, for a > 0.2,
+            // where the 4-term far-wing series is not converged to SERIES_EPS
+            double vr;
+            if (a <= 0.2) {
+                vr = 5.0;
+            } else {
+                const double a2 = a * a;
+                const double e5 = fabs(94.25 * a2 * a2 - 18.0 * a2 * a2 * a2 + a2 * a2 * a2 * a2);
+                const double vs = sqrt(sqrt(sqrt(e5 / SERIES_EPS)));
+                vr = fmax(fmax(vs, 2.0 * a), 3.2);
+            }
+            const double r = vr / inv;
+            rmax = r > rmax ? r : rmax;
+            if (l == lane) {
+                my_hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
+                my_a = a;
+            }
+            if (A.want_f32) {
+                const double sd = DELTA_UNIT * inv;
+                const double q = 1.0 / (sd * sd);
+                const double base = scaling * (a / 1.7724538509055159);
+                double e2 = 1.5, e3 = 3.75, e4 = 0.0;
+                if (!(a <= 0.2)) {
+                    const double a2 = a * a;
+                    e2 = 1.5 - a2;
+                    e3 = 3.75 - 5.0 * a2 + a2 * a2;
+                    e4 = -26.25 * a2 + 10.5 * a2 * a2 - a2 * a2 * a2;
+                }
+                c[0] = base * q;
+                c[1] = base * e2 * q * q;
+                c[2] = base * e3 * q * q * q;
+                c[3] = base * e4 * (q * q) * (q * q);
+            }
+        }
+        if (A.want_f32) {
+            // MFMA A-operand layout: lane i of step s holds coefficient k = 2s + i/32 of layer
+            // 32 mb + i%32; slot 2s + mb of float4 i.
+            float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
+            const int ll = l % WAVE, mb = ll >> 5, row = ll & 31;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int li = (k & 1) * 32 + row;
+                const int slot = (k >> 1) * 2 + mb;
+                dst[li * 4 + slot] = (float)c[k];
+            }
         }
     }
-    // src/line_absorption.jl:109-115: windows are maxima over layers
+    // src/line_absorption.jl:109-115: windows are maxima over layers (NaN if any is NaN)
     const bool wnan = ballot(any_nan) != 0;
-    double dwin = wave_max_nan(dmax);
-    double lwin = wave_max_nan(lmax);
-    double hw = hwmax;
-    for (int off = 32; off >= 1; off >>= 1) {
-        double y = __shfl_xor(hw, off, WAVE);
-        hw = y > hw ? y : hw;
-    }
+    const double dwin = wave_max(dmax);
+    const double lwin = wave_max(lmax);
+    const double reach = wave_max(rmax);
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
     const int64_t n = A.n_wl;
-    const int64_t lb = ss_first0(A.wls, n, wl - window, lane);
-    const int64_t ub = ss_last0(A.wls, n, wl + window, lane);
-    const int64_t jc = wave_lower_bound<double>(A.wls, n, wl, lane);
+    const double last = A.wls[n - 1];
+    // lanes 0..4 search for lb, ub, jc, near-zone start and end in parallel
+    double x = wl;
+    bool upper = false;
+    const double h = reach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
+    if (lane == 0) x = wl - window;
+    if (lane == 1) { x = wl + window; upper = true; }
+    if (lane == 3) x = wl - h;
+    if (lane == 4) { x = wl + h; upper = true; }
+    if (lane <= 1 && x >= 1.0) x = x * 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
+    const bool past = !(x <= last);           // beyond the grid (or NaN)
+    const int64_t r = wave_bounds(A.wls, n, gi.g0, gi.ginv, x, upper, lane < 5 && !past, lane);
+    int64_t res = r;
+    if (lane < 5 && past) res = n;  // lower bound n; upper bound n (last index n - 1)
+    const int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
+    const int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
+    const int64_t jc = __builtin_amdgcn_readlane((int)res, 2);
     int64_t clo = 1, chi = 0;
-    if (lb <= ub && hw >= 0.0) {
-        // conservative superset of the core region; grid points outside it are far wing in
-        // every layer
-        const double h = hw * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
-        clo = wave_lower_bound<double>(A.wls, n, wl - h, lane);
-        chi = wave_upper_bound<double>(A.wls, n, wl + h, lane) - 1;
+    if (lb <= ub && reach >= 0.0) {
+        clo = __builtin_amdgcn_readlane((int)res, 3);
+        chi = (int64_t)__builtin_amdgcn_readlane((int)res, 4) - 1;
         clo = clo < lb ? lb : clo;
         chi = chi > ub ? ub : chi;
     }
-    if (A.count_branches && lb <= ub && lane < L && lane < WAVE) {
-        // evaluations of this layer by branch: core points lie within my_hw of lam0
+    if (A.count_branches && lb <= ub && lane < L) {
+        // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
         int64_t core_n = 0;
         if (my_hw >= 0.0) {
-            int64_t lo = 0, hi = n;
-            const double xl = wl - my_hw, xr = wl + my_hw;
-            while (lo < hi) {  // first index with wls >= xl... strictly inside: > xl
-                const int64_t mid = lo + (hi - lo) / 2;
-                if (A.wls[mid] <= xl) lo = mid + 1; else hi = mid;
-            }
-            int64_t c0 = lo;
-            lo = 0;
-            hi = n;
-            while (lo < hi) {  // first index with wls >= xr
-                const int64_t mid = lo + (hi - lo) / 2;
-                if (A.wls[mid] < xr) lo = mid + 1; else hi = mid;
-            }
-            int64_t c1 = lo - 1;
+            int64_t c0 = 0, c1 = 0;
+            if (!lane_bound_guess(A.wls, n, gi.g0, gi.ginv, wl - my_hw, true, c0))
+                c0 = lb;  // counting only: fall back to the window edge
+            if (!lane_bound_guess(A.wls, n, gi.g0, gi.ginv, wl + my_hw, false, c1))
+                c1 = ub + 1;
+            c1 -= 1;
             c0 = c0 < lb ? lb : c0;
             c1 = c1 > ub ? ub : c1;
             core_n = c1 >= c0 ? c1 - c0 + 1 : 0;
@@ -387,9 +445,6 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     }
 }
 
-// --------------------------------------------------------------------------------------------
-// Stable partition of lines by class: counts per block, scan, scatter (deterministic).
-// --------------------------------------------------------------------------------------------
 constexpr int PART_BLOCK = 256;
 
 __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, const LineMeta* __restrict__ meta,
@@ -504,16 +559,14 @@ __global__ void k_ranges(int ntiles, int tile_w, int64_t n_wl, const int* __rest
 struct ProfileArgs {
     int64_t n_wl;
     int L;
-    int Ls;  // record stride per line (layers padded to whole blocks)
-    int n_lblocks;
+    int Ls;          // rec64 stride per line
+    int n_lblocks;   // layer blocks of a work item (F64_GROUP layers, or 64 for the MFMA kernel)
     int ntiles, tile_w;
     int64_t n_items;
     const double* wls;
     const LineMeta* meta;
     const double4* rec64;
-    const float4* recA;
-    const float2* recD;
-    const uint64_t* dmask;
+    const float4* recM;
     const int* cls_lines;
     const int2* ranges;
     void* alpha;
@@ -521,8 +574,16 @@ struct ProfileArgs {
     unsigned long long* counters;
 };
 
+// The wavefront-uniform fields of one line, taken from the candidate lane that loaded them.
+struct LineView {
+    int line;
+    double lam0;
+    int lb, ub, clo, chi;
+};
+
 // Visit, in a fixed order (class by class, line index within a class), every line whose window
-// overlaps grid points [j0, j1] of this tile; calls f(line) with a wave-uniform line index.
+// overlaps grid points [j0, j1] of this tile; f(view, next_line) is called with wave-uniform
+// values (next_line: the following line of the same batch, or -1, for prefetching).
 template <class F>
 __device__ inline void for_each_line(const ProfileArgs& A, int tile, int64_t j0, int64_t j1,
                                      int lane, F&& f) {
@@ -531,17 +592,26 @@ __device__ inline void for_each_line(const ProfileArgs& A, int tile, int64_t j0,
         for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
             const int k = k0 + lane;
             int line = 0;
+            LineMeta m = {0.0, 1, 0, 1, 0, 0, 0};
             bool ovl = false;
             if (k < r.y) {
                 line = A.cls_lines[k];
-                const LineMeta m = A.meta[line];
+                m = A.meta[line];
                 ovl = m.lb <= j1 && m.ub >= j0;
             }
             uint64_t mask = ballot(ovl);
             while (mask) {
                 const int b = __ffsll((unsigned long long)mask) - 1;
                 mask &= mask - 1;
-                f(readlane_i32(line, b));
+                LineView v;
+                v.line = readlane_i32(line, b);
+                v.lam0 = readlane_f64(m.lam0, b);
+                v.lb = readlane_i32(m.lb, b);
+                v.ub = readlane_i32(m.ub, b);
+                v.clo = readlane_i32(m.clo, b);
+                v.chi = readlane_i32(m.chi, b);
+                const int nb = mask ? readlane_i32(line, __ffsll((unsigned long long)mask) - 1) : -1;
+                f(v, nb);
             }
         }
     }
@@ -553,13 +623,19 @@ __device__ inline int wave_take(unsigned long long* counter, int lane) {
     return __builtin_amdgcn_readfirstlane((int)v);
 }
 
+template <class T>
+__device__ inline void add_alpha(void* alpha, int f32, int64_t k, T v) {
+    if (f32) ((float*)alpha)[k] += (float)v;
+    else ((double*)alpha)[k] += (double)v;
+}
+
 // Float64 form: the reference's line_profile / voigt_hjerting per evaluation.  A work item is
 // (tile, group of G layers); the G x P accumulators stay in registers.
 template <int G, int P>
 __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
-    if (A.counters[3] != 0) return;  // unsorted line list: nothing is written
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written
     for (;;) {
         const int item = wave_take(&A.counters[0], lane);
         if (item >= A.n_items) break;
@@ -580,18 +656,15 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
 #pragma unroll
             for (int l = 0; l < G; ++l) acc[p][l] = 0.0;
 
-        for_each_line(A, tile, j0, j1, lane, [&](int line) {
-            const KCONST LineMeta* mp = uniform_ptr(A.meta + line);
-            const double lam0 = mp->lam0;
-            const int lb = mp->lb, ub = mp->ub;
+        for_each_line(A, tile, j0, j1, lane, [&](const LineView& v, int) {
             double dl[P];
             bool inw[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                inw[p] = jj[p] >= lb && jj[p] <= ub;
-                dl[p] = fabs(lam[p] - lam0);  // abs(λ - λ₀), src/line_absorption.jl:248
+                inw[p] = jj[p] >= v.lb && jj[p] <= v.ub;
+                dl[p] = fabs(lam[p] - v.lam0);  // abs(λ - λ₀), src/line_absorption.jl:248
             }
-            const KCONST double4* rp = uniform_ptr(A.rec64 + (int64_t)line * A.Ls + l0);
+            const KCONST double4* rp = uniform_ptr(A.rec64 + (int64_t)v.line * A.Ls + l0);
 #pragma unroll
             for (int l = 0; l < G; ++l) {
                 const double4 r = rp[l];  // (1/(σ√2), a, scaling); zero for padding layers
@@ -607,170 +680,124 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
         for (int p = 0; p < P; ++p) {
             if (jj[p] > j1) continue;
 #pragma unroll
-            for (int l = 0; l < G; ++l) {
-                if (l0 + l < L) {
-                    const int64_t k = (int64_t)(l0 + l) + (int64_t)L * jj[p];
-                    if (A.alpha_f32) ((float*)A.alpha)[k] += (float)acc[p][l];
-                    else ((double*)A.alpha)[k] += acc[p][l];
-                }
-            }
+            for (int l = 0; l < G; ++l)
+                if (l0 + l < L) add_alpha(A.alpha, A.alpha_f32, (int64_t)(l0 + l) + (int64_t)L * jj[p], acc[p][l]);
         }
     }
 }
 
 // Which Hunger branch (src/line_absorption.jl:286-310) applies, decided in Float64 exactly as
-// voigt_hjerting does: true for the Voigt-core branches (B: a <= 0.2 and v < 5;
-// C: 0.2 < a <= 1.4 and a + v < 3.2), false for the far-wing branches (A and D).
-__device__ inline bool is_core_branch(double a, double v) {
-    if (a <= 0.2) return !(v >= 5.0);
-    if (a <= 1.4) return a + v < 3.2;
-    return false;
+// voigt_hjerting does: 0 = A (far wing, a <= 0.2), 1 = B (core, a <= 0.2), 2 = C (core,
+// 0.2 < a <= 1.4), 3 = D (far wing / strongly damped).
+__device__ inline int hunger_branch(double a, double v) {
+    if (a <= 0.2 && v >= 5.0) return 0;
+    if (a <= 0.2) return 1;
+    if (a <= 1.4 && a + v < 3.2) return 2;
+    return 3;
 }
 
-// Float32 form with Float64 fix-ups, far-wing branches only (A and D of voigt_hjerting).
-// Lanes = wavelength points, LB layers per item in registers.  Where a Voigt core may lie in the
-// wavefront's points, core evaluations are masked out with the exact Float64 branch test; they
-// are added by k_core_f32.
-template <int LB, int P>
-__global__ __launch_bounds__(WAVE) void k_profile_f32(ProfileArgs A) {
+// ---------------------------------------------------------------------------------------------
+// Float32 far wing on the matrix cores.  A work item is (tile of 32*NB points, block of 64
+// layers); the 64 x 32NB block of alpha is held as 2 x NB 32x32 MFMA accumulators.  For each line
+// that overlaps the tile: A = the line's 64 x 4 coefficient block, B = (R, R^2, R^3, R^4) of the
+// tile's points (zero outside the window and inside the near zone), acc += A B with
+// v_mfma_f32_32x32x2_f32 (K = 2 per instruction, two steps).
+// ---------------------------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int NB>
+__global__ __launch_bounds__(WAVE) void k_wing_mfma(ProfileArgs A) {
     const int lane = threadIdx.x;
+    const int n = lane & 31;      // column (point) within a 32-point block
+    const bool hi = lane >= 32;   // K half: coefficient/power index k = 2 s + hi
     const int L = A.L;
-    if (A.counters[3] != 0) return;  // unsorted line list: nothing is written
+    if (A.counters[3] != 0) return;
     for (;;) {
         const int item = wave_take(&A.counters[0], lane);
         if (item >= A.n_items) break;
         const int tile = item / A.n_lblocks;
-        const int l0 = (item % A.n_lblocks) * LB;
+        const int lblk = item % A.n_lblocks;
         const int64_t j0 = (int64_t)tile * A.tile_w;
         const int64_t j1 = min(j0 + A.tile_w, A.n_wl) - 1;
-        double lam[P];
-        int jj[P];
+        double lam[NB];
+        int jj[NB];
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            jj[p] = (int)(j0 + p * WAVE + lane);
-            lam[p] = jj[p] <= j1 ? A.wls[jj[p]] : 0.0;
+        for (int b = 0; b < NB; ++b) {
+            jj[b] = (int)(j0 + 32 * b + n);
+            lam[b] = jj[b] <= j1 ? A.wls[jj[b]] : 0.0;
         }
-        float acc[P][LB];
+        floatx16 acc[2][NB];
 #pragma unroll
-        for (int p = 0; p < P; ++p)
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int l = 0; l < LB; ++l) acc[p][l] = 0.0f;
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[mb][b][r] = 0.0f;
 
-        for_each_line(A, tile, j0, j1, lane, [&](int line) {
-            const KCONST LineMeta* mp = uniform_ptr(A.meta + line);
-            const double lam0 = mp->lam0;
-            const int lb = mp->lb, ub = mp->ub, clo = mp->clo, chi = mp->chi;
-            // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values, scaled and
-            // rounded once to Float32.  R = 1/delta^2 is shared by all layers.
-            double dl[P];
-            float d2[P], R[P], R2[P], R3[P];
-            bool inw[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                inw[p] = jj[p] >= lb && jj[p] <= ub;
-                dl[p] = lam[p] - lam0;
-                const float d = (float)(dl[p] * (1.0 / DELTA_UNIT));
-                const float dd = d * d;
-                const float r = __builtin_amdgcn_rcpf(dd);
-                R[p] = inw[p] ? r : 0.0f;
-                R2[p] = R[p] * R[p];
-                R3[p] = R2[p] * R[p];
-                d2[p] = inw[p] ? dd : __builtin_huge_valf();
+        const float4* recM = A.recM;
+        float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool have_nxt = false;
+        int nxt_line = -1;
+        for_each_line(A, tile, j0, j1, lane, [&](const LineView& v, int next_line) {
+            // coefficient block of this line (prefetched while the previous line computed)
+            float4 a4;
+            if (have_nxt && nxt_line == v.line) a4 = nxt;
+            else a4 = recM[((int64_t)v.line * A.n_lblocks + lblk) * WAVE + lane];
+            have_nxt = false;
+            if (next_line >= 0) {
+                nxt = recM[((int64_t)next_line * A.n_lblocks + lblk) * WAVE + lane];
+                nxt_line = next_line;
+                have_nxt = true;
             }
-            const bool core = clo <= chi && !(chi < j0 || clo > j1);
-            const KCONST float4* ra = uniform_ptr(A.recA + (int64_t)line * A.Ls + l0);
-            const KCONST float2* rd = uniform_ptr(A.recD + (int64_t)line * A.Ls + l0);
-            const uint64_t dm = *uniform_ptr(A.dmask + (int64_t)line * A.n_lblocks + l0 / WAVE);
-            if (!core) {
-                // records of 8 layers are fetched together: one scalar-load wait per 8 layers
 #pragma unroll
-                for (int g = 0; g < LB; g += 8) {
-                    float4 r[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) r[k] = ra[g + k];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const int l = g + k;
-                        if (!((dm >> l) & 1)) {
-#pragma unroll
-                            for (int p = 0; p < P; ++p) {
-                                float a = acc[p][l];
-                                a = fmaf(r[k].x, R[p], a);
-                                a = fmaf(r[k].y, R2[p], a);
-                                a = fmaf(r[k].z, R3[p], a);
-                                acc[p][l] = a;
-                            }
-                        } else {
-                            const float2 e = rd[l];
-#pragma unroll
-                            for (int p = 0; p < P; ++p) {
-                                const float rho = __builtin_amdgcn_rcpf(fmaf(d2[p], e.y, 1.0f));
-                                float h = fmaf(rho, e.x, r[k].w);
-                                h = fmaf(h, rho, r[k].z);
-                                h = fmaf(h, rho, r[k].y);
-                                h = fmaf(h, rho, r[k].x);
-                                acc[p][l] = fmaf(h, rho, acc[p][l]);
-                            }
-                        }
-                    }
-                }
-            } else {
-                double adl[P];
-#pragma unroll
-                for (int p = 0; p < P; ++p) adl[p] = fabs(dl[p]);
-                const KCONST double4* rq = uniform_ptr(A.rec64 + (int64_t)line * A.Ls + l0);
-#pragma unroll
-                for (int l = 0; l < LB; ++l) {
-                    const float4 r = ra[l];
-                    const double4 q = rq[l];
-                    const bool kind_d = (dm >> l) & 1;
-                    float2 e = make_float2(0.f, 0.f);
-                    if (kind_d) e = rd[l];
-#pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        float w;
-                        if (!kind_d) {
-                            w = fmaf(r.z, R3[p], fmaf(r.y, R2[p], r.x * R[p]));
-                        } else {
-                            const float rho = __builtin_amdgcn_rcpf(fmaf(d2[p], e.y, 1.0f));
-                            float h = fmaf(rho, e.x, r.w);
-                            h = fmaf(h, rho, r.z);
-                            h = fmaf(h, rho, r.y);
-                            h = fmaf(h, rho, r.x);
-                            w = h * rho;
-                        }
-                        const bool wing = inw[p] && !is_core_branch(q.y, adl[p] * q.x);
-                        acc[p][l] += wing ? w : 0.0f;
-                    }
-                }
+            for (int b = 0; b < NB; ++b) {
+                const int c0 = (int)j0 + 32 * b;
+                if (v.ub < c0 || v.lb > c0 + 31) continue;              // block outside window
+                if (v.clo <= c0 && v.chi >= c0 + 31) continue;          // block inside near zone
+                const int j = jj[b];
+                const bool use = j >= v.lb && j <= v.ub && !(j >= v.clo && j <= v.chi);
+                // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values
+                const float d = (float)((lam[b] - v.lam0) * (1.0 / DELTA_UNIT));
+                float R = __builtin_amdgcn_rcpf(d * d);
+                R = use ? R : 0.0f;
+                const float R2 = R * R;
+                const float b0 = hi ? R2 : R;    // k = 0 | 1: R | R^2
+                const float b1 = b0 * R2;        // k = 2 | 3: R^3 | R^4
+                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, b0, acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, b0, acc[1][b], 0, 0, 0);
+                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1, acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1, acc[1][b], 0, 0, 0);
             }
         });
 
+        // accumulator layout of v_mfma_f32_32x32x2_f32: lane i, register r holds
+        // row 8*(r/4) + 4*(i/32) + r%4, column i%32
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            if (jj[p] > j1) continue;
+        for (int b = 0; b < NB; ++b) {
+            if (jj[b] > j1) continue;
 #pragma unroll
-            for (int l = 0; l < LB; ++l) {
-                if (l0 + l < L) {
-                    const int64_t k = (int64_t)(l0 + l) + (int64_t)L * jj[p];
-                    if (A.alpha_f32) ((float*)A.alpha)[k] += acc[p][l];
-                    else ((double*)A.alpha)[k] += (double)acc[p][l];
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
+                    if (layer < L)
+                        add_alpha(A.alpha, A.alpha_f32, (int64_t)layer + (int64_t)L * jj[b], acc[mb][b][r]);
                 }
-            }
         }
     }
 }
 
-// Voigt-core evaluations of the Float32 form (branches B and C), lanes = layers.  A work item is
-// (tile of CT points, block of 64 layers); for every line whose core range [clo, chi] meets the
-// tile, in line order, each lane evaluates its layer at the tile's points that pass the exact
-// Float64 branch test.  Runs after k_profile_f32 and adds its own sums into alpha.
+// ---------------------------------------------------------------------------------------------
+// Float32 near zone, lanes = layers.  A work item is (tile of CT points, block of 64 layers);
+// for every line whose near zone [clo, chi] meets the tile (in line order), each lane evaluates
+// its layer exactly, branch by branch, at the tile's points of the zone.  Runs after
+// k_wing_mfma and adds its own sums into alpha.
+// ---------------------------------------------------------------------------------------------
 constexpr int CT = 32;
 
-struct CoreArgs {
+struct NearArgs {
     int64_t n_wl, n_lines;
-    int L;
-    int Ls;
+    int L, Ls;
     int n_lblocks, ntiles;
     int64_t n_items;
     const double* wls;
@@ -782,11 +809,26 @@ struct CoreArgs {
     unsigned long long* counters;
 };
 
-__global__ __launch_bounds__(WAVE) void k_core_f32(CoreArgs A) {
+__device__ inline float voigt_far_f32(float a, float v, bool branch_d) {
+    const float isp = 0.5641895835477563f;  // 1/sqrt(pi)
+    if (!branch_d) {
+        // branch A: (a/sqrt(pi)) w (1 + 1.5 w + 3.75 w^2), w = 1/v^2
+        const float w = __builtin_amdgcn_rcpf(v * v);
+        return (a * isp * w) * (1.0f + 1.5f * w + 3.75f * (w * w));
+    }
+    // branch D
+    const float r2 = v * v * __builtin_amdgcn_rcpf(a * a);
+    const float a_invu = 0.70710678118654752f * __builtin_amdgcn_rcpf((r2 + 1.0f) * a);
+    const float a2 = a_invu * a_invu;
+    return 0.79788456080286536f * a_invu *
+           (1.0f + (3.0f * r2 - 1.0f + ((r2 - 2.0f) * 15.0f * r2 + 2.0f) * a2) * a2);
+}
+
+__global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
     if (A.counters[3] != 0) return;
-    const int cw = (int)A.counters[4];  // max distance from jc to a core point, in points
+    const int cw = (int)A.counters[4];  // max distance from jc to a near-zone point, in points
     for (;;) {
         const int item = wave_take(&A.counters[5], lane);
         if (item >= A.n_items) break;
@@ -799,8 +841,8 @@ __global__ __launch_bounds__(WAVE) void k_core_f32(CoreArgs A) {
 #pragma unroll
         for (int t = 0; t < CT; ++t) acc[t] = 0.0f;
         // candidate lines: jc in [j0 - cw - 1, j1 + cw + 1] (jc is monotone in the line index)
-        const int64_t i0 = wave_lower_bound_i32(A.jc_all, A.n_lines, (int)(j0 - cw - 1), lane);
-        const int64_t i1 = wave_upper_bound_i32(A.jc_all, A.n_lines, (int)(j1 + cw + 1), lane);
+        const int64_t i0 = wave_lower_bound<int>(A.jc_all, A.n_lines, (int)(j0 - cw - 1), lane);
+        const int64_t i1 = wave_upper_bound<int>(A.jc_all, A.n_lines, (int)(j1 + cw + 1), lane);
         for (int64_t i = i0; i < i1; ++i) {
             const int line = __builtin_amdgcn_readfirstlane((int)i);
             const LineMeta m = *uniform_ptr(A.meta + line);
@@ -812,21 +854,22 @@ __global__ __launch_bounds__(WAVE) void k_core_f32(CoreArgs A) {
                 const int64_t j = j0 + t;
                 if (j < m.clo || j > m.chi) continue;  // wavefront-uniform
                 const double v64 = fabs(*uniform_ptr(A.wls + j) - m.lam0) * q.x;
-                if (lv && is_core_branch(q.y, v64)) {
+                const int br = hunger_branch(q.y, v64);
+                float h;
+                if (br == 1 || br == 2) {
                     const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
-                    acc[t] += voigt_core_f32(a32, (float)v64, piece, q.y > 0.2) * sc;
+                    h = voigt_core_f32(a32, (float)v64, piece, br == 2);
+                } else {
+                    h = voigt_far_f32(a32, (float)v64, br == 3);
                 }
+                if (lv) acc[t] += h * sc;
             }
         }
         if (lv) {
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
                 const int64_t j = j0 + t;
-                if (j <= j1) {
-                    const int64_t k = (int64_t)l + (int64_t)L * j;
-                    if (A.alpha_f32) ((float*)A.alpha)[k] += acc[t];
-                    else ((double*)A.alpha)[k] += (double)acc[t];
-                }
+                if (j <= j1) add_alpha(A.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * j, acc[t]);
             }
         }
     }
@@ -853,7 +896,7 @@ struct korg_ctx {
     korg_tuning tuning{};
     int64_t last_n_lines = 0;
     // device scratch (grow-only)
-    DevBuf meta, jc_all, rec64, recA, recD, dmask, tab, counters, blk, cstart, cls_lines, cls_jc, ranges;
+    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_lines, cls_jc, ranges;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -902,32 +945,8 @@ int grow(korg_ctx* ctx, DevBuf& b, size_t bytes) {
         if (rc_ != KORG_OK) return rc_; \
     } while (0)
 
-struct Plan {
-    int P;     // wavelength points per lane
-    int LB;    // layers per work item
-    int wpc;   // persistent wavefronts per compute unit
-};
-
 constexpr int F64_GROUP = 8;  // layers per work item of the Float64 form
-
-Plan choose_plan(const korg_ctx* ctx, int L, int precision) {
-    Plan p;
-    if (precision == KORG_PRECISION_FP64) {
-        p.P = ctx->tuning.points_per_lane == 2 ? 2 : 1;
-        p.LB = F64_GROUP;
-    } else {
-        p.P = ctx->tuning.points_per_lane > 0 ? (ctx->tuning.points_per_lane >= 2 ? 2 : 1) : 2;
-        p.LB = L <= 8 ? 8 : L <= 16 ? 16 : L <= 32 ? 32 : L <= 56 ? 56 : 64;
-    }
-    p.wpc = ctx->tuning.waves_per_cu > 0 ? ctx->tuning.waves_per_cu : 12;
-    return p;
-}
-
-template <int LB>
-void launch_profile_f32(int P, dim3 grid, hipStream_t s, const ProfileArgs& A) {
-    if (P == 2) hipLaunchKernelGGL((k_profile_f32<LB, 2>), grid, dim3(WAVE), 0, s, A);
-    else hipLaunchKernelGGL((k_profile_f32<LB, 1>), grid, dim3(WAVE), 0, s, A);
-}
+constexpr int MFMA_NB = 4;    // 32-point blocks per work item of the far-wing MFMA kernel
 
 int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg_species* sp,
              const korg_layers* ly, const korg_continuum* cn, int alpha_dtype, int precision) {
@@ -964,24 +983,23 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     hipStream_t s = ctx->stream;
-    const Plan plan = choose_plan(ctx, L, precision);
-    const int tile_w = WAVE * plan.P;
-    const int ntiles = (int)((nwl + tile_w - 1) / tile_w);
-    const int n_lblocks = (L + plan.LB - 1) / plan.LB;
-    const int Ls = n_lblocks * plan.LB;  // records padded to whole layer blocks
-    const int nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
     const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
+    const int P64 = ctx->tuning.points_per_lane == 2 ? 2 : 1;
+    const int tile_w = f32 ? 32 * MFMA_NB : WAVE * P64;
+    const int ntiles = (int)((nwl + tile_w - 1) / tile_w);
+    const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
+    const int Ls = n_groups * F64_GROUP;              // rec64 stride: whole Float64 groups
+    const int n_lblocks = (L + WAVE - 1) / WAVE;      // 64-layer blocks (Float32 records)
+    const int nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
+    const int wpc = ctx->tuning.waves_per_cu > 0 ? ctx->tuning.waves_per_cu : 12;
 
     TRY(grow(ctx, ctx->meta, sizeof(LineMeta) * N));
     TRY(grow(ctx, ctx->jc_all, sizeof(int) * N));
     TRY(grow(ctx, ctx->rec64, sizeof(double4) * N * Ls));
-    if (f32) {
-        TRY(grow(ctx, ctx->recA, sizeof(float4) * N * Ls));
-        TRY(grow(ctx, ctx->recD, sizeof(float2) * N * Ls));
-        TRY(grow(ctx, ctx->dmask, sizeof(uint64_t) * N * n_lblocks));
-    }
+    if (f32) TRY(grow(ctx, ctx->recM, sizeof(float4) * (size_t)N * n_lblocks * WAVE));
     TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * L));
     TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * N_COUNTERS));
+    TRY(grow(ctx, ctx->gridinfo, sizeof(GridInfo)));
     TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
     TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
     TRY(grow(ctx, ctx->cls_lines, sizeof(int) * N));
@@ -989,14 +1007,17 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
 
     auto* counters = (unsigned long long*)ctx->counters.p;
+    auto* gi = (GridInfo*)ctx->gridinfo.p;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
     hipLaunchKernelGGL(k_layers, dim3((L + 63) / 64), dim3(64), 0, s, L, ly->temps, ly->n_HI,
-                       ly->n_HeI, ly->n_H2, (LayerTab*)ctx->tab.p, counters);
+                       ly->n_HeI, ly->n_H2, (LayerTab*)ctx->tab.p, counters, g->all_wls, nwl,
+                       cn->knots, cn->mode == KORG_CNTM_GRID ? cn->n_knots : (int64_t)0, gi);
     SetupArgs sa;
     sa.n_lines = N;
     sa.n_wl = nwl;
     sa.L = L;
     sa.Ls = Ls;
+    sa.n_lblocks = n_lblocks;
     sa.wls = g->all_wls;
     sa.wl = ln->wl;
     sa.log_gf = ln->log_gf;
@@ -1014,20 +1035,18 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     sa.n_e = ly->n_e;
     sa.xi = ly->xi;
     sa.tab = (const LayerTab*)ctx->tab.p;
+    sa.gi = gi;
     sa.cntm_mode = cn->mode;
     sa.n_knots = cn->n_knots;
     sa.knots = cn->knots;
     sa.cntm = cn->values;
     sa.cutoff = cutoff;
-    sa.want_rec32 = f32 ? 1 : 0;
-    sa.n_lblocks = n_lblocks;
+    sa.want_f32 = f32 ? 1 : 0;
     sa.count_branches = ctx->tuning.count_branches ? 1 : 0;
     sa.meta = (LineMeta*)ctx->meta.p;
     sa.jc_all = (int*)ctx->jc_all.p;
     sa.rec64 = (double4*)ctx->rec64.p;
-    sa.recA = (float4*)ctx->recA.p;
-    sa.recD = (float2*)ctx->recD.p;
-    sa.dmask = (uint64_t*)ctx->dmask.p;
+    sa.recM = (float4*)ctx->recM.p;
     sa.counters = counters;
     hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
     HIP_TRY(ctx, hipGetLastError());
@@ -1050,58 +1069,51 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     pa.n_wl = nwl;
     pa.L = L;
     pa.Ls = Ls;
-    pa.n_lblocks = n_lblocks;
+    pa.n_lblocks = f32 ? n_lblocks : n_groups;
     pa.ntiles = ntiles;
     pa.tile_w = tile_w;
-    pa.n_items = (int64_t)ntiles * n_lblocks;
+    pa.n_items = (int64_t)ntiles * pa.n_lblocks;
     pa.wls = g->all_wls;
     pa.meta = (const LineMeta*)ctx->meta.p;
     pa.rec64 = (const double4*)ctx->rec64.p;
-    pa.recA = (const float4*)ctx->recA.p;
-    pa.recD = (const float2*)ctx->recD.p;
-    pa.dmask = (const uint64_t*)ctx->dmask.p;
+    pa.recM = (const float4*)ctx->recM.p;
     pa.cls_lines = (const int*)ctx->cls_lines.p;
     pa.ranges = (const int2*)ctx->ranges.p;
     pa.alpha = alpha;
     pa.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
     pa.counters = counters;
-    int64_t nwaves = (int64_t)ctx->n_cu * plan.wpc;
+    int64_t nwaves = (int64_t)ctx->n_cu * wpc;
     if (nwaves > pa.n_items) nwaves = pa.n_items;
     const dim3 grid((unsigned)nwaves);
     if (!f32) {
-        if (plan.P == 2) hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 2>), grid, dim3(WAVE), 0, s, pa);
+        if (P64 == 2) hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 2>), grid, dim3(WAVE), 0, s, pa);
         else hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 1>), grid, dim3(WAVE), 0, s, pa);
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-    } else {
-        switch (plan.LB) {
-            case 8: launch_profile_f32<8>(plan.P, grid, s, pa); break;
-            case 16: launch_profile_f32<16>(plan.P, grid, s, pa); break;
-            case 32: launch_profile_f32<32>(plan.P, grid, s, pa); break;
-            case 56: launch_profile_f32<56>(plan.P, grid, s, pa); break;
-            default: launch_profile_f32<64>(plan.P, grid, s, pa); break;
-        }
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-        CoreArgs ca;
-        ca.n_wl = nwl;
-        ca.n_lines = N;
-        ca.L = L;
-        ca.Ls = Ls;
-        ca.n_lblocks = (L + WAVE - 1) / WAVE;
-        ca.ntiles = (int)((nwl + CT - 1) / CT);
-        ca.n_items = (int64_t)ca.ntiles * ca.n_lblocks;
-        ca.wls = g->all_wls;
-        ca.meta = (const LineMeta*)ctx->meta.p;
-        ca.jc_all = (const int*)ctx->jc_all.p;
-        ca.rec64 = (const double4*)ctx->rec64.p;
-        ca.alpha = alpha;
-        ca.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
-        ca.counters = counters;
+    } else {
+        hipLaunchKernelGGL((k_wing_mfma<MFMA_NB>), grid, dim3(WAVE), 0, s, pa);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        NearArgs na;
+        na.n_wl = nwl;
+        na.n_lines = N;
+        na.L = L;
+        na.Ls = Ls;
+        na.n_lblocks = n_lblocks;
+        na.ntiles = (int)((nwl + CT - 1) / CT);
+        na.n_items = (int64_t)na.ntiles * n_lblocks;
+        na.wls = g->all_wls;
+        na.meta = (const LineMeta*)ctx->meta.p;
+        na.jc_all = (const int*)ctx->jc_all.p;
+        na.rec64 = (const double4*)ctx->rec64.p;
+        na.alpha = alpha;
+        na.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+        na.counters = counters;
         int64_t cwaves = (int64_t)ctx->n_cu * 16;
-        if (cwaves > ca.n_items) cwaves = ca.n_items;
-        hipLaunchKernelGGL(k_core_f32, dim3((unsigned)cwaves), dim3(WAVE), 0, s, ca);
+        if (cwaves > na.n_items) cwaves = na.n_items;
+        hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        HIP_TRY(ctx, hipGetLastError());
     }
-    HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     return KORG_OK;
 }
@@ -1193,7 +1205,7 @@ void korg_destroy(korg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recA, &ctx->recD, &ctx->dmask, &ctx->tab, &ctx->counters, &ctx->blk,
+    DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
                       &ctx->cstart, &ctx->cls_lines, &ctx->cls_jc, &ctx->ranges, &ctx->h_alpha,
                       &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
