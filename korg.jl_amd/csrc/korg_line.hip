@@ -7,7 +7,8 @@
 //   k_setup       one wavefront per line, lanes = layers (Float64): Doppler and Lorentz widths,
 //                 amplitudes, the window (max over layers) and the per-line records the profile
 //                 kernels read
-//   k_count / k_scan / k_scatter   stable partition of the lines into window-width classes
+//   k_count / k_scan / k_scatter   stable partitions of the lines into window-width classes and
+//                 (Float32 form) near-zone-reach classes
 //   k_ranges      per (class, wavelength tile): index range of candidate lines
 //   Float64 form:
 //     k_profile_f64  line_profile/voigt_hjerting per evaluation (the reference's expressions)
@@ -250,7 +251,10 @@ __device__ inline double cntm_lerp(const double* knots, const double* vals, int6
     return (1.0 - t) * vals[i] + t * vals[i + 1];
 }
 
-__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
+#ifndef KORG_SETUP_ATTR
+#define KORG_SETUP_ATTR
+#endif
+__global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (i >= A.n_lines) return;  // whole wavefront
@@ -434,32 +438,70 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         mt.cls = cls;
         A.meta[i] = mt;
         A.jc_all[i] = (int32_t)jc;
-        if (clo <= chi) {
-            const int64_t d = max(jc - clo, chi - jc) + 1;
-            atomicMax(&A.counters[4], (unsigned long long)d);
-        }
-        if (lb <= ub) {
-            atomicAdd(&A.counters[1], (unsigned long long)((ub - lb + 1) * (int64_t)L));
-            atomicAdd(&A.counters[2], 1ull);
-        }
     }
 }
 
 constexpr int PART_BLOCK = 256;
 
-__global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, const LineMeta* __restrict__ meta,
-                                                      int* __restrict__ block_counts) {
-    __shared__ int cnt[N_CLASSES + 1];
-    if (threadIdx.x <= N_CLASSES) cnt[threadIdx.x] = 0;
+// Partition keys.  WINDOW: lines by window width (for the profile kernels' tiles); NEAR: lines
+// by near-zone reach, the largest distance in points from the centre index jc to a near-zone
+// point (for k_near's tiles).  Both fit the same class table: a line of class c can only touch
+// tiles within class_wmax(c) points of its jc.
+enum { KEY_WINDOW = 0, KEY_NEAR = 1 };
+
+__device__ inline int line_class(const LineMeta& m, int key) {
+    if (key == KEY_WINDOW) return m.cls;
+    if (m.clo > m.chi) return N_CLASSES;
+    const int64_t d = max(m.jc - m.clo, m.chi - m.jc) + 1;
+    int c = 0;
+    while (c < N_CLASSES - 1 && d > class_wmax(c)) ++c;
+    return c;
+}
+
+// Per block of lines: class counts for the partition, plus the call's totals (evaluations, lines
+// used, and the largest distance in points from a line's centre index to its near zone), reduced
+// in LDS and added with one atomic per block.
+__global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
+                                                      const LineMeta* __restrict__ meta,
+                                                      int* __restrict__ block_counts,
+                                                      int* __restrict__ near_counts,
+                                                      unsigned long long* __restrict__ counters) {
+    __shared__ int cnt[N_CLASSES + 1], ncnt[N_CLASSES + 1];
+    __shared__ unsigned long long evals, used, reach;
+    if (threadIdx.x <= N_CLASSES) cnt[threadIdx.x] = ncnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) evals = used = reach = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
-    const int c = i < n ? meta[i].cls : N_CLASSES;
+    int c = N_CLASSES, cn = N_CLASSES;
+    if (i < n) {
+        const LineMeta m = meta[i];
+        c = m.cls;
+        cn = line_class(m, KEY_NEAR);
+        if (m.lb <= m.ub) {
+            atomicAdd(&evals, (unsigned long long)(m.ub - m.lb + 1) * (unsigned long long)L);
+            atomicAdd(&used, 1ull);
+        }
+        if (m.clo <= m.chi) {
+            const int d = max(m.jc - m.clo, m.chi - m.jc) + 1;
+            atomicMax(&reach, (unsigned long long)d);
+        }
+    }
     for (int k = 0; k < N_CLASSES; ++k) {
         const uint64_t b = ballot(c == k);
         if ((threadIdx.x % WAVE) == 0 && b) atomicAdd(&cnt[k], __popcll(b));
+        const uint64_t bn = ballot(cn == k);
+        if ((threadIdx.x % WAVE) == 0 && bn) atomicAdd(&ncnt[k], __popcll(bn));
     }
     __syncthreads();
-    if (threadIdx.x < N_CLASSES) block_counts[(int64_t)blockIdx.x * N_CLASSES + threadIdx.x] = cnt[threadIdx.x];
+    if (threadIdx.x < N_CLASSES) {
+        block_counts[(int64_t)blockIdx.x * N_CLASSES + threadIdx.x] = cnt[threadIdx.x];
+        near_counts[(int64_t)blockIdx.x * N_CLASSES + threadIdx.x] = ncnt[threadIdx.x];
+    }
+    if (threadIdx.x == 0) {
+        if (evals) atomicAdd(&counters[1], evals);
+        if (used) atomicAdd(&counters[2], used);
+        if (reach) atomicMax(&counters[4], reach);
+    }
 }
 
 // One block: exclusive scan of block_counts per class; class_start[c] = first slot of class c,
@@ -506,11 +548,11 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
                                                         const int* __restrict__ block_offsets,
                                                         const int* __restrict__ class_start,
                                                         int* __restrict__ cls_lines,
-                                                        int* __restrict__ cls_jc) {
+                                                        int* __restrict__ cls_jc, int key) {
     __shared__ int wave_cnt[PART_BLOCK / WAVE][N_CLASSES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
-    const int c = i < n ? meta[i].cls : N_CLASSES;
+    const int c = i < n ? line_class(meta[i], key) : N_CLASSES;
     int rank = 0;
     for (int k = 0; k < N_CLASSES; ++k) {
         const uint64_t b = ballot(c == k);
@@ -648,7 +690,7 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             jj[p] = (int)(j0 + p * WAVE + lane);
-            lam[p] = jj[p] <= j1 ? A.wls[jj[p]] : 0.0;
+            lam[p] = A.wls[min((int64_t)jj[p], j1)];  // points past j1 are never used
         }
         double acc[P][G];
 #pragma unroll
@@ -705,8 +747,11 @@ __device__ inline int hunger_branch(double a, double v) {
 // ---------------------------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+#ifndef KORG_WING_ATTR
+#define KORG_WING_ATTR
+#endif
 template <int NB>
-__global__ __launch_bounds__(WAVE) void k_wing_mfma(ProfileArgs A) {
+__global__ __launch_bounds__(WAVE) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A) {
     const int lane = threadIdx.x;
     const int n = lane & 31;      // column (point) within a 32-point block
     const bool hi = lane >= 32;   // K half: coefficient/power index k = 2 s + hi
@@ -724,7 +769,7 @@ __global__ __launch_bounds__(WAVE) void k_wing_mfma(ProfileArgs A) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             jj[b] = (int)(j0 + 32 * b + n);
-            lam[b] = jj[b] <= j1 ? A.wls[jj[b]] : 0.0;
+            lam[b] = A.wls[min((int64_t)jj[b], j1)];  // points past j1 are never used
         }
         floatx16 acc[2][NB];
 #pragma unroll
@@ -749,11 +794,15 @@ __global__ __launch_bounds__(WAVE) void k_wing_mfma(ProfileArgs A) {
                 nxt_line = next_line;
                 have_nxt = true;
             }
+            // B operands of every block first, then the MFMAs with the two K steps of each
+            // accumulator 2*NB instructions apart
+            float b0[NB], b1[NB];
+            bool on[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int c0 = (int)j0 + 32 * b;
-                if (v.ub < c0 || v.lb > c0 + 31) continue;              // block outside window
-                if (v.clo <= c0 && v.chi >= c0 + 31) continue;          // block inside near zone
+                on[b] = !(v.ub < c0 || v.lb > c0 + 31) &&        // block meets the window
+                        !(v.clo <= c0 && v.chi >= c0 + 31);      // and is not all near zone
                 const int j = jj[b];
                 const bool use = j >= v.lb && j <= v.ub && !(j >= v.clo && j <= v.chi);
                 // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values
@@ -761,12 +810,20 @@ __global__ __launch_bounds__(WAVE) void k_wing_mfma(ProfileArgs A) {
                 float R = __builtin_amdgcn_rcpf(d * d);
                 R = use ? R : 0.0f;
                 const float R2 = R * R;
-                const float b0 = hi ? R2 : R;    // k = 0 | 1: R | R^2
-                const float b1 = b0 * R2;        // k = 2 | 3: R^3 | R^4
-                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, b0, acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, b0, acc[1][b], 0, 0, 0);
-                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1, acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1, acc[1][b], 0, 0, 0);
+                b0[b] = hi ? R2 : R;      // k = 0 | 1: R | R^2
+                b1[b] = b0[b] * R2;       // k = 2 | 3: R^3 | R^4
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (!on[b]) continue;
+                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, b0[b], acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, b0[b], acc[1][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (!on[b]) continue;
+                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1[b], acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1[b], acc[1][b], 0, 0, 0);
             }
         });
 
@@ -802,7 +859,8 @@ struct NearArgs {
     int64_t n_items;
     const double* wls;
     const LineMeta* meta;
-    const int* jc_all;
+    const int* near_lines;  // lines partitioned by near-zone reach class
+    const int2* ranges;     // per (class, CT-point tile): candidate index range in near_lines
     const double4* rec64;
     void* alpha;
     int alpha_f32;
@@ -828,7 +886,6 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
     if (A.counters[3] != 0) return;
-    const int cw = (int)A.counters[4];  // max distance from jc to a near-zone point, in points
     for (;;) {
         const int item = wave_take(&A.counters[5], lane);
         if (item >= A.n_items) break;
@@ -837,32 +894,51 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
         const bool lv = l < L;
         const int64_t j0 = (int64_t)tile * CT;
         const int64_t j1 = min(j0 + CT, A.n_wl) - 1;
+        double lam[CT];  // the tile's wavelengths (wavefront-uniform)
+        const KCONST double* wt = uniform_ptr(A.wls + j0);
+#pragma unroll
+        for (int t = 0; t < CT; ++t) lam[t] = wt[min((int64_t)t, j1 - j0)];  // in bounds
         float acc[CT];
 #pragma unroll
         for (int t = 0; t < CT; ++t) acc[t] = 0.0f;
-        // candidate lines: jc in [j0 - cw - 1, j1 + cw + 1] (jc is monotone in the line index)
-        const int64_t i0 = wave_lower_bound<int>(A.jc_all, A.n_lines, (int)(j0 - cw - 1), lane);
-        const int64_t i1 = wave_upper_bound<int>(A.jc_all, A.n_lines, (int)(j1 + cw + 1), lane);
-        for (int64_t i = i0; i < i1; ++i) {
-            const int line = __builtin_amdgcn_readfirstlane((int)i);
-            const LineMeta m = *uniform_ptr(A.meta + line);
-            if (m.clo > m.chi || m.chi < j0 || m.clo > j1) continue;
-            const double4 q = lv ? A.rec64[(int64_t)line * A.Ls + l] : make_double4(0, 0, 0, 0);
-            const float a32 = (float)q.y, sc = (float)q.z;
-#pragma unroll
-            for (int t = 0; t < CT; ++t) {
-                const int64_t j = j0 + t;
-                if (j < m.clo || j > m.chi) continue;  // wavefront-uniform
-                const double v64 = fabs(*uniform_ptr(A.wls + j) - m.lam0) * q.x;
-                const int br = hunger_branch(q.y, v64);
-                float h;
-                if (br == 1 || br == 2) {
-                    const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
-                    h = voigt_core_f32(a32, (float)v64, piece, br == 2);
-                } else {
-                    h = voigt_far_f32(a32, (float)v64, br == 3);
+        // candidates: class by class, line index within a class; 64 checked at once
+        for (int c = 0; c < N_CLASSES; ++c) {
+            const int2 r = A.ranges[(int64_t)c * A.ntiles + tile];
+            for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
+                const int k = k0 + lane;
+                int line = 0;
+                LineMeta m = {0.0, 1, 0, 1, 0, 0, 0};
+                bool ovl = false;
+                if (k < r.y) {
+                    line = A.near_lines[k];
+                    m = A.meta[line];
+                    ovl = m.clo <= m.chi && m.clo <= j1 && m.chi >= j0;
                 }
-                if (lv) acc[t] += h * sc;
+                uint64_t mask = ballot(ovl);
+                while (mask) {
+                    const int b = __ffsll((unsigned long long)mask) - 1;
+                    mask &= mask - 1;
+                    const int ln = readlane_i32(line, b);
+                    const double lam0 = readlane_f64(m.lam0, b);
+                    const int clo = readlane_i32(m.clo, b), chi = readlane_i32(m.chi, b);
+                    const double4 q = lv ? A.rec64[(int64_t)ln * A.Ls + l] : make_double4(0, 0, 0, 0);
+                    const float a32 = (float)q.y, sc = (float)q.z;
+#pragma unroll
+                    for (int t = 0; t < CT; ++t) {
+                        const int64_t j = j0 + t;
+                        if (j < clo || j > chi) continue;  // wavefront-uniform
+                        const double v64 = fabs(lam[t] - lam0) * q.x;
+                        const int br = hunger_branch(q.y, v64);
+                        float h;
+                        if (br == 1 || br == 2) {
+                            const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
+                            h = voigt_core_f32(a32, (float)v64, piece, br == 2);
+                        } else {
+                            h = voigt_far_f32(a32, (float)v64, br == 3);
+                        }
+                        acc[t] += h * sc;
+                    }
+                }
             }
         }
         if (lv) {
@@ -896,7 +972,8 @@ struct korg_ctx {
     korg_tuning tuning{};
     int64_t last_n_lines = 0;
     // device scratch (grow-only)
-    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_lines, cls_jc, ranges;
+    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_lines, cls_jc, ranges,
+        nblk, ncstart, near_lines, near_jc, near_ranges;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -946,7 +1023,10 @@ int grow(korg_ctx* ctx, DevBuf& b, size_t bytes) {
     } while (0)
 
 constexpr int F64_GROUP = 8;  // layers per work item of the Float64 form
-constexpr int MFMA_NB = 4;    // 32-point blocks per work item of the far-wing MFMA kernel
+#ifndef KORG_MFMA_NB
+#define KORG_MFMA_NB 2
+#endif
+constexpr int MFMA_NB = KORG_MFMA_NB;  // 32-point blocks per work item of the far-wing kernel
 
 int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg_species* sp,
              const korg_layers* ly, const korg_continuum* cn, int alpha_dtype, int precision) {
@@ -1005,6 +1085,16 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     TRY(grow(ctx, ctx->cls_lines, sizeof(int) * N));
     TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
     TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
+    const int nt_near = (int)((nwl + CT - 1) / CT);
+    if (f32) {
+        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
+        TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
+        TRY(grow(ctx, ctx->near_lines, sizeof(int) * N));
+        TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
+        TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
+    } else {
+        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
+    }
 
     auto* counters = (unsigned long long*)ctx->counters.p;
     auto* gi = (GridInfo*)ctx->gridinfo.p;
@@ -1052,16 +1142,28 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
 
-    hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
-                       (const LineMeta*)ctx->meta.p, (int*)ctx->blk.p);
+    hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L,
+                       (const LineMeta*)ctx->meta.p, (int*)ctx->blk.p, (int*)ctx->nblk.p, counters);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->blk.p,
                        (int*)ctx->cstart.p);
     hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
                        (const LineMeta*)ctx->meta.p, (const int*)ctx->blk.p,
-                       (const int*)ctx->cstart.p, (int*)ctx->cls_lines.p, (int*)ctx->cls_jc.p);
+                       (const int*)ctx->cstart.p, (int*)ctx->cls_lines.p, (int*)ctx->cls_jc.p,
+                       (int)KEY_WINDOW);
     hipLaunchKernelGGL(k_ranges, dim3((ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
                        ntiles, tile_w, nwl, (const int*)ctx->cstart.p, (const int*)ctx->cls_jc.p,
                        (int2*)ctx->ranges.p);
+    if (f32) {
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->nblk.p,
+                           (int*)ctx->ncstart.p);
+        hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
+                           (const LineMeta*)ctx->meta.p, (const int*)ctx->nblk.p,
+                           (const int*)ctx->ncstart.p, (int*)ctx->near_lines.p,
+                           (int*)ctx->near_jc.p, (int)KEY_NEAR);
+        hipLaunchKernelGGL(k_ranges, dim3((nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
+                           nt_near, CT, nwl, (const int*)ctx->ncstart.p,
+                           (const int*)ctx->near_jc.p, (int2*)ctx->near_ranges.p);
+    }
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
 
@@ -1100,11 +1202,12 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         na.L = L;
         na.Ls = Ls;
         na.n_lblocks = n_lblocks;
-        na.ntiles = (int)((nwl + CT - 1) / CT);
+        na.ntiles = nt_near;
         na.n_items = (int64_t)na.ntiles * n_lblocks;
         na.wls = g->all_wls;
         na.meta = (const LineMeta*)ctx->meta.p;
-        na.jc_all = (const int*)ctx->jc_all.p;
+        na.near_lines = (const int*)ctx->near_lines.p;
+        na.ranges = (const int2*)ctx->near_ranges.p;
         na.rec64 = (const double4*)ctx->rec64.p;
         na.alpha = alpha;
         na.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
@@ -1206,7 +1309,7 @@ void korg_destroy(korg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
-                      &ctx->cstart, &ctx->cls_lines, &ctx->cls_jc, &ctx->ranges, &ctx->h_alpha,
+                      &ctx->cstart, &ctx->cls_lines, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_lines, &ctx->near_jc, &ctx->near_ranges, &ctx->h_alpha,
                       &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
     for (DevBuf* b : bufs)
