@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 1
+#define KORG_LINE_ABI_VERSION 2
 
 enum korg_status {
     KORG_OK = 0,
@@ -123,6 +123,11 @@ typedef struct {
      * korg_tuning.count_branches is set: far wing a <= 0.2 (A), core a <= 0.2 (B),
      * core 0.2 < a <= 1.4 (C), far wing / strongly damped (D) */
     double n_evals_branch[4];
+    /* Float32 form only (0 for Float64): evaluations done exactly by the near-zone kernel
+     * (the rest of n_evals are far-wing series terms on the matrix cores), and the
+     * v_mfma_f32_32x32x2f32 instructions the far-wing kernel issued (4096 flops each) */
+    double n_evals_near;
+    double n_wing_mfma;
 } korg_stats;
 
 typedef struct korg_ctx korg_ctx;

@@ -467,9 +467,9 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
                                                       int* __restrict__ near_counts,
                                                       unsigned long long* __restrict__ counters) {
     __shared__ int cnt[N_CLASSES + 1], ncnt[N_CLASSES + 1];
-    __shared__ unsigned long long evals, used, reach;
+    __shared__ unsigned long long evals, used, reach, near_evals, wing_blocks;
     if (threadIdx.x <= N_CLASSES) cnt[threadIdx.x] = ncnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) evals = used = reach = 0;
+    if (threadIdx.x == 0) evals = used = reach = near_evals = wing_blocks = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
     int c = N_CLASSES, cn = N_CLASSES;
@@ -480,10 +480,16 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
         if (m.lb <= m.ub) {
             atomicAdd(&evals, (unsigned long long)(m.ub - m.lb + 1) * (unsigned long long)L);
             atomicAdd(&used, 1ull);
+            // 32-point blocks (aligned to 32 grid points) the far-wing kernel multiplies for
+            // this line: those meeting the window less those wholly inside the near zone
+            int nb = (m.ub >> 5) - (m.lb >> 5) + 1;
+            if (m.clo <= m.chi) nb -= max(0, ((m.chi + 1) >> 5) - ((m.clo + 31) >> 5));
+            atomicAdd(&wing_blocks, (unsigned long long)nb);
         }
         if (m.clo <= m.chi) {
             const int d = max(m.jc - m.clo, m.chi - m.jc) + 1;
             atomicMax(&reach, (unsigned long long)d);
+            atomicAdd(&near_evals, (unsigned long long)(m.chi - m.clo + 1) * (unsigned long long)L);
         }
     }
     for (int k = 0; k < N_CLASSES; ++k) {
@@ -501,6 +507,8 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
         if (evals) atomicAdd(&counters[1], evals);
         if (used) atomicAdd(&counters[2], used);
         if (reach) atomicMax(&counters[4], reach);
+        if (near_evals) atomicAdd(&counters[10], near_evals);
+        if (wing_blocks) atomicAdd(&counters[11], wing_blocks);
     }
 }
 
@@ -971,6 +979,8 @@ struct korg_ctx {
     korg_stats stats{};
     korg_tuning tuning{};
     int64_t last_n_lines = 0;
+    int last_precision = KORG_PRECISION_FP64;
+    int last_n_lblocks = 0;
     // device scratch (grow-only)
     DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_lines, cls_jc, ranges,
         nblk, ncstart, near_lines, near_jc, near_ranges;
@@ -1060,6 +1070,8 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     const int L = ly->n_layers;
     const int64_t nwl = g->n_wl;
     ctx->last_n_lines = N;
+    ctx->last_precision = precision;
+    ctx->last_n_lblocks = (L + WAVE - 1) / WAVE;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     hipStream_t s = ctx->stream;
@@ -1230,7 +1242,7 @@ int finish_stats(korg_ctx* ctx) {
     HIP_TRY(ctx, hipEventElapsedTime(&t24, ctx->ev[2], ctx->ev[4]));
     HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
     HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-    unsigned long long c[10];
+    unsigned long long c[12];
     static_assert(sizeof c <= sizeof(unsigned long long) * N_COUNTERS, "counters");
     HIP_TRY(ctx, hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
     if (c[3] & 2)
@@ -1247,6 +1259,9 @@ int finish_stats(korg_ctx* ctx) {
     ctx->stats.core_ms = t43;
     ctx->stats.total_ms = t03;
     for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
+    const bool f32 = ctx->last_precision == KORG_PRECISION_FP32_FIXUP;
+    ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
+    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * 4.0 * ctx->last_n_lblocks : 0.0;
     return KORG_OK;
 }
 

@@ -56,13 +56,16 @@ class KorgStats(ctypes.Structure):
     _fields_ = [("n_evals", ctypes.c_double), ("n_lines_used", ctypes.c_int64),
                 ("setup_ms", ctypes.c_double), ("bin_ms", ctypes.c_double),
                 ("profile_ms", ctypes.c_double), ("core_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4)]
+                ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4),
+                ("n_evals_near", ctypes.c_double), ("n_wing_mfma", ctypes.c_double)]
 
 
 class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
 
+
+ABI_VERSION = 2  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -112,6 +115,9 @@ def load():
     L.korg_get_windows.restype = i32
     L.korg_set_tuning.argtypes = [vp, ctypes.POINTER(KorgTuning)]
     L.korg_set_tuning.restype = i32
+    if L.korg_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
+                          f"expect {ABI_VERSION}; rebuild it")
     _lib = L
     return L
 

@@ -240,6 +240,28 @@ def test_solar_wide_full_size_properties(ctx):
     assert err <= FP32_RTOL, err
 
 
+def test_work_counts(ctx):
+    """Near-zone and far-wing counts of the Float32 form: the near zone is part of the windows,
+    and the far-wing MFMAs (one 32-point x 64-layer block = 4 instructions = 4096 * 4 flops =
+    8 flops per covered (layer, point)) cover every far-zone evaluation.  The Float64 form
+    reports neither."""
+    P, _, _ = workloads.solar_small()
+    gpu_alpha(ctx, P, "fp32")
+    s = ctx.stats()
+    assert 0 < s["n_evals_near"] < s["n_evals"]
+    assert s["n_evals_near"] % P.n_layers == 0
+    n_lblocks = -(-P.n_layers // 64)
+    assert s["n_wing_mfma"] % (4 * n_lblocks) == 0
+    assert 4096 * s["n_wing_mfma"] >= 8 * (s["n_evals"] - s["n_evals_near"])
+    # every line with a window issues at most ceil(W/32) + 1 blocks
+    lb, ub = ctx.windows(len(P.lines))
+    w = (ub - lb + 1)[ub >= lb]
+    assert s["n_wing_mfma"] <= 4 * n_lblocks * np.sum(w // 32 + 2)
+    gpu_alpha(ctx, P, "fp64")
+    s = ctx.stats()
+    assert s["n_evals_near"] == 0 and s["n_wing_mfma"] == 0
+
+
 def test_invalid_species_index_rejected(ctx):
     """A species index outside the species table is reported, and alpha is left alone (the
     device checks the index before any read through it)."""
