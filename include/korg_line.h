@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 2
+#define KORG_LINE_ABI_VERSION 3
 
 enum korg_status {
     KORG_OK = 0,
@@ -128,6 +128,9 @@ typedef struct {
      * v_mfma_f32_32x32x2f32 instructions the far-wing kernel issued (4096 flops each) */
     double n_evals_near;
     double n_wing_mfma;
+    /* Float32 form only: far-zone evaluations that include the R^3, R^4 series terms (beyond
+     * each line's tail radius those terms are below 1e-6 of the leading term and are skipped) */
+    double n_evals_tail;
 } korg_stats;
 
 typedef struct korg_ctx korg_ctx;

@@ -51,6 +51,10 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr double SERIES_EPS = 2e-8;  // relative truncation error allowed in the far-wing series
+// Relative size below which the R^3 and R^4 terms of the far-wing series are dropped (beyond the
+// line's tail radius, where they are smaller than this fraction of the leading term in every
+// layer).
+constexpr double SERIES_TAIL_EPS = 1e-6;
 
 // --------------------------------------------------------------------------------------------
 // wave helpers
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         // invalid species index: flag the call as failed (nothing is added to alpha)
         if (lane == 0) {
             atomicOr(&A.counters[3], 2ull);
-            LineMeta mt = {wl, 1, 0, 1, 0, 0, N_CLASSES};
+            LineMeta mt = {wl, 1, 0, 1, 0, 0, N_CLASSES, 1, 0};
             A.meta[i] = mt;
             A.jc_all[i] = 0;
         }
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         kidx = __builtin_amdgcn_readfirstlane((int)ub) - 1;
         kidx = kidx < 0 ? 0 : (kidx > A.n_knots - 2 ? A.n_knots - 2 : kidx);
     }
-    double dmax = 0.0, lmax = 0.0, rmax = -1.0;
+    double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0;
     bool any_nan = false;
     double my_hw = -1.0, my_a = 0.0;  // layer `lane` (first 64 layers), for branch counting
     for (int l = L + lane; l < A.Ls; l += WAVE) A.rec64[i * A.Ls + l] = make_double4(0, 0, 0, 0);
@@ -337,6 +341,21 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             }
             const double r = vr / inv;
             rmax = r > rmax ? r : rmax;
+            // tail radius of this layer, as v: beyond it |e3| w^2 + |e4| w^3 <= SERIES_TAIL_EPS
+            // (w = 1/v^2; e_k the series coefficients relative to the leading term)
+            double vt;
+            if (a <= 0.2) {
+                vt = sqrt(sqrt(3.75 / SERIES_TAIL_EPS));
+            } else {
+                const double a2 = a * a;
+                const double e3 = fabs(3.75 - 5.0 * a2 + a2 * a2);
+                const double e4 = fabs(-26.25 * a2 + 10.5 * a2 * a2 - a2 * a2 * a2);
+                const double v3 = sqrt(sqrt(2.0 * e3 / SERIES_TAIL_EPS));
+                const double v4 = cbrt(sqrt(2.0 * e4 / SERIES_TAIL_EPS));
+                vt = fmax(v3, v4);
+            }
+            const double rt = vt / inv;
+            tmax = rt > tmax ? rt : tmax;
             if (l == lane) {
                 my_hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
                 my_a = a;
@@ -376,6 +395,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double dwin = wave_max(dmax);
     const double lwin = wave_max(lmax);
     const double reach = wave_max(rmax);
+    const double treach = wave_max(tmax);
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
     const int64_t n = A.n_wl;
     const double last = A.wls[n - 1];
@@ -383,15 +403,18 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     double x = wl;
     bool upper = false;
     const double h = reach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
+    const double ht = treach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
     if (lane == 0) x = wl - window;
     if (lane == 1) { x = wl + window; upper = true; }
     if (lane == 3) x = wl - h;
     if (lane == 4) { x = wl + h; upper = true; }
+    if (lane == 5) x = wl - ht;
+    if (lane == 6) { x = wl + ht; upper = true; }
     if (lane <= 1 && x >= 1.0) x = x * 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
     const bool past = !(x <= last);           // beyond the grid (or NaN)
-    const int64_t r = wave_bounds(A.wls, n, gi.g0, gi.ginv, x, upper, lane < 5 && !past, lane);
+    const int64_t r = wave_bounds(A.wls, n, gi.g0, gi.ginv, x, upper, lane < 7 && !past, lane);
     int64_t res = r;
-    if (lane < 5 && past) res = n;  // lower bound n; upper bound n (last index n - 1)
+    if (lane < 7 && past) res = n;  // lower bound n; upper bound n (last index n - 1)
     const int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
     const int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
     const int64_t jc = __builtin_amdgcn_readlane((int)res, 2);
@@ -401,6 +424,13 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         chi = (int64_t)__builtin_amdgcn_readlane((int)res, 4) - 1;
         clo = clo < lb ? lb : clo;
         chi = chi > ub ? ub : chi;
+    }
+    int64_t flo = 1, fhi = 0;
+    if (lb <= ub && treach >= 0.0) {
+        flo = __builtin_amdgcn_readlane((int)res, 5);
+        fhi = (int64_t)__builtin_amdgcn_readlane((int)res, 6) - 1;
+        flo = flo < lb ? lb : flo;
+        fhi = fhi > ub ? ub : fhi;
     }
     if (A.count_branches && lb <= ub && lane < L) {
         // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
@@ -436,6 +466,8 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         mt.chi = (int32_t)chi;
         mt.jc = (int32_t)jc;
         mt.cls = cls;
+        mt.flo = (int32_t)flo;
+        mt.fhi = (int32_t)fhi;
         A.meta[i] = mt;
         A.jc_all[i] = (int32_t)jc;
     }
@@ -461,15 +493,15 @@ __device__ inline int line_class(const LineMeta& m, int key) {
 // Per block of lines: class counts for the partition, plus the call's totals (evaluations, lines
 // used, and the largest distance in points from a line's centre index to its near zone), reduced
 // in LDS and added with one atomic per block.
-__global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
+__global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L, int wing_tile_w,
                                                       const LineMeta* __restrict__ meta,
                                                       int* __restrict__ block_counts,
                                                       int* __restrict__ near_counts,
                                                       unsigned long long* __restrict__ counters) {
     __shared__ int cnt[N_CLASSES + 1], ncnt[N_CLASSES + 1];
-    __shared__ unsigned long long evals, used, reach, near_evals, wing_blocks;
+    __shared__ unsigned long long evals, used, reach, near_evals, wing_mfma, tail_evals;
     if (threadIdx.x <= N_CLASSES) cnt[threadIdx.x] = ncnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) evals = used = reach = near_evals = wing_blocks = 0;
+    if (threadIdx.x == 0) evals = used = reach = near_evals = wing_mfma = tail_evals = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
     int c = N_CLASSES, cn = N_CLASSES;
@@ -480,11 +512,17 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
         if (m.lb <= m.ub) {
             atomicAdd(&evals, (unsigned long long)(m.ub - m.lb + 1) * (unsigned long long)L);
             atomicAdd(&used, 1ull);
-            // 32-point blocks (aligned to 32 grid points) the far-wing kernel multiplies for
-            // this line: those meeting the window less those wholly inside the near zone
-            int nb = (m.ub >> 5) - (m.lb >> 5) + 1;
-            if (m.clo <= m.chi) nb -= max(0, ((m.chi + 1) >> 5) - ((m.clo + 31) >> 5));
-            atomicAdd(&wing_blocks, (unsigned long long)nb);
+            // MFMAs of the far-wing kernel for this line and one 64-layer block: 2 per 32-point
+            // block of every tile meeting the window, 2 more where the tile meets the tail zone
+            const int tiles = m.ub / wing_tile_w - m.lb / wing_tile_w + 1;
+            const int full = m.flo <= m.fhi ? m.fhi / wing_tile_w - m.flo / wing_tile_w + 1 : 0;
+            atomicAdd(&wing_mfma, (unsigned long long)(wing_tile_w / 16) * (tiles + full));
+            // far-zone evaluations that use the R^3, R^4 terms: tail zone less near zone
+            if (m.flo <= m.fhi) {
+                int64_t t = m.fhi - m.flo + 1;
+                if (m.clo <= m.chi) t -= max(0, min(m.chi, m.fhi) - max(m.clo, m.flo) + 1);
+                atomicAdd(&tail_evals, (unsigned long long)t * (unsigned long long)L);
+            }
         }
         if (m.clo <= m.chi) {
             const int d = max(m.jc - m.clo, m.chi - m.jc) + 1;
@@ -508,7 +546,8 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L,
         if (used) atomicAdd(&counters[2], used);
         if (reach) atomicMax(&counters[4], reach);
         if (near_evals) atomicAdd(&counters[10], near_evals);
-        if (wing_blocks) atomicAdd(&counters[11], wing_blocks);
+        if (wing_mfma) atomicAdd(&counters[11], wing_mfma);
+        if (tail_evals) atomicAdd(&counters[12], tail_evals);
     }
 }
 
@@ -555,7 +594,7 @@ __global__ __launch_bounds__(1024) void k_scan(int nblocks, int* __restrict__ bl
 __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMeta* __restrict__ meta,
                                                         const int* __restrict__ block_offsets,
                                                         const int* __restrict__ class_start,
-                                                        int* __restrict__ cls_lines,
+                                                        Cand* __restrict__ cands,
                                                         int* __restrict__ cls_jc, int key) {
     __shared__ int wave_cnt[PART_BLOCK / WAVE][N_CLASSES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -572,8 +611,19 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
         int before = 0;
         for (int ww = 0; ww < w; ++ww) before += wave_cnt[ww][c];
         const int pos = class_start[c] + block_offsets[(int64_t)blockIdx.x * N_CLASSES + c] + before + rank;
-        cls_lines[pos] = (int)i;
-        cls_jc[pos] = meta[i].jc;
+        const LineMeta m = meta[i];
+        Cand cd;
+        cd.lam0 = m.lam0;
+        cd.lb = m.lb;
+        cd.ub = m.ub;
+        cd.clo = m.clo;
+        cd.chi = m.chi;
+        cd.flo = m.flo;
+        cd.fhi = m.fhi;
+        cd.line = (int)i;
+        cd.pad = 0;
+        cands[pos] = cd;
+        cls_jc[pos] = m.jc;
     }
 }
 
@@ -614,10 +664,9 @@ struct ProfileArgs {
     int ntiles, tile_w;
     int64_t n_items;
     const double* wls;
-    const LineMeta* meta;
     const double4* rec64;
     const float4* recM;
-    const int* cls_lines;
+    const Cand* cands;   // lines in window-class partition order
     const int2* ranges;
     void* alpha;
     int alpha_f32;
@@ -628,41 +677,97 @@ struct ProfileArgs {
 struct LineView {
     int line;
     double lam0;
-    int lb, ub, clo, chi;
+    int lb, ub, clo, chi, flo, fhi;
 };
 
-// Visit, in a fixed order (class by class, line index within a class), every line whose window
-// overlaps grid points [j0, j1] of this tile; f(view, next_line) is called with wave-uniform
-// values (next_line: the following line of the same batch, or -1, for prefetching).
-template <class F>
-__device__ inline void for_each_line(const ProfileArgs& A, int tile, int64_t j0, int64_t j1,
-                                     int lane, F&& f) {
+// s_waitcnt operand that waits for every vector memory load (vmcnt = 0) and leaves the export
+// and LDS/scalar counters unconstrained.
+constexpr int VMCNT0 = 0x0F70;
+
+// Lowest set lane of `mask`, removed from it; -1 when empty.
+__device__ inline int pop_lane(uint64_t& mask) {
+    if (!mask) return -1;
+    const int b = __ffsll((unsigned long long)mask) - 1;
+    mask &= mask - 1;
+    return b;
+}
+
+// Visit, in a fixed order (class by class, partition order within a class), every line whose
+// zone overlaps grid points [j0, j1] of this tile: the window [lb, ub] (KEY_WINDOW) or the near
+// zone [clo, chi] (KEY_NEAR).  body(view, operand) is called with wavefront-uniform views.
+// The per-line operand load(line) -> T is software-pipelined two lines deep: the load for line
+// i + 2 is issued before body() runs for line i, and three operand registers rotate through a
+// three-way unrolled loop, so no operand is copied and each wait is for the oldest load only.
+// Every path through the loop issues the same loads (past the end of a batch, lane 63's line is
+// loaded as a stand-in), which keeps the wait counts exact.  The next batch's 64 candidate
+// records are loaded behind the first two operands of the current batch.  With nparts > 1 only
+// share `part` of every class's candidates is visited (the shares of a tile's waves).
+template <class T, class Load, class Body>
+__device__ inline void for_each_line_pf(const int2* __restrict__ ranges, const Cand* __restrict__ cands,
+                                        int ntiles, int tile, int64_t j0, int64_t j1, int lane,
+                                        int key, Load&& load, Body&& body, int part = 0,
+                                        int nparts = 1) {
     for (int c = 0; c < N_CLASSES; ++c) {
-        const int2 r = A.ranges[(int64_t)c * A.ntiles + tile];
+        int2 r = ranges[(int64_t)c * ntiles + tile];
+        if (nparts > 1) {
+            // part `part` of nparts contiguous, near-equal shares of the class's candidates
+            const int len = r.y - r.x;
+            r = make_int2(r.x + (int)(((int64_t)len * part) / nparts),
+                          r.x + (int)(((int64_t)len * (part + 1)) / nparts));
+        }
+        if (r.x >= r.y) continue;
+        Cand cc = cands[min(r.x + lane, r.y - 1)];
         for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
-            const int k = k0 + lane;
-            int line = 0;
-            LineMeta m = {0.0, 1, 0, 1, 0, 0, 0};
-            bool ovl = false;
-            if (k < r.y) {
-                line = A.cls_lines[k];
-                m = A.meta[line];
-                ovl = m.lb <= j1 && m.ub >= j0;
-            }
+            const bool valid = k0 + lane < r.y;
+            const bool ovl = valid && (key == KEY_WINDOW
+                                           ? (cc.lb <= j1 && cc.ub >= j0)
+                                           : (cc.clo <= cc.chi && cc.clo <= j1 && cc.chi >= j0));
             uint64_t mask = ballot(ovl);
-            while (mask) {
-                const int b = __ffsll((unsigned long long)mask) - 1;
-                mask &= mask - 1;
-                LineView v;
-                v.line = readlane_i32(line, b);
-                v.lam0 = readlane_f64(m.lam0, b);
-                v.lb = readlane_i32(m.lb, b);
-                v.ub = readlane_i32(m.ub, b);
-                v.clo = readlane_i32(m.clo, b);
-                v.chi = readlane_i32(m.chi, b);
-                const int nb = mask ? readlane_i32(line, __ffsll((unsigned long long)mask) - 1) : -1;
-                f(v, nb);
+            const int kn = min(k0 + WAVE + lane, r.y - 1);  // next batch (clamped: always valid)
+            Cand cn;
+            if (!mask) {
+                cn = cands[kn];
+                cc = cn;
+                continue;
             }
+            auto view = [&](int b) {
+                LineView v;
+                v.line = readlane_i32(cc.line, b);
+                v.lam0 = readlane_f64(cc.lam0, b);
+                v.lb = readlane_i32(cc.lb, b);
+                v.ub = readlane_i32(cc.ub, b);
+                v.clo = readlane_i32(cc.clo, b);
+                v.chi = readlane_i32(cc.chi, b);
+                v.flo = readlane_i32(cc.flo, b);
+                v.fhi = readlane_i32(cc.fhi, b);
+                return v;
+            };
+            auto line_of = [&](int b) { return readlane_i32(cc.line, b >= 0 ? b : WAVE - 1); };
+            int b0 = pop_lane(mask);
+            int b1 = pop_lane(mask);
+            T x0 = load(line_of(b0));
+            T x1 = load(line_of(b1));
+            cn = cands[kn];
+            for (;;) {
+                const int b2 = pop_lane(mask);
+                const T x2 = load(line_of(b2));
+                body(view(b0), x0);
+                if (b1 < 0) break;
+                const int b3 = pop_lane(mask);
+                x0 = load(line_of(b3));
+                body(view(b1), x1);
+                if (b2 < 0) break;
+                const int b4 = pop_lane(mask);
+                x1 = load(line_of(b4));
+                body(view(b2), x2);
+                if (b3 < 0) break;
+                b0 = b3;
+                b1 = b4;
+            }
+            // retire the stand-in loads left in flight by the exit, so that no load into the
+            // operand registers is pending when the next batch enters the loop
+            __builtin_amdgcn_s_waitcnt(VMCNT0);
+            cc = cn;
         }
     }
 }
@@ -706,7 +811,9 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
 #pragma unroll
             for (int l = 0; l < G; ++l) acc[p][l] = 0.0;
 
-        for_each_line(A, tile, j0, j1, lane, [&](const LineView& v, int) {
+        auto no_operand = [](int) { return 0; };
+        for_each_line_pf<int>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_WINDOW,
+                              no_operand, [&](const LineView& v, int) {
             double dl[P];
             bool inw[P];
 #pragma unroll
@@ -787,68 +894,63 @@ __global__ __launch_bounds__(WAVE) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[mb][b][r] = 0.0f;
 
-        const float4* recM = A.recM;
-        float4 nxt = make_float4(0.f, 0.f, 0.f, 0.f);
-        bool have_nxt = false;
-        int nxt_line = -1;
-        for_each_line(A, tile, j0, j1, lane, [&](const LineView& v, int next_line) {
-            // coefficient block of this line (prefetched while the previous line computed)
-            float4 a4;
-            if (have_nxt && nxt_line == v.line) a4 = nxt;
-            else a4 = recM[((int64_t)v.line * A.n_lblocks + lblk) * WAVE + lane];
-            have_nxt = false;
-            if (next_line >= 0) {
-                nxt = recM[((int64_t)next_line * A.n_lblocks + lblk) * WAVE + lane];
-                nxt_line = next_line;
-                have_nxt = true;
-            }
-            // B operands of every block first, then the MFMAs with the two K steps of each
-            // accumulator 2*NB instructions apart
+        const float4* __restrict__ recM = A.recM + (int64_t)lblk * WAVE + lane;
+        const int64_t rstride = (int64_t)A.n_lblocks * WAVE;
+        auto load_a = [&](int line) { return recM[line * rstride]; };
+        for_each_line_pf<float4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_WINDOW,
+                                 load_a, [&](const LineView& v, const float4& a4) {
+            // every path waits for the operand here, also when all blocks are skipped below
+            asm volatile("" ::"v"(a4.x), "v"(a4.y), "v"(a4.z), "v"(a4.w));
+            // the R^3, R^4 step is needed only where the tile meets the line's tail zone
+            const bool full = v.flo <= j1 && v.fhi >= j0;
             float b0[NB], b1[NB];
-            bool on[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const int c0 = (int)j0 + 32 * b;
-                on[b] = !(v.ub < c0 || v.lb > c0 + 31) &&        // block meets the window
-                        !(v.clo <= c0 && v.chi >= c0 + 31);      // and is not all near zone
                 const int j = jj[b];
                 const bool use = j >= v.lb && j <= v.ub && !(j >= v.clo && j <= v.chi);
                 // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values
                 const float d = (float)((lam[b] - v.lam0) * (1.0 / DELTA_UNIT));
                 float R = __builtin_amdgcn_rcpf(d * d);
-                R = use ? R : 0.0f;
+                R = use ? R : 0.0f;  // zero outside the window and inside the near zone
                 const float R2 = R * R;
                 b0[b] = hi ? R2 : R;      // k = 0 | 1: R | R^2
                 b1[b] = b0[b] * R2;       // k = 2 | 3: R^3 | R^4
             }
+            // first K step (R, R^2) for every block; a block outside the window has B = 0
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                if (!on[b]) continue;
                 acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, b0[b], acc[0][b], 0, 0, 0);
                 acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, b0[b], acc[1][b], 0, 0, 0);
             }
+            if (full) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (!on[b]) continue;
-                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1[b], acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1[b], acc[1][b], 0, 0, 0);
+                for (int b = 0; b < NB; ++b) {
+                    acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1[b], acc[0][b], 0, 0, 0);
+                    acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1[b], acc[1][b], 0, 0, 0);
+                }
             }
+            // keep the accumulators in AGPRs on every path (no copies at branch merges)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) asm volatile("" : "+a"(acc[0][b]), "+a"(acc[1][b]));
         });
 
         // accumulator layout of v_mfma_f32_32x32x2_f32: lane i, register r holds
         // row 8*(r/4) + 4*(i/32) + r%4, column i%32
+        auto store = [&](auto* alpha) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (jj[b] > j1) continue;
+            for (int b = 0; b < NB; ++b) {
+                if (jj[b] > j1) continue;
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb)
+                for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
-                    if (layer < L)
-                        add_alpha(A.alpha, A.alpha_f32, (int64_t)layer + (int64_t)L * jj[b], acc[mb][b][r]);
-                }
-        }
+                    for (int r = 0; r < 16; ++r) {
+                        const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
+                        if (layer < L) alpha[(int64_t)layer + (int64_t)L * jj[b]] += acc[mb][b][r];
+                    }
+            }
+        };
+        if (A.alpha_f32) store((float*)A.alpha);
+        else store((double*)A.alpha);
     }
 }
 
@@ -866,9 +968,8 @@ struct NearArgs {
     int n_lblocks, ntiles;
     int64_t n_items;
     const double* wls;
-    const LineMeta* meta;
-    const int* near_lines;  // lines partitioned by near-zone reach class
-    const int2* ranges;     // per (class, CT-point tile): candidate index range in near_lines
+    const Cand* cands;      // lines in near-zone reach-class partition order
+    const int2* ranges;     // per (class, CT-point tile): candidate index range in cands
     const double4* rec64;
     void* alpha;
     int alpha_f32;
@@ -910,45 +1011,29 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
 #pragma unroll
         for (int t = 0; t < CT; ++t) acc[t] = 0.0f;
         // candidates: class by class, line index within a class; 64 checked at once
-        for (int c = 0; c < N_CLASSES; ++c) {
-            const int2 r = A.ranges[(int64_t)c * A.ntiles + tile];
-            for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
-                const int k = k0 + lane;
-                int line = 0;
-                LineMeta m = {0.0, 1, 0, 1, 0, 0, 0};
-                bool ovl = false;
-                if (k < r.y) {
-                    line = A.near_lines[k];
-                    m = A.meta[line];
-                    ovl = m.clo <= m.chi && m.clo <= j1 && m.chi >= j0;
-                }
-                uint64_t mask = ballot(ovl);
-                while (mask) {
-                    const int b = __ffsll((unsigned long long)mask) - 1;
-                    mask &= mask - 1;
-                    const int ln = readlane_i32(line, b);
-                    const double lam0 = readlane_f64(m.lam0, b);
-                    const int clo = readlane_i32(m.clo, b), chi = readlane_i32(m.chi, b);
-                    const double4 q = lv ? A.rec64[(int64_t)ln * A.Ls + l] : make_double4(0, 0, 0, 0);
-                    const float a32 = (float)q.y, sc = (float)q.z;
+        const double4* __restrict__ rq = A.rec64 + (lv ? l : 0);
+        auto load_q = [&](int line) {
+            return lv ? rq[(int64_t)line * A.Ls] : make_double4(0.0, 0.0, 0.0, 0.0);
+        };
+        for_each_line_pf<double4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_NEAR,
+                                  load_q, [&](const LineView& v, const double4& q) {
+            const float a32 = (float)q.y, sc = (float)q.z;
 #pragma unroll
-                    for (int t = 0; t < CT; ++t) {
-                        const int64_t j = j0 + t;
-                        if (j < clo || j > chi) continue;  // wavefront-uniform
-                        const double v64 = fabs(lam[t] - lam0) * q.x;
-                        const int br = hunger_branch(q.y, v64);
-                        float h;
-                        if (br == 1 || br == 2) {
-                            const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
-                            h = voigt_core_f32(a32, (float)v64, piece, br == 2);
-                        } else {
-                            h = voigt_far_f32(a32, (float)v64, br == 3);
-                        }
-                        acc[t] += h * sc;
-                    }
+            for (int t = 0; t < CT; ++t) {
+                const int64_t j = j0 + t;
+                if (j < v.clo || j > v.chi) continue;  // wavefront-uniform
+                const double v64 = fabs(lam[t] - v.lam0) * q.x;
+                const int br = hunger_branch(q.y, v64);
+                float h;
+                if (br == 1 || br == 2) {
+                    const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
+                    h = voigt_core_f32(a32, (float)v64, piece, br == 2);
+                } else {
+                    h = voigt_far_f32(a32, (float)v64, br == 3);
                 }
+                acc[t] += h * sc;
             }
-        }
+        });
         if (lv) {
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
@@ -982,8 +1067,8 @@ struct korg_ctx {
     int last_precision = KORG_PRECISION_FP64;
     int last_n_lblocks = 0;
     // device scratch (grow-only)
-    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_lines, cls_jc, ranges,
-        nblk, ncstart, near_lines, near_jc, near_ranges;
+    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
+        nblk, ncstart, near_cand, near_jc, near_ranges;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -1094,14 +1179,14 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     TRY(grow(ctx, ctx->gridinfo, sizeof(GridInfo)));
     TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
     TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
-    TRY(grow(ctx, ctx->cls_lines, sizeof(int) * N));
+    TRY(grow(ctx, ctx->cls_cand, sizeof(Cand) * N));
     TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
     TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
     const int nt_near = (int)((nwl + CT - 1) / CT);
     if (f32) {
         TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
         TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
-        TRY(grow(ctx, ctx->near_lines, sizeof(int) * N));
+        TRY(grow(ctx, ctx->near_cand, sizeof(Cand) * N));
         TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
         TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
     } else {
@@ -1154,13 +1239,13 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
 
-    hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L,
+    hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L, 32 * MFMA_NB,
                        (const LineMeta*)ctx->meta.p, (int*)ctx->blk.p, (int*)ctx->nblk.p, counters);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->blk.p,
                        (int*)ctx->cstart.p);
     hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
                        (const LineMeta*)ctx->meta.p, (const int*)ctx->blk.p,
-                       (const int*)ctx->cstart.p, (int*)ctx->cls_lines.p, (int*)ctx->cls_jc.p,
+                       (const int*)ctx->cstart.p, (Cand*)ctx->cls_cand.p, (int*)ctx->cls_jc.p,
                        (int)KEY_WINDOW);
     hipLaunchKernelGGL(k_ranges, dim3((ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
                        ntiles, tile_w, nwl, (const int*)ctx->cstart.p, (const int*)ctx->cls_jc.p,
@@ -1170,7 +1255,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
                            (int*)ctx->ncstart.p);
         hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
                            (const LineMeta*)ctx->meta.p, (const int*)ctx->nblk.p,
-                           (const int*)ctx->ncstart.p, (int*)ctx->near_lines.p,
+                           (const int*)ctx->ncstart.p, (Cand*)ctx->near_cand.p,
                            (int*)ctx->near_jc.p, (int)KEY_NEAR);
         hipLaunchKernelGGL(k_ranges, dim3((nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
                            nt_near, CT, nwl, (const int*)ctx->ncstart.p,
@@ -1188,10 +1273,9 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     pa.tile_w = tile_w;
     pa.n_items = (int64_t)ntiles * pa.n_lblocks;
     pa.wls = g->all_wls;
-    pa.meta = (const LineMeta*)ctx->meta.p;
     pa.rec64 = (const double4*)ctx->rec64.p;
     pa.recM = (const float4*)ctx->recM.p;
-    pa.cls_lines = (const int*)ctx->cls_lines.p;
+    pa.cands = (const Cand*)ctx->cls_cand.p;
     pa.ranges = (const int2*)ctx->ranges.p;
     pa.alpha = alpha;
     pa.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
@@ -1217,8 +1301,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         na.ntiles = nt_near;
         na.n_items = (int64_t)na.ntiles * n_lblocks;
         na.wls = g->all_wls;
-        na.meta = (const LineMeta*)ctx->meta.p;
-        na.near_lines = (const int*)ctx->near_lines.p;
+        na.cands = (const Cand*)ctx->near_cand.p;
         na.ranges = (const int2*)ctx->near_ranges.p;
         na.rec64 = (const double4*)ctx->rec64.p;
         na.alpha = alpha;
@@ -1242,7 +1325,7 @@ int finish_stats(korg_ctx* ctx) {
     HIP_TRY(ctx, hipEventElapsedTime(&t24, ctx->ev[2], ctx->ev[4]));
     HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
     HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-    unsigned long long c[12];
+    unsigned long long c[13];
     static_assert(sizeof c <= sizeof(unsigned long long) * N_COUNTERS, "counters");
     HIP_TRY(ctx, hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
     if (c[3] & 2)
@@ -1261,7 +1344,8 @@ int finish_stats(korg_ctx* ctx) {
     for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
     const bool f32 = ctx->last_precision == KORG_PRECISION_FP32_FIXUP;
     ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
-    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * 4.0 * ctx->last_n_lblocks : 0.0;
+    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * ctx->last_n_lblocks : 0.0;
+    ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
     return KORG_OK;
 }
 
@@ -1324,7 +1408,7 @@ void korg_destroy(korg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
-                      &ctx->cstart, &ctx->cls_lines, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_lines, &ctx->near_jc, &ctx->near_ranges, &ctx->h_alpha,
+                      &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->h_alpha,
                       &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
     for (DevBuf* b : bufs)

@@ -30,8 +30,20 @@ struct LineMeta {
     int32_t chi;   //   (clo > chi: none)
     int32_t jc;    // first grid index with lambda >= lam0 (unclamped; monotone in lam0)
     int32_t cls;   // window-width class, or N_CLASSES when skipped
+    int32_t flo;   // grid-index range where the far-wing series needs its R^3, R^4 terms in
+    int32_t fhi;   //   some layer (flo > fhi: nowhere)
 };
-static_assert(sizeof(LineMeta) == 32, "LineMeta layout");
+static_assert(sizeof(LineMeta) == 40, "LineMeta layout");
+
+// A line as the profile kernels read it: the LineMeta fields they use plus the line's index,
+// stored in partition order so that a batch of 64 candidates is one coalesced load.
+struct Cand {
+    double lam0;
+    int32_t lb, ub, clo, chi, flo, fhi;
+    int32_t line;
+    int32_t pad;
+};
+static_assert(sizeof(Cand) == 40, "Cand layout");
 
 // Window-width classes used to bin lines onto wavelength tiles: class c holds lines whose window
 // has at most CLASS_WMAX[c] points.

@@ -57,7 +57,8 @@ class KorgStats(ctypes.Structure):
                 ("setup_ms", ctypes.c_double), ("bin_ms", ctypes.c_double),
                 ("profile_ms", ctypes.c_double), ("core_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4),
-                ("n_evals_near", ctypes.c_double), ("n_wing_mfma", ctypes.c_double)]
+                ("n_evals_near", ctypes.c_double), ("n_wing_mfma", ctypes.c_double),
+                ("n_evals_tail", ctypes.c_double)]
 
 
 class KorgTuning(ctypes.Structure):
@@ -65,7 +66,7 @@ class KorgTuning(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
 
 
-ABI_VERSION = 2  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 3  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",

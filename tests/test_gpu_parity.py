@@ -241,25 +241,29 @@ def test_solar_wide_full_size_properties(ctx):
 
 
 def test_work_counts(ctx):
-    """Near-zone and far-wing counts of the Float32 form: the near zone is part of the windows,
-    and the far-wing MFMAs (one 32-point x 64-layer block = 4 instructions = 4096 * 4 flops =
-    8 flops per covered (layer, point)) cover every far-zone evaluation.  The Float64 form
-    reports neither."""
+    """Near-zone, tail and far-wing counts of the Float32 form: the near zone is part of the
+    windows, tail evaluations are part of the far zone, and the far-wing MFMAs cover every
+    far-zone evaluation with the (R, R^2) step (2 instructions per 32-point x 64-layer block =
+    4 flops per covered (layer, point)) and every tail evaluation with the (R^3, R^4) step.
+    The Float64 form reports none of them."""
     P, _, _ = workloads.solar_small()
     gpu_alpha(ctx, P, "fp32")
     s = ctx.stats()
+    far = s["n_evals"] - s["n_evals_near"]
     assert 0 < s["n_evals_near"] < s["n_evals"]
     assert s["n_evals_near"] % P.n_layers == 0
+    assert 0 < s["n_evals_tail"] <= far and s["n_evals_tail"] % P.n_layers == 0
     n_lblocks = -(-P.n_layers // 64)
     assert s["n_wing_mfma"] % (4 * n_lblocks) == 0
-    assert 4096 * s["n_wing_mfma"] >= 8 * (s["n_evals"] - s["n_evals_near"])
-    # every line with a window issues at most ceil(W/32) + 1 blocks
+    assert 4096 * s["n_wing_mfma"] >= 4 * far + 4 * s["n_evals_tail"]
+    # a line with a W-point window meets at most W/64 + 2 tiles of 64 points, each costing at
+    # most 8 instructions per 64-layer block
     lb, ub = ctx.windows(len(P.lines))
     w = (ub - lb + 1)[ub >= lb]
-    assert s["n_wing_mfma"] <= 4 * n_lblocks * np.sum(w // 32 + 2)
+    assert s["n_wing_mfma"] <= 8 * n_lblocks * np.sum(w // 64 + 2)
     gpu_alpha(ctx, P, "fp64")
     s = ctx.stats()
-    assert s["n_evals_near"] == 0 and s["n_wing_mfma"] == 0
+    assert s["n_evals_near"] == 0 and s["n_wing_mfma"] == 0 and s["n_evals_tail"] == 0
 
 
 def test_invalid_species_index_rejected(ctx):

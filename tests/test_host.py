@@ -155,7 +155,7 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 2
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_struct_layouts_match_header():
@@ -165,7 +165,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.KorgLayers) == 8 * 8
     assert ctypes.sizeof(_lib.KorgGrid) == 16
     assert ctypes.sizeof(_lib.KorgContinuum) == 32
-    assert ctypes.sizeof(_lib.KorgStats) == 56 + 32 + 16
+    assert ctypes.sizeof(_lib.KorgStats) == 56 + 32 + 24
     assert ctypes.sizeof(_lib.KorgTuning) == 16
 
 
