@@ -27,6 +27,27 @@
  * alpha once.
  *
  * Transcendentals come from the C library; Julia's own exp/log/pow may differ in the last ulp.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   line_profile            the reference's own assertions (test/line_absorption.jl:4-17)
+ *   searchsorted_first/last the reference's known indices (test/wavelengths.jl:60-86)
+ *   voigt_hjerting          a = 0 Gaussian and large-a Lorentzian limits; distance from the exact
+ *                           Voigt function Re w(v + ia) bounded per Hunger branch
+ *   harris_series           H1 against its Dawson-function form; H2 exactly
+ *   inverse_*_density       pdf(inverse(rho)) == rho
+ *   doppler_width           most-probable-speed known value; microturbulence in quadrature
+ *   scaled_stark            the (T/1e4)^(1/6) law
+ *   scaled_vdW              simple: (T/1e4)^0.3; ABO: twice the Maxwell average of
+ *                           sigma0 (v/v0)^-alpha by quadrature, for several alpha and T
+ *   sigma_line              the classical cross-section pi e^2 / (m_e c)
+ *   cntm_interp             exact interpolant of a quadratic continuum (same windows as the
+ *                           values at line centres)
+ *   line_absorption         line integral == amplitude, Doppler and Lorentz FWHMs, both thread
+ *                           schedules agree, accumulation, empty list, molecules
+ * Parity unpinned: the last ulp of Julia's libm (no reference test pins values below 1e-7
+ * relative), and voigt_hjerting branch C beyond Hunger's own accuracy (its coefficients are
+ * checked only through the bounded distance from the exact Voigt function and the reference's
+ * normalisation test).
  */
 #include <math.h>
 #include <stdint.h>

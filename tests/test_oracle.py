@@ -148,10 +148,56 @@ def test_abo_limits():
     alpha = 0.227
     r = oracle.scaled_vdW((sig, alpha), m, 2 * T) / oracle.scaled_vdW((sig, alpha), m, T)
     assert r == pytest.approx(2 ** ((1 - alpha) / 2), rel=1e-13)
-    # the Gamma-function factor: ratio between two alphas at vbar == v0
-    T0 = 1e12 / (8 * kboltz_cgs / math.pi * (1 / (1.008 * amu_cgs) + 1 / m))
-    g1 = oracle.scaled_vdW((sig, 0.3), m, T0) / (2 * sig * 1e6)
-    assert g1 == pytest.approx((4 / math.pi) ** 0.15 * gamma_fn(1.85), rel=1e-12)
+
+
+
+@pytest.mark.parametrize("alpha", [0.227, 0.3, 0.45])
+@pytest.mark.parametrize("T", [3500.0, 6000.0, 12000.0])
+def test_abo_rate_is_maxwell_average(alpha, T):
+    """The ABO broadening rate per perturber is twice the Maxwell-averaged rate <sigma(v) v> of
+    the cross-section sigma(v) = sigma0 (v / v0)^-alpha (v0 = 1e6 cm/s) over the distribution of
+    relative speeds of the radiator and a hydrogen atom (reduced mass mu): the average is done
+    here by numerical quadrature of the physical definition, not by the closed form."""
+    from scipy.integrate import quad
+    m = 55.85 * amu_cgs
+    sig = 1.3917417470792187e-14
+    mu = 1.0 / (1.0 / (1.008 * amu_cgs) + 1.0 / m)
+    s = math.sqrt(kboltz_cgs * T / mu)  # speed scale of the Maxwell distribution
+
+    def integrand(x):  # x = v / s; f(v) dv = sqrt(2/pi) x^2 exp(-x^2/2) dx
+        v = x * s
+        return sig * (v / 1e6) ** (-alpha) * v * math.sqrt(2 / math.pi) * x * x * math.exp(-x * x / 2)
+
+    rate, _ = quad(integrand, 0, 40, epsabs=0, epsrel=1e-13, limit=200)
+    assert oracle.scaled_vdW((sig, alpha), m, T) == pytest.approx(2 * rate, rel=1e-10)
+
+
+def test_continuum_interpolation_on_knots():
+    """The gridded continuum is interpolated linearly between the knots that bracket a line
+    centre: for a quadratic f(x) = x^2 that interpolant is exactly f(x) + (x - x_i)(x_{i+1} - x)
+    on [x_i, x_{i+1}].  Passing those values at the line centres (KORG_CNTM_AT_LINES) must give
+    the same windows as passing the knots (KORG_CNTM_GRID); the continuum only sets the cutoff."""
+    from korg_line import workloads
+    P, _, _ = workloads.solar_small(n_lines=400)
+    knots = np.sort(np.concatenate([np.linspace(P.all_wls[0] - 3e-8, P.all_wls[-1] + 3e-8, 23),
+                                    [5030.55e-8, 5030.9e-8]]))
+    x = (knots - knots[0]) / (knots[-1] - knots[0])  # 0..1
+    scale = np.geomspace(1e-9, 1e-6, P.n_layers)     # a different level per layer
+    grid_vals = scale[:, None] * (1e-3 + x[None, :] ** 2)
+    wl = P.lines.wl * (1e-8 if P.lines.wl[0] >= 1 else 1.0)
+    i = np.clip(np.searchsorted(knots, wl, side="right") - 1, 0, len(knots) - 2)
+    xl = (wl - knots[0]) / (knots[-1] - knots[0])
+    exact = 1e-3 + xl ** 2 + (xl - x[i]) * (x[i + 1] - xl)
+    at_lines = exact[:, None] * scale[None, :]       # (lines, layers)
+    mk = lambda mode, vals, kn: Problem(P.all_wls, P.lines, P.species_mass, P.species_is_mol,
+                                       P.n_div_U, P.temps, P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi,
+                                       mode, vals, kn, P.cutoff_threshold)
+    a = np.zeros((P.n_layers, P.n_wl), order="F")
+    lb1, ub1, _ = oracle.line_absorption(a, mk(1, grid_vals, knots), return_windows=True)
+    lb2, ub2, _ = oracle.line_absorption(a, mk(0, at_lines, None), return_windows=True)
+    assert np.array_equal(lb1, lb2) and np.array_equal(ub1, ub2)
+    assert np.any(lb1 != oracle.line_absorption(a, mk(0, at_lines * 10.0, None),
+                                                 return_windows=True)[0])
 
 
 def test_sigma_line_classical_cross_section():
