@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-fraction", type=float, default=0.05,
+    ap.add_argument("--cpu-fraction", type=float, default=0.5,
                     help="fraction of the lines timed on the CPU oracle (every k-th line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flux", action="store_true")

@@ -595,7 +595,9 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
                                                         const int* __restrict__ block_offsets,
                                                         const int* __restrict__ class_start,
                                                         Cand* __restrict__ cands,
-                                                        int* __restrict__ cls_jc, int key) {
+                                                        int* __restrict__ cls_jc,
+                                                        int2* __restrict__ keyrng,
+                                                        int* __restrict__ part_line, int key) {
     __shared__ int wave_cnt[PART_BLOCK / WAVE][N_CLASSES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
@@ -624,6 +626,9 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
         cd.pad = 0;
         cands[pos] = cd;
         cls_jc[pos] = m.jc;
+        // the key's grid-index range and the line index, compact, for the tile-list passes
+        keyrng[pos] = key == KEY_WINDOW ? make_int2(m.lb, m.ub) : make_int2(m.clo, m.chi);
+        part_line[pos] = (int)i;
     }
 }
 
@@ -653,6 +658,66 @@ __global__ void k_ranges(int ntiles, int tile_w, int64_t n_wl, const int* __rest
     ranges[(int64_t)c * ntiles + tile] = make_int2(b, lo);
 }
 
+// Exact per-tile line lists.  One wavefront per (tile, class) scans the class's candidates for
+// the tile (the range from k_ranges) and keeps those whose key range meets the tile.  The count
+// pass writes the number per (tile, class); after an exclusive scan over (tile, class) in
+// tile-major order, the fill pass writes the line indices: for every tile, class by class and in
+// partition order within a class -- the same fixed order as the candidate scans.
+__global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int64_t n_wl,
+                                                     const int2* __restrict__ ranges,
+                                                     const int2* __restrict__ keyrng,
+                                                     const int* __restrict__ part_line,
+                                                     const int64_t* __restrict__ tc_start,
+                                                     int* __restrict__ tc_count,
+                                                     int* __restrict__ lists) {
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x;  // c * ntiles + tile
+    const int c = t / ntiles, tile = t % ntiles;
+    const int64_t j0 = (int64_t)tile * tile_w;
+    const int64_t j1 = min(j0 + tile_w, n_wl) - 1;
+    const int2 r = ranges[t];
+    const int64_t slot = (int64_t)tile * N_CLASSES + c;
+    const int64_t base = tc_start ? tc_start[slot] : 0;
+    int64_t cnt = 0;
+    for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
+        const int k = k0 + lane;
+        bool ovl = false;
+        if (k < r.y) {
+            const int2 kr = keyrng[k];
+            ovl = kr.x <= kr.y && kr.x <= j1 && kr.y >= j0;
+        }
+        const uint64_t b = ballot(ovl);
+        if (lists && ovl) lists[base + cnt + __popcll(b & ((1ull << lane) - 1))] = part_line[k];
+        cnt += __popcll(b);
+    }
+    if (!tc_start && lane == 0) tc_count[slot] = (int)cnt;
+}
+
+// One block: exclusive scan of n counts into start[0..n] (start[n] = total).
+__global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __restrict__ count,
+                                                    int64_t* __restrict__ start) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t a = t * chunk, b = min(a + chunk, n);
+    int64_t sum = 0;
+    for (int64_t i = a; i < b; ++i) sum += count[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
+    for (int64_t i = a; i < b; ++i) {
+        start[i] = run;
+        run += count[i];
+    }
+    if (t == 1023) start[n] = part[1023];
+}
+
 // --------------------------------------------------------------------------------------------
 // profile kernels
 // --------------------------------------------------------------------------------------------
@@ -668,6 +733,9 @@ struct ProfileArgs {
     const float4* recM;
     const Cand* cands;   // lines in window-class partition order
     const int2* ranges;
+    const LineMeta* meta;
+    const int64_t* tile_start;  // [tile * N_CLASSES]: first entry of the tile's list (far wing)
+    const int* tile_lines;      // per tile, the lines whose window meets it (far wing)
     void* alpha;
     int alpha_f32;
     unsigned long long* counters;
@@ -855,26 +923,36 @@ __device__ inline int hunger_branch(double a, double v) {
 
 // ---------------------------------------------------------------------------------------------
 // Float32 far wing on the matrix cores.  A work item is (tile of 32*NB points, block of 64
-// layers); the 64 x 32NB block of alpha is held as 2 x NB 32x32 MFMA accumulators.  For each line
-// that overlaps the tile: A = the line's 64 x 4 coefficient block, B = (R, R^2, R^3, R^4) of the
-// tile's points (zero outside the window and inside the near zone), acc += A B with
-// v_mfma_f32_32x32x2_f32 (K = 2 per instruction, two steps).
+// layers), owned by one workgroup of WING_WAVES waves; each wave takes a contiguous part of the
+// tile's line list and holds the 64 x 32NB block of alpha as 2 x NB 32x32 MFMA accumulators.  For
+// each line: A = the line's 64 x 4 coefficient block, B = (R, R^2, R^3, R^4) of the tile's points
+// (zero outside the window and inside the near zone), acc += A B with v_mfma_f32_32x32x2_f32
+// (K = 2 per instruction, two steps).  The waves' blocks are summed through LDS in wave order and
+// added to alpha by wave 0.
 // ---------------------------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int WING_WAVES = 4;  // waves per far-wing workgroup; each takes a part of a tile's lines
 
+// three waves per SIMD (at most 168 registers): measured faster than two waves with no spills
 #ifndef KORG_WING_ATTR
-#define KORG_WING_ATTR
+#define KORG_WING_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 #endif
-template <int NB>
-__global__ __launch_bounds__(WAVE) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A) {
-    const int lane = threadIdx.x;
+template <int NB, class AlphaT>
+__global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A) {
+    // partial blocks of waves 1..WING_WAVES-1: [wave][accumulator][register quad][lane]
+    __shared__ float4 s_acc[WING_WAVES - 1][2 * NB][4][WAVE];
+    __shared__ int s_item;
+    const int lane = threadIdx.x % WAVE;
+    const int w = threadIdx.x / WAVE;
     const int n = lane & 31;      // column (point) within a 32-point block
     const bool hi = lane >= 32;   // K half: coefficient/power index k = 2 s + hi
     const int L = A.L;
-    if (A.counters[3] != 0) return;
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written (the same for all waves)
     for (;;) {
-        const int item = wave_take(&A.counters[0], lane);
-        if (item >= A.n_items) break;
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&A.counters[0], 1ull);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= A.n_items) break;  // the same for the whole workgroup
         const int tile = item / A.n_lblocks;
         const int lblk = item % A.n_lblocks;
         const int64_t j0 = (int64_t)tile * A.tile_w;
@@ -894,63 +972,163 @@ __global__ __launch_bounds__(WAVE) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[mb][b][r] = 0.0f;
 
+        // keep the accumulators in AGPRs on every path (no copies at branch merges)
+        auto pin_acc = [&] {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) asm volatile("" : "+a"(acc[0][b]), "+a"(acc[1][b]));
+        };
         const float4* __restrict__ recM = A.recM + (int64_t)lblk * WAVE + lane;
         const int64_t rstride = (int64_t)A.n_lblocks * WAVE;
         auto load_a = [&](int line) { return recM[line * rstride]; };
-        for_each_line_pf<float4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_WINDOW,
-                                 load_a, [&](const LineView& v, const float4& a4) {
-            // every path waits for the operand here, also when all blocks are skipped below
-            asm volatile("" ::"v"(a4.x), "v"(a4.y), "v"(a4.z), "v"(a4.w));
-            // the R^3, R^4 step is needed only where the tile meets the line's tail zone
-            const bool full = v.flo <= j1 && v.fhi >= j0;
-            float b0[NB], b1[NB];
+        // B operands of line `bl` of the batch (R, R^2 | R^3, R^4 per point)
+        struct Prep {
+            float b0[NB];  // k = 0 | 1: R | R^2 (zero outside the window and in the near zone)
+            float r2[NB];  // R^2
+            bool full;     // the tile meets the line's tail zone: the R^3, R^4 step is needed
+        };
+        auto prep = [&](const LineMeta& m, int bl) {
+            Prep p;
+            const double lam0 = readlane_f64(m.lam0, bl);
+            const int lb = readlane_i32(m.lb, bl), ub = readlane_i32(m.ub, bl);
+            const int clo = readlane_i32(m.clo, bl), chi = readlane_i32(m.chi, bl);
+            const int flo = readlane_i32(m.flo, bl), fhi = readlane_i32(m.fhi, bl);
+            p.full = flo <= j1 && fhi >= j0;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int j = jj[b];
-                const bool use = j >= v.lb && j <= v.ub && !(j >= v.clo && j <= v.chi);
+                const bool use = j >= lb && j <= ub && !(j >= clo && j <= chi);
                 // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values
-                const float d = (float)((lam[b] - v.lam0) * (1.0 / DELTA_UNIT));
+                const float d = (float)((lam[b] - lam0) * (1.0 / DELTA_UNIT));
                 float R = __builtin_amdgcn_rcpf(d * d);
-                R = use ? R : 0.0f;  // zero outside the window and inside the near zone
-                const float R2 = R * R;
-                b0[b] = hi ? R2 : R;      // k = 0 | 1: R | R^2
-                b1[b] = b0[b] * R2;       // k = 2 | 3: R^3 | R^4
+                R = use ? R : 0.0f;
+                p.r2[b] = R * R;
+                p.b0[b] = hi ? p.r2[b] : R;
             }
+            return p;
+        };
+        // acc += A B for one line
+        auto mfma = [&](const float4& a4, const Prep& p) {
+            // every path waits for the operand here, also when the second step is skipped
+            asm volatile("" ::"v"(a4.x), "v"(a4.y), "v"(a4.z), "v"(a4.w));
             // first K step (R, R^2) for every block; a block outside the window has B = 0
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, b0[b], acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, b0[b], acc[1][b], 0, 0, 0);
+                acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, p.b0[b], acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, p.b0[b], acc[1][b], 0, 0, 0);
             }
-            if (full) {
+            if (p.full) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
-                    acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1[b], acc[0][b], 0, 0, 0);
-                    acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1[b], acc[1][b], 0, 0, 0);
+                    const float b1 = p.b0[b] * p.r2[b];  // k = 2 | 3: R^3 | R^4
+                    acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1, acc[0][b], 0, 0, 0);
+                    acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1, acc[1][b], 0, 0, 0);
                 }
             }
-            // keep the accumulators in AGPRs on every path (no copies at branch merges)
-#pragma unroll
-            for (int b = 0; b < NB; ++b) asm volatile("" : "+a"(acc[0][b]), "+a"(acc[1][b]));
-        });
+            pin_acc();
+        };
+        // the tile's lines, 64 per batch: line indices are loaded two batches ahead, their
+        // LineMeta one batch ahead, and the operands one pair of lines ahead through four
+        // registers (a batch holds a multiple of four lines, so the rotation is in phase at every
+        // batch).  Positions past the end of the list prepare a stand-in line and issue nothing.
+        // wave w takes the w-th of WING_WAVES contiguous, near-equal parts of the tile's list
+        const int64_t t0 = A.tile_start[(int64_t)tile * N_CLASSES];
+        const int64_t t1 = A.tile_start[(int64_t)(tile + 1) * N_CLASSES];
+        const int64_t s0 = t0 + ((t1 - t0) * w) / WING_WAVES;
+        const int64_t s1 = t0 + ((t1 - t0) * (w + 1)) / WING_WAVES;
+        if (s0 < s1) {
+            const int* __restrict__ tl = A.tile_lines;
+            auto ld_idx = [&](int64_t k) { return tl[min(k + lane, s1 - 1)]; };  // clamped
+            int idxA = ld_idx(s0);
+            int idxB = ld_idx(s0 + WAVE);
+            LineMeta mA = A.meta[idxA];
+            int idxC = ld_idx(s0 + 2 * WAVE);
+            LineMeta mB = A.meta[idxB];
+            // line of position b of the current batch (b < 128; past the end: a stand-in)
+            auto line_at = [&](int b) {
+                return b < WAVE ? readlane_i32(idxA, b) : readlane_i32(idxB, b - WAVE);
+            };
+            float4 x0 = load_a(line_at(0));
+            float4 x1 = load_a(line_at(1));
+            float4 x2, x3;
+            for (int64_t kb = s0; kb < s1; kb += WAVE) {
+                const int nb = (int)min((int64_t)WAVE, s1 - kb);
+                // the batch's indices and LineMeta are waited for here, once per batch, not
+                // inside the line loop (where a wait would also drain the operand prefetches)
+                asm volatile("" ::"v"(idxA), "v"(idxB), "v"(mA.lam0), "v"(mA.lb), "v"(mA.ub),
+                             "v"(mA.clo), "v"(mA.chi), "v"(mA.flo), "v"(mA.fhi));
+                // lines in pairs: the two lines' operand chains are independent and interleave
+                for (int b = 0; b < nb; b += 4) {
+                    x2 = load_a(line_at(b + 2));
+                    x3 = load_a(line_at(b + 3));
+                    {
+                        const Prep pa = prep(mA, b), pb = prep(mA, b + 1);
+                        mfma(x0, pa);
+                        if (b + 1 >= nb) break;
+                        mfma(x1, pb);
+                    }
+                    if (b + 2 >= nb) break;
+                    x0 = load_a(line_at(b + 4));
+                    x1 = load_a(line_at(b + 5));
+                    {
+                        const Prep pc = prep(mA, b + 2), pd = prep(mA, b + 3);
+                        mfma(x2, pc);
+                        if (b + 3 >= nb) break;
+                        mfma(x3, pd);
+                    }
+                }
+                mA = mB;
+                idxA = idxB;
+                idxB = idxC;
+                mB = A.meta[idxB];
+                idxC = ld_idx(kb + 3 * WAVE);
+            }
+        }
 
-        // accumulator layout of v_mfma_f32_32x32x2_f32: lane i, register r holds
-        // row 8*(r/4) + 4*(i/32) + r%4, column i%32
-        auto store = [&](auto* alpha) {
+        // sum the waves' blocks in wave order (a fixed order: results are deterministic)
+        if (w > 0) {
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        s_acc[w - 1][mb * NB + b][q][lane] =
+                            make_float4(acc[mb][b][4 * q], acc[mb][b][4 * q + 1],
+                                        acc[mb][b][4 * q + 2], acc[mb][b][4 * q + 3]);
+        }
+        __syncthreads();
+        if (w == 0) {
+            // accumulator layout of v_mfma_f32_32x32x2_f32: lane i, register r holds
+            // row 8*(r/4) + 4*(i/32) + r%4, column i%32
+            AlphaT* __restrict__ alpha = (AlphaT*)A.alpha;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 if (jj[b] > j1) continue;
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
-                        if (layer < L) alpha[(int64_t)layer + (int64_t)L * jj[b]] += acc[mb][b][r];
+                    for (int q = 0; q < 4; ++q) {
+                        float4 v = make_float4(acc[mb][b][4 * q], acc[mb][b][4 * q + 1],
+                                               acc[mb][b][4 * q + 2], acc[mb][b][4 * q + 3]);
+#pragma unroll
+                        for (int ww = 0; ww < WING_WAVES - 1; ++ww) {
+                            const float4 p = s_acc[ww][mb * NB + b][q][lane];
+                            v.x += p.x;
+                            v.y += p.y;
+                            v.z += p.z;
+                            v.w += p.w;
+                        }
+                        const float vr[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int r = 4 * q + e;
+                            const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
+                            if (layer < L) alpha[(int64_t)layer + (int64_t)L * jj[b]] += (AlphaT)vr[e];
+                        }
                     }
             }
-        };
-        if (A.alpha_f32) store((float*)A.alpha);
-        else store((double*)A.alpha);
+        }
+        __syncthreads();  // s_acc and s_item are reused by the next item
     }
 }
 
@@ -1068,7 +1246,8 @@ struct korg_ctx {
     int last_n_lblocks = 0;
     // device scratch (grow-only)
     DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
-        nblk, ncstart, near_cand, near_jc, near_ranges;
+        nblk, ncstart, near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line,
+        tc_count, tc_start, tile_lines;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -1180,6 +1359,12 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
     TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
     TRY(grow(ctx, ctx->cls_cand, sizeof(Cand) * N));
+    TRY(grow(ctx, ctx->cls_rng, sizeof(int2) * N));
+    TRY(grow(ctx, ctx->cls_line, sizeof(int) * N));
+    if (f32) {
+        TRY(grow(ctx, ctx->tc_count, sizeof(int) * (size_t)ntiles * N_CLASSES));
+        TRY(grow(ctx, ctx->tc_start, sizeof(int64_t) * ((size_t)ntiles * N_CLASSES + 1)));
+    }
     TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
     TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
     const int nt_near = (int)((nwl + CT - 1) / CT);
@@ -1187,6 +1372,8 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
         TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
         TRY(grow(ctx, ctx->near_cand, sizeof(Cand) * N));
+        TRY(grow(ctx, ctx->near_rng, sizeof(int2) * N));
+        TRY(grow(ctx, ctx->near_line, sizeof(int) * N));
         TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
         TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
     } else {
@@ -1246,17 +1433,41 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
                        (const LineMeta*)ctx->meta.p, (const int*)ctx->blk.p,
                        (const int*)ctx->cstart.p, (Cand*)ctx->cls_cand.p, (int*)ctx->cls_jc.p,
-                       (int)KEY_WINDOW);
+                       (int2*)ctx->cls_rng.p, (int*)ctx->cls_line.p, (int)KEY_WINDOW);
     hipLaunchKernelGGL(k_ranges, dim3((ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
                        ntiles, tile_w, nwl, (const int*)ctx->cstart.p, (const int*)ctx->cls_jc.p,
                        (int2*)ctx->ranges.p);
+    if (f32) {
+        // exact per-tile line lists for the far-wing kernel: count, scan, size, fill
+        const int64_t ntc = (int64_t)ntiles * N_CLASSES;
+        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, ntiles, tile_w, nwl,
+                           (const int2*)ctx->ranges.p, (const int2*)ctx->cls_rng.p,
+                           (const int*)ctx->cls_line.p, (const int64_t*)nullptr,
+                           (int*)ctx->tc_count.p, (int*)nullptr);
+        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, ntc, (const int*)ctx->tc_count.p,
+                           (int64_t*)ctx->tc_start.p);
+        HIP_TRY(ctx, hipGetLastError());
+        int64_t total = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&total, (int64_t*)ctx->tc_start.p + ntc, sizeof total,
+                                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (total > (int64_t)INT32_MAX)
+            return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                           "more than 2^31 (line, tile) pairs in one call; split the line list");
+        TRY(grow(ctx, ctx->tile_lines, sizeof(int) * (size_t)(total > 0 ? total : 1)));
+        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, ntiles, tile_w, nwl,
+                           (const int2*)ctx->ranges.p, (const int2*)ctx->cls_rng.p,
+                           (const int*)ctx->cls_line.p, (const int64_t*)ctx->tc_start.p,
+                           (int*)nullptr, (int*)ctx->tile_lines.p);
+    }
     if (f32) {
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->nblk.p,
                            (int*)ctx->ncstart.p);
         hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
                            (const LineMeta*)ctx->meta.p, (const int*)ctx->nblk.p,
                            (const int*)ctx->ncstart.p, (Cand*)ctx->near_cand.p,
-                           (int*)ctx->near_jc.p, (int)KEY_NEAR);
+                           (int*)ctx->near_jc.p, (int2*)ctx->near_rng.p, (int*)ctx->near_line.p,
+                           (int)KEY_NEAR);
         hipLaunchKernelGGL(k_ranges, dim3((nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
                            nt_near, CT, nwl, (const int*)ctx->ncstart.p,
                            (const int*)ctx->near_jc.p, (int2*)ctx->near_ranges.p);
@@ -1276,6 +1487,9 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     pa.rec64 = (const double4*)ctx->rec64.p;
     pa.recM = (const float4*)ctx->recM.p;
     pa.cands = (const Cand*)ctx->cls_cand.p;
+    pa.meta = (const LineMeta*)ctx->meta.p;
+    pa.tile_start = (const int64_t*)ctx->tc_start.p;
+    pa.tile_lines = (const int*)ctx->tile_lines.p;
     pa.ranges = (const int2*)ctx->ranges.p;
     pa.alpha = alpha;
     pa.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
@@ -1289,7 +1503,11 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     } else {
-        hipLaunchKernelGGL((k_wing_mfma<MFMA_NB>), grid, dim3(WAVE), 0, s, pa);
+        int64_t nwg = (int64_t)ctx->n_cu * ((wpc + WING_WAVES - 1) / WING_WAVES);
+        if (nwg > pa.n_items) nwg = pa.n_items;
+        const dim3 wgrid((unsigned)nwg), wblock(WAVE * WING_WAVES);
+        if (pa.alpha_f32) hipLaunchKernelGGL((k_wing_mfma<MFMA_NB, float>), wgrid, wblock, 0, s, pa);
+        else hipLaunchKernelGGL((k_wing_mfma<MFMA_NB, double>), wgrid, wblock, 0, s, pa);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
         NearArgs na;
@@ -1408,8 +1626,9 @@ void korg_destroy(korg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
-                      &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->h_alpha,
-                      &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
+                      &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->cls_rng,
+                      &ctx->cls_line, &ctx->near_rng, &ctx->near_line, &ctx->tc_count, &ctx->tc_start,
+                      &ctx->tile_lines, &ctx->h_alpha, &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
