@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 3
+#define KORG_LINE_ABI_VERSION 4
 
 enum korg_status {
     KORG_OK = 0,
@@ -182,6 +182,42 @@ typedef struct {
     int32_t count_branches;    /* 1: fill korg_stats.n_evals_branch (extra setup cost) */
 } korg_tuning;
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
+
+/*
+ * Gradients of the adjoint: dL/dx for each per-layer input x of korg_layers, given
+ * G = dL/dalpha.  A NULL member is not written.
+ */
+typedef struct {
+    double* T;        /* [n_layers] dL/dT */
+    double* n_e;      /* [n_layers] dL/dn_e */
+    double* xi;       /* [n_layers] dL/dxi (xi in cm/s) */
+    double* n_HI;     /* [n_layers] dL/dn(H I) (through the vdW perturber density) */
+    double* n_HeI;    /* [n_layers] dL/dn(He I) */
+    double* n_H2;     /* [n_layers] dL/dn(H2) */
+    double* n_div_U;  /* [n_species * n_layers] dL/d(n_s / U_s), laid out as korg_layers.n_div_U */
+} korg_layer_grads;
+
+/*
+ * Vector-Jacobian product of line_absorption! (src/line_absorption.jl:33-129) with respect to
+ * its per-layer inputs: the reverse-mode derivative a ChainRulesCore rrule of the line opacity
+ * needs (the reference differentiates this call with ForwardDiff duals, src/synthesize.jl:305-306
+ * under Fit).  grad_alpha: G = dL/dalpha, column-major [n_layers x n_wl], dtype korg_dtype.
+ * Windows are held fixed (they are integer functions of the inputs); every other dependence,
+ * including the branch of voigt_hjerting in use, is differentiated exactly.  Computed in
+ * Float64; deterministic.  Host pointers; synchronous.
+ */
+int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dtype,
+                             const korg_grid* grid, const korg_lines* lines,
+                             const korg_species* species, const korg_layers* layers,
+                             const korg_continuum* cntm, double cutoff_threshold,
+                             korg_layer_grads* grads);
+
+/* The same on DEVICE memory (grad_alpha, the structs' arrays and the grads members). */
+int korg_line_absorption_vjp_device(korg_ctx* ctx, const void* grad_alpha, int grad_dtype,
+                                    const korg_grid* grid, const korg_lines* lines,
+                                    const korg_species* species, const korg_layers* layers,
+                                    const korg_continuum* cntm, double cutoff_threshold,
+                                    korg_layer_grads* grads, int synchronize);
 
 #ifdef __cplusplus
 }

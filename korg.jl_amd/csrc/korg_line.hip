@@ -1222,6 +1222,152 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Adjoint (vector-Jacobian product) of the line opacity, in Float64.  Given G = dL/dalpha, a work
+// item is (line, chunk of up to VJP_CHUNK window points): one wavefront, lane = layer (layers l,
+// l + 64, ... when L > 64).  Each lane sums over the chunk's points, in point order,
+//   S_H = sum G H,  S_a = sum G dH/da,  S_v = sum G v dH/dv,
+// and turns them into the item's share of dL/dx for its layer,
+//   scaling (da/dx S_a + dln(1/(sigma sqrt 2))/dx S_v) + dscaling/dx S_H,
+// for x = T, n_e, xi, n_HI, n_HeI, n_H2 and the n_div_U of the line's species.  The windows are
+// integer functions of the inputs and are held fixed.  k_vjp_reduce sums the items in a fixed
+// order, so the gradients are deterministic.
+// ---------------------------------------------------------------------------------------------
+constexpr int VJP_CHUNK = 2048;
+constexpr int VJP_NQ = 7;  // T, n_e, xi, n_HI, n_HeI, n_H2, n_div_U
+
+__global__ __launch_bounds__(256) void k_vjp_count(int64_t n, const LineMeta* __restrict__ meta,
+                                                   int* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LineMeta m = meta[i];
+    cnt[i] = m.lb <= m.ub ? (m.ub - m.lb + VJP_CHUNK) / VJP_CHUNK : 0;
+}
+
+struct VjpArgs {
+    int64_t n_lines, n_items;
+    int L, Ls;
+    const double* wls;
+    const LineMeta* meta;
+    const double4* rec64;
+    const int64_t* item_start;  // [n_lines + 1]: first item of each line
+    const double *temps, *n_e, *xi, *n_div_U;
+    const LayerTab* tab;
+    const double *wl, *log_gf, *E_lower, *gamma_stark, *vdW1, *vdW2;
+    const int32_t* species;
+    const double* mass;
+    const uint8_t* is_mol;
+    const void* G;
+    int g_f32;
+    double* contrib;     // [item][VJP_NQ][L]
+    int* item_species;   // [item]
+    const unsigned long long* counters;
+};
+
+__global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
+    const int lane = threadIdx.x;
+    const int L = A.L;
+    if (A.counters[3] != 0) return;  // failed setup
+    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
+    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;  // as k_layers
+    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
+    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+    for (int64_t item = blockIdx.x; item < A.n_items; item += gridDim.x) {
+        // the line of this item: the largest i with item_start[i] <= item
+        int64_t lo = 0, hi = A.n_lines;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (A.item_start[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int64_t i = lo;
+        const LineMeta m = A.meta[i];
+        const int64_t j0 = m.lb + (item - A.item_start[i]) * VJP_CHUNK;
+        const int64_t j1 = min(j0 + VJP_CHUNK, (int64_t)m.ub + 1);  // exclusive
+        const double lam0 = m.lam0;
+        const int32_t s = A.species[i];
+        const double mass = A.mass[s];
+        const bool mol = A.is_mol[s] != 0;
+        const double E_up = A.E_lower[i] + c_cgs * hplanck_eV / lam0;  // src/line_absorption.jl:101
+        const double gf = pow(10.0, A.log_gf[i]);
+        const double sl = sigma_line(lam0);
+        const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * PI));
+        for (int l = lane; l < L; l += WAVE) {
+            const double4 r = A.rec64[i * A.Ls + l];
+            const double inv = r.x, a = r.y, scaling = r.z;
+            double SH = 0.0, Sa = 0.0, Sv = 0.0;
+            for (int64_t j = j0; j < j1; ++j) {
+                const double g = A.g_f32 ? (double)((const float*)A.G)[l + (int64_t)L * j]
+                                         : ((const double*)A.G)[l + (int64_t)L * j];
+                const double v = fabs(A.wls[j] - lam0) * inv;
+                double Ha, Hv;
+                const double H = voigt_hjerting64_grad(a, v, Ha, Hv);
+                SH += g * H;
+                Sa += g * Ha;
+                Sv += g * v * Hv;
+            }
+            const LayerTab tb = A.tab[l];
+            const double T = A.temps[l], xi = A.xi[l];
+            // d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 (src/line_absorption.jl:190)
+            const double q = kboltz_cgs * T / mass + xi * xi / 2.0;
+            const double dli_dT = -(kboltz_cgs / mass) / (2.0 * q);
+            const double dli_dxi = -xi / (2.0 * q);
+            // dGamma/dx (src/line_absorption.jl:90-99)
+            double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
+            if (!mol) {
+                const double st = A.gamma_stark[i] * tb.t16;
+                const double v1 = A.vdW1[i], v2 = A.vdW2[i];
+                vdw = (v2 == -1.0) ? v1 * tb.t03 : scaled_vdW_abo(v1, v2, mass, T);
+                const double dvdw_dT = (v2 == -1.0 ? 0.3 : (1.0 - v2) / 2.0) * vdw / T;
+                dG_dne = st;
+                dG_dT = A.n_e[l] * st / (6.0 * T) + tb.n_eff * dvdw_dT;
+            }
+            const double ig = inv * gscale;
+            const double da_dT = ig * dG_dT + a * dli_dT;
+            const double da_dxi = a * dli_dxi;
+            // scaling = inv / sqrt(pi) * gf * sigma_line * levels * n/U (src/line_absorption.jl:101-106)
+            const double nU = A.n_div_U[(int64_t)s * L + l];
+            const double el = exp(-tb.beta * A.E_lower[i]), eu = exp(-tb.beta * E_up);
+            const double dlev_dT = (tb.beta / T) * (A.E_lower[i] * el - E_up * eu);
+            const double base = inv / 1.7724538509055159 * gf * sl;
+            const double ds_dT = scaling * dli_dT + base * nU * dlev_dT;
+            const double ds_dxi = scaling * dli_dxi;
+            double* c = A.contrib + item * VJP_NQ * L + l;
+            c[0 * L] = scaling * (da_dT * Sa + dli_dT * Sv) + ds_dT * SH;
+            c[1 * L] = scaling * (ig * dG_dne * Sa);
+            c[2 * L] = scaling * (da_dxi * Sa + dli_dxi * Sv) + ds_dxi * SH;
+            c[3 * L] = scaling * (ig * vdw * Sa);
+            c[4 * L] = scaling * (ig * c_He * vdw * Sa);
+            c[5 * L] = scaling * (ig * c_H2 * vdw * Sa);
+            c[6 * L] = base * (el - eu) * SH;
+        }
+        if (lane == 0) A.item_species[item] = s;
+    }
+}
+
+// One block per output: (quantity q < 6, layer l) sums every item's share; (6 + species s, layer
+// l) sums the n_div_U shares of the items of species s.  A fixed per-thread order and a fixed tree:
+// deterministic.  out = [T | n_e | xi | n_HI | n_HeI | n_H2] x L, then n_div_U [S x L].
+__global__ __launch_bounds__(256) void k_vjp_reduce(int64_t n_items, int L,
+                                                    const double* __restrict__ contrib,
+                                                    const int* __restrict__ item_species,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int q = b / L, l = b % L;
+    double sum = 0.0;
+    for (int64_t it = t; it < n_items; it += 256) {
+        if (q < 6) sum += contrib[(it * VJP_NQ + q) * L + l];
+        else if (item_species[it] == q - 6) sum += contrib[(it * VJP_NQ + 6) * L + l];
+    }
+    red[t] = sum;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[b] = red[0];
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -1247,7 +1393,8 @@ struct korg_ctx {
     // device scratch (grow-only)
     DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
         nblk, ncstart, near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line,
-        tc_count, tc_start, tile_lines;
+        tc_count, tc_start, tile_lines, vjp_cnt, vjp_start, vjp_contrib, vjp_species, vjp_out,
+        h_grad;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -1326,29 +1473,17 @@ int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg
     return KORG_OK;
 }
 
-// The device pipeline; every pointer is a device pointer.
-int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
-               const korg_lines* ln, const korg_species* sp, const korg_layers* ly,
-               const korg_continuum* cn, double cutoff, int precision) {
+// Per-layer tables (k_layers) and the per-line setup (k_setup): windows, zones, the Float64
+// profile records and, for the Float32 form, the MFMA coefficient blocks.  Records ev[0], ev[1].
+int launch_setup(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg_species* sp,
+                 const korg_layers* ly, const korg_continuum* cn, double cutoff, bool f32) {
     const int64_t N = ln->n_lines;
     const int L = ly->n_layers;
     const int64_t nwl = g->n_wl;
-    ctx->last_n_lines = N;
-    ctx->last_precision = precision;
-    ctx->last_n_lblocks = (L + WAVE - 1) / WAVE;
-    std::memset(&ctx->stats, 0, sizeof ctx->stats);
-    if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     hipStream_t s = ctx->stream;
-    const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
-    const int P64 = ctx->tuning.points_per_lane == 2 ? 2 : 1;
-    const int tile_w = f32 ? 32 * MFMA_NB : WAVE * P64;
-    const int ntiles = (int)((nwl + tile_w - 1) / tile_w);
     const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
-    const int Ls = n_groups * F64_GROUP;              // rec64 stride: whole Float64 groups
-    const int n_lblocks = (L + WAVE - 1) / WAVE;      // 64-layer blocks (Float32 records)
-    const int nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
-    const int wpc = ctx->tuning.waves_per_cu > 0 ? ctx->tuning.waves_per_cu : 12;
-
+    const int Ls = n_groups * F64_GROUP;
+    const int n_lblocks = (L + WAVE - 1) / WAVE;
     TRY(grow(ctx, ctx->meta, sizeof(LineMeta) * N));
     TRY(grow(ctx, ctx->jc_all, sizeof(int) * N));
     TRY(grow(ctx, ctx->rec64, sizeof(double4) * N * Ls));
@@ -1356,30 +1491,6 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * L));
     TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * N_COUNTERS));
     TRY(grow(ctx, ctx->gridinfo, sizeof(GridInfo)));
-    TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-    TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
-    TRY(grow(ctx, ctx->cls_cand, sizeof(Cand) * N));
-    TRY(grow(ctx, ctx->cls_rng, sizeof(int2) * N));
-    TRY(grow(ctx, ctx->cls_line, sizeof(int) * N));
-    if (f32) {
-        TRY(grow(ctx, ctx->tc_count, sizeof(int) * (size_t)ntiles * N_CLASSES));
-        TRY(grow(ctx, ctx->tc_start, sizeof(int64_t) * ((size_t)ntiles * N_CLASSES + 1)));
-    }
-    TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
-    TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
-    const int nt_near = (int)((nwl + CT - 1) / CT);
-    if (f32) {
-        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-        TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
-        TRY(grow(ctx, ctx->near_cand, sizeof(Cand) * N));
-        TRY(grow(ctx, ctx->near_rng, sizeof(int2) * N));
-        TRY(grow(ctx, ctx->near_line, sizeof(int) * N));
-        TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
-        TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
-    } else {
-        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-    }
-
     auto* counters = (unsigned long long*)ctx->counters.p;
     auto* gi = (GridInfo*)ctx->gridinfo.p;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
@@ -1425,6 +1536,57 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+    return KORG_OK;
+}
+
+// The device pipeline; every pointer is a device pointer.
+int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
+               const korg_lines* ln, const korg_species* sp, const korg_layers* ly,
+               const korg_continuum* cn, double cutoff, int precision) {
+    const int64_t N = ln->n_lines;
+    const int L = ly->n_layers;
+    const int64_t nwl = g->n_wl;
+    ctx->last_n_lines = N;
+    ctx->last_precision = precision;
+    ctx->last_n_lblocks = (L + WAVE - 1) / WAVE;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
+    hipStream_t s = ctx->stream;
+    const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
+    const int P64 = ctx->tuning.points_per_lane == 2 ? 2 : 1;
+    const int tile_w = f32 ? 32 * MFMA_NB : WAVE * P64;
+    const int ntiles = (int)((nwl + tile_w - 1) / tile_w);
+    const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
+    const int Ls = n_groups * F64_GROUP;              // rec64 stride: whole Float64 groups
+    const int n_lblocks = (L + WAVE - 1) / WAVE;      // 64-layer blocks (Float32 records)
+    const int nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
+    const int wpc = ctx->tuning.waves_per_cu > 0 ? ctx->tuning.waves_per_cu : 12;
+
+    TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
+    TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
+    TRY(grow(ctx, ctx->cls_cand, sizeof(Cand) * N));
+    TRY(grow(ctx, ctx->cls_rng, sizeof(int2) * N));
+    TRY(grow(ctx, ctx->cls_line, sizeof(int) * N));
+    if (f32) {
+        TRY(grow(ctx, ctx->tc_count, sizeof(int) * (size_t)ntiles * N_CLASSES));
+        TRY(grow(ctx, ctx->tc_start, sizeof(int64_t) * ((size_t)ntiles * N_CLASSES + 1)));
+    }
+    TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
+    TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
+    const int nt_near = (int)((nwl + CT - 1) / CT);
+    if (f32) {
+        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
+        TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
+        TRY(grow(ctx, ctx->near_cand, sizeof(Cand) * N));
+        TRY(grow(ctx, ctx->near_rng, sizeof(int2) * N));
+        TRY(grow(ctx, ctx->near_line, sizeof(int) * N));
+        TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
+        TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
+    } else {
+        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
+    }
+    TRY(launch_setup(ctx, g, ln, sp, ly, cn, cutoff, f32));
+    auto* counters = (unsigned long long*)ctx->counters.p;
 
     hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L, 32 * MFMA_NB,
                        (const LineMeta*)ctx->meta.p, (int*)ctx->blk.p, (int*)ctx->nblk.p, counters);
@@ -1534,6 +1696,121 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     return KORG_OK;
 }
 
+// The adjoint pipeline on device pointers: setup (windows and Float64 records), one item per
+// (line, chunk of window points), per-item shares, fixed-order reduction, and the copy of the
+// gradients to `out` (hipMemcpyDeviceToDevice or DeviceToHost).
+int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const korg_lines* ln,
+            const korg_species* sp, const korg_layers* ly, const korg_continuum* cn,
+            double cutoff, korg_layer_grads* out, hipMemcpyKind kind) {
+    const int64_t N = ln->n_lines;
+    const int L = ly->n_layers;
+    const int64_t S = sp->n_species;
+    const int nout = (int)((6 + S) * L);
+    hipStream_t s = ctx->stream;
+    ctx->last_n_lines = N;
+    ctx->last_precision = KORG_PRECISION_FP64;
+    ctx->last_n_lblocks = 0;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    TRY(grow(ctx, ctx->vjp_out, sizeof(double) * (size_t)nout));
+    double* res = (double*)ctx->vjp_out.p;
+    if (N > 0) {
+        const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
+        const int Ls = n_groups * F64_GROUP;
+        TRY(grow(ctx, ctx->vjp_cnt, sizeof(int) * N));
+        TRY(grow(ctx, ctx->vjp_start, sizeof(int64_t) * (N + 1)));
+        TRY(launch_setup(ctx, g, ln, sp, ly, cn, cutoff, false));
+        hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N,
+                           (const LineMeta*)ctx->meta.p, (int*)ctx->vjp_cnt.p);
+        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, N, (const int*)ctx->vjp_cnt.p,
+                           (int64_t*)ctx->vjp_start.p);
+        HIP_TRY(ctx, hipGetLastError());
+        int64_t n_items = 0;
+        unsigned long long flags = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&n_items, (int64_t*)ctx->vjp_start.p + N, sizeof n_items,
+                                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(&flags, (unsigned long long*)ctx->counters.p + 3, sizeof flags,
+                                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (flags & 2)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "a line's species index is outside [0, n_species)");
+        if (flags & 1)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "the line list is not sorted by wavelength");
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+        if (n_items > 0) {
+            TRY(grow(ctx, ctx->vjp_contrib, sizeof(double) * (size_t)n_items * VJP_NQ * L));
+            TRY(grow(ctx, ctx->vjp_species, sizeof(int) * (size_t)n_items));
+            VjpArgs va;
+            va.n_lines = N;
+            va.n_items = n_items;
+            va.L = L;
+            va.Ls = Ls;
+            va.wls = g->all_wls;
+            va.meta = (const LineMeta*)ctx->meta.p;
+            va.rec64 = (const double4*)ctx->rec64.p;
+            va.item_start = (const int64_t*)ctx->vjp_start.p;
+            va.temps = ly->temps;
+            va.n_e = ly->n_e;
+            va.xi = ly->xi;
+            va.n_div_U = ly->n_div_U;
+            va.tab = (const LayerTab*)ctx->tab.p;
+            va.wl = ln->wl;
+            va.log_gf = ln->log_gf;
+            va.E_lower = ln->E_lower;
+            va.gamma_stark = ln->gamma_stark;
+            va.vdW1 = ln->vdW1;
+            va.vdW2 = ln->vdW2;
+            va.species = ln->species;
+            va.mass = sp->mass;
+            va.is_mol = sp->is_molecule;
+            va.G = G;
+            va.g_f32 = g_dtype == KORG_DTYPE_F32;
+            va.contrib = (double*)ctx->vjp_contrib.p;
+            va.item_species = (int*)ctx->vjp_species.p;
+            va.counters = (const unsigned long long*)ctx->counters.p;
+            int64_t grid = (int64_t)ctx->n_cu * 16;
+            if (grid > n_items) grid = n_items;
+            hipLaunchKernelGGL(k_vjp, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        if (n_items > 0) {
+            hipLaunchKernelGGL(k_vjp_reduce, dim3((unsigned)nout), dim3(256), 0, s, n_items, L,
+                               (const double*)ctx->vjp_contrib.p, (const int*)ctx->vjp_species.p,
+                               res);
+        } else {
+            HIP_TRY(ctx, hipMemsetAsync(res, 0, sizeof(double) * (size_t)nout, s));
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    } else {
+        HIP_TRY(ctx, hipMemsetAsync(res, 0, sizeof(double) * (size_t)nout, s));
+    }
+    double* dst[7] = {out->T, out->n_e, out->xi, out->n_HI, out->n_HeI, out->n_H2, out->n_div_U};
+    for (int q = 0; q < 7; ++q) {
+        if (!dst[q]) continue;
+        const size_t cnt = q < 6 ? (size_t)L : (size_t)(S * L);
+        HIP_TRY(ctx, hipMemcpyAsync(dst[q], res + (size_t)q * L, sizeof(double) * cnt, kind, s));
+    }
+    return KORG_OK;
+}
+
+// Times of the last adjoint call (setup, items, reduction, total).
+int finish_vjp_stats(korg_ctx* ctx) {
+    if (ctx->last_n_lines == 0) return KORG_OK;
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+    float t01 = 0, t24 = 0, t43 = 0, t03 = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+    HIP_TRY(ctx, hipEventElapsedTime(&t24, ctx->ev[2], ctx->ev[4]));
+    HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
+    HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+    ctx->stats.setup_ms = t01;
+    ctx->stats.profile_ms = t24;
+    ctx->stats.core_ms = t43;
+    ctx->stats.total_ms = t03;
+    return KORG_OK;
+}
+
 int finish_stats(korg_ctx* ctx) {
     if (ctx->last_n_lines == 0) return KORG_OK;
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
@@ -1576,6 +1853,40 @@ int stage(korg_ctx* ctx, DevBuf& b, const T* src, size_t n, const T** out) {
     TRY(grow(ctx, b, sizeof(T) * n));
     HIP_TRY(ctx, hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream));
     *out = (const T*)b.p;
+    return KORG_OK;
+}
+
+// Copies the host arrays of one call into the context's staging buffers and points the device-side
+// structs (already copied from the host structs) at them.
+int stage_inputs(korg_ctx* ctx, const korg_grid* grid, const korg_lines* lines,
+                 const korg_species* species, const korg_layers* layers, const korg_continuum* cntm,
+                 korg_grid* g, korg_lines* ln, korg_species* sp, korg_layers* ly,
+                 korg_continuum* cn) {
+    const int64_t N = lines->n_lines, L = layers->n_layers, S = species->n_species;
+    TRY(stage(ctx, ctx->h_wls, grid->all_wls, grid->n_wl, &g->all_wls));
+    TRY(stage(ctx, ctx->h_lines[0], lines->wl, N, &ln->wl));
+    TRY(stage(ctx, ctx->h_lines[1], lines->log_gf, N, &ln->log_gf));
+    TRY(stage(ctx, ctx->h_lines[2], lines->E_lower, N, &ln->E_lower));
+    TRY(stage(ctx, ctx->h_lines[3], lines->gamma_rad, N, &ln->gamma_rad));
+    TRY(stage(ctx, ctx->h_lines[4], lines->gamma_stark, N, &ln->gamma_stark));
+    TRY(stage(ctx, ctx->h_lines[5], lines->vdW1, N, &ln->vdW1));
+    TRY(stage(ctx, ctx->h_lines[6], lines->vdW2, N, &ln->vdW2));
+    TRY(stage(ctx, ctx->h_lines[7], lines->species, N, &ln->species));
+    TRY(stage(ctx, ctx->h_species_mass, species->mass, S, &sp->mass));
+    TRY(stage(ctx, ctx->h_species_mol, species->is_molecule, S, &sp->is_molecule));
+    TRY(stage(ctx, ctx->h_layers[0], layers->temps, L, &ly->temps));
+    TRY(stage(ctx, ctx->h_layers[1], layers->n_e, L, &ly->n_e));
+    TRY(stage(ctx, ctx->h_layers[2], layers->n_div_U, S * L, &ly->n_div_U));
+    TRY(stage(ctx, ctx->h_layers[3], layers->n_HI, L, &ly->n_HI));
+    TRY(stage(ctx, ctx->h_layers[4], layers->n_HeI, L, &ly->n_HeI));
+    TRY(stage(ctx, ctx->h_layers[5], layers->n_H2, L, &ly->n_H2));
+    TRY(stage(ctx, ctx->h_layers[6], layers->xi, L, &ly->xi));
+    if (cntm->mode == KORG_CNTM_GRID) {
+        TRY(stage(ctx, ctx->h_knots, cntm->knots, cntm->n_knots, &cn->knots));
+        TRY(stage(ctx, ctx->h_cntm, cntm->values, (size_t)L * cntm->n_knots, &cn->values));
+    } else {
+        TRY(stage(ctx, ctx->h_cntm, cntm->values, (size_t)L * N, &cn->values));
+    }
     return KORG_OK;
 }
 
@@ -1628,7 +1939,8 @@ void korg_destroy(korg_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
                       &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->cls_rng,
                       &ctx->cls_line, &ctx->near_rng, &ctx->near_line, &ctx->tc_count, &ctx->tc_start,
-                      &ctx->tile_lines, &ctx->h_alpha, &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
+                      &ctx->tile_lines, &ctx->vjp_cnt, &ctx->vjp_start, &ctx->vjp_contrib,
+                      &ctx->vjp_species, &ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1712,30 +2024,7 @@ int korg_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg
     korg_species sp = *species;
     korg_layers ly = *layers;
     korg_continuum cn = *cntm;
-    TRY(stage(ctx, ctx->h_wls, grid->all_wls, grid->n_wl, &g.all_wls));
-    TRY(stage(ctx, ctx->h_lines[0], lines->wl, N, &ln.wl));
-    TRY(stage(ctx, ctx->h_lines[1], lines->log_gf, N, &ln.log_gf));
-    TRY(stage(ctx, ctx->h_lines[2], lines->E_lower, N, &ln.E_lower));
-    TRY(stage(ctx, ctx->h_lines[3], lines->gamma_rad, N, &ln.gamma_rad));
-    TRY(stage(ctx, ctx->h_lines[4], lines->gamma_stark, N, &ln.gamma_stark));
-    TRY(stage(ctx, ctx->h_lines[5], lines->vdW1, N, &ln.vdW1));
-    TRY(stage(ctx, ctx->h_lines[6], lines->vdW2, N, &ln.vdW2));
-    TRY(stage(ctx, ctx->h_lines[7], lines->species, N, &ln.species));
-    TRY(stage(ctx, ctx->h_species_mass, species->mass, S, &sp.mass));
-    TRY(stage(ctx, ctx->h_species_mol, species->is_molecule, S, &sp.is_molecule));
-    TRY(stage(ctx, ctx->h_layers[0], layers->temps, L, &ly.temps));
-    TRY(stage(ctx, ctx->h_layers[1], layers->n_e, L, &ly.n_e));
-    TRY(stage(ctx, ctx->h_layers[2], layers->n_div_U, S * L, &ly.n_div_U));
-    TRY(stage(ctx, ctx->h_layers[3], layers->n_HI, L, &ly.n_HI));
-    TRY(stage(ctx, ctx->h_layers[4], layers->n_HeI, L, &ly.n_HeI));
-    TRY(stage(ctx, ctx->h_layers[5], layers->n_H2, L, &ly.n_H2));
-    TRY(stage(ctx, ctx->h_layers[6], layers->xi, L, &ly.xi));
-    if (cntm->mode == KORG_CNTM_GRID) {
-        TRY(stage(ctx, ctx->h_knots, cntm->knots, cntm->n_knots, &cn.knots));
-        TRY(stage(ctx, ctx->h_cntm, cntm->values, (size_t)L * cntm->n_knots, &cn.values));
-    } else {
-        TRY(stage(ctx, ctx->h_cntm, cntm->values, (size_t)L * N, &cn.values));
-    }
+    TRY(stage_inputs(ctx, grid, lines, species, layers, cntm, &g, &ln, &sp, &ly, &cn));
     TRY(grow(ctx, ctx->h_alpha, abytes));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_alpha.p, alpha, abytes, hipMemcpyHostToDevice, ctx->stream));
     TRY(run_device(ctx, ctx->h_alpha.p, alpha_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold,
@@ -1743,6 +2032,61 @@ int korg_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg
     HIP_TRY(ctx, hipMemcpyAsync(alpha, ctx->h_alpha.p, abytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return finish_stats(ctx);
+}
+
+int korg_line_absorption_vjp_device(korg_ctx* ctx, const void* grad_alpha, int grad_dtype,
+                                    const korg_grid* grid, const korg_lines* lines,
+                                    const korg_species* species, const korg_layers* layers,
+                                    const korg_continuum* cntm, double cutoff_threshold,
+                                    korg_layer_grads* grads, int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(validate(ctx, grid, lines, species, layers, cntm, grad_dtype, KORG_PRECISION_FP64));
+    if (!grads) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grads is null");
+    if (!grad_alpha && lines->n_lines > 0)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grad_alpha is null");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_vjp(ctx, grad_alpha, grad_dtype, grid, lines, species, layers, cntm, cutoff_threshold,
+                grads, hipMemcpyDeviceToDevice));
+    if (synchronize) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_vjp_stats(ctx));
+    }
+    return KORG_OK;
+}
+
+int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dtype,
+                             const korg_grid* grid, const korg_lines* lines,
+                             const korg_species* species, const korg_layers* layers,
+                             const korg_continuum* cntm, double cutoff_threshold,
+                             korg_layer_grads* grads) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(validate(ctx, grid, lines, species, layers, cntm, grad_dtype, KORG_PRECISION_FP64));
+    if (!grads) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grads is null");
+    if (!grad_alpha && lines->n_lines > 0)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grad_alpha is null");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int64_t N = lines->n_lines, L = layers->n_layers, S = species->n_species;
+    korg_grid g = *grid;
+    korg_lines ln = *lines;
+    korg_species sp = *species;
+    korg_layers ly = *layers;
+    korg_continuum cn = *cntm;
+    TRY(stage_inputs(ctx, grid, lines, species, layers, cntm, &g, &ln, &sp, &ly, &cn));
+    const void* Gd = nullptr;
+    if (N > 0) {
+        const size_t gbytes = (grad_dtype == KORG_DTYPE_F32 ? 4 : 8) * (size_t)L * (size_t)grid->n_wl;
+        TRY(grow(ctx, ctx->h_grad, gbytes));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_grad.p, grad_alpha, gbytes, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        Gd = ctx->h_grad.p;
+    }
+    (void)S;
+    TRY(run_vjp(ctx, Gd, grad_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold, grads,
+                hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return finish_vjp_stats(ctx);
 }
 
 int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {

@@ -169,4 +169,87 @@ __device__ inline float voigt_core_f32(float a, float v, int piece, bool branch_
     return psi * (M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a);
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Derivatives for the adjoint.  The partial derivatives in a and v of the Hunger branch chosen
+// at (a, v) -- the derivative of voigt_hjerting as written (src/line_absorption.jl:286-310),
+// which is what forward-mode differentiation of the reference yields.  Returns H.
+// ---------------------------------------------------------------------------------------------
+__device__ inline void harris_series64_dv(double v, double H0, double& dH0, double& dH1,
+                                          double& dH2) {
+    const double v2 = v * v;
+    dH0 = -2.0 * v * H0;
+    if (v < 1.3) {
+        dH1 = -0.15516677 + (2.0 * 3.288675912 + (3.0 * -2.34357915 + 4.0 * 0.42139162 * v) * v) * v;
+    } else if (v < 2.4) {
+        dH1 = 9.39456063 + (2.0 * -6.61487486 + (3.0 * 1.98919585 - 4.0 * 0.22041650 * v) * v) * v;
+    } else {
+        const double P = 0.554153432 +
+                         (0.278711796 + (-0.1883256872 + (0.042991293 - 0.003278278 * v) * v) * v) * v;
+        const double dP = 0.278711796 +
+                          (2.0 * -0.1883256872 + (3.0 * 0.042991293 - 4.0 * 0.003278278 * v) * v) * v;
+        const double q = v2 - 1.5;
+        dH1 = (dP * q - 2.0 * v * P) / (q * q);
+    }
+    dH2 = H0 * (-6.0 * v + 4.0 * v2 * v);
+}
+
+__device__ inline double voigt_hjerting64_grad(double a, double v, double& dHda, double& dHdv) {
+    const double sqrt_pi = 1.7724538509055159;
+    const double v2 = v * v;
+    if (a <= 0.2 && v >= 5.0) {
+        const double w = 1.0 / v2;
+        const double poly = 1.0 + 1.5 * w + 3.75 * (w * w);
+        dHda = w / sqrt_pi * poly;
+        dHdv = a / sqrt_pi * (1.0 + 3.0 * w + 11.25 * w * w) * (-2.0 * w / v);
+        return (a / sqrt_pi * w) * poly;
+    } else if (a <= 0.2) {
+        double H1, H2, dH0, dH1, dH2;
+        const double H0 = harris_series64(v, H1, H2);
+        harris_series64_dv(v, H0, dH0, dH1, dH2);
+        dHda = H1 + 2.0 * H2 * a;
+        dHdv = dH0 + (dH1 + dH2 * a) * a;
+        return H0 + (H1 + H2 * a) * a;
+    } else if (a <= 1.4 && a + v < 3.2) {
+        double H1, H2, dH0, dH1, dH2;
+        const double H0 = harris_series64(v, H1, H2);
+        harris_series64_dv(v, H0, dH0, dH1, dH2);
+        const double k = 2.0 / sqrt_pi, k3 = 2.0 / (3.0 * sqrt_pi);
+        const double M0 = H0;
+        const double M1 = H1 + k * M0;
+        const double M2 = H2 - M0 + k * M1;
+        const double M3 = k3 * (1.0 - H2) - (2.0 / 3.0) * v2 * M1 + k * M2;
+        const double M4 = 2.0 / 3.0 * v2 * v2 * M0 - k3 * M1 + k * M3;
+        const double dM0 = dH0;
+        const double dM1 = dH1 + k * dM0;
+        const double dM2 = dH2 - dM0 + k * dM1;
+        const double dM3 = -k3 * dH2 - (2.0 / 3.0) * (2.0 * v * M1 + v2 * dM1) + k * dM2;
+        const double dM4 = 2.0 / 3.0 * (4.0 * v2 * v * M0 + v2 * v2 * dM0) - k3 * dM1 + k * dM3;
+        const double psi = 0.979895023 + (-0.962846325 + (0.532770573 - 0.122727278 * a) * a) * a;
+        const double dpsi = -0.962846325 + (2.0 * 0.532770573 - 3.0 * 0.122727278 * a) * a;
+        const double S = M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a;
+        const double dSda = M1 + (2.0 * M2 + (3.0 * M3 + 4.0 * M4 * a) * a) * a;
+        const double dSdv = dM0 + (dM1 + (dM2 + (dM3 + dM4 * a) * a) * a) * a;
+        dHda = dpsi * S + psi * dSda;
+        dHdv = psi * dSdv;
+        return psi * S;
+    } else {
+        // u = a / (sqrt(2) (v^2 + a^2)), r^2 = v^2 / a^2; H = sqrt(2/pi) u P(r^2, u)
+        const double r2 = v2 / (a * a);
+        const double u = 1.0 / 1.4142135623730951 / ((r2 + 1.0) * a);
+        const double u2 = u * u;
+        const double c4 = (r2 - 2.0) * 15.0 * r2 + 2.0;
+        const double P = 1.0 + (3.0 * r2 - 1.0 + c4 * u2) * u2;
+        const double den = 1.4142135623730951 * (v2 + a * a) * (v2 + a * a);
+        const double du_da = (v2 - a * a) / den, du_dv = -2.0 * a * v / den;
+        const double dr2_da = -2.0 * v2 / (a * a * a), dr2_dv = 2.0 * v / (a * a);
+        const double dP_dr2 = 3.0 * u2 + (30.0 * r2 - 30.0) * u2 * u2;
+        const double dP_du = 2.0 * (3.0 * r2 - 1.0) * u + 4.0 * c4 * u2 * u;
+        const double s2p = 0.79788456080286536;  // sqrt(2/pi)
+        dHda = s2p * (P * du_da + u * (dP_dr2 * dr2_da + dP_du * du_da));
+        dHdv = s2p * (P * du_dv + u * (dP_dr2 * dr2_dv + dP_du * du_dv));
+        return s2p * u * P;
+    }
+}
+
 }  // namespace korg

@@ -61,16 +61,22 @@ class KorgStats(ctypes.Structure):
                 ("n_evals_tail", ctypes.c_double)]
 
 
+class KorgLayerGrads(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2",
+                                               "n_div_U")]
+
+
 class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
 
 
-ABI_VERSION = 3  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 4  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
-            "korg_synchronize", "korg_get_stats", "korg_get_windows", "korg_set_tuning"]
+            "korg_synchronize", "korg_get_stats", "korg_get_windows", "korg_set_tuning",
+            "korg_line_absorption_vjp", "korg_line_absorption_vjp_device"]
 
 _lib = None
 
@@ -116,6 +122,13 @@ def load():
     L.korg_get_windows.restype = i32
     L.korg_set_tuning.argtypes = [vp, ctypes.POINTER(KorgTuning)]
     L.korg_set_tuning.restype = i32
+    vjp = [vp, vp, i32, ctypes.POINTER(KorgGrid), ctypes.POINTER(KorgLines),
+           ctypes.POINTER(KorgSpecies), ctypes.POINTER(KorgLayers), ctypes.POINTER(KorgContinuum),
+           d, ctypes.POINTER(KorgLayerGrads)]
+    L.korg_line_absorption_vjp.argtypes = vjp
+    L.korg_line_absorption_vjp.restype = i32
+    L.korg_line_absorption_vjp_device.argtypes = vjp + [i32]
+    L.korg_line_absorption_vjp_device.restype = i32
     if L.korg_abi_version() != ABI_VERSION:
         raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
                           f"expect {ABI_VERSION}; rebuild it")

@@ -167,6 +167,50 @@ class Context:
             1 if synchronize else 0)
         _lib.check(self._h, rc)
 
+    def line_absorption_vjp(self, G, problem):
+        """Adjoint on host arrays: given ``G`` = dL/dalpha (float64/float32, (n_layers, n_wl),
+        Fortran-ordered), returns dL/dx for the per-layer inputs: a dict with ``T``, ``n_e``,
+        ``xi``, ``n_HI``, ``n_HeI``, ``n_H2`` (length n_layers) and ``n_div_U``
+        (n_species x n_layers).  Windows are held fixed; Float64; deterministic."""
+        if G.shape != (problem.n_layers, problem.n_wl):
+            raise ValueError(f"G has shape {G.shape}, expected {(problem.n_layers, problem.n_wl)}")
+        if not G.flags.f_contiguous:
+            raise ValueError("G must be column-major (Fortran order), like alpha")
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        if G.dtype not in dt:
+            raise TypeError("G must be float64 or float32")
+        L, S = problem.n_layers, problem.species_mass.size
+        out = {k: np.zeros(L) for k in ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2")}
+        out["n_div_U"] = np.zeros((S, L))
+        grads = _lib.KorgLayerGrads(*[out[k].ctypes.data for k in
+                                      ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2", "n_div_U")])
+        structs, keep = _host_structs(problem)
+        g, lines, sp, ly, cn = structs
+        rc = self._L.korg_line_absorption_vjp(self._h, _ptr(G), dt[G.dtype], ctypes.byref(g),
+                                              ctypes.byref(lines), ctypes.byref(sp),
+                                              ctypes.byref(ly), ctypes.byref(cn),
+                                              problem.cutoff_threshold, ctypes.byref(grads))
+        del keep
+        _lib.check(self._h, rc)
+        return out
+
+    def line_absorption_vjp_device(self, G_dev, dprob, out_dev, synchronize=True):
+        """Adjoint on device arrays: ``G_dev`` a (n_wl, n_layers) device array (the layout of
+        ``DeviceProblem.new_alpha``); ``out_dev`` a dict of device arrays keyed as the host form's
+        result (missing keys are not written)."""
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        P = dprob.problem
+        if tuple(G_dev.shape) != (P.n_wl, P.n_layers) or G_dev.dtype not in dt:
+            raise ValueError("G_dev must be a (n_wl, n_layers) float32/float64 device array")
+        names = ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2", "n_div_U")
+        grads = _lib.KorgLayerGrads(*[out_dev[k].ptr if k in out_dev else None for k in names])
+        rc = self._L.korg_line_absorption_vjp_device(
+            self._h, ctypes.c_void_p(G_dev.ptr), dt[G_dev.dtype], ctypes.byref(dprob.grid),
+            ctypes.byref(dprob.lines), ctypes.byref(dprob.species), ctypes.byref(dprob.layers),
+            ctypes.byref(dprob.cntm), P.cutoff_threshold, ctypes.byref(grads),
+            1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
     def synchronize(self):
         _lib.check(self._h, self._L.korg_synchronize(self._h))
 
@@ -197,3 +241,50 @@ def line_absorption_(alpha, linelist, wls, temps, n_e, n_densities, partition_fn
                                alpha_cntm, cutoff_threshold)
     (ctx or default_context()).line_absorption(alpha, P, precision)
     return None
+
+
+def line_absorption_vjp_(G, linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                         alpha_cntm, cutoff_threshold=3e-4, ctx=None):
+    """Reverse-mode derivative of ``line_absorption_`` with respect to its per-layer arguments,
+    given ``G`` = dL/dalpha (shape (n_layers, n_wl), column-major): the pullback a
+    ChainRulesCore ``rrule`` of the line opacity returns.
+
+    Returns a dict:
+
+    * ``temps``: dL/dT per layer, including the partition functions' dependence on T
+      (d(n/U)/dT = -n U'(ln T) / (U^2 T), U' by a central difference of the callable);
+    * ``n_e`` and ``xi`` per layer (``xi`` summed to a scalar when ``xi`` is a scalar);
+    * ``n_densities``: {species: dL/dn per layer} -- the line species through n/U, and H I,
+      He I and H2 through the van der Waals perturber density;
+    * ``n_div_U``: the raw derivative with respect to n/U (species x layers, line-list order).
+
+    The windows are held fixed (integer functions of the inputs); the adjoint runs in Float64.
+    """
+    if not linelist.is_sorted():
+        linelist = linelist.sorted()
+    P = Problem.from_korg_args(linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                               alpha_cntm, cutoff_threshold)
+    G = np.asfortranarray(G)
+    raw = (ctx or default_context()).line_absorption_vjp(G, P)
+    temps = np.asarray(temps, dtype=np.float64)
+    lnT = np.log(temps)
+    dT = raw["T"].copy()
+    dens = {}
+    used = np.unique(linelist.species_index) if len(linelist) else []
+    for s in used:
+        sp = linelist.species[s]
+        n = np.asarray(n_densities[sp], dtype=np.float64)
+        U = np.asarray(partition_fns[sp](lnT), dtype=np.float64)
+        h = 1e-6
+        dU = (np.asarray(partition_fns[sp](lnT + h)) - np.asarray(partition_fns[sp](lnT - h))) / (2 * h)
+        dens[sp] = raw["n_div_U"][s] / U
+        dT += raw["n_div_U"][s] * (-n * dU / (U * U * temps))
+    from .species import Species
+    for code, key in (("H I", "n_HI"), ("He I", "n_HeI"), ("H2", "n_H2")):
+        sp = Species(code)
+        if sp in n_densities:
+            dens[sp] = dens.get(sp, 0.0) + raw[key]
+    dxi = raw["xi"] if np.ndim(xi) else float(np.sum(raw["xi"]))
+    return {"temps": dT, "n_e": raw["n_e"], "xi": dxi, "n_densities": dens,
+            "n_div_U": raw["n_div_U"]}
+

@@ -401,3 +401,236 @@ int oracle_line_absorption(double* alpha, int64_t L, int64_t nwl, const double* 
     free(sig); free(gam); free(amp); free(lb); free(ub); free(beta); free(n_eff);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------------------------------ */
+/* Adjoint of the line opacity (the reverse-mode derivative of line_absorption! with respect to
+ * its per-layer inputs).  The windows are integer functions of the inputs and are held fixed,
+ * as ForwardDiff does through searchsorted; everything else is differentiated exactly,
+ * including the branch structure of voigt_hjerting (the derivative of the branch chosen). */
+
+/* d/dv of the Harris series pieces (src/line_absorption.jl:251-265): writes H0', H1', H2'. */
+static void harris_series_dv(double v, double H0, double H1, double* dH0, double* dH1,
+                             double* dH2) {
+    const double v2 = v * v;
+    *dH0 = -2.0 * v * H0;
+    if (v < 1.3) {
+        *dH1 = -0.15516677 + (2.0 * 3.288675912 + (3.0 * -2.34357915 + 4.0 * 0.42139162 * v) * v) * v;
+    } else if (v < 2.4) {
+        *dH1 = 9.39456063 + (2.0 * -6.61487486 + (3.0 * 1.98919585 - 4.0 * 0.22041650 * v) * v) * v;
+    } else {
+        const double P = 0.554153432 +
+                         (0.278711796 + (-0.1883256872 + (0.042991293 - 0.003278278 * v) * v) * v) * v;
+        const double dP = 0.278711796 +
+                          (2.0 * -0.1883256872 + (3.0 * 0.042991293 - 4.0 * 0.003278278 * v) * v) * v;
+        const double q = v2 - 1.5;
+        (void)H1;
+        *dH1 = (dP * q - 2.0 * v * P) / (q * q);
+    }
+    *dH2 = H0 * (-6.0 * v + 4.0 * v2 * v);
+}
+
+/* voigt_hjerting (src/line_absorption.jl:286-310) and its partial derivatives in a and v. */
+double oracle_voigt_hjerting_grad(double a, double v, double* dHda, double* dHdv) {
+    const double sqrt_pi = sqrt(M_PI);
+    const double v2 = v * v;
+    if (a <= 0.2 && v >= 5.0) {
+        const double w = 1.0 / v2;
+        const double poly = 1.0 + 1.5 * w + 3.75 * (w * w);
+        *dHda = w / sqrt_pi * poly;
+        /* d/dv [w (1 + 1.5 w + 3.75 w^2)] = (1 + 3 w + 11.25 w^2) dw/dv, dw/dv = -2 w / v */
+        *dHdv = a / sqrt_pi * (1.0 + 3.0 * w + 11.25 * w * w) * (-2.0 * w / v);
+        return (a / sqrt_pi * w) * poly;
+    } else if (a <= 0.2) {
+        double H1, H2, dH0, dH1, dH2;
+        const double H0 = oracle_harris_series(v, &H1, &H2);
+        harris_series_dv(v, H0, H1, &dH0, &dH1, &dH2);
+        *dHda = H1 + 2.0 * H2 * a;
+        *dHdv = dH0 + (dH1 + dH2 * a) * a;
+        return H0 + (H1 + H2 * a) * a;
+    } else if (a <= 1.4 && a + v < 3.2) {
+        double H1, H2, dH0, dH1, dH2;
+        const double H0 = oracle_harris_series(v, &H1, &H2);
+        harris_series_dv(v, H0, H1, &dH0, &dH1, &dH2);
+        const double k = 2.0 / sqrt_pi, k3 = 2.0 / (3.0 * sqrt_pi);
+        const double M0 = H0;
+        const double M1 = H1 + k * M0;
+        const double M2 = H2 - M0 + k * M1;
+        const double M3 = k3 * (1.0 - H2) - (2.0 / 3.0) * v2 * M1 + k * M2;
+        const double M4 = 2.0 / 3.0 * v2 * v2 * M0 - k3 * M1 + k * M3;
+        const double dM0 = dH0;
+        const double dM1 = dH1 + k * dM0;
+        const double dM2 = dH2 - dM0 + k * dM1;
+        const double dM3 = -k3 * dH2 - (2.0 / 3.0) * (2.0 * v * M1 + v2 * dM1) + k * dM2;
+        const double dM4 = 2.0 / 3.0 * (4.0 * v2 * v * M0 + v2 * v2 * dM0) - k3 * dM1 + k * dM3;
+        const double psi = 0.979895023 + (-0.962846325 + (0.532770573 - 0.122727278 * a) * a) * a;
+        const double dpsi = -0.962846325 + (2.0 * 0.532770573 - 3.0 * 0.122727278 * a) * a;
+        const double S = M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a;
+        const double dSda = M1 + (2.0 * M2 + (3.0 * M3 + 4.0 * M4 * a) * a) * a;
+        const double dSdv = dM0 + (dM1 + (dM2 + (dM3 + dM4 * a) * a) * a) * a;
+        *dHda = dpsi * S + psi * dSda;
+        *dHdv = psi * dSdv;
+        return psi * S;
+    } else {
+        const double r2 = v2 / (a * a);
+        const double u = 1.0 / sqrt(2.0) / ((r2 + 1.0) * a);
+        const double u2 = u * u;
+        const double c4 = (r2 - 2.0) * 15.0 * r2 + 2.0; /* 15 r^4 - 30 r^2 + 2 */
+        const double P = 1.0 + (3.0 * r2 - 1.0 + c4 * u2) * u2;
+        /* u = a / (sqrt(2) (v^2 + a^2)); r^2 = v^2 / a^2 */
+        const double den = sqrt(2.0) * (v2 + a * a) * (v2 + a * a);
+        const double du_da = (v2 - a * a) / den, du_dv = -2.0 * a * v / den;
+        const double dr2_da = -2.0 * v2 / (a * a * a), dr2_dv = 2.0 * v / (a * a);
+        const double dP_dr2 = 3.0 * u2 + (30.0 * r2 - 30.0) * u2 * u2;
+        const double dP_du = 2.0 * (3.0 * r2 - 1.0) * u + 4.0 * c4 * u2 * u;
+        const double s2p = sqrt(2.0 / M_PI);
+        *dHda = s2p * (P * du_da + u * (dP_dr2 * dr2_da + dP_du * du_da));
+        *dHdv = s2p * (P * du_dv + u * (dP_dr2 * dr2_dv + dP_du * du_dv));
+        return s2p * u * P;
+    }
+}
+
+/*
+ * Vector-Jacobian product of oracle_line_absorption: given G = dL/dalpha (column-major
+ * [L x nwl], G[l + L*j]), writes dL/d(input) per layer:
+ *   g_T[l], g_ne[l], g_xi[l], g_nHI[l], g_nHeI[l], g_nH2[l]  and  g_nU[s*L + l] (n_div_U).
+ * Inputs as oracle_line_absorption.  Each line's contribution per layer is
+ *   scaling * (da/dx * S_a + dln(1/(sigma sqrt 2))/dx * S_v) + dscaling/dx * S_H
+ * with S_H = sum_j G H, S_a = sum_j G dH/da, S_v = sum_j G v dH/dv over the line's window; the
+ * contributions are summed in line order (deterministic).  Returns 0.
+ */
+int oracle_line_absorption_vjp(const double* G, int64_t L, int64_t nwl, const double* wls,
+                               int64_t nlines, const double* wl, const double* log_gf,
+                               const double* E_lower, const double* gamma_rad,
+                               const double* gamma_stark, const double* vdW1, const double* vdW2,
+                               const int32_t* species_index, const double* species_mass,
+                               const uint8_t* species_is_mol, int64_t n_species,
+                               const double* n_div_U, const double* temps, const double* n_e,
+                               const double* n_HI, const double* n_HeI, const double* n_H2,
+                               const double* xi, int cntm_mode, const double* cntm_values,
+                               int64_t n_knots, const double* knots, double cutoff_threshold,
+                               int nthreads, double* g_T, double* g_ne, double* g_xi,
+                               double* g_nHI, double* g_nHeI, double* g_nH2, double* g_nU) {
+    for (int64_t l = 0; l < L; ++l) g_T[l] = g_ne[l] = g_xi[l] = g_nHI[l] = g_nHeI[l] = g_nH2[l] = 0.0;
+    for (int64_t k = 0; k < n_species * L; ++k) g_nU[k] = 0.0;
+    if (nlines == 0) return 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    problem_t P = {L, nwl, nlines, wls, wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2,
+                   species_index, species_mass, species_is_mol, n_div_U, temps, n_e, xi,
+                   cntm_mode, cntm_values, knots, n_knots, cutoff_threshold, NULL, NULL};
+    double* beta = malloc(sizeof(double) * L);
+    double* n_eff = malloc(sizeof(double) * L);
+    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
+    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;
+    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
+    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+    for (int64_t l = 0; l < L; ++l) {
+        beta[l] = 1.0 / (kboltz_eV * temps[l]);
+        n_eff[l] = n_HI[l] + c_He * n_HeI[l] + c_H2 * n_H2[l];
+    }
+    P.beta = beta;
+    P.n_eff = n_eff;
+    enum { NG = 7 }; /* T, ne, xi, nHI, nHeI, nH2, nU */
+    double* contrib = calloc((size_t)(nlines * L * NG), sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+        double* sig = malloc(sizeof(double) * L);
+        double* gam = malloc(sizeof(double) * L);
+        double* amp = malloc(sizeof(double) * L);
+        double* rho = malloc(sizeof(double) * L);
+        double* tmp = malloc(sizeof(double) * L);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < nlines; ++i) {
+            int64_t lb, ub;
+            line_setup(&P, i, sig, gam, amp, rho, tmp, &lb, &ub);
+            if (lb > ub) continue;
+            const int32_t s = species_index[i];
+            const double m = species_mass[s];
+            const int mol = species_is_mol[s] != 0;
+            const double lam0 = wl[i];
+            const double E_up = E_lower[i] + c_cgs * hplanck_eV / lam0;
+            const double gf = pow(10.0, log_gf[i]);
+            const double sl = oracle_sigma_line(lam0);
+            const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * M_PI));
+            for (int64_t l = 0; l < L; ++l) {
+                const double T = temps[l];
+                const double inv = 1.0 / (sig[l] * sqrt(2.0));
+                const double a = gam[l] * inv;
+                const double scaling = inv / sqrt(M_PI) * amp[l];
+                double SH = 0.0, Sa = 0.0, Sv = 0.0;
+                for (int64_t j = lb - 1; j < ub; ++j) {
+                    const double g = G[j * L + l];
+                    if (g == 0.0) continue;
+                    const double v = fabs(wls[j] - lam0) * inv;
+                    double Ha, Hv;
+                    const double H = oracle_voigt_hjerting_grad(a, v, &Ha, &Hv);
+                    SH += g * H;
+                    Sa += g * Ha;
+                    Sv += g * v * Hv;
+                }
+                /* d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 */
+                const double q = kboltz_cgs * T / m + xi[l] * xi[l] / 2.0;
+                const double dlinv_dT = -(kboltz_cgs / m) / (2.0 * q);
+                const double dlinv_dxi = -xi[l] / (2.0 * q);
+                /* dGamma / dx */
+                double dG_dT = 0.0, dG_dne = 0.0, dG_dnHI = 0.0, dG_dnHeI = 0.0, dG_dnH2 = 0.0;
+                if (!mol) {
+                    const double st = oracle_scaled_stark(gamma_stark[i], T);
+                    const double vdw = oracle_scaled_vdW(vdW1[i], vdW2[i], m, T);
+                    const double dvdw_dT = (vdW2[i] == -1.0 ? 0.3 : (1.0 - vdW2[i]) / 2.0) * vdw / T;
+                    dG_dne = st;
+                    dG_dT = n_e[l] * st / (6.0 * T) + n_eff[l] * dvdw_dT;
+                    dG_dnHI = vdw;
+                    dG_dnHeI = c_He * vdw;
+                    dG_dnH2 = c_H2 * vdw;
+                }
+                /* da/dx = inv dgamma/dx + a dln(inv)/dx */
+                const double da_dT = inv * gscale * dG_dT + a * dlinv_dT;
+                const double da_dxi = a * dlinv_dxi;
+                const double da_dne = inv * gscale * dG_dne;
+                const double da_dnHI = inv * gscale * dG_dnHI;
+                const double da_dnHeI = inv * gscale * dG_dnHeI;
+                const double da_dnH2 = inv * gscale * dG_dnH2;
+                /* scaling = inv / sqrt(pi) * gf * sigma_line * levels * nU */
+                const double nU = n_div_U[(int64_t)s * L + l];
+                const double bl = beta[l];
+                const double el = exp(-bl * E_lower[i]), eu = exp(-bl * E_up);
+                const double dlev_dT = (bl / T) * (E_lower[i] * el - E_up * eu);
+                const double base = inv / sqrt(M_PI) * gf * sl;
+                const double ds_dT = scaling * dlinv_dT + base * nU * dlev_dT;
+                const double ds_dxi = scaling * dlinv_dxi;
+                const double ds_dnU = base * (el - eu);
+                double* c = contrib + (i * L + l) * NG;
+                c[0] = scaling * (da_dT * Sa + dlinv_dT * Sv) + ds_dT * SH;
+                c[1] = scaling * (da_dne * Sa);
+                c[2] = scaling * (da_dxi * Sa + dlinv_dxi * Sv) + ds_dxi * SH;
+                c[3] = scaling * (da_dnHI * Sa);
+                c[4] = scaling * (da_dnHeI * Sa);
+                c[5] = scaling * (da_dnH2 * Sa);
+                c[6] = ds_dnU * SH;
+            }
+        }
+        free(sig); free(gam); free(amp); free(rho); free(tmp);
+    }
+    for (int64_t i = 0; i < nlines; ++i) {
+        const int32_t s = species_index[i];
+        for (int64_t l = 0; l < L; ++l) {
+            const double* c = contrib + (i * L + l) * NG;
+            g_T[l] += c[0];
+            g_ne[l] += c[1];
+            g_xi[l] += c[2];
+            g_nHI[l] += c[3];
+            g_nHeI[l] += c[4];
+            g_nH2[l] += c[5];
+            g_nU[(int64_t)s * L + l] += c[6];
+        }
+    }
+    free(contrib);
+    free(beta);
+    free(n_eff);
+    return 0;
+}

@@ -52,6 +52,9 @@ def lib():
             f.argtypes = [ctypes.c_void_p, i64, d]
             f.restype = i64
         L.oracle_line_absorption.restype = ctypes.c_int
+        L.oracle_voigt_hjerting_grad.argtypes = [d, d, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.oracle_voigt_hjerting_grad.restype = d
+        L.oracle_line_absorption_vjp.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -136,3 +139,42 @@ def line_absorption(alpha, problem, nthreads=0, mode=DETERMINISTIC, return_windo
     if return_windows:
         return lb, ub, nev.value
     return None
+
+
+def voigt_hjerting_grad(a, v):
+    """(H, dH/da, dH/dv) of the branch of voigt_hjerting chosen at (a, v)."""
+    da, dv = ctypes.c_double(), ctypes.c_double()
+    h = lib().oracle_voigt_hjerting_grad(a, v, ctypes.byref(da), ctypes.byref(dv))
+    return h, da.value, dv.value
+
+
+GRAD_NAMES = ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2")
+
+
+def line_absorption_vjp(G, problem, nthreads=0):
+    """dL/d(per-layer inputs) given G = dL/dalpha (float64, shape (L, n_wl), Fortran order).
+
+    Returns a dict: ``T``, ``n_e``, ``xi``, ``n_HI``, ``n_HeI``, ``n_H2`` (length L each) and
+    ``n_div_U`` (n_species x L), the derivative with respect to n_s / U_s of each layer.
+    """
+    P = problem
+    G = np.asfortranarray(G, dtype=np.float64)
+    L, nwl = G.shape
+    assert (L, nwl) == (P.n_layers, P.n_wl)
+    n = len(P.lines)
+    S = len(P.species_mass)
+    out = {k: np.zeros(L) for k in GRAD_NAMES}
+    out["n_div_U"] = np.zeros((S, L))
+    ll = P.lines
+    rc = lib().oracle_line_absorption_vjp(
+        _p(G), ctypes.c_int64(L), ctypes.c_int64(nwl), _p(P.all_wls), ctypes.c_int64(n),
+        _p(ll.wl), _p(ll.log_gf), _p(ll.E_lower), _p(ll.gamma_rad), _p(ll.gamma_stark),
+        _p(ll.vdW1), _p(ll.vdW2), _p(ll.species_index), _p(P.species_mass), _p(P.species_is_mol),
+        ctypes.c_int64(S), _p(P.n_div_U), _p(P.temps), _p(P.n_e), _p(P.n_HI), _p(P.n_HeI),
+        _p(P.n_H2), _p(P.xi), ctypes.c_int(P.cntm_mode), _p(P.cntm_values),
+        ctypes.c_int64(P.n_knots), _p(P.cntm_knots), ctypes.c_double(P.cutoff_threshold),
+        ctypes.c_int(nthreads), *[_p(out[k]) for k in GRAD_NAMES], _p(out["n_div_U"]))
+    if rc != 0:
+        raise RuntimeError(f"oracle_line_absorption_vjp returned {rc}")
+    return out
+

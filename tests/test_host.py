@@ -155,18 +155,36 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 3
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 4
 
 
-def test_struct_layouts_match_header():
-    """ctypes mirrors of the ABI structs have the C sizes (x86-64 LP64)."""
-    assert ctypes.sizeof(_lib.KorgLines) == 8 * 9
-    assert ctypes.sizeof(_lib.KorgSpecies) == 24
-    assert ctypes.sizeof(_lib.KorgLayers) == 8 * 8
-    assert ctypes.sizeof(_lib.KorgGrid) == 16
-    assert ctypes.sizeof(_lib.KorgContinuum) == 32
-    assert ctypes.sizeof(_lib.KorgStats) == 56 + 32 + 24
-    assert ctypes.sizeof(_lib.KorgTuning) == 16
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of an ABI struct has the size and field offsets that the C compiler
+    gives the struct declared in include/korg_line.h."""
+    import subprocess
+    structs = {"korg_lines": _lib.KorgLines, "korg_species": _lib.KorgSpecies,
+               "korg_layers": _lib.KorgLayers, "korg_grid": _lib.KorgGrid,
+               "korg_continuum": _lib.KorgContinuum, "korg_stats": _lib.KorgStats,
+               "korg_tuning": _lib.KorgTuning, "korg_layer_grads": _lib.KorgLayerGrads}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "korg_line.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.check_call(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)])
+    got = {}
+    for row in subprocess.check_output([str(exe)], text=True).splitlines():
+        cname, what, val = row.split()
+        got[(cname, what)] = int(val)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
 
 
 def test_device_path_fails_loudly_without_gpu():
