@@ -125,17 +125,20 @@ class Atmosphere:
         return cls(m["T"], m["Pe"], m["Pg"], m["lgP_HI"], m["lgP_H2"], m["A_X"], m["lgTau5"])
 
     @classmethod
-    def scaled(cls, Teff, logg, M_H, base="sun"):
+    def scaled(cls, Teff, logg, M_H, base="sun", abundances=None):
         """Synthetic star: the solar model's structure scaled to (Teff, log g, [M/H]).
 
-        T scales with Teff/5777, pressures with sqrt(g/g_sun), metal abundances by 10^[M/H].  This
-        is a stand-in for ``interpolate_marcs``, whose atmosphere grids are not available offline.
+        T scales with Teff/5777, pressures with sqrt(g/g_sun), metal abundances by 10^[M/H], and
+        ``abundances`` ({element symbol: [X/H] offset}) moves single elements further.  This is a
+        stand-in for ``interpolate_marcs``, whose atmosphere grids are not available offline.
         """
         m = _read_model(base)
         tscale = Teff / 5777.0
         pscale = math.sqrt(10.0 ** (logg - 4.44))
         A = m["A_X"].copy()
         A[2:] += M_H
+        for el, dx in (abundances or {}).items():
+            A[atomic_numbers[el] - 1] += dx
         return cls(m["T"] * tscale, m["Pe"] * pscale * tscale ** 6, m["Pg"] * pscale,
                    m["lgP_HI"] + math.log10(pscale), m["lgP_H2"] + math.log10(pscale), A,
                    m["lgTau5"])
