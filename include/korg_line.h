@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 4
+#define KORG_LINE_ABI_VERSION 5
 
 enum korg_status {
     KORG_OK = 0,
@@ -162,6 +162,31 @@ int korg_line_absorption_device(korg_ctx* ctx, void* alpha, int alpha_dtype,
                                 const korg_species* species, const korg_layers* layers,
                                 const korg_continuum* cntm, double cutoff_threshold,
                                 int precision, int synchronize);
+
+/*
+ * One star -- one line_absorption! call -- of a batch (device pointers throughout).  Its line
+ * list is sorted by wavelength and indexes the batch's shared species table; every star has the
+ * same number of layers.
+ */
+typedef struct {
+    void* alpha;              /* column-major [n_layers x grid.n_wl], the batch's alpha dtype */
+    korg_grid grid;
+    korg_lines lines;
+    korg_layers layers;
+    korg_continuum cntm;
+    double cutoff_threshold;
+} korg_star;
+
+/*
+ * Many line_absorption! calls in one pipeline (DEVICE memory): all stars' lines are set up in one
+ * launch, binned onto the stars' wavelength tiles, and every profile kernel is a single persistent
+ * launch over a (star, wavelength tile) work queue.  Each star's alpha receives exactly what
+ * korg_line_absorption_device would add for it alone (bitwise).  `stars` is a host array of
+ * n_stars structs.
+ */
+int korg_line_absorption_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                                 const korg_species* species, int alpha_dtype, int precision,
+                                 int synchronize);
 
 /* Use an existing HIP stream (hipStream_t passed as void*) instead of the context's own. */
 int korg_set_stream(korg_ctx* ctx, void* stream);

@@ -187,62 +187,95 @@ struct GridInfo {
     double k0, kinv;  // the same for the continuum knots
 };
 
-__global__ void k_layers(int L, const double* __restrict__ temps, const double* __restrict__ n_HI,
-                         const double* __restrict__ n_HeI, const double* __restrict__ n_H2,
-                         LayerTab* __restrict__ tab, unsigned long long* __restrict__ counters,
-                         const double* __restrict__ wls, int64_t n_wl,
-                         const double* __restrict__ knots, int64_t n_knots,
-                         GridInfo* __restrict__ gi) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l == 0) {
-        for (int k = 0; k < N_COUNTERS; ++k) counters[k] = 0;
-        GridInfo g;
-        g.g0 = wls[0];
-        g.ginv = n_wl > 1 ? (double)(n_wl - 1) / (wls[n_wl - 1] - wls[0]) : 0.0;
-        g.k0 = n_knots > 1 ? knots[0] : 0.0;
-        g.kinv = n_knots > 1 ? (double)(n_knots - 1) / (knots[n_knots - 1] - knots[0]) : 0.0;
-        *gi = g;
+// One star -- one line_absorption! call -- of a batch.  Grid indices in every kernel are global:
+// the star's points occupy [grid_off, grid_off + n_wl) of the concatenation of all stars' grids,
+// each padded to a multiple of STAR_ALIGN points so that no tile of any kernel straddles two
+// stars; its lines are [line_off, line_off + n_lines) of the concatenated line lists.
+constexpr int STAR_ALIGN = 128;
+struct StarDev {
+    const double* wls;
+    int64_t n_wl, grid_off;
+    int64_t line_off, n_lines;
+    void* alpha;  // column-major [L x n_wl]
+    const double *wl, *log_gf, *E_lower, *gamma_rad, *gamma_stark, *vdW1, *vdW2;
+    const int32_t* species;
+    const double *temps, *n_e, *n_div_U, *n_HI, *n_HeI, *n_H2, *xi;
+    int32_t cntm_mode, pad0;
+    int64_t n_knots;
+    const double *knots, *cntm;
+    double cutoff;
+    GridInfo gi;  // written by k_layers
+};
+
+// The star holding global grid point j (the last star with grid_off <= j).
+__device__ inline int star_of_point(const StarDev* stars, int n, int64_t j) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (stars[mid].grid_off <= j) lo = mid; else hi = mid;
     }
-    if (l >= L) return;
+    return lo;
+}
+
+// The star holding global line i.
+__device__ inline int star_of_line(const StarDev* stars, int n, int64_t i) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (stars[mid].line_off <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Per-layer tables of every star (block = star); block 0 also clears the call's counters, and
+// each star's grid summary is stored in its StarDev.
+__global__ void k_layers(int n_stars, int L, StarDev* __restrict__ stars, LayerTab* __restrict__ tab,
+                         unsigned long long* __restrict__ counters) {
+    const int st = blockIdx.x;
+    StarDev& S = stars[st];
+    if (st == 0 && threadIdx.x == 0)
+        for (int k = 0; k < N_COUNTERS; ++k) counters[k] = 0;
+    if (threadIdx.x == 0) {
+        const int64_t n_knots = S.cntm_mode == KORG_CNTM_GRID ? S.n_knots : 0;
+        GridInfo g;
+        g.g0 = S.wls[0];
+        g.ginv = S.n_wl > 1 ? (double)(S.n_wl - 1) / (S.wls[S.n_wl - 1] - S.wls[0]) : 0.0;
+        g.k0 = n_knots > 1 ? S.knots[0] : 0.0;
+        g.kinv = n_knots > 1 ? (double)(n_knots - 1) / (S.knots[n_knots - 1] - S.knots[0]) : 0.0;
+        S.gi = g;
+    }
     const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
     const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;
     const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
     const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
-    const double T = temps[l];
-    LayerTab t;
-    t.beta = 1.0 / (kboltz_eV * T);
-    t.n_eff = n_HI[l] + c_He * n_HeI[l] + c_H2 * n_H2[l];
-    t.t16 = pow(T / 10000.0, 1.0 / 6.0);
-    t.t03 = pow(T / 10000.0, 0.3);
-    tab[l] = t;
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        const double T = S.temps[l];
+        LayerTab t;
+        t.beta = 1.0 / (kboltz_eV * T);
+        t.n_eff = S.n_HI[l] + c_He * S.n_HeI[l] + c_H2 * S.n_H2[l];
+        t.t16 = pow(T / 10000.0, 1.0 / 6.0);
+        t.t03 = pow(T / 10000.0, 0.3);
+        tab[(int64_t)st * L + l] = t;
+    }
 }
 
 // --------------------------------------------------------------------------------------------
 // k_setup: one wavefront per line; lane l handles layers l, l+64, ...
 // --------------------------------------------------------------------------------------------
 struct SetupArgs {
-    int64_t n_lines, n_wl;
+    int64_t n_lines;  // all stars
     int L;
     int Ls;          // rec64 stride per line (L padded to a whole number of Float64 groups)
     int n_lblocks;   // 64-layer blocks of the Float32 records
-    const double* wls;
-    const double *wl, *log_gf, *E_lower, *gamma_rad, *gamma_stark, *vdW1, *vdW2;
-    const int32_t* species;
+    int n_stars;
+    const StarDev* stars;
     int n_species;
     const double* mass;
     const uint8_t* is_mol;
-    const double* n_div_U;
-    const double *temps, *n_e, *xi;
-    const LayerTab* tab;
-    const GridInfo* gi;
-    int cntm_mode;
-    int64_t n_knots;
-    const double *knots, *cntm;
-    double cutoff;
+    const LayerTab* tab;  // [star * L + l]
     int want_f32;
     int count_branches;
     LineMeta* meta;
-    int* jc_all;
     double4* rec64;  // [line * Ls + l]: (1/(sigma sqrt 2), a, scaling, 0)
     float4* recM;    // [(line * n_lblocks + b) * 64 + lane]: far-wing coefficients, MFMA A layout
     unsigned long long* counters;
@@ -263,34 +296,36 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (i >= A.n_lines) return;  // whole wavefront
     const int L = A.L;
-    const double wl = A.wl[i];
-    const int32_t s = A.species[i];
+    const int st = star_of_line(A.stars, A.n_stars, i);
+    const StarDev& S = A.stars[st];
+    const int64_t li = i - S.line_off;  // index in the star's own line list
+    const double wl = S.wl[li];
+    const int32_t s = S.species[li];
     if (s < 0 || s >= A.n_species) {
         // invalid species index: flag the call as failed (nothing is added to alpha)
         if (lane == 0) {
             atomicOr(&A.counters[3], 2ull);
             LineMeta mt = {wl, 1, 0, 1, 0, 0, N_CLASSES, 1, 0};
             A.meta[i] = mt;
-            A.jc_all[i] = 0;
         }
         return;
     }
-    if (lane == 0 && i > 0 && A.wl[i - 1] > wl) atomicOr(&A.counters[3], 1ull);
-    const GridInfo gi = *A.gi;
+    if (lane == 0 && li > 0 && S.wl[li - 1] > wl) atomicOr(&A.counters[3], 1ull);
+    const GridInfo gi = S.gi;
     const double m = A.mass[s];
     const bool mol = A.is_mol[s] != 0;
-    const double gamma_rad = A.gamma_rad[i], gamma_stark = A.gamma_stark[i];
-    const double vdW1 = A.vdW1[i], vdW2 = A.vdW2[i];
-    const double E_lower = A.E_lower[i];
+    const double gamma_rad = S.gamma_rad[li], gamma_stark = S.gamma_stark[li];
+    const double vdW1 = S.vdW1[li], vdW2 = S.vdW2[li];
+    const double E_lower = S.E_lower[li];
     const double E_upper = E_lower + c_cgs * hplanck_eV / wl;  // src/line_absorption.jl:101
-    const double gf = pow(10.0, A.log_gf[i]);
+    const double gf = pow(10.0, S.log_gf[li]);
     const double sl = sigma_line(wl);
     int64_t kidx = 0;
-    if (A.cntm_mode == KORG_CNTM_GRID) {
+    if (S.cntm_mode == KORG_CNTM_GRID) {
         // knot interval holding wl: clamp(searchsortedlast(knots, wl), 1, n_knots - 1) - 1
-        int64_t ub = wave_bounds(A.knots, A.n_knots, gi.k0, gi.kinv, wl, true, lane == 0, lane);
+        int64_t ub = wave_bounds(S.knots, S.n_knots, gi.k0, gi.kinv, wl, true, lane == 0, lane);
         kidx = __builtin_amdgcn_readfirstlane((int)ub) - 1;
-        kidx = kidx < 0 ? 0 : (kidx > A.n_knots - 2 ? A.n_knots - 2 : kidx);
+        kidx = kidx < 0 ? 0 : (kidx > S.n_knots - 2 ? S.n_knots - 2 : kidx);
     }
     double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0;
     bool any_nan = false;
@@ -300,24 +335,24 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     for (int l = lane; l < Lm; l += WAVE) {
         double c[4] = {0.0, 0.0, 0.0, 0.0};
         if (l < L) {
-            const LayerTab tb = A.tab[l];
-            const double T = A.temps[l];
+            const LayerTab tb = A.tab[(int64_t)st * L + l];
+            const double T = S.temps[l];
             // src/line_absorption.jl:86-106
-            const double sigma = doppler_width(wl, T, m, A.xi[l]);
+            const double sigma = doppler_width(wl, T, m, S.xi[l]);
             double G = gamma_rad;
             if (!mol) {
-                G = G + A.n_e[l] * (gamma_stark * tb.t16);
+                G = G + S.n_e[l] * (gamma_stark * tb.t16);
                 const double vdw = (vdW2 == -1.0) ? vdW1 * tb.t03 : scaled_vdW_abo(vdW1, vdW2, m, T);
                 G = G + tb.n_eff * vdw;
             }
             const double gamma = G * (wl * wl) / (c_cgs * (4.0 * PI));
             const double levels = exp(-tb.beta * E_lower) - exp(-tb.beta * E_upper);
-            const double amp = gf * sl * levels * A.n_div_U[(int64_t)s * L + l];
+            const double amp = gf * sl * levels * S.n_div_U[(int64_t)s * L + l];
             // src/line_absorption.jl:108-112
-            const double cn = (A.cntm_mode == KORG_CNTM_GRID)
-                                  ? cntm_lerp(A.knots, A.cntm + (int64_t)l * A.n_knots, kidx, wl)
-                                  : A.cntm[i * L + l];
-            const double rho = cn * A.cutoff / amp;
+            const double cn = (S.cntm_mode == KORG_CNTM_GRID)
+                                  ? cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, kidx, wl)
+                                  : S.cntm[li * L + l];
+            const double rho = cn * S.cutoff / amp;
             const double dg = inverse_gaussian_density(rho, sigma);
             const double dl = inverse_lorentz_density(rho, gamma);
             any_nan |= (dg != dg) || (dl != dl);
@@ -397,8 +432,8 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double reach = wave_max(rmax);
     const double treach = wave_max(tmax);
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
-    const int64_t n = A.n_wl;
-    const double last = A.wls[n - 1];
+    const int64_t n = S.n_wl;
+    const double last = S.wls[n - 1];
     // lanes 0..4 search for lb, ub, jc, near-zone start and end in parallel
     double x = wl;
     bool upper = false;
@@ -412,7 +447,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     if (lane == 6) { x = wl + ht; upper = true; }
     if (lane <= 1 && x >= 1.0) x = x * 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
     const bool past = !(x <= last);           // beyond the grid (or NaN)
-    const int64_t r = wave_bounds(A.wls, n, gi.g0, gi.ginv, x, upper, lane < 7 && !past, lane);
+    const int64_t r = wave_bounds(S.wls, n, gi.g0, gi.ginv, x, upper, lane < 7 && !past, lane);
     int64_t res = r;
     if (lane < 7 && past) res = n;  // lower bound n; upper bound n (last index n - 1)
     const int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
@@ -437,9 +472,9 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         int64_t core_n = 0;
         if (my_hw >= 0.0) {
             int64_t c0 = 0, c1 = 0;
-            if (!lane_bound_guess(A.wls, n, gi.g0, gi.ginv, wl - my_hw, true, c0))
+            if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, wl - my_hw, true, c0))
                 c0 = lb;  // counting only: fall back to the window edge
-            if (!lane_bound_guess(A.wls, n, gi.g0, gi.ginv, wl + my_hw, false, c1))
+            if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, wl + my_hw, false, c1))
                 c1 = ub + 1;
             c1 -= 1;
             c0 = c0 < lb ? lb : c0;
@@ -458,18 +493,19 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         while (cls < N_CLASSES - 1 && W > class_wmax(cls)) ++cls;
     }
     if (lane == 0) {
+        // grid indices are stored global (offset by the star's place in the concatenated grids)
+        const int64_t go = S.grid_off;
         LineMeta mt;
         mt.lam0 = wl;
-        mt.lb = (int32_t)lb;
-        mt.ub = (int32_t)ub;
-        mt.clo = (int32_t)clo;
-        mt.chi = (int32_t)chi;
-        mt.jc = (int32_t)jc;
+        mt.lb = (int32_t)(lb + go);
+        mt.ub = (int32_t)(ub + go);
+        mt.clo = (int32_t)(clo + go);
+        mt.chi = (int32_t)(chi + go);
+        mt.jc = (int32_t)(jc + go);
         mt.cls = cls;
-        mt.flo = (int32_t)flo;
-        mt.fhi = (int32_t)fhi;
+        mt.flo = (int32_t)(flo + go);
+        mt.fhi = (int32_t)(fhi + go);
         A.meta[i] = mt;
-        A.jc_all[i] = (int32_t)jc;
     }
 }
 
@@ -728,7 +764,8 @@ struct ProfileArgs {
     int n_lblocks;   // layer blocks of a work item (F64_GROUP layers, or 64 for the MFMA kernel)
     int ntiles, tile_w;
     int64_t n_items;
-    const double* wls;
+    const StarDev* stars;  // grid values and alpha of each star
+    int n_stars;
     const double4* rec64;
     const float4* recM;
     const Cand* cands;   // lines in window-class partition order
@@ -736,7 +773,6 @@ struct ProfileArgs {
     const LineMeta* meta;
     const int64_t* tile_start;  // [tile * N_CLASSES]: first entry of the tile's list (far wing)
     const int* tile_lines;      // per tile, the lines whose window meets it (far wing)
-    void* alpha;
     int alpha_f32;
     unsigned long long* counters;
 };
@@ -865,13 +901,16 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
         const int tile = item / A.n_lblocks;
         const int l0 = (item % A.n_lblocks) * G;
         const int64_t j0 = (int64_t)tile * A.tile_w;
-        const int64_t j1 = min(j0 + A.tile_w, A.n_wl) - 1;
+        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;  // the star's last point at most
+        if (j1 < j0) continue;  // padding between two stars' grids: no points
         double lam[P];
         int jj[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             jj[p] = (int)(j0 + p * WAVE + lane);
-            lam[p] = A.wls[min((int64_t)jj[p], j1)];  // points past j1 are never used
+            lam[p] = S.wls[min((int64_t)jj[p], j1) - go];  // points past j1 are never used
         }
         double acc[P][G];
 #pragma unroll
@@ -906,7 +945,8 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
             if (jj[p] > j1) continue;
 #pragma unroll
             for (int l = 0; l < G; ++l)
-                if (l0 + l < L) add_alpha(A.alpha, A.alpha_f32, (int64_t)(l0 + l) + (int64_t)L * jj[p], acc[p][l]);
+                if (l0 + l < L)
+                    add_alpha(S.alpha, A.alpha_f32, (int64_t)(l0 + l) + (int64_t)L * (jj[p] - go), acc[p][l]);
         }
     }
 }
@@ -956,13 +996,17 @@ __global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(
         const int tile = item / A.n_lblocks;
         const int lblk = item % A.n_lblocks;
         const int64_t j0 = (int64_t)tile * A.tile_w;
-        const int64_t j1 = min(j0 + A.tile_w, A.n_wl) - 1;
+        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        // the star's last point at most; below j0 for padding between two stars' grids (such a
+        // tile has no lines and writes nothing)
+        const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;
         double lam[NB];
         int jj[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             jj[b] = (int)(j0 + 32 * b + n);
-            lam[b] = A.wls[min((int64_t)jj[b], j1)];  // points past j1 are never used
+            lam[b] = S.wls[min((int64_t)jj[b], j1) - go];  // points past j1 are never used
         }
         floatx16 acc[2][NB];
 #pragma unroll
@@ -1100,7 +1144,7 @@ __global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(
         if (w == 0) {
             // accumulator layout of v_mfma_f32_32x32x2_f32: lane i, register r holds
             // row 8*(r/4) + 4*(i/32) + r%4, column i%32
-            AlphaT* __restrict__ alpha = (AlphaT*)A.alpha;
+            AlphaT* __restrict__ alpha = (AlphaT*)S.alpha;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 if (jj[b] > j1) continue;
@@ -1123,7 +1167,7 @@ __global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(
                         for (int e = 0; e < 4; ++e) {
                             const int r = 4 * q + e;
                             const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
-                            if (layer < L) alpha[(int64_t)layer + (int64_t)L * jj[b]] += (AlphaT)vr[e];
+                            if (layer < L) alpha[(int64_t)layer + (int64_t)L * (jj[b] - go)] += (AlphaT)vr[e];
                         }
                     }
             }
@@ -1145,11 +1189,11 @@ struct NearArgs {
     int L, Ls;
     int n_lblocks, ntiles;
     int64_t n_items;
-    const double* wls;
+    const StarDev* stars;
+    int n_stars;
     const Cand* cands;      // lines in near-zone reach-class partition order
     const int2* ranges;     // per (class, CT-point tile): candidate index range in cands
     const double4* rec64;
-    void* alpha;
     int alpha_f32;
     unsigned long long* counters;
 };
@@ -1180,9 +1224,12 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
         const int l = (item % A.n_lblocks) * WAVE + lane;
         const bool lv = l < L;
         const int64_t j0 = (int64_t)tile * CT;
-        const int64_t j1 = min(j0 + CT, A.n_wl) - 1;
+        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        const int64_t j1 = min(j0 + CT, go + S.n_wl) - 1;
+        if (j1 < j0) continue;  // padding between two stars' grids: no points
         double lam[CT];  // the tile's wavelengths (wavefront-uniform)
-        const KCONST double* wt = uniform_ptr(A.wls + j0);
+        const KCONST double* wt = uniform_ptr(S.wls + (j0 - go));
 #pragma unroll
         for (int t = 0; t < CT; ++t) lam[t] = wt[min((int64_t)t, j1 - j0)];  // in bounds
         float acc[CT];
@@ -1216,7 +1263,7 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
                 const int64_t j = j0 + t;
-                if (j <= j1) add_alpha(A.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * j, acc[t]);
+                if (j <= j1) add_alpha(S.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * (j - go), acc[t]);
             }
         }
     }
@@ -1391,10 +1438,12 @@ struct korg_ctx {
     int last_precision = KORG_PRECISION_FP64;
     int last_n_lblocks = 0;
     // device scratch (grow-only)
-    DevBuf meta, jc_all, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
+    DevBuf meta, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
         nblk, ncstart, near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line,
         tc_count, tc_start, tile_lines, vjp_cnt, vjp_start, vjp_contrib, vjp_species, vjp_out,
-        h_grad;
+        h_grad, stars;
+    std::vector<StarDev> stars_host;  // the star table of the last call (host copy)
+    int last_n_stars = 0;
     // staging for the host-memory entry point
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -1473,84 +1522,120 @@ int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg
     return KORG_OK;
 }
 
-// Per-layer tables (k_layers) and the per-line setup (k_setup): windows, zones, the Float64
-// profile records and, for the Float32 form, the MFMA coefficient blocks.  Records ev[0], ev[1].
-int launch_setup(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg_species* sp,
-                 const korg_layers* ly, const korg_continuum* cn, double cutoff, bool f32) {
-    const int64_t N = ln->n_lines;
-    const int L = ly->n_layers;
-    const int64_t nwl = g->n_wl;
+// One star of a call, as the host sees it: device pointers.
+struct StarHost {
+    void* alpha;
+    korg_grid grid;
+    korg_lines lines;
+    korg_layers layers;
+    korg_continuum cntm;
+    double cutoff;
+};
+
+// Lays the stars out in the global index spaces (grids padded to STAR_ALIGN points, lines
+// concatenated) and uploads the star table.  Returns the totals.
+int upload_stars(korg_ctx* ctx, int n_stars, const StarHost* st, int64_t* n_lines_total,
+                 int64_t* n_wl_total) {
+    auto& h = ctx->stars_host;
+    h.assign(n_stars, StarDev{});
+    int64_t go = 0, lo = 0;
+    for (int k = 0; k < n_stars; ++k) {
+        const StarHost& x = st[k];
+        StarDev& d = h[k];
+        d.wls = x.grid.all_wls;
+        d.n_wl = x.grid.n_wl;
+        d.grid_off = go;
+        d.line_off = lo;
+        d.n_lines = x.lines.n_lines;
+        d.alpha = x.alpha;
+        d.wl = x.lines.wl;
+        d.log_gf = x.lines.log_gf;
+        d.E_lower = x.lines.E_lower;
+        d.gamma_rad = x.lines.gamma_rad;
+        d.gamma_stark = x.lines.gamma_stark;
+        d.vdW1 = x.lines.vdW1;
+        d.vdW2 = x.lines.vdW2;
+        d.species = x.lines.species;
+        d.temps = x.layers.temps;
+        d.n_e = x.layers.n_e;
+        d.n_div_U = x.layers.n_div_U;
+        d.n_HI = x.layers.n_HI;
+        d.n_HeI = x.layers.n_HeI;
+        d.n_H2 = x.layers.n_H2;
+        d.xi = x.layers.xi;
+        d.cntm_mode = x.cntm.mode;
+        d.n_knots = x.cntm.n_knots;
+        d.knots = x.cntm.knots;
+        d.cntm = x.cntm.values;
+        d.cutoff = x.cutoff;
+        go += (x.grid.n_wl + STAR_ALIGN - 1) / STAR_ALIGN * STAR_ALIGN;
+        lo += x.lines.n_lines;
+    }
+    if (go > (int64_t)INT32_MAX || lo > (int64_t)INT32_MAX)
+        return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                       "more than 2^31 grid points or lines in one call; split the batch");
+    TRY(grow(ctx, ctx->stars, sizeof(StarDev) * (size_t)n_stars));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->stars.p, h.data(), sizeof(StarDev) * (size_t)n_stars,
+                                hipMemcpyHostToDevice, ctx->stream));
+    ctx->last_n_stars = n_stars;
+    *n_lines_total = lo;
+    *n_wl_total = go;
+    return KORG_OK;
+}
+
+// Per-layer tables (k_layers) and the per-line setup (k_setup) of the uploaded stars: windows,
+// zones, the Float64 profile records and, for the Float32 form, the MFMA coefficient blocks.
+// Records ev[0], ev[1].
+int launch_setup(korg_ctx* ctx, int n_stars, int L, int64_t N, const korg_species* sp, bool f32) {
     hipStream_t s = ctx->stream;
     const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
     const int Ls = n_groups * F64_GROUP;
     const int n_lblocks = (L + WAVE - 1) / WAVE;
     TRY(grow(ctx, ctx->meta, sizeof(LineMeta) * N));
-    TRY(grow(ctx, ctx->jc_all, sizeof(int) * N));
     TRY(grow(ctx, ctx->rec64, sizeof(double4) * N * Ls));
     if (f32) TRY(grow(ctx, ctx->recM, sizeof(float4) * (size_t)N * n_lblocks * WAVE));
-    TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * L));
+    TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * (size_t)n_stars * L));
     TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * N_COUNTERS));
-    TRY(grow(ctx, ctx->gridinfo, sizeof(GridInfo)));
     auto* counters = (unsigned long long*)ctx->counters.p;
-    auto* gi = (GridInfo*)ctx->gridinfo.p;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
-    hipLaunchKernelGGL(k_layers, dim3((L + 63) / 64), dim3(64), 0, s, L, ly->temps, ly->n_HI,
-                       ly->n_HeI, ly->n_H2, (LayerTab*)ctx->tab.p, counters, g->all_wls, nwl,
-                       cn->knots, cn->mode == KORG_CNTM_GRID ? cn->n_knots : (int64_t)0, gi);
+    hipLaunchKernelGGL(k_layers, dim3(n_stars), dim3(64), 0, s, n_stars, L, (StarDev*)ctx->stars.p,
+                       (LayerTab*)ctx->tab.p, counters);
     SetupArgs sa;
     sa.n_lines = N;
-    sa.n_wl = nwl;
     sa.L = L;
     sa.Ls = Ls;
     sa.n_lblocks = n_lblocks;
-    sa.wls = g->all_wls;
-    sa.wl = ln->wl;
-    sa.log_gf = ln->log_gf;
-    sa.E_lower = ln->E_lower;
-    sa.gamma_rad = ln->gamma_rad;
-    sa.gamma_stark = ln->gamma_stark;
-    sa.vdW1 = ln->vdW1;
-    sa.vdW2 = ln->vdW2;
-    sa.species = ln->species;
+    sa.n_stars = n_stars;
+    sa.stars = (const StarDev*)ctx->stars.p;
     sa.n_species = sp->n_species;
     sa.mass = sp->mass;
     sa.is_mol = sp->is_molecule;
-    sa.n_div_U = ly->n_div_U;
-    sa.temps = ly->temps;
-    sa.n_e = ly->n_e;
-    sa.xi = ly->xi;
     sa.tab = (const LayerTab*)ctx->tab.p;
-    sa.gi = gi;
-    sa.cntm_mode = cn->mode;
-    sa.n_knots = cn->n_knots;
-    sa.knots = cn->knots;
-    sa.cntm = cn->values;
-    sa.cutoff = cutoff;
     sa.want_f32 = f32 ? 1 : 0;
     sa.count_branches = ctx->tuning.count_branches ? 1 : 0;
     sa.meta = (LineMeta*)ctx->meta.p;
-    sa.jc_all = (int*)ctx->jc_all.p;
     sa.rec64 = (double4*)ctx->rec64.p;
     sa.recM = (float4*)ctx->recM.p;
     sa.counters = counters;
-    hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
+    if (N > 0)
+        hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
     return KORG_OK;
 }
 
-// The device pipeline; every pointer is a device pointer.
-int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
-               const korg_lines* ln, const korg_species* sp, const korg_layers* ly,
-               const korg_continuum* cn, double cutoff, int precision) {
-    const int64_t N = ln->n_lines;
-    const int L = ly->n_layers;
-    const int64_t nwl = g->n_wl;
+// The device pipeline over a batch of stars (every pointer a device pointer; one alpha dtype and
+// one layer count for the batch): setup, partition, far wing, near zone.
+int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_species* sp, int L,
+              int alpha_dtype, int precision) {
+    int64_t N = 0, nwl = 0;
+    for (int k = 0; k < n_stars; ++k) N += stars[k].lines.n_lines;
     ctx->last_n_lines = N;
     ctx->last_precision = precision;
     ctx->last_n_lblocks = (L + WAVE - 1) / WAVE;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
+    TRY(upload_stars(ctx, n_stars, stars, &N, &nwl));
     hipStream_t s = ctx->stream;
     const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
     const int P64 = ctx->tuning.points_per_lane == 2 ? 2 : 1;
@@ -1585,7 +1670,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     } else {
         TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
     }
-    TRY(launch_setup(ctx, g, ln, sp, ly, cn, cutoff, f32));
+    TRY(launch_setup(ctx, n_stars, L, N, sp, f32));
     auto* counters = (unsigned long long*)ctx->counters.p;
 
     hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L, 32 * MFMA_NB,
@@ -1645,7 +1730,8 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     pa.ntiles = ntiles;
     pa.tile_w = tile_w;
     pa.n_items = (int64_t)ntiles * pa.n_lblocks;
-    pa.wls = g->all_wls;
+    pa.stars = (const StarDev*)ctx->stars.p;
+    pa.n_stars = n_stars;
     pa.rec64 = (const double4*)ctx->rec64.p;
     pa.recM = (const float4*)ctx->recM.p;
     pa.cands = (const Cand*)ctx->cls_cand.p;
@@ -1653,7 +1739,6 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     pa.tile_start = (const int64_t*)ctx->tc_start.p;
     pa.tile_lines = (const int*)ctx->tile_lines.p;
     pa.ranges = (const int2*)ctx->ranges.p;
-    pa.alpha = alpha;
     pa.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
     pa.counters = counters;
     int64_t nwaves = (int64_t)ctx->n_cu * wpc;
@@ -1680,11 +1765,11 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
         na.n_lblocks = n_lblocks;
         na.ntiles = nt_near;
         na.n_items = (int64_t)na.ntiles * n_lblocks;
-        na.wls = g->all_wls;
+        na.stars = (const StarDev*)ctx->stars.p;
+        na.n_stars = n_stars;
         na.cands = (const Cand*)ctx->near_cand.p;
         na.ranges = (const int2*)ctx->near_ranges.p;
         na.rec64 = (const double4*)ctx->rec64.p;
-        na.alpha = alpha;
         na.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
         na.counters = counters;
         int64_t cwaves = (int64_t)ctx->n_cu * 16;
@@ -1694,6 +1779,14 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     return KORG_OK;
+}
+
+// One line_absorption! call on device pointers: a batch of one star.
+int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
+               const korg_lines* ln, const korg_species* sp, const korg_layers* ly,
+               const korg_continuum* cn, double cutoff, int precision) {
+    const StarHost one = {alpha, *g, *ln, *ly, *cn, cutoff};
+    return run_batch(ctx, 1, &one, sp, ly->n_layers, alpha_dtype, precision);
 }
 
 // The adjoint pipeline on device pointers: setup (windows and Float64 records), one item per
@@ -1718,7 +1811,10 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
         const int Ls = n_groups * F64_GROUP;
         TRY(grow(ctx, ctx->vjp_cnt, sizeof(int) * N));
         TRY(grow(ctx, ctx->vjp_start, sizeof(int64_t) * (N + 1)));
-        TRY(launch_setup(ctx, g, ln, sp, ly, cn, cutoff, false));
+        const StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
+        int64_t n_lines = 0, n_wl = 0;
+        TRY(upload_stars(ctx, 1, &one, &n_lines, &n_wl));
+        TRY(launch_setup(ctx, 1, L, N, sp, false));
         hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N,
                            (const LineMeta*)ctx->meta.p, (int*)ctx->vjp_cnt.p);
         hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, N, (const int*)ctx->vjp_cnt.p,
@@ -1936,11 +2032,12 @@ void korg_destroy(korg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    DevBuf* bufs[] = {&ctx->meta, &ctx->jc_all, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
+    DevBuf* bufs[] = {&ctx->meta, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
                       &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->cls_rng,
                       &ctx->cls_line, &ctx->near_rng, &ctx->near_line, &ctx->tc_count, &ctx->tc_start,
                       &ctx->tile_lines, &ctx->vjp_cnt, &ctx->vjp_start, &ctx->vjp_contrib,
-                      &ctx->vjp_species, &ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
+                      &ctx->vjp_species, &ctx->vjp_out, &ctx->h_grad, &ctx->stars, &ctx->h_alpha,
+                      &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
                       &ctx->h_cntm};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -2002,6 +2099,40 @@ int korg_line_absorption_device(korg_ctx* ctx, void* alpha, int alpha_dtype, con
     return KORG_OK;
 }
 
+int korg_line_absorption_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                                 const korg_species* species, int alpha_dtype, int precision,
+                                 int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    if (n_stars < 1 || !stars) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no stars");
+    if (n_stars > 0x7fffffff) return set_err(ctx, KORG_ERR_UNSUPPORTED, "too many stars");
+    std::vector<StarHost> hs((size_t)n_stars);
+    const int L = stars[0].layers.n_layers;
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_star& x = stars[k];
+        const int rc = validate(ctx, &x.grid, &x.lines, species, &x.layers, &x.cntm, alpha_dtype,
+                                precision);
+        if (rc != KORG_OK) {
+            ctx->err = "star " + std::to_string(k) + ": " + ctx->err;
+            return rc;
+        }
+        if (x.layers.n_layers != L)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld has %d layers, star 0 has %d: one layer count per batch",
+                           (long long)k, x.layers.n_layers, L);
+        if (!x.alpha && x.lines.n_lines > 0)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: alpha is null", (long long)k);
+        hs[k] = StarHost{x.alpha, x.grid, x.lines, x.layers, x.cntm, x.cutoff_threshold};
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_batch(ctx, (int)n_stars, hs.data(), species, L, alpha_dtype, precision));
+    if (synchronize) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_stats(ctx));
+    }
+    return KORG_OK;
+}
+
 int korg_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* grid,
                          const korg_lines* lines, const korg_species* species,
                          const korg_layers* layers, const korg_continuum* cntm,
@@ -2016,7 +2147,7 @@ int korg_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg
     }
     if (!alpha) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha is null");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int64_t N = lines->n_lines, L = layers->n_layers, S = species->n_species;
+    const int64_t L = layers->n_layers;
     const size_t esz = alpha_dtype == KORG_DTYPE_F32 ? 4 : 8;
     const size_t abytes = esz * (size_t)L * (size_t)grid->n_wl;
     korg_grid g = *grid;

@@ -66,17 +66,24 @@ class KorgLayerGrads(ctypes.Structure):
                                                "n_div_U")]
 
 
+class KorgStar(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("grid", KorgGrid), ("lines", KorgLines),
+                ("layers", KorgLayers), ("cntm", KorgContinuum),
+                ("cutoff_threshold", ctypes.c_double)]
+
+
 class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
 
 
-ABI_VERSION = 4  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 5  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
             "korg_synchronize", "korg_get_stats", "korg_get_windows", "korg_set_tuning",
-            "korg_line_absorption_vjp", "korg_line_absorption_vjp_device"]
+            "korg_line_absorption_vjp", "korg_line_absorption_vjp_device",
+            "korg_line_absorption_batched"]
 
 _lib = None
 
@@ -129,6 +136,9 @@ def load():
     L.korg_line_absorption_vjp.restype = i32
     L.korg_line_absorption_vjp_device.argtypes = vjp + [i32]
     L.korg_line_absorption_vjp_device.restype = i32
+    L.korg_line_absorption_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
+                                               ctypes.POINTER(KorgSpecies), i32, i32, i32]
+    L.korg_line_absorption_batched.restype = i32
     if L.korg_abi_version() != ABI_VERSION:
         raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
                           f"expect {ABI_VERSION}; rebuild it")

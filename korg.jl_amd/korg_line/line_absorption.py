@@ -167,6 +167,34 @@ class Context:
             1 if synchronize else 0)
         _lib.check(self._h, rc)
 
+    def line_absorption_batched(self, alphas, dprobs, precision="fp32", synchronize=True):
+        """Many stars in one pipeline: ``alphas[k]`` (a ``DeviceProblem.new_alpha`` array) receives
+        the line opacity of ``dprobs[k]``.  Every problem must share one species table and one
+        layer count; all alphas have one dtype."""
+        if len(alphas) != len(dprobs) or not dprobs:
+            raise ValueError("one alpha per problem, at least one star")
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        dtypes = {a.dtype for a in alphas}
+        if len(dtypes) != 1 or next(iter(dtypes)) not in dt:
+            raise TypeError("all alphas must share one dtype, float32 or float64")
+        P0 = dprobs[0].problem
+        for d, a in zip(dprobs, alphas):
+            P = d.problem
+            if not (np.array_equal(P.species_mass, P0.species_mass) and
+                    np.array_equal(P.species_is_mol, P0.species_is_mol)):
+                raise ValueError("every star must use the same species table")
+            if tuple(a.shape) != (P.n_wl, P.n_layers):
+                raise ValueError("alpha shape does not match its problem")
+        stars = (_lib.KorgStar * len(dprobs))()
+        for k, (d, a) in enumerate(zip(dprobs, alphas)):
+            stars[k] = _lib.KorgStar(a.ptr, d.grid, d.lines, d.layers, d.cntm,
+                                     d.problem.cutoff_threshold)
+        rc = self._L.korg_line_absorption_batched(self._h, len(dprobs), stars,
+                                                  ctypes.byref(dprobs[0].species),
+                                                  dt[next(iter(dtypes))], PRECISIONS[precision],
+                                                  1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
     def line_absorption_vjp(self, G, problem):
         """Adjoint on host arrays: given ``G`` = dL/dalpha (float64/float32, (n_layers, n_wl),
         Fortran-ordered), returns dL/dx for the per-layer inputs: a dict with ``T``, ``n_e``,

@@ -104,3 +104,36 @@ def solar_wide_subset(seed=2, n_lines=300_000, wl_range=(4000, 7000), sub_range=
     keep = np.nonzero((ll.wl >= lo) & (ll.wl <= hi))[0]
     wls = Wavelengths(sub_range)
     return make_problem(atm, ll.take(keep), wls, 1.0), atm, wls
+
+
+# stream ids of the batched workload's per-star draws
+_S_TEFF, _S_LOGG, _S_MH, _S_WIDTH, _S_START, _S_NLINES = 101, 102, 103, 104, 105, 106
+
+
+def batched_stars(n_stars=256, seed=3, scale=1.0, wl_range=(4000.0, 7000.0)):
+    """Many stars with uneven line counts and wavelength ranges.
+
+    Star k: Teff ~ U[4000, 6500] K, log g ~ U[1, 5], [M/H] ~ U[-2.5, 0.5] (a scaled solar
+    structure, ``Atmosphere.scaled``), a window of width ~ U[50, 1000] A * scale inside
+    ``wl_range`` at 0.01 A, and its own seeded list of ~ U[5,000, 150,000] * scale lines over the
+    window, drawn from the solar_small distribution (log gf ~ U[-4, 0.5], E_lower ~ U[0, 5] eV).
+    Every list shares one species table.  Returns a list of (Problem, Atmosphere, Wavelengths).
+    """
+    u = lambda stream: uniform(seed, stream, n_stars)
+    teff = 4000.0 + 2500.0 * u(_S_TEFF)
+    logg = 1.0 + 4.0 * u(_S_LOGG)
+    mh = -2.5 + 3.0 * u(_S_MH)
+    width = (50.0 + 950.0 * u(_S_WIDTH)) * scale
+    lo, hi = wl_range
+    start = lo + (hi - lo - width) * u(_S_START)
+    n_lines = np.maximum(1, np.round((5_000 + 145_000 * u(_S_NLINES)) * scale)).astype(int)
+    out = []
+    for k in range(n_stars):
+        atm = Atmosphere.scaled(float(teff[k]), float(logg[k]), float(mh[k]))
+        a = round(float(start[k]), 2)
+        b = round(a + float(width[k]), 2)
+        wls = Wavelengths((a, b))
+        ll = synthetic_linelist(seed * 1_000_003 + k, int(n_lines[k]), a, b, (-4.0, 0.5), (0.0, 5.0))
+        out.append((make_problem(atm, ll, wls, 1.0), atm, wls))
+    return out
+
