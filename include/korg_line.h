@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 5
+#define KORG_LINE_ABI_VERSION 6
 
 enum korg_status {
     KORG_OK = 0,
@@ -40,6 +40,8 @@ enum korg_status {
     KORG_ERR_OUT_OF_MEMORY = 3,
     KORG_ERR_NO_DEVICE = 4,
     KORG_ERR_UNSUPPORTED = 5,
+    /* a plan's per-tile line lists did not fit their buffer (alpha was left unchanged) */
+    KORG_ERR_CAPACITY = 6,
 };
 
 /* Arithmetic of the profile evaluation. */
@@ -181,8 +183,8 @@ typedef struct {
  * Many line_absorption! calls in one pipeline (DEVICE memory): all stars' lines are set up in one
  * launch, binned onto the stars' wavelength tiles, and every profile kernel is a single persistent
  * launch over a (star, wavelength tile) work queue.  Each star's alpha receives exactly what
- * korg_line_absorption_device would add for it alone (bitwise).  `stars` is a host array of
- * n_stars structs.
+ * korg_line_absorption_device would add for it alone under the same tuning (bitwise).  `stars`
+ * is a host array of n_stars structs.
  */
 int korg_line_absorption_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                                  const korg_species* species, int alpha_dtype, int precision,
@@ -199,14 +201,20 @@ int korg_get_stats(const korg_ctx* ctx, korg_stats* out);
  */
 int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines);
 
-/* Kernel tuning (0 = library default / autotuned choice). */
+/* Kernel tuning (0 = library default).  No counterpart in the reference, whose only knob is
+ * tasks_per_thread (src/line_absorption.jl:34); korg_autotune_batched chooses these. */
 typedef struct {
-    int32_t points_per_lane;   /* wavelength points per lane: 1 or 2 */
-    int32_t waves_per_tile;    /* reserved, 0 */
-    int32_t waves_per_cu;      /* persistent waves launched per compute unit */
+    int32_t points_per_lane;   /* Float64 form: wavelength points per lane, 1 or 2 (default 1) */
+    int32_t waves_per_tile;    /* Float32 far wing: waves sharing a tile's line list, 1, 2 or 4
+                                  (default 4; at most 2 with 128-point tiles) */
+    int32_t waves_per_cu;      /* persistent profile waves launched per compute unit (default 12) */
     int32_t count_branches;    /* 1: fill korg_stats.n_evals_branch (extra setup cost) */
+    int32_t tile_points;       /* Float32 far wing: wavelength points per tile, 32, 64 or 128
+                                  (default 64) */
+    int32_t near_waves_per_cu; /* Float32 near zone: persistent waves per compute unit (default 16) */
 } korg_tuning;
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
+int korg_get_tuning(const korg_ctx* ctx, korg_tuning* out);
 
 /*
  * Gradients of the adjoint: dL/dx for each per-layer input x of korg_layers, given
@@ -243,6 +251,56 @@ int korg_line_absorption_vjp_device(korg_ctx* ctx, const void* grad_alpha, int g
                                     const korg_species* species, const korg_layers* layers,
                                     const korg_continuum* cntm, double cutoff_threshold,
                                     korg_layer_grads* grads, int synchronize);
+
+/*
+ * Plans: a batched forward call (korg_line_absorption_batched) captured once in a HIP graph for
+ * fixed device pointers and replayed with no host round trip -- the form for a fit that
+ * re-evaluates the same stars with new inputs in the same buffers (Korg's Fit re-runs synthesize
+ * per iteration, src/Fit/fit_via_synthesis.jl:304).  No counterpart in the reference.
+ *
+ * korg_plan_batched validates the stars, shapes the pipeline under the context's current
+ * tuning, measures the per-tile line lists once on the current inputs, allocates them `headroom`
+ * times larger (0: 1.25; else in [1, 16]) and captures the graph.  The plan owns its device
+ * scratch; it keeps `ctx` (destroy plans before their context).  Every replay ADDS into the
+ * alphas, as the direct call does, and reads the arrays' contents at replay time.
+ *
+ * korg_plan_launch replays on the context's stream (synchronize = 0: returns at once; call
+ * korg_plan_wait before reading the results).  A replay whose inputs need more tile-list entries
+ * than allocated writes nothing; korg_plan_wait then grows the lists, captures the graph again
+ * and replays it, so a finished launch has always added the full result.  korg_get_stats gives
+ * the counts of the last finished replay (total_ms: the replay's stream time).
+ */
+typedef struct korg_plan korg_plan;
+typedef struct {
+    int64_t n_stars;
+    int64_t n_lines;        /* lines of all stars */
+    int64_t n_points;       /* grid points of all stars, each star padded to a multiple of 128 */
+    int64_t tile_capacity;  /* entries allocated for the per-tile line lists (Float32 form) */
+    int64_t tile_entries;   /* entries the last finished replay needed */
+    int32_t n_captures;     /* graph captures so far (1 + regrowths) */
+    int32_t pad;
+    korg_tuning tuning;     /* the tuning the plan was captured with */
+} korg_plan_info;
+int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                      const korg_species* species, int alpha_dtype, int precision,
+                      double headroom, korg_plan** out);
+int korg_plan_launch(korg_plan* plan, int synchronize);
+int korg_plan_wait(korg_plan* plan);
+int korg_plan_get_info(const korg_plan* plan, korg_plan_info* out);
+void korg_plan_destroy(korg_plan* plan);
+
+/*
+ * Autotuner: times the batched pipeline on these stars (into scratch alphas; the caller's are not
+ * touched) over the tile shape (Float32: 32/64/128-point tiles x 1/2/4 waves per tile; Float64:
+ * 1/2 points per lane), then persistent waves per CU, then near-zone waves per CU, and makes the
+ * fastest the context's tuning.  cache_path (may be NULL): a text file of results keyed by device,
+ * precision, alpha dtype, layer count and the power-of-two classes of the star, line and point
+ * counts; a cached key is used without timing, a new result is appended.  No counterpart in the
+ * reference.
+ */
+int korg_autotune_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                          const korg_species* species, int alpha_dtype, int precision,
+                          const char* cache_path, korg_tuning* out);
 
 #ifdef __cplusplus
 }

@@ -179,8 +179,14 @@ struct LayerTab {
 
 // Device counters of one call (reset by k_layers): [0] profile queue, [1] evaluations,
 // [2] lines used, [3] error flags, [4] near-zone reach in points, [5] near queue,
-// [6..9] evaluations by Hunger branch.
+// [6..9] evaluations by Hunger branch, [10] near-zone evaluations, [11] far-wing MFMAs per layer
+// block, [12] tail-zone evaluations, [13] entries of the per-tile line lists.
+// Error flags: 1 lines not sorted, 2 species index out of range, 4 the per-tile line lists need
+// more entries than their buffer holds.  With any flag set no kernel writes alpha.
 constexpr int N_COUNTERS = 16;
+constexpr unsigned long long FLAG_UNSORTED = 1, FLAG_SPECIES = 2, FLAG_CAPACITY = 4;
+// Largest far-wing series coefficient kept in Float32 (2^120: headroom below FLT_MAX ~ 2^128).
+constexpr double F32_COEF_MAX = 1.329227995784916e36;
 
 struct GridInfo {
     double g0, ginv;  // first point and inverse mean spacing of the wavelength grid
@@ -329,6 +335,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     }
     double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0;
     bool any_nan = false;
+    bool f32_bad = false;  // a far-wing coefficient of some layer is beyond Float32 range
     double my_hw = -1.0, my_a = 0.0;  // layer `lane` (first 64 layers), for branch counting
     for (int l = L + lane; l < A.Ls; l += WAVE) A.rec64[i * A.Ls + l] = make_double4(0, 0, 0, 0);
     const int Lm = A.n_lblocks * WAVE;
@@ -410,6 +417,8 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
                 c[1] = base * e2 * q * q;
                 c[2] = base * e3 * q * q * q;
                 c[3] = base * e4 * (q * q) * (q * q);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) f32_bad |= !(fabs(c[k]) <= F32_COEF_MAX);  // or NaN
             }
         }
         if (A.want_f32) {
@@ -423,6 +432,19 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
                 const int slot = (k >> 1) * 2 + mb;
                 dst[li * 4 + slot] = (float)c[k];
             }
+        }
+    }
+    // Extreme damping (a ~ 1e6 and more, where e4 ~ a^6) puts the series coefficients out of
+    // Float32 range: c_k R^k stays small where the series is used, but neither factor is
+    // representable (and inf * 0 is NaN).  Such a line skips the series: its coefficients are
+    // zero and the near-zone kernel evaluates its whole window exactly.
+    const bool series_off = A.want_f32 && ballot(f32_bad) != 0;
+    if (series_off) {
+        for (int l = lane; l < Lm; l += WAVE) {
+            float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
+            const int ll = l % WAVE, mb = ll >> 5, row = ll & 31;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) dst[((k & 1) * 32 + row) * 4 + (k >> 1) * 2 + mb] = 0.0f;
         }
     }
     // src/line_absorption.jl:109-115: windows are maxima over layers (NaN if any is NaN)
@@ -466,6 +488,12 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         fhi = (int64_t)__builtin_amdgcn_readlane((int)res, 6) - 1;
         flo = flo < lb ? lb : flo;
         fhi = fhi > ub ? ub : fhi;
+    }
+    if (series_off && lb <= ub) {
+        clo = lb;
+        chi = ub;
+        flo = 1;
+        fhi = 0;
     }
     if (A.count_branches && lb <= ub && lane < L) {
         // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
@@ -705,7 +733,9 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
                                                      const int* __restrict__ part_line,
                                                      const int64_t* __restrict__ tc_start,
                                                      int* __restrict__ tc_count,
-                                                     int* __restrict__ lists) {
+                                                     int* __restrict__ lists,
+                                                     const unsigned long long* __restrict__ counters) {
+    if (tc_start && counters[3] != 0) return;  // fill pass: the lists do not fit (or setup failed)
     const int lane = threadIdx.x;
     const int t = blockIdx.x;  // c * ntiles + tile
     const int c = t / ntiles, tile = t % ntiles;
@@ -729,9 +759,11 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
     if (!tc_start && lane == 0) tc_count[slot] = (int)cnt;
 }
 
-// One block: exclusive scan of n counts into start[0..n] (start[n] = total).
+// One block: exclusive scan of n counts into start[0..n] (start[n] = total).  With counters
+// given, the total is recorded in counters[13] and FLAG_CAPACITY is raised when it exceeds cap.
 __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __restrict__ count,
-                                                    int64_t* __restrict__ start) {
+                                                    int64_t* __restrict__ start, int64_t cap,
+                                                    unsigned long long* __restrict__ counters) {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x;
     const int64_t chunk = (n + 1023) / 1024;
@@ -751,7 +783,13 @@ __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __rest
         start[i] = run;
         run += count[i];
     }
-    if (t == 1023) start[n] = part[1023];
+    if (t == 1023) {
+        start[n] = part[1023];
+        if (counters) {
+            counters[13] = (unsigned long long)part[1023];
+            if (part[1023] > cap) atomicOr(&counters[3], FLAG_CAPACITY);
+        }
+    }
 }
 
 // --------------------------------------------------------------------------------------------
@@ -963,7 +1001,7 @@ __device__ inline int hunger_branch(double a, double v) {
 
 // ---------------------------------------------------------------------------------------------
 // Float32 far wing on the matrix cores.  A work item is (tile of 32*NB points, block of 64
-// layers), owned by one workgroup of WING_WAVES waves; each wave takes a contiguous part of the
+// layers), owned by one workgroup of WW waves; each wave takes a contiguous part of the
 // tile's line list and holds the 64 x 32NB block of alpha as 2 x NB 32x32 MFMA accumulators.  For
 // each line: A = the line's 64 x 4 coefficient block, B = (R, R^2, R^3, R^4) of the tile's points
 // (zero outside the window and inside the near zone), acc += A B with v_mfma_f32_32x32x2_f32
@@ -971,16 +1009,15 @@ __device__ inline int hunger_branch(double a, double v) {
 // added to alpha by wave 0.
 // ---------------------------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
-constexpr int WING_WAVES = 4;  // waves per far-wing workgroup; each takes a part of a tile's lines
 
-// three waves per SIMD (at most 168 registers): measured faster than two waves with no spills
-#ifndef KORG_WING_ATTR
-#define KORG_WING_ATTR __attribute__((amdgpu_waves_per_eu(3)))
-#endif
-template <int NB, class AlphaT>
-__global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(ProfileArgs A) {
-    // partial blocks of waves 1..WING_WAVES-1: [wave][accumulator][register quad][lane]
-    __shared__ float4 s_acc[WING_WAVES - 1][2 * NB][4][WAVE];
+// Registers: three waves per SIMD (at most 168 registers) for 32- and 64-point tiles, measured
+// faster than two waves with no spills; two for 128-point tiles, whose 128 accumulator registers
+// leave too few for three.
+template <int NB, int WW, class AlphaT>
+__global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(NB >= 4 ? 2 : 3)))
+void k_wing_mfma(ProfileArgs A) {
+    // partial blocks of waves 1..WW-1: [wave][accumulator][register quad][lane]
+    __shared__ float4 s_acc[WW > 1 ? WW - 1 : 1][2 * NB][4][WAVE];
     __shared__ int s_item;
     const int lane = threadIdx.x % WAVE;
     const int w = threadIdx.x / WAVE;
@@ -1074,11 +1111,11 @@ __global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(
         // LineMeta one batch ahead, and the operands one pair of lines ahead through four
         // registers (a batch holds a multiple of four lines, so the rotation is in phase at every
         // batch).  Positions past the end of the list prepare a stand-in line and issue nothing.
-        // wave w takes the w-th of WING_WAVES contiguous, near-equal parts of the tile's list
+        // wave w takes the w-th of WW contiguous, near-equal parts of the tile's list
         const int64_t t0 = A.tile_start[(int64_t)tile * N_CLASSES];
         const int64_t t1 = A.tile_start[(int64_t)(tile + 1) * N_CLASSES];
-        const int64_t s0 = t0 + ((t1 - t0) * w) / WING_WAVES;
-        const int64_t s1 = t0 + ((t1 - t0) * (w + 1)) / WING_WAVES;
+        const int64_t s0 = t0 + ((t1 - t0) * w) / WW;
+        const int64_t s1 = t0 + ((t1 - t0) * (w + 1)) / WW;
         if (s0 < s1) {
             const int* __restrict__ tl = A.tile_lines;
             auto ld_idx = [&](int64_t k) { return tl[min(k + lane, s1 - 1)]; };  // clamped
@@ -1155,7 +1192,7 @@ __global__ __launch_bounds__(WAVE * WING_WAVES) KORG_WING_ATTR void k_wing_mfma(
                         float4 v = make_float4(acc[mb][b][4 * q], acc[mb][b][4 * q + 1],
                                                acc[mb][b][4 * q + 2], acc[mb][b][4 * q + 3]);
 #pragma unroll
-                        for (int ww = 0; ww < WING_WAVES - 1; ++ww) {
+                        for (int ww = 0; ww < WW - 1; ++ww) {
                             const float4 p = s_acc[ww][mb * NB + b][q][lane];
                             v.x += p.x;
                             v.y += p.y;
@@ -1425,26 +1462,50 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Device scratch of the forward pipeline (grow-only).  The context owns one for its direct calls;
+// every plan owns its own, so the pointers a captured graph holds stay valid whatever later calls
+// on the context allocate.
+struct Workspace {
+    DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
+        near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
+        tc_start, tile_lines, stars;
+    int64_t tile_cap = 0;             // entries tile_lines holds
+    std::vector<StarDev> stars_host;  // the star table (host copy)
+    void release() {
+        DevBuf* bufs[] = {&meta,     &rec64,     &recM,      &tab,        &counters,  &blk,
+                          &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
+                          &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
+                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars};
+        for (DevBuf* b : bufs) {
+            if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+        tile_cap = 0;
+    }
+};
+
 struct korg_ctx {
     int device = 0;
     int n_cu = 0;
+    std::string arch;  // gcnArchName of the device
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[5] = {};
+    // [0] start, [1] setup done, [2] partition done, [3] end, [4] far wing done,
+    // [5] tile-list total known, [6] tile-list fill launched (the host sizes the lists between)
+    hipEvent_t ev[7] = {};
     std::string err;
     korg_stats stats{};
     korg_tuning tuning{};
     int64_t last_n_lines = 0;
     int last_precision = KORG_PRECISION_FP64;
     int last_n_lblocks = 0;
-    // device scratch (grow-only)
-    DevBuf meta, rec64, recM, tab, counters, gridinfo, blk, cstart, cls_cand, cls_jc, ranges,
-        nblk, ncstart, near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line,
-        tc_count, tc_start, tile_lines, vjp_cnt, vjp_start, vjp_contrib, vjp_species, vjp_out,
-        h_grad, stars;
-    std::vector<StarDev> stars_host;  // the star table of the last call (host copy)
-    int last_n_stars = 0;
-    // staging for the host-memory entry point
+    bool last_sized = false;  // the last forward call sized the tile lists on the host (ev[5], ev[6])
+    Workspace ws;
+    std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
+    // adjoint scratch
+    DevBuf vjp_cnt, vjp_start, vjp_contrib, vjp_species, vjp_out, h_grad;
+    // staging for the host-memory entry points
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
 };
@@ -1493,10 +1554,6 @@ int grow(korg_ctx* ctx, DevBuf& b, size_t bytes) {
     } while (0)
 
 constexpr int F64_GROUP = 8;  // layers per work item of the Float64 form
-#ifndef KORG_MFMA_NB
-#define KORG_MFMA_NB 2
-#endif
-constexpr int MFMA_NB = KORG_MFMA_NB;  // 32-point blocks per work item of the far-wing kernel
 
 int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg_species* sp,
              const korg_layers* ly, const korg_continuum* cn, int alpha_dtype, int precision) {
@@ -1532,12 +1589,42 @@ struct StarHost {
     double cutoff;
 };
 
+// Validates the stars of a batched call (one layer count, one species table for all).
+int collect_stars(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                  const korg_species* species, int alpha_dtype, int precision,
+                  std::vector<StarHost>& hs, int* L_out) {
+    if (n_stars < 1 || !stars) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no stars");
+    if (n_stars > 0x7fffffff) return set_err(ctx, KORG_ERR_UNSUPPORTED, "too many stars");
+    hs.resize((size_t)n_stars);
+    const int L = stars[0].layers.n_layers;
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_star& x = stars[k];
+        const int rc = validate(ctx, &x.grid, &x.lines, species, &x.layers, &x.cntm, alpha_dtype,
+                                precision);
+        if (rc != KORG_OK) {
+            ctx->err = "star " + std::to_string(k) + ": " + ctx->err;
+            return rc;
+        }
+        if (x.layers.n_layers != L)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld has %d layers, star 0 has %d: one layer count per batch",
+                           (long long)k, x.layers.n_layers, L);
+        if (!x.alpha && x.lines.n_lines > 0)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: alpha is null", (long long)k);
+        hs[k] = StarHost{x.alpha, x.grid, x.lines, x.layers, x.cntm, x.cutoff_threshold};
+    }
+    *L_out = L;
+    return KORG_OK;
+}
+
 // Lays the stars out in the global index spaces (grids padded to STAR_ALIGN points, lines
-// concatenated) and uploads the star table.  Returns the totals.
-int upload_stars(korg_ctx* ctx, int n_stars, const StarHost* st, int64_t* n_lines_total,
-                 int64_t* n_wl_total) {
-    auto& h = ctx->stars_host;
+// concatenated) and uploads the star table to ws.stars on `s`.  Returns the totals.
+int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const StarHost* st,
+                 int64_t* n_lines_total, int64_t* n_wl_total) {
+    auto& h = ws.stars_host;
     h.assign(n_stars, StarDev{});
+    ctx->last_grid_off.resize(n_stars);
+    ctx->last_line_off.resize(n_stars);
     int64_t go = 0, lo = 0;
     for (int k = 0; k < n_stars; ++k) {
         const StarHost& x = st[k];
@@ -1568,217 +1655,312 @@ int upload_stars(korg_ctx* ctx, int n_stars, const StarHost* st, int64_t* n_line
         d.knots = x.cntm.knots;
         d.cntm = x.cntm.values;
         d.cutoff = x.cutoff;
+        ctx->last_grid_off[k] = go;
+        ctx->last_line_off[k] = lo;
         go += (x.grid.n_wl + STAR_ALIGN - 1) / STAR_ALIGN * STAR_ALIGN;
         lo += x.lines.n_lines;
     }
     if (go > (int64_t)INT32_MAX || lo > (int64_t)INT32_MAX)
         return set_err(ctx, KORG_ERR_UNSUPPORTED,
                        "more than 2^31 grid points or lines in one call; split the batch");
-    TRY(grow(ctx, ctx->stars, sizeof(StarDev) * (size_t)n_stars));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->stars.p, h.data(), sizeof(StarDev) * (size_t)n_stars,
-                                hipMemcpyHostToDevice, ctx->stream));
-    ctx->last_n_stars = n_stars;
+    TRY(grow(ctx, ws.stars, sizeof(StarDev) * (size_t)n_stars));
+    HIP_TRY(ctx, hipMemcpyAsync(ws.stars.p, h.data(), sizeof(StarDev) * (size_t)n_stars,
+                                hipMemcpyHostToDevice, s));
     *n_lines_total = lo;
     *n_wl_total = go;
     return KORG_OK;
 }
 
-// Per-layer tables (k_layers) and the per-line setup (k_setup) of the uploaded stars: windows,
-// zones, the Float64 profile records and, for the Float32 form, the MFMA coefficient blocks.
-// Records ev[0], ev[1].
-int launch_setup(korg_ctx* ctx, int n_stars, int L, int64_t N, const korg_species* sp, bool f32) {
-    hipStream_t s = ctx->stream;
-    const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
-    const int Ls = n_groups * F64_GROUP;
-    const int n_lblocks = (L + WAVE - 1) / WAVE;
-    TRY(grow(ctx, ctx->meta, sizeof(LineMeta) * N));
-    TRY(grow(ctx, ctx->rec64, sizeof(double4) * N * Ls));
-    if (f32) TRY(grow(ctx, ctx->recM, sizeof(float4) * (size_t)N * n_lblocks * WAVE));
-    TRY(grow(ctx, ctx->tab, sizeof(LayerTab) * (size_t)n_stars * L));
-    TRY(grow(ctx, ctx->counters, sizeof(unsigned long long) * N_COUNTERS));
-    auto* counters = (unsigned long long*)ctx->counters.p;
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
-    hipLaunchKernelGGL(k_layers, dim3(n_stars), dim3(64), 0, s, n_stars, L, (StarDev*)ctx->stars.p,
-                       (LayerTab*)ctx->tab.p, counters);
-    SetupArgs sa;
-    sa.n_lines = N;
-    sa.L = L;
-    sa.Ls = Ls;
-    sa.n_lblocks = n_lblocks;
-    sa.n_stars = n_stars;
-    sa.stars = (const StarDev*)ctx->stars.p;
-    sa.n_species = sp->n_species;
-    sa.mass = sp->mass;
-    sa.is_mol = sp->is_molecule;
-    sa.tab = (const LayerTab*)ctx->tab.p;
-    sa.want_f32 = f32 ? 1 : 0;
-    sa.count_branches = ctx->tuning.count_branches ? 1 : 0;
-    sa.meta = (LineMeta*)ctx->meta.p;
-    sa.rec64 = (double4*)ctx->rec64.p;
-    sa.recM = (float4*)ctx->recM.p;
-    sa.counters = counters;
-    if (N > 0)
-        hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+// Sizes and launch shapes of one forward call over a batch of stars.
+struct Batch {
+    int n_stars = 0, L = 0;
+    int64_t N = 0, nwl = 0;
+    bool f32 = false, alpha_f32 = false;
+    int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, count_branches = 0;
+    int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
+    korg_species sp{};
+};
+
+// The far-wing kernel variants: (32-point blocks per tile, waves per tile).
+bool wing_variant_ok(int NB, int WW) {
+    return (NB == 1 || NB == 2 || NB == 4) && (WW == 1 || WW == 2 || WW == 4) && !(NB == 4 && WW == 4);
+}
+
+int check_tuning(korg_ctx* ctx, const korg_tuning* t) {
+    const int tp = t->tile_points, wt = t->waves_per_tile;
+    if (t->points_per_lane < 0 || t->points_per_lane > 2 || t->waves_per_cu < 0 ||
+        t->waves_per_cu > 64 || t->near_waves_per_cu < 0 || t->near_waves_per_cu > 64 ||
+        t->count_branches < 0 || t->count_branches > 1 ||
+        !(tp == 0 || tp == 32 || tp == 64 || tp == 128) || !(wt == 0 || wt == 1 || wt == 2 || wt == 4))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "tuning out of range");
+    if (!wing_variant_ok(tp ? tp / 32 : 2, wt ? wt : 4))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                       "tuning: 128-point tiles take at most 2 waves per tile");
     return KORG_OK;
 }
 
-// The device pipeline over a batch of stars (every pointer a device pointer; one alpha dtype and
-// one layer count for the batch): setup, partition, far wing, near zone.
-int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_species* sp, int L,
-              int alpha_dtype, int precision) {
-    int64_t N = 0, nwl = 0;
-    for (int k = 0; k < n_stars; ++k) N += stars[k].lines.n_lines;
-    ctx->last_n_lines = N;
-    ctx->last_precision = precision;
-    ctx->last_n_lblocks = (L + WAVE - 1) / WAVE;
-    std::memset(&ctx->stats, 0, sizeof ctx->stats);
-    if (N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
-    TRY(upload_stars(ctx, n_stars, stars, &N, &nwl));
-    hipStream_t s = ctx->stream;
-    const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
-    const int P64 = ctx->tuning.points_per_lane == 2 ? 2 : 1;
-    const int tile_w = f32 ? 32 * MFMA_NB : WAVE * P64;
-    const int ntiles = (int)((nwl + tile_w - 1) / tile_w);
-    const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
-    const int Ls = n_groups * F64_GROUP;              // rec64 stride: whole Float64 groups
-    const int n_lblocks = (L + WAVE - 1) / WAVE;      // 64-layer blocks (Float32 records)
-    const int nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
-    const int wpc = ctx->tuning.waves_per_cu > 0 ? ctx->tuning.waves_per_cu : 12;
+// Computes the batch's shapes from the tuning, uploads the star table and grows every buffer of
+// ws except the per-tile line lists (sized from the device count, by the caller).
+int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost* stars,
+               const korg_species* sp, int L, int alpha_dtype, int precision, Workspace& ws,
+               hipStream_t s, Batch& b) {
+    b = Batch{};
+    b.n_stars = n_stars;
+    b.L = L;
+    b.sp = *sp;
+    b.f32 = precision == KORG_PRECISION_FP32_FIXUP;
+    b.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+    b.NB = t.tile_points ? t.tile_points / 32 : 2;
+    b.WW = t.waves_per_tile ? t.waves_per_tile : 4;
+    b.P64 = t.points_per_lane == 2 ? 2 : 1;
+    b.wpc = t.waves_per_cu > 0 ? t.waves_per_cu : 12;
+    b.near_wpc = t.near_waves_per_cu > 0 ? t.near_waves_per_cu : 16;
+    b.count_branches = t.count_branches ? 1 : 0;
+    for (int k = 0; k < n_stars; ++k) b.N += stars[k].lines.n_lines;
+    if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
+    TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
+    const int64_t N = b.N;
+    b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
+    b.ntiles = (int)((b.nwl + b.tile_w - 1) / b.tile_w);
+    b.nt_near = (int)((b.nwl + CT - 1) / CT);
+    b.n_groups = (L + F64_GROUP - 1) / F64_GROUP;
+    b.Ls = b.n_groups * F64_GROUP;        // rec64 stride: whole Float64 groups
+    b.n_lblocks = (L + WAVE - 1) / WAVE;  // 64-layer blocks (Float32 records)
+    b.nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
+    TRY(grow(ctx, ws.meta, sizeof(LineMeta) * N));
+    TRY(grow(ctx, ws.rec64, sizeof(double4) * N * b.Ls));
+    if (b.f32) TRY(grow(ctx, ws.recM, sizeof(float4) * (size_t)N * b.n_lblocks * WAVE));
+    TRY(grow(ctx, ws.tab, sizeof(LayerTab) * (size_t)n_stars * L));
+    TRY(grow(ctx, ws.counters, sizeof(unsigned long long) * N_COUNTERS));
+    TRY(grow(ctx, ws.blk, sizeof(int) * (size_t)b.nblocks * N_CLASSES));
+    TRY(grow(ctx, ws.nblk, sizeof(int) * (size_t)b.nblocks * N_CLASSES));
+    TRY(grow(ctx, ws.cstart, sizeof(int) * (N_CLASSES + 1)));
+    TRY(grow(ctx, ws.cls_cand, sizeof(Cand) * N));
+    TRY(grow(ctx, ws.cls_rng, sizeof(int2) * N));
+    TRY(grow(ctx, ws.cls_line, sizeof(int) * N));
+    TRY(grow(ctx, ws.cls_jc, sizeof(int) * N));
+    TRY(grow(ctx, ws.ranges, sizeof(int2) * (size_t)b.ntiles * N_CLASSES));
+    if (b.f32) {
+        TRY(grow(ctx, ws.tc_count, sizeof(int) * (size_t)b.ntiles * N_CLASSES));
+        TRY(grow(ctx, ws.tc_start, sizeof(int64_t) * ((size_t)b.ntiles * N_CLASSES + 1)));
+        TRY(grow(ctx, ws.ncstart, sizeof(int) * (N_CLASSES + 1)));
+        TRY(grow(ctx, ws.near_cand, sizeof(Cand) * N));
+        TRY(grow(ctx, ws.near_rng, sizeof(int2) * N));
+        TRY(grow(ctx, ws.near_line, sizeof(int) * N));
+        TRY(grow(ctx, ws.near_jc, sizeof(int) * N));
+        TRY(grow(ctx, ws.near_ranges, sizeof(int2) * (size_t)b.nt_near * N_CLASSES));
+    }
+    return KORG_OK;
+}
 
-    TRY(grow(ctx, ctx->blk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-    TRY(grow(ctx, ctx->cstart, sizeof(int) * (N_CLASSES + 1)));
-    TRY(grow(ctx, ctx->cls_cand, sizeof(Cand) * N));
-    TRY(grow(ctx, ctx->cls_rng, sizeof(int2) * N));
-    TRY(grow(ctx, ctx->cls_line, sizeof(int) * N));
-    if (f32) {
-        TRY(grow(ctx, ctx->tc_count, sizeof(int) * (size_t)ntiles * N_CLASSES));
-        TRY(grow(ctx, ctx->tc_start, sizeof(int64_t) * ((size_t)ntiles * N_CLASSES + 1)));
-    }
-    TRY(grow(ctx, ctx->cls_jc, sizeof(int) * N));
-    TRY(grow(ctx, ctx->ranges, sizeof(int2) * (size_t)ntiles * N_CLASSES));
-    const int nt_near = (int)((nwl + CT - 1) / CT);
-    if (f32) {
-        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-        TRY(grow(ctx, ctx->ncstart, sizeof(int) * (N_CLASSES + 1)));
-        TRY(grow(ctx, ctx->near_cand, sizeof(Cand) * N));
-        TRY(grow(ctx, ctx->near_rng, sizeof(int2) * N));
-        TRY(grow(ctx, ctx->near_line, sizeof(int) * N));
-        TRY(grow(ctx, ctx->near_jc, sizeof(int) * N));
-        TRY(grow(ctx, ctx->near_ranges, sizeof(int2) * (size_t)nt_near * N_CLASSES));
-    } else {
-        TRY(grow(ctx, ctx->nblk, sizeof(int) * (size_t)nblocks * N_CLASSES));
-    }
-    TRY(launch_setup(ctx, n_stars, L, N, sp, f32));
-    auto* counters = (unsigned long long*)ctx->counters.p;
+// Per-layer tables (k_layers, which also clears the counters) and the per-line setup (k_setup):
+// windows, zones, the Float64 profile records and, for the Float32 form, the MFMA coefficient
+// blocks.  `timed` records ev[0], ev[1].
+int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed) {
+    const int64_t N = b.N;
+    auto* counters = (unsigned long long*)ws.counters.p;
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_layers, dim3(b.n_stars), dim3(64), 0, s, b.n_stars, b.L,
+                       (StarDev*)ws.stars.p, (LayerTab*)ws.tab.p, counters);
+    SetupArgs sa;
+    sa.n_lines = N;
+    sa.L = b.L;
+    sa.Ls = b.Ls;
+    sa.n_lblocks = b.n_lblocks;
+    sa.n_stars = b.n_stars;
+    sa.stars = (const StarDev*)ws.stars.p;
+    sa.n_species = b.sp.n_species;
+    sa.mass = b.sp.mass;
+    sa.is_mol = b.sp.is_molecule;
+    sa.tab = (const LayerTab*)ws.tab.p;
+    sa.want_f32 = b.f32 ? 1 : 0;
+    sa.count_branches = b.count_branches;
+    sa.meta = (LineMeta*)ws.meta.p;
+    sa.rec64 = (double4*)ws.rec64.p;
+    sa.recM = (float4*)ws.recM.p;
+    sa.counters = counters;
+    hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
+    HIP_TRY(ctx, hipGetLastError());
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+    return KORG_OK;
+}
 
-    hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(PART_BLOCK), 0, s, N, L, 32 * MFMA_NB,
-                       (const LineMeta*)ctx->meta.p, (int*)ctx->blk.p, (int*)ctx->nblk.p, counters);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->blk.p,
-                       (int*)ctx->cstart.p);
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
-                       (const LineMeta*)ctx->meta.p, (const int*)ctx->blk.p,
-                       (const int*)ctx->cstart.p, (Cand*)ctx->cls_cand.p, (int*)ctx->cls_jc.p,
-                       (int2*)ctx->cls_rng.p, (int*)ctx->cls_line.p, (int)KEY_WINDOW);
-    hipLaunchKernelGGL(k_ranges, dim3((ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
-                       ntiles, tile_w, nwl, (const int*)ctx->cstart.p, (const int*)ctx->cls_jc.p,
-                       (int2*)ctx->ranges.p);
-    if (f32) {
-        // exact per-tile line lists for the far-wing kernel: count, scan, size, fill
-        const int64_t ntc = (int64_t)ntiles * N_CLASSES;
-        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, ntiles, tile_w, nwl,
-                           (const int2*)ctx->ranges.p, (const int2*)ctx->cls_rng.p,
-                           (const int*)ctx->cls_line.p, (const int64_t*)nullptr,
-                           (int*)ctx->tc_count.p, (int*)nullptr);
-        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, ntc, (const int*)ctx->tc_count.p,
-                           (int64_t*)ctx->tc_start.p);
-        HIP_TRY(ctx, hipGetLastError());
-        int64_t total = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(&total, (int64_t*)ctx->tc_start.p + ntc, sizeof total,
-                                    hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        if (total > (int64_t)INT32_MAX)
-            return set_err(ctx, KORG_ERR_UNSUPPORTED,
-                           "more than 2^31 (line, tile) pairs in one call; split the line list");
-        TRY(grow(ctx, ctx->tile_lines, sizeof(int) * (size_t)(total > 0 ? total : 1)));
-        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, ntiles, tile_w, nwl,
-                           (const int2*)ctx->ranges.p, (const int2*)ctx->cls_rng.p,
-                           (const int*)ctx->cls_line.p, (const int64_t*)ctx->tc_start.p,
-                           (int*)nullptr, (int*)ctx->tile_lines.p);
-    }
-    if (f32) {
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nblocks, (int*)ctx->nblk.p,
-                           (int*)ctx->ncstart.p);
-        hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(PART_BLOCK), 0, s, N,
-                           (const LineMeta*)ctx->meta.p, (const int*)ctx->nblk.p,
-                           (const int*)ctx->ncstart.p, (Cand*)ctx->near_cand.p,
-                           (int*)ctx->near_jc.p, (int2*)ctx->near_rng.p, (int*)ctx->near_line.p,
-                           (int)KEY_NEAR);
-        hipLaunchKernelGGL(k_ranges, dim3((nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
-                           nt_near, CT, nwl, (const int*)ctx->ncstart.p,
-                           (const int*)ctx->near_jc.p, (int2*)ctx->near_ranges.p);
+// First half of the pipeline on `s`: setup, the class partitions, and (Float32 form) the per-tile
+// list counts and their scan, which flags FLAG_CAPACITY when the lists exceed tile_cap entries.
+// `timed` records ev[0], ev[1], ev[5].
+int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int64_t tile_cap,
+                  bool timed) {
+    TRY(enqueue_setup(ctx, ws, b, s, timed));
+    const int64_t N = b.N;
+    auto* counters = (unsigned long long*)ws.counters.p;
+
+    hipLaunchKernelGGL(k_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N, b.L, 32 * b.NB,
+                       (const LineMeta*)ws.meta.p, (int*)ws.blk.p, (int*)ws.nblk.p, counters);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.blk.p,
+                       (int*)ws.cstart.p);
+    hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
+                       (const LineMeta*)ws.meta.p, (const int*)ws.blk.p, (const int*)ws.cstart.p,
+                       (Cand*)ws.cls_cand.p, (int*)ws.cls_jc.p, (int2*)ws.cls_rng.p,
+                       (int*)ws.cls_line.p, (int)KEY_WINDOW);
+    hipLaunchKernelGGL(k_ranges, dim3((b.ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
+                       b.ntiles, b.tile_w, b.nwl, (const int*)ws.cstart.p, (const int*)ws.cls_jc.p,
+                       (int2*)ws.ranges.p);
+    if (b.f32) {
+        // near-zone partition (by reach class) and its per-tile candidate ranges
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.nblk.p,
+                           (int*)ws.ncstart.p);
+        hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
+                           (const LineMeta*)ws.meta.p, (const int*)ws.nblk.p,
+                           (const int*)ws.ncstart.p, (Cand*)ws.near_cand.p, (int*)ws.near_jc.p,
+                           (int2*)ws.near_rng.p, (int*)ws.near_line.p, (int)KEY_NEAR);
+        hipLaunchKernelGGL(k_ranges, dim3((b.nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
+                           b.nt_near, CT, b.nwl, (const int*)ws.ncstart.p,
+                           (const int*)ws.near_jc.p, (int2*)ws.near_ranges.p);
+        // exact per-tile line lists of the far-wing kernel: count and scan
+        const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, b.ntiles, b.tile_w,
+                           b.nwl, (const int2*)ws.ranges.p, (const int2*)ws.cls_rng.p,
+                           (const int*)ws.cls_line.p, (const int64_t*)nullptr, (int*)ws.tc_count.p,
+                           (int*)nullptr, (const unsigned long long*)counters);
+        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, ntc, (const int*)ws.tc_count.p,
+                           (int64_t*)ws.tc_start.p, tile_cap, counters);
     }
     HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+    if (timed && b.f32) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+    return KORG_OK;
+}
 
+template <int NB, int WW>
+void launch_wing(const ProfileArgs& pa, dim3 grid, hipStream_t s) {
+    if (pa.alpha_f32) hipLaunchKernelGGL((k_wing_mfma<NB, WW, float>), grid, dim3(WAVE * WW), 0, s, pa);
+    else hipLaunchKernelGGL((k_wing_mfma<NB, WW, double>), grid, dim3(WAVE * WW), 0, s, pa);
+}
+
+// Second half on `s`: (Float32 form) the per-tile list fill, then the profile kernels that add
+// into alpha.  `timed` records ev[6], ev[2], ev[4], ev[3].
+int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed) {
+    auto* counters = (unsigned long long*)ws.counters.p;
+    if (b.f32) {
+        if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], s));
+        const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+        hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, b.ntiles, b.tile_w,
+                           b.nwl, (const int2*)ws.ranges.p, (const int2*)ws.cls_rng.p,
+                           (const int*)ws.cls_line.p, (const int64_t*)ws.tc_start.p,
+                           (int*)nullptr, (int*)ws.tile_lines.p, (const unsigned long long*)counters);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
     ProfileArgs pa;
-    pa.n_wl = nwl;
-    pa.L = L;
-    pa.Ls = Ls;
-    pa.n_lblocks = f32 ? n_lblocks : n_groups;
-    pa.ntiles = ntiles;
-    pa.tile_w = tile_w;
-    pa.n_items = (int64_t)ntiles * pa.n_lblocks;
-    pa.stars = (const StarDev*)ctx->stars.p;
-    pa.n_stars = n_stars;
-    pa.rec64 = (const double4*)ctx->rec64.p;
-    pa.recM = (const float4*)ctx->recM.p;
-    pa.cands = (const Cand*)ctx->cls_cand.p;
-    pa.meta = (const LineMeta*)ctx->meta.p;
-    pa.tile_start = (const int64_t*)ctx->tc_start.p;
-    pa.tile_lines = (const int*)ctx->tile_lines.p;
-    pa.ranges = (const int2*)ctx->ranges.p;
-    pa.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+    pa.n_wl = b.nwl;
+    pa.L = b.L;
+    pa.Ls = b.Ls;
+    pa.n_lblocks = b.f32 ? b.n_lblocks : b.n_groups;
+    pa.ntiles = b.ntiles;
+    pa.tile_w = b.tile_w;
+    pa.n_items = (int64_t)b.ntiles * pa.n_lblocks;
+    pa.stars = (const StarDev*)ws.stars.p;
+    pa.n_stars = b.n_stars;
+    pa.rec64 = (const double4*)ws.rec64.p;
+    pa.recM = (const float4*)ws.recM.p;
+    pa.cands = (const Cand*)ws.cls_cand.p;
+    pa.meta = (const LineMeta*)ws.meta.p;
+    pa.tile_start = (const int64_t*)ws.tc_start.p;
+    pa.tile_lines = (const int*)ws.tile_lines.p;
+    pa.ranges = (const int2*)ws.ranges.p;
+    pa.alpha_f32 = b.alpha_f32;
     pa.counters = counters;
-    int64_t nwaves = (int64_t)ctx->n_cu * wpc;
-    if (nwaves > pa.n_items) nwaves = pa.n_items;
-    const dim3 grid((unsigned)nwaves);
-    if (!f32) {
-        if (P64 == 2) hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 2>), grid, dim3(WAVE), 0, s, pa);
+    if (!b.f32) {
+        int64_t nwaves = (int64_t)ctx->n_cu * b.wpc;
+        if (nwaves > pa.n_items) nwaves = pa.n_items;
+        const dim3 grid((unsigned)nwaves);
+        if (b.P64 == 2) hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 2>), grid, dim3(WAVE), 0, s, pa);
         else hipLaunchKernelGGL((k_profile_f64<F64_GROUP, 1>), grid, dim3(WAVE), 0, s, pa);
         HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     } else {
-        int64_t nwg = (int64_t)ctx->n_cu * ((wpc + WING_WAVES - 1) / WING_WAVES);
+        int64_t nwg = (int64_t)ctx->n_cu * ((b.wpc + b.WW - 1) / b.WW);
         if (nwg > pa.n_items) nwg = pa.n_items;
-        const dim3 wgrid((unsigned)nwg), wblock(WAVE * WING_WAVES);
-        if (pa.alpha_f32) hipLaunchKernelGGL((k_wing_mfma<MFMA_NB, float>), wgrid, wblock, 0, s, pa);
-        else hipLaunchKernelGGL((k_wing_mfma<MFMA_NB, double>), wgrid, wblock, 0, s, pa);
+        const dim3 wgrid((unsigned)nwg);
+        switch (b.NB * 8 + b.WW) {
+            case 1 * 8 + 1: launch_wing<1, 1>(pa, wgrid, s); break;
+            case 1 * 8 + 2: launch_wing<1, 2>(pa, wgrid, s); break;
+            case 1 * 8 + 4: launch_wing<1, 4>(pa, wgrid, s); break;
+            case 2 * 8 + 1: launch_wing<2, 1>(pa, wgrid, s); break;
+            case 2 * 8 + 2: launch_wing<2, 2>(pa, wgrid, s); break;
+            case 2 * 8 + 4: launch_wing<2, 4>(pa, wgrid, s); break;
+            case 4 * 8 + 1: launch_wing<4, 1>(pa, wgrid, s); break;
+            case 4 * 8 + 2: launch_wing<4, 2>(pa, wgrid, s); break;
+            default:
+                return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no far-wing kernel for %d-point "
+                               "tiles with %d waves per tile", 32 * b.NB, b.WW);
+        }
         HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
         NearArgs na;
-        na.n_wl = nwl;
-        na.n_lines = N;
-        na.L = L;
-        na.Ls = Ls;
-        na.n_lblocks = n_lblocks;
-        na.ntiles = nt_near;
-        na.n_items = (int64_t)na.ntiles * n_lblocks;
-        na.stars = (const StarDev*)ctx->stars.p;
-        na.n_stars = n_stars;
-        na.cands = (const Cand*)ctx->near_cand.p;
-        na.ranges = (const int2*)ctx->near_ranges.p;
-        na.rec64 = (const double4*)ctx->rec64.p;
-        na.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+        na.n_wl = b.nwl;
+        na.n_lines = b.N;
+        na.L = b.L;
+        na.Ls = b.Ls;
+        na.n_lblocks = b.n_lblocks;
+        na.ntiles = b.nt_near;
+        na.n_items = (int64_t)na.ntiles * b.n_lblocks;
+        na.stars = (const StarDev*)ws.stars.p;
+        na.n_stars = b.n_stars;
+        na.cands = (const Cand*)ws.near_cand.p;
+        na.ranges = (const int2*)ws.near_ranges.p;
+        na.rec64 = (const double4*)ws.rec64.p;
+        na.alpha_f32 = b.alpha_f32;
         na.counters = counters;
-        int64_t cwaves = (int64_t)ctx->n_cu * 16;
+        int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
         if (cwaves > na.n_items) cwaves = na.n_items;
         hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
         HIP_TRY(ctx, hipGetLastError());
     }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     return KORG_OK;
+}
+
+// Reads the per-tile list total (after enqueue_front on s), waits for it, and grows ws.tile_lines
+// to `headroom` times it.
+int size_tile_lists(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, double headroom,
+                    int64_t* total_out) {
+    const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+    int64_t total = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&total, (int64_t*)ws.tc_start.p + ntc, sizeof total,
+                                hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (total > (int64_t)INT32_MAX)
+        return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                       "more than 2^31 (line, tile) pairs in one call; split the line list");
+    int64_t cap = (int64_t)((double)total * headroom) + (headroom > 1.0 ? 4096 : 0);
+    if (cap < total) cap = total;
+    if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
+    TRY(grow(ctx, ws.tile_lines, sizeof(int) * (size_t)(cap > 0 ? cap : 1)));
+    ws.tile_cap = cap;
+    *total_out = total;
+    return KORG_OK;
+}
+
+// The eager device pipeline over a batch of stars (every pointer a device pointer; one alpha
+// dtype and one layer count for the batch), on the context's stream and workspace.
+int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_species* sp, int L,
+              int alpha_dtype, int precision) {
+    hipStream_t s = ctx->stream;
+    Batch b;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    ctx->last_n_lines = 0;
+    TRY(plan_batch(ctx, ctx->tuning, n_stars, stars, sp, L, alpha_dtype, precision, ctx->ws, s, b));
+    ctx->last_n_lines = b.N;
+    ctx->last_precision = precision;
+    ctx->last_n_lblocks = b.n_lblocks;
+    ctx->last_sized = b.f32;
+    if (b.N == 0) return KORG_OK;
+    TRY(enqueue_front(ctx, ctx->ws, b, s, INT64_MAX, true));
+    if (b.f32) {
+        int64_t total = 0;
+        TRY(size_tile_lists(ctx, ctx->ws, b, s, 1.0, &total));
+    }
+    return enqueue_back(ctx, ctx->ws, b, s, true);
 }
 
 // One line_absorption! call on device pointers: a batch of one star.
@@ -1787,6 +1969,52 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
                const korg_continuum* cn, double cutoff, int precision) {
     const StarHost one = {alpha, *g, *ln, *ly, *cn, cutoff};
     return run_batch(ctx, 1, &one, sp, ly->n_layers, alpha_dtype, precision);
+}
+
+// Counters of a finished forward call into ctx->stats; an error for a raised flag.
+int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks) {
+    if (c[3] & FLAG_SPECIES)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                       "a line's species index is outside [0, n_species); alpha was left unchanged");
+    if (c[3] & FLAG_UNSORTED)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                       "the line list is not sorted by wavelength; alpha was left unchanged");
+    if (c[3] & FLAG_CAPACITY)
+        return set_err(ctx, KORG_ERR_CAPACITY,
+                       "per-tile line lists exceeded their buffer; alpha was left unchanged");
+    ctx->stats.n_evals = (double)c[1];
+    ctx->stats.n_lines_used = (int64_t)c[2];
+    for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
+    ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
+    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * n_lblocks : 0.0;
+    ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
+    return KORG_OK;
+}
+
+float elapsed(korg_ctx* ctx, int a, int b) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return t;
+}
+
+int finish_stats(korg_ctx* ctx) {
+    if (ctx->last_n_lines == 0) return KORG_OK;
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+    unsigned long long c[N_COUNTERS];
+    HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
+                            ctx->last_n_lblocks));
+    // device time: the host's sizing of the tile lists (ev[5] -> ev[6]) is left out
+    const float gap = ctx->last_sized ? elapsed(ctx, 5, 6) : 0.0f;
+    ctx->stats.setup_ms = elapsed(ctx, 0, 1);
+    ctx->stats.bin_ms = elapsed(ctx, 1, 2) - gap;
+    ctx->stats.profile_ms = elapsed(ctx, 2, 4);
+    ctx->stats.core_ms = elapsed(ctx, 4, 3);
+    ctx->stats.total_ms = elapsed(ctx, 0, 3) - gap;
+    return KORG_OK;
 }
 
 // The adjoint pipeline on device pointers: setup (windows and Float64 records), one item per
@@ -1803,34 +2031,35 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
     ctx->last_n_lines = N;
     ctx->last_precision = KORG_PRECISION_FP64;
     ctx->last_n_lblocks = 0;
+    ctx->last_sized = false;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     TRY(grow(ctx, ctx->vjp_out, sizeof(double) * (size_t)nout));
     double* res = (double*)ctx->vjp_out.p;
     if (N > 0) {
-        const int n_groups = (L + F64_GROUP - 1) / F64_GROUP;
-        const int Ls = n_groups * F64_GROUP;
+        Workspace& ws = ctx->ws;
+        const StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
+        Batch b;
+        TRY(plan_batch(ctx, ctx->tuning, 1, &one, sp, L, KORG_DTYPE_F64, KORG_PRECISION_FP64, ws, s,
+                       b));
         TRY(grow(ctx, ctx->vjp_cnt, sizeof(int) * N));
         TRY(grow(ctx, ctx->vjp_start, sizeof(int64_t) * (N + 1)));
-        const StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
-        int64_t n_lines = 0, n_wl = 0;
-        TRY(upload_stars(ctx, 1, &one, &n_lines, &n_wl));
-        TRY(launch_setup(ctx, 1, L, N, sp, false));
+        TRY(enqueue_setup(ctx, ws, b, s, true));
         hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N,
-                           (const LineMeta*)ctx->meta.p, (int*)ctx->vjp_cnt.p);
+                           (const LineMeta*)ws.meta.p, (int*)ctx->vjp_cnt.p);
         hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, N, (const int*)ctx->vjp_cnt.p,
-                           (int64_t*)ctx->vjp_start.p);
+                           (int64_t*)ctx->vjp_start.p, (int64_t)0, (unsigned long long*)nullptr);
         HIP_TRY(ctx, hipGetLastError());
         int64_t n_items = 0;
         unsigned long long flags = 0;
         HIP_TRY(ctx, hipMemcpyAsync(&n_items, (int64_t*)ctx->vjp_start.p + N, sizeof n_items,
                                     hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(&flags, (unsigned long long*)ctx->counters.p + 3, sizeof flags,
+        HIP_TRY(ctx, hipMemcpyAsync(&flags, (unsigned long long*)ws.counters.p + 3, sizeof flags,
                                     hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipStreamSynchronize(s));
-        if (flags & 2)
+        if (flags & FLAG_SPECIES)
             return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
                            "a line's species index is outside [0, n_species)");
-        if (flags & 1)
+        if (flags & FLAG_UNSORTED)
             return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "the line list is not sorted by wavelength");
         HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
         if (n_items > 0) {
@@ -1840,16 +2069,16 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
             va.n_lines = N;
             va.n_items = n_items;
             va.L = L;
-            va.Ls = Ls;
+            va.Ls = b.Ls;
             va.wls = g->all_wls;
-            va.meta = (const LineMeta*)ctx->meta.p;
-            va.rec64 = (const double4*)ctx->rec64.p;
+            va.meta = (const LineMeta*)ws.meta.p;
+            va.rec64 = (const double4*)ws.rec64.p;
             va.item_start = (const int64_t*)ctx->vjp_start.p;
             va.temps = ly->temps;
             va.n_e = ly->n_e;
             va.xi = ly->xi;
             va.n_div_U = ly->n_div_U;
-            va.tab = (const LayerTab*)ctx->tab.p;
+            va.tab = (const LayerTab*)ws.tab.p;
             va.wl = ln->wl;
             va.log_gf = ln->log_gf;
             va.E_lower = ln->E_lower;
@@ -1863,7 +2092,7 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
             va.g_f32 = g_dtype == KORG_DTYPE_F32;
             va.contrib = (double*)ctx->vjp_contrib.p;
             va.item_species = (int*)ctx->vjp_species.p;
-            va.counters = (const unsigned long long*)ctx->counters.p;
+            va.counters = (const unsigned long long*)ws.counters.p;
             int64_t grid = (int64_t)ctx->n_cu * 16;
             if (grid > n_items) grid = n_items;
             hipLaunchKernelGGL(k_vjp, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
@@ -1895,48 +2124,10 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
 int finish_vjp_stats(korg_ctx* ctx) {
     if (ctx->last_n_lines == 0) return KORG_OK;
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
-    float t01 = 0, t24 = 0, t43 = 0, t03 = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t24, ctx->ev[2], ctx->ev[4]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-    ctx->stats.setup_ms = t01;
-    ctx->stats.profile_ms = t24;
-    ctx->stats.core_ms = t43;
-    ctx->stats.total_ms = t03;
-    return KORG_OK;
-}
-
-int finish_stats(korg_ctx* ctx) {
-    if (ctx->last_n_lines == 0) return KORG_OK;
-    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
-    float t01 = 0, t12 = 0, t24 = 0, t43 = 0, t03 = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t24, ctx->ev[2], ctx->ev[4]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t43, ctx->ev[4], ctx->ev[3]));
-    HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-    unsigned long long c[13];
-    static_assert(sizeof c <= sizeof(unsigned long long) * N_COUNTERS, "counters");
-    HIP_TRY(ctx, hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
-    if (c[3] & 2)
-        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                       "a line's species index is outside [0, n_species); alpha was left unchanged");
-    if (c[3] & 1)
-        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                       "the line list is not sorted by wavelength; alpha was left unchanged");
-    ctx->stats.n_evals = (double)c[1];
-    ctx->stats.n_lines_used = (int64_t)c[2];
-    ctx->stats.setup_ms = t01;
-    ctx->stats.bin_ms = t12;
-    ctx->stats.profile_ms = t24;
-    ctx->stats.core_ms = t43;
-    ctx->stats.total_ms = t03;
-    for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
-    const bool f32 = ctx->last_precision == KORG_PRECISION_FP32_FIXUP;
-    ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
-    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * ctx->last_n_lblocks : 0.0;
-    ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
+    ctx->stats.setup_ms = elapsed(ctx, 0, 1);
+    ctx->stats.profile_ms = elapsed(ctx, 2, 4);
+    ctx->stats.core_ms = elapsed(ctx, 4, 3);
+    ctx->stats.total_ms = elapsed(ctx, 0, 3);
     return KORG_OK;
 }
 
@@ -1988,6 +2179,266 @@ int stage_inputs(korg_ctx* ctx, const korg_grid* grid, const korg_lines* lines,
 
 }  // namespace
 
+// A batched forward call captured in a HIP graph for fixed device pointers: the whole pipeline,
+// with no host round trip, replayed by korg_plan_launch.  Owns its workspace.
+struct korg_plan {
+    korg_ctx* ctx = nullptr;
+    Workspace ws;
+    std::vector<StarHost> stars;
+    korg_species sp{};
+    int L = 0, alpha_dtype = 0, precision = 0;
+    korg_tuning tuning{};
+    Batch b;
+    double headroom = 1.25;  // tile-list entries allocated per entry measured
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* h_counters = nullptr;  // pinned; the graph's last node copies the counters
+    bool pending = false;                      // launched, counters not yet read
+    int n_captures = 0;
+};
+
+namespace {
+
+void plan_drop_graph(korg_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+void plan_free(korg_plan* p) {
+    if (!p) return;
+    korg_ctx* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    plan_drop_graph(p);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->h_counters) (void)hipHostFree(p->h_counters);
+    p->ws.release();
+    delete p;
+}
+
+// Captures the plan's pipeline -- front half, back half, and the copy of the counters to pinned
+// host memory -- into a graph and instantiates it.
+int plan_capture(korg_plan* p) {
+    korg_ctx* ctx = p->ctx;
+    plan_drop_graph(p);
+    HIP_TRY(ctx, hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_front(ctx, p->ws, p->b, p->cap_stream, p->ws.tile_cap, false);
+    if (rc == KORG_OK) rc = enqueue_back(ctx, p->ws, p->b, p->cap_stream, false);
+    if (rc == KORG_OK) {
+        const hipError_t e = hipMemcpyAsync(p->h_counters, p->ws.counters.p,
+                                            sizeof(unsigned long long) * N_COUNTERS,
+                                            hipMemcpyDeviceToHost, p->cap_stream);
+        if (e != hipSuccess)
+            rc = set_err(ctx, KORG_ERR_HIP, "hipMemcpyAsync of the counters failed: %s",
+                         hipGetErrorString(e));
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc != KORG_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (rc != KORG_OK) return rc;
+        return set_err(ctx, KORG_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    }
+    p->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    ++p->n_captures;
+    return KORG_OK;
+}
+
+// Shapes the batch, measures the per-tile line lists with one pass of the front half (Float32
+// form), allocates them with headroom, and captures the graph.
+int plan_build(korg_plan* p) {
+    korg_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    TRY(plan_batch(ctx, p->tuning, (int)p->stars.size(), p->stars.data(), &p->sp, p->L,
+                   p->alpha_dtype, p->precision, p->ws, s, p->b));
+    if (p->b.N == 0) return KORG_OK;
+    if (p->b.f32) {
+        TRY(enqueue_front(ctx, p->ws, p->b, s, INT64_MAX, false));
+        int64_t total = 0;
+        TRY(size_tile_lists(ctx, p->ws, p->b, s, p->headroom, &total));
+    }
+    return plan_capture(p);
+}
+
+int plan_launch(korg_plan* p) {
+    korg_ctx* ctx = p->ctx;
+    HIP_TRY(ctx, hipEventRecord(p->ev0, ctx->stream));
+    HIP_TRY(ctx, hipGraphLaunch(p->exec, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(p->ev1, ctx->stream));
+    p->pending = true;
+    return KORG_OK;
+}
+
+// Waits for the last replay and reads its counters.  A replay whose tile lists did not fit wrote
+// nothing: the lists are grown to the total it measured, the graph captured again and replayed.
+int plan_wait(korg_plan* p) {
+    korg_ctx* ctx = p->ctx;
+    if (!p->pending) return KORG_OK;
+    HIP_TRY(ctx, hipEventSynchronize(p->ev1));
+    p->pending = false;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    const unsigned long long flags = p->h_counters[3];
+    if ((flags & FLAG_CAPACITY) && !(flags & (FLAG_SPECIES | FLAG_UNSORTED))) {
+        const int64_t total = (int64_t)p->h_counters[13];
+        if (total > (int64_t)INT32_MAX)
+            return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                           "more than 2^31 (line, tile) pairs in one call; split the line list");
+        int64_t cap = (int64_t)((double)total * p->headroom) + 4096;
+        if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
+        TRY(grow(ctx, p->ws.tile_lines, sizeof(int) * (size_t)cap));
+        p->ws.tile_cap = cap;
+        TRY(plan_capture(p));
+        TRY(plan_launch(p));
+        HIP_TRY(ctx, hipEventSynchronize(p->ev1));
+        p->pending = false;
+    }
+    TRY(stats_from_counters(ctx, p->h_counters, p->b.f32, p->b.n_lblocks));
+    float t = 0;
+    if (hipEventElapsedTime(&t, p->ev0, p->ev1) == hipSuccess) ctx->stats.total_ms = t;
+    else (void)hipGetLastError();
+    return KORG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------------------------
+int ilog2(int64_t x) {
+    int k = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++k;
+    }
+    return k;
+}
+
+// Key of a tuning-cache entry: device, precision, alpha dtype, layers, and the power-of-two class
+// of the star, line and grid-point counts.
+std::string tune_key(const korg_ctx* ctx, int precision, int alpha_dtype, int L, int64_t n_stars,
+                     int64_t N, int64_t nwl) {
+    char k[256];
+    snprintf(k, sizeof k, "%s/cu%d/p%d/a%d/L%d/s%d/n%d/w%d", ctx->arch.c_str(), ctx->n_cu,
+             precision, alpha_dtype, L, ilog2(n_stars), ilog2(N), ilog2(nwl));
+    return k;
+}
+
+// Cache file: one entry per line, "key tile_points waves_per_tile waves_per_cu
+// near_waves_per_cu points_per_lane ms"; '#' starts a comment; the last entry of a key wins.
+bool cache_lookup(const char* path, const std::string& key, korg_tuning* out) {
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        char k[256];
+        int tp, wt, wc, nw, pl;
+        double ms;
+        if (sscanf(line, "%255s %d %d %d %d %d %lf", k, &tp, &wt, &wc, &nw, &pl, &ms) != 7) continue;
+        if (key != k) continue;
+        korg_tuning t{};
+        t.tile_points = tp;
+        t.waves_per_tile = wt;
+        t.waves_per_cu = wc;
+        t.near_waves_per_cu = nw;
+        t.points_per_lane = pl;
+        if (check_tuning(nullptr, &t) == KORG_OK) {
+            *out = t;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+int cache_store(korg_ctx* ctx, const char* path, const std::string& key, const korg_tuning& t,
+                double ms) {
+    FILE* f = fopen(path, "a");
+    if (!f) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "cannot append to tuning cache %s", path);
+    fprintf(f, "%s %d %d %d %d %d %.4f\n", key.c_str(), t.tile_points, t.waves_per_tile,
+            t.waves_per_cu, t.near_waves_per_cu, t.points_per_lane, ms);
+    fclose(f);
+    return KORG_OK;
+}
+
+// Device time of the eager pipeline under tuning t: the best of `reps` runs.
+int time_tuning(korg_ctx* ctx, const korg_tuning& t, const std::vector<StarHost>& hs,
+                const korg_species* sp, int L, int alpha_dtype, int precision, int reps,
+                double* ms) {
+    ctx->tuning = t;
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+        TRY(run_batch(ctx, (int)hs.size(), hs.data(), sp, L, alpha_dtype, precision));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_stats(ctx));
+        best = std::min(best, (double)ctx->stats.total_ms);
+    }
+    *ms = best;
+    return KORG_OK;
+}
+
+// Coordinate search: tile shape (Float32 form: points per tile x waves per tile; Float64 form:
+// points per lane), then persistent waves per CU, then (Float32) near-zone waves per CU.
+int autotune(korg_ctx* ctx, const std::vector<StarHost>& hs, const korg_species* sp, int L,
+             int alpha_dtype, int precision, korg_tuning* best, double* best_ms) {
+    const bool f32 = precision == KORG_PRECISION_FP32_FIXUP;
+    korg_tuning cur{};
+    cur.waves_per_cu = 12;
+    cur.near_waves_per_cu = 16;
+    double bms = 1e300;
+    korg_tuning bt = cur;
+    auto consider = [&](const korg_tuning& t) -> int {
+        double ms = 0;
+        TRY(time_tuning(ctx, t, hs, sp, L, alpha_dtype, precision, 2, &ms));
+        if (ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+        return KORG_OK;
+    };
+    if (f32) {
+        const int shapes[][2] = {{32, 1}, {32, 2}, {32, 4}, {64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}};
+        for (const auto& sh : shapes) {
+            korg_tuning t = cur;
+            t.tile_points = sh[0];
+            t.waves_per_tile = sh[1];
+            TRY(consider(t));
+        }
+    } else {
+        for (int pl : {1, 2}) {
+            korg_tuning t = cur;
+            t.points_per_lane = pl;
+            TRY(consider(t));
+        }
+    }
+    const korg_tuning shape = bt;
+    for (int wc : {8, 16, 24}) {
+        korg_tuning t = shape;
+        t.waves_per_cu = wc;
+        TRY(consider(t));
+    }
+    if (f32) {
+        const korg_tuning occ = bt;
+        for (int nw : {8, 32}) {
+            korg_tuning t = occ;
+            t.near_waves_per_cu = nw;
+            TRY(consider(t));
+        }
+    }
+    *best = bt;
+    *best_ms = bms;
+    return KORG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int korg_abi_version(void) { return KORG_LINE_ABI_VERSION; }
@@ -2013,6 +2464,9 @@ int korg_create(korg_ctx** out, int device) {
         return KORG_ERR_HIP;
     }
     ctx->n_cu = prop.multiProcessorCount;
+    ctx->arch = prop.gcnArchName;
+    const size_t colon = ctx->arch.find(':');
+    if (colon != std::string::npos) ctx->arch.resize(colon);
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return KORG_ERR_HIP;
@@ -2031,14 +2485,11 @@ int korg_create(korg_ctx** out, int device) {
 void korg_destroy(korg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    DevBuf* bufs[] = {&ctx->meta, &ctx->rec64, &ctx->recM, &ctx->gridinfo, &ctx->tab, &ctx->counters, &ctx->blk,
-                      &ctx->cstart, &ctx->cls_cand, &ctx->cls_jc, &ctx->ranges, &ctx->nblk, &ctx->ncstart, &ctx->near_cand, &ctx->near_jc, &ctx->near_ranges, &ctx->cls_rng,
-                      &ctx->cls_line, &ctx->near_rng, &ctx->near_line, &ctx->tc_count, &ctx->tc_start,
-                      &ctx->tile_lines, &ctx->vjp_cnt, &ctx->vjp_start, &ctx->vjp_contrib,
-                      &ctx->vjp_species, &ctx->vjp_out, &ctx->h_grad, &ctx->stars, &ctx->h_alpha,
-                      &ctx->h_wls, &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots,
-                      &ctx->h_cntm};
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws.release();
+    DevBuf* bufs[] = {&ctx->vjp_cnt, &ctx->vjp_start, &ctx->vjp_contrib, &ctx->vjp_species,
+                      &ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls,
+                      &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
@@ -2073,11 +2524,14 @@ int korg_get_stats(const korg_ctx* ctx, korg_stats* out) {
 
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* t) {
     if (!ctx || !t) return KORG_ERR_INVALID_ARGUMENT;
-    if (t->points_per_lane < 0 || t->points_per_lane > 2 || t->waves_per_cu < 0 ||
-        t->waves_per_cu > 64 || t->waves_per_tile != 0 || t->count_branches < 0 ||
-        t->count_branches > 1)
-        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "tuning out of range");
+    TRY(check_tuning(ctx, t));
     ctx->tuning = *t;
+    return KORG_OK;
+}
+
+int korg_get_tuning(const korg_ctx* ctx, korg_tuning* out) {
+    if (!ctx || !out) return KORG_ERR_INVALID_ARGUMENT;
+    *out = ctx->tuning;
     return KORG_OK;
 }
 
@@ -2104,26 +2558,9 @@ int korg_line_absorption_batched(korg_ctx* ctx, int64_t n_stars, const korg_star
                                  int synchronize) {
     if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
     ctx->err.clear();
-    if (n_stars < 1 || !stars) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no stars");
-    if (n_stars > 0x7fffffff) return set_err(ctx, KORG_ERR_UNSUPPORTED, "too many stars");
-    std::vector<StarHost> hs((size_t)n_stars);
-    const int L = stars[0].layers.n_layers;
-    for (int64_t k = 0; k < n_stars; ++k) {
-        const korg_star& x = stars[k];
-        const int rc = validate(ctx, &x.grid, &x.lines, species, &x.layers, &x.cntm, alpha_dtype,
-                                precision);
-        if (rc != KORG_OK) {
-            ctx->err = "star " + std::to_string(k) + ": " + ctx->err;
-            return rc;
-        }
-        if (x.layers.n_layers != L)
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                           "star %lld has %d layers, star 0 has %d: one layer count per batch",
-                           (long long)k, x.layers.n_layers, L);
-        if (!x.alpha && x.lines.n_lines > 0)
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: alpha is null", (long long)k);
-        hs[k] = StarHost{x.alpha, x.grid, x.lines, x.layers, x.cntm, x.cutoff_threshold};
-    }
+    std::vector<StarHost> hs;
+    int L = 0;
+    TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     TRY(run_batch(ctx, (int)n_stars, hs.data(), species, L, alpha_dtype, precision));
     if (synchronize) {
@@ -2198,7 +2635,7 @@ int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dty
     if (!grad_alpha && lines->n_lines > 0)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grad_alpha is null");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int64_t N = lines->n_lines, L = layers->n_layers, S = species->n_species;
+    const int64_t N = lines->n_lines, L = layers->n_layers;
     korg_grid g = *grid;
     korg_lines ln = *lines;
     korg_species sp = *species;
@@ -2213,7 +2650,6 @@ int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dty
                                     ctx->stream));
         Gd = ctx->h_grad.p;
     }
-    (void)S;
     TRY(run_vjp(ctx, Gd, grad_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold, grads,
                 hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -2228,11 +2664,155 @@ int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {
     if (n_lines == 0) return KORG_OK;
     std::vector<LineMeta> m(n_lines);
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipMemcpy(m.data(), ctx->meta.p, sizeof(LineMeta) * n_lines, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(m.data(), ctx->ws.meta.p, sizeof(LineMeta) * n_lines,
+                           hipMemcpyDeviceToHost));
+    // grid indices are global over the call's stars: make them 1-based within each line's star
+    const auto& lo = ctx->last_line_off;
+    const auto& go = ctx->last_grid_off;
+    size_t k = 0;
     for (int64_t i = 0; i < n_lines; ++i) {
-        lb[i] = (int64_t)m[i].lb + 1;
-        ub[i] = (int64_t)m[i].ub + 1;
+        while (k + 1 < lo.size() && lo[k + 1] <= i) ++k;
+        const int64_t off = k < go.size() ? go[k] : 0;
+        lb[i] = (int64_t)m[i].lb - off + 1;
+        ub[i] = (int64_t)m[i].ub - off + 1;
     }
+    return KORG_OK;
+}
+
+int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                      const korg_species* species, int alpha_dtype, int precision,
+                      double headroom, korg_plan** out) {
+    if (!ctx || !out) return KORG_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    ctx->err.clear();
+    if (!(headroom == 0.0 || (headroom >= 1.0 && headroom <= 16.0)))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "headroom must be 0 or in [1, 16]");
+    korg_plan* p = new korg_plan();
+    p->ctx = ctx;
+    int rc = collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, p->stars, &p->L);
+    if (rc != KORG_OK) {
+        delete p;
+        return rc;
+    }
+    p->sp = *species;
+    p->alpha_dtype = alpha_dtype;
+    p->precision = precision;
+    p->tuning = ctx->tuning;
+    p->headroom = headroom > 0.0 ? headroom : 1.25;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev1);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&p->h_counters, sizeof(unsigned long long) * N_COUNTERS,
+                          hipHostMallocDefault);
+    if (e != hipSuccess) {
+        rc = set_err(ctx, KORG_ERR_HIP, "plan resources: %s", hipGetErrorString(e));
+    } else {
+        std::memset(p->h_counters, 0, sizeof(unsigned long long) * N_COUNTERS);
+        rc = plan_build(p);
+    }
+    if (rc != KORG_OK) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return KORG_OK;
+}
+
+int korg_plan_launch(korg_plan* p, int synchronize) {
+    if (!p) return KORG_ERR_INVALID_ARGUMENT;
+    korg_ctx* ctx = p->ctx;
+    ctx->err.clear();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (p->b.N == 0) {
+        std::memset(&ctx->stats, 0, sizeof ctx->stats);
+        return KORG_OK;
+    }
+    // the previous replay's counters are read before this one overwrites them
+    if (p->pending) TRY(plan_wait(p));
+    TRY(plan_launch(p));
+    if (synchronize) TRY(plan_wait(p));
+    return KORG_OK;
+}
+
+int korg_plan_wait(korg_plan* p) {
+    if (!p) return KORG_ERR_INVALID_ARGUMENT;
+    p->ctx->err.clear();
+    HIP_TRY(p->ctx, hipSetDevice(p->ctx->device));
+    return plan_wait(p);
+}
+
+int korg_plan_get_info(const korg_plan* p, korg_plan_info* out) {
+    if (!p || !out) return KORG_ERR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof *out);
+    out->n_stars = (int64_t)p->stars.size();
+    out->n_lines = p->b.N;
+    out->n_points = p->b.nwl;
+    out->tile_capacity = p->ws.tile_cap;
+    out->tile_entries = p->b.f32 ? (int64_t)p->h_counters[13] : 0;
+    out->n_captures = p->n_captures;
+    out->tuning = p->tuning;
+    return KORG_OK;
+}
+
+void korg_plan_destroy(korg_plan* p) { plan_free(p); }
+
+int korg_autotune_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                          const korg_species* species, int alpha_dtype, int precision,
+                          const char* cache_path, korg_tuning* out) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    std::vector<StarHost> hs;
+    int L = 0;
+    TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
+    int64_t N = 0, nwl = 0;
+    for (const StarHost& x : hs) {
+        N += x.lines.n_lines;
+        nwl += x.grid.n_wl;
+    }
+    const korg_tuning keep = ctx->tuning;
+    const std::string key = tune_key(ctx, precision, alpha_dtype, L, n_stars, N, nwl);
+    korg_tuning best{};
+    if (cache_path && cache_lookup(cache_path, key, &best)) {
+        best.count_branches = keep.count_branches;
+        ctx->tuning = best;
+        if (out) *out = best;
+        return KORG_OK;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (N == 0) {  // nothing to time
+        if (out) *out = keep;
+        return KORG_OK;
+    }
+    // the candidates add into a scratch copy of the alphas, never into the caller's
+    const size_t esz = alpha_dtype == KORG_DTYPE_F32 ? 4 : 8;
+    size_t bytes = 0;
+    for (const StarHost& x : hs) bytes += esz * (size_t)L * (size_t)x.grid.n_wl;
+    DevBuf scratch;
+    TRY(grow(ctx, scratch, bytes));
+    size_t off = 0;
+    for (StarHost& x : hs) {
+        x.alpha = (char*)scratch.p + off;
+        off += esz * (size_t)L * (size_t)x.grid.n_wl;
+    }
+    int rc = KORG_OK;
+    hipError_t e = hipMemsetAsync(scratch.p, 0, bytes, ctx->stream);
+    if (e != hipSuccess) rc = set_err(ctx, KORG_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    double ms = 0;
+    if (rc == KORG_OK) rc = autotune(ctx, hs, species, L, alpha_dtype, precision, &best, &ms);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(scratch.p);
+    ctx->ws.release();  // sized for the largest candidate; the next call grows what it needs
+    ctx->last_n_lines = 0;
+    ctx->tuning = keep;
+    if (rc != KORG_OK) return rc;
+    best.count_branches = keep.count_branches;
+    if (cache_path) TRY(cache_store(ctx, cache_path, key, best, ms));
+    ctx->tuning = best;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    ctx->stats.total_ms = ms;
+    if (out) *out = best;
     return KORG_OK;
 }
 

@@ -16,6 +16,7 @@ KORG_ERR_HIP = 2
 KORG_ERR_OUT_OF_MEMORY = 3
 KORG_ERR_NO_DEVICE = 4
 KORG_ERR_UNSUPPORTED = 5
+KORG_ERR_CAPACITY = 6
 
 PRECISION_FP64 = 0
 PRECISION_FP32_FIXUP = 1
@@ -74,16 +75,26 @@ class KorgStar(ctypes.Structure):
 
 class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
-                ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32)]
+                ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32),
+                ("tile_points", ctypes.c_int32), ("near_waves_per_cu", ctypes.c_int32)]
 
 
-ABI_VERSION = 5  # KORG_LINE_ABI_VERSION of include/korg_line.h
+class KorgPlanInfo(ctypes.Structure):
+    _fields_ = [("n_stars", ctypes.c_int64), ("n_lines", ctypes.c_int64),
+                ("n_points", ctypes.c_int64), ("tile_capacity", ctypes.c_int64),
+                ("tile_entries", ctypes.c_int64), ("n_captures", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("tuning", KorgTuning)]
+
+
+ABI_VERSION = 6  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
             "korg_synchronize", "korg_get_stats", "korg_get_windows", "korg_set_tuning",
             "korg_line_absorption_vjp", "korg_line_absorption_vjp_device",
-            "korg_line_absorption_batched"]
+            "korg_line_absorption_batched", "korg_get_tuning", "korg_plan_batched",
+            "korg_plan_launch", "korg_plan_wait", "korg_plan_get_info", "korg_plan_destroy",
+            "korg_autotune_batched"]
 
 _lib = None
 
@@ -139,6 +150,23 @@ def load():
     L.korg_line_absorption_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
                                                ctypes.POINTER(KorgSpecies), i32, i32, i32]
     L.korg_line_absorption_batched.restype = i32
+    L.korg_get_tuning.argtypes = [vp, ctypes.POINTER(KorgTuning)]
+    L.korg_get_tuning.restype = i32
+    L.korg_plan_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar), ctypes.POINTER(KorgSpecies),
+                                    i32, i32, d, ctypes.POINTER(vp)]
+    L.korg_plan_batched.restype = i32
+    L.korg_plan_launch.argtypes = [vp, i32]
+    L.korg_plan_launch.restype = i32
+    L.korg_plan_wait.argtypes = [vp]
+    L.korg_plan_wait.restype = i32
+    L.korg_plan_get_info.argtypes = [vp, ctypes.POINTER(KorgPlanInfo)]
+    L.korg_plan_get_info.restype = i32
+    L.korg_plan_destroy.argtypes = [vp]
+    L.korg_plan_destroy.restype = None
+    L.korg_autotune_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
+                                        ctypes.POINTER(KorgSpecies), i32, i32, ctypes.c_char_p,
+                                        ctypes.POINTER(KorgTuning)]
+    L.korg_autotune_batched.restype = i32
     if L.korg_abi_version() != ABI_VERSION:
         raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
                           f"expect {ABI_VERSION}; rebuild it")

@@ -55,13 +55,7 @@ class DeviceProblem:
         P = problem
         ll = P.lines
         self.problem = P
-        self.t = {k: DeviceArray.from_numpy(v) for k, v in {
-            "all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
-            "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
-            "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
-            "is_mol": P.species_is_mol, "n_div_U": P.n_div_U, "temps": P.temps, "n_e": P.n_e,
-            "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
-            "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}.items()}
+        self.t = {k: DeviceArray.from_numpy(v) for k, v in self._arrays(P).items()}
         t = self.t
         d = lambda k: t[k].ptr
         self.grid = _lib.KorgGrid(P.n_wl, d("all_wls"))
@@ -72,6 +66,31 @@ class DeviceProblem:
                                       d("n_HeI"), d("n_H2"), d("xi"))
         self.cntm = _lib.KorgContinuum(P.cntm_mode, P.n_knots, d("knots") if P.n_knots else None,
                                        d("cntm"))
+
+    def _arrays(self, P):
+        ll = P.lines
+        return {"all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
+                "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
+                "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
+                "is_mol": P.species_is_mol, "n_div_U": P.n_div_U, "temps": P.temps, "n_e": P.n_e,
+                "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
+                "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}
+
+    def update(self, problem):
+        """Writes ``problem`` (same sizes and continuum mode) into this problem's device buffers,
+        so that a ``Plan`` captured over them sees the new inputs at its next replay."""
+        P = problem
+        new = self._arrays(P)
+        if (P.n_wl, P.n_layers, len(P.lines), P.cntm_mode, P.n_knots) != (
+                self.problem.n_wl, self.problem.n_layers, len(self.problem.lines),
+                self.problem.cntm_mode, self.problem.n_knots):
+            raise ValueError("update needs a problem of the same sizes and continuum mode")
+        for k, v in new.items():
+            if v.shape != self.t[k].shape or v.dtype != self.t[k].dtype:
+                raise ValueError(f"update: {k} changes shape or dtype")
+        for k, v in new.items():
+            self.t[k].copy_from(v)
+        self.problem = P
 
     def new_alpha(self, dtype="float32", fill=None):
         """Device alpha in the library's layout: a C-ordered (n_wl, n_layers) buffer is the
@@ -113,9 +132,17 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def set_tuning(self, points_per_lane=0, waves_per_cu=0, count_branches=0):
-        t = _lib.KorgTuning(points_per_lane, 0, waves_per_cu, count_branches)
+    def set_tuning(self, points_per_lane=0, waves_per_cu=0, count_branches=0, tile_points=0,
+                   waves_per_tile=0, near_waves_per_cu=0):
+        """Kernel tuning (0 = library default); see ``korg_tuning`` in include/korg_line.h."""
+        t = _lib.KorgTuning(points_per_lane, waves_per_tile, waves_per_cu, count_branches,
+                            tile_points, near_waves_per_cu)
         _lib.check(self._h, self._L.korg_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = _lib.KorgTuning()
+        _lib.check(self._h, self._L.korg_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
 
     def set_stream(self, stream_handle):
         _lib.check(self._h, self._L.korg_set_stream(self._h, ctypes.c_void_p(stream_handle)))
@@ -167,10 +194,9 @@ class Context:
             1 if synchronize else 0)
         _lib.check(self._h, rc)
 
-    def line_absorption_batched(self, alphas, dprobs, precision="fp32", synchronize=True):
-        """Many stars in one pipeline: ``alphas[k]`` (a ``DeviceProblem.new_alpha`` array) receives
-        the line opacity of ``dprobs[k]``.  Every problem must share one species table and one
-        layer count; all alphas have one dtype."""
+    @staticmethod
+    def _stars(alphas, dprobs):
+        """The korg_star array of a batch, and its alpha dtype code."""
         if len(alphas) != len(dprobs) or not dprobs:
             raise ValueError("one alpha per problem, at least one star")
         dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
@@ -189,11 +215,34 @@ class Context:
         for k, (d, a) in enumerate(zip(dprobs, alphas)):
             stars[k] = _lib.KorgStar(a.ptr, d.grid, d.lines, d.layers, d.cntm,
                                      d.problem.cutoff_threshold)
+        return stars, dt[next(iter(dtypes))]
+
+    def line_absorption_batched(self, alphas, dprobs, precision="fp32", synchronize=True):
+        """Many stars in one pipeline: ``alphas[k]`` (a ``DeviceProblem.new_alpha`` array) receives
+        the line opacity of ``dprobs[k]``.  Every problem must share one species table and one
+        layer count; all alphas have one dtype."""
+        stars, dtc = self._stars(alphas, dprobs)
         rc = self._L.korg_line_absorption_batched(self._h, len(dprobs), stars,
-                                                  ctypes.byref(dprobs[0].species),
-                                                  dt[next(iter(dtypes))], PRECISIONS[precision],
-                                                  1 if synchronize else 0)
+                                                  ctypes.byref(dprobs[0].species), dtc,
+                                                  PRECISIONS[precision], 1 if synchronize else 0)
         _lib.check(self._h, rc)
+
+    def plan_batched(self, alphas, dprobs, precision="fp32", headroom=0.0):
+        """A ``Plan``: the batched call captured in a HIP graph for these device buffers."""
+        return Plan(self, alphas, dprobs, precision, headroom)
+
+    def autotune_batched(self, alphas, dprobs, precision="fp32", cache_path=None):
+        """Times the tuning candidates on these stars (into scratch alphas), makes the fastest
+        this context's tuning and returns it; ``cache_path``: optional results file."""
+        stars, dtc = self._stars(alphas, dprobs)
+        t = _lib.KorgTuning()
+        rc = self._L.korg_autotune_batched(self._h, len(dprobs), stars,
+                                           ctypes.byref(dprobs[0].species), dtc,
+                                           PRECISIONS[precision],
+                                           None if cache_path is None else str(cache_path).encode(),
+                                           ctypes.byref(t))
+        _lib.check(self._h, rc)
+        return {k: getattr(t, k) for k, _ in t._fields_}
 
     def line_absorption_vjp(self, G, problem):
         """Adjoint on host arrays: given ``G`` = dL/dalpha (float64/float32, (n_layers, n_wl),
@@ -244,6 +293,52 @@ class Context:
 
 
 _default_ctx = None
+
+
+class Plan:
+    """A batched forward call captured once in a HIP graph (``korg_plan_batched``) and replayed
+    by ``launch``.  Holds its alphas, problems and context, whose device buffers the graph uses;
+    new inputs are written into those buffers between replays."""
+
+    def __init__(self, ctx, alphas, dprobs, precision="fp32", headroom=0.0):
+        stars, dtc = ctx._stars(alphas, dprobs)
+        self.ctx, self.alphas, self.dprobs = ctx, list(alphas), list(dprobs)
+        h = ctypes.c_void_p()
+        rc = ctx._L.korg_plan_batched(ctx._h, len(dprobs), stars, ctypes.byref(dprobs[0].species),
+                                      dtc, PRECISIONS[precision], float(headroom), ctypes.byref(h))
+        _lib.check(ctx._h, rc)
+        self._p = h
+
+    def launch(self, synchronize=True):
+        """Replays the graph (adds into the alphas); with synchronize, waits and reads the counters."""
+        _lib.check(self.ctx._h, self.ctx._L.korg_plan_launch(self._p, 1 if synchronize else 0))
+
+    def wait(self):
+        _lib.check(self.ctx._h, self.ctx._L.korg_plan_wait(self._p))
+
+    def info(self):
+        i = _lib.KorgPlanInfo()
+        _lib.check(self.ctx._h, self.ctx._L.korg_plan_get_info(self._p, ctypes.byref(i)))
+        out = {k: getattr(i, k) for k, _ in i._fields_ if k not in ("pad", "tuning")}
+        out["tuning"] = {k: getattr(i.tuning, k) for k, _ in i.tuning._fields_}
+        return out
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self.ctx._L.korg_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def default_context():

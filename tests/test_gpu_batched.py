@@ -72,10 +72,34 @@ def test_batched_fp64_matches_oracle(ctx):
     dprobs = [DeviceProblem(P) for P, _, _ in stars]
     alphas = [d.new_alpha("float64") for d in dprobs]
     ctx.line_absorption_batched(alphas, dprobs, "fp64")
+    lb, ub = ctx.windows(sum(len(P.lines) for P, _, _ in stars))  # 1-based within each star
+    off = 0
     for (P, _, _), a in zip(stars, alphas):
         ref = np.zeros((P.n_layers, P.n_wl), order="F")
-        oracle.line_absorption(ref, P)
+        rlb, rub, _ = oracle.line_absorption(ref, P, return_windows=True)
         np.testing.assert_allclose(a.numpy().T, ref, rtol=1e-12, atol=1e-300)
+        n = len(P.lines)
+        used = rlb <= rub
+        assert np.array_equal(used, lb[off:off + n] <= ub[off:off + n])
+        assert np.array_equal(rlb[used], lb[off:off + n][used])
+        assert np.array_equal(rub[used], ub[off:off + n][used])
+        off += n
+
+
+def test_batched_fp32_random_atmospheres_match_oracle(ctx):
+    """Random atmospheres reach damping parameters of 1e6 and more in deep layers (star 5 here):
+    every star's Float32 alpha is finite and within 1e-4 of the Float64 oracle."""
+    from test_gpu_parity import FP32_RTOL, rel_err
+    stars = workloads.batched_stars(n_stars=6, seed=21, scale=0.04)
+    dprobs = [DeviceProblem(P) for P, _, _ in stars]
+    alphas = [d.new_alpha("float32") for d in dprobs]
+    ctx.line_absorption_batched(alphas, dprobs, "fp32")
+    for k, ((P, _, _), a) in enumerate(zip(stars, alphas)):
+        ref = np.zeros((P.n_layers, P.n_wl), order="F")
+        oracle.line_absorption(ref, P)
+        got = a.numpy().T
+        assert np.all(np.isfinite(got)), k
+        assert rel_err(got, ref) <= FP32_RTOL, (k, rel_err(got, ref))
 
 
 def test_batched_rejects_mismatched_layers(ctx):

@@ -215,6 +215,27 @@ def test_molecules_abo_and_missing_perturbers(ctx):
     check_fp32(ctx, P, ref)
 
 
+def test_extreme_damping_lines(ctx):
+    """Radiative damping up to 1e18 s^-1 (a ~ 1e3 .. 1e7): past a ~ 1e4 the far-wing series
+    coefficients leave Float32 range and the Float32 form evaluates the line exactly over its
+    whole window.  Finite everywhere and within the Float32 tolerance of the oracle."""
+    atm = Atmosphere.from_model("sun")
+    wls = Wavelengths((5000, 5010))
+    lines = [Line(5000.5 + 0.9 * k, 2.0, "Fe I", 1.0, g, 1e-5, 1e-7)
+             for k, g in enumerate([1e12, 1e14, 1e15, 1e16, 1e17, 1e18])]
+    lines += workloads.synthetic_linelist(31, 200, 4995.0, 5015.0, (-4.0, 0.5), (0.0, 5.0)).to_lines()
+    ll = LineList.from_lines(sorted(lines, key=lambda x: x.wl))
+    knots, vals = continuum_grid(atm, float(wls.all_wls[0]), float(wls.all_wls[-1]))
+    P = Problem.from_korg_args(ll, wls, atm.temps, atm.n_e, atm.n_densities, atm.partition_fns,
+                               1e5, ContinuumGrid(knots, vals))
+    ref, lb, ub, _ = oracle_alpha(P, windows=True)
+    broad = np.isin(ll.gamma_rad, [1e16, 1e17, 1e18])
+    assert np.sum(lb[broad] <= ub[broad]) >= 2  # the extreme lines have windows
+    got = gpu_alpha(ctx, P, "fp32")
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= FP32_RTOL, rel_err(got, ref)
+
+
 # ----------------------------------------------------------------------------- full size
 def test_solar_wide_full_size_properties(ctx):
     """300,000 lines x 300,001 points x 56 layers: windows identical to the oracle's window pass
