@@ -3,9 +3,16 @@
 Workload (the headline configuration): solar stand-in atmosphere (56 layers), 4000-7000 A at
 0.01 A (300,001 points), 300,000 seeded synthetic lines; alpha is 56 x 300,001 Float32 in HBM.
 A step is one full line_absorption! pass (setup, binning, profile accumulation) over
-device-resident inputs.  Prints one JSON line.
+device-resident inputs, replayed from a HIP graph under the autotuned tiling; per-kernel times
+come from HIP events around the same kernels in direct calls.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+``--workload batched`` measures the batched form instead (one JSON line of its own): 256 stars,
+Teff ~ U[4000, 6500] K, log g ~ U[1, 5], [M/H] ~ U[-2.5, 0.5] (scaled solar stand-ins), each with
+a window of U[50, 1000] A inside 4000-7000 A at 0.01 A and its own list of U[5k, 150k] lines;
+FP32 alpha; the tuning from the autotuner (its results file ships in the package); the whole
+pipeline captured in a HIP graph, a step being one replay over all 256 stars.
 """
 import argparse
 import json
@@ -51,6 +58,11 @@ def main():
     ap.add_argument("--no-flux", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (PMC byte-counter scale)")
+    ap.add_argument("--workload", choices=("solar_wide", "batched"), default="solar_wide")
+    ap.add_argument("--stars", type=int, default=256, help="batched workload: number of stars")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="batched workload: autotuner results file (default: the one shipped "
+                         "in the package; a file without this shape's entry is timed and appended)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -60,6 +72,9 @@ def main():
               "path shards by wavelength tiles or by star, which this build does not run.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import numpy as np
     from korg_line import Context, DeviceProblem, rt, workloads
@@ -72,38 +87,65 @@ def main():
     ctx = Context(0)
     D = DeviceProblem(P)
     a32 = D.new_alpha("float32")
+    a64 = D.new_alpha("float64")
     device_sync()
+    cache = args.tuning_cache or TUNING_CACHE
+    tune32 = ctx.autotune_batched([a32], [D], "fp32", cache_path=cache)
+    tune64 = ctx.autotune_batched([a64], [D], "fp64", cache_path=cache)
+    log(f"tuning: fp32 {tune32}, fp64 {tune64}")
 
-    def run(alpha, precision, steps, warmup):
+    def use(tuning):
+        ctx.set_tuning(**{k: v for k, v in tuning.items() if k != "count_branches"})
+
+    def direct(alpha, precision, steps, warmup):
+        """Direct calls (events around every kernel): the per-kernel device times."""
         for _ in range(warmup):
             ctx.line_absorption_device(alpha, D, precision, synchronize=True)
-        device_sync()
-        ctx.synchronize()
         stats = []
-        t0 = time.perf_counter()
         for _ in range(steps):
             ctx.line_absorption_device(alpha, D, precision, synchronize=True)
             stats.append(ctx.stats())
-        ctx.synchronize()
         device_sync()
-        return (time.perf_counter() - t0) / steps, stats
+        return stats
+
+    def replay(alpha, precision, steps, warmup):
+        """The timed steps: replays of the whole pass captured in a HIP graph."""
+        with ctx.plan_batched([alpha], [D], precision) as plan:
+            for _ in range(warmup):
+                plan.launch()
+            device_sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                plan.launch()
+            device_sync()
+            return (time.perf_counter() - t0) / steps
 
     log("timing the Float32 form ...")
-    t32, st32 = run(a32, "fp32", args.steps, args.warmup)
-    log(f"  {t32 * 1e3:.3f} ms/step")
+    use(tune32)
+    st32 = direct(a32, "fp32", args.steps, args.warmup)
+    t32 = replay(a32, "fp32", args.steps, args.warmup)
+    direct32 = float(np.mean([s["total_ms"] for s in st32]))
+    log(f"  {t32 * 1e3:.3f} ms/step (graph replay); direct calls {direct32:.3f} ms device time")
     log("timing the Float64 form ...")
-    a64 = D.new_alpha("float64")
-    t64, st64 = run(a64, "fp64", args.fp64_steps, 1)
+    use(tune64)
+    st64 = direct(a64, "fp64", 1, 1)
+    t64 = replay(a64, "fp64", args.fp64_steps, 1)
     log(f"  {t64 * 1e3:.3f} ms/step")
+    use(tune32)
 
     # branch mix (untimed) and accuracy of one fresh pass of each form
-    ctx.set_tuning(count_branches=1)
     z32 = D.new_alpha("float32")
     z64 = D.new_alpha("float64")
+    use(tune64)
     ctx.line_absorption_device(z64, D, "fp64")
+    use(tune32)
     ctx.line_absorption_device(z32, D, "fp32")
+    ctx.set_tuning(**{**{k: v for k, v in tune32.items() if k != "count_branches"},
+                      "count_branches": 1})
+    zb = D.new_alpha("float32")
+    ctx.line_absorption_device(zb, D, "fp32")
     sb = ctx.stats()
-    ctx.set_tuning()
+    use(tune32)
     nA, nB, nC, nD = sb["n_evals_branch"]
     E = sb["n_evals"]
     lines64 = z64.numpy().T  # (layers, points)
@@ -157,7 +199,10 @@ def main():
                         "points), 56-layer solar stand-in atmosphere, FP32 alpha resident in HBM",
             "precision": "fp32 profiles + fp64 setup/window/branch/offset fix-ups",
             "parallelism": "single GPU",
+            "tuning": tune32,
+            "step": "one HIP graph replay of the whole pass (setup, binning, profiles)",
         },
+        "direct_call_ms": direct32,
         "voigt_evals_per_s": E / t32,
         "n_evals": E,
         "evals_by_branch": {"A_far_wing": nA, "B_core": nB, "C_core": nC, "D_damped_wing": nD},
@@ -188,6 +233,149 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+TUNING_CACHE = os.path.join(ROOT, "korg.jl_amd", "korg_line", "tuning_mi355x.txt")
+
+
+def bench_batched(args):
+    """The batched forward over many stars as one graph replay per step."""
+    import numpy as np
+    from korg_line import Context, DeviceProblem, rt, workloads
+    from korg_line.hipmem import synchronize as device_sync
+
+    t = time.time()
+    stars = workloads.batched_stars(n_stars=args.stars, seed=3)
+    probs = [P for P, _, _ in stars]
+    n_lines = sum(len(P.lines) for P in probs)
+    n_pts = sum(P.n_wl for P in probs)
+    L = probs[0].n_layers
+    log(f"{len(probs)} stars built in {time.time() - t:.1f}s: {n_lines} lines, {n_pts} points, "
+        f"{L} layers ({n_pts * L * 4 / 1e9:.2f} GB of FP32 alpha)")
+    ctx = Context(0)
+    dprobs = [DeviceProblem(P) for P in probs]
+    a32 = [d.new_alpha("float32") for d in dprobs]
+    device_sync()
+    t = time.time()
+    tuning = ctx.autotune_batched(a32, dprobs, "fp32", cache_path=args.tuning_cache or TUNING_CACHE)
+    log(f"tuning {tuning} ({time.time() - t:.1f}s)")
+    t = time.time()
+    plan = ctx.plan_batched(a32, dprobs, "fp32")
+    log(f"plan captured in {time.time() - t:.2f}s: {plan.info()}")
+
+    def replay(steps):
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.launch(synchronize=True)
+            stats.append(ctx.stats())
+        return (time.perf_counter() - t0) / steps, stats
+
+    replay(args.warmup)
+    device_sync()
+    t32, st32 = replay(args.steps)
+    device_sync()
+    log(f"  FP32 graph replay: {t32 * 1e3:.2f} ms/step ({len(probs) / t32:.1f} spectra/s)")
+    E = st32[-1]["n_evals"]
+    stream_ms = float(np.mean([s["total_ms"] for s in st32]))
+
+    # the all-FP64 form of the same batch (direct call), and its accuracy against it
+    ctx.set_tuning()
+    a64 = [d.new_alpha("float64") for d in dprobs]
+    ctx.line_absorption_batched(a64, dprobs, "fp64")
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        ctx.line_absorption_batched(a64, dprobs, "fp64")
+    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    log(f"  FP64 direct: {t64 * 1e3:.1f} ms/step")
+    # accuracy on fresh results of every 16th star (alpha and normalized flux)
+    ctx.set_tuning(**{k: v for k, v in tuning.items() if k != "count_branches"})
+    sample = list(range(0, len(probs), 16))
+    z32 = [dprobs[k].new_alpha("float32") for k in sample]
+    z64 = [dprobs[k].new_alpha("float64") for k in sample]
+    ctx.line_absorption_batched(z32, [dprobs[k] for k in sample], "fp32")
+    ctx.line_absorption_batched(z64, [dprobs[k] for k in sample], "fp64")
+    alpha_err = flux_err = 0.0
+    for k, b32, b64 in zip(sample, z32, z64):
+        P, atm, wls = stars[k]
+        l64 = b64.numpy().T
+        l32 = b32.numpy().astype(np.float64).T
+        m = np.abs(l64) >= 1e-8 * np.abs(l64).max()
+        if m.any():
+            alpha_err = max(alpha_err, float(np.max(np.abs(l32[m] - l64[m]) / np.abs(l64[m]))))
+        cntm = np.array([np.interp(wls.all_wls, P.cntm_knots, P.cntm_values[l]) for l in range(L)])
+        aref = np.array([np.interp(5000e-8, P.cntm_knots, P.cntm_values[l]) for l in range(L)])
+        tau = 10.0 ** atm.lgTau5
+        f64 = rt.normalized_flux(cntm + l64, cntm, aref, tau, atm.temps, wls.all_wls)
+        f32 = rt.normalized_flux(cntm + l32, cntm, aref, tau, atm.temps, wls.all_wls)
+        flux_err = max(flux_err, float(np.max(np.abs(f32 - f64))))
+    log(f"  sample of {len(sample)} stars: alpha max rel err {alpha_err:.3e}, flux {flux_err:.3e}")
+
+    cpu = None if args.no_cpu else cpu_baseline_batched(stars, E / len(probs))
+    out = {
+        "metric": METRIC,
+        "value": len(probs) / t32,
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {len(probs)} stars (Teff U[4000,6500], log g U[1,5], "
+                        f"[M/H] U[-2.5,0.5]), windows U[50,1000] A in 4000-7000 A at 0.01 A, "
+                        f"U[5k,150k] lines each ({n_lines} lines, {n_pts} points), 56 layers, "
+                        "FP32 alpha in HBM, one HIP graph replay per step",
+            "precision": "fp32 profiles + fp64 setup/window/branch/offset fix-ups",
+            "parallelism": "single GPU",
+            "tuning": tuning,
+        },
+        "voigt_evals_per_s": E / t32,
+        "n_evals": E,
+        "replay_stream_ms": stream_ms,
+        "fp64_spectra_per_s": len(probs) / t64,
+        "fp64_ms_per_step": t64 * 1e3,
+        "fp32_over_fp64": t64 / t32,
+        "alpha_max_rel_err_fp32_vs_fp64": alpha_err,
+        "flux_max_abs_err_fp32_vs_fp64": flux_err,
+        "accuracy_sample": f"every 16th star ({len(sample)} stars)",
+        "plan": plan.info(),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    ctx.close()
+
+
+def cpu_baseline_batched(stars, evals_per_star):
+    """The Float64 CPU restatement on whole stars, smallest first, for about 20 s of work; the
+    rate in evaluations per second gives spectra/s at the batch's mean evaluations per star."""
+    import numpy as np
+    from oracle import oracle
+    threads = min(16, os.cpu_count() or 1)
+    order = np.argsort([len(P.lines) * P.n_wl for P, _, _ in stars])
+    done_evals, dt, n = 0.0, 0.0, 0
+    for k in order:
+        P = stars[k][0]
+        a = np.zeros((P.n_layers, P.n_wl), order="F")
+        t0 = time.perf_counter()
+        _, _, e = oracle.line_absorption(a, P, nthreads=threads, mode=oracle.KORG_CHUNKS,
+                                         return_windows=True)
+        dt += time.perf_counter() - t0
+        done_evals += e
+        n += 1
+        if dt > 20.0:
+            break
+    rate = done_evals / dt
+    log(f"cpu baseline: {n} stars ({done_evals:.3g} evals) in {dt:.1f}s on {threads} threads")
+    return {"value": rate / evals_per_star, "unit": "spectra/s", "cores": threads, "kind": "port",
+            "evals_per_s": rate,
+            "sample": f"the {n} smallest stars of the batch, whole, Float64, reference thread "
+                      f"schedule; spectra/s = evaluations/s / mean evaluations per star"}
 
 
 def copy_calibration(mib, reps=5):
