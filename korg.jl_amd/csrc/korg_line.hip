@@ -214,7 +214,8 @@ struct StarDev {
 };
 
 // The star holding global grid point j (the last star with grid_off <= j).
-__device__ inline int star_of_point(const StarDev* stars, int n, int64_t j) {
+template <class P>
+__device__ inline int star_of_point(P stars, int n, int64_t j) {
     int lo = 0, hi = n;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -224,7 +225,8 @@ __device__ inline int star_of_point(const StarDev* stars, int n, int64_t j) {
 }
 
 // The star holding global line i.
-__device__ inline int star_of_line(const StarDev* stars, int n, int64_t i) {
+template <class P>
+__device__ inline int star_of_line(P stars, int n, int64_t i) {
     int lo = 0, hi = n;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -279,13 +281,24 @@ struct SetupArgs {
     const double* mass;
     const uint8_t* is_mol;
     const LayerTab* tab;  // [star * L + l]
-    int want_f32;
-    int count_branches;
     LineMeta* meta;
     double4* rec64;  // [line * Ls + l]: (1/(sigma sqrt 2), a, scaling, 0)
     float4* recM;    // [(line * n_lblocks + b) * 64 + lane]: far-wing coefficients, MFMA A layout
     unsigned long long* counters;
 };
+
+constexpr double INV_SQRT_PI = 0.5641895835477563;
+// (3.75 / SERIES_TAIL_EPS)^(1/4): the tail radius of branch A (a <= 0.2), as v
+constexpr double VT_SMALL_A = 44.00558683966967;
+
+// x^p for x >= 0 to about 1e-6 relative (Float32 log2/exp2 of the mantissa-exponent split, so any
+// double exponent is fine), enlarged by 1e-5: an upper bound for a zone radius.
+__device__ inline double root_up(double x, float p) {
+    int e;
+    const double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    const float lg = __builtin_amdgcn_logf((float)m) + (float)e;
+    return (double)__builtin_amdgcn_exp2f(lg * p) * 1.00001;
+}
 
 // Continuum at x for layer l on the coarse grid: linear interpolation between the bracketing
 // knots (index found once per line by the caller).
@@ -294,16 +307,23 @@ __device__ inline double cntm_lerp(const double* knots, const double* vals, int6
     return (1.0 - t) * vals[i] + t * vals[i + 1];
 }
 
+// Three waves per SIMD (at most 168 registers, a few spilled): measured faster than two.  The
+// Float32 coefficient block (F32) and the branch counting (CB) are compile-time choices, so the
+// common variants carry neither's registers.
 #ifndef KORG_SETUP_ATTR
-#define KORG_SETUP_ATTR
+#define KORG_SETUP_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 #endif
+template <bool F32, bool CB>
 __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int lane = threadIdx.x % WAVE;
-    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    // the wave's line, its star and the star's fields are wavefront-uniform: scalar registers
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     if (i >= A.n_lines) return;  // whole wavefront
     const int L = A.L;
-    const int st = star_of_line(A.stars, A.n_stars, i);
-    const StarDev& S = A.stars[st];
+    const KCONST StarDev* stars = uniform_ptr(A.stars);
+    const int st = star_of_line(stars, A.n_stars, i);
+    const KCONST StarDev& S = stars[st];
     const int64_t li = i - S.line_off;  // index in the star's own line list
     const double wl = S.wl[li];
     const int32_t s = S.species[li];
@@ -324,7 +344,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double vdW1 = S.vdW1[li], vdW2 = S.vdW2[li];
     const double E_lower = S.E_lower[li];
     const double E_upper = E_lower + c_cgs * hplanck_eV / wl;  // src/line_absorption.jl:101
-    const double gf = pow(10.0, S.log_gf[li]);
+    const double gf = exp10(S.log_gf[li]);
     const double sl = sigma_line(wl);
     int64_t kidx = 0;
     if (S.cntm_mode == KORG_CNTM_GRID) {
@@ -366,46 +386,39 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             dmax = dg > dmax ? dg : dmax;
             lmax = dl > lmax ? dl : lmax;
             // profile parameters of line_profile (src/line_absorption.jl:245-249)
-            const double inv = 1.0 / (sigma * 1.4142135623730951);
+            const double s2 = sigma * 1.4142135623730951;  // sigma sqrt 2
+            const double inv = 1.0 / s2;
             const double a = gamma * inv;
-            const double scaling = inv / 1.7724538509055159 * amp;
+            const double scaling = inv * INV_SQRT_PI * amp;
             A.rec64[i * A.Ls + l] = make_double4(inv, a, scaling, 0.0);
-            // near zone of this layer, as v: the Voigt core (branches B and C) and, for a > 0.2,
-            // where the 4-term far-wing series is not converged to SERIES_EPS
-            double vr;
-            if (a <= 0.2) {
-                vr = 5.0;
-            } else {
+            // Zone radii of this layer, as v.  Near zone: the Voigt core (branches B and C) and,
+            // for a > 0.2, where the 4-term far-wing series is not converged to SERIES_EPS.  Tail
+            // radius: beyond it |e3| w^2 + |e4| w^3 <= SERIES_TAIL_EPS (w = 1/v^2; e_k the series
+            // coefficients relative to the leading term).  They only place the switch between
+            // exact evaluation and the series, so the roots are taken in Float32 and enlarged to
+            // stay on the safe side.
+            double vr = 5.0, vt = VT_SMALL_A;
+            if (!(a <= 0.2)) {
                 const double a2 = a * a;
                 const double e5 = fabs(94.25 * a2 * a2 - 18.0 * a2 * a2 * a2 + a2 * a2 * a2 * a2);
-                const double vs = sqrt(sqrt(sqrt(e5 / SERIES_EPS)));
-                vr = fmax(fmax(vs, 2.0 * a), 3.2);
-            }
-            const double r = vr / inv;
-            rmax = r > rmax ? r : rmax;
-            // tail radius of this layer, as v: beyond it |e3| w^2 + |e4| w^3 <= SERIES_TAIL_EPS
-            // (w = 1/v^2; e_k the series coefficients relative to the leading term)
-            double vt;
-            if (a <= 0.2) {
-                vt = sqrt(sqrt(3.75 / SERIES_TAIL_EPS));
-            } else {
-                const double a2 = a * a;
                 const double e3 = fabs(3.75 - 5.0 * a2 + a2 * a2);
                 const double e4 = fabs(-26.25 * a2 + 10.5 * a2 * a2 - a2 * a2 * a2);
-                const double v3 = sqrt(sqrt(2.0 * e3 / SERIES_TAIL_EPS));
-                const double v4 = cbrt(sqrt(2.0 * e4 / SERIES_TAIL_EPS));
-                vt = fmax(v3, v4);
+                vr = fmax(fmax(root_up(e5 * (1.0 / SERIES_EPS), 0.125f), 2.0 * a), 3.2);
+                vt = fmax(root_up(e3 * (2.0 / SERIES_TAIL_EPS), 0.25f),
+                          root_up(e4 * (2.0 / SERIES_TAIL_EPS), 1.0f / 6.0f));
             }
-            const double rt = vt / inv;
+            const double r = vr * s2;
+            rmax = r > rmax ? r : rmax;
+            const double rt = vt * s2;
             tmax = rt > tmax ? rt : tmax;
             if (l == lane) {
                 my_hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
                 my_a = a;
             }
-            if (A.want_f32) {
-                const double sd = DELTA_UNIT * inv;
-                const double q = 1.0 / (sd * sd);
-                const double base = scaling * (a / 1.7724538509055159);
+            if (F32) {
+                const double sq = s2 * (1.0 / DELTA_UNIT);
+                const double q = sq * sq;  // 1 / (DELTA_UNIT inv)^2
+                const double base = scaling * (a * INV_SQRT_PI);
                 double e2 = 1.5, e3 = 3.75, e4 = 0.0;
                 if (!(a <= 0.2)) {
                     const double a2 = a * a;
@@ -421,7 +434,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
                 for (int k = 0; k < 4; ++k) f32_bad |= !(fabs(c[k]) <= F32_COEF_MAX);  // or NaN
             }
         }
-        if (A.want_f32) {
+        if (F32) {
             // MFMA A-operand layout: lane i of step s holds coefficient k = 2s + i/32 of layer
             // 32 mb + i%32; slot 2s + mb of float4 i.
             float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
@@ -438,7 +451,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     // Float32 range: c_k R^k stays small where the series is used, but neither factor is
     // representable (and inf * 0 is NaN).  Such a line skips the series: its coefficients are
     // zero and the near-zone kernel evaluates its whole window exactly.
-    const bool series_off = A.want_f32 && ballot(f32_bad) != 0;
+    const bool series_off = F32 && ballot(f32_bad) != 0;
     if (series_off) {
         for (int l = lane; l < Lm; l += WAVE) {
             float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
@@ -495,7 +508,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         flo = 1;
         fhi = 0;
     }
-    if (A.count_branches && lb <= ub && lane < L) {
+    if (CB && lb <= ub && lane < L) {
         // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
         int64_t core_n = 0;
         if (my_hw >= 0.0) {
@@ -939,7 +952,8 @@ __global__ __launch_bounds__(WAVE) void k_profile_f64(ProfileArgs A) {
         const int tile = item / A.n_lblocks;
         const int l0 = (item % A.n_lblocks) * G;
         const int64_t j0 = (int64_t)tile * A.tile_w;
-        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
         const int64_t go = S.grid_off;
         const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;  // the star's last point at most
         if (j1 < j0) continue;  // padding between two stars' grids: no points
@@ -1028,12 +1042,13 @@ void k_wing_mfma(ProfileArgs A) {
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(&A.counters[0], 1ull);
         __syncthreads();
-        const int item = s_item;
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= A.n_items) break;  // the same for the whole workgroup
         const int tile = item / A.n_lblocks;
         const int lblk = item % A.n_lblocks;
         const int64_t j0 = (int64_t)tile * A.tile_w;
-        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
         const int64_t go = S.grid_off;
         // the star's last point at most; below j0 for padding between two stars' grids (such a
         // tile has no lines and writes nothing)
@@ -1261,7 +1276,8 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
         const int l = (item % A.n_lblocks) * WAVE + lane;
         const bool lv = l < L;
         const int64_t j0 = (int64_t)tile * CT;
-        const StarDev& S = A.stars[star_of_point(A.stars, A.n_stars, j0)];
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
         const int64_t go = S.grid_off;
         const int64_t j1 = min(j0 + CT, go + S.n_wl) - 1;
         if (j1 < j0) continue;  // padding between two stars' grids: no points
@@ -1773,13 +1789,18 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.mass = b.sp.mass;
     sa.is_mol = b.sp.is_molecule;
     sa.tab = (const LayerTab*)ws.tab.p;
-    sa.want_f32 = b.f32 ? 1 : 0;
-    sa.count_branches = b.count_branches;
     sa.meta = (LineMeta*)ws.meta.p;
     sa.rec64 = (double4*)ws.rec64.p;
     sa.recM = (float4*)ws.recM.p;
     sa.counters = counters;
-    hipLaunchKernelGGL(k_setup, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, sa);
+    const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);
+    if (b.f32) {
+        if (b.count_branches) hipLaunchKernelGGL((k_setup<true, true>), sgrid, sblock, 0, s, sa);
+        else hipLaunchKernelGGL((k_setup<true, false>), sgrid, sblock, 0, s, sa);
+    } else {
+        if (b.count_branches) hipLaunchKernelGGL((k_setup<false, true>), sgrid, sblock, 0, s, sa);
+        else hipLaunchKernelGGL((k_setup<false, false>), sgrid, sblock, 0, s, sa);
+    }
     HIP_TRY(ctx, hipGetLastError());
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
     return KORG_OK;
