@@ -31,7 +31,7 @@
 // R = 1/delta^2 shared by all layers, so phi = scaling H = sum_{k=1..4} c_k R^k with per-(line,
 // layer) coefficients c_k.  Summed over lines this is a matrix product: alpha[layer, point] +=
 // C[layer, (line,k)] * P[(line,k), point].  The 4-term truncation is exact for branch A and is
-// used for branch D only where |e5| w^4 <= 2e-8; nearer points belong to the near zone.
+// used for branch D only where |e5| w^4 <= SERIES_EPS; nearer points belong to the near zone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,7 +50,11 @@ using namespace korg;
 namespace {
 
 constexpr int WAVE = 64;
-constexpr double SERIES_EPS = 2e-8;  // relative truncation error allowed in the far-wing series
+// Relative truncation error allowed in the far-wing series at the edge of a line's near zone (the
+// first dropped term against the leading one).  2e-6 instead of 2e-8 shrinks the near zones of
+// strongly damped layers by 1.8x; the measured Float32 alpha error is unchanged (1.3e-5 at the
+// 300,000-line workload, 1.1e-5 over a batch of random atmospheres), since other terms dominate it.
+constexpr double SERIES_EPS = 2e-6;
 // Relative size below which the R^3 and R^4 terms of the far-wing series are dropped (beyond the
 // line's tail radius, where they are smaller than this fraction of the leading term in every
 // layer).
