@@ -1017,6 +1017,12 @@ __device__ inline int hunger_branch(double a, double v) {
     return 3;
 }
 
+// Bits i in [0, 32) with base + i in [lo, hi].
+__device__ inline uint32_t span_bits(int lo, int hi, int base) {
+    const int a = max(lo - base, 0), z = min(hi - base, 31);
+    return a > z ? 0u : (uint32_t)(((2ull << (z - a)) - 1) << a);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Float32 far wing on the matrix cores.  A work item is (tile of 32*NB points, block of 64
 // layers), owned by one workgroup of WW waves; each wave takes a contiguous part of the
@@ -1062,7 +1068,8 @@ void k_wing_mfma(ProfileArgs A) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             jj[b] = (int)(j0 + 32 * b + n);
-            lam[b] = S.wls[min((int64_t)jj[b], j1) - go];  // points past j1 are never used
+            // in units of DELTA_UNIT (points past j1 are never used)
+            lam[b] = S.wls[min((int64_t)jj[b], j1) - go] * (1.0 / DELTA_UNIT);
         }
         floatx16 acc[2][NB];
 #pragma unroll
@@ -1086,19 +1093,34 @@ void k_wing_mfma(ProfileArgs A) {
             float r2[NB];  // R^2
             bool full;     // the tile meets the line's tail zone: the R^3, R^4 step is needed
         };
-        auto prep = [&](const LineMeta& m, int bl) {
-            Prep p;
-            const double lam0 = readlane_f64(m.lam0, bl);
-            const int lb = readlane_i32(m.lb, bl), ub = readlane_i32(m.ub, bl);
-            const int clo = readlane_i32(m.clo, bl), chi = readlane_i32(m.chi, bl);
-            const int flo = readlane_i32(m.flo, bl), fhi = readlane_i32(m.fhi, bl);
-            p.full = flo <= j1 && fhi >= j0;
+        // Per batch of 64 lines, each lane describes its own line: for every 32-point block, the
+        // bits of the points in the window and outside the near zone, and (one ballot) whether the
+        // tile meets the line's tail zone.  Per line this leaves lane reads and scalar tests.
+        const int j0s = (int)j0, j1s = (int)j1;  // grid indices are below 2^31
+        uint32_t mk[NB];
+        uint64_t full_mask = 0;
+        double lam0s = 0.0;  // the lane's line centre in units of DELTA_UNIT
+        auto describe = [&](const LineMeta& m) {
+            lam0s = m.lam0 * (1.0 / DELTA_UNIT);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const int j = jj[b];
-                const bool use = j >= lb && j <= ub && !(j >= clo && j <= chi);
-                // Float64 fix-up: lambda_j - lambda0 from the exact Float64 grid values
-                const float d = (float)((lam[b] - lam0) * (1.0 / DELTA_UNIT));
+                const int base = j0s + 32 * b;
+                mk[b] = span_bits(m.lb, m.ub, base) & ~span_bits(m.clo, m.chi, base);
+            }
+            full_mask = ballot(m.flo <= j1s && m.fhi >= j0s);
+        };
+        auto prep = [&](const LineMeta& m, int bl) {
+            Prep p;
+            const double lam0 = readlane_f64(lam0s, bl);
+            p.full = (full_mask >> bl) & 1;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                // lanes n and n + 32 hold the same point
+                const uint32_t mw = (uint32_t)readlane_i32((int)mk[b], bl);
+                const bool use = __builtin_amdgcn_inverse_ballot_w64(mw | ((uint64_t)mw << 32));
+                // Float64 fix-up: lambda_j - lambda0 from the Float64 grid values (both scaled
+                // to DELTA_UNIT, which adds a rounding of ~1e-10 of the offset)
+                const float d = (float)(lam[b] - lam0);
                 float R = __builtin_amdgcn_rcpf(d * d);
                 R = use ? R : 0.0f;
                 p.r2[b] = R * R;
@@ -1156,6 +1178,7 @@ void k_wing_mfma(ProfileArgs A) {
                 // inside the line loop (where a wait would also drain the operand prefetches)
                 asm volatile("" ::"v"(idxA), "v"(idxB), "v"(mA.lam0), "v"(mA.lb), "v"(mA.ub),
                              "v"(mA.clo), "v"(mA.chi), "v"(mA.flo), "v"(mA.fhi));
+                describe(mA);
                 // lines in pairs: the two lines' operand chains are independent and interleave
                 for (int b = 0; b < nb; b += 4) {
                     x2 = load_a(line_at(b + 2));
