@@ -1320,13 +1320,16 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
         auto load_q = [&](int line) {
             return lv ? rq[(int64_t)line * A.Ls] : make_double4(0.0, 0.0, 0.0, 0.0);
         };
+        const int j0s = (int)j0;
         for_each_line_pf<double4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_NEAR,
                                   load_q, [&](const LineView& v, const double4& q) {
             const float a32 = (float)q.y, sc = (float)q.z;
+            // the line's near zone in tile coordinates (grid indices are below 2^31, so these
+            // wavefront-uniform tests stay on the scalar unit)
+            const int tlo = v.clo - j0s, thi = v.chi - j0s;
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
-                const int64_t j = j0 + t;
-                if (j < v.clo || j > v.chi) continue;  // wavefront-uniform
+                if (t < tlo || t > thi) continue;  // wavefront-uniform
                 const double v64 = fabs(lam[t] - v.lam0) * q.x;
                 const int br = hunger_branch(q.y, v64);
                 float h;
