@@ -55,6 +55,18 @@ def lib():
         L.oracle_voigt_hjerting_grad.argtypes = [d, d, ctypes.POINTER(d), ctypes.POINTER(d)]
         L.oracle_voigt_hjerting_grad.restype = d
         L.oracle_line_absorption_vjp.restype = ctypes.c_int
+        L.oracle_blackbody.argtypes = [d, d]
+        L.oracle_blackbody.restype = d
+        L.oracle_expint2.argtypes = [d, ctypes.POINTER(d)]
+        L.oracle_expint2.restype = d
+        L.oracle_transfer_core.argtypes = [d, d, d, ctypes.POINTER(d), ctypes.POINTER(d),
+                                           ctypes.POINTER(d)]
+        L.oracle_transfer_core.restype = d
+        vp = ctypes.c_void_p
+        L.oracle_rt_flux.argtypes = [i64, i64, vp, vp, vp, vp, vp, vp]
+        L.oracle_rt_flux.restype = ctypes.c_int
+        L.oracle_rt_flux_vjp.argtypes = [i64, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_rt_flux_vjp.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -178,3 +190,54 @@ def line_absorption_vjp(G, problem, nthreads=0):
         raise RuntimeError(f"oracle_line_absorption_vjp returned {rc}")
     return out
 
+
+
+# ---------------------------------------------------------------------------------------------
+# radiative transfer (rt_oracle.c): src/RadiativeTransfer/RadiativeTransfer.jl, src/synthesize.jl
+
+
+def blackbody(T, lam):
+    return lib().oracle_blackbody(float(T), float(lam))
+
+
+def expint2(x):
+    """Korg's approximate E2(x) and its derivative (the derivative of the approximation)."""
+    dE = ctypes.c_double()
+    E = lib().oracle_expint2(float(x), ctypes.byref(dE))
+    return E, dE.value
+
+
+def transfer_core(tau, m, b):
+    """expint_transfer_integral_core(tau, m, b) and its partials (d/dtau, d/dm, d/db)."""
+    dt, dm, db = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    v = lib().oracle_transfer_core(float(tau), float(m), float(b), ctypes.byref(dt),
+                                   ctypes.byref(dm), ctypes.byref(db))
+    return v, dt.value, dm.value, db.value
+
+
+def _rt_args(alpha, wls, temps, tau_ref, alpha_ref):
+    alpha = np.asfortranarray(alpha, dtype=np.float64)  # (L, nwl), column-major
+    c = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    return alpha, c(wls), c(temps), c(tau_ref), c(alpha_ref)
+
+
+def rt_flux(alpha, wls, temps, tau_ref, alpha_ref):
+    """Astrophysical flux F[j] (plane-parallel, anchored tau, linear source, expint form) of
+    alpha (n_layers, n_wl) with the blackbody source of each layer at wls (cm)."""
+    alpha, wls, temps, tau_ref, alpha_ref = _rt_args(alpha, wls, temps, tau_ref, alpha_ref)
+    F = np.zeros(wls.size)
+    rc = lib().oracle_rt_flux(alpha.shape[0], wls.size, _p(alpha), _p(wls), _p(temps),
+                              _p(tau_ref), _p(alpha_ref), _p(F))
+    assert rc == 0
+    return F
+
+
+def rt_flux_vjp(alpha, wls, temps, tau_ref, alpha_ref, gF):
+    """dL/dalpha (n_layers, n_wl) given gF = dL/dF (n_wl)."""
+    alpha, wls, temps, tau_ref, alpha_ref = _rt_args(alpha, wls, temps, tau_ref, alpha_ref)
+    gF = np.ascontiguousarray(gF, dtype=np.float64)
+    g = np.zeros(alpha.shape, order="F")
+    rc = lib().oracle_rt_flux_vjp(alpha.shape[0], wls.size, _p(alpha), _p(wls), _p(temps),
+                                  _p(tau_ref), _p(alpha_ref), _p(gF), _p(g))
+    assert rc == 0
+    return g
