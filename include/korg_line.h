@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 6
+#define KORG_LINE_ABI_VERSION 7
 
 enum korg_status {
     KORG_OK = 0,
@@ -301,6 +301,41 @@ void korg_plan_destroy(korg_plan* plan);
 int korg_autotune_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                           const korg_species* species, int alpha_dtype, int precision,
                           const char* cache_path, korg_tuning* out);
+
+/*
+ * The fit step of a batch of stars: the line opacity (forward) into each star's alpha (zeroed
+ * first), radiative transfer, chi^2 between the normalized flux and the data, and the gradient of
+ * chi^2 with respect to each star's per-layer inputs through the line-opacity adjoint.  This is
+ * what one Newton/LBFGS iteration of Korg's Fit needs of the line opacity
+ * (src/Fit/fit_via_synthesis.jl:304; the reference differentiates `synthesize` with ForwardDiff,
+ * src/synthesize.jl:305-306, 310).
+ *
+ * Radiative transfer is Korg's plane-parallel default (src/RadiativeTransfer/RadiativeTransfer.jl
+ * :62-66, 77-146): anchored optical depth (:244-253) on tau_ref and alpha_ref, the blackbody of each
+ * layer's temperature as source (src/synthesize.jl:285, 412-417), the flux-only linear-source
+ * solution in exponential integrals (:379-400) with Korg's approximate E2 (:408-480), in Float64.
+ * The normalized flux is F(alpha_cntm + alpha_lines) / F(alpha_cntm); alpha_cntm is each star's
+ * continuum on knots (KORG_CNTM_GRID required).  chi^2 = sum_j ((F_norm_j - data_j) / sigma_j)^2.
+ * The gradient through alpha_lines is the adjoint of korg_line_absorption_vjp (Float64, windows
+ * held fixed) of G = dchi^2/dalpha_lines from the transfer's reverse mode.  Device pointers.
+ */
+typedef struct {
+    const double* tau_ref;    /* [n_layers] reference optical depth of the anchored scheme */
+    const double* alpha_ref;  /* [n_layers] absorption at the reference wavelength */
+    const double* data;       /* [n_wl] observed normalized flux */
+    const double* sigma;      /* [n_wl] its uncertainties */
+    double* flux;             /* out [n_wl]: the model's normalized flux (may be NULL) */
+    double* chi2;             /* out [1] */
+    korg_layer_grads grads;   /* out: dchi^2/d(per-layer inputs), device arrays (NULL: skipped) */
+} korg_fit_target;
+int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                              const korg_fit_target* targets, const korg_species* species,
+                              int alpha_dtype, int precision, int synchronize);
+/* The fit step captured once as a plan (replayed by korg_plan_launch, as korg_plan_batched; a
+ * replay recomputes everything from the inputs in the captured buffers). */
+int korg_plan_fit(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                  const korg_fit_target* targets, const korg_species* species, int alpha_dtype,
+                  int precision, double headroom, korg_plan** out);
 
 #ifdef __cplusplus
 }

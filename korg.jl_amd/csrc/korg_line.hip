@@ -184,7 +184,8 @@ struct LayerTab {
 // Device counters of one call (reset by k_layers): [0] profile queue, [1] evaluations,
 // [2] lines used, [3] error flags, [4] near-zone reach in points, [5] near queue,
 // [6..9] evaluations by Hunger branch, [10] near-zone evaluations, [11] far-wing MFMAs per layer
-// block, [12] tail-zone evaluations, [13] entries of the per-tile line lists.
+// block, [12] tail-zone evaluations, [13] entries of the per-tile line lists, [14] segments of the
+// adjoint.
 // Error flags: 1 lines not sorted, 2 species index out of range, 4 the per-tile line lists need
 // more entries than their buffer holds.  With any flag set no kernel writes alpha.
 constexpr int N_COUNTERS = 16;
@@ -215,6 +216,14 @@ struct StarDev {
     const double *knots, *cntm;
     double cutoff;
     GridInfo gi;  // written by k_layers
+    // adjoint: dL/dalpha in (dtype of the call), and the outputs dL/d(T, n_e, xi, n_HI, n_HeI,
+    // n_H2) [L] and dL/d(n_div_U) [S x L] (a null output is not written)
+    void* G;
+    double* gq[7];
+    // fit step: the anchored scheme's reference optical depth and absorption [L], the data and
+    // its uncertainty [n_wl]; outputs: the normalized flux [n_wl] (may be null) and chi^2 [1]
+    const double *tau_ref, *alpha_ref, *data, *sigma;
+    double *flux, *chi2;
 };
 
 // The star holding global grid point j (the last star with grid_off <= j).
@@ -777,10 +786,11 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
 }
 
 // One block: exclusive scan of n counts into start[0..n] (start[n] = total).  With counters
-// given, the total is recorded in counters[13] and FLAG_CAPACITY is raised when it exceeds cap.
+// given, the total is recorded in counters[slot] and FLAG_CAPACITY is raised when it exceeds cap.
 __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __restrict__ count,
                                                     int64_t* __restrict__ start, int64_t cap,
-                                                    unsigned long long* __restrict__ counters) {
+                                                    unsigned long long* __restrict__ counters,
+                                                    int slot) {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x;
     const int64_t chunk = (n + 1023) / 1024;
@@ -803,7 +813,7 @@ __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __rest
     if (t == 1023) {
         start[n] = part[1023];
         if (counters) {
-            counters[13] = (unsigned long long)part[1023];
+            counters[slot] = (unsigned long long)part[1023];
             if (part[1023] > cap) atomicOr(&counters[3], FLAG_CAPACITY);
         }
     }
@@ -1353,18 +1363,21 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Adjoint (vector-Jacobian product) of the line opacity, in Float64.  Given G = dL/dalpha, a work
-// item is (line, chunk of up to VJP_CHUNK window points): one wavefront, lane = layer (layers l,
-// l + 64, ... when L > 64).  Each lane sums over the chunk's points, in point order,
+// Adjoint (vector-Jacobian product) of the line opacity, in Float64, over a batch of stars.  Given
+// G = dL/dalpha per star, an item is (line, chunk of up to VJP_CHUNK window points); lanes =
+// layers (layers l, l + 64, ... when L > 64).  Each lane sums over the item's points, in point
+// order,
 //   S_H = sum G H,  S_a = sum G dH/da,  S_v = sum G v dH/dv,
 // and turns them into the item's share of dL/dx for its layer,
 //   scaling (da/dx S_a + dln(1/(sigma sqrt 2))/dx S_v) + dscaling/dx S_H,
 // for x = T, n_e, xi, n_HI, n_HeI, n_H2 and the n_div_U of the line's species.  The windows are
-// integer functions of the inputs and are held fixed.  k_vjp_reduce sums the items in a fixed
-// order, so the gradients are deterministic.
+// integer functions of the inputs and are held fixed.
+// A segment is up to VJP_SEG consecutive items of one star, handled by one wavefront in item order;
+// it leaves one partial sum per output (k_vjp), and k_vjp_reduce sums a star's segments in a fixed
+// order: the gradients are deterministic, and the scratch is per segment, not per item.
 // ---------------------------------------------------------------------------------------------
 constexpr int VJP_CHUNK = 2048;
-constexpr int VJP_NQ = 7;  // T, n_e, xi, n_HI, n_HeI, n_H2, n_div_U
+constexpr int VJP_SEG = 64;
 
 __global__ __launch_bounds__(256) void k_vjp_count(int64_t n, const LineMeta* __restrict__ meta,
                                                    int* __restrict__ cnt) {
@@ -1374,128 +1387,410 @@ __global__ __launch_bounds__(256) void k_vjp_count(int64_t n, const LineMeta* __
     cnt[i] = m.lb <= m.ub ? (m.ub - m.lb + VJP_CHUNK) / VJP_CHUNK : 0;
 }
 
+// Segments per star from the items' exclusive scan over all lines (item_start[N] = total).
+__global__ void k_vjp_segs(int n_stars, const StarDev* __restrict__ stars,
+                           const int64_t* __restrict__ item_start, int* __restrict__ seg_cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_stars) return;
+    const StarDev& S = stars[s];
+    const int64_t n = item_start[S.line_off + S.n_lines] - item_start[S.line_off];
+    seg_cnt[s] = (int)((n + VJP_SEG - 1) / VJP_SEG);
+}
+
 struct VjpArgs {
-    int64_t n_lines, n_items;
-    int L, Ls;
-    const double* wls;
+    int n_stars;
+    int L, Ls, n_species;
+    const StarDev* stars;
     const LineMeta* meta;
     const double4* rec64;
-    const int64_t* item_start;  // [n_lines + 1]: first item of each line
-    const double *temps, *n_e, *xi, *n_div_U;
-    const LayerTab* tab;
-    const double *wl, *log_gf, *E_lower, *gamma_stark, *vdW1, *vdW2;
-    const int32_t* species;
+    const LayerTab* tab;          // [star * L + l]
+    const int64_t* item_start;    // [N + 1]: first item of each line (global)
+    const int64_t* seg_start;     // [n_stars + 1]: first segment of each star
     const double* mass;
     const uint8_t* is_mol;
-    const void* G;
     int g_f32;
-    double* contrib;     // [item][VJP_NQ][L]
-    int* item_species;   // [item]
+    double* partial;              // [segment][6 + n_species][L]
+    int64_t seg_cap;              // segments the partials hold
     const unsigned long long* counters;
 };
 
 __global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
-    if (A.counters[3] != 0) return;  // failed setup
+    const int NQ = 6 + A.n_species;
+    if (A.counters[3] != 0) return;  // failed setup, or the segments do not fit
+    const int64_t n_segs = A.seg_start[A.n_stars];
     const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
     const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;  // as k_layers
     const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
     const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
-    for (int64_t item = blockIdx.x; item < A.n_items; item += gridDim.x) {
-        // the line of this item: the largest i with item_start[i] <= item
-        int64_t lo = 0, hi = A.n_lines;
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) / 2;
-            if (A.item_start[mid] <= item) lo = mid; else hi = mid;
-        }
-        const int64_t i = lo;
-        const LineMeta m = A.meta[i];
-        const int64_t j0 = m.lb + (item - A.item_start[i]) * VJP_CHUNK;
-        const int64_t j1 = min(j0 + VJP_CHUNK, (int64_t)m.ub + 1);  // exclusive
-        const double lam0 = m.lam0;
-        const int32_t s = A.species[i];
-        const double mass = A.mass[s];
-        const bool mol = A.is_mol[s] != 0;
-        const double E_up = A.E_lower[i] + c_cgs * hplanck_eV / lam0;  // src/line_absorption.jl:101
-        const double gf = pow(10.0, A.log_gf[i]);
-        const double sl = sigma_line(lam0);
-        const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * PI));
-        for (int l = lane; l < L; l += WAVE) {
-            const double4 r = A.rec64[i * A.Ls + l];
-            const double inv = r.x, a = r.y, scaling = r.z;
-            double SH = 0.0, Sa = 0.0, Sv = 0.0;
-            for (int64_t j = j0; j < j1; ++j) {
-                const double g = A.g_f32 ? (double)((const float*)A.G)[l + (int64_t)L * j]
-                                         : ((const double*)A.G)[l + (int64_t)L * j];
-                const double v = fabs(A.wls[j] - lam0) * inv;
-                double Ha, Hv;
-                const double H = voigt_hjerting64_grad(a, v, Ha, Hv);
-                SH += g * H;
-                Sa += g * Ha;
-                Sv += g * v * Hv;
+    for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
+        // the star of this segment and its item range
+        int st = 0;
+        {
+            int lo = 0, hi = A.n_stars;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (A.seg_start[mid] <= seg) lo = mid; else hi = mid;
             }
-            const LayerTab tb = A.tab[l];
-            const double T = A.temps[l], xi = A.xi[l];
-            // d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 (src/line_absorption.jl:190)
-            const double q = kboltz_cgs * T / mass + xi * xi / 2.0;
-            const double dli_dT = -(kboltz_cgs / mass) / (2.0 * q);
-            const double dli_dxi = -xi / (2.0 * q);
-            // dGamma/dx (src/line_absorption.jl:90-99)
-            double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
-            if (!mol) {
-                const double st = A.gamma_stark[i] * tb.t16;
-                const double v1 = A.vdW1[i], v2 = A.vdW2[i];
-                vdw = (v2 == -1.0) ? v1 * tb.t03 : scaled_vdW_abo(v1, v2, mass, T);
-                const double dvdw_dT = (v2 == -1.0 ? 0.3 : (1.0 - v2) / 2.0) * vdw / T;
-                dG_dne = st;
-                dG_dT = A.n_e[l] * st / (6.0 * T) + tb.n_eff * dvdw_dT;
-            }
-            const double ig = inv * gscale;
-            const double da_dT = ig * dG_dT + a * dli_dT;
-            const double da_dxi = a * dli_dxi;
-            // scaling = inv / sqrt(pi) * gf * sigma_line * levels * n/U (src/line_absorption.jl:101-106)
-            const double nU = A.n_div_U[(int64_t)s * L + l];
-            const double el = exp(-tb.beta * A.E_lower[i]), eu = exp(-tb.beta * E_up);
-            const double dlev_dT = (tb.beta / T) * (A.E_lower[i] * el - E_up * eu);
-            const double base = inv / 1.7724538509055159 * gf * sl;
-            const double ds_dT = scaling * dli_dT + base * nU * dlev_dT;
-            const double ds_dxi = scaling * dli_dxi;
-            double* c = A.contrib + item * VJP_NQ * L + l;
-            c[0 * L] = scaling * (da_dT * Sa + dli_dT * Sv) + ds_dT * SH;
-            c[1 * L] = scaling * (ig * dG_dne * Sa);
-            c[2 * L] = scaling * (da_dxi * Sa + dli_dxi * Sv) + ds_dxi * SH;
-            c[3 * L] = scaling * (ig * vdw * Sa);
-            c[4 * L] = scaling * (ig * c_He * vdw * Sa);
-            c[5 * L] = scaling * (ig * c_H2 * vdw * Sa);
-            c[6 * L] = base * (el - eu) * SH;
+            st = lo;
         }
-        if (lane == 0) A.item_species[item] = s;
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[st];
+        const int64_t go = S.grid_off;
+        const int64_t first = A.item_start[S.line_off];
+        const int64_t it0 = first + (seg - A.seg_start[st]) * VJP_SEG;
+        const int64_t it1 = min(it0 + VJP_SEG, A.item_start[S.line_off + S.n_lines]);
+        double* part = A.partial + seg * NQ * L;
+        for (int q = 0; q < NQ; ++q)
+            for (int l = lane; l < L; l += WAVE) part[q * L + l] = 0.0;
+        // lines of the segment's items: the largest line with item_start <= item, found once
+        int64_t i = S.line_off;
+        {
+            int64_t lo = S.line_off, hi = S.line_off + S.n_lines;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) / 2;
+                if (A.item_start[mid] <= it0) lo = mid; else hi = mid;
+            }
+            i = lo;
+        }
+        for (int64_t item = it0; item < it1; ++item) {
+            while (A.item_start[i + 1] <= item) ++i;  // lines with no items are passed over
+            const int64_t li = i - S.line_off;
+            const LineMeta m = A.meta[i];
+            const int64_t j0 = m.lb + (item - A.item_start[i]) * VJP_CHUNK;
+            const int64_t j1 = min(j0 + VJP_CHUNK, (int64_t)m.ub + 1);  // exclusive
+            const double lam0 = m.lam0;
+            const int32_t s = S.species[li];
+            const double mass = A.mass[s];
+            const bool mol = A.is_mol[s] != 0;
+            const double E_lower = S.E_lower[li];
+            const double E_up = E_lower + c_cgs * hplanck_eV / lam0;  // src/line_absorption.jl:101
+            const double gf = exp10(S.log_gf[li]);
+            const double sl = sigma_line(lam0);
+            const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * PI));
+            for (int l = lane; l < L; l += WAVE) {
+                const double4 r = A.rec64[i * A.Ls + l];
+                const double inv = r.x, a = r.y, scaling = r.z;
+                double SH = 0.0, Sa = 0.0, Sv = 0.0;
+                for (int64_t j = j0; j < j1; ++j) {
+                    const int64_t jl = j - go;  // the star's own point index
+                    const double g = A.g_f32 ? (double)((const float*)S.G)[l + (int64_t)L * jl]
+                                             : ((const double*)S.G)[l + (int64_t)L * jl];
+                    const double v = fabs(S.wls[jl] - lam0) * inv;
+                    double Ha, Hv;
+                    const double H = voigt_hjerting64_grad(a, v, Ha, Hv);
+                    SH += g * H;
+                    Sa += g * Ha;
+                    Sv += g * v * Hv;
+                }
+                const LayerTab tb = A.tab[(int64_t)st * L + l];
+                const double T = S.temps[l], xi = S.xi[l];
+                // d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 (src/line_absorption.jl:190)
+                const double q = kboltz_cgs * T / mass + xi * xi / 2.0;
+                const double dli_dT = -(kboltz_cgs / mass) / (2.0 * q);
+                const double dli_dxi = -xi / (2.0 * q);
+                // dGamma/dx (src/line_absorption.jl:90-99)
+                double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
+                if (!mol) {
+                    const double stk = S.gamma_stark[li] * tb.t16;
+                    const double v1 = S.vdW1[li], v2 = S.vdW2[li];
+                    vdw = (v2 == -1.0) ? v1 * tb.t03 : scaled_vdW_abo(v1, v2, mass, T);
+                    const double dvdw_dT = (v2 == -1.0 ? 0.3 : (1.0 - v2) / 2.0) * vdw / T;
+                    dG_dne = stk;
+                    dG_dT = S.n_e[l] * stk / (6.0 * T) + tb.n_eff * dvdw_dT;
+                }
+                const double ig = inv * gscale;
+                const double da_dT = ig * dG_dT + a * dli_dT;
+                const double da_dxi = a * dli_dxi;
+                // scaling = inv / sqrt(pi) * gf * sigma_line * levels * n/U (src/line_absorption.jl:101-106)
+                const double nU = S.n_div_U[(int64_t)s * L + l];
+                const double el = exp(-tb.beta * E_lower), eu = exp(-tb.beta * E_up);
+                const double dlev_dT = (tb.beta / T) * (E_lower * el - E_up * eu);
+                const double base = inv * INV_SQRT_PI * gf * sl;
+                const double ds_dT = scaling * dli_dT + base * nU * dlev_dT;
+                const double ds_dxi = scaling * dli_dxi;
+                part[0 * L + l] += scaling * (da_dT * Sa + dli_dT * Sv) + ds_dT * SH;
+                part[1 * L + l] += scaling * (ig * dG_dne * Sa);
+                part[2 * L + l] += scaling * (da_dxi * Sa + dli_dxi * Sv) + ds_dxi * SH;
+                part[3 * L + l] += scaling * (ig * vdw * Sa);
+                part[4 * L + l] += scaling * (ig * c_He * vdw * Sa);
+                part[5 * L + l] += scaling * (ig * c_H2 * vdw * Sa);
+                part[(6 + s) * L + l] += base * (el - eu) * SH;
+            }
+        }
     }
 }
 
-// One block per output: (quantity q < 6, layer l) sums every item's share; (6 + species s, layer
-// l) sums the n_div_U shares of the items of species s.  A fixed per-thread order and a fixed tree:
-// deterministic.  out = [T | n_e | xi | n_HI | n_HeI | n_H2] x L, then n_div_U [S x L].
-__global__ __launch_bounds__(256) void k_vjp_reduce(int64_t n_items, int L,
-                                                    const double* __restrict__ contrib,
-                                                    const int* __restrict__ item_species,
-                                                    double* __restrict__ out) {
+// One block per (star, output): output q < 6 is (quantity q, layer l), q = 6 + s the n_div_U of
+// species s at layer l.  The star's segments are summed in a fixed per-thread order and a fixed
+// tree (deterministic) and the sum goes to the star's output array of that quantity.
+__global__ __launch_bounds__(256) void k_vjp_reduce(int L, int n_species,
+                                                    const StarDev* __restrict__ stars,
+                                                    const int64_t* __restrict__ seg_start,
+                                                    const double* __restrict__ partial,
+                                                    const unsigned long long* __restrict__ counters) {
     __shared__ double red[256];
-    const int b = blockIdx.x, t = threadIdx.x;
-    const int q = b / L, l = b % L;
+    if (counters[3] != 0) return;  // nothing was summed (the segments did not fit)
+    const int NQ = 6 + n_species;
+    const int st = blockIdx.y;
+    const int b = blockIdx.x, t = threadIdx.x;  // b = q * L + l
+    const int64_t s0 = seg_start[st], s1 = seg_start[st + 1];
     double sum = 0.0;
-    for (int64_t it = t; it < n_items; it += 256) {
-        if (q < 6) sum += contrib[(it * VJP_NQ + q) * L + l];
-        else if (item_species[it] == q - 6) sum += contrib[(it * VJP_NQ + 6) * L + l];
-    }
+    for (int64_t sg = s0 + t; sg < s1; sg += 256) sum += partial[sg * NQ * L + b];
     red[t] = sum;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (t < h) red[t] += red[t + h];
         __syncthreads();
     }
-    if (t == 0) out[b] = red[0];
+    if (t == 0) {
+        const StarDev& S = stars[st];
+        const int q = b / L, l = b % L;
+        double* dst = S.gq[q < 6 ? q : 6];
+        if (dst) dst[q < 6 ? l : (int64_t)(q - 6) * L + l] = red[0];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Radiative transfer on the device, for the fit step: Korg's plane-parallel default
+// (src/RadiativeTransfer/RadiativeTransfer.jl:62-66, 77-146): the anchored optical depth (:244-253)
+// with the blackbody source (src/synthesize.jl:285, 412-417) and the flux-only linear-source
+// solution in exponential integrals (:379-400), with Korg's approximate E2 (:408-480).  One thread
+// per wavelength point; the flux of alpha_cntm + alpha_lines and of alpha_cntm alone give the
+// normalized flux, chi^2 = sum ((F / F_c - data) / sigma)^2, and by a reverse sweep over the layers
+// G = dchi^2/dalpha_lines, which the line adjoint consumes.  Float64.
+// ---------------------------------------------------------------------------------------------
+__device__ inline double blackbody64(double T, double lam) {
+    const double h = hplanck_cgs, c = c_cgs, k = kboltz_cgs;
+    const double l2 = lam * lam;
+    return 2 * h * c * c / (l2 * l2 * lam) / expm1(h * c / lam / k / T);
+}
+
+// Korg's E2 pieces (src/RadiativeTransfer/RadiativeTransfer.jl:427-480): polynomials of degree 5
+// in y = x - n about n = 2..8, their coefficients as published there.
+__constant__ double E2_POLY[7][6] = {
+    {0.037534261820486914, -0.04890051070806112, 0.033833820809153176, -0.016916910404576574,
+     0.007048712668573576, -0.0026785108140579598},
+    {0.010641925085272673, -0.013048381094197039, 0.008297844727977323, -0.003687930990212144,
+     0.0013061422257001345, -0.0003995258572729822},
+    {0.0031982292493385146, -0.0037793524098489054, 0.0022894548610917728, -0.0009539395254549051,
+     0.00031003034577284415, -8.466213288412284e-5},
+    {0.000996469042708825, -0.0011482955912753257, 0.0006737946999085467, -0.00026951787996341863,
+     8.310134632205409e-5, -2.1202073223788938e-5},
+    {0.0003182574636904001, -0.0003600824521626587, 0.00020656268138886323, -8.032993165122457e-5,
+     2.390771775334065e-5, -5.8334831318151185e-6},
+    {0.00010350984428214624, -0.00011548173161033826, 6.513442611103688e-5, -2.4813114708966427e-5,
+     7.200234178941151e-6, -1.7027366981408086e-6},
+    {3.413764515111217e-5, -3.76656228439249e-5, 2.096641424390699e-5, -7.862405341465122e-6,
+     2.2386015208338193e-6, -5.173353514609864e-7},
+};
+
+// Korg's E2 (:408-480) and, when d is given, the derivative of the approximation.
+__device__ inline double expint2_korg(double x, double* d) {
+    if (x == 0.0) {
+        if (d) *d = -INFINITY;
+        return 1.0;
+    }
+    if (x < 1.1) {
+        const double g = 0.57721566490153286060651209008240243104215933593992;
+        const double P = -0.5 + (0.08333333333333333 + (-0.013888888888888888 +
+                                                        0.0020833333333333333 * x) * x) * x;
+        const double Q = (log(x) + g - 1) + P * x;
+        if (d) {
+            const double dP = 0.08333333333333333 + (2 * -0.013888888888888888 +
+                                                     3 * 0.0020833333333333333 * x) * x;
+            *d = Q + 1 + x * (P + x * dP);
+        }
+        return 1 + Q * x;
+    }
+    if (x >= 9.0) {
+        const double u = 1 / x;
+        const double R = 1 + (-2 + (6 + (-24 + 120 * u) * u) * u) * u;
+        const double ex = exp(-x);
+        const double e = ex * R * u;
+        if (d) {
+            const double dR = -2 + (12 + (-72 + 480 * u) * u) * u;
+            *d = -e - ex * (R + u * dR) * (u * u);
+        }
+        return e;
+    }
+    // polynomials of degree 5 in y = x - n about n = 2..8 (coefficients of :427-480)
+    const int n = x < 2.5 ? 2 : x < 3.5 ? 3 : x < 4.5 ? 4 : x < 5.5 ? 5 : x < 6.5 ? 6 : x < 7.5 ? 7 : 8;
+    const double* c = E2_POLY[n - 2];
+    const double y = x - n;
+    if (d) *d = c[1] + (2 * c[2] + (3 * c[3] + (4 * c[4] + 5 * c[5] * y) * y) * y) * y;
+    return c[0] + (c[1] + (c[2] + (c[3] + (c[4] + c[5] * y) * y) * y) * y) * y;
+}
+
+// Per star and layer: f = tau_ref / alpha_ref and ln tau_ref (the anchored scheme's factors).
+struct RtTab {
+    double f, ltau;
+};
+
+__global__ void k_rt_tab(int n_stars, int L, const StarDev* __restrict__ stars, RtTab* __restrict__ tab) {
+    const int st = blockIdx.x;
+    const StarDev& S = stars[st];
+    for (int l = threadIdx.x; l < L; l += blockDim.x)
+        tab[(int64_t)st * L + l] = RtTab{S.tau_ref[l] / S.alpha_ref[l], log(S.tau_ref[l])};
+}
+
+// Zeroes every star's alpha: thread t of the global (point, layer) space.
+__global__ void k_zero_alpha(int n_stars, int L, int64_t n_total, const StarDev* __restrict__ stars,
+                             int alpha_f32) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_total * L;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / L;
+        const int l = (int)(t % L);
+        const StarDev& S = stars[star_of_point(stars, n_stars, j)];
+        const int64_t jl = j - S.grid_off;
+        if (jl >= S.n_wl) continue;  // padding between stars
+        const int64_t k = (int64_t)l + (int64_t)L * jl;
+        if (alpha_f32) ((float*)S.alpha)[k] = 0.0f;
+        else ((double*)S.alpha)[k] = 0.0;
+    }
+}
+
+struct RtArgs {
+    int n_stars, L;
+    int64_t n_total;  // global grid points
+    const StarDev* stars;
+    const RtTab* tab;  // [star * L + l]
+    int alpha_f32;
+    double* chi2_part;  // [global point]; G (Float64) goes to each star's StarDev::G
+};
+
+// alpha_cntm of layer l at the point (continuum interpolated on the star's knots)
+__device__ inline double cntm_at(const KCONST StarDev& S, int l, int64_t kidx, double x) {
+    return cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, kidx, x);
+}
+
+__global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
+    const int L = A.L;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n_total) return;
+    // stars' grids are padded to 128 points: the 64 points of a wavefront share one star
+    const KCONST StarDev* stars = uniform_ptr(A.stars);
+    const int st = star_of_point(stars, A.n_stars, j);
+    const KCONST StarDev& S = stars[st];
+    const int64_t jl = j - S.grid_off;
+    if (jl >= S.n_wl) {
+        A.chi2_part[j] = 0.0;
+        return;
+    }
+    const RtTab* tb = A.tab + (int64_t)st * L;
+    const double lam = S.wls[jl];
+    // knot interval of lam: clamp(searchsortedlast(knots, lam), 1, n_knots - 1) - 1
+    int64_t kidx = 0;
+    {
+        int64_t lo = 0, hi = S.n_knots;  // last k with knots[k] <= lam
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (S.knots[mid] <= lam) lo = mid; else hi = mid;
+        }
+        kidx = lo > S.n_knots - 2 ? S.n_knots - 2 : lo;
+    }
+    auto alpha_lines = [&](int l) {
+        const int64_t k = (int64_t)l + (int64_t)L * jl;
+        return A.alpha_f32 ? (double)((const float*)S.alpha)[k] : ((const double*)S.alpha)[k];
+    };
+    // forward: flux of the total and of the continuum alone
+    double F = 0.0, Fc = 0.0;
+    double tau0 = 0.0, tc0 = 0.0;
+    double ac0 = cntm_at(S, 0, kidx, lam);
+    double u0 = (ac0 + alpha_lines(0)) * tb[0].f, uc0 = ac0 * tb[0].f;
+    double S0 = blackbody64(S.temps[0], lam);
+    double E0 = 1.0, x0 = 1.0, Ec0 = 1.0, xc0 = 1.0;  // E2 and e^{-tau} at tau = 0
+    for (int i = 1; i < L; ++i) {
+        const double ac = cntm_at(S, i, kidx, lam);
+        const double u1 = (ac + alpha_lines(i)) * tb[i].f, uc1 = ac * tb[i].f;
+        const double dl = tb[i].ltau - tb[i - 1].ltau;
+        const double tau1 = tau0 + 0.5 * (u1 + u0) * dl, tc1 = tc0 + 0.5 * (uc1 + uc0) * dl;
+        const double S1 = blackbody64(S.temps[i], lam);
+        const double E1 = expint2_korg(tau1, nullptr), x1 = exp(-tau1);
+        const double Ec1 = expint2_korg(tc1, nullptr), xc1 = exp(-tc1);
+        {
+            const double m = (S1 - S0) / (tau1 - tau0), b = S0 - m * tau0;
+            F += (tau1 * E1 * (3 * b + 2 * m * tau1) - x1 * (3 * b + 2 * m * (tau1 + 1))) / 6 -
+                 (tau0 * E0 * (3 * b + 2 * m * tau0) - x0 * (3 * b + 2 * m * (tau0 + 1))) / 6;
+        }
+        {
+            const double m = (S1 - S0) / (tc1 - tc0), b = S0 - m * tc0;
+            Fc += (tc1 * Ec1 * (3 * b + 2 * m * tc1) - xc1 * (3 * b + 2 * m * (tc1 + 1))) / 6 -
+                  (tc0 * Ec0 * (3 * b + 2 * m * tc0) - xc0 * (3 * b + 2 * m * (tc0 + 1))) / 6;
+        }
+        tau0 = tau1; tc0 = tc1; u0 = u1; uc0 = uc1; S0 = S1;
+        E0 = E1; x0 = x1; Ec0 = Ec1; xc0 = xc1;
+    }
+    F *= 2 * PI;
+    Fc *= 2 * PI;
+    const double Fn = F / Fc;
+    const double sg = S.sigma[jl];
+    const double r = (Fn - S.data[jl]) / sg;
+    A.chi2_part[j] = r * r;
+    if (S.flux) S.flux[jl] = Fn;
+    // reverse: G[l] = dchi^2/dalpha_total[l] = dchi^2/dalpha_lines[l]
+    const double w = 2 * PI * (2 * r / sg / Fc);  // 2 pi dchi^2/dF
+    double* G = (double*)S.G + (int64_t)L * jl;
+    // walk the intervals from the deepest up, recomputing tau_i from tau_{i+1}
+    double tau1 = tau0, u1 = u0, S1 = S0;
+    double dE1;
+    double E1 = expint2_korg(tau1, &dE1), x1 = exp(-tau1);
+    double pending = 0.0;  // contribution to gtau[i + 1] from interval i + 1 (already walked)
+    double sfx = 0.0;      // sum of gtau[k] for k >= i + 2
+    for (int i = L - 2; i >= 0; --i) {
+        const double ac = cntm_at(S, i, kidx, lam);
+        const double u_i = (ac + alpha_lines(i)) * tb[i].f;
+        const double dl = tb[i + 1].ltau - tb[i].ltau;
+        const double tau_i = i == 0 ? 0.0 : tau1 - 0.5 * (u1 + u_i) * dl;
+        const double S_i = blackbody64(S.temps[i], lam);
+        double dE_i;
+        const double E_i = expint2_korg(tau_i, i == 0 ? nullptr : &dE_i), x_i = exp(-tau_i);
+        if (i == 0) dE_i = 0.0;  // tau_0 = 0 is fixed: its derivative is never used
+        const double dt = tau1 - tau_i;
+        const double m = (S1 - S_i) / dt, b = S_i - m * tau_i;
+        // partials of core(tau, m, b) = (tau E (3b + 2m tau) - e^{-tau} (3b + 2m (tau + 1))) / 6
+        auto partials = [&](double t, double E, double dE, double x, double& ct, double& cm, double& cb) {
+            const double Aq = 3 * b + 2 * m * t, Bq = 3 * b + 2 * m * (t + 1);
+            ct = ((E + t * dE) * Aq + t * E * 2 * m + x * Bq - x * 2 * m) / 6;
+            cm = (t * E * 2 * t - x * 2 * (t + 1)) / 6;
+            cb = (t * E * 3 - x * 3) / 6;
+        };
+        double ct1, cm1, cb1, ct0, cm0, cb0;
+        partials(tau1, E1, dE1, x1, ct1, cm1, cb1);
+        partials(tau_i, E_i, dE_i, x_i, ct0, cm0, cb0);
+        const double Tm = cm1 - cm0, Tb = cb1 - cb0;
+        const double dm_dt1 = -m / dt, dm_dt0 = m / dt;
+        const double db_dt1 = -tau_i * dm_dt1, db_dt0 = -m - tau_i * dm_dt0;
+        const double g_up = w * (ct1 + Tm * dm_dt1 + Tb * db_dt1) + pending;  // gtau[i + 1], final
+        pending = w * (-ct0 + Tm * dm_dt0 + Tb * db_dt0);
+        // dtau_k/du_j = dl_j/2 [1 <= j <= k] + dl_{j+1}/2 [j + 1 <= k]; layer i + 1 is final now
+        const double sfx_next = sfx;       // sum over k >= i + 2
+        sfx += g_up;                       // sum over k >= i + 1
+        double gu = 0.5 * dl * sfx;        // dl = ltau[i + 1] - ltau[i]
+        if (i + 2 < L) gu += 0.5 * (tb[i + 2].ltau - tb[i + 1].ltau) * sfx_next;
+        G[i + 1] = gu * tb[i + 1].f;
+        tau1 = tau_i; u1 = u_i; S1 = S_i; E1 = E_i; dE1 = dE_i; x1 = x_i;
+    }
+    // layer 0: tau_0 is fixed, u_0 enters tau_k (k >= 1) through the first interval
+    G[0] = 0.5 * (tb[1].ltau - tb[0].ltau) * sfx * tb[0].f;
+}
+
+// chi^2 of every star: one block per star, its points summed in a fixed order and tree.
+__global__ __launch_bounds__(256) void k_star_chi2(const StarDev* __restrict__ stars,
+                                                   const double* __restrict__ chi2_part) {
+    __shared__ double red[256];
+    const StarDev& S = stars[blockIdx.x];
+    const int t = threadIdx.x;
+    double sum = 0.0;
+    for (int64_t j = t; j < S.n_wl; j += 256) sum += chi2_part[S.grid_off + j];
+    red[t] = sum;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0 && S.chi2) *S.chi2 = red[0];
 }
 
 }  // namespace
@@ -1515,19 +1810,25 @@ struct Workspace {
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
+    // adjoint and fit step
+    DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part;
     int64_t tile_cap = 0;             // entries tile_lines holds
+    int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
     void release() {
         DevBuf* bufs[] = {&meta,     &rec64,     &recM,      &tab,        &counters,  &blk,
                           &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
-                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars};
+                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &vjp_cnt,
+                          &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
+                          &chi2_part};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->cap = 0;
         }
         tile_cap = 0;
+        seg_cap = 0;
     }
 };
 
@@ -1549,8 +1850,8 @@ struct korg_ctx {
     bool last_sized = false;  // the last forward call sized the tile lists on the host (ev[5], ev[6])
     Workspace ws;
     std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
-    // adjoint scratch
-    DevBuf vjp_cnt, vjp_start, vjp_contrib, vjp_species, vjp_out, h_grad;
+    // adjoint outputs of the host-memory entry point, and its staged G
+    DevBuf vjp_out, h_grad;
     // staging for the host-memory entry points
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
@@ -1633,6 +1934,11 @@ struct StarHost {
     korg_layers layers;
     korg_continuum cntm;
     double cutoff;
+    // adjoint input and outputs, fit target (device pointers; null when unused)
+    void* G = nullptr;
+    double* gq[7] = {};
+    const double *tau_ref = nullptr, *alpha_ref = nullptr, *data = nullptr, *sigma = nullptr;
+    double *flux = nullptr, *chi2 = nullptr;
 };
 
 // Validates the stars of a batched call (one layer count, one species table for all).
@@ -1701,6 +2007,14 @@ int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const
         d.knots = x.cntm.knots;
         d.cntm = x.cntm.values;
         d.cutoff = x.cutoff;
+        d.G = x.G;
+        for (int q = 0; q < 7; ++q) d.gq[q] = x.gq[q];
+        d.tau_ref = x.tau_ref;
+        d.alpha_ref = x.alpha_ref;
+        d.data = x.data;
+        d.sigma = x.sigma;
+        d.flux = x.flux;
+        d.chi2 = x.chi2;
         ctx->last_grid_off[k] = go;
         ctx->last_line_off[k] = lo;
         go += (x.grid.n_wl + STAR_ALIGN - 1) / STAR_ALIGN * STAR_ALIGN;
@@ -1762,9 +2076,8 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.wpc = t.waves_per_cu > 0 ? t.waves_per_cu : 12;
     b.near_wpc = t.near_waves_per_cu > 0 ? t.near_waves_per_cu : 16;
     b.count_branches = t.count_branches ? 1 : 0;
-    for (int k = 0; k < n_stars; ++k) b.N += stars[k].lines.n_lines;
-    if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
+    if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     const int64_t N = b.N;
     b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
     b.ntiles = (int)((b.nwl + b.tile_w - 1) / b.tile_w);
@@ -1874,7 +2187,7 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
                            (const int*)ws.cls_line.p, (const int64_t*)nullptr, (int*)ws.tc_count.p,
                            (int*)nullptr, (const unsigned long long*)counters);
         hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, ntc, (const int*)ws.tc_count.p,
-                           (int64_t*)ws.tc_start.p, tile_cap, counters);
+                           (int64_t*)ws.tc_start.p, tile_cap, counters, 13);
     }
     HIP_TRY(ctx, hipGetLastError());
     if (timed && b.f32) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
@@ -2068,107 +2381,199 @@ int finish_stats(korg_ctx* ctx) {
     return KORG_OK;
 }
 
-// The adjoint pipeline on device pointers: setup (windows and Float64 records), one item per
-// (line, chunk of window points), per-item shares, fixed-order reduction, and the copy of the
-// gradients to `out` (hipMemcpyDeviceToDevice or DeviceToHost).
+// Buffers of the adjoint over the batch b (grown before any capture; enqueue functions allocate
+// nothing).
+int prepare_vjp(korg_ctx* ctx, Workspace& ws, const Batch& b) {
+    TRY(grow(ctx, ws.vjp_cnt, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
+    TRY(grow(ctx, ws.vjp_start, sizeof(int64_t) * (size_t)(b.N + 1)));
+    TRY(grow(ctx, ws.seg_cnt, sizeof(int) * (size_t)b.n_stars));
+    TRY(grow(ctx, ws.seg_start, sizeof(int64_t) * (size_t)(b.n_stars + 1)));
+    return KORG_OK;
+}
+
+// Adjoint counts on s (the setup already enqueued): items per line, their scan, segments per star,
+// their scan, which raises FLAG_CAPACITY when the segments exceed seg_cap.
+int enqueue_vjp_counts(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s,
+                       int64_t seg_cap) {
+    auto* counters = (unsigned long long*)ws.counters.p;
+    if (b.N > 0)
+        hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((b.N + 255) / 256)), dim3(256), 0, s, b.N,
+                           (const LineMeta*)ws.meta.p, (int*)ws.vjp_cnt.p);
+    hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, b.N, (const int*)ws.vjp_cnt.p,
+                       (int64_t*)ws.vjp_start.p, (int64_t)INT64_MAX, (unsigned long long*)nullptr, 0);
+    hipLaunchKernelGGL(k_vjp_segs, dim3((unsigned)((b.n_stars + 255) / 256)), dim3(256), 0, s,
+                       b.n_stars, (const StarDev*)ws.stars.p, (const int64_t*)ws.vjp_start.p,
+                       (int*)ws.seg_cnt.p);
+    hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, (int64_t)b.n_stars,
+                       (const int*)ws.seg_cnt.p, (int64_t*)ws.seg_start.p, seg_cap, counters, 14);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
+// Reads the segment total (after enqueue_vjp_counts on s), waits for it, and grows the partials to
+// headroom times it.
+int size_segments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, double headroom) {
+    int64_t total = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&total, (int64_t*)ws.seg_start.p + b.n_stars, sizeof total,
+                                hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    int64_t cap = (int64_t)((double)total * headroom) + (headroom > 1.0 ? 64 : 0);
+    if (cap < total) cap = total;
+    const int NQ = 6 + b.sp.n_species;
+    TRY(grow(ctx, ws.vjp_partial, sizeof(double) * (size_t)(cap > 0 ? cap : 1) * NQ * b.L));
+    ws.seg_cap = cap;
+    return KORG_OK;
+}
+
+// The adjoint's items and the per-star reduction into the StarDev outputs, on s.
+int enqueue_vjp(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int g_f32) {
+    VjpArgs va;
+    va.n_stars = b.n_stars;
+    va.L = b.L;
+    va.Ls = b.Ls;
+    va.n_species = b.sp.n_species;
+    va.stars = (const StarDev*)ws.stars.p;
+    va.meta = (const LineMeta*)ws.meta.p;
+    va.rec64 = (const double4*)ws.rec64.p;
+    va.tab = (const LayerTab*)ws.tab.p;
+    va.item_start = (const int64_t*)ws.vjp_start.p;
+    va.seg_start = (const int64_t*)ws.seg_start.p;
+    va.mass = b.sp.mass;
+    va.is_mol = b.sp.is_molecule;
+    va.g_f32 = g_f32;
+    va.partial = (double*)ws.vjp_partial.p;
+    va.seg_cap = ws.seg_cap;
+    va.counters = (const unsigned long long*)ws.counters.p;
+    const int64_t grid = (int64_t)ctx->n_cu * 16;
+    hipLaunchKernelGGL(k_vjp, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    HIP_TRY(ctx, hipGetLastError());
+    const int NQ = 6 + b.sp.n_species;
+    hipLaunchKernelGGL(k_vjp_reduce, dim3((unsigned)(NQ * b.L), (unsigned)b.n_stars), dim3(256), 0, s,
+                       b.L, b.sp.n_species, (const StarDev*)ws.stars.p,
+                       (const int64_t*)ws.seg_start.p, (const double*)ws.vjp_partial.p,
+                       (const unsigned long long*)ws.counters.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
+// The adjoint pipeline of one star on device pointers: setup (windows and Float64 records), the
+// item and segment counts, the items by segments, and the fixed-order reduction into the
+// outputs `gq` (device pointers).
 int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const korg_lines* ln,
             const korg_species* sp, const korg_layers* ly, const korg_continuum* cn,
-            double cutoff, korg_layer_grads* out, hipMemcpyKind kind) {
+            double cutoff, double* const gq[7]) {
     const int64_t N = ln->n_lines;
     const int L = ly->n_layers;
-    const int64_t S = sp->n_species;
-    const int nout = (int)((6 + S) * L);
     hipStream_t s = ctx->stream;
     ctx->last_n_lines = N;
     ctx->last_precision = KORG_PRECISION_FP64;
     ctx->last_n_lblocks = 0;
     ctx->last_sized = false;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
-    TRY(grow(ctx, ctx->vjp_out, sizeof(double) * (size_t)nout));
-    double* res = (double*)ctx->vjp_out.p;
-    if (N > 0) {
-        Workspace& ws = ctx->ws;
-        const StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
-        Batch b;
-        TRY(plan_batch(ctx, ctx->tuning, 1, &one, sp, L, KORG_DTYPE_F64, KORG_PRECISION_FP64, ws, s,
-                       b));
-        TRY(grow(ctx, ctx->vjp_cnt, sizeof(int) * N));
-        TRY(grow(ctx, ctx->vjp_start, sizeof(int64_t) * (N + 1)));
-        TRY(enqueue_setup(ctx, ws, b, s, true));
-        hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N,
-                           (const LineMeta*)ws.meta.p, (int*)ctx->vjp_cnt.p);
-        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, N, (const int*)ctx->vjp_cnt.p,
-                           (int64_t*)ctx->vjp_start.p, (int64_t)0, (unsigned long long*)nullptr);
-        HIP_TRY(ctx, hipGetLastError());
-        int64_t n_items = 0;
-        unsigned long long flags = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(&n_items, (int64_t*)ctx->vjp_start.p + N, sizeof n_items,
-                                    hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(&flags, (unsigned long long*)ws.counters.p + 3, sizeof flags,
-                                    hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        if (flags & FLAG_SPECIES)
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                           "a line's species index is outside [0, n_species)");
-        if (flags & FLAG_UNSORTED)
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "the line list is not sorted by wavelength");
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-        if (n_items > 0) {
-            TRY(grow(ctx, ctx->vjp_contrib, sizeof(double) * (size_t)n_items * VJP_NQ * L));
-            TRY(grow(ctx, ctx->vjp_species, sizeof(int) * (size_t)n_items));
-            VjpArgs va;
-            va.n_lines = N;
-            va.n_items = n_items;
-            va.L = L;
-            va.Ls = b.Ls;
-            va.wls = g->all_wls;
-            va.meta = (const LineMeta*)ws.meta.p;
-            va.rec64 = (const double4*)ws.rec64.p;
-            va.item_start = (const int64_t*)ctx->vjp_start.p;
-            va.temps = ly->temps;
-            va.n_e = ly->n_e;
-            va.xi = ly->xi;
-            va.n_div_U = ly->n_div_U;
-            va.tab = (const LayerTab*)ws.tab.p;
-            va.wl = ln->wl;
-            va.log_gf = ln->log_gf;
-            va.E_lower = ln->E_lower;
-            va.gamma_stark = ln->gamma_stark;
-            va.vdW1 = ln->vdW1;
-            va.vdW2 = ln->vdW2;
-            va.species = ln->species;
-            va.mass = sp->mass;
-            va.is_mol = sp->is_molecule;
-            va.G = G;
-            va.g_f32 = g_dtype == KORG_DTYPE_F32;
-            va.contrib = (double*)ctx->vjp_contrib.p;
-            va.item_species = (int*)ctx->vjp_species.p;
-            va.counters = (const unsigned long long*)ws.counters.p;
-            int64_t grid = (int64_t)ctx->n_cu * 16;
-            if (grid > n_items) grid = n_items;
-            hipLaunchKernelGGL(k_vjp, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
-            HIP_TRY(ctx, hipGetLastError());
-        }
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-        if (n_items > 0) {
-            hipLaunchKernelGGL(k_vjp_reduce, dim3((unsigned)nout), dim3(256), 0, s, n_items, L,
-                               (const double*)ctx->vjp_contrib.p, (const int*)ctx->vjp_species.p,
-                               res);
-        } else {
-            HIP_TRY(ctx, hipMemsetAsync(res, 0, sizeof(double) * (size_t)nout, s));
-        }
-        HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-    } else {
-        HIP_TRY(ctx, hipMemsetAsync(res, 0, sizeof(double) * (size_t)nout, s));
+    if (N == 0) {  // no lines: every gradient is zero
+        for (int q = 0; q < 7; ++q)
+            if (gq[q])
+                HIP_TRY(ctx, hipMemsetAsync(gq[q], 0, sizeof(double) * (size_t)(q < 6 ? L : sp->n_species * L), s));
+        return KORG_OK;
     }
-    double* dst[7] = {out->T, out->n_e, out->xi, out->n_HI, out->n_HeI, out->n_H2, out->n_div_U};
-    for (int q = 0; q < 7; ++q) {
-        if (!dst[q]) continue;
-        const size_t cnt = q < 6 ? (size_t)L : (size_t)(S * L);
-        HIP_TRY(ctx, hipMemcpyAsync(dst[q], res + (size_t)q * L, sizeof(double) * cnt, kind, s));
+    Workspace& ws = ctx->ws;
+    StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
+    one.G = const_cast<void*>(G);
+    for (int q = 0; q < 7; ++q) one.gq[q] = gq[q];
+    Batch b;
+    TRY(plan_batch(ctx, ctx->tuning, 1, &one, sp, L, KORG_DTYPE_F64, KORG_PRECISION_FP64, ws, s, b));
+    TRY(prepare_vjp(ctx, ws, b));
+    TRY(enqueue_setup(ctx, ws, b, s, true));
+    TRY(enqueue_vjp_counts(ctx, ws, b, s, INT64_MAX));
+    TRY(size_segments(ctx, ws, b, s, 1.0));
+    unsigned long long flags = 0;
+    HIP_TRY(ctx, hipMemcpy(&flags, (unsigned long long*)ws.counters.p + 3, sizeof flags,
+                           hipMemcpyDeviceToHost));
+    if (flags & FLAG_SPECIES)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "a line's species index is outside [0, n_species)");
+    if (flags & FLAG_UNSORTED)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "the line list is not sorted by wavelength");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+    TRY(enqueue_vjp(ctx, ws, b, s, g_dtype == KORG_DTYPE_F32));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    return KORG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// fit step: forward, radiative transfer, chi^2 and its gradient through the adjoint
+// ---------------------------------------------------------------------------------------------
+struct FitHost {
+    const double *tau_ref, *alpha_ref, *data, *sigma;
+    double *flux, *chi2;
+    korg_layer_grads grads;
+};
+
+// Points each star's G into one Float64 buffer of the workspace (sum over stars of L x n_wl).
+int place_fit_G(korg_ctx* ctx, Workspace& ws, std::vector<StarHost>& hs, int L) {
+    size_t total = 0;
+    for (const StarHost& x : hs) total += (size_t)L * (size_t)x.grid.n_wl;
+    TRY(grow(ctx, ws.fit_G, sizeof(double) * total));
+    size_t off = 0;
+    for (StarHost& x : hs) {
+        x.G = (double*)ws.fit_G.p + off;
+        off += (size_t)L * (size_t)x.grid.n_wl;
     }
     return KORG_OK;
+}
+
+int prepare_fit(korg_ctx* ctx, Workspace& ws, const Batch& b) {
+    TRY(prepare_vjp(ctx, ws, b));
+    TRY(grow(ctx, ws.rt_tab, sizeof(RtTab) * (size_t)b.n_stars * b.L));
+    TRY(grow(ctx, ws.chi2_part, sizeof(double) * (size_t)(b.nwl > 0 ? b.nwl : 1)));
+    return KORG_OK;
+}
+
+// Radiative transfer, chi^2 and G of every star (alpha_lines complete), on s.
+int enqueue_rt(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_rt_tab, dim3(b.n_stars), dim3(64), 0, s, b.n_stars, b.L,
+                       (const StarDev*)ws.stars.p, (RtTab*)ws.rt_tab.p);
+    RtArgs ra;
+    ra.n_stars = b.n_stars;
+    ra.L = b.L;
+    ra.n_total = b.nwl;
+    ra.stars = (const StarDev*)ws.stars.p;
+    ra.tab = (const RtTab*)ws.rt_tab.p;
+    ra.alpha_f32 = b.alpha_f32;
+    ra.chi2_part = (double*)ws.chi2_part.p;
+    hipLaunchKernelGGL(k_rt_fit, dim3((unsigned)((b.nwl + 255) / 256)), dim3(256), 0, s, ra);
+    hipLaunchKernelGGL(k_star_chi2, dim3(b.n_stars), dim3(256), 0, s, (const StarDev*)ws.stars.p,
+                       (const double*)ws.chi2_part.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
+// The whole fit step on s with the capacities of ws (tile lists, adjoint segments): zero the
+// alphas, the forward, radiative transfer, chi^2 and G, the adjoint.  With sizing, the host reads
+// the list and segment totals in between and grows them (the direct form); without, the
+// capacities are fixed and a shortfall raises FLAG_CAPACITY (the captured form).
+int enqueue_fit(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool sizing,
+                double headroom) {
+    hipLaunchKernelGGL(k_zero_alpha, dim3((unsigned)std::min<int64_t>((b.nwl * b.L + 255) / 256, 1 << 20)),
+                       dim3(256), 0, s, b.n_stars, b.L, b.nwl, (const StarDev*)ws.stars.p,
+                       b.alpha_f32 ? 1 : 0);
+    HIP_TRY(ctx, hipGetLastError());
+    if (b.N > 0) {
+        TRY(enqueue_front(ctx, ws, b, s, sizing ? INT64_MAX : ws.tile_cap, false));
+        if (sizing && b.f32) {
+            int64_t total = 0;
+            TRY(size_tile_lists(ctx, ws, b, s, headroom, &total));
+        }
+        TRY(enqueue_back(ctx, ws, b, s, false));
+    } else {
+        hipLaunchKernelGGL(k_layers, dim3(b.n_stars), dim3(64), 0, s, b.n_stars, b.L,
+                           (StarDev*)ws.stars.p, (LayerTab*)ws.tab.p,
+                           (unsigned long long*)ws.counters.p);
+    }
+    TRY(enqueue_rt(ctx, ws, b, s));
+    TRY(enqueue_vjp_counts(ctx, ws, b, s, sizing ? INT64_MAX : ws.seg_cap));
+    if (sizing) TRY(size_segments(ctx, ws, b, s, headroom));
+    return enqueue_vjp(ctx, ws, b, s, 0);
 }
 
 // Times of the last adjoint call (setup, items, reduction, total).
@@ -2178,6 +2583,70 @@ int finish_vjp_stats(korg_ctx* ctx) {
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
     ctx->stats.profile_ms = elapsed(ctx, 2, 4);
     ctx->stats.core_ms = elapsed(ctx, 4, 3);
+    ctx->stats.total_ms = elapsed(ctx, 0, 3);
+    return KORG_OK;
+}
+
+// Validates the fit targets of a batch and puts them (and the gradient outputs) into the stars.
+int collect_fit(korg_ctx* ctx, int64_t n_stars, const korg_fit_target* t,
+                std::vector<StarHost>& hs) {
+    if (!t) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no fit targets");
+    if (n_stars > 65535)
+        return set_err(ctx, KORG_ERR_UNSUPPORTED, "at most 65535 stars per fit step");
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_fit_target& x = t[k];
+        StarHost& h = hs[k];
+        if (h.layers.n_layers < 2)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: radiative transfer needs >= 2 layers",
+                           (long long)k);
+        if (h.cntm.mode != KORG_CNTM_GRID)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld: the fit step needs the continuum on knots (KORG_CNTM_GRID)",
+                           (long long)k);
+        if (!x.tau_ref || !x.alpha_ref || !x.data || !x.sigma || !x.chi2)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld: tau_ref, alpha_ref, data, sigma and chi2 are required", (long long)k);
+        h.tau_ref = x.tau_ref;
+        h.alpha_ref = x.alpha_ref;
+        h.data = x.data;
+        h.sigma = x.sigma;
+        h.flux = x.flux;
+        h.chi2 = x.chi2;
+        double* const gq[7] = {x.grads.T, x.grads.n_e, x.grads.xi, x.grads.n_HI, x.grads.n_HeI,
+                               x.grads.n_H2, x.grads.n_div_U};
+        for (int q = 0; q < 7; ++q) h.gq[q] = gq[q];
+    }
+    return KORG_OK;
+}
+
+// The fit step of a batch, directly (the host sizes the tile lists and the adjoint's segments).
+int run_fit(korg_ctx* ctx, std::vector<StarHost>& hs, const korg_species* sp, int L,
+            int alpha_dtype, int precision) {
+    hipStream_t s = ctx->stream;
+    Workspace& ws = ctx->ws;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    TRY(place_fit_G(ctx, ws, hs, L));
+    Batch b;
+    TRY(plan_batch(ctx, ctx->tuning, (int)hs.size(), hs.data(), sp, L, alpha_dtype, precision, ws,
+                   s, b));
+    ctx->last_n_lines = b.N;
+    ctx->last_precision = precision;
+    ctx->last_n_lblocks = b.n_lblocks;
+    ctx->last_sized = false;
+    TRY(prepare_fit(ctx, ws, b));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+    TRY(enqueue_fit(ctx, ws, b, s, true, 1.0));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    return KORG_OK;
+}
+
+// Counters and time of the last direct fit step (its time includes the host's sizing).
+int finish_fit_stats(korg_ctx* ctx) {
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+    unsigned long long c[N_COUNTERS];
+    HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
+                            ctx->last_n_lblocks));
     ctx->stats.total_ms = elapsed(ctx, 0, 3);
     return KORG_OK;
 }
@@ -2248,6 +2717,7 @@ struct korg_plan {
     unsigned long long* h_counters = nullptr;  // pinned; the graph's last node copies the counters
     bool pending = false;                      // launched, counters not yet read
     int n_captures = 0;
+    bool fit = false;  // the fit step (korg_plan_fit) rather than the forward alone
 };
 
 namespace {
@@ -2279,8 +2749,13 @@ int plan_capture(korg_plan* p) {
     korg_ctx* ctx = p->ctx;
     plan_drop_graph(p);
     HIP_TRY(ctx, hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_front(ctx, p->ws, p->b, p->cap_stream, p->ws.tile_cap, false);
-    if (rc == KORG_OK) rc = enqueue_back(ctx, p->ws, p->b, p->cap_stream, false);
+    int rc = KORG_OK;
+    if (p->fit) {
+        rc = enqueue_fit(ctx, p->ws, p->b, p->cap_stream, false, 0.0);
+    } else {
+        rc = enqueue_front(ctx, p->ws, p->b, p->cap_stream, p->ws.tile_cap, false);
+        if (rc == KORG_OK) rc = enqueue_back(ctx, p->ws, p->b, p->cap_stream, false);
+    }
     if (rc == KORG_OK) {
         const hipError_t e = hipMemcpyAsync(p->h_counters, p->ws.counters.p,
                                             sizeof(unsigned long long) * N_COUNTERS,
@@ -2308,8 +2783,27 @@ int plan_capture(korg_plan* p) {
 int plan_build(korg_plan* p) {
     korg_ctx* ctx = p->ctx;
     hipStream_t s = ctx->stream;
+    if (p->fit) TRY(place_fit_G(ctx, p->ws, p->stars, p->L));
     TRY(plan_batch(ctx, p->tuning, (int)p->stars.size(), p->stars.data(), &p->sp, p->L,
                    p->alpha_dtype, p->precision, p->ws, s, p->b));
+    if (p->fit) {
+        // one measuring pass of the setup: tile lists (Float32) and adjoint segments
+        TRY(prepare_fit(ctx, p->ws, p->b));
+        if (p->b.N > 0) {
+            TRY(enqueue_front(ctx, p->ws, p->b, s, INT64_MAX, false));
+            if (p->b.f32) {
+                int64_t total = 0;
+                TRY(size_tile_lists(ctx, p->ws, p->b, s, p->headroom, &total));
+            }
+        } else {
+            hipLaunchKernelGGL(k_layers, dim3(p->b.n_stars), dim3(64), 0, s, p->b.n_stars, p->b.L,
+                               (StarDev*)p->ws.stars.p, (LayerTab*)p->ws.tab.p,
+                               (unsigned long long*)p->ws.counters.p);
+        }
+        TRY(enqueue_vjp_counts(ctx, p->ws, p->b, s, INT64_MAX));
+        TRY(size_segments(ctx, p->ws, p->b, s, p->headroom));
+        return plan_capture(p);
+    }
     if (p->b.N == 0) return KORG_OK;
     if (p->b.f32) {
         TRY(enqueue_front(ctx, p->ws, p->b, s, INT64_MAX, false));
@@ -2342,10 +2836,19 @@ int plan_wait(korg_plan* p) {
         if (total > (int64_t)INT32_MAX)
             return set_err(ctx, KORG_ERR_UNSUPPORTED,
                            "more than 2^31 (line, tile) pairs in one call; split the line list");
-        int64_t cap = (int64_t)((double)total * p->headroom) + 4096;
-        if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
-        TRY(grow(ctx, p->ws.tile_lines, sizeof(int) * (size_t)cap));
-        p->ws.tile_cap = cap;
+        if (total > p->ws.tile_cap) {
+            int64_t cap = (int64_t)((double)total * p->headroom) + 4096;
+            if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
+            TRY(grow(ctx, p->ws.tile_lines, sizeof(int) * (size_t)cap));
+            p->ws.tile_cap = cap;
+        }
+        const int64_t segs = (int64_t)p->h_counters[14];
+        if (p->fit && segs > p->ws.seg_cap) {
+            const int64_t cap = (int64_t)((double)segs * p->headroom) + 64;
+            TRY(grow(ctx, p->ws.vjp_partial,
+                     sizeof(double) * (size_t)cap * (6 + p->sp.n_species) * p->L));
+            p->ws.seg_cap = cap;
+        }
         TRY(plan_capture(p));
         TRY(plan_launch(p));
         HIP_TRY(ctx, hipEventSynchronize(p->ev1));
@@ -2538,8 +3041,7 @@ void korg_destroy(korg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ws.release();
-    DevBuf* bufs[] = {&ctx->vjp_cnt, &ctx->vjp_start, &ctx->vjp_contrib, &ctx->vjp_species,
-                      &ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls,
+    DevBuf* bufs[] = {&ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls,
                       &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -2665,8 +3167,10 @@ int korg_line_absorption_vjp_device(korg_ctx* ctx, const void* grad_alpha, int g
     if (!grad_alpha && lines->n_lines > 0)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "grad_alpha is null");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* const gq[7] = {grads->T, grads->n_e, grads->xi, grads->n_HI, grads->n_HeI, grads->n_H2,
+                           grads->n_div_U};
     TRY(run_vjp(ctx, grad_alpha, grad_dtype, grid, lines, species, layers, cntm, cutoff_threshold,
-                grads, hipMemcpyDeviceToDevice));
+                gq));
     if (synchronize) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         TRY(finish_vjp_stats(ctx));
@@ -2701,8 +3205,21 @@ int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dty
                                     ctx->stream));
         Gd = ctx->h_grad.p;
     }
-    TRY(run_vjp(ctx, Gd, grad_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold, grads,
-                hipMemcpyDeviceToHost));
+    // the gradients go to device scratch, then to the caller's host arrays
+    const int64_t S = species->n_species;
+    TRY(grow(ctx, ctx->vjp_out, sizeof(double) * (size_t)((6 + S) * L)));
+    double* res = (double*)ctx->vjp_out.p;
+    double* const dst[7] = {grads->T, grads->n_e, grads->xi, grads->n_HI, grads->n_HeI,
+                            grads->n_H2, grads->n_div_U};
+    double* gq[7];
+    for (int q = 0; q < 7; ++q) gq[q] = dst[q] ? res + (size_t)q * L : nullptr;
+    TRY(run_vjp(ctx, Gd, grad_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold, gq));
+    for (int q = 0; q < 7; ++q) {
+        if (!dst[q]) continue;
+        const size_t cnt = q < 6 ? (size_t)L : (size_t)(S * L);
+        HIP_TRY(ctx, hipMemcpyAsync(dst[q], gq[q], sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return finish_vjp_stats(ctx);
 }
@@ -2730,9 +3247,14 @@ int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {
     return KORG_OK;
 }
 
-int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
-                      const korg_species* species, int alpha_dtype, int precision,
-                      double headroom, korg_plan** out) {
+}  // extern "C"
+
+namespace {
+
+// A plan of the forward (targets null) or of the fit step.
+int plan_create(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                const korg_fit_target* targets, const korg_species* species, int alpha_dtype,
+                int precision, double headroom, korg_plan** out) {
     if (!ctx || !out) return KORG_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     ctx->err.clear();
@@ -2741,6 +3263,10 @@ int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
     korg_plan* p = new korg_plan();
     p->ctx = ctx;
     int rc = collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, p->stars, &p->L);
+    if (rc == KORG_OK && targets) {
+        p->fit = true;
+        rc = collect_fit(ctx, n_stars, targets, p->stars);
+    }
     if (rc != KORG_OK) {
         delete p;
         return rc;
@@ -2771,12 +3297,47 @@ int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
     return KORG_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int korg_plan_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                      const korg_species* species, int alpha_dtype, int precision,
+                      double headroom, korg_plan** out) {
+    return plan_create(ctx, n_stars, stars, nullptr, species, alpha_dtype, precision, headroom, out);
+}
+
+int korg_plan_fit(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                  const korg_fit_target* targets, const korg_species* species, int alpha_dtype,
+                  int precision, double headroom, korg_plan** out) {
+    if (!targets) return ctx ? set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no fit targets") : KORG_ERR_INVALID_ARGUMENT;
+    return plan_create(ctx, n_stars, stars, targets, species, alpha_dtype, precision, headroom, out);
+}
+
+int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                              const korg_fit_target* targets, const korg_species* species,
+                              int alpha_dtype, int precision, int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    std::vector<StarHost> hs;
+    int L = 0;
+    TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
+    TRY(collect_fit(ctx, n_stars, targets, hs));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_fit(ctx, hs, species, L, alpha_dtype, precision));
+    if (synchronize) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_fit_stats(ctx));
+    }
+    return KORG_OK;
+}
+
 int korg_plan_launch(korg_plan* p, int synchronize) {
     if (!p) return KORG_ERR_INVALID_ARGUMENT;
     korg_ctx* ctx = p->ctx;
     ctx->err.clear();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (p->b.N == 0) {
+    if (!p->exec) {  // a forward plan without lines: nothing to do
         std::memset(&ctx->stats, 0, sizeof ctx->stats);
         return KORG_OK;
     }

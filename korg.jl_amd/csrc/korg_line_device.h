@@ -15,6 +15,7 @@ constexpr double electron_charge_cgs = 4.80320425e-10;
 constexpr double amu_cgs = 1.6605402e-24;
 constexpr double kboltz_eV = 8.617333262145e-5;
 constexpr double hplanck_eV = 4.135667696e-15;
+constexpr double hplanck_cgs = 6.62607015e-27;
 constexpr double PI = 3.141592653589793;
 
 // Wavelength offsets in the Float32 kernel are measured in this unit (cm): 0.01 A.

@@ -103,6 +103,47 @@ class DeviceProblem:
         return DeviceArray.from_numpy(np.asarray(fill, dtype=dt).T)
 
 
+class FitTarget:
+    """The fit step's target for one star, in device memory: the anchored scheme's reference
+    optical depth and absorption (per layer), the data and its uncertainties (per grid point), and
+    the outputs -- the model's normalized flux, chi^2 and dchi^2/d(per-layer inputs)."""
+
+    GRADS = ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2", "n_div_U")
+
+    def __init__(self, problem, tau_ref, alpha_ref, data, sigma):
+        from .hipmem import DeviceArray
+        P = problem
+        L, n, S = P.n_layers, P.n_wl, P.species_mass.size
+        sigma = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (n,))
+        self.t = {"tau_ref": DeviceArray.from_numpy(np.asarray(tau_ref, np.float64)),
+                  "alpha_ref": DeviceArray.from_numpy(np.asarray(alpha_ref, np.float64)),
+                  "data": DeviceArray.from_numpy(np.asarray(data, np.float64)),
+                  "sigma": DeviceArray.from_numpy(np.ascontiguousarray(sigma)),
+                  "flux": DeviceArray.zeros((n,), np.float64),
+                  "chi2": DeviceArray.zeros((1,), np.float64)}
+        self.g = {k: DeviceArray.zeros((S, L) if k == "n_div_U" else (L,), np.float64)
+                  for k in self.GRADS}
+        for k in ("tau_ref", "alpha_ref"):
+            assert self.t[k].shape == (L,), k
+        assert self.t["data"].shape == (n,)
+        t = self.t
+        self.struct = _lib.KorgFitTarget(t["tau_ref"].ptr, t["alpha_ref"].ptr, t["data"].ptr,
+                                         t["sigma"].ptr, t["flux"].ptr, t["chi2"].ptr,
+                                         _lib.KorgLayerGrads(*[self.g[k].ptr for k in self.GRADS]))
+
+    def chi2(self):
+        return float(self.t["chi2"].numpy()[0])
+
+    def flux(self):
+        return self.t["flux"].numpy()
+
+    def grads(self):
+        return {k: v.numpy() for k, v in self.g.items()}
+
+    def set_data(self, data):
+        self.t["data"].copy_from(np.asarray(data, np.float64))
+
+
 class Context:
     """One library context (HIP stream + device scratch) on one device."""
 
@@ -231,6 +272,29 @@ class Context:
         """A ``Plan``: the batched call captured in a HIP graph for these device buffers."""
         return Plan(self, alphas, dprobs, precision, headroom)
 
+    def fit_gradient_batched(self, alphas, dprobs, targets, precision="fp32", synchronize=True):
+        """The fit step (korg_fit_gradient_batched): alphas[k] receives star k's line opacity;
+        targets[k] (a ``FitTarget``) its normalized flux, chi^2 and gradient."""
+        stars, dtc = self._stars(alphas, dprobs)
+        tg = self._targets(targets, dprobs)
+        rc = self._L.korg_fit_gradient_batched(self._h, len(dprobs), stars, tg,
+                                               ctypes.byref(dprobs[0].species), dtc,
+                                               PRECISIONS[precision], 1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
+    def plan_fit(self, alphas, dprobs, targets, precision="fp32", headroom=0.0):
+        """A ``Plan`` of the fit step (korg_plan_fit)."""
+        return Plan(self, alphas, dprobs, precision, headroom, targets=targets)
+
+    @staticmethod
+    def _targets(targets, dprobs):
+        if len(targets) != len(dprobs):
+            raise ValueError("one FitTarget per star")
+        tg = (_lib.KorgFitTarget * len(targets))()
+        for k, t in enumerate(targets):
+            tg[k] = t.struct
+        return tg
+
     def autotune_batched(self, alphas, dprobs, precision="fp32", cache_path=None):
         """Times the tuning candidates on these stars (into scratch alphas), makes the fastest
         this context's tuning and returns it; ``cache_path``: optional results file."""
@@ -300,12 +364,19 @@ class Plan:
     by ``launch``.  Holds its alphas, problems and context, whose device buffers the graph uses;
     new inputs are written into those buffers between replays."""
 
-    def __init__(self, ctx, alphas, dprobs, precision="fp32", headroom=0.0):
+    def __init__(self, ctx, alphas, dprobs, precision="fp32", headroom=0.0, targets=None):
         stars, dtc = ctx._stars(alphas, dprobs)
         self.ctx, self.alphas, self.dprobs = ctx, list(alphas), list(dprobs)
+        self.targets = None if targets is None else list(targets)
         h = ctypes.c_void_p()
-        rc = ctx._L.korg_plan_batched(ctx._h, len(dprobs), stars, ctypes.byref(dprobs[0].species),
-                                      dtc, PRECISIONS[precision], float(headroom), ctypes.byref(h))
+        if targets is None:
+            rc = ctx._L.korg_plan_batched(ctx._h, len(dprobs), stars,
+                                          ctypes.byref(dprobs[0].species), dtc,
+                                          PRECISIONS[precision], float(headroom), ctypes.byref(h))
+        else:
+            rc = ctx._L.korg_plan_fit(ctx._h, len(dprobs), stars, ctx._targets(targets, dprobs),
+                                      ctypes.byref(dprobs[0].species), dtc, PRECISIONS[precision],
+                                      float(headroom), ctypes.byref(h))
         _lib.check(ctx._h, rc)
         self._p = h
 
