@@ -8,6 +8,11 @@ come from HIP events around the same kernels in direct calls.  Prints one JSON l
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
+``--workload fit`` measures the fit step (one JSON line of its own): 64 stars drawn as for the
+batched form, 5000-6000 A at 0.01 A (100,001 points), 100,000 shared lines; per step, the line
+opacity in Float32 with Float64 fix-ups, radiative transfer, chi^2 against a noisy copy of the
+flux and its gradient through the line-opacity adjoint, all in one HIP graph replay.
+
 ``--workload batched`` measures the batched form instead (one JSON line of its own): 256 stars,
 Teff ~ U[4000, 6500] K, log g ~ U[1, 5], [M/H] ~ U[-2.5, 0.5] (scaled solar stand-ins), each with
 a window of U[50, 1000] A inside 4000-7000 A at 0.01 A and its own list of U[5k, 150k] lines;
@@ -58,7 +63,7 @@ def main():
     ap.add_argument("--no-flux", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (PMC byte-counter scale)")
-    ap.add_argument("--workload", choices=("solar_wide", "batched"), default="solar_wide")
+    ap.add_argument("--workload", choices=("solar_wide", "batched", "fit"), default="solar_wide")
     ap.add_argument("--stars", type=int, default=256, help="batched workload: number of stars")
     ap.add_argument("--tuning-cache", default=None,
                     help="batched workload: autotuner results file (default: the one shipped "
@@ -74,6 +79,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "fit":
+        bench_fit(args)
         return
 
     import numpy as np
@@ -349,6 +357,132 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
     plan.close()
     ctx.close()
+
+
+def bench_fit(args):
+    """The fit step of 64 stars as one graph replay per step (forward Float32 + adjoint)."""
+    import numpy as np
+    from korg_line import Context, DeviceProblem, FitTarget, Wavelengths, workloads
+    from korg_line.hipmem import synchronize as device_sync
+    from korg_line.atmosphere import Atmosphere
+
+    n_stars = 64 if args.stars == 256 else args.stars
+    t = time.time()
+    u = lambda stream: workloads.uniform(7, stream, n_stars)
+    teff = 4000.0 + 2500.0 * u(101)
+    logg = 1.0 + 4.0 * u(102)
+    mh = -2.5 + 3.0 * u(103)
+    wls = Wavelengths((5000.0, 6000.0))
+    ll = workloads.synthetic_linelist(7, 100_000, 5000.0, 6000.0, (-4.0, 0.5), (0.0, 5.0))
+    atms = [Atmosphere.scaled(float(teff[k]), float(logg[k]), float(mh[k])) for k in range(n_stars)]
+    probs = [workloads.make_problem(a, ll, wls, 1.0) for a in atms]
+    L = probs[0].n_layers
+    log(f"{n_stars} stars built in {time.time() - t:.1f}s: {len(ll)} shared lines, "
+        f"{probs[0].n_wl} points, {L} layers")
+    ctx = Context(0)
+    cache = args.tuning_cache or TUNING_CACHE
+    dprobs = [DeviceProblem(P) for P in probs]
+    a32 = [d.new_alpha("float32") for d in dprobs]
+    tuning = ctx.autotune_batched(a32, dprobs, "fp32", cache_path=cache)
+    refs = [(10.0 ** a.lgTau5, a.continuum(np.array([5000e-8]))[:, 0]) for a in atms]
+    rng = np.random.default_rng(7)
+    targets = [FitTarget(P, tr, ar, np.ones(P.n_wl), 0.01) for P, (tr, ar) in zip(probs, refs)]
+    # data: the model flux with lines 10% deeper, plus noise -- an iterate away from the optimum,
+    # where the gradient is dominated by the systematic residuals (at the optimum it is a sum of
+    # noise and any flux error is amplified by sqrt(points))
+    ctx.fit_gradient_batched(a32, dprobs, targets, "fp32")
+    for tg in targets:
+        f = tg.flux()
+        tg.set_data(1 - 1.1 * (1 - f) + 0.01 * rng.standard_normal(f.size))
+    device_sync()
+    t = time.time()
+    plan = ctx.plan_fit(a32, dprobs, targets, "fp32")
+    log(f"fit plan captured in {time.time() - t:.2f}s: {plan.info()}")
+    for _ in range(args.warmup):
+        plan.launch()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.launch()
+    device_sync()
+    tstep = (time.perf_counter() - t0) / args.steps
+    log(f"  fit step (graph replay): {tstep * 1e3:.1f} ms ({n_stars / tstep:.1f} gradients/s)")
+    g32 = [tg.grads() for tg in targets]
+    c32 = np.array([tg.chi2() for tg in targets])
+    # the same step with the Float64 forward (the all-Float64 chain)
+    ctx.set_tuning()
+    a64 = [d.new_alpha("float64") for d in dprobs]
+    t64s = [FitTarget(P, tr, ar, tg.t["data"].numpy(), 0.01)
+            for P, (tr, ar), tg in zip(probs, refs, targets)]
+    t0 = time.perf_counter()
+    ctx.fit_gradient_batched(a64, dprobs, t64s, "fp64")
+    t64 = time.perf_counter() - t0
+    c64 = np.array([tg.chi2() for tg in t64s])
+    gerr = 0.0  # per star and quantity: |g32 - g64| / |g64| (2-norms over layers)
+    for ga, gb in zip(g32, [tg.grads() for tg in t64s]):
+        for q in ("T", "n_e", "xi", "n_div_U"):
+            nb = np.linalg.norm(gb[q])
+            if nb > 0:
+                gerr = max(gerr, float(np.linalg.norm(ga[q] - gb[q]) / nb))
+    cerr = float(np.max(np.abs(c32 - c64) / np.abs(c64)))
+    log(f"  Float64 step (direct): {t64 * 1e3:.1f} ms; gradient error {gerr:.2e}, chi2 {cerr:.2e}")
+    cpu = None if args.no_cpu else cpu_baseline_fit(probs[0], refs[0], targets[0].t["data"].numpy())
+    out = {
+        "metric": METRIC,
+        "value": n_stars / tstep,
+        "unit": "gradient-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tstep * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fit step: {n_stars} stars (Teff U[4000,6500], log g U[1,5], [M/H] "
+                        "U[-2.5,0.5]), 5000-6000 A at 0.01 A (100,001 points), 100,000 shared "
+                        "lines, 56 layers; per step: line opacity (fp32 + fp64 fix-ups), radiative "
+                        "transfer, chi^2, line-opacity adjoint (fp64), one HIP graph replay",
+            "parallelism": "single GPU",
+            "tuning": tuning,
+        },
+        "fp64_forward_step_ms": t64 * 1e3,
+        "gradient_max_rel_err_vs_fp64_chain": gerr,
+        "chi2_max_rel_err_vs_fp64_chain": cerr,
+        "reference_gradient": "the all-Float64 device chain (Korg's ForwardDiff needs Julia, "
+                              "which is absent); the adjoint itself is pinned by central "
+                              "differences and the Float64 oracle",
+        "plan": plan.info(),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    ctx.close()
+
+
+def cpu_baseline_fit(P, ref, data):
+    """The Float64 oracle chain (line opacity, transfer, its reverse mode, line adjoint) for one
+    star of the fit workload."""
+    import numpy as np
+    from oracle import oracle
+    threads = min(16, os.cpu_count() or 1)
+    tau_ref, alpha_ref = ref
+    t0 = time.perf_counter()
+    al = np.zeros((P.n_layers, P.n_wl), order="F")
+    oracle.line_absorption(al, P, nthreads=threads, mode=oracle.KORG_CHUNKS)
+    ac = np.array([np.interp(P.all_wls, P.cntm_knots, P.cntm_values[l]) for l in range(P.n_layers)])
+    F = oracle.rt_flux(ac + al, P.all_wls, P.temps, tau_ref, alpha_ref)
+    Fc = oracle.rt_flux(ac, P.all_wls, P.temps, tau_ref, alpha_ref)
+    r = (F / Fc - data) / 0.01
+    G = oracle.rt_flux_vjp(ac + al, P.all_wls, P.temps, tau_ref, alpha_ref, 2 * r / 0.01 / Fc)
+    oracle.line_absorption_vjp(np.asfortranarray(G), P, nthreads=threads)
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: one star's gradient in {dt:.1f}s on {threads} threads")
+    return {"value": 1.0 / dt, "unit": "gradient-evals/s", "cores": threads, "kind": "port",
+            "sample": "one star of the workload, whole: Float64 oracle line opacity (reference "
+                      "thread schedule), transfer and its reverse mode, line adjoint"}
 
 
 def cpu_baseline_batched(stars, evals_per_star):
