@@ -1073,13 +1073,21 @@ void k_wing_mfma(ProfileArgs A) {
         // the star's last point at most; below j0 for padding between two stars' grids (such a
         // tile has no lines and writes nothing)
         const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;
-        double lam[NB];
+        // Even NB: the lanes of a wave hold the 64 consecutive points of a pair of blocks (lane i:
+        // point 64 q + i), each lane evaluates its own point, and one lane-half swap turns the
+        // pair's (R, R^2) into the two blocks' B operands.  NB = 1: both halves hold the block's
+        // 32 points.
+        constexpr bool PAIR = NB % 2 == 0;
+        constexpr int NL = PAIR ? NB / 2 : NB;  // points evaluated per lane
+        double lam[NL];
         int jj[NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            jj[b] = (int)(j0 + 32 * b + n);
+        for (int b = 0; b < NB; ++b) jj[b] = (int)(j0 + 32 * b + n);
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int64_t jq = PAIR ? j0 + 64 * q + lane : (int64_t)jj[q];
             // in units of DELTA_UNIT (points past j1 are never used)
-            lam[b] = S.wls[min((int64_t)jj[b], j1) - go] * (1.0 / DELTA_UNIT);
+            lam[q] = S.wls[min(jq, j1) - go] * (1.0 / DELTA_UNIT);
         }
         floatx16 acc[2][NB];
 #pragma unroll
@@ -1100,7 +1108,7 @@ void k_wing_mfma(ProfileArgs A) {
         // B operands of line `bl` of the batch (R, R^2 | R^3, R^4 per point)
         struct Prep {
             float b0[NB];  // k = 0 | 1: R | R^2 (zero outside the window and in the near zone)
-            float r2[NB];  // R^2
+            float b1[NB];  // k = 2 | 3: R^3 | R^4, set when `full`
             bool full;     // the tile meets the line's tail zone: the R^3, R^4 step is needed
         };
         // Per batch of 64 lines, each lane describes its own line: for every 32-point block, the
@@ -1119,22 +1127,43 @@ void k_wing_mfma(ProfileArgs A) {
             }
             full_mask = ballot(m.flo <= j1s && m.fhi >= j0s);
         };
+        // lanes 32..63 of x exchanged with lanes 0..31 of y: x becomes (x_lo | y_lo), y (x_hi | y_hi)
+        auto swap_halves = [](float& x, float& y) {
+            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y),
+                                                            false, false);
+            x = __uint_as_float(r[0]);
+            y = __uint_as_float(r[1]);
+        };
         auto prep = [&](const LineMeta& m, int bl) {
             Prep p;
             const double lam0 = readlane_f64(lam0s, bl);
             p.full = (full_mask >> bl) & 1;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                // lanes n and n + 32 hold the same point
-                const uint32_t mw = (uint32_t)readlane_i32((int)mk[b], bl);
-                const bool use = __builtin_amdgcn_inverse_ballot_w64(mw | ((uint64_t)mw << 32));
+            for (int q = 0; q < NL; ++q) {
+                const uint32_t mw = (uint32_t)readlane_i32((int)mk[q * (PAIR ? 2 : 1)], bl);
+                // the lane's point in the window and outside the near zone
+                const uint32_t mw1 = PAIR ? (uint32_t)readlane_i32((int)mk[2 * q + 1], bl) : mw;
+                const bool use = __builtin_amdgcn_inverse_ballot_w64(mw | ((uint64_t)mw1 << 32));
                 // Float64 fix-up: lambda_j - lambda0 from the Float64 grid values (both scaled
                 // to DELTA_UNIT, which adds a rounding of ~1e-10 of the offset)
-                const float d = (float)(lam[b] - lam0);
+                const float d = (float)(lam[q] - lam0);
                 float R = __builtin_amdgcn_rcpf(d * d);
                 R = use ? R : 0.0f;
-                p.r2[b] = R * R;
-                p.b0[b] = hi ? p.r2[b] : R;
+                float R2 = R * R;
+                if constexpr (PAIR) {
+                    if (p.full) {
+                        float R3 = R * R2, R4 = R2 * R2;
+                        swap_halves(R3, R4);
+                        p.b1[2 * q] = R3;
+                        p.b1[2 * q + 1] = R4;
+                    }
+                    swap_halves(R, R2);  // (R, R^2) of block 2q, then of block 2q + 1
+                    p.b0[2 * q] = R;
+                    p.b0[2 * q + 1] = R2;
+                } else {
+                    p.b0[q] = hi ? R2 : R;  // lanes n and n + 32 hold the same point
+                    if (p.full) p.b1[q] = p.b0[q] * R2;
+                }
             }
             return p;
         };
@@ -1151,9 +1180,8 @@ void k_wing_mfma(ProfileArgs A) {
             if (p.full) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
-                    const float b1 = p.b0[b] * p.r2[b];  // k = 2 | 3: R^3 | R^4
-                    acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, b1, acc[0][b], 0, 0, 0);
-                    acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, b1, acc[1][b], 0, 0, 0);
+                    acc[0][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, p.b1[b], acc[0][b], 0, 0, 0);
+                    acc[1][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, p.b1[b], acc[1][b], 0, 0, 0);
                 }
             }
             pin_acc();
