@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,6 +60,10 @@ constexpr double SERIES_EPS = 2e-6;
 // line's tail radius, where they are smaller than this fraction of the leading term in every
 // layer).
 constexpr double SERIES_TAIL_EPS = 1e-6;
+// The same bound for the derivatives of the series in the Float32 adjoint: its near zone (exact
+// evaluation) extends to where the first dropped term's a- and v-derivatives, relative to the
+// leading term's, are below this.
+constexpr double SERIES_ADJ_EPS = 2e-7;
 
 // --------------------------------------------------------------------------------------------
 // wave helpers
@@ -298,6 +303,7 @@ struct SetupArgs {
     double4* rec64;  // [line * Ls + l]: (1/(sigma sqrt 2), a, scaling, 0)
     float4* recM;    // [(line * n_lblocks + b) * 64 + lane]: far-wing coefficients, MFMA A layout
     unsigned long long* counters;
+    int2* azone;     // Float32 form, when given: each line's adjoint near zone (global indices)
 };
 
 constexpr double INV_SQRT_PI = 0.5641895835477563;
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         kidx = __builtin_amdgcn_readfirstlane((int)ub) - 1;
         kidx = kidx < 0 ? 0 : (kidx > S.n_knots - 2 ? S.n_knots - 2 : kidx);
     }
-    double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0;
+    double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0, amax = -1.0;
     bool any_nan = false;
     bool f32_bad = false;  // a far-wing coefficient of some layer is beyond Float32 range
     double my_hw = -1.0, my_a = 0.0;  // layer `lane` (first 64 layers), for branch counting
@@ -422,6 +428,18 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             }
             const double r = vr * s2;
             rmax = r > rmax ? r : rmax;
+            if (F32 && A.azone) {
+                // adjoint near zone: the dropped term e5 w^5 of the series, differentiated,
+                // (a e5)' w^4 and 5 e5 w^4 (against the leading term's), below SERIES_ADJ_EPS
+                double va = vr;
+                if (!(a <= 0.2)) {
+                    const double a4 = (a * a) * (a * a);
+                    const double e5 = fabs(a4 * (94.25 - 18.0 * a * a + a4));
+                    const double f5 = fabs(a4 * (471.25 - 126.0 * a * a + 9.0 * a4));
+                    va = fmax(va, root_up(fmax(f5, 10.0 * e5) * (1.0 / SERIES_ADJ_EPS), 0.125f));
+                }
+                amax = fmax(amax, va * s2);
+            }
             const double rt = vt * s2;
             tmax = rt > tmax ? rt : tmax;
             if (l == lane) {
@@ -479,6 +497,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double lwin = wave_max(lmax);
     const double reach = wave_max(rmax);
     const double treach = wave_max(tmax);
+    const double areach = (F32 && A.azone) ? wave_max(amax) : -1.0;
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
     const int64_t n = S.n_wl;
     const double last = S.wls[n - 1];
@@ -487,17 +506,21 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     bool upper = false;
     const double h = reach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
     const double ht = treach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
+    const double ha = areach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
     if (lane == 0) x = wl - window;
     if (lane == 1) { x = wl + window; upper = true; }
     if (lane == 3) x = wl - h;
     if (lane == 4) { x = wl + h; upper = true; }
     if (lane == 5) x = wl - ht;
     if (lane == 6) { x = wl + ht; upper = true; }
+    if (lane == 7) x = wl - ha;
+    if (lane == 8) { x = wl + ha; upper = true; }
     if (lane <= 1 && x >= 1.0) x = x * 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
     const bool past = !(x <= last);           // beyond the grid (or NaN)
-    const int64_t r = wave_bounds(S.wls, n, gi.g0, gi.ginv, x, upper, lane < 7 && !past, lane);
+    const int nsearch = (F32 && A.azone) ? 9 : 7;
+    const int64_t r = wave_bounds(S.wls, n, gi.g0, gi.ginv, x, upper, lane < nsearch && !past, lane);
     int64_t res = r;
-    if (lane < 7 && past) res = n;  // lower bound n; upper bound n (last index n - 1)
+    if (lane < nsearch && past) res = n;  // lower bound n; upper bound n (last index n - 1)
     const int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
     const int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
     const int64_t jc = __builtin_amdgcn_readlane((int)res, 2);
@@ -515,11 +538,20 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         flo = flo < lb ? lb : flo;
         fhi = fhi > ub ? ub : fhi;
     }
+    int64_t alo = clo, ahi = chi;  // the adjoint's near zone contains the forward's
+    if (F32 && A.azone && lb <= ub && areach >= 0.0) {
+        alo = min((int64_t)__builtin_amdgcn_readlane((int)res, 7), clo);
+        ahi = max((int64_t)__builtin_amdgcn_readlane((int)res, 8) - 1, chi);
+        alo = alo < lb ? lb : alo;
+        ahi = ahi > ub ? ub : ahi;
+    }
     if (series_off && lb <= ub) {
         clo = lb;
         chi = ub;
         flo = 1;
         fhi = 0;
+        alo = lb;
+        ahi = ub;
     }
     if (CB && lb <= ub && lane < L) {
         // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
@@ -560,6 +592,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         mt.flo = (int32_t)(flo + go);
         mt.fhi = (int32_t)(fhi + go);
         A.meta[i] = mt;
+        if (F32 && A.azone) A.azone[i] = make_int2((int)(alo + go), (int)(ahi + go));
     }
 }
 
@@ -641,19 +674,20 @@ __global__ __launch_bounds__(PART_BLOCK) void k_count(int64_t n, int L, int wing
     }
 }
 
-// One block: exclusive scan of block_counts per class; class_start[c] = first slot of class c,
-// class_start[N_CLASSES] = total.
+// One block: exclusive scan of block_counts ([block][NC]) per class; class_start[c] = first slot of
+// class c, class_start[NC] = total.
+template <int NC>
 __global__ __launch_bounds__(1024) void k_scan(int nblocks, int* __restrict__ block_counts,
                                                int* __restrict__ class_start) {
     __shared__ int part[1024];
-    __shared__ int totals[N_CLASSES];
+    __shared__ int totals[NC];
     const int t = threadIdx.x;
-    for (int c = 0; c < N_CLASSES; ++c) {
+    for (int c = 0; c < NC; ++c) {
         // each thread owns a contiguous run of blocks
         const int per = (nblocks + 1023) / 1024;
         const int b0 = t * per, b1 = min(nblocks, b0 + per);
         int s = 0;
-        for (int b = b0; b < b1; ++b) s += block_counts[(int64_t)b * N_CLASSES + c];
+        for (int b = b0; b < b1; ++b) s += block_counts[(int64_t)b * NC + c];
         part[t] = s;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -664,8 +698,8 @@ __global__ __launch_bounds__(1024) void k_scan(int nblocks, int* __restrict__ bl
         }
         int run = part[t] - s;  // exclusive prefix of this thread's run
         for (int b = b0; b < b1; ++b) {
-            const int v = block_counts[(int64_t)b * N_CLASSES + c];
-            block_counts[(int64_t)b * N_CLASSES + c] = run;
+            const int v = block_counts[(int64_t)b * NC + c];
+            block_counts[(int64_t)b * NC + c] = run;
             run += v;
         }
         if (t == 1023) totals[c] = part[t];
@@ -673,11 +707,11 @@ __global__ __launch_bounds__(1024) void k_scan(int nblocks, int* __restrict__ bl
     }
     if (t == 0) {
         int acc = 0;
-        for (int c = 0; c < N_CLASSES; ++c) {
+        for (int c = 0; c < NC; ++c) {
             class_start[c] = acc;
             acc += totals[c];
         }
-        class_start[N_CLASSES] = acc;
+        class_start[NC] = acc;
     }
 }
 
@@ -1404,15 +1438,37 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
 // it leaves one partial sum per output (k_vjp), and k_vjp_reduce sums a star's segments in a fixed
 // order: the gradients are deterministic, and the scratch is per segment, not per item.
 // ---------------------------------------------------------------------------------------------
+// No point of the window lies outside the near zone [z.x, z.y] (or the window is empty).
+__device__ inline bool far_zone_empty(const LineMeta& m, int2 z) {
+    return m.lb > m.ub || (z.x <= m.lb && z.y >= m.ub);
+}
+
 constexpr int VJP_CHUNK = 2048;
 constexpr int VJP_SEG = 64;
+// Points whose loads are in flight together: 8 for the Float32 evaluation, 2 for the Float64
+// one (whose registers allow no more at two waves per SIMD).
+template <bool SERIES>
+constexpr int vjp_batch() { return SERIES ? 8 : 2; }
 
+// Items per line: its window in chunks (exact form), or (series form: the far zone comes from the
+// moments) its near zone in chunks, at least one item for a line with a window (the first item of
+// a line carries the far-zone sums).
+// (azone: the adjoint's near zones, given in the series form.)
 __global__ __launch_bounds__(256) void k_vjp_count(int64_t n, const LineMeta* __restrict__ meta,
+                                                   const int2* __restrict__ azone,
                                                    int* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const LineMeta m = meta[i];
-    cnt[i] = m.lb <= m.ub ? (m.ub - m.lb + VJP_CHUNK) / VJP_CHUNK : 0;
+    if (m.lb > m.ub) {
+        cnt[i] = 0;
+    } else if (!azone) {
+        cnt[i] = (m.ub - m.lb + VJP_CHUNK) / VJP_CHUNK;
+    } else {
+        const int2 z = azone[i];
+        const int nz = z.x <= z.y ? z.y - z.x + 1 : 0;
+        cnt[i] = nz > 0 ? (nz + VJP_CHUNK - 1) / VJP_CHUNK : 1;
+    }
 }
 
 // Segments per star from the items' exclusive scan over all lines (item_start[N] = total).
@@ -1440,8 +1496,16 @@ struct VjpArgs {
     double* partial;              // [segment][6 + n_species][L]
     int64_t seg_cap;              // segments the partials hold
     const unsigned long long* counters;
+    // series form: items cover the adjoint's near zones (evaluated in Float32), and a line's
+    // first item adds its far-zone sums from the moments (k_vjp_moments' layout, G scaled by
+    // 2^gexp[star * L + l])
+    int n_lblocks;
+    const int2* azone;
+    const float4* mom;
+    const int* gexp;
 };
 
+template <bool SERIES>
 __global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
@@ -1486,8 +1550,12 @@ __global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
             while (A.item_start[i + 1] <= item) ++i;  // lines with no items are passed over
             const int64_t li = i - S.line_off;
             const LineMeta m = A.meta[i];
-            const int64_t j0 = m.lb + (item - A.item_start[i]) * VJP_CHUNK;
-            const int64_t j1 = min(j0 + VJP_CHUNK, (int64_t)m.ub + 1);  // exclusive
+            const bool first = item == A.item_start[i];
+            // the item's points: a chunk of the window, or (series form) of the near zone
+            const int2 z = SERIES ? A.azone[i] : make_int2(m.lb, m.ub);
+            const int64_t j0 = z.x + (item - A.item_start[i]) * VJP_CHUNK;
+            const int64_t j1 = z.x > z.y ? j0 : min(j0 + VJP_CHUNK, (int64_t)z.y + 1);  // exclusive
+            const bool far = SERIES && first && !far_zone_empty(m, z);
             const double lam0 = m.lam0;
             const int32_t s = S.species[li];
             const double mass = A.mass[s];
@@ -1501,16 +1569,63 @@ __global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
                 const double4 r = A.rec64[i * A.Ls + l];
                 const double inv = r.x, a = r.y, scaling = r.z;
                 double SH = 0.0, Sa = 0.0, Sv = 0.0;
-                for (int64_t j = j0; j < j1; ++j) {
-                    const int64_t jl = j - go;  // the star's own point index
-                    const double g = A.g_f32 ? (double)((const float*)S.G)[l + (int64_t)L * jl]
-                                             : ((const double*)S.G)[l + (int64_t)L * jl];
-                    const double v = fabs(S.wls[jl] - lam0) * inv;
-                    double Ha, Hv;
-                    const double H = voigt_hjerting64_grad(a, v, Ha, Hv);
-                    SH += g * H;
-                    Sa += g * Ha;
-                    Sv += g * v * Hv;
+                // points in batches: the batch's loads are issued before its evaluations (points
+                // past j1 load a stand-in and are skipped)
+                constexpr int NBT = vjp_batch<SERIES>();
+                for (int64_t jb = j0; jb < j1; jb += NBT) {
+                    double gv[NBT], lv[NBT];
+#pragma unroll
+                    for (int u = 0; u < NBT; ++u) {
+                        const int64_t jl = min(jb + u, j1 - 1) - go;  // the star's own index
+                        gv[u] = A.g_f32 ? (double)((const float*)S.G)[l + (int64_t)L * jl]
+                                        : ((const double*)S.G)[l + (int64_t)L * jl];
+                        lv[u] = S.wls[jl];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NBT; ++u) {
+                        if (jb + u >= j1) break;
+                        const double g = gv[u];
+                        const double v = fabs(lv[u] - lam0) * inv;
+                        if (SERIES) {
+                            float Ha, Hv;
+                            const float H = voigt_hjerting_grad_f32(a, v, Ha, Hv);
+                            SH += g * H;
+                            Sa += g * Ha;
+                            Sv += g * (v * Hv);
+                        } else {
+                            double Ha, Hv;
+                            const double H = voigt_hjerting64_grad(a, v, Ha, Hv);
+                            SH += g * H;
+                            Sa += g * Ha;
+                            Sv += g * v * Hv;
+                        }
+                    }
+                }
+                if (far) {
+                    // far-zone sums from the moments: Q_k = q^k M_k, e_k and (a e_k)' of the
+                    // series (branch A: a <= 0.2, else D)
+                    const float4 mk = A.mom[(i * A.n_lblocks + l / WAVE) * WAVE + l % WAVE];
+                    const int e = A.gexp[(int64_t)st * L + l];
+                    const double w = 1.0 / (inv * DELTA_UNIT);
+                    const double q = w * w;
+                    const double Q1 = ldexp((double)mk.x, e) * q;
+                    const double Q2 = ldexp((double)mk.y, e) * (q * q);
+                    const double Q3 = ldexp((double)mk.z, e) * (q * q * q);
+                    const double Q4 = ldexp((double)mk.w, e) * ((q * q) * (q * q));
+                    double e2 = 1.5, e3 = 3.75, e4 = 0.0, f2 = 1.5, f3 = 3.75, f4 = 0.0;
+                    if (!(a <= 0.2)) {
+                        const double a2 = a * a;
+                        e2 = 1.5 - a2;
+                        e3 = 3.75 - 5.0 * a2 + a2 * a2;
+                        e4 = -26.25 * a2 + 10.5 * a2 * a2 - a2 * a2 * a2;
+                        // (a e_k)' = e_k + a e_k'
+                        f2 = 1.5 - 3.0 * a2;
+                        f3 = 3.75 - 15.0 * a2 + 5.0 * a2 * a2;
+                        f4 = -78.75 * a2 + 52.5 * a2 * a2 - 7.0 * a2 * a2 * a2;
+                    }
+                    SH += a * INV_SQRT_PI * (Q1 + e2 * Q2 + e3 * Q3 + e4 * Q4);
+                    Sa += INV_SQRT_PI * (Q1 + f2 * Q2 + f3 * Q3 + f4 * Q4);
+                    Sv += a * INV_SQRT_PI * (-2.0 * Q1 - 4.0 * e2 * Q2 - 6.0 * e3 * Q3 - 8.0 * e4 * Q4);
                 }
                 const LayerTab tb = A.tab[(int64_t)st * L + l];
                 const double T = S.temps[l], xi = S.xi[l];
@@ -1656,11 +1771,15 @@ struct RtTab {
     double f, ltau;
 };
 
-__global__ void k_rt_tab(int n_stars, int L, const StarDev* __restrict__ stars, RtTab* __restrict__ tab) {
+// Also resets the per-(star, layer) scale exponents of G (when given) for k_g_exp's maxima.
+__global__ void k_rt_tab(int n_stars, int L, const StarDev* __restrict__ stars, RtTab* __restrict__ tab,
+                         int* __restrict__ gexp, int gexp_none) {
     const int st = blockIdx.x;
     const StarDev& S = stars[st];
-    for (int l = threadIdx.x; l < L; l += blockDim.x)
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
         tab[(int64_t)st * L + l] = RtTab{S.tau_ref[l] / S.alpha_ref[l], log(S.tau_ref[l])};
+        if (gexp) gexp[(int64_t)st * L + l] = gexp_none;
+    }
 }
 
 // Zeroes every star's alpha: thread t of the global (point, layer) space.
@@ -1821,6 +1940,274 @@ __global__ __launch_bounds__(256) void k_star_chi2(const StarDev* __restrict__ s
     if (t == 0 && S.chi2) *S.chi2 = red[0];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Adjoint of the Float32 form (the fit step at Float32 precision): the far zone of every line
+// through moments of the series, the near zone exactly (k_vjp over near-zone items).
+//
+// In a line's far zone the forward adds phi = scaling H with the series of the header,
+// H = (a/sqrt(pi)) sum_k e_k(a) v^-2k, v^-2 = q R (R = 1/delta^2, q = 1/(DELTA_UNIT inv)^2).  The
+// sums the chain rule takes over those points therefore follow from four moments per (line, layer),
+//   M_k = sum_{j in far zone} G_j R_j^k,  k = 1..4:
+//   sum G H = (a/sqrt(pi)) sum_k e_k q^k M_k,   sum G dH/da = (1/sqrt(pi)) sum_k (a e_k)' q^k M_k,
+//   sum G v dH/dv = (a/sqrt(pi)) sum_k (-2k) e_k q^k M_k.
+// The moments of a group of lines that share points are a matrix product: for 16 layers x
+// (4 lines x 4 powers), G[layer, point] P[point, (line, k)] on the matrix cores
+// (v_mfma_f32_16x16x4_f32, K = 4 points).  Lines are grouped by window width in classes of a
+// factor 2, so the lines of a group cover nearly the same points.  G enters in Float32, scaled
+// per (star, layer) by the power of two that brings its largest magnitude into [0.5, 1).  The R^3
+// and R^4 moments cover the whole far zone (the forward drops those terms beyond the tail radius,
+// where they are below SERIES_TAIL_EPS of the leading one).
+// ---------------------------------------------------------------------------------------------
+constexpr int ADJ_NC = 24;        // grouping classes: window width W in (2^(c-1), 2^c] points
+constexpr int ADJ_GROUP = 4;      // lines per group: the 4 line slots of the 16 B columns
+constexpr int GEXP_NONE = -2000;  // scale exponent of an all-zero layer of G
+
+__device__ inline int adj_class(const LineMeta& m, int2 z) {
+    if (far_zone_empty(m, z)) return ADJ_NC;
+    const uint32_t W = (uint32_t)(m.ub - m.lb + 1);
+    const int c = W <= 1 ? 0 : 32 - __clz(W - 1);
+    return c < ADJ_NC ? c : ADJ_NC - 1;
+}
+
+// Class counts per block of lines (the partition's first pass).
+__global__ __launch_bounds__(PART_BLOCK) void k_adj_count(int64_t n, const LineMeta* __restrict__ meta,
+                                                          const int2* __restrict__ azone,
+                                                          int* __restrict__ block_counts) {
+    __shared__ int cnt[ADJ_NC];
+    if (threadIdx.x < ADJ_NC) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    if (i < n) {
+        const int c = adj_class(meta[i], azone[i]);
+        if (c < ADJ_NC) atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < ADJ_NC)
+        block_counts[(int64_t)blockIdx.x * ADJ_NC + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter of the lines with a far zone by class: within a class, in line order.
+__global__ __launch_bounds__(PART_BLOCK) void k_adj_scatter(int64_t n, const LineMeta* __restrict__ meta,
+                                                            const int2* __restrict__ azone,
+                                                            const int* __restrict__ block_offsets,
+                                                            const int* __restrict__ class_start,
+                                                            int* __restrict__ adj_line) {
+    __shared__ int wave_cnt[PART_BLOCK / WAVE][ADJ_NC];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    const int c = i < n ? adj_class(meta[i], azone[i]) : ADJ_NC;
+    int rank = 0;
+    for (int k = 0; k < ADJ_NC; ++k) {
+        const uint64_t b = ballot(c == k);
+        if (c == k) rank = __popcll(b & ((1ull << lane) - 1));
+        if (lane == 0) wave_cnt[w][k] = __popcll(b);
+    }
+    __syncthreads();
+    if (c < ADJ_NC) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += wave_cnt[ww][c];
+        adj_line[class_start[c] + block_offsets[(int64_t)blockIdx.x * ADJ_NC + c] + before + rank] = (int)i;
+    }
+}
+
+// Per (star, layer): the binary exponent e of max_j |G[l, j]| (max |G| 2^-e in [0.5, 1)), by an
+// order-free atomicMax over wavefronts.  A wavefront per 128 points (one star: the stars' grids are
+// padded to 128 points); lanes = layers.
+__global__ __launch_bounds__(256) void k_g_exp(int n_stars, int L, int64_t n_total,
+                                               const StarDev* __restrict__ stars, int* __restrict__ gexp) {
+    const int lane = threadIdx.x % WAVE;
+    const int64_t j0 = ((int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * STAR_ALIGN;
+    if (j0 >= n_total) return;
+    const int st = star_of_point(stars, n_stars, j0);
+    const StarDev& S = stars[st];
+    const int64_t jl0 = j0 - S.grid_off, jl1 = min(jl0 + STAR_ALIGN, S.n_wl);
+    const double* __restrict__ G = (const double*)S.G;
+    for (int l = lane; l < L; l += WAVE) {
+        double mx = 0.0;
+        for (int64_t jl = jl0; jl < jl1; ++jl) mx = fmax(mx, fabs(G[l + (int64_t)L * jl]));
+        if (mx > 0.0) {
+            int e;
+            frexp(mx, &e);
+            atomicMax(&gexp[(int64_t)st * L + l], e);
+        }
+    }
+}
+
+// K steps per block of the moments kernel (loads in flight together), per Float32 partial sum
+// (a multiple of the block), and the points the kernel may read past the last grid point.
+constexpr int MOM_BLOCK = 4;
+constexpr int MOM_CHUNK = 16;
+constexpr int MOM_PAD = 4 * MOM_BLOCK;
+
+// G in Float32, scaled by 2^-e per (star, layer), as [layer block][global point][64 layers] with
+// MOM_PAD points after the last (layers past L and points outside every star zero): the moments'
+// A operand, 1 KB per 4 points.  Also the wavelengths in units of DELTA_UNIT on the same global
+// points (zero outside the stars).
+__global__ __launch_bounds__(256) void k_g_pack(int n_stars, int L, int n_lblocks, int64_t n_total,
+                                                const StarDev* __restrict__ stars,
+                                                const int* __restrict__ gexp, float* __restrict__ g32,
+                                                double* __restrict__ lam_s) {
+    const int lane = threadIdx.x % WAVE;
+    const int64_t j0 = ((int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * STAR_ALIGN;
+    if (j0 >= n_total) return;
+    const int st = star_of_point(stars, n_stars, j0);
+    const StarDev& S = stars[st];
+    const int64_t go = S.grid_off;
+    const int64_t jl0 = j0 - go;
+    // the last chunk also fills the padding after the grid
+    const int64_t np = STAR_ALIGN + (j0 + STAR_ALIGN >= n_total ? MOM_PAD : 0);
+    const double* __restrict__ G = (const double*)S.G;
+    const int64_t stride = (n_total + MOM_PAD) * WAVE;
+    for (int b = 0; b < n_lblocks; ++b) {
+        const int l = b * WAVE + lane;
+        const int e = l < L ? gexp[(int64_t)st * L + l] : 0;
+        float* __restrict__ dst = g32 + (int64_t)b * stride + lane;
+        for (int64_t p = 0; p < np; ++p) {
+            const int64_t jl = jl0 + p;
+            dst[(j0 + p) * WAVE] = l < L && jl < S.n_wl ? (float)ldexp(G[l + (int64_t)L * jl], -e) : 0.0f;
+        }
+    }
+    for (int64_t p = lane; p < np; p += WAVE) {
+        const int64_t jl = jl0 + p;
+        lam_s[j0 + p] = jl < S.n_wl ? S.wls[jl] * (1.0 / DELTA_UNIT) : 0.0;
+    }
+}
+
+struct MomArgs {
+    int n_stars, L, n_lblocks;
+    int64_t n_total;           // global grid points
+    const StarDev* stars;
+    const LineMeta* meta;
+    const int2* azone;         // the adjoint's near zones
+    const int* adj_line;       // the lines with a far zone, by class (ascending width)
+    const int* adj_start;      // [ADJ_NC + 1]; adj_start[ADJ_NC] = the number of such lines
+    const float* g32;          // k_g_pack's layouts
+    const double* lam_s;
+    float4* mom;               // [(line * n_lblocks + b) * 64 + l % 64]: (M1, M2, M3, M4) / 2^e
+    unsigned long long* counters;
+};
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// One wavefront per (group of ADJ_GROUP lines, 64-layer block), widest classes first.  Lane i of a
+// K step: A = scaled G of layers 4 (i % 16) + m (m = 0..3, one MFMA each) at point j0 + i / 16;
+// B = R^(p+1) of line slot (i % 16) / 4 at that point, p = i % 4 (zero outside the line's far
+// zone: its window less its adjoint near zone).  The lines of a group are handled star by star (a group can straddle two stars), each
+// pass over the union of its lines' windows.  The wavefront's 52 VGPRs and 16 AGPRs allow 8 waves
+// per SIMD, which hide the L2 latency of the G and wavelength loads.
+__global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
+    const int lane = threadIdx.x;
+    if (A.counters[3] != 0) return;  // failed setup, or a capacity shortfall
+    const int64_t n_part = A.adj_start[ADJ_NC];
+    const int64_t n_groups = (n_part + ADJ_GROUP - 1) / ADJ_GROUP;
+    const int64_t n_items = n_groups * A.n_lblocks;
+    const int col = lane & 15, kk = lane >> 4;
+    const int slot = col >> 2, pw = col & 3;
+    for (;;) {
+        const int item = wave_take(&A.counters[15], lane);
+        if (item >= n_items) break;
+        const int64_t g = n_groups - 1 - item / A.n_lblocks;
+        const int lblk = item % A.n_lblocks;
+        const int64_t p0 = g * ADJ_GROUP;
+        const int nl = (int)min((int64_t)ADJ_GROUP, n_part - p0);
+        int line = -1, st = -1;
+        LineMeta m = {0.0, 1, 0, 1, 0, 0, ADJ_NC, 1, 0};
+        int2 z = make_int2(1, 0);
+        if (slot < nl) {
+            line = A.adj_line[p0 + slot];
+            m = A.meta[line];
+            z = A.azone[line];
+            st = star_of_line(A.stars, A.n_stars, line);
+        }
+        uint32_t todo = (1u << nl) - 1;  // line slots not yet handled
+        while (todo) {
+            const int sc = readlane_i32(st, 4 * (__ffs(todo) - 1));  // lane 4k holds slot k
+            uint32_t pass = 0;
+            int u0 = INT_MAX, u1 = INT_MIN;
+            for (int k = 0; k < nl; ++k) {
+                if (readlane_i32(st, 4 * k) != sc || !((todo >> k) & 1)) continue;
+                pass |= 1u << k;
+                u0 = min(u0, readlane_i32(m.lb, 4 * k));
+                u1 = max(u1, readlane_i32(m.ub, 4 * k));
+            }
+            todo &= ~pass;
+            const bool act = slot < nl && ((pass >> slot) & 1);
+            const double lam0s = m.lam0 * (1.0 / DELTA_UNIT);
+            // the lane's far zone [lb, clo) and (chi, ub] as bounds on the step's first point j0
+            // (the lane's point is j0 + kk); not in this pass: no points
+            const int lbr = (act ? m.lb : 1) - kk, ubr = (act ? m.ub : 0) - kk;
+            const int clor = z.x - kk, chir = z.y - kk;
+            // Float32 sums over chunks of MOM_CHUNK K steps, added into Float64 totals: a far
+            // zone spans thousands of points and its moments enter differences (core against
+            // wing) in the gradients of the damping.  Chunks end at fixed points (multiples of
+            // 4 MOM_CHUNK), so a line's sums do not depend on its group either.
+            floatx4 acc[4];
+            double tot[4][4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tot[q][r] = 0.0;
+            }
+            // K steps start at multiples of 4 (a star's grid starts at one): every point falls in
+            // the same step whatever lines share the group, and steps outside a line's far zone add
+            // exact zeros, so a line's moments do not depend on its group.  Steps go in blocks of
+            // MOM_BLOCK, the block's loads issued before its MFMAs; the last block reads at most
+            // 4 MOM_BLOCK - 1 points past the union (B = 0 there; zeros past the grid).
+            const int js = u0 & ~(4 * MOM_BLOCK - 1);
+            const float4* __restrict__ gp =
+                (const float4*)(A.g32 + ((int64_t)lblk * (A.n_total + MOM_PAD) + js) * WAVE);
+            const double* __restrict__ lp = A.lam_s + js;
+            for (int jb = js; jb <= u1; jb += 4 * MOM_BLOCK, gp += MOM_BLOCK * WAVE, lp += 4 * MOM_BLOCK) {
+                // A: layers 4 col .. 4 col + 3 at point j0 + kk; the wavelength of that point
+                float4 a[MOM_BLOCK];
+                double lam[MOM_BLOCK];
+#pragma unroll
+                for (int u = 0; u < MOM_BLOCK; ++u) {
+                    a[u] = gp[u * WAVE + lane];
+                    lam[u] = lp[4 * u + kk];
+                }
+#pragma unroll
+                for (int u = 0; u < MOM_BLOCK; ++u) {
+                    const int j0 = jb + 4 * u;
+                    const float d = (float)(lam[u] - lam0s);
+                    float R = __builtin_amdgcn_rcpf(d * d);
+                    const bool far = j0 >= lbr && j0 <= ubr && (j0 < clor || j0 > chir);
+                    R = far ? R : 0.0f;
+                    const float R2 = R * R;
+                    const float b = ((pw & 1) ? R2 : R) * ((pw & 2) ? R2 : 1.0f);
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b, acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b, acc[1], 0, 0, 0);
+                    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, b, acc[2], 0, 0, 0);
+                    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, b, acc[3], 0, 0, 0);
+                }
+                // chunk ends (MOM_CHUNK steps) fall on block ends
+                if (((jb >> 2) & (MOM_CHUNK - 1)) == MOM_CHUNK - MOM_BLOCK) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) tot[q][r] += (double)acc[q][r];
+                        acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tot[q][r] += (double)acc[q][r];
+            if (act) {
+                // accumulator layout of v_mfma_f32_16x16x4_f32: lane i, register r holds row
+                // 4 (i / 16) + r, column i % 16; row i of MFMA m is layer 4 i + m of the block
+                float* __restrict__ dst = (float*)(A.mom + ((int64_t)line * A.n_lblocks + lblk) * WAVE);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dst[(4 * (4 * kk + r) + q) * 4 + pw] = (float)tot[q][r];
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -1838,8 +2225,9 @@ struct Workspace {
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
-    // adjoint and fit step
-    DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part;
+    // adjoint and fit step (adj_*, gexp, g32: the series form's moments)
+    DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part, adj_blk,
+        adj_start, adj_line, gexp, g32, lam_s, azone;
     int64_t tile_cap = 0;             // entries tile_lines holds
     int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
@@ -1849,7 +2237,8 @@ struct Workspace {
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &vjp_cnt,
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
-                          &chi2_part};
+                          &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
+                          &lam_s,     &azone};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -2064,6 +2453,7 @@ struct Batch {
     int n_stars = 0, L = 0;
     int64_t N = 0, nwl = 0;
     bool f32 = false, alpha_f32 = false;
+    bool adj = false;  // Float32 fit step: the adjoint's series form (near zones, moments)
     int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, count_branches = 0;
     int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
     korg_species sp{};
@@ -2164,6 +2554,7 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.rec64 = (double4*)ws.rec64.p;
     sa.recM = (float4*)ws.recM.p;
     sa.counters = counters;
+    sa.azone = b.adj ? (int2*)ws.azone.p : nullptr;
     const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);
     if (b.f32) {
         if (b.count_branches) hipLaunchKernelGGL((k_setup<true, true>), sgrid, sblock, 0, s, sa);
@@ -2188,7 +2579,7 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
 
     hipLaunchKernelGGL(k_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N, b.L, 32 * b.NB,
                        (const LineMeta*)ws.meta.p, (int*)ws.blk.p, (int*)ws.nblk.p, counters);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.blk.p,
+    hipLaunchKernelGGL(k_scan<N_CLASSES>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.blk.p,
                        (int*)ws.cstart.p);
     hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
                        (const LineMeta*)ws.meta.p, (const int*)ws.blk.p, (const int*)ws.cstart.p,
@@ -2199,7 +2590,7 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
                        (int2*)ws.ranges.p);
     if (b.f32) {
         // near-zone partition (by reach class) and its per-tile candidate ranges
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.nblk.p,
+        hipLaunchKernelGGL(k_scan<N_CLASSES>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.nblk.p,
                            (int*)ws.ncstart.p);
         hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
                            (const LineMeta*)ws.meta.p, (const int*)ws.nblk.p,
@@ -2426,7 +2817,8 @@ int enqueue_vjp_counts(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t
     auto* counters = (unsigned long long*)ws.counters.p;
     if (b.N > 0)
         hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((b.N + 255) / 256)), dim3(256), 0, s, b.N,
-                           (const LineMeta*)ws.meta.p, (int*)ws.vjp_cnt.p);
+                           (const LineMeta*)ws.meta.p, b.adj ? (const int2*)ws.azone.p : nullptr,
+                           (int*)ws.vjp_cnt.p);
     hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, b.N, (const int*)ws.vjp_cnt.p,
                        (int64_t*)ws.vjp_start.p, (int64_t)INT64_MAX, (unsigned long long*)nullptr, 0);
     hipLaunchKernelGGL(k_vjp_segs, dim3((unsigned)((b.n_stars + 255) / 256)), dim3(256), 0, s,
@@ -2472,8 +2864,13 @@ int enqueue_vjp(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int
     va.partial = (double*)ws.vjp_partial.p;
     va.seg_cap = ws.seg_cap;
     va.counters = (const unsigned long long*)ws.counters.p;
+    va.n_lblocks = b.n_lblocks;
+    va.azone = (const int2*)ws.azone.p;
+    va.mom = (const float4*)ws.recM.p;
+    va.gexp = (const int*)ws.gexp.p;
     const int64_t grid = (int64_t)ctx->n_cu * 16;
-    hipLaunchKernelGGL(k_vjp, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    if (b.adj) hipLaunchKernelGGL(k_vjp<true>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    else hipLaunchKernelGGL(k_vjp<false>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
     HIP_TRY(ctx, hipGetLastError());
     const int NQ = 6 + b.sp.n_species;
     hipLaunchKernelGGL(k_vjp_reduce, dim3((unsigned)(NQ * b.L), (unsigned)b.n_stars), dim3(256), 0, s,
@@ -2550,17 +2947,65 @@ int place_fit_G(korg_ctx* ctx, Workspace& ws, std::vector<StarHost>& hs, int L) 
     return KORG_OK;
 }
 
-int prepare_fit(korg_ctx* ctx, Workspace& ws, const Batch& b) {
+int prepare_fit(korg_ctx* ctx, Workspace& ws, Batch& b) {
+    b.adj = b.f32;
     TRY(prepare_vjp(ctx, ws, b));
     TRY(grow(ctx, ws.rt_tab, sizeof(RtTab) * (size_t)b.n_stars * b.L));
     TRY(grow(ctx, ws.chi2_part, sizeof(double) * (size_t)(b.nwl > 0 ? b.nwl : 1)));
+    TRY(grow(ctx, ws.gexp, sizeof(int) * (size_t)b.n_stars * b.L));
+    if (b.f32) {
+        if ((b.N + ADJ_GROUP - 1) / ADJ_GROUP * b.n_lblocks > INT_MAX)
+            return set_err(ctx, KORG_ERR_UNSUPPORTED, "too many lines x layer blocks for the adjoint");
+        TRY(grow(ctx, ws.adj_blk, sizeof(int) * (size_t)(b.nblocks > 0 ? b.nblocks : 1) * ADJ_NC));
+        TRY(grow(ctx, ws.adj_start, sizeof(int) * (ADJ_NC + 1)));
+        TRY(grow(ctx, ws.adj_line, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.azone, sizeof(int2) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.g32, sizeof(float) * WAVE * (size_t)b.n_lblocks * (size_t)(b.nwl + MOM_PAD)));
+        TRY(grow(ctx, ws.lam_s, sizeof(double) * (size_t)(b.nwl + MOM_PAD)));
+    }
+    return KORG_OK;
+}
+
+// Series form of the adjoint, first part (G of every star complete): G's scale and Float32 copy,
+// the lines with a far zone grouped by window width, and their moments into ws.recM (the forward's
+// coefficient blocks, which the step no longer needs).
+int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s) {
+    const unsigned gblocks = (unsigned)((b.nwl / STAR_ALIGN + 3) / 4);
+    hipLaunchKernelGGL(k_g_exp, dim3(gblocks), dim3(256), 0, s, b.n_stars, b.L, b.nwl,
+                       (const StarDev*)ws.stars.p, (int*)ws.gexp.p);
+    hipLaunchKernelGGL(k_g_pack, dim3(gblocks), dim3(256), 0, s, b.n_stars, b.L, b.n_lblocks, b.nwl,
+                       (const StarDev*)ws.stars.p, (const int*)ws.gexp.p, (float*)ws.g32.p,
+                       (double*)ws.lam_s.p);
+    hipLaunchKernelGGL(k_adj_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
+                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (int*)ws.adj_blk.p);
+    hipLaunchKernelGGL(k_scan<ADJ_NC>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.adj_blk.p,
+                       (int*)ws.adj_start.p);
+    hipLaunchKernelGGL(k_adj_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
+                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (const int*)ws.adj_blk.p,
+                       (const int*)ws.adj_start.p, (int*)ws.adj_line.p);
+    MomArgs ma;
+    ma.n_stars = b.n_stars;
+    ma.L = b.L;
+    ma.n_lblocks = b.n_lblocks;
+    ma.n_total = b.nwl;
+    ma.stars = (const StarDev*)ws.stars.p;
+    ma.meta = (const LineMeta*)ws.meta.p;
+    ma.azone = (const int2*)ws.azone.p;
+    ma.adj_line = (const int*)ws.adj_line.p;
+    ma.adj_start = (const int*)ws.adj_start.p;
+    ma.g32 = (const float*)ws.g32.p;
+    ma.lam_s = (const double*)ws.lam_s.p;
+    ma.mom = (float4*)ws.recM.p;
+    ma.counters = (unsigned long long*)ws.counters.p;
+    hipLaunchKernelGGL(k_vjp_moments, dim3((unsigned)(ctx->n_cu * 32)), dim3(WAVE), 0, s, ma);
+    HIP_TRY(ctx, hipGetLastError());
     return KORG_OK;
 }
 
 // Radiative transfer, chi^2 and G of every star (alpha_lines complete), on s.
 int enqueue_rt(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s) {
     hipLaunchKernelGGL(k_rt_tab, dim3(b.n_stars), dim3(64), 0, s, b.n_stars, b.L,
-                       (const StarDev*)ws.stars.p, (RtTab*)ws.rt_tab.p);
+                       (const StarDev*)ws.stars.p, (RtTab*)ws.rt_tab.p, (int*)ws.gexp.p, GEXP_NONE);
     RtArgs ra;
     ra.n_stars = b.n_stars;
     ra.L = b.L;
@@ -2599,6 +3044,7 @@ int enqueue_fit(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, boo
                            (unsigned long long*)ws.counters.p);
     }
     TRY(enqueue_rt(ctx, ws, b, s));
+    if (b.adj && b.N > 0) TRY(enqueue_moments(ctx, ws, b, s));
     TRY(enqueue_vjp_counts(ctx, ws, b, s, sizing ? INT64_MAX : ws.seg_cap));
     if (sizing) TRY(size_segments(ctx, ws, b, s, headroom));
     return enqueue_vjp(ctx, ws, b, s, 0);

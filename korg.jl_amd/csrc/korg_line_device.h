@@ -195,6 +195,84 @@ __device__ inline void harris_series64_dv(double v, double H0, double& dH0, doub
     dH2 = H0 * (-6.0 * v + 4.0 * v2 * v);
 }
 
+// The same partial derivatives evaluated in Float32, the branch and the piece of the Harris series
+// chosen on the Float64 a and v (as the Float32 forward chooses them).  Returns H.
+__device__ inline float voigt_hjerting_grad_f32(double a64, double v64, float& dHda, float& dHdv) {
+    const float isp = 0.5641895835477563f;  // 1/sqrt(pi)
+    const float a = (float)a64, v = (float)v64;
+    const float v2 = v * v;
+    if (a64 <= 0.2 && v64 >= 5.0) {
+        const float w = __builtin_amdgcn_rcpf(v2);
+        const float poly = 1.0f + 1.5f * w + 3.75f * (w * w);
+        dHda = isp * w * poly;
+        dHdv = a * isp * (1.0f + 3.0f * w + 11.25f * w * w) * (-2.0f * w * __builtin_amdgcn_rcpf(v));
+        return a * dHda;
+    }
+    if (a64 <= 0.2 || (a64 <= 1.4 && a64 + v64 < 3.2)) {
+        const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
+        const float H0 = __expf(-v2);
+        const float H1 = harris_h1_f32(v, v2, piece);
+        const float H2 = (1.0f - 2.0f * v2) * H0;
+        const float dH0 = -2.0f * v * H0;
+        float dH1;
+        if (piece == 0) {
+            dH1 = -0.15516677f + (2.0f * 3.288675912f + (3.0f * -2.34357915f + 4.0f * 0.42139162f * v) * v) * v;
+        } else if (piece == 1) {
+            dH1 = 9.39456063f + (2.0f * -6.61487486f + (3.0f * 1.98919585f - 4.0f * 0.22041650f * v) * v) * v;
+        } else {
+            const float P = 0.554153432f +
+                            (0.278711796f + (-0.1883256872f + (0.042991293f - 0.003278278f * v) * v) * v) * v;
+            const float dP = 0.278711796f +
+                             (2.0f * -0.1883256872f + (3.0f * 0.042991293f - 4.0f * 0.003278278f * v) * v) * v;
+            const float iq = __builtin_amdgcn_rcpf(v2 - 1.5f);
+            dH1 = (dP - 2.0f * v * P * iq) * iq;
+        }
+        const float dH2 = H0 * (-6.0f * v + 4.0f * v2 * v);
+        if (a64 <= 0.2) {
+            dHda = H1 + 2.0f * H2 * a;
+            dHdv = dH0 + (dH1 + dH2 * a) * a;
+            return H0 + (H1 + H2 * a) * a;
+        }
+        const float k = 2.0f * isp, k3 = (2.0f / 3.0f) * isp;
+        const float M0 = H0;
+        const float M1 = H1 + k * M0;
+        const float M2 = H2 - M0 + k * M1;
+        const float M3 = k3 * (1.0f - H2) - (2.0f / 3.0f) * v2 * M1 + k * M2;
+        const float M4 = (2.0f / 3.0f) * v2 * v2 * M0 - k3 * M1 + k * M3;
+        const float dM0 = dH0;
+        const float dM1 = dH1 + k * dM0;
+        const float dM2 = dH2 - dM0 + k * dM1;
+        const float dM3 = -k3 * dH2 - (2.0f / 3.0f) * (2.0f * v * M1 + v2 * dM1) + k * dM2;
+        const float dM4 = (2.0f / 3.0f) * (4.0f * v2 * v * M0 + v2 * v2 * dM0) - k3 * dM1 + k * dM3;
+        const float psi = 0.979895023f + (-0.962846325f + (0.532770573f - 0.122727278f * a) * a) * a;
+        const float dpsi = -0.962846325f + (2.0f * 0.532770573f - 3.0f * 0.122727278f * a) * a;
+        const float S = M0 + (M1 + (M2 + (M3 + M4 * a) * a) * a) * a;
+        const float dSda = M1 + (2.0f * M2 + (3.0f * M3 + 4.0f * M4 * a) * a) * a;
+        const float dSdv = dM0 + (dM1 + (dM2 + (dM3 + dM4 * a) * a) * a) * a;
+        dHda = dpsi * S + psi * dSda;
+        dHdv = psi * dSdv;
+        return psi * S;
+    }
+    // branch D: H = sqrt(2/pi) u P(r^2, u), u = a / (sqrt(2) (v^2 + a^2)), r^2 = v^2 / a^2
+    const float ia = __builtin_amdgcn_rcpf(a);
+    const float r2 = v2 * ia * ia;
+    const float s = v2 + a * a;
+    const float is = __builtin_amdgcn_rcpf(s);
+    const float u = 0.70710678118654752f * a * is;
+    const float u2 = u * u;
+    const float c4 = (r2 - 2.0f) * 15.0f * r2 + 2.0f;
+    const float P = 1.0f + (3.0f * r2 - 1.0f + c4 * u2) * u2;
+    const float iden = 0.70710678118654752f * is * is;  // 1 / (sqrt(2) (v^2 + a^2)^2)
+    const float du_da = (v2 - a * a) * iden, du_dv = -2.0f * a * v * iden;
+    const float dr2_da = -2.0f * r2 * ia, dr2_dv = 2.0f * v * ia * ia;
+    const float dP_dr2 = 3.0f * u2 + (30.0f * r2 - 30.0f) * u2 * u2;
+    const float dP_du = 2.0f * (3.0f * r2 - 1.0f) * u + 4.0f * c4 * u2 * u;
+    const float s2p = 0.79788456080286536f;  // sqrt(2/pi)
+    dHda = s2p * (P * du_da + u * (dP_dr2 * dr2_da + dP_du * du_da));
+    dHdv = s2p * (P * du_dv + u * (dP_dr2 * dr2_dv + dP_du * du_dv));
+    return s2p * u * P;
+}
+
 __device__ inline double voigt_hjerting64_grad(double a, double v, double& dHda, double& dHdv) {
     const double sqrt_pi = 1.7724538509055159;
     const double v2 = v * v;
