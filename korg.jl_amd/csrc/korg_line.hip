@@ -190,10 +190,10 @@ struct LayerTab {
 // [2] lines used, [3] error flags, [4] near-zone reach in points, [5] near queue,
 // [6..9] evaluations by Hunger branch, [10] near-zone evaluations, [11] far-wing MFMAs per layer
 // block, [12] tail-zone evaluations, [13] entries of the per-tile line lists, [14] segments of the
-// adjoint.
+// adjoint, [16..23] the adjoint moments' queues (one per part of the group list).
 // Error flags: 1 lines not sorted, 2 species index out of range, 4 the per-tile line lists need
 // more entries than their buffer holds.  With any flag set no kernel writes alpha.
-constexpr int N_COUNTERS = 16;
+constexpr int N_COUNTERS = 24;
 constexpr unsigned long long FLAG_UNSORTED = 1, FLAG_SPECIES = 2, FLAG_CAPACITY = 4;
 // Largest far-wing series coefficient kept in Float32 (2^120: headroom below FLT_MAX ~ 2^128).
 constexpr double F32_COEF_MAX = 1.329227995784916e36;
@@ -1987,11 +1987,13 @@ __global__ __launch_bounds__(PART_BLOCK) void k_adj_count(int64_t n, const LineM
 }
 
 // Stable scatter of the lines with a far zone by class: within a class, in line order.
+// Also marks each group's leader: adj_lead[line] = the group's position in adj_line, else -1.
 __global__ __launch_bounds__(PART_BLOCK) void k_adj_scatter(int64_t n, const LineMeta* __restrict__ meta,
                                                             const int2* __restrict__ azone,
                                                             const int* __restrict__ block_offsets,
                                                             const int* __restrict__ class_start,
-                                                            int* __restrict__ adj_line) {
+                                                            int* __restrict__ adj_line,
+                                                            int* __restrict__ adj_lead) {
     __shared__ int wave_cnt[PART_BLOCK / WAVE][ADJ_NC];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
@@ -2003,10 +2005,47 @@ __global__ __launch_bounds__(PART_BLOCK) void k_adj_scatter(int64_t n, const Lin
         if (lane == 0) wave_cnt[w][k] = __popcll(b);
     }
     __syncthreads();
+    int lead = -1;
     if (c < ADJ_NC) {
         int before = 0;
         for (int ww = 0; ww < w; ++ww) before += wave_cnt[ww][c];
-        adj_line[class_start[c] + block_offsets[(int64_t)blockIdx.x * ADJ_NC + c] + before + rank] = (int)i;
+        const int in_class = block_offsets[(int64_t)blockIdx.x * ADJ_NC + c] + before + rank;
+        adj_line[class_start[c] + in_class] = (int)i;
+        // the first line of each group of ADJ_GROUP within the class leads it
+        if (in_class % ADJ_GROUP == 0) lead = class_start[c] + in_class;
+    }
+    if (i < n) adj_lead[i] = lead;
+}
+
+// The groups in line order (star, then wavelength) -- the order the moments kernel takes them in,
+// so that the groups in flight together read nearby points of G: a stable compaction of the
+// leaders (block counts, their scan by k_scan<1>, the scatter of the leaders' group positions).
+__global__ __launch_bounds__(PART_BLOCK) void k_lead_count(int64_t n, const int* __restrict__ adj_lead,
+                                                           int* __restrict__ block_counts) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    const uint64_t b = ballot(i < n && adj_lead[i] >= 0);
+    if ((threadIdx.x % WAVE) == 0 && b) atomicAdd(&cnt, __popcll(b));
+    __syncthreads();
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(PART_BLOCK) void k_lead_scatter(int64_t n, const int* __restrict__ adj_lead,
+                                                             const int* __restrict__ block_offsets,
+                                                             int* __restrict__ groups) {
+    __shared__ int wave_cnt[PART_BLOCK / WAVE];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    const int lead = i < n ? adj_lead[i] : -1;
+    const uint64_t b = ballot(lead >= 0);
+    if (lane == 0) wave_cnt[w] = __popcll(b);
+    __syncthreads();
+    if (lead >= 0) {
+        int before = 0;
+        for (int ww = 0; ww < w; ++ww) before += wave_cnt[ww];
+        groups[block_offsets[blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1))] = lead;
     }
 }
 
@@ -2034,7 +2073,9 @@ __global__ __launch_bounds__(256) void k_g_exp(int n_stars, int L, int64_t n_tot
 }
 
 // K steps per block of the moments kernel (loads in flight together), per Float32 partial sum
-// (a multiple of the block), and the points the kernel may read past the last grid point.
+// (a multiple of the block), and the points the kernel may read past the last grid point; the
+// parts of its item list (one per XCD).
+constexpr int MOM_PARTS = 8;
 constexpr int MOM_BLOCK = 4;
 constexpr int MOM_CHUNK = 16;
 constexpr int MOM_PAD = 4 * MOM_BLOCK;
@@ -2081,6 +2122,8 @@ struct MomArgs {
     const int2* azone;         // the adjoint's near zones
     const int* adj_line;       // the lines with a far zone, by class (ascending width)
     const int* adj_start;      // [ADJ_NC + 1]; adj_start[ADJ_NC] = the number of such lines
+    const int* groups;         // the groups' positions in adj_line, in line order
+    const int* n_groups;       // their number (device)
     const float* g32;          // k_g_pack's layouts
     const double* lam_s;
     float4* mom;               // [(line * n_lblocks + b) * 64 + l % 64]: (M1, M2, M3, M4) / 2^e
@@ -2098,18 +2141,29 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
     const int lane = threadIdx.x;
     if (A.counters[3] != 0) return;  // failed setup, or a capacity shortfall
-    const int64_t n_part = A.adj_start[ADJ_NC];
-    const int64_t n_groups = (n_part + ADJ_GROUP - 1) / ADJ_GROUP;
-    const int64_t n_items = n_groups * A.n_lblocks;
+    const int64_t n_items = (int64_t)A.n_groups[0] * A.n_lblocks;
     const int col = lane & 15, kk = lane >> 4;
     const int slot = col >> 2, pw = col & 3;
+    // The item list in MOM_PARTS contiguous parts, part x first taken by the waves of
+    // workgroups x, x + MOM_PARTS, ... (workgroups go to the XCDs round robin, so the waves of
+    // one XCD share one L2 and work on nearby points); a wave whose part is used up moves on to
+    // the next parts.
+    int part = blockIdx.x % MOM_PARTS, parts_left = MOM_PARTS;
     for (;;) {
-        const int item = wave_take(&A.counters[15], lane);
-        if (item >= n_items) break;
-        const int64_t g = n_groups - 1 - item / A.n_lblocks;
-        const int lblk = item % A.n_lblocks;
-        const int64_t p0 = g * ADJ_GROUP;
-        const int nl = (int)min((int64_t)ADJ_GROUP, n_part - p0);
+        const int64_t p_lo = n_items * part / MOM_PARTS, p_hi = n_items * (part + 1) / MOM_PARTS;
+        const int64_t item = p_lo + wave_take(&A.counters[16 + part], lane);
+        if (item >= p_hi) {
+            if (--parts_left == 0) break;
+            part = (part + 1) % MOM_PARTS;
+            continue;
+        }
+        const int64_t g = item / A.n_lblocks;
+        const int lblk = (int)(item % A.n_lblocks);
+        const int64_t p0 = A.groups[g];
+        // the group's lines: up to ADJ_GROUP from p0, within the leader's class
+        const int l0 = A.adj_line[p0];
+        const int64_t cend = A.adj_start[adj_class(A.meta[l0], A.azone[l0]) + 1];
+        const int nl = (int)min((int64_t)ADJ_GROUP, cend - p0);
         int line = -1, st = -1;
         LineMeta m = {0.0, 1, 0, 1, 0, 0, ADJ_NC, 1, 0};
         int2 z = make_int2(1, 0);
@@ -2227,7 +2281,7 @@ struct Workspace {
         tc_start, tile_lines, stars;
     // adjoint and fit step (adj_*, gexp, g32: the series form's moments)
     DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part, adj_blk,
-        adj_start, adj_line, gexp, g32, lam_s, azone;
+        adj_start, adj_line, gexp, g32, lam_s, azone, adj_lead, adj_groups, adj_gstart;
     int64_t tile_cap = 0;             // entries tile_lines holds
     int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
@@ -2238,7 +2292,7 @@ struct Workspace {
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &vjp_cnt,
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
-                          &lam_s,     &azone};
+                          &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -2960,6 +3014,9 @@ int prepare_fit(korg_ctx* ctx, Workspace& ws, Batch& b) {
         TRY(grow(ctx, ws.adj_start, sizeof(int) * (ADJ_NC + 1)));
         TRY(grow(ctx, ws.adj_line, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
         TRY(grow(ctx, ws.azone, sizeof(int2) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.adj_lead, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.adj_groups, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.adj_gstart, sizeof(int) * 2));
         TRY(grow(ctx, ws.g32, sizeof(float) * WAVE * (size_t)b.n_lblocks * (size_t)(b.nwl + MOM_PAD)));
         TRY(grow(ctx, ws.lam_s, sizeof(double) * (size_t)(b.nwl + MOM_PAD)));
     }
@@ -2982,7 +3039,13 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
                        (int*)ws.adj_start.p);
     hipLaunchKernelGGL(k_adj_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
                        (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (const int*)ws.adj_blk.p,
-                       (const int*)ws.adj_start.p, (int*)ws.adj_line.p);
+                       (const int*)ws.adj_start.p, (int*)ws.adj_line.p, (int*)ws.adj_lead.p);
+    hipLaunchKernelGGL(k_lead_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
+                       (const int*)ws.adj_lead.p, (int*)ws.adj_blk.p);
+    hipLaunchKernelGGL(k_scan<1>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.adj_blk.p,
+                       (int*)ws.adj_gstart.p);
+    hipLaunchKernelGGL(k_lead_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
+                       (const int*)ws.adj_lead.p, (const int*)ws.adj_blk.p, (int*)ws.adj_groups.p);
     MomArgs ma;
     ma.n_stars = b.n_stars;
     ma.L = b.L;
@@ -2993,6 +3056,8 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     ma.azone = (const int2*)ws.azone.p;
     ma.adj_line = (const int*)ws.adj_line.p;
     ma.adj_start = (const int*)ws.adj_start.p;
+    ma.groups = (const int*)ws.adj_groups.p;
+    ma.n_groups = (const int*)ws.adj_gstart.p + 1;
     ma.g32 = (const float*)ws.g32.p;
     ma.lam_s = (const double*)ws.lam_s.p;
     ma.mom = (float4*)ws.recM.p;
