@@ -819,11 +819,12 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
     if (!tc_start && lane == 0) tc_count[slot] = (int)cnt;
 }
 
-// One block: exclusive scan of n counts into start[0..n] (start[n] = total).  With counters
-// given, the total is recorded in counters[slot] and FLAG_CAPACITY is raised when it exceeds cap.
-__global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __restrict__ count,
-                                                    int64_t* __restrict__ start, int64_t cap,
-                                                    unsigned long long* __restrict__ counters,
+// One block: exclusive scan of n counts into start[0..n] (start[n] = total; start may be count
+// itself).  With counters given, the total is recorded in counters[slot] and FLAG_CAPACITY is
+// raised when it exceeds cap.
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const T* count, int64_t* start,
+                                                    int64_t cap, unsigned long long* __restrict__ counters,
                                                     int slot) {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x;
@@ -841,8 +842,9 @@ __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __rest
     }
     int64_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
     for (int64_t i = a; i < b; ++i) {
+        const int64_t c = count[i];
         start[i] = run;
-        run += count[i];
+        run += c;
     }
     if (t == 1023) {
         start[n] = part[1023];
@@ -851,6 +853,63 @@ __global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const int* __rest
             if (part[1023] > cap) atomicOr(&counters[3], FLAG_CAPACITY);
         }
     }
+}
+
+// The same scan over many blocks for long inputs: block totals of SCAN_TILE counts
+// (k_scan_part), their scan in place by one block (k_scan_flat<int64_t>, which also raises the
+// capacity flag), and each block's scan from its offset (k_scan_fill).  Reads are coalesced.
+constexpr int SCAN_TILE = 8192;  // counts per block: 1024 threads x 8
+
+__global__ __launch_bounds__(1024) void k_scan_part(int64_t n, const int* __restrict__ count,
+                                                    int64_t* __restrict__ part) {
+    __shared__ int64_t red[1024];
+    const int t = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_TILE / 1024; ++k) {
+        const int64_t i = base + k * 1024 + t;
+        if (i < n) sum += count[i];
+    }
+    red[t] = sum;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_fill(int64_t n, const int* __restrict__ count,
+                                                    const int64_t* __restrict__ part,
+                                                    int64_t* __restrict__ start) {
+    __shared__ int64_t sh[1024];
+    const int t = threadIdx.x;
+    constexpr int PER = SCAN_TILE / 1024;
+    const int64_t i0 = (int64_t)blockIdx.x * SCAN_TILE + PER * t;  // this thread's counts
+    int v[PER];
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = i0 + k < n ? count[i0 + k] : 0;
+        sum += v[k];
+    }
+    sh[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const int64_t x = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    int64_t run = part[blockIdx.x] + sh[t] - sum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (i0 + k < n) {
+            start[i0 + k] = run;
+            run += v[k];
+        }
+    if (blockIdx.x == 0 && t == 0) start[n] = part[gridDim.x];
 }
 
 // --------------------------------------------------------------------------------------------
@@ -2281,7 +2340,7 @@ struct Workspace {
         tc_start, tile_lines, stars;
     // adjoint and fit step (adj_*, gexp, g32: the series form's moments)
     DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part, adj_blk,
-        adj_start, adj_line, gexp, g32, lam_s, azone, adj_lead, adj_groups, adj_gstart;
+        adj_start, adj_line, gexp, g32, lam_s, azone, adj_lead, adj_groups, adj_gstart, scan_part;
     int64_t tile_cap = 0;             // entries tile_lines holds
     int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
@@ -2292,7 +2351,8 @@ struct Workspace {
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &vjp_cnt,
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
-                          &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart};
+                          &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
+                          &scan_part};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -2531,6 +2591,25 @@ int check_tuning(korg_ctx* ctx, const korg_tuning* t) {
     return KORG_OK;
 }
 
+// Exclusive scan of n counts into start[0..n] on s (see k_scan_flat); part: scratch of
+// scan_blocks(n) + 1 entries.
+int64_t scan_blocks(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void enqueue_scan(hipStream_t s, int64_t n, const int* count, int64_t* start, int64_t cap,
+                  unsigned long long* counters, int slot, int64_t* part) {
+    if (n <= SCAN_TILE) {
+        hipLaunchKernelGGL(k_scan_flat<int>, dim3(1), dim3(1024), 0, s, n, count, start, cap,
+                           counters, slot);
+        return;
+    }
+    const int64_t nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_part, dim3((unsigned)nb), dim3(1024), 0, s, n, count, part);
+    hipLaunchKernelGGL(k_scan_flat<int64_t>, dim3(1), dim3(1024), 0, s, nb, (const int64_t*)part,
+                       part, cap, counters, slot);
+    hipLaunchKernelGGL(k_scan_fill, dim3((unsigned)nb), dim3(1024), 0, s, n, count,
+                       (const int64_t*)part, start);
+}
+
 // Computes the batch's shapes from the tuning, uploads the star table and grows every buffer of
 // ws except the per-tile line lists (sized from the device count, by the caller).
 int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost* stars,
@@ -2549,10 +2628,14 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.near_wpc = t.near_waves_per_cu > 0 ? t.near_waves_per_cu : 16;
     b.count_branches = t.count_branches ? 1 : 0;
     TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
-    if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
-    const int64_t N = b.N;
     b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
     b.ntiles = (int)((b.nwl + b.tile_w - 1) / b.tile_w);
+    // scratch of the multi-block scans (tile lists, adjoint items and segments)
+    TRY(grow(ctx, ws.scan_part, sizeof(int64_t) *
+                                    (size_t)(scan_blocks(std::max<int64_t>({(int64_t)b.ntiles * N_CLASSES, b.N,
+                                                                             (int64_t)n_stars})) + 1)));
+    if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
+    const int64_t N = b.N;
     b.nt_near = (int)((b.nwl + CT - 1) / CT);
     b.n_groups = (L + F64_GROUP - 1) / F64_GROUP;
     b.Ls = b.n_groups * F64_GROUP;        // rec64 stride: whole Float64 groups
@@ -2659,8 +2742,8 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
                            b.nwl, (const int2*)ws.ranges.p, (const int2*)ws.cls_rng.p,
                            (const int*)ws.cls_line.p, (const int64_t*)nullptr, (int*)ws.tc_count.p,
                            (int*)nullptr, (const unsigned long long*)counters);
-        hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, ntc, (const int*)ws.tc_count.p,
-                           (int64_t*)ws.tc_start.p, tile_cap, counters, 13);
+        enqueue_scan(s, ntc, (const int*)ws.tc_count.p, (int64_t*)ws.tc_start.p, tile_cap, counters,
+                     13, (int64_t*)ws.scan_part.p);
     }
     HIP_TRY(ctx, hipGetLastError());
     if (timed && b.f32) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
@@ -2873,13 +2956,13 @@ int enqueue_vjp_counts(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t
         hipLaunchKernelGGL(k_vjp_count, dim3((unsigned)((b.N + 255) / 256)), dim3(256), 0, s, b.N,
                            (const LineMeta*)ws.meta.p, b.adj ? (const int2*)ws.azone.p : nullptr,
                            (int*)ws.vjp_cnt.p);
-    hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, b.N, (const int*)ws.vjp_cnt.p,
-                       (int64_t*)ws.vjp_start.p, (int64_t)INT64_MAX, (unsigned long long*)nullptr, 0);
+    enqueue_scan(s, b.N, (const int*)ws.vjp_cnt.p, (int64_t*)ws.vjp_start.p, INT64_MAX, nullptr, 0,
+                 (int64_t*)ws.scan_part.p);
     hipLaunchKernelGGL(k_vjp_segs, dim3((unsigned)((b.n_stars + 255) / 256)), dim3(256), 0, s,
                        b.n_stars, (const StarDev*)ws.stars.p, (const int64_t*)ws.vjp_start.p,
                        (int*)ws.seg_cnt.p);
-    hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, s, (int64_t)b.n_stars,
-                       (const int*)ws.seg_cnt.p, (int64_t*)ws.seg_start.p, seg_cap, counters, 14);
+    enqueue_scan(s, b.n_stars, (const int*)ws.seg_cnt.p, (int64_t*)ws.seg_start.p, seg_cap,
+                 counters, 14, (int64_t*)ws.scan_part.p);
     HIP_TRY(ctx, hipGetLastError());
     return KORG_OK;
 }
