@@ -316,8 +316,11 @@ int korg_autotune_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars
  * solution in exponential integrals (:379-400) with Korg's approximate E2 (:408-480), in Float64.
  * The normalized flux is F(alpha_cntm + alpha_lines) / F(alpha_cntm); alpha_cntm is each star's
  * continuum on knots (KORG_CNTM_GRID required).  chi^2 = sum_j ((F_norm_j - data_j) / sigma_j)^2.
- * The gradient through alpha_lines is the adjoint of korg_line_absorption_vjp (Float64, windows
- * held fixed) of G = dchi^2/dalpha_lines from the transfer's reverse mode.  Device pointers.
+ * The gradient through alpha_lines is the line-opacity adjoint of G = dchi^2/dalpha_lines from the
+ * transfer's reverse mode, windows held fixed: at KORG_PRECISION_FP64 that of
+ * korg_line_absorption_vjp (Float64 throughout); at KORG_PRECISION_FP32_FIXUP the far zone of each
+ * line through Float32 moments of the far-wing series on the matrix cores, its near zone by exact
+ * Float32 evaluation (gradient within a few 1e-5 of the Float64 step).  Device pointers.
  */
 typedef struct {
     const double* tau_ref;    /* [n_layers] reference optical depth of the anchored scheme */
