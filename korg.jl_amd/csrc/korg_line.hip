@@ -311,7 +311,7 @@ struct SetupArgs {
 
 constexpr double INV_SQRT_PI = 0.5641895835477563;
 // (3.75 / SERIES_TAIL_EPS)^(1/4): the tail radius of branch A (a <= 0.2), as v
-constexpr double VT_SMALL_A = 24.746160075680034;
+constexpr double VT_SMALL_A = 24.746160019198825;
 
 // x^p for x >= 0 to about 1e-6 relative (Float32 log2/exp2 of the mantissa-exponent split, so any
 // double exponent is fine), enlarged by 1e-5: an upper bound for a zone radius.
