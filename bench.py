@@ -249,7 +249,7 @@ TUNING_CACHE = os.path.join(ROOT, "korg.jl_amd", "korg_line", "tuning_mi355x.txt
 def bench_batched(args):
     """The batched forward over many stars as one graph replay per step."""
     import numpy as np
-    from korg_line import Context, DeviceProblem, rt, workloads
+    from korg_line import Context, DeviceProblem, FluxTarget, rt, workloads
     from korg_line.hipmem import synchronize as device_sync
 
     t = time.time()
@@ -319,6 +319,26 @@ def bench_batched(args):
         flux_err = max(flux_err, float(np.max(np.abs(f32 - f64))))
     log(f"  sample of {len(sample)} stars: alpha max rel err {alpha_err:.3e}, flux {flux_err:.3e}")
 
+    # the rows on either side of the line opacity, over the whole batch: the continuum on the grid
+    # (an HBM write of every alpha, korg_continuum_batched) and the emergent flux of continuum +
+    # lines (korg_flux_batched); host-timed over 3 synchronous calls each
+    ctx.continuum_batched(a32, dprobs)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.continuum_batched(a32, dprobs)
+    tc = (time.perf_counter() - t0) / 3
+    cbytes = 4.0 * L * n_pts
+    fts = [FluxTarget(P, 10.0 ** atm.lgTau5, atm.continuum(np.array([5000e-8]))[:, 0])
+           for P, atm, _ in stars]
+    ctx.line_absorption_batched(a32, dprobs, "fp32")
+    ctx.flux_batched(a32, dprobs, fts)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.flux_batched(a32, dprobs, fts)
+    tf = (time.perf_counter() - t0) / 3
+    log(f"  continuum on the grid: {tc * 1e3:.2f} ms ({cbytes / tc / 1e9:.0f} GB/s); flux of "
+        f"{len(probs)} spectra: {tf * 1e3:.2f} ms")
+
     cpu = None if args.no_cpu else cpu_baseline_batched(stars, E / len(probs))
     out = {
         "metric": METRIC,
@@ -351,6 +371,10 @@ def bench_batched(args):
         "alpha_max_rel_err_fp32_vs_fp64": alpha_err,
         "flux_max_abs_err_fp32_vs_fp64": flux_err,
         "accuracy_sample": f"every 16th star ({len(sample)} stars)",
+        "continuum_on_grid": {"ms": tc * 1e3, "bytes_written": cbytes, "GB_per_s": cbytes / tc / 1e9,
+                              "frac_of_hbm_peak": cbytes / tc / 8e12},
+        "flux": {"ms": tf * 1e3, "spectra_per_s": len(probs) / tf,
+                 "scheme": "Korg's planar default (anchored tau, linear source, E2), Float64"},
         "plan": plan.info(),
         "cpu_baseline": cpu,
     }

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 7
+#define KORG_LINE_ABI_VERSION 8
 
 enum korg_status {
     KORG_OK = 0,
@@ -339,6 +339,33 @@ int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* s
 int korg_plan_fit(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                   const korg_fit_target* targets, const korg_species* species, int alpha_dtype,
                   int precision, double headroom, korg_plan** out);
+
+/*
+ * The continuum of every star on its grid: alpha[l, j] = the linear interpolation of layer l's
+ * continuum on its knots (KORG_CNTM_GRID required) at lambda_j, overwriting alpha -- the alpha
+ * that synthesize fills before the lines are added (src/synthesize.jl:225-226, 248-250; outside
+ * the knots the end intervals are extended).  The line opacity is then added by
+ * korg_line_absorption_batched or _device, which accumulate.  Device pointers; the stars' line
+ * lists are not read.  No counterpart as a separate call in the reference.
+ */
+int korg_continuum_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars, int alpha_dtype,
+                           int synchronize);
+
+/*
+ * The emergent flux of every star by Korg's plane-parallel default transfer (the scheme of
+ * korg_fit_gradient_batched, in Float64): flux from each star's alpha taken as the total
+ * absorption (the continuum included, as korg_continuum_batched and the lines leave it), cntm
+ * from the continuum alone -- synthesize's flux and cntm in erg/s/cm^4/A
+ * (src/synthesize.jl:286-290, 310-318).  Device pointers.
+ */
+typedef struct {
+    const double* tau_ref;    /* [n_layers] reference optical depth of the anchored scheme */
+    const double* alpha_ref;  /* [n_layers] absorption at the reference wavelength */
+    double* flux;             /* out [n_wl] (may be NULL) */
+    double* cntm;             /* out [n_wl] (may be NULL) */
+} korg_flux_target;
+int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                      const korg_flux_target* targets, int alpha_dtype, int synchronize);
 
 #ifdef __cplusplus
 }

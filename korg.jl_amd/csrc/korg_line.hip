@@ -232,6 +232,7 @@ struct StarDev {
     // its uncertainty [n_wl]; outputs: the normalized flux [n_wl] (may be null) and chi^2 [1]
     const double *tau_ref, *alpha_ref, *data, *sigma;
     double *flux, *chi2;
+    double* flux_c;  // flux of the continuum alone (flux-only transfer)
 };
 
 // The star holding global grid point j (the last star with grid_off <= j).
@@ -1860,6 +1861,44 @@ __global__ void k_zero_alpha(int n_stars, int L, int64_t n_total, const StarDev*
     }
 }
 
+// Each star's alpha set to its continuum on the grid (KORG_CNTM_GRID): per layer, the linear
+// interpolation of the knots' values at every grid point -- the alpha that synthesize starts the
+// lines from (src/synthesize.jl:225-226, 248-250), with the interpolation the window search uses
+// (cntm_lerp; outside the knots, the end intervals extended).  A wavefront per 64 points (no chunk
+// straddles two stars: the grids are padded to 128 points); lanes = layers, so every store covers
+// a point's layers contiguously.
+__global__ __launch_bounds__(256) void k_cntm_fill(int n_stars, int L, int64_t n_total,
+                                                   const StarDev* __restrict__ stars, int alpha_f32) {
+    const int lane = threadIdx.x % WAVE;
+    const int64_t j0 = ((int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
+    if (j0 >= n_total) return;
+    const StarDev& S = stars[star_of_point(stars, n_stars, j0)];
+    const int64_t jl0 = j0 - S.grid_off, jl1 = min(jl0 + WAVE, S.n_wl);
+    if (jl0 >= jl1) return;  // padding between two stars' grids
+    // lane p: the knot interval of point jl0 + p (last knot <= lambda, within [0, n_knots - 2])
+    int kidx;
+    {
+        const double x = S.wls[min(jl0 + lane, jl1 - 1)];
+        int64_t lo = 0, hi = S.n_knots;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (S.knots[mid] <= x) lo = mid; else hi = mid;
+        }
+        kidx = (int)(lo > S.n_knots - 2 ? S.n_knots - 2 : lo);
+    }
+    for (int64_t p = 0; p < jl1 - jl0; ++p) {
+        const int64_t jl = jl0 + p;
+        const int ki = __shfl(kidx, (int)p);
+        const double x = S.wls[jl];
+        for (int l = lane; l < L; l += WAVE) {
+            const double v = cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, ki, x);
+            const int64_t k = (int64_t)l + (int64_t)L * jl;
+            if (alpha_f32) ((float*)S.alpha)[k] = (float)v;
+            else ((double*)S.alpha)[k] = v;
+        }
+    }
+}
+
 struct RtArgs {
     int n_stars, L;
     int64_t n_total;  // global grid points
@@ -1867,6 +1906,9 @@ struct RtArgs {
     const RtTab* tab;  // [star * L + l]
     int alpha_f32;
     double* chi2_part;  // [global point]; G (Float64) goes to each star's StarDev::G
+    // flux only: alpha holds the total absorption (continuum included); the flux of alpha and of
+    // the continuum alone go to StarDev::flux and ::flux_c (either may be null); no chi^2, no G
+    int flux_only;
 };
 
 // alpha_cntm of layer l at the point (continuum interpolated on the star's knots)
@@ -1884,7 +1926,7 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     const KCONST StarDev& S = stars[st];
     const int64_t jl = j - S.grid_off;
     if (jl >= S.n_wl) {
-        A.chi2_part[j] = 0.0;
+        if (!A.flux_only) A.chi2_part[j] = 0.0;
         return;
     }
     const RtTab* tb = A.tab + (int64_t)st * L;
@@ -1903,16 +1945,18 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
         const int64_t k = (int64_t)l + (int64_t)L * jl;
         return A.alpha_f32 ? (double)((const float*)S.alpha)[k] : ((const double*)S.alpha)[k];
     };
+    // the total absorption of layer l, given its continuum ac
+    auto alpha_total = [&](int l, double ac) { return A.flux_only ? alpha_lines(l) : ac + alpha_lines(l); };
     // forward: flux of the total and of the continuum alone
     double F = 0.0, Fc = 0.0;
     double tau0 = 0.0, tc0 = 0.0;
     double ac0 = cntm_at(S, 0, kidx, lam);
-    double u0 = (ac0 + alpha_lines(0)) * tb[0].f, uc0 = ac0 * tb[0].f;
+    double u0 = alpha_total(0, ac0) * tb[0].f, uc0 = ac0 * tb[0].f;
     double S0 = blackbody64(S.temps[0], lam);
     double E0 = 1.0, x0 = 1.0, Ec0 = 1.0, xc0 = 1.0;  // E2 and e^{-tau} at tau = 0
     for (int i = 1; i < L; ++i) {
         const double ac = cntm_at(S, i, kidx, lam);
-        const double u1 = (ac + alpha_lines(i)) * tb[i].f, uc1 = ac * tb[i].f;
+        const double u1 = alpha_total(i, ac) * tb[i].f, uc1 = ac * tb[i].f;
         const double dl = tb[i].ltau - tb[i - 1].ltau;
         const double tau1 = tau0 + 0.5 * (u1 + u0) * dl, tc1 = tc0 + 0.5 * (uc1 + uc0) * dl;
         const double S1 = blackbody64(S.temps[i], lam);
@@ -1933,6 +1977,12 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     }
     F *= 2 * PI;
     Fc *= 2 * PI;
+    if (A.flux_only) {
+        // in erg/s/cm^4/A, as synthesize returns them (src/synthesize.jl:314-318)
+        if (S.flux) S.flux[jl] = F * 1e-8;
+        if (S.flux_c) S.flux_c[jl] = Fc * 1e-8;
+        return;
+    }
     const double Fn = F / Fc;
     const double sg = S.sigma[jl];
     const double r = (Fn - S.data[jl]) / sg;
@@ -2473,6 +2523,7 @@ struct StarHost {
     double* gq[7] = {};
     const double *tau_ref = nullptr, *alpha_ref = nullptr, *data = nullptr, *sigma = nullptr;
     double *flux = nullptr, *chi2 = nullptr;
+    double* flux_c = nullptr;  // flux-only transfer: the continuum's flux
 };
 
 // Validates the stars of a batched call (one layer count, one species table for all).
@@ -2549,6 +2600,7 @@ int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const
         d.sigma = x.sigma;
         d.flux = x.flux;
         d.chi2 = x.chi2;
+        d.flux_c = x.flux_c;
         ctx->last_grid_off[k] = go;
         ctx->last_line_off[k] = lo;
         go += (x.grid.n_wl + STAR_ALIGN - 1) / STAR_ALIGN * STAR_ALIGN;
@@ -3165,6 +3217,7 @@ int enqueue_rt(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s) {
     ra.tab = (const RtTab*)ws.rt_tab.p;
     ra.alpha_f32 = b.alpha_f32;
     ra.chi2_part = (double*)ws.chi2_part.p;
+    ra.flux_only = 0;
     hipLaunchKernelGGL(k_rt_fit, dim3((unsigned)((b.nwl + 255) / 256)), dim3(256), 0, s, ra);
     hipLaunchKernelGGL(k_star_chi2, dim3(b.n_stars), dim3(256), 0, s, (const StarDev*)ws.stars.p,
                        (const double*)ws.chi2_part.p);
@@ -3199,6 +3252,71 @@ int enqueue_fit(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, boo
     TRY(enqueue_vjp_counts(ctx, ws, b, s, sizing ? INT64_MAX : ws.seg_cap));
     if (sizing) TRY(size_segments(ctx, ws, b, s, headroom));
     return enqueue_vjp(ctx, ws, b, s, 0);
+}
+
+// The stars of korg_continuum_batched / korg_flux_batched: what those read (the grid, the
+// continuum on knots, alpha; for the transfer the temperatures and at least 2 layers).
+int collect_rt_stars(korg_ctx* ctx, int64_t n_stars, const korg_star* stars, int alpha_dtype,
+                     bool transfer, std::vector<StarHost>& hs, int* L_out) {
+    if (n_stars < 1 || !stars) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no stars");
+    if (n_stars > 65535) return set_err(ctx, KORG_ERR_UNSUPPORTED, "at most 65535 stars per call");
+    if (alpha_dtype != KORG_DTYPE_F64 && alpha_dtype != KORG_DTYPE_F32)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "unknown alpha dtype");
+    const int L = stars[0].layers.n_layers;
+    hs.resize((size_t)n_stars);
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_star& x = stars[k];
+        if (x.layers.n_layers != L || L < (transfer ? 2 : 1))
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld: %d layers (one count per batch, at least %d)", (long long)k,
+                           x.layers.n_layers, transfer ? 2 : 1);
+        if (x.grid.n_wl < 1 || !x.grid.all_wls || !x.alpha)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: empty grid or null alpha",
+                           (long long)k);
+        if (x.cntm.mode != KORG_CNTM_GRID || x.cntm.n_knots < 2 || !x.cntm.knots || !x.cntm.values)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "star %lld: the continuum must be given on >= 2 knots (KORG_CNTM_GRID)",
+                           (long long)k);
+        if (transfer && !x.layers.temps)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: no temperatures", (long long)k);
+        hs[k] = StarHost{x.alpha, x.grid, x.lines, x.layers, x.cntm, x.cutoff_threshold};
+    }
+    *L_out = L;
+    return KORG_OK;
+}
+
+int run_continuum(korg_ctx* ctx, std::vector<StarHost>& hs, int L, int alpha_dtype) {
+    hipStream_t s = ctx->stream;
+    Workspace& ws = ctx->ws;
+    int64_t N = 0, nwl = 0;
+    TRY(upload_stars(ctx, ws, s, (int)hs.size(), hs.data(), &N, &nwl));
+    hipLaunchKernelGGL(k_cntm_fill, dim3((unsigned)((nwl / WAVE + 3) / 4)), dim3(256), 0, s,
+                       (int)hs.size(), L, nwl, (const StarDev*)ws.stars.p,
+                       alpha_dtype == KORG_DTYPE_F32 ? 1 : 0);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
+int run_flux(korg_ctx* ctx, std::vector<StarHost>& hs, int L, int alpha_dtype) {
+    hipStream_t s = ctx->stream;
+    Workspace& ws = ctx->ws;
+    int64_t N = 0, nwl = 0;
+    TRY(upload_stars(ctx, ws, s, (int)hs.size(), hs.data(), &N, &nwl));
+    TRY(grow(ctx, ws.rt_tab, sizeof(RtTab) * hs.size() * (size_t)L));
+    hipLaunchKernelGGL(k_rt_tab, dim3((unsigned)hs.size()), dim3(64), 0, s, (int)hs.size(), L,
+                       (const StarDev*)ws.stars.p, (RtTab*)ws.rt_tab.p, (int*)nullptr, 0);
+    RtArgs ra;
+    ra.n_stars = (int)hs.size();
+    ra.L = L;
+    ra.n_total = nwl;
+    ra.stars = (const StarDev*)ws.stars.p;
+    ra.tab = (const RtTab*)ws.rt_tab.p;
+    ra.alpha_f32 = alpha_dtype == KORG_DTYPE_F32 ? 1 : 0;
+    ra.chi2_part = nullptr;
+    ra.flux_only = 1;
+    hipLaunchKernelGGL(k_rt_fit, dim3((unsigned)((nwl + 255) / 256)), dim3(256), 0, s, ra);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
 }
 
 // Times of the last adjoint call (setup, items, reduction, total).
@@ -3954,6 +4072,43 @@ int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* s
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         TRY(finish_fit_stats(ctx));
     }
+    return KORG_OK;
+}
+
+int korg_continuum_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars, int alpha_dtype,
+                           int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    std::vector<StarHost> hs;
+    int L = 0;
+    TRY(collect_rt_stars(ctx, n_stars, stars, alpha_dtype, false, hs, &L));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_continuum(ctx, hs, L, alpha_dtype));
+    if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return KORG_OK;
+}
+
+int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
+                      const korg_flux_target* targets, int alpha_dtype, int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    if (!targets) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no flux targets");
+    std::vector<StarHost> hs;
+    int L = 0;
+    TRY(collect_rt_stars(ctx, n_stars, stars, alpha_dtype, true, hs, &L));
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_flux_target& t = targets[k];
+        if (!t.tau_ref || !t.alpha_ref)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: tau_ref and alpha_ref are required",
+                           (long long)k);
+        hs[k].tau_ref = t.tau_ref;
+        hs[k].alpha_ref = t.alpha_ref;
+        hs[k].flux = t.flux;
+        hs[k].flux_c = t.cntm;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_flux(ctx, hs, L, alpha_dtype));
+    if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;
 }
 

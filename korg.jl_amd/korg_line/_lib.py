@@ -85,6 +85,11 @@ class KorgFitTarget(ctypes.Structure):
                 ("chi2", ctypes.c_void_p), ("grads", KorgLayerGrads)]
 
 
+class KorgFluxTarget(ctypes.Structure):
+    _fields_ = [("tau_ref", ctypes.c_void_p), ("alpha_ref", ctypes.c_void_p),
+                ("flux", ctypes.c_void_p), ("cntm", ctypes.c_void_p)]
+
+
 class KorgPlanInfo(ctypes.Structure):
     _fields_ = [("n_stars", ctypes.c_int64), ("n_lines", ctypes.c_int64),
                 ("n_points", ctypes.c_int64), ("tile_capacity", ctypes.c_int64),
@@ -92,7 +97,7 @@ class KorgPlanInfo(ctypes.Structure):
                 ("pad", ctypes.c_int32), ("tuning", KorgTuning)]
 
 
-ABI_VERSION = 7  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 8  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -100,7 +105,8 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_line_absorption_vjp", "korg_line_absorption_vjp_device",
             "korg_line_absorption_batched", "korg_get_tuning", "korg_plan_batched",
             "korg_plan_launch", "korg_plan_wait", "korg_plan_get_info", "korg_plan_destroy",
-            "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit"]
+            "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit",
+            "korg_continuum_batched", "korg_flux_batched"]
 
 _lib = None
 
@@ -180,6 +186,11 @@ def load():
     L.korg_plan_fit.argtypes = [vp, i64, ctypes.POINTER(KorgStar), ctypes.POINTER(KorgFitTarget),
                                 ctypes.POINTER(KorgSpecies), i32, i32, d, ctypes.POINTER(vp)]
     L.korg_plan_fit.restype = i32
+    L.korg_continuum_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar), i32, i32]
+    L.korg_continuum_batched.restype = i32
+    L.korg_flux_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar), ctypes.POINTER(KorgFluxTarget),
+                                    i32, i32]
+    L.korg_flux_batched.restype = i32
     if L.korg_abi_version() != ABI_VERSION:
         raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
                           f"expect {ABI_VERSION}; rebuild it")

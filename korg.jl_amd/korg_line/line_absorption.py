@@ -144,6 +144,30 @@ class FitTarget:
         self.t["data"].copy_from(np.asarray(data, np.float64))
 
 
+class FluxTarget:
+    """korg_flux_batched's target for one star, in device memory: the anchored scheme's reference
+    optical depth and absorption (per layer), and the outputs -- the flux of the total absorption
+    and of the continuum alone (per grid point, erg/s/cm^4/A)."""
+
+    def __init__(self, problem, tau_ref, alpha_ref):
+        from .hipmem import DeviceArray
+        P = problem
+        self.t = {"tau_ref": DeviceArray.from_numpy(np.asarray(tau_ref, np.float64)),
+                  "alpha_ref": DeviceArray.from_numpy(np.asarray(alpha_ref, np.float64)),
+                  "flux": DeviceArray.zeros((P.n_wl,), np.float64),
+                  "cntm": DeviceArray.zeros((P.n_wl,), np.float64)}
+        for k in ("tau_ref", "alpha_ref"):
+            if self.t[k].shape != (P.n_layers,):
+                raise ValueError(f"{k} has one value per layer")
+        self.struct = _lib.KorgFluxTarget(*[self.t[k].ptr for k in ("tau_ref", "alpha_ref", "flux", "cntm")])
+
+    def flux(self):
+        return self.t["flux"].numpy()
+
+    def cntm(self):
+        return self.t["cntm"].numpy()
+
+
 class Context:
     """One library context (HIP stream + device scratch) on one device."""
 
@@ -294,6 +318,26 @@ class Context:
         for k, t in enumerate(targets):
             tg[k] = t.struct
         return tg
+
+    def continuum_batched(self, alphas, dprobs, synchronize=True):
+        """korg_continuum_batched: alphas[k] set to star k's continuum on its grid (the problems'
+        continuum on knots), the alpha that synthesize starts the lines from."""
+        stars, dtc = self._stars(alphas, dprobs)
+        rc = self._L.korg_continuum_batched(self._h, len(dprobs), stars, dtc, 1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
+    def flux_batched(self, alphas, dprobs, targets, synchronize=True):
+        """korg_flux_batched: the emergent flux of alphas[k] taken as star k's total absorption,
+        and of its continuum alone (erg/s/cm^4/A, as synthesize), into targets[k] (a
+        ``FluxTarget``)."""
+        stars, dtc = self._stars(alphas, dprobs)
+        if len(targets) != len(dprobs):
+            raise ValueError("one FluxTarget per star")
+        tg = (_lib.KorgFluxTarget * len(targets))()
+        for k, t in enumerate(targets):
+            tg[k] = t.struct
+        rc = self._L.korg_flux_batched(self._h, len(dprobs), stars, tg, dtc, 1 if synchronize else 0)
+        _lib.check(self._h, rc)
 
     def autotune_batched(self, alphas, dprobs, precision="fp32", cache_path=None):
         """Times the tuning candidates on these stars (into scratch alphas), makes the fastest
