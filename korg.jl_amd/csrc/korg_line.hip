@@ -1865,36 +1865,53 @@ __global__ void k_zero_alpha(int n_stars, int L, int64_t n_total, const StarDev*
 // interpolation of the knots' values at every grid point -- the alpha that synthesize starts the
 // lines from (src/synthesize.jl:225-226, 248-250), with the interpolation the window search uses
 // (cntm_lerp; outside the knots, the end intervals extended).  A wavefront per 64 points (no chunk
-// straddles two stars: the grids are padded to 128 points); lanes = layers, so every store covers
-// a point's layers contiguously.
+// straddles two stars: the grids are padded to 128 points): lane p finds point p's knot interval
+// and weight, then the chunk's alpha -- one contiguous run of 64 L elements -- is written with
+// consecutive lanes on consecutive elements (coalesced stores), each element taking its point's
+// interval from the lane that found it.
 __global__ __launch_bounds__(256) void k_cntm_fill(int n_stars, int L, int64_t n_total,
                                                    const StarDev* __restrict__ stars, int alpha_f32) {
     const int lane = threadIdx.x % WAVE;
     const int64_t j0 = ((int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
     if (j0 >= n_total) return;
     const StarDev& S = stars[star_of_point(stars, n_stars, j0)];
-    const int64_t jl0 = j0 - S.grid_off, jl1 = min(jl0 + WAVE, S.n_wl);
-    if (jl0 >= jl1) return;  // padding between two stars' grids
-    // lane p: the knot interval of point jl0 + p (last knot <= lambda, within [0, n_knots - 2])
+    const int64_t jl0 = j0 - S.grid_off;
+    const int np = (int)(min(jl0 + WAVE, S.n_wl) - jl0);
+    if (np <= 0) return;  // padding between two stars' grids
+    // lane p: point jl0 + p's knot interval (last knot <= lambda, within [0, n_knots - 2]) and
+    // its interpolation weight, as cntm_lerp computes it
     int kidx;
+    double tw;
     {
-        const double x = S.wls[min(jl0 + lane, jl1 - 1)];
+        const double x = S.wls[jl0 + min(lane, np - 1)];
         int64_t lo = 0, hi = S.n_knots;
         while (hi - lo > 1) {
             const int64_t mid = (lo + hi) / 2;
             if (S.knots[mid] <= x) lo = mid; else hi = mid;
         }
         kidx = (int)(lo > S.n_knots - 2 ? S.n_knots - 2 : lo);
+        tw = (x - S.knots[kidx]) / (S.knots[kidx + 1] - S.knots[kidx]);
     }
-    for (int64_t p = 0; p < jl1 - jl0; ++p) {
-        const int64_t jl = jl0 + p;
-        const int ki = __shfl(kidx, (int)p);
-        const double x = S.wls[jl];
-        for (int l = lane; l < L; l += WAVE) {
-            const double v = cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, ki, x);
-            const int64_t k = (int64_t)l + (int64_t)L * jl;
-            if (alpha_f32) ((float*)S.alpha)[k] = (float)v;
-            else ((double*)S.alpha)[k] = v;
+    const int64_t n_el = (int64_t)np * L;
+    const int64_t base = (int64_t)L * jl0;  // the chunk's first alpha element
+    int p = lane / L, l = lane % L;          // element e = p L + l of the chunk
+    // every lane runs every step (the shuffles read from all lanes); past the chunk's elements
+    // a lane stands in for its last point and stores nothing
+    for (int64_t e0 = 0; e0 < n_el; e0 += WAVE) {
+        const int64_t e = e0 + lane;
+        const int src = min(p, np - 1);
+        const int ki = __shfl(kidx, src);
+        const double t = __shfl(tw, src);
+        if (e < n_el) {
+            const double* v = S.cntm + (int64_t)l * S.n_knots + ki;
+            const double val = (1.0 - t) * v[0] + t * v[1];
+            if (alpha_f32) ((float*)S.alpha)[base + e] = (float)val;
+            else ((double*)S.alpha)[base + e] = val;
+        }
+        l += WAVE;
+        while (l >= L) {
+            l -= L;
+            ++p;
         }
     }
 }
