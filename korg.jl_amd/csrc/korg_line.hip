@@ -67,6 +67,12 @@ constexpr double SERIES_TAIL_EPS = 1e-5;
 // evaluation) extends to where the first dropped term's a- and v-derivatives, relative to the
 // leading term's, are below this.
 constexpr double SERIES_ADJ_EPS = 2e-7;
+// The far wing's matrix products in bfloat16 with hi/lo splits of both operands (three products,
+// about 16 significant bits) instead of Float32.
+#ifndef KORG_WING_BF16
+#define KORG_WING_BF16 1
+#endif
+constexpr bool WING_BF16 = KORG_WING_BF16;
 
 // --------------------------------------------------------------------------------------------
 // wave helpers
@@ -469,7 +475,23 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
                 for (int k = 0; k < 4; ++k) f32_bad |= !(fabs(c[k]) <= F32_COEF_MAX);  // or NaN
             }
         }
-        if (F32) {
+        if (F32 && WING_BF16) {
+            // bfloat16 form: per (line, layer) the four coefficients' high parts, then their low
+            // parts (c - high, rounded again): 16 B at [(line * n_lblocks + block) * 64 + l % 64]
+            uint32_t hw[2], lw[2];
+#pragma unroll
+            for (int k = 0; k < 4; k += 2) {
+                const float x0 = (float)c[k], x1 = (float)c[k + 1];
+                const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
+                const __bf16 g0 = (__bf16)(x0 - (float)h0), g1 = (__bf16)(x1 - (float)h1);
+                hw[k / 2] = (uint32_t)__builtin_bit_cast(uint16_t, h0) |
+                            ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+                lw[k / 2] = (uint32_t)__builtin_bit_cast(uint16_t, g0) |
+                            ((uint32_t)__builtin_bit_cast(uint16_t, g1) << 16);
+            }
+            ((uint4*)A.recM)[(i * A.n_lblocks + l / WAVE) * WAVE + l % WAVE] =
+                make_uint4(hw[0], hw[1], lw[0], lw[1]);
+        } else if (F32) {
             // MFMA A-operand layout: lane i of step s holds coefficient k = 2s + i/32 of layer
             // 32 mb + i%32; slot 2s + mb of float4 i.
             float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
@@ -488,12 +510,9 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     // zero and the near-zone kernel evaluates its whole window exactly.
     const bool series_off = F32 && ballot(f32_bad) != 0;
     if (series_off) {
-        for (int l = lane; l < Lm; l += WAVE) {
-            float* dst = (float*)(A.recM + (i * A.n_lblocks + l / WAVE) * WAVE);
-            const int ll = l % WAVE, mb = ll >> 5, row = ll & 31;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) dst[((k & 1) * 32 + row) * 4 + (k >> 1) * 2 + mb] = 0.0f;
-        }
+        // (either layout: every coefficient slot of the line's blocks is zeroed)
+        for (int l = lane; l < Lm; l += WAVE)
+            A.recM[(i * A.n_lblocks + l / WAVE) * WAVE + l % WAVE] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // src/line_absorption.jl:109-115: windows are maxima over layers (NaN if any is NaN)
     const bool wnan = ballot(any_nan) != 0;
@@ -1381,6 +1400,194 @@ void k_wing_mfma(ProfileArgs A) {
                         for (int e = 0; e < 4; ++e) {
                             const int r = 4 * q + e;
                             const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * (int)hi + (r & 3);
+                            if (layer < L) alpha[(int64_t)layer + (int64_t)L * (jj[b] - go)] += (AlphaT)vr[e];
+                        }
+                    }
+            }
+        }
+        __syncthreads();  // s_acc and s_item are reused by the next item
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The far wing with bfloat16 matrix products.  The same work items and line lists as
+// k_wing_mfma; the tile's lines go four at a time through v_mfma_f32_32x32x16_bf16, K = 16 =
+// (4 lines x 4 powers): A = the 4 lines' coefficients of 32 layers (from k_setup's bfloat16
+// layout), B = R^(p+1) of the 4 lines at the block's 32 points.  Each operand is split into a
+// bfloat16 high part and the bfloat16 of the remainder; acc += A_hi B_hi + A_hi B_lo + A_lo B_hi
+// (Float32 accumulation) keeps about 16 significant bits of every term.  Lane i of the wave holds
+// K rows 8 (i / 32) .. + 7 -- line slots 2 (i / 32) and 2 (i / 32) + 1, powers 1..4 each -- for
+// column (point) i % 32; the R^3, R^4 terms are always included.
+// ---------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)x[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(x[e] - (float)h);
+    }
+}
+
+// Registers: the accumulators (32 NB) plus two lines' operands per block leave room for three
+// waves per SIMD only with 32-point tiles.
+template <int NB, int WW, class AlphaT>
+__global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(NB >= 2 ? 2 : 3)))
+void k_wing_bf16(ProfileArgs A) {
+    // partial blocks of waves 1..WW-1: [wave][accumulator][register quad][lane]
+    __shared__ float4 s_acc[WW > 1 ? WW - 1 : 1][2 * NB][4][WAVE];
+    __shared__ int s_item;
+    const int lane = threadIdx.x % WAVE;
+    const int w = threadIdx.x / WAVE;
+    const int n = lane & 31;       // column (point) within a 32-point block
+    const int hf = lane >> 5;      // K rows 8 hf .. 8 hf + 7: line slots 2 hf, 2 hf + 1
+    const bool hi_half = hf != 0;
+    const int L = A.L;
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written (the same for all waves)
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&A.counters[0], 1ull);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= A.n_items) break;  // the same for the whole workgroup
+        const int tile = item / A.n_lblocks;
+        const int lblk = item % A.n_lblocks;
+        const int64_t j0 = (int64_t)tile * A.tile_w;
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        // the star's last point at most; below j0 for padding between two stars' grids (such a
+        // tile has no lines and writes nothing)
+        const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;
+        double lam[NB];
+        int jj[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            jj[b] = (int)(j0 + 32 * b + n);
+            // in units of DELTA_UNIT (points past j1 are never used)
+            lam[b] = S.wls[min((int64_t)jj[b], j1) - go] * (1.0 / DELTA_UNIT);
+        }
+        floatx16 acc[2][NB];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[mb][b][r] = 0.0f;
+        auto pin_acc = [&] {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) asm volatile("" : "+a"(acc[0][b]), "+a"(acc[1][b]));
+        };
+        // coefficients of (line, layer 32 mb + n): 16 B, high parts then low parts
+        const uint4* __restrict__ recB = (const uint4*)A.recM + (int64_t)lblk * WAVE + n;
+        const int64_t rstride = (int64_t)A.n_lblocks * WAVE;
+        // per batch of 64 lines, each lane describes its own line (as k_wing_mfma)
+        const int j0s = (int)j0, j1s = (int)j1;
+        (void)j1s;
+        uint32_t mk[NB];
+        double lam0s = 0.0;
+        auto describe = [&](const LineMeta& m) {
+            lam0s = m.lam0 * (1.0 / DELTA_UNIT);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int base = j0s + 32 * b;
+                mk[b] = span_bits(m.lb, m.ub, base) & ~span_bits(m.clo, m.chi, base);
+            }
+        };
+        const int64_t t0 = A.tile_start[(int64_t)tile * N_CLASSES];
+        const int64_t t1 = A.tile_start[(int64_t)(tile + 1) * N_CLASSES];
+        const int64_t s0 = t0 + ((t1 - t0) * w) / WW;
+        const int64_t s1 = t0 + ((t1 - t0) * (w + 1)) / WW;
+        const int* __restrict__ tl = A.tile_lines;
+        for (int64_t kb = s0; kb < s1; kb += WAVE) {
+            const int nb = (int)min((int64_t)WAVE, s1 - kb);
+            const int idx = tl[min(kb + lane, s1 - 1)];  // clamped: a stand-in past the end
+            describe(A.meta[idx]);
+            for (int g = 0; g < nb; g += 4) {
+                // the group's batch positions g .. g + 3; past nb: stand-ins with no points
+                const int q0 = min(g, nb - 1), q1 = min(g + 1, nb - 1);
+                const int q2 = min(g + 2, nb - 1), q3 = min(g + 3, nb - 1);
+                const int la = hi_half ? readlane_i32(idx, q2) : readlane_i32(idx, q0);
+                const int lb2 = hi_half ? readlane_i32(idx, q3) : readlane_i32(idx, q1);
+                uint4 ua[2], ub[2];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    ua[mb] = recB[la * rstride + 32 * mb];
+                    ub[mb] = recB[lb2 * rstride + 32 * mb];
+                }
+                const double l0a = hi_half ? readlane_f64(lam0s, q2) : readlane_f64(lam0s, q0);
+                const double l0b = hi_half ? readlane_f64(lam0s, q3) : readlane_f64(lam0s, q1);
+                bf16x8 bh[NB], bl[NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const uint32_t m0 = (uint32_t)readlane_i32((int)mk[b], q0);
+                    const uint32_t m1 = g + 1 < nb ? (uint32_t)readlane_i32((int)mk[b], q1) : 0u;
+                    const uint32_t m2 = g + 2 < nb ? (uint32_t)readlane_i32((int)mk[b], q2) : 0u;
+                    const uint32_t m3 = g + 3 < nb ? (uint32_t)readlane_i32((int)mk[b], q3) : 0u;
+                    const bool ua_ = __builtin_amdgcn_inverse_ballot_w64(m0 | ((uint64_t)m2 << 32));
+                    const bool ub_ = __builtin_amdgcn_inverse_ballot_w64(m1 | ((uint64_t)m3 << 32));
+                    const float da = (float)(lam[b] - l0a), db = (float)(lam[b] - l0b);
+                    float Ra = __builtin_amdgcn_rcpf(da * da), Rb = __builtin_amdgcn_rcpf(db * db);
+                    Ra = ua_ ? Ra : 0.0f;
+                    Rb = ub_ ? Rb : 0.0f;
+                    const float Ra2 = Ra * Ra, Rb2 = Rb * Rb;
+                    const float x[8] = {Ra, Ra2, Ra2 * Ra, Ra2 * Ra2, Rb, Rb2, Rb2 * Rb, Rb2 * Rb2};
+                    split_bf16(x, bh[b], bl[b]);
+                }
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].x, ua[mb].y, ub[mb].x, ub[mb].y));
+                    const bf16x8 al = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].z, ua[mb].w, ub[mb].z, ub[mb].w));
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[mb][b], 0, 0, 0);
+                        acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[mb][b], 0, 0, 0);
+                        acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[mb][b], 0, 0, 0);
+                    }
+                }
+                pin_acc();
+            }
+        }
+
+        // sum the waves' blocks in wave order (a fixed order: results are deterministic)
+        if (w > 0) {
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        s_acc[w - 1][mb * NB + b][q][lane] =
+                            make_float4(acc[mb][b][4 * q], acc[mb][b][4 * q + 1],
+                                        acc[mb][b][4 * q + 2], acc[mb][b][4 * q + 3]);
+        }
+        __syncthreads();
+        if (w == 0) {
+            // accumulator layout of the 32x32 MFMAs: lane i, register r holds row
+            // 8*(r/4) + 4*(i/32) + r%4, column i%32
+            AlphaT* __restrict__ alpha = (AlphaT*)S.alpha;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (jj[b] > j1) continue;
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float4 v = make_float4(acc[mb][b][4 * q], acc[mb][b][4 * q + 1],
+                                               acc[mb][b][4 * q + 2], acc[mb][b][4 * q + 3]);
+#pragma unroll
+                        for (int ww = 0; ww < WW - 1; ++ww) {
+                            const float4 p = s_acc[ww][mb * NB + b][q][lane];
+                            v.x += p.x;
+                            v.y += p.y;
+                            v.z += p.z;
+                            v.w += p.w;
+                        }
+                        const float vr[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int r = 4 * q + e;
+                            const int layer = lblk * WAVE + 32 * mb + 8 * (r >> 2) + 4 * hf + (r & 3);
                             if (layer < L) alpha[(int64_t)layer + (int64_t)L * (jj[b] - go)] += (AlphaT)vr[e];
                         }
                     }
@@ -2824,6 +3031,11 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
 
 template <int NB, int WW>
 void launch_wing(const ProfileArgs& pa, dim3 grid, hipStream_t s) {
+    if (WING_BF16) {
+        if (pa.alpha_f32) hipLaunchKernelGGL((k_wing_bf16<NB, WW, float>), grid, dim3(WAVE * WW), 0, s, pa);
+        else hipLaunchKernelGGL((k_wing_bf16<NB, WW, double>), grid, dim3(WAVE * WW), 0, s, pa);
+        return;
+    }
     if (pa.alpha_f32) hipLaunchKernelGGL((k_wing_mfma<NB, WW, float>), grid, dim3(WAVE * WW), 0, s, pa);
     else hipLaunchKernelGGL((k_wing_mfma<NB, WW, double>), grid, dim3(WAVE * WW), 0, s, pa);
 }
