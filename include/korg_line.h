@@ -127,11 +127,13 @@ typedef struct {
     double n_evals_branch[4];
     /* Float32 form only (0 for Float64): evaluations done exactly by the near-zone kernel
      * (the rest of n_evals are far-wing series terms on the matrix cores), and the
-     * v_mfma_f32_32x32x2f32 instructions the far-wing kernel issued (4096 flops each) */
+     * v_mfma_f32_32x32x16_bf16 instructions the far-wing kernel issued (32768 flops each; three
+     * per group of four lines, 32-point block and 32-layer half, counted per line) */
     double n_evals_near;
     double n_wing_mfma;
-    /* Float32 form only: far-zone evaluations that include the R^3, R^4 series terms (beyond
-     * each line's tail radius those terms are below 1e-6 of the leading term and are skipped) */
+    /* Float32 form only: far-zone evaluations inside the lines' tail radii, where the R^3, R^4
+     * series terms exceed 1e-5 of the leading term (the bfloat16 far wing includes them
+     * everywhere) */
     double n_evals_tail;
 } korg_stats;
 

@@ -2655,6 +2655,7 @@ struct korg_ctx {
     int64_t last_n_lines = 0;
     int last_precision = KORG_PRECISION_FP64;
     int last_n_lblocks = 0;
+    int last_nb = 0;  // 32-point blocks per far-wing tile of the last call
     bool last_sized = false;  // the last forward call sized the tile lists on the host (ev[5], ev[6])
     Workspace ws;
     std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
@@ -3157,6 +3158,7 @@ int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_spec
     ctx->last_n_lines = b.N;
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
+    ctx->last_nb = b.NB;
     ctx->last_sized = b.f32;
     if (b.N == 0) return KORG_OK;
     TRY(enqueue_front(ctx, ctx->ws, b, s, INT64_MAX, true));
@@ -3176,7 +3178,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
 }
 
 // Counters of a finished forward call into ctx->stats; an error for a raised flag.
-int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks) {
+int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks, int nb) {
     if (c[3] & FLAG_SPECIES)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
                        "a line's species index is outside [0, n_species); alpha was left unchanged");
@@ -3190,7 +3192,11 @@ int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, in
     ctx->stats.n_lines_used = (int64_t)c[2];
     for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
     ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
-    ctx->stats.n_wing_mfma = f32 ? (double)c[11] * n_lblocks : 0.0;
+    // far-wing MFMAs: bfloat16, three per (group of 4 lines, 32-point block, 32-layer half),
+    // counted per line-tile entry of the lists (groups rounded up are not counted); Float32, two
+    // per 32-point block and K step
+    ctx->stats.n_wing_mfma = !f32 ? 0.0 : WING_BF16 ? (double)c[13] * n_lblocks * 1.5 * nb
+                                                     : (double)c[11] * n_lblocks;
     ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
     return KORG_OK;
 }
@@ -3210,7 +3216,7 @@ int finish_stats(korg_ctx* ctx) {
     unsigned long long c[N_COUNTERS];
     HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
     TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
-                            ctx->last_n_lblocks));
+                            ctx->last_n_lblocks, ctx->last_nb));
     // device time: the host's sizing of the tile lists (ev[5] -> ev[6]) is left out
     const float gap = ctx->last_sized ? elapsed(ctx, 5, 6) : 0.0f;
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
@@ -3604,6 +3610,7 @@ int run_fit(korg_ctx* ctx, std::vector<StarHost>& hs, const korg_species* sp, in
     ctx->last_n_lines = b.N;
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
+    ctx->last_nb = b.NB;
     ctx->last_sized = false;
     TRY(prepare_fit(ctx, ws, b));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
@@ -3618,7 +3625,7 @@ int finish_fit_stats(korg_ctx* ctx) {
     unsigned long long c[N_COUNTERS];
     HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
     TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
-                            ctx->last_n_lblocks));
+                            ctx->last_n_lblocks, ctx->last_nb));
     ctx->stats.total_ms = elapsed(ctx, 0, 3);
     return KORG_OK;
 }
@@ -3826,7 +3833,7 @@ int plan_wait(korg_plan* p) {
         HIP_TRY(ctx, hipEventSynchronize(p->ev1));
         p->pending = false;
     }
-    TRY(stats_from_counters(ctx, p->h_counters, p->b.f32, p->b.n_lblocks));
+    TRY(stats_from_counters(ctx, p->h_counters, p->b.f32, p->b.n_lblocks, p->b.NB));
     float t = 0;
     if (hipEventElapsedTime(&t, p->ev0, p->ev1) == hipSuccess) ctx->stats.total_ms = t;
     else (void)hipGetLastError();
