@@ -263,10 +263,10 @@ def test_solar_wide_full_size_properties(ctx):
 
 def test_work_counts(ctx):
     """Near-zone, tail and far-wing counts of the Float32 form: the near zone is part of the
-    windows, tail evaluations are part of the far zone, and the far-wing MFMAs cover every
-    far-zone evaluation with the (R, R^2) step (2 instructions per 32-point x 64-layer block =
-    4 flops per covered (layer, point)) and every tail evaluation with the (R^3, R^4) step.
-    The Float64 form reports none of them."""
+    windows, tail evaluations are part of the far zone, and the far-wing bfloat16 MFMAs (three
+    split products per group of 4 lines, 32-point block and 32-layer half: 1.5 NB per line-tile
+    entry and 64-layer block, for tiles of 32 NB points) cover every far-zone evaluation, each
+    MFMA triple covering 4 lines x 32 points x 32 layers.  The Float64 form reports none of them."""
     P, _, _ = workloads.solar_small()
     gpu_alpha(ctx, P, "fp32")
     s = ctx.stats()
@@ -275,13 +275,14 @@ def test_work_counts(ctx):
     assert s["n_evals_near"] % P.n_layers == 0
     assert 0 < s["n_evals_tail"] <= far and s["n_evals_tail"] % P.n_layers == 0
     n_lblocks = -(-P.n_layers // 64)
-    assert s["n_wing_mfma"] % (4 * n_lblocks) == 0
-    assert 4096 * s["n_wing_mfma"] >= 4 * far + 4 * s["n_evals_tail"]
-    # a line with a W-point window meets at most W/64 + 2 tiles of 64 points, each costing at
-    # most 8 instructions per 64-layer block
+    entries_nb = 2 * s["n_wing_mfma"] / (3 * n_lblocks)  # line-tile entries x NB
+    assert entries_nb == round(entries_nb) and entries_nb > 0
+    assert 4096 * s["n_wing_mfma"] / 3 >= far
+    # a line with a W-point window meets at most W/(32 NB) + 2 tiles, each costing 1.5 NB
+    # instructions per 64-layer block; NB <= 4
     lb, ub = ctx.windows(len(P.lines))
     w = (ub - lb + 1)[ub >= lb]
-    assert s["n_wing_mfma"] <= 8 * n_lblocks * np.sum(w // 64 + 2)
+    assert s["n_wing_mfma"] <= 1.5 * n_lblocks * np.sum(w / 32 + 8)
     gpu_alpha(ctx, P, "fp64")
     s = ctx.stats()
     assert s["n_evals_near"] == 0 and s["n_wing_mfma"] == 0 and s["n_evals_tail"] == 0
