@@ -13,7 +13,8 @@
 //   Float64 form:
 //     k_profile_f64  line_profile/voigt_hjerting per evaluation (the reference's expressions)
 //   Float32 form with Float64 fix-ups:
-//     k_wing_mfma    far wing of every line as a rank-4 update on the matrix cores (FP32 MFMA)
+//     k_wing_bf16    far wing of every line as a rank-4 update on the matrix cores (bfloat16
+//                    MFMA on split high/low operands; k_wing_mfma, FP32 MFMA, when WING_BF16 = 0)
 //     k_near         the "near zone" of each line (Voigt cores, and where the damped-wing series
 //                    is not yet converged): exact branch-by-branch evaluation
 // Each profile work item owns its block of alpha outright; lines are added in a fixed order, and
@@ -1601,7 +1602,7 @@ void k_wing_bf16(ProfileArgs A) {
 // Float32 near zone, lanes = layers.  A work item is (tile of CT points, block of 64 layers);
 // for every line whose near zone [clo, chi] meets the tile (in line order), each lane evaluates
 // its layer exactly, branch by branch, at the tile's points of the zone.  Runs after
-// k_wing_mfma and adds its own sums into alpha.
+// the far-wing kernel and adds its own sums into alpha.
 // ---------------------------------------------------------------------------------------------
 constexpr int CT = 32;
 
