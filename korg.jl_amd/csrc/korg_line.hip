@@ -1421,14 +1421,41 @@ void k_wing_mfma(ProfileArgs A) {
 // column (point) i % 32; the R^3, R^4 terms are always included.
 // ---------------------------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
-__device__ inline void split_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+// The two selectors of split_pair, bfloat16 pairs (-1, 0) and (0, -1), held in registers.  As
+// compile-time constants the compiler folds (-1, 0) into the inline constant -1.0, which the
+// hardware reads as the 32-bit pattern of Float32 -1.0, i.e. the pair (0, -1): the wrong element
+// (seen on the device with an exhaustive-magnitude check of split_pair).
+struct SplitSel {
+    uint32_t m0, m1;
+};
+__device__ inline SplitSel split_selectors() {
+    SplitSel s = {0x0000BF80u, 0xBF800000u};
+    asm volatile("" : "+v"(s.m0), "+v"(s.m1));
+    return s;
+}
+
+// High and low bfloat16 parts of a pair of floats, packed as the MFMA operands want them: one
+// v_cvt_pk_bf16_f32 (round to nearest even) gives both high parts; x - high is read back from the
+// packed pair by v_dot2_f32_bf16 (x + (-1) high + 0 other high, exact: the difference of a float
+// and its bfloat16 rounding is a float); one more conversion packs the two remainders.
+__device__ inline void split_pair(float x0, float x1, SplitSel sel, uint32_t& hi, uint32_t& lo) {
+    const bf16x2 h = __builtin_convertvector((floatx2){x0, x1}, bf16x2);
+    const float r0 = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf16x2, sel.m0), x0, false);
+    const float r1 = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf16x2, sel.m1), x1, false);
+    const bf16x2 l = __builtin_convertvector((floatx2){r0, r1}, bf16x2);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+__device__ inline void split_bf16(const float (&x)[8], SplitSel sel, bf16x8& hi, bf16x8& lo) {
+    uint32_t h[4], l[4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16)x[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(x[e] - (float)h);
-    }
+    for (int e = 0; e < 4; ++e) split_pair(x[2 * e], x[2 * e + 1], sel, h[e], l[e]);
+    hi = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+    lo = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
 // Registers: the accumulators (32 NB) plus two lines' operands per block leave room for three
@@ -1440,11 +1467,14 @@ void k_wing_bf16(ProfileArgs A) {
     __shared__ float4 s_acc[WW > 1 ? WW - 1 : 1][2 * NB][4][WAVE];
     __shared__ int s_item;
     const int lane = threadIdx.x % WAVE;
-    const int w = threadIdx.x / WAVE;
+    // wavefront-uniform (scalar registers): the wave's share of the line list, the batch and
+    // group positions and the lines' readlanes below all stay scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int n = lane & 31;       // column (point) within a 32-point block
     const int hf = lane >> 5;      // K rows 8 hf .. 8 hf + 7: line slots 2 hf, 2 hf + 1
     const bool hi_half = hf != 0;
     const int L = A.L;
+    const SplitSel sel = split_selectors();
     if (A.counters[3] != 0) return;  // failed setup: nothing is written (the same for all waves)
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(&A.counters[0], 1ull);
@@ -1508,16 +1538,21 @@ void k_wing_bf16(ProfileArgs A) {
                 // the group's batch positions g .. g + 3; past nb: stand-ins with no points
                 const int q0 = min(g, nb - 1), q1 = min(g + 1, nb - 1);
                 const int q2 = min(g + 2, nb - 1), q3 = min(g + 3, nb - 1);
-                const int la = hi_half ? readlane_i32(idx, q2) : readlane_i32(idx, q0);
-                const int lb2 = hi_half ? readlane_i32(idx, q3) : readlane_i32(idx, q1);
+                // both halves' lines are read (uniform readlanes), then each lane selects its own
+                const int i0 = readlane_i32(idx, q0), i1 = readlane_i32(idx, q1);
+                const int i2 = readlane_i32(idx, q2), i3 = readlane_i32(idx, q3);
+                const int la = hi_half ? i2 : i0;
+                const int lb2 = hi_half ? i3 : i1;
                 uint4 ua[2], ub[2];
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb) {
                     ua[mb] = recB[la * rstride + 32 * mb];
                     ub[mb] = recB[lb2 * rstride + 32 * mb];
                 }
-                const double l0a = hi_half ? readlane_f64(lam0s, q2) : readlane_f64(lam0s, q0);
-                const double l0b = hi_half ? readlane_f64(lam0s, q3) : readlane_f64(lam0s, q1);
+                const double c0 = readlane_f64(lam0s, q0), c1 = readlane_f64(lam0s, q1);
+                const double c2 = readlane_f64(lam0s, q2), c3 = readlane_f64(lam0s, q3);
+                const double l0a = hi_half ? c2 : c0;
+                const double l0b = hi_half ? c3 : c1;
                 bf16x8 bh[NB], bl[NB];
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
@@ -1533,7 +1568,7 @@ void k_wing_bf16(ProfileArgs A) {
                     Rb = ub_ ? Rb : 0.0f;
                     const float Ra2 = Ra * Ra, Rb2 = Rb * Rb;
                     const float x[8] = {Ra, Ra2, Ra2 * Ra, Ra2 * Ra2, Rb, Rb2, Rb2 * Rb, Rb2 * Rb2};
-                    split_bf16(x, bh[b], bl[b]);
+                    split_bf16(x, sel, bh[b], bl[b]);
                 }
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb) {
