@@ -263,3 +263,105 @@ def _approximate_gammas_vec(wl, species, species_index, E_lower):
             lv = 6.33 + 0.4 * _log10(drbar2) + c03 + log10_k
         logvdw[sel] = np.where(chi < Eu, 0.0, lv)
     return stark, logvdw
+
+
+# ---------------------------------------------------------------------------------------------
+# Linelist ingest: Korg's HDF5 formats and filter_linelist
+# ---------------------------------------------------------------------------------------------
+def _species_table(formula_rows, charge):
+    """Distinct (atoms, charge) pairs of per-line formula rows -> (species list, int32 index)."""
+    keys = np.concatenate([formula_rows.astype(np.int64),
+                           np.asarray(charge, dtype=np.int64)[:, None]], axis=1)
+    uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+    species = []
+    for row in uniq:
+        atoms = tuple(int(z) for z in row[:-1] if z != 0)
+        species.append(Species(atoms, charge=int(row[-1])))
+    return species, inv.reshape(-1).astype(np.int32)
+
+
+def _tentothe_or_missing(x):
+    """``tentotheOrMissing`` (``src/linelist.jl:438``): 0 means missing, else 10^x."""
+    x = np.asarray(x, dtype=np.float64)
+    out = np.full(x.shape, np.nan)
+    nz = x != 0
+    out[nz] = _pow10(x[nz])
+    return out
+
+
+def _id_or_missing(x):
+    """``idOrMissing`` (``src/linelist.jl:439``): 0 means missing."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.where(x == 0, np.nan, x)
+
+
+def galah_linelist_from_columns(formula, ionization, wl, log_gf, E_lo, gamma_rad, gamma_stark,
+                                vdW):
+    """Lines from the GALAH DR3 file's columns, as Korg builds them (``get_GALAH_DR3_linelist``,
+    ``src/linelist.jl:894-922``): formula rows of up to 3 atomic numbers (a diatomic or triatomic
+    molecule's sorted), charge = ionization - 1, wavelengths in (vacuum) Angstrom, gamma_rad and
+    gamma_stark as log10 (0 = missing: Korg's defaults), vdW as stored (0 = missing), and the H I
+    lines removed.  Returns a ``LineList`` in the columns' order.
+    """
+    formula = np.asarray(formula)
+    ion = np.asarray(ionization).astype(np.int64)
+    # single atom when the second entry is 0, else the sorted first two or all three
+    rows = np.zeros_like(formula)
+    single = formula[:, 1] == 0
+    two = ~single & (formula[:, 2] == 0)
+    three = ~single & ~two
+    rows[single, 0] = formula[single, 0]
+    rows[two, :2] = np.sort(formula[two, :2], axis=1)
+    rows[three] = np.sort(formula[three], axis=1)
+    keep = ~(single & (formula[:, 0] == 1) & (ion == 1))  # H I
+    species, sidx = _species_table(rows[keep], ion[keep] - 1)
+    col = lambda x: np.asarray(x).astype(np.float64)[keep]
+    return LineList.from_arrays(col(wl), col(log_gf), species, sidx, col(E_lo),
+                                gamma_rad=_tentothe_or_missing(col(gamma_rad)),
+                                gamma_stark=_tentothe_or_missing(col(gamma_stark)),
+                                vdW=_id_or_missing(col(vdW)))
+
+
+GALAH_COLUMNS = ("formula", "ionization", "wl", "log_gf", "E_lo", "gamma_rad", "gamma_stark", "vdW")
+
+
+def read_galah_dr3_linelist(path):
+    """The GALAH DR3 linelist file (``galah_dr3_linelist.h5``), read with the HDF5 reader and
+    converted by ``galah_linelist_from_columns``."""
+    from .hdf5 import H5File
+    f = H5File(path)
+    return galah_linelist_from_columns(*(f.read(k) for k in GALAH_COLUMNS))
+
+
+def read_korg_linelist(path):
+    """A linelist written by Korg's ``save_linelist`` (``src/linelist.jl:785-818``), read as
+    ``read_korg_linelist`` does (``src/linelist.jl:820-839``): wavelengths in cm, the formula as
+    Korg's 6-slot atomic-number array (zeros first), charge, E_lower, gamma_rad, gamma_stark and
+    the (vdW_1, vdW_2) pair, all stored as given (no defaults are imputed).
+    """
+    from .hdf5 import H5File
+    f = H5File(path)
+    formula = f.read("formula")  # (n, 6)
+    if np.any(formula.max(axis=1) == 0):
+        raise ValueError(f"{path}: a line has an empty formula")
+    species, sidx = _species_table(formula, f.read("charge"))
+    return LineList(f.read("wl"), f.read("log_gf"), f.read("E_lower"), f.read("gamma_rad"),
+                    f.read("gamma_stark"), f.read("vdW_1"), f.read("vdW_2"), sidx, species)
+
+
+def filter_linelist(linelist, wls, line_buffer):
+    """``filter_linelist`` (``src/synthesize.jl:331-350``): the lines (of a wavelength-sorted
+    list) within ``line_buffer`` cm of each of the grid's ranges, each line at most once, in order.
+    """
+    wl = linelist.wl
+    idx, last, start = [], 0, 0
+    for _, _, n in wls.wl_ranges:
+        lo, hi = wls.all_wls[start], wls.all_wls[start + n - 1]
+        start += n
+        first = int(np.searchsorted(wl, lo - line_buffer, side="left")) + 1  # 1-based
+        first = max(first, last + 1)
+        last = int(np.searchsorted(wl, hi + line_buffer, side="right"))
+        if last >= first:
+            idx.append(np.arange(first - 1, last))
+    sel = np.concatenate(idx) if idx else np.zeros(0, dtype=np.int64)
+    return linelist.take(sel)

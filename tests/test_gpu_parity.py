@@ -62,6 +62,23 @@ def check_fp32(ctx, P, ref=None):
     return got, err
 
 
+# ----------------------------------------------------------------------------- real lines
+def test_galah_lines_fp64_and_fp32(ctx):
+    """Real lines: the GALAH DR3 Fe, Ti, Ni and Ca I-II lines of 4795-4825 A (tests/golden, raw
+    file columns), built as Korg builds them, including 20+ packed-ABO van der Waals lines, on
+    the 4800-4820 A grid of the solar atmosphere."""
+    import os
+    from korg_line.linelist import GALAH_COLUMNS, galah_linelist_from_columns
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "galah_4800_fe_ti_ni_ca.npz"))
+    ll = galah_linelist_from_columns(*(z[k] for k in GALAH_COLUMNS))
+    assert len(ll) == 204 and np.count_nonzero(ll.vdW2 != -1.0) >= 20 and ll.is_sorted()
+    atm = Atmosphere.from_model("sun")
+    P = workloads.make_problem(atm, ll, Wavelengths((4800, 4820)))
+    ref = check_fp64(ctx, P)
+    check_fp32(ctx, P, ref)
+
+
 # ----------------------------------------------------------------------------- solar, c1
 def test_solar_small_fp64_and_fp32(ctx):
     P, atm, wls = workloads.solar_small()
