@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 8
+#define KORG_LINE_ABI_VERSION 9
 
 enum korg_status {
     KORG_OK = 0,
@@ -368,6 +368,60 @@ typedef struct {
 } korg_flux_target;
 int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                       const korg_flux_target* targets, int alpha_dtype, int synchronize);
+
+/*
+ * Hydrogen lines (src/hydrogen_line_absorption.jl:75-150): the Stehle & Hutcheon (1999) Stark
+ * profiles of the Lyman, Balmer and Paschen series to n = 30, plus the Barklem et al. (2000)
+ * resonance profiles of H alpha, beta and gamma, with the MHD occupation probabilities
+ * (src/statmech.jl:463-497).  The Brackett series (src/hydrogen_line_absorption.jl:152-169) is not
+ * included: its lines lie beyond 14,434 A (the series limit less the 150 A window), and grids that
+ * reach past that are rejected.
+ *
+ * korg_set_hydrogen_tables copies the tables to the device once per context, replacing Korg's
+ * load-time _load_stark_profiles (src/hydrogen_line_absorption.jl:6-39).  Per transition, in
+ * Korg's (the file's) order: its levels and log gf, and offsets into packed host arrays of its
+ * table axes -- temperatures [n_T], electron densities [n_ne], log(delta_nu / F0) [n_dnu] (entry 0
+ * is not read: Korg's first knot is -floatmax) -- its log profile [n_dnu][n_ne][n_T] (log 0 given
+ * as -700, as Korg clips it) and its line centre [n_ne][n_T] in cm.
+ */
+typedef struct {
+    int32_t lower, upper;      /* principal quantum numbers */
+    int32_t n_T, n_ne, n_dnu;  /* axis lengths */
+    int32_t reserved;
+    double log_gf;
+    int64_t off_T, off_ne, off_dnu, off_profile, off_lambda0;
+} korg_htransition;
+typedef struct {
+    int32_t n_transitions;
+    const korg_htransition* transitions;
+    const double* temps;
+    const double* n_e;
+    const double* log_dnu;
+    const double* log_profile;
+    const double* lambda0;
+    int64_t n_temps, n_n_e, n_log_dnu, n_log_profile, n_lambda0;  /* packed array lengths */
+} korg_htables;
+int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* tables);
+
+/*
+ * synthesize's hydrogen-line loop (src/synthesize.jl:293-301): for every layer l,
+ * hydrogen_line_absorption!(alpha[l, :], wls, T_l, n_e_l, n_HI_l, n_HeI_l, U_HI_l, xi_l,
+ * window_size; use_MHD), accumulating into alpha (column-major [n_layers x n_wl], f64 or f32).
+ * Device pointers throughout; U_HI is the H I partition function at each layer's temperature.
+ * window_size in cm (synthesize's default: 150e-8).  Float64 arithmetic.
+ */
+typedef struct {
+    int32_t n_layers;
+    const double* temps;
+    const double* n_e;
+    const double* n_HI;
+    const double* n_HeI;
+    const double* U_HI;
+    const double* xi;  /* cm/s */
+} korg_hlayers;
+int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                  const korg_grid* grid, const korg_hlayers* layers,
+                                  double window_size, int use_MHD, int synchronize);
 
 #ifdef __cplusplus
 }

@@ -2634,6 +2634,219 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Hydrogen lines (src/hydrogen_line_absorption.jl:75-150), Float64.  k_hsetup: one thread per
+// (layer, transition) -- the table-range check, the line centre and the (T, n_e)-blended log
+// profile slice (bilinear over the transition's temperature and density axes: Korg's trilinear
+// interpolation, separated), the MHD-weighted amplitude, the grid window and, for H alpha, beta,
+// gamma, the resonance-broadened Voigt parameters.  k_hlines: a 32-point tile of the grid per
+// workgroup, lanes = layers; the transitions in the tables' order, each adding its resonance
+// profile and then its Stark profile; the tile's sums are added into alpha once (no atomics).
+// ---------------------------------------------------------------------------------------------
+constexpr double RydbergH_eV = 13.598287264;
+constexpr double eV_to_cgs = 1.602e-12;
+constexpr double bohr_radius_cgs = 5.29177210903e-9;
+constexpr double H_MASS = 1.008 * amu_cgs;
+constexpr int HCT = 32;  // grid points per k_hlines tile
+
+struct HTrans {  // device copy of korg_htransition
+    int32_t lower, upper, n_T, n_ne, n_dnu, pad;
+    double log_gf;
+    int64_t off_T, off_ne, off_dnu, off_profile, off_lambda0;
+};
+
+struct HPair {
+    double nu0, F0, amp;
+    double lam_res, inv, a, scaling;  // resonance profile (inv == 0: none)
+    int32_t lb, ub;                   // grid window, 0-based; lb > ub: the pair adds nothing
+};
+
+struct HArgs {
+    int L, n_trans, max_dnu, use_MHD;
+    int64_t n_wl;
+    double window;
+    const double *wls, *T, *ne, *nH, *nHe, *U, *xi;
+    const HTrans* trans;
+    const double *kT, *kne, *kdnu, *logP, *lam0;
+    HPair* pairs;
+    double* slices;  // [pair][max_dnu]
+    void* alpha;
+    int alpha_f32;
+};
+
+// src/statmech.jl:463-497 (without the Hubeny generalization)
+__device__ inline double hummer_mihalas_w(double T, double n, double nH, double nHe, double ne) {
+    const double r_level = sqrt(5.0 / 2.0 * (n * n * n * n) + 1.0 / 2.0 * (n * n)) * bohr_radius_cgs;
+    const double rH = r_level + sqrt(3.0) * bohr_radius_cgs, rHe = r_level + 1.02 * bohr_radius_cgs;
+    const double neutral = nH * (rH * rH * rH) + nHe * (rHe * rHe * rHe);
+    const double K = n > 3 ? 16.0 / 3.0 * ((n / (n + 1)) * (n / (n + 1))) *
+                                 ((n + 7.0 / 6.0) / (n * n + n + 1.0 / 2.0))
+                           : 1.0;
+    const double chi = RydbergH_eV / (n * n) * eV_to_cgs;
+    const double e = electron_charge_cgs;
+    const double q = (e * e) / (chi * sqrt(K));
+    const double charged = 16.0 * (q * q * q) * ne;
+    return exp(-4.0 * PI / 3.0 * (neutral + charged));
+}
+
+// the cell i of ascending knots[0..n) with knots[i] <= x < knots[i + 1], for knots[0] < x < knots[n-1]
+__device__ inline int knot_cell(const double* k, int n, double x) {
+    int i = 0;
+    while (i + 2 < n && k[i + 1] <= x) ++i;
+    return i;
+}
+
+__device__ inline int64_t lower_bound_f64(const double* a, int64_t n, double x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        if (a[m] < x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__device__ inline int64_t upper_bound_f64(const double* a, int64_t n, double x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        if (a[m] <= x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void k_hsetup(HArgs A) {
+    // no fused multiply-adds: the line centre (and so the window edges, which often fall on grid
+    // points exactly: the tables' centres and the grids are both in steps of 0.01 A) is rounded
+    // as the plain Float64 expression is, operation by operation
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= A.L * A.n_trans) return;
+    const int l = p / A.n_trans, t = p % A.n_trans;
+    const HTrans tr = A.trans[t];
+    const double T = A.T[l], ne = A.ne[l];
+    HPair h = {};
+    h.lb = 1;
+    h.ub = 0;
+    const double* kT = A.kT + tr.off_T;
+    const double* kN = A.kne + tr.off_ne;
+    // src/hydrogen_line_absorption.jl:102-104: only inside the tables' (T, n_e) range
+    if (!(kT[0] < T && T < kT[tr.n_T - 1] && kN[0] < ne && ne < kN[tr.n_ne - 1])) {
+        A.pairs[p] = h;
+        return;
+    }
+    const int i = knot_cell(kT, tr.n_T, T), j = knot_cell(kN, tr.n_ne, ne);
+    const double s = (T - kT[i]) / (kT[i + 1] - kT[i]);
+    const double u = (ne - kN[j]) / (kN[j + 1] - kN[j]);
+    const int nT = tr.n_T;
+    auto bilinear = [&](const double* tab) {
+        const double lo = tab[j * nT + i] * (1 - s) + tab[j * nT + i + 1] * s;
+        const double hi = tab[(j + 1) * nT + i] * (1 - s) + tab[(j + 1) * nT + i + 1] * s;
+        return lo * (1 - u) + hi * u;
+    };
+    double lam0 = bilinear(A.lam0 + tr.off_lambda0);
+    // :106-111
+    const double beta = 1.0 / (kboltz_eV * T);
+    const double Elo = RydbergH_eV * (1.0 - 1.0 / (double)(tr.lower * tr.lower));
+    const double Eup = RydbergH_eV * (1.0 - 1.0 / (double)(tr.upper * tr.upper));
+    const double w = A.use_MHD ? hummer_mihalas_w(T, (double)tr.upper, A.nH[l], A.nHe[l], ne) : 1.0;
+    const double levels = w * (exp(-beta * Elo) - exp(-beta * Eup)) / A.U[l];
+    const double amp = pow(10.0, tr.log_gf) * A.nH[l] * sigma_line(lam0) * levels;
+    // :113-117
+    const int64_t lb = lower_bound_f64(A.wls, A.n_wl, lam0 - A.window);
+    const int64_t ub = upper_bound_f64(A.wls, A.n_wl, lam0 + A.window) - 1;
+    if (lb >= ub) {
+        A.pairs[p] = h;
+        return;
+    }
+    h.lb = (int32_t)lb;
+    h.ub = (int32_t)ub;
+    h.amp = amp;
+    // :120-136: H alpha, beta, gamma also get the Barklem et al. resonance profile, centred on
+    // fixed wavelengths (which then also centre the Stark profile)
+    if (tr.lower == 2 && tr.upper >= 3 && tr.upper <= 5) {
+        double sABO, aABO;
+        if (tr.upper == 3) {
+            lam0 = 6.56460998e-5, sABO = 1180.0, aABO = 0.677;
+        } else if (tr.upper == 4) {
+            lam0 = 4.8626810200000004e-5, sABO = 2320.0, aABO = 0.455;
+        } else {
+            lam0 = 4.34168232e-5, sABO = 4208.0, aABO = 0.380;
+        }
+        const double Gamma = scaled_vdW_abo(sABO * bohr_radius_cgs * bohr_radius_cgs, aABO, H_MASS, T) * A.nH[l];
+        const double gamma = Gamma * lam0 * lam0 / (c_cgs * 4.0 * PI);
+        const double sigma = doppler_width(lam0, T, H_MASS, A.xi[l]);
+        const double inv = 1.0 / (sigma * sqrt(2.0));
+        h.lam_res = lam0;
+        h.inv = inv;
+        h.a = gamma * inv;
+        h.scaling = inv / sqrt(PI) * amp;
+    }
+    // :139-142
+    h.nu0 = c_cgs / lam0;
+    h.F0 = 1.25e-9 * pow(ne, 2.0 / 3.0);
+    A.pairs[p] = h;
+    double* sl = A.slices + (int64_t)p * A.max_dnu;
+    const double* P = A.logP + tr.off_profile;
+    for (int k = 0; k < tr.n_dnu; ++k) sl[k] = bilinear(P + (int64_t)k * tr.n_ne * nT);
+}
+
+__global__ __launch_bounds__(64) void k_hlines(HArgs A) {
+#pragma clang fp contract(off)
+    const int l = blockIdx.y * 64 + threadIdx.x;
+    const bool lv = l < A.L;
+    const int64_t j0 = (int64_t)blockIdx.x * HCT;
+    const int jn = (int)min((int64_t)HCT, A.n_wl - j0);
+    double acc[HCT];
+#pragma unroll
+    for (int q = 0; q < HCT; ++q) acc[q] = 0.0;
+    for (int t = 0; t < A.n_trans; ++t) {
+        const int64_t p = (int64_t)(lv ? l : 0) * A.n_trans + t;
+        const HPair h = A.pairs[p];
+        const bool ov = lv && h.lb <= h.ub && h.lb < j0 + jn && h.ub >= j0;
+        if (!ballot(ov)) continue;  // wavefront-uniform
+        const HTrans tr = A.trans[t];
+        const double* kn = A.kdnu + tr.off_dnu;  // kn[0] stands for -floatmax
+        const int nd = tr.n_dnu;
+        const double* sl = A.slices + p * A.max_dnu;
+        const double k1 = kn[1], klast = kn[nd - 1];
+#pragma unroll
+        for (int q = 0; q < HCT; ++q) {
+            const int64_t jj = j0 + q;
+            if (!(q < jn && ov && jj >= h.lb && jj <= h.ub)) continue;
+            const double lam = A.wls[jj];
+            if (h.inv != 0.0) acc[q] += voigt_hjerting64(h.a, fabs(lam - h.lam_res) * h.inv) * h.scaling;
+            const double nu = c_cgs / lam;
+            double x = fabs(nu - h.nu0) / h.F0;
+            if (x == 0.0) x = 2.2250738585072014e-308;  // _zero2epsilon: the smallest normal double
+            const double y = log(x);
+            double v;
+            if (y < k1) {
+                v = sl[1];  // the first cell's weight is 1: its left knot is -floatmax
+            } else if (y >= klast) {
+                v = sl[nd - 1];  // flat beyond the last knot
+            } else {
+                int lo = 1, hi = nd - 1;  // kn[lo] <= y < kn[hi]
+                while (hi - lo > 1) {
+                    const int m = (lo + hi) >> 1;
+                    if (kn[m] <= y) lo = m;
+                    else hi = m;
+                }
+                const double wgt = (y - kn[lo]) / (kn[lo + 1] - kn[lo]);
+                v = sl[lo] * (1 - wgt) + sl[lo + 1] * wgt;
+            }
+            acc[q] += exp(v) * (c_cgs / (lam * lam)) * h.amp;
+        }
+    }
+    if (!lv) return;
+    for (int q = 0; q < jn; ++q) {
+        const int64_t k = (int64_t)l + (int64_t)A.L * (j0 + q);
+        if (A.alpha_f32) ((float*)A.alpha)[k] += (float)acc[q];
+        else ((double*)A.alpha)[k] += acc[q];
+    }
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -2700,6 +2913,9 @@ struct korg_ctx {
     // staging for the host-memory entry points
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
+    // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
+    DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices;
+    int hl_ntrans = 0, hl_max_dnu = 0;
 };
 
 namespace {
@@ -4056,8 +4272,10 @@ void korg_destroy(korg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ws.release();
-    DevBuf* bufs[] = {&ctx->vjp_out, &ctx->h_grad, &ctx->h_alpha, &ctx->h_wls,
-                      &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm};
+    DevBuf* bufs[] = {&ctx->vjp_out,  &ctx->h_grad,   &ctx->h_alpha,   &ctx->h_wls,
+                      &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm,
+                      &ctx->hl_trans, &ctx->hl_T,     &ctx->hl_ne,     &ctx->hl_dnu,
+                      &ctx->hl_logP,  &ctx->hl_lam0,  &ctx->hl_pairs,  &ctx->hl_slices};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
@@ -4380,6 +4598,120 @@ int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     TRY(run_flux(ctx, hs, L, alpha_dtype));
+    if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return KORG_OK;
+}
+
+int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* t) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    if (!t || t->n_transitions < 1 || t->n_transitions > 4096 || !t->transitions || !t->temps ||
+        !t->n_e || !t->log_dnu || !t->log_profile || !t->lambda0)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "hydrogen tables: missing arrays or no transitions");
+    std::vector<HTrans> tr(t->n_transitions);
+    int max_dnu = 0;
+    auto fits = [](int64_t off, int64_t len, int64_t n) { return off >= 0 && len >= 0 && off <= n - len; };
+    auto ascending = [](const double* a, int64_t from, int64_t n) {
+        for (int64_t i = from + 1; i < n; ++i)
+            if (!(a[i] > a[i - 1])) return false;
+        return true;
+    };
+    for (int k = 0; k < t->n_transitions; ++k) {
+        const korg_htransition& x = t->transitions[k];
+        const int64_t nP = (int64_t)x.n_dnu * x.n_ne * x.n_T, nL = (int64_t)x.n_ne * x.n_T;
+        if (x.lower < 1 || x.upper <= x.lower || x.upper > 64 || x.n_T < 2 || x.n_ne < 2 ||
+            x.n_dnu < 2 || x.n_dnu > 1024 || !fits(x.off_T, x.n_T, t->n_temps) ||
+            !fits(x.off_ne, x.n_ne, t->n_n_e) || !fits(x.off_dnu, x.n_dnu, t->n_log_dnu) ||
+            !fits(x.off_profile, nP, t->n_log_profile) || !fits(x.off_lambda0, nL, t->n_lambda0) ||
+            !ascending(t->temps + x.off_T, 0, x.n_T) || !ascending(t->n_e + x.off_ne, 0, x.n_ne) ||
+            !ascending(t->log_dnu + x.off_dnu, 1, x.n_dnu))
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "hydrogen tables: transition %d (%d -> %d) has invalid levels, axes or offsets",
+                           k, x.lower, x.upper);
+        tr[k] = {x.lower, x.upper, x.n_T, x.n_ne, x.n_dnu, 0, x.log_gf,
+                 x.off_T, x.off_ne, x.off_dnu, x.off_profile, x.off_lambda0};
+        max_dnu = std::max(max_dnu, x.n_dnu);
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct {
+        DevBuf* b;
+        const void* src;
+        size_t bytes;
+    } up[] = {{&ctx->hl_trans, tr.data(), tr.size() * sizeof(HTrans)},
+              {&ctx->hl_T, t->temps, (size_t)t->n_temps * 8},
+              {&ctx->hl_ne, t->n_e, (size_t)t->n_n_e * 8},
+              {&ctx->hl_dnu, t->log_dnu, (size_t)t->n_log_dnu * 8},
+              {&ctx->hl_logP, t->log_profile, (size_t)t->n_log_profile * 8},
+              {&ctx->hl_lam0, t->lambda0, (size_t)t->n_lambda0 * 8}};
+    for (auto& u : up) {
+        TRY(grow(ctx, *u.b, u.bytes));
+        HIP_TRY(ctx, hipMemcpyAsync(u.b->p, u.src, u.bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->hl_ntrans = t->n_transitions;
+    ctx->hl_max_dnu = max_dnu;
+    return KORG_OK;
+}
+
+int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* grid,
+                                  const korg_hlayers* layers, double window_size, int use_MHD,
+                                  int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    if (ctx->hl_ntrans == 0)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no hydrogen tables: call korg_set_hydrogen_tables first");
+    if (!alpha || !grid || !grid->all_wls || grid->n_wl < 1 || grid->n_wl > INT32_MAX - HCT)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha and a grid of 1 .. 2^31 points are required");
+    if (alpha_dtype != KORG_DTYPE_F64 && alpha_dtype != KORG_DTYPE_F32)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha_dtype must be KORG_DTYPE_F64 or _F32");
+    if (!layers || layers->n_layers < 1 || layers->n_layers > 65535 || !layers->temps ||
+        !layers->n_e || !layers->n_HI || !layers->n_HeI || !layers->U_HI || !layers->xi)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "hydrogen lines: layer arrays missing");
+    if (!(window_size >= 0.0) || !std::isfinite(window_size))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "window_size must be finite and >= 0");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // the Brackett series (src/hydrogen_line_absorption.jl:152-169) is not included: refuse grids
+    // that reach the windows of its lines (all beyond its series limit, 14,589 A)
+    double last = 0.0;
+    HIP_TRY(ctx, hipMemcpyAsync(&last, grid->all_wls + grid->n_wl - 1, sizeof last,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const double brackett_limit = hplanck_eV * c_cgs / (RydbergH_eV / 16.0);
+    if (!(last < brackett_limit - window_size))
+        return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                       "the grid reaches %.1f A; hydrogen lines of the Brackett series (from %.1f A, "
+                       "less the window) are not included", last * 1e8, brackett_limit * 1e8);
+    const int L = layers->n_layers, NT = ctx->hl_ntrans;
+    TRY(grow(ctx, ctx->hl_pairs, (size_t)L * NT * sizeof(HPair)));
+    TRY(grow(ctx, ctx->hl_slices, (size_t)L * NT * ctx->hl_max_dnu * sizeof(double)));
+    HArgs a;
+    a.L = L;
+    a.n_trans = NT;
+    a.max_dnu = ctx->hl_max_dnu;
+    a.use_MHD = use_MHD ? 1 : 0;
+    a.n_wl = grid->n_wl;
+    a.window = window_size;
+    a.wls = grid->all_wls;
+    a.T = layers->temps;
+    a.ne = layers->n_e;
+    a.nH = layers->n_HI;
+    a.nHe = layers->n_HeI;
+    a.U = layers->U_HI;
+    a.xi = layers->xi;
+    a.trans = (const HTrans*)ctx->hl_trans.p;
+    a.kT = (const double*)ctx->hl_T.p;
+    a.kne = (const double*)ctx->hl_ne.p;
+    a.kdnu = (const double*)ctx->hl_dnu.p;
+    a.logP = (const double*)ctx->hl_logP.p;
+    a.lam0 = (const double*)ctx->hl_lam0.p;
+    a.pairs = (HPair*)ctx->hl_pairs.p;
+    a.slices = (double*)ctx->hl_slices.p;
+    a.alpha = alpha;
+    a.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+    hipLaunchKernelGGL(k_hsetup, dim3((L * NT + 63) / 64), dim3(64), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_hlines, dim3((unsigned)((grid->n_wl + HCT - 1) / HCT), (L + 63) / 64), dim3(64), 0,
+                       ctx->stream, a);
+    HIP_TRY(ctx, hipGetLastError());
     if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;
 }

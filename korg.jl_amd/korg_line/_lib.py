@@ -97,7 +97,28 @@ class KorgPlanInfo(ctypes.Structure):
                 ("pad", ctypes.c_int32), ("tuning", KorgTuning)]
 
 
-ABI_VERSION = 8  # KORG_LINE_ABI_VERSION of include/korg_line.h
+class KorgHTransition(ctypes.Structure):
+    _fields_ = [("lower", ctypes.c_int32), ("upper", ctypes.c_int32), ("n_T", ctypes.c_int32),
+                ("n_ne", ctypes.c_int32), ("n_dnu", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("log_gf", ctypes.c_double), ("off_T", ctypes.c_int64), ("off_ne", ctypes.c_int64),
+                ("off_dnu", ctypes.c_int64), ("off_profile", ctypes.c_int64),
+                ("off_lambda0", ctypes.c_int64)]
+
+
+class KorgHTables(ctypes.Structure):
+    _fields_ = [("n_transitions", ctypes.c_int32), ("transitions", ctypes.POINTER(KorgHTransition)),
+                ("temps", _c_dp), ("n_e", _c_dp), ("log_dnu", _c_dp), ("log_profile", _c_dp),
+                ("lambda0", _c_dp), ("n_temps", ctypes.c_int64), ("n_n_e", ctypes.c_int64),
+                ("n_log_dnu", ctypes.c_int64), ("n_log_profile", ctypes.c_int64),
+                ("n_lambda0", ctypes.c_int64)]
+
+
+class KorgHLayers(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int32), ("temps", _c_dp), ("n_e", _c_dp), ("n_HI", _c_dp),
+                ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
+
+
+ABI_VERSION = 9  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -106,7 +127,8 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_line_absorption_batched", "korg_get_tuning", "korg_plan_batched",
             "korg_plan_launch", "korg_plan_wait", "korg_plan_get_info", "korg_plan_destroy",
             "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit",
-            "korg_continuum_batched", "korg_flux_batched"]
+            "korg_continuum_batched", "korg_flux_batched", "korg_set_hydrogen_tables",
+            "korg_hydrogen_line_absorption"]
 
 _lib = None
 
@@ -191,6 +213,11 @@ def load():
     L.korg_flux_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar), ctypes.POINTER(KorgFluxTarget),
                                     i32, i32]
     L.korg_flux_batched.restype = i32
+    L.korg_set_hydrogen_tables.argtypes = [vp, ctypes.POINTER(KorgHTables)]
+    L.korg_set_hydrogen_tables.restype = i32
+    L.korg_hydrogen_line_absorption.argtypes = [vp, vp, i32, ctypes.POINTER(KorgGrid),
+                                                ctypes.POINTER(KorgHLayers), d, i32, i32]
+    L.korg_hydrogen_line_absorption.restype = i32
     if L.korg_abi_version() != ABI_VERSION:
         raise ImportError(f"{LIB_PATH} has ABI version {L.korg_abi_version()}, these bindings "
                           f"expect {ABI_VERSION}; rebuild it")

@@ -339,6 +339,39 @@ class Context:
         rc = self._L.korg_flux_batched(self._h, len(dprobs), stars, tg, dtc, 1 if synchronize else 0)
         _lib.check(self._h, rc)
 
+    def set_hydrogen_tables(self, tables=None):
+        """korg_set_hydrogen_tables: the Stark tables (a ``hydrogen.StarkTables``; default: the
+        shipped file) to the device, once per context."""
+        from .hydrogen import StarkTables
+        t = tables if tables is not None else StarkTables()
+        _lib.check(self._h, self._L.korg_set_hydrogen_tables(self._h, ctypes.byref(t.struct)))
+        self._htables = t
+
+    def hydrogen_line_absorption(self, alpha, dprob, U_HI, window_size=150e-8, use_MHD=None,
+                                 synchronize=True):
+        """korg_hydrogen_line_absorption: synthesize's hydrogen lines (src/synthesize.jl:293-301)
+        added into ``alpha`` (a device alpha of ``dprob``) with the problem's grid and layers;
+        ``U_HI``: the H I partition function at each layer's temperature; ``use_MHD`` defaults as
+        synthesize does (grids ending below 13,000 A, src/synthesize.jl:166-168)."""
+        from .hipmem import DeviceArray
+        if getattr(self, "_htables", None) is None:
+            self.set_hydrogen_tables()
+        P = dprob.problem
+        if use_MHD is None:
+            use_MHD = bool(P.all_wls[-1] < 13_000e-8)
+        U = np.ascontiguousarray(U_HI, dtype=np.float64)
+        if U.shape != (P.n_layers,):
+            raise ValueError(f"U_HI must have one value per layer ({P.n_layers})")
+        self._hl_U = DeviceArray.from_numpy(U)  # held until the next call
+        t = dprob.t
+        lay = _lib.KorgHLayers(P.n_layers, t["temps"].ptr, t["n_e"].ptr, t["n_HI"].ptr,
+                               t["n_HeI"].ptr, self._hl_U.ptr, t["xi"].ptr)
+        dt = _lib.DTYPE_F32 if alpha.dtype == np.float32 else _lib.DTYPE_F64
+        rc = self._L.korg_hydrogen_line_absorption(self._h, alpha.ptr, dt, ctypes.byref(dprob.grid),
+                                                   ctypes.byref(lay), float(window_size),
+                                                   1 if use_MHD else 0, 1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
     def autotune_batched(self, alphas, dprobs, precision="fp32", cache_path=None):
         """Times the tuning candidates on these stars (into scratch alphas), makes the fastest
         this context's tuning and returns it; ``cache_path``: optional results file."""

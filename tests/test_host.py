@@ -155,7 +155,7 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 8
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 9
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -167,7 +167,9 @@ def test_struct_layouts_match_header(tmp_path):
                "korg_continuum": _lib.KorgContinuum, "korg_stats": _lib.KorgStats,
                "korg_tuning": _lib.KorgTuning, "korg_layer_grads": _lib.KorgLayerGrads,
                "korg_star": _lib.KorgStar, "korg_plan_info": _lib.KorgPlanInfo,
-               "korg_fit_target": _lib.KorgFitTarget, "korg_flux_target": _lib.KorgFluxTarget}
+               "korg_fit_target": _lib.KorgFitTarget, "korg_flux_target": _lib.KorgFluxTarget,
+               "korg_htransition": _lib.KorgHTransition, "korg_htables": _lib.KorgHTables,
+               "korg_hlayers": _lib.KorgHLayers}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "korg_line.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
