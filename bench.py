@@ -60,6 +60,7 @@ def main():
                     help="fraction of the lines timed on the CPU oracle (every k-th line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flux", action="store_true")
+    ap.add_argument("--no-hydrogen", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (PMC byte-counter scale)")
     ap.add_argument("--workload", choices=("solar_wide", "batched", "fit"), default="solar_wide")
@@ -184,6 +185,8 @@ def main():
     issued = MFMA_FLOPS * n_mfma / (wing_ms * 1e-3) / 1e12
     ref_flops = sum(c * f for c, f in zip((nA, nB, nC, nD), REF_FLOPS))
 
+    hyd = None if args.no_hydrogen else hydrogen_lines_bench(ctx, D, atm)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(P, args.cpu_fraction)
@@ -241,8 +244,33 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if hyd is not None:
+        out["hydrogen_lines"] = hyd
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def hydrogen_lines_bench(ctx, D, atm, reps=5):
+    """synthesize's hydrogen lines (korg_hydrogen_line_absorption) on the same grid and layers:
+    the 84 tabulated transitions, 150 A windows, MHD as synthesize chooses it, into a scratch
+    Float32 alpha; host-timed over ``reps`` calls between device synchronizations."""
+    import numpy as np
+    from korg_line.atmosphere import partition_function
+    from korg_line.hipmem import synchronize as device_sync
+    U = partition_function("H_I")(np.log(atm.temps))
+    a = D.new_alpha("float32")
+    ctx.set_hydrogen_tables()
+    ctx.hydrogen_line_absorption(a, D, U)  # warm-up
+    device_sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.hydrogen_line_absorption(a, D, U, synchronize=False)
+    device_sync()
+    ms = (time.perf_counter() - t) / reps * 1e3
+    log(f"hydrogen lines: {ms:.3f} ms per call")
+    return {"ms": ms, "transitions": 84, "window_A": 150.0,
+            "scheme": "Stehle-Hutcheon Stark tables (Lyman/Balmer/Paschen to n = 30), Barklem "
+                      "resonance for H alpha-gamma, MHD; Float64, host-timed"}
 
 
 TUNING_CACHE = os.path.join(ROOT, "korg.jl_amd", "korg_line", "tuning_mi355x.txt")
