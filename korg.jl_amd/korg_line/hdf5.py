@@ -32,6 +32,7 @@ MSG_LINK, MSG_LAYOUT, MSG_FILTERS, MSG_ATTRIBUTE, MSG_CONTINUATION, MSG_SYMBOL_T
     0x6, 0x8, 0xB, 0xC, 0x10, 0x11)
 
 FILTER_DEFLATE, FILTER_SHUFFLE, FILTER_FLETCHER32 = 1, 2, 3
+MAX_DATASET_BYTES = 1 << 34  # 16 GiB: larger shapes mean a corrupt header here
 
 
 class H5File:
@@ -63,7 +64,14 @@ class H5File:
 
     # -- primitive reads ----------------------------------------------------------------------
     def _u(self, p, n):
+        if p < 0 or p + n > len(self._d):
+            raise H5Error(f"read of {n} bytes at {p:#x} is past the end of the file (truncated?)")
         return int.from_bytes(self._d[p:p + n], "little")
+
+    def _bytes(self, p, n):
+        if p < 0 or n < 0 or p + n > len(self._d):
+            raise H5Error(f"block of {n} bytes at {p:#x} is past the end of the file (truncated?)")
+        return self._d[p:p + n]
 
     def _addr(self, a):
         return a + self.base
@@ -308,7 +316,7 @@ class H5File:
                 self._chunks(child, ndim, out)
             else:
                 c = self._addr(child)
-                out.append((mask, d[c:c + size], offs))
+                out.append((mask, self._bytes(c, size), offs))
             q += key_size + 8
 
     def shape(self, path):
@@ -322,16 +330,18 @@ class H5File:
         n = 1
         for x in shape:
             n *= x
+        if n * dt.itemsize > MAX_DATASET_BYTES:
+            raise H5Error(f"{path}: {n} elements of {dt} exceed this reader's limit")
         kind = info["layout"][0]
         if kind == "compact":
             _, q, size = info["layout"]
-            return np.frombuffer(self._d[q:q + n * dt.itemsize], dtype=dt).reshape(shape).copy()
+            return np.frombuffer(self._bytes(q, n * dt.itemsize), dtype=dt).reshape(shape).copy()
         if kind == "contiguous":
             _, addr, _ = info["layout"]
             if addr == UNDEF:  # never written: the fill value (zero)
                 return np.zeros(shape, dtype=dt)
             q = self._addr(addr)
-            return np.frombuffer(self._d[q:q + n * dt.itemsize], dtype=dt).reshape(shape).copy()
+            return np.frombuffer(self._bytes(q, n * dt.itemsize), dtype=dt).reshape(shape).copy()
         _, btree, cdims = info["layout"]
         cshape = cdims[:-1]
         if cdims[-1] != dt.itemsize or len(cshape) != len(shape):
@@ -346,6 +356,8 @@ class H5File:
             cn *= x
         for mask, raw, offs in chunks:
             raw = self._unfilter(raw, info["filters"], mask, dt.itemsize)
+            if len(raw) < cn * dt.itemsize:
+                raise H5Error(f"{path}: a chunk holds {len(raw)} bytes, {cn * dt.itemsize} expected")
             block = np.frombuffer(raw[:cn * dt.itemsize], dtype=dt).reshape(cshape)
             sel_out, sel_in = [], []
             for o, c, s in zip(offs, cshape, shape):

@@ -166,3 +166,12 @@ def test_read_korg_linelist_requires_its_columns():
     """The Korg-format reader needs Korg's column names; the Lyman file has none of them."""
     with pytest.raises(KeyError):
         read_korg_linelist(LYMAN)
+
+
+def test_hdf5_truncated_file_is_reported(tmp_path):
+    """A file cut short fails with H5Error, not with garbage values."""
+    raw = open(LYMAN, "rb").read()
+    p = tmp_path / "cut.h5"
+    p.write_bytes(raw[:len(raw) - 20000])  # the profile's data (32 KB) runs past the new end
+    with pytest.raises(H5Error):
+        H5File(str(p)).read("profile")
