@@ -1812,7 +1812,8 @@ struct VjpArgs {
 };
 
 template <bool SERIES>
-__global__ __launch_bounds__(WAVE) void k_vjp(VjpArgs A) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(SERIES ? 3 : 1)))
+void k_vjp(VjpArgs A) {
     const int lane = threadIdx.x;
     const int L = A.L;
     const int NQ = 6 + A.n_species;
