@@ -145,12 +145,16 @@ __device__ inline double voigt_hjerting64(double a, double v) {
 // in Float64.  piece: 0 (v < 1.3), 1 (v < 2.4), 2 (otherwise), decided on the Float64 v.
 // --------------------------------------------------------------------------------------------
 __device__ inline float harris_h1_f32(float v, float v2, int piece) {
-    if (piece == 0)
-        return -1.12470432f + (-0.15516677f + (3.288675912f + (-2.34357915f + 0.42139162f * v) * v) * v) * v;
-    if (piece == 1)
-        return -4.48480194f + (9.39456063f + (-6.61487486f + (1.98919585f - 0.22041650f * v) * v) * v) * v;
-    return (0.554153432f + (0.278711796f + (-0.1883256872f + (0.042991293f - 0.003278278f * v) * v) * v) * v) *
-           __builtin_amdgcn_rcpf(v2 - 1.5f);
+    if (piece == 2)
+        return (0.554153432f + (0.278711796f + (-0.1883256872f + (0.042991293f - 0.003278278f * v) * v) * v) * v) *
+               __builtin_amdgcn_rcpf(v2 - 1.5f);
+    // pieces 0 and 1 are both quartics: one Horner chain with the piece's coefficients (selected,
+    // not branched on, so that lanes in different pieces do not diverge); the values are the same
+    const bool p0 = piece == 0;
+    const float c0 = p0 ? -1.12470432f : -4.48480194f, c1 = p0 ? -0.15516677f : 9.39456063f;
+    const float c2 = p0 ? 3.288675912f : -6.61487486f, c3 = p0 ? -2.34357915f : 1.98919585f;
+    const float c4 = p0 ? 0.42139162f : -0.22041650f;
+    return c0 + (c1 + (c2 + (c3 + c4 * v) * v) * v) * v;
 }
 
 // Branch B (a <= 0.2, v < 5) or C (0.2 < a <= 1.4, a + v < 3.2) in Float32.
