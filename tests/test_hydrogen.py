@@ -181,3 +181,25 @@ def test_device_rejects_brackett_range_and_missing_tables(ctx):
     fresh._htables = True  # skip the upload: the call must report the missing tables
     with pytest.raises(_lib.KorgError):
         device_hlines(fresh, Wavelengths((4860.0, 4861.0)).all_wls, lay, 150e-8, True)
+
+
+@pytest.mark.gpu
+def test_context_wrapper_on_a_device_problem(ctx):
+    """Context.hydrogen_line_absorption on a DeviceProblem (its grid and layers, synthesize's MHD
+    default: on below 13,000 A) adds exactly what the direct call computes, onto existing alpha."""
+    from korg_line import DeviceProblem, workloads
+    from korg_line.atmosphere import Atmosphere, partition_function
+    atm = Atmosphere.from_model("sun")
+    P = workloads.make_problem(atm, workloads.synthetic_linelist(3, 20, 6540.0, 6590.0, (-3.0, 0.0),
+                                                                 (0.0, 4.0)),
+                               Wavelengths((6550.0, 6575.0)))
+    D = DeviceProblem(P)
+    base = np.random.default_rng(1).uniform(1.0, 2.0, (P.n_layers, P.n_wl))
+    a = D.new_alpha("float64", fill=base)
+    U = partition_function("H_I")(np.log(P.temps))
+    ctx.hydrogen_line_absorption(a, D, U)
+    lay = dict(T=P.temps, ne=P.n_e, nH=P.n_HI, nHe=P.n_HeI, U=U, xi=P.xi)
+    direct = device_hlines(ctx, P.all_wls, lay, 150e-8, True)
+    # alpha - base is exact to half an ulp of base (base in [1, 2): 1.1e-16)
+    np.testing.assert_allclose(a.numpy().T - base, direct, rtol=1e-12, atol=4.5e-16)
+    assert np.max(direct) > 0
