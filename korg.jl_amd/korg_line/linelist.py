@@ -365,3 +365,92 @@ def filter_linelist(linelist, wls, line_buffer):
             idx.append(np.arange(first - 1, last))
     sel = np.concatenate(idx) if idx else np.zeros(0, dtype=np.int64)
     return linelist.take(sel)
+
+
+def air_to_vacuum(lam):
+    """Air to vacuum wavelength by Birch & Downs (1994), the formula VALD publishes
+    (``src/utils.jl:240-251``): ``lam`` in Angstrom if >= 1, else in cm."""
+    cgs = lam < 1
+    if cgs:
+        lam = lam * 1e8
+    s = 1e4 / lam
+    n = 1 + 0.00008336624212083 + 0.02408926869968 / (130.1065924522 - s ** 2) + \
+        0.0001599740894897 / (38.92568793293 - s ** 2)
+    if cgs:
+        lam = lam * 1e-8
+    return lam * n
+
+
+_VALD_COLUMNS = {
+    (True, True): ["species", "wl", "E_low", "loggf", "gamma_rad", "gamma_stark", "gamma_vdW",
+                   "lande", "reference"],
+    (True, False): ["species", "wl", "E_low", "Vmic", "loggf", "gamma_rad", "gamma_stark",
+                    "gamma_vdW", "lande", "depth", "reference"],
+    (False, True): ["species", "wl", "loggf", "E_low", "J_lo", "E_up", "J_up", "lower_lande",
+                    "upper_lande", "mean_lande", "gamma_rad", "gamma_stark", "gamma_vdW"],
+}
+_VALD_COLUMNS[(False, False)] = _VALD_COLUMNS[(False, True)]
+
+
+def read_vald_linelist(path):
+    """A VALD linelist (short or long format, "extract all" or "extract stellar"), read as
+    ``parse_vald_linelist`` does (``src/linelist.jl:540-634``): energies in eV (converted from
+    cm^-1 when the header says so), wavelengths in vacuum (converted from air when the header
+    says so), gamma_rad 0 imputed by Korg's default, gamma_stark as log10 and vdW as given (0 =
+    missing).  Files whose oscillator strengths VALD did not scale by isotopic abundance are read
+    only if no isotope is named in their references (the isotopic-abundance table is not part of
+    this library).  Returns a ``LineList`` in file order.
+    """
+    import csv
+    import re
+    with open(path) as f:
+        lines = [l.rstrip("\n") for l in f if len(l.rstrip("\n")) > 0 and not l.startswith("#")]
+    if lines[0].startswith(" WARNING: Output was truncated to 100000 lines"):
+        lines = lines[1:]
+    lines = [l.replace("'", '"') for l in lines]
+    extractall = not re.match(r"^\s+\d", lines[0])
+    first = 2 if extractall else 3  # 0-based index of the first line of the body
+    header = lines[first - 1]
+    unscaled = any(l.startswith("* oscillator strengths were NOT scaled ") for l in lines)
+    if not unscaled and not any(l.startswith("* oscillator strengths were scaled ") for l in lines):
+        raise ValueError("can't detect whether the log(gf)s are scaled by isotopic abundance")
+    short = not re.match(r'^"? ', lines[first + 1])
+    body = lines[first::(1 if short else 4)]
+    end = next((k for k, l in enumerate(body) if l[0] != '"' or not l[1].isupper()), len(body))
+    body = body[:end]
+    cols = _VALD_COLUMNS[(short, extractall)]
+    rows = [dict(zip(cols, r)) for r in csv.reader(body, quotechar='"', skipinitialspace=True)]
+
+    def num(r, k):
+        return float(r[k])
+    E_low = np.array([num(r, "E_low") for r in rows])
+    if "cm" in header:
+        E_low = E_low * c_cgs * hplanck_eV
+    elif "eV" not in header:
+        raise ValueError(f"can't determine the energy units from the header: {header!r}")
+    wl = np.array([num(r, "wl") for r in rows])
+    if "air" in header:
+        wl = np.array([air_to_vacuum(x) for x in wl])
+    elif "vac" not in header:
+        raise ValueError(f"can't determine vacuum or air wavelengths from the header: {header!r}")
+    wl = 1e-8 * wl
+    if unscaled:
+        refs = [lines[first + 3 + 4 * k] for k in range(len(rows))] if not short else \
+            [r.get("reference", "") for r in rows]
+        if any(re.search(r"\((\d\d?\d?)\)([A-Z][a-z]?)", ref) for ref in refs):
+            raise NotImplementedError("isotopic scaling of unscaled VALD log(gf)s is not supported")
+    loggf = np.array([num(r, "loggf") for r in rows])
+    g_rad = np.array([num(r, "gamma_rad") for r in rows])
+    gamma_rad = np.where(g_rad == 0, np.nan, _pow10(np.where(g_rad == 0, 0.0, g_rad)))
+    names = [r["species"].strip() for r in rows]
+    species, index = [], {}
+    sidx = np.empty(len(rows), dtype=np.int32)
+    for k, nm in enumerate(names):
+        sp = Species(nm)
+        if sp not in index:
+            index[sp] = len(species)
+            species.append(sp)
+        sidx[k] = index[sp]
+    return LineList.from_arrays(wl, loggf, species, sidx, E_low, gamma_rad=gamma_rad,
+                                gamma_stark=_tentothe_or_missing([num(r, "gamma_stark") for r in rows]),
+                                vdW=_id_or_missing([num(r, "gamma_vdW") for r in rows]))

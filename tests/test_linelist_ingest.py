@@ -175,3 +175,65 @@ def test_hdf5_truncated_file_is_reported(tmp_path):
     p.write_bytes(raw[:len(raw) - 20000])  # the profile's data (32 KB) runs past the new end
     with pytest.raises(H5Error):
         H5File(str(p)).read("profile")
+
+
+# -------------------------------------------------------------------------------------- VALD
+VALD = os.path.join(HERE, "golden", "vald")  # the reference's test files (test/data/linelists)
+
+
+def test_vald_short_format_abo_and_missing_params():
+    """test/linelist.jl:192-223, value by value."""
+    from korg_line.linelist import read_vald_linelist
+    ll = read_vald_linelist(os.path.join(VALD, "linelist.vald"))
+    L = ll.to_lines()
+    assert len(L) == 6
+    assert L[0].wl == pytest.approx(3000.0414 * 1e-8, rel=1e-14)
+    assert L[0].log_gf == -2.957
+    assert L[0].species == Species("Fe_I")
+    assert L[0].E_lower == pytest.approx(3.3014, rel=1e-14)
+    assert L[0].gamma_rad == pytest.approx(1.905460717963248e7, rel=1e-14)
+    assert L[0].gamma_stark == pytest.approx(0.0001230268770812381, rel=1e-14)
+    assert L[0].vdW[0] == pytest.approx(4.6773514128719815e-8, rel=1e-14)
+    # imputation of missing broadening parameters
+    assert L[1].gamma_rad == pytest.approx(818252.5391161365, rel=1e-14)
+    assert L[1].gamma_stark == L[0].gamma_stark and L[1].vdW == L[0].vdW
+    assert L[2].gamma_rad == L[1].gamma_rad
+    assert L[2].gamma_stark == pytest.approx(0.00019044182974029873, rel=1e-14)
+    assert L[2].vdW == L[0].vdW
+    assert L[3].gamma_rad == L[0].gamma_rad and L[3].gamma_stark == L[2].gamma_stark
+    assert L[3].vdW[0] == pytest.approx(9.953360714197118e-8, rel=1e-14)
+    assert L[4].gamma_rad == L[0].gamma_rad and L[4].gamma_stark == L[0].gamma_stark
+    assert L[4].vdW == L[3].vdW
+    # ABO parameters
+    assert L[5].vdW[0] == pytest.approx(1.3917417470792187e-14, rel=1e-14)
+    assert L[5].vdW[1] == pytest.approx(0.227, rel=1e-12)
+
+
+def test_vald_various_formats():
+    """test/linelist.jl:225-247: short/long, extract all/stellar, air wavenumbers, quotes."""
+    from korg_line.linelist import read_vald_linelist
+    rd = lambda f: read_vald_linelist(os.path.join(VALD, f)).to_lines()
+    short_all = rd("short-extract-all.vald")
+    long_air = rd("long-extract-all-air-wavenumber.vald")
+    long_air_nq = rd("long-extract-all-air-wavenumber-noquotes.vald")
+    short_stellar = rd("short-extract-stellar.vald")
+    long_stellar = rd("long-extract-stellar.vald")
+    assert len(short_all) == len(short_stellar) == len(long_air) == len(long_air_nq) == \
+        len(long_stellar) == 2
+    key = lambda l: (l.wl, l.log_gf, l.species, l.E_lower, l.gamma_rad, l.gamma_stark, l.vdW)
+    assert key(long_air[0]) == key(long_air_nq[0])
+    assert key(short_all[0]) == key(short_stellar[0]) == key(long_stellar[0])
+    # with a unit conversion (air wavelengths, cm^-1 energies): approximately equal
+    assert short_all[0].wl == pytest.approx(long_air[0].wl, rel=1e-7)
+    assert short_all[0].log_gf == long_air[0].log_gf and short_all[0].species == long_air[0].species
+    assert short_all[0].E_lower == pytest.approx(long_air[0].E_lower, abs=1e-3)
+    assert short_all[0].gamma_rad == long_air[0].gamma_rad
+    assert short_all[0].gamma_stark == long_air[0].gamma_stark and short_all[0].vdW == long_air[0].vdW
+
+
+def test_air_to_vacuum_against_the_published_table_value():
+    """Birch & Downs: n(air) ~ 1.000278 near 5000 A; vacuum_to_air(air_to_vacuum) needs none."""
+    from korg_line.linelist import air_to_vacuum
+    v = air_to_vacuum(5000.0)
+    assert 1.00027 < v / 5000.0 < 1.00029
+    assert air_to_vacuum(5000e-8) == pytest.approx(v * 1e-8, rel=1e-15)
