@@ -2802,11 +2802,13 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
     double acc[HCT];
 #pragma unroll
     for (int q = 0; q < HCT; ++q) acc[q] = 0.0;
+    bool touched = false;  // wavefront-uniform: some transition reaches the tile
     for (int t = 0; t < A.n_trans; ++t) {
         const int64_t p = (int64_t)(lv ? l : 0) * A.n_trans + t;
         const HPair h = A.pairs[p];
         const bool ov = lv && h.lb <= h.ub && h.lb < j0 + jn && h.ub >= j0;
         if (!ballot(ov)) continue;  // wavefront-uniform
+        touched = true;
         const HTrans tr = A.trans[t];
         const double* kn = A.kdnu + tr.off_dnu;  // kn[0] stands for -floatmax
         const int nd = tr.n_dnu;
@@ -2840,7 +2842,7 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
             acc[q] += exp(v) * (c_cgs / (lam * lam)) * h.amp;
         }
     }
-    if (!lv) return;
+    if (!lv || !touched) return;  // a tile no transition reaches leaves alpha as it is
     for (int q = 0; q < jn; ++q) {
         const int64_t k = (int64_t)l + (int64_t)A.L * (j0 + q);
         if (A.alpha_f32) ((float*)A.alpha)[k] += (float)acc[q];
@@ -4634,6 +4636,7 @@ int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* t) {
         max_dnu = std::max(max_dnu, x.n_dnu);
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->hl_ntrans = 0;  // until the upload completes: a failed upload leaves no tables set
     struct {
         DevBuf* b;
         const void* src;
