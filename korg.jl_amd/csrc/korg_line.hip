@@ -2671,6 +2671,7 @@ struct HArgs {
     const double *kT, *kne, *kdnu, *logP, *lam0;
     HPair* pairs;
     double* slices;  // [pair][max_dnu]
+    int* hull;       // [transition][2]: the union of its windows over the layers (lb > ub: none)
     void* alpha;
     int alpha_f32;
 };
@@ -2715,6 +2716,14 @@ __device__ inline int64_t upper_bound_f64(const double* a, int64_t n, double x) 
         else hi = m;
     }
     return lo;
+}
+
+__global__ __launch_bounds__(64) void k_hinit(HArgs A) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t < A.n_trans) {
+        A.hull[2 * t] = INT_MAX;
+        A.hull[2 * t + 1] = -1;
+    }
 }
 
 __global__ __launch_bounds__(64) void k_hsetup(HArgs A) {
@@ -2764,6 +2773,8 @@ __global__ __launch_bounds__(64) void k_hsetup(HArgs A) {
     h.lb = (int32_t)lb;
     h.ub = (int32_t)ub;
     h.amp = amp;
+    atomicMin(&A.hull[2 * t], (int)lb);
+    atomicMax(&A.hull[2 * t + 1], (int)ub);
     // :120-136: H alpha, beta, gamma also get the Barklem et al. resonance profile, centred on
     // fixed wavelengths (which then also centre the Stark profile)
     if (tr.lower == 2 && tr.upper >= 3 && tr.upper <= 5) {
@@ -2804,6 +2815,8 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
     for (int q = 0; q < HCT; ++q) acc[q] = 0.0;
     bool touched = false;  // wavefront-uniform: some transition reaches the tile
     for (int t = 0; t < A.n_trans; ++t) {
+        // the transition's windows over all layers miss the tile: skip it (uniform loads)
+        if (A.hull[2 * t] > j0 + jn - 1 || A.hull[2 * t + 1] < j0) continue;
         const int64_t p = (int64_t)(lv ? l : 0) * A.n_trans + t;
         const HPair h = A.pairs[p];
         const bool ov = lv && h.lb <= h.ub && h.lb < j0 + jn && h.ub >= j0;
@@ -2917,7 +2930,7 @@ struct korg_ctx {
     DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
         h_cntm;
     // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
-    DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices;
+    DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices, hl_hull;
     int hl_ntrans = 0, hl_max_dnu = 0;
 };
 
@@ -4278,7 +4291,8 @@ void korg_destroy(korg_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->vjp_out,  &ctx->h_grad,   &ctx->h_alpha,   &ctx->h_wls,
                       &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm,
                       &ctx->hl_trans, &ctx->hl_T,     &ctx->hl_ne,     &ctx->hl_dnu,
-                      &ctx->hl_logP,  &ctx->hl_lam0,  &ctx->hl_pairs,  &ctx->hl_slices};
+                      &ctx->hl_logP,  &ctx->hl_lam0,  &ctx->hl_pairs,  &ctx->hl_slices,
+                      &ctx->hl_hull};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
@@ -4688,6 +4702,7 @@ int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, c
     const int L = layers->n_layers, NT = ctx->hl_ntrans;
     TRY(grow(ctx, ctx->hl_pairs, (size_t)L * NT * sizeof(HPair)));
     TRY(grow(ctx, ctx->hl_slices, (size_t)L * NT * ctx->hl_max_dnu * sizeof(double)));
+    TRY(grow(ctx, ctx->hl_hull, (size_t)NT * 2 * sizeof(int)));
     HArgs a;
     a.L = L;
     a.n_trans = NT;
@@ -4710,8 +4725,10 @@ int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, c
     a.lam0 = (const double*)ctx->hl_lam0.p;
     a.pairs = (HPair*)ctx->hl_pairs.p;
     a.slices = (double*)ctx->hl_slices.p;
+    a.hull = (int*)ctx->hl_hull.p;
     a.alpha = alpha;
     a.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+    hipLaunchKernelGGL(k_hinit, dim3((NT + 63) / 64), dim3(64), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_hsetup, dim3((L * NT + 63) / 64), dim3(64), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_hlines, dim3((unsigned)((grid->n_wl + HCT - 1) / HCT), (L + 63) / 64), dim3(64), 0,
                        ctx->stream, a);
