@@ -399,7 +399,8 @@ def read_vald_linelist(path):
     says so), gamma_rad 0 imputed by Korg's default, gamma_stark as log10 and vdW as given (0 =
     missing).  Files whose oscillator strengths VALD did not scale by isotopic abundance are read
     only if no isotope is named in their references (the isotopic-abundance table is not part of
-    this library).  Returns a ``LineList`` in file order.
+    this library).  As ``read_linelist``, lines of charge outside 0..2 and H I lines are dropped
+    and the list is sorted by wavelength.
     """
     import csv
     import re
@@ -451,6 +452,67 @@ def read_vald_linelist(path):
             index[sp] = len(species)
             species.append(sp)
         sidx[k] = index[sp]
-    return LineList.from_arrays(wl, loggf, species, sidx, E_low, gamma_rad=gamma_rad,
-                                gamma_stark=_tentothe_or_missing([num(r, "gamma_stark") for r in rows]),
-                                vdW=_id_or_missing([num(r, "gamma_vdW") for r in rows]))
+    ll = LineList.from_arrays(wl, loggf, species, sidx, E_low, gamma_rad=gamma_rad,
+                              gamma_stark=_tentothe_or_missing([num(r, "gamma_stark") for r in rows]),
+                              vdW=_id_or_missing([num(r, "gamma_vdW") for r in rows]))
+    return _finish_text_linelist(ll)
+
+
+def _finish_text_linelist(ll):
+    """``read_linelist``'s last steps for every text format (``src/linelist.jl:416-424``): lines of
+    charge outside 0..2 and H I lines are dropped, and the list is sorted by wavelength (stable)."""
+    ok = np.array([0 <= sp.charge <= 2 and sp != Species("H I") for sp in ll.species], dtype=bool)
+    ll = ll.take(np.flatnonzero(ok[ll.species_index])) if ll.species else ll
+    return ll if ll.is_sorted() else ll.sorted()
+
+
+def _or_zero(s):
+    """``parse_or_zero`` (``src/linelist.jl:442-449``): a blank field reads as 0."""
+    s = s.strip()
+    return float(s) if s else 0.0
+
+
+def read_kurucz_linelist(path, vacuum=False):
+    """An atomic Kurucz linelist (160-column rows) as ``read_linelist(...; format="kurucz")`` reads
+    it (``parse_kurucz_linelist``, ``src/linelist.jl:451-511``, then the common filter and sort):
+    wavelengths in nm (air, converted to vacuum unless ``vacuum``), E_lower the smaller of the two
+    levels' |wavenumber|, log gf plus the hyperfine column, gamma_rad and gamma_stark as log10 and
+    vdW as given (0 = missing).  No isotopic-abundance table is part of this library, so Kurucz's
+    own isotope column is added to log gf (Korg's ``isotopic_abundances = nothing`` path; with its
+    default table Korg recomputes that adjustment for lines that name an isotope).
+    Molecular Kurucz files (first row of 100 columns or fewer) are not supported.
+    """
+    with open(path) as f:
+        rows = [r.rstrip("\n") for r in f]
+    first = next((r for r in rows if r.strip()), "")
+    if len(first) <= 100:
+        raise ValueError(f"{path}: molecular Kurucz linelists are not supported")
+    wl, loggf, E, grad, gstark, vdw, names = [], [], [], [], [], [], []
+    for row in rows:
+        if row == "":
+            continue
+        if len(row) == 159:  # a missing column in the wavelength field
+            row = " " + row
+        row = row.ljust(160)
+        e1 = abs(float(row[24:36])) * c_cgs * hplanck_eV
+        e2 = abs(float(row[52:64])) * c_cgs * hplanck_eV
+        E.append(min(e1, e2))
+        names.append(row[18:24])
+        loggf.append(float(row[11:18]) + _or_zero(row[109:115]) + _or_zero(row[118:124]))
+        lam = float(row[0:11]) * 1e-7
+        wl.append(lam if vacuum else air_to_vacuum(lam))
+        grad.append(_or_zero(row[80:86]))
+        gstark.append(_or_zero(row[86:92]))
+        vdw.append(_or_zero(row[92:98]))
+    species, index = [], {}
+    sidx = np.empty(len(names), dtype=np.int32)
+    for k, nm in enumerate(names):
+        sp = Species(nm)
+        if sp not in index:
+            index[sp] = len(species)
+            species.append(sp)
+        sidx[k] = index[sp]
+    ll = LineList.from_arrays(np.array(wl), np.array(loggf), species, sidx, np.array(E),
+                              gamma_rad=_tentothe_or_missing(grad),
+                              gamma_stark=_tentothe_or_missing(gstark), vdW=_id_or_missing(vdw))
+    return _finish_text_linelist(ll)

@@ -237,3 +237,34 @@ def test_air_to_vacuum_against_the_published_table_value():
     v = air_to_vacuum(5000.0)
     assert 1.00027 < v / 5000.0 < 1.00029
     assert air_to_vacuum(5000e-8) == pytest.approx(v * 1e-8, rel=1e-15)
+
+
+# ------------------------------------------------------------------------------------ Kurucz
+KURUCZ = os.path.join(HERE, "golden", "kurucz")  # the reference's test files (test/data/linelists)
+
+
+@pytest.mark.parametrize("fname", ["gfallvac08oct17.stub.dat", "gfallvac08oct17-missing-col.stub.dat"])
+def test_kurucz_linelist_parsing(fname):
+    """test/linelist.jl:154-166, value by value."""
+    from korg_line.linelist import read_kurucz_linelist
+    ll = read_kurucz_linelist(os.path.join(KURUCZ, fname))
+    assert ll.is_sorted() and len(ll) == 987
+    L0 = ll.to_lines()[0]
+    assert L0.wl == pytest.approx(0.0007234041763337705, rel=1e-14)
+    assert L0.log_gf == -0.826 and L0.species == Species("Be_II")
+    assert L0.E_lower == pytest.approx(17.360339371573698, rel=1e-14)
+    assert L0.gamma_rad == pytest.approx(8.511380382023759e7, rel=1e-14)
+    assert L0.gamma_stark == pytest.approx(0.003890451449942805, rel=1e-14)
+    assert L0.vdW[0] == pytest.approx(1.2302687708123812e-7, rel=1e-14)
+
+
+def test_kurucz_short_lines_and_molecular():
+    """test/linelist.jl:168-189: stripped rows parse; a molecular list is refused."""
+    from korg_line.linelist import read_kurucz_linelist
+    ll = read_kurucz_linelist(os.path.join(KURUCZ, "gfallvac08oct17-short-lines.stub.dat"))
+    assert len(ll) >= 2
+    L0 = ll.to_lines()[0]
+    assert L0.wl == pytest.approx(0.0007234041763337705, rel=1e-14)
+    assert L0.log_gf == -0.826 and L0.species == Species("Be_II")
+    with pytest.raises(ValueError):
+        read_kurucz_linelist(os.path.join(KURUCZ, "kurucz_cn.txt"))
