@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 9
+#define KORG_LINE_ABI_VERSION 10
 
 enum korg_status {
     KORG_OK = 0,
@@ -166,6 +166,33 @@ int korg_line_absorption_device(korg_ctx* ctx, void* alpha, int alpha_dtype,
                                 const korg_species* species, const korg_layers* layers,
                                 const korg_continuum* cntm, double cutoff_threshold,
                                 int precision, int synchronize);
+
+/*
+ * korg_line_absorption_device with no device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results
+ * (bitwise) as korg_line_absorption_device.
+ *
+ * Its scratch is a fixed device array of KORG_NOALLOC_SCRATCH_BYTES per GPU, held from the
+ * library's load on that GPU until the process ends; korg_create resolves its address.  The call
+ * carves the pipeline's buffers from it, makes no host round trip before returning (synchronize
+ * = 0 leaves everything enqueued) and never calls hipMalloc/hipFree.  Per line it needs about
+ * 32 B x (n_layers rounded up to 8) + 1 KB per 64 layers (Float32) + 0.2 KB, plus about 4 B per grid
+ * point and 4 B per (line, 64-point tile) pair: for 56 layers about 20,000 lines on 100,000 points
+ * in Float32.  A problem that does not fit returns KORG_ERR_CAPACITY with alpha unchanged (checked
+ * on the host before any launch, or for the per-tile lists on the device, in which case no kernel
+ * writes alpha).
+ *
+ * Every call on one GPU, from any context, shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap in time on one GPU (other streams, other contexts, other host
+ * threads) race and give wrong results.  Calls on one context's stream are ordered and safe.
+ */
+#define KORG_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int korg_line_absorption_device_noalloc(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                        const korg_grid* grid, const korg_lines* lines,
+                                        const korg_species* species, const korg_layers* layers,
+                                        const korg_continuum* cntm, double cutoff_threshold,
+                                        int precision, int synchronize);
 
 /*
  * One star -- one line_absorption! call -- of a batch (device pointers throughout).  Its line

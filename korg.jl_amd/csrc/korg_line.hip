@@ -41,6 +41,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2868,9 +2869,24 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
 // ============================================================================================
 // host side
 // ============================================================================================
+
+// Scratch of korg_line_absorption_device_noalloc: one fixed array per GPU, held from the library's
+// load on that device until the process ends.  Host code never passes this name to a kernel (on
+// the host it is only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns for the current device (korg_create resolves and keeps it).
+static __device__ __align__(256) unsigned char scratch[KORG_NOALLOC_SCRATCH_BYTES];
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+};
+
+// Bump allocation from a fixed device block: while a context's `arena` is set, grow() carves
+// buffers from it instead of calling hipMalloc (and frees nothing).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t off = 0;
 };
 
 // Device scratch of the forward pipeline (grow-only).  The context owns one for its direct calls;
@@ -2886,6 +2902,16 @@ struct Workspace {
     int64_t tile_cap = 0;             // entries tile_lines holds
     int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
+    // Drops every buffer without freeing it (a workspace carved from an Arena).
+    void forget() {
+        DevBuf* bufs[] = {&meta,     &rec64,     &recM,      &tab,        &counters,  &blk,
+                          &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
+                          &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
+                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part};
+        for (DevBuf* b : bufs) *b = DevBuf{};
+        tile_cap = 0;
+        seg_cap = 0;
+    }
     void release() {
         DevBuf* bufs[] = {&meta,     &rec64,     &recM,      &tab,        &counters,  &blk,
                           &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
@@ -2923,6 +2949,14 @@ struct korg_ctx {
     int last_nb = 0;  // 32-point blocks per far-wing tile of the last call
     bool last_sized = false;  // the last forward call sized the tile lists on the host (ev[5], ev[6])
     Workspace ws;
+    // the allocation-free entry point: its workspace (carved from the device's fixed scratch at
+    // every call), the scratch's device address on this context's device, and whether the last
+    // forward call used it (korg_synchronize, korg_get_windows then read ws_fixed)
+    Workspace ws_fixed;
+    void* scratch_ptr = nullptr;
+    std::string scratch_err;
+    Arena* arena = nullptr;
+    bool last_fixed = false;
     std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
     // adjoint outputs of the host-memory entry point, and its staged G
     DevBuf vjp_out, h_grad;
@@ -2956,6 +2990,17 @@ int set_err(korg_ctx* ctx, int code, const char* fmt, ...) {
 int grow(korg_ctx* ctx, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return KORG_OK;
+    if (ctx->arena) {  // carve from the fixed block; nothing is allocated or freed
+        Arena& a = *ctx->arena;
+        const size_t off = (a.off + 255) & ~(size_t)255;
+        if (off > a.cap || bytes > a.cap - off)
+            return set_err(ctx, KORG_ERR_CAPACITY, "scratch exhausted (%zu of %zu bytes in use, "
+                           "%zu more needed)", a.off, a.cap, bytes);
+        b.p = a.base + off;
+        b.cap = bytes;
+        a.off = off + bytes;
+        return KORG_OK;
+    }
     if (b.p) {
         HIP_TRY(ctx, hipFree(b.p));
         b.p = nullptr;
@@ -3414,27 +3459,85 @@ int size_tile_lists(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s,
     return KORG_OK;
 }
 
+// Device address of `scratch` on each device, resolved once per device with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+// Resolves the fixed scratch of ctx's device (ctx->device must be current).  A failure is kept in
+// ctx->scratch_err and reported by the allocation-free entry point only.
+void resolve_scratch(korg_ctx* ctx) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= ctx->device) g_scratch_ptr.resize(ctx->device + 1, nullptr);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            ctx->scratch_err = std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e);
+            return;
+        }
+        g_scratch_ptr[ctx->device] = p;
+    }
+    ctx->scratch_ptr = g_scratch_ptr[ctx->device];
+}
+
+// Clears a context's arena when it goes out of scope.
+struct ArenaScope {
+    korg_ctx* ctx;
+    Arena arena;
+    ArenaScope(korg_ctx* c, void* base, size_t cap) : ctx(c) {
+        arena.base = (char*)base;
+        arena.cap = cap;
+        ctx->arena = &arena;
+    }
+    ~ArenaScope() { ctx->arena = nullptr; }
+};
+
 // The eager device pipeline over a batch of stars (every pointer a device pointer; one alpha
-// dtype and one layer count for the batch), on the context's stream and workspace.
+// dtype and one layer count for the batch), on the context's stream.
+//   fixed = false: on ctx->ws (grown with hipMalloc); the host reads the per-tile list total
+//                  between the halves and sizes the lists exactly.
+//   fixed = true:  on ctx->ws_fixed, carved from the device's fixed scratch, with no allocation
+//                  and no host round trip: the per-tile lists take what the other buffers leave of
+//                  the scratch, and when they would need more the device raises FLAG_CAPACITY and
+//                  no kernel writes alpha (reported by finish_stats).
 int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_species* sp, int L,
-              int alpha_dtype, int precision) {
+              int alpha_dtype, int precision, bool fixed = false) {
     hipStream_t s = ctx->stream;
+    Workspace& ws = fixed ? ctx->ws_fixed : ctx->ws;
     Batch b;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     ctx->last_n_lines = 0;
-    TRY(plan_batch(ctx, ctx->tuning, n_stars, stars, sp, L, alpha_dtype, precision, ctx->ws, s, b));
+    ctx->last_fixed = fixed;
+    if (!fixed) {
+        TRY(plan_batch(ctx, ctx->tuning, n_stars, stars, sp, L, alpha_dtype, precision, ws, s, b));
+    } else {
+        ws.forget();
+        ArenaScope as(ctx, ctx->scratch_ptr, (size_t)KORG_NOALLOC_SCRATCH_BYTES);
+        TRY(plan_batch(ctx, ctx->tuning, n_stars, stars, sp, L, alpha_dtype, precision, ws, s, b));
+        if (b.N > 0 && b.f32) {
+            const size_t off = (as.arena.off + 255) & ~(size_t)255;
+            const int64_t cap = std::min<int64_t>(
+                off < as.arena.cap ? (int64_t)((as.arena.cap - off) / sizeof(int)) : 0, INT32_MAX);
+            if (cap < 1)
+                return set_err(ctx, KORG_ERR_CAPACITY, "scratch exhausted before the tile lists");
+            TRY(grow(ctx, ws.tile_lines, sizeof(int) * (size_t)cap));
+            ws.tile_cap = cap;
+        }
+    }
     ctx->last_n_lines = b.N;
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
     ctx->last_nb = b.NB;
-    ctx->last_sized = b.f32;
+    ctx->last_sized = b.f32 && !fixed;
     if (b.N == 0) return KORG_OK;
-    TRY(enqueue_front(ctx, ctx->ws, b, s, INT64_MAX, true));
-    if (b.f32) {
+    TRY(enqueue_front(ctx, ws, b, s, fixed ? ws.tile_cap : INT64_MAX, true));
+    if (b.f32 && !fixed) {
         int64_t total = 0;
-        TRY(size_tile_lists(ctx, ctx->ws, b, s, 1.0, &total));
+        TRY(size_tile_lists(ctx, ws, b, s, 1.0, &total));
     }
-    return enqueue_back(ctx, ctx->ws, b, s, true);
+    return enqueue_back(ctx, ws, b, s, true);
 }
 
 // One line_absorption! call on device pointers: a batch of one star.
@@ -3482,9 +3585,17 @@ int finish_stats(korg_ctx* ctx) {
     if (ctx->last_n_lines == 0) return KORG_OK;
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
     unsigned long long c[N_COUNTERS];
-    HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
-    TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
-                            ctx->last_n_lblocks, ctx->last_nb));
+    const Workspace& ws = ctx->last_fixed ? ctx->ws_fixed : ctx->ws;
+    HIP_TRY(ctx, hipMemcpy(c, ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    const int rc = stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
+                                       ctx->last_n_lblocks, ctx->last_nb);
+    if (rc == KORG_ERR_CAPACITY && ctx->last_fixed)
+        return set_err(ctx, KORG_ERR_CAPACITY,
+                       "the per-tile line lists need %llu entries, more than the %lld the fixed "
+                       "scratch (%lld bytes) has left for them; alpha was left unchanged",
+                       c[13], (long long)ctx->ws_fixed.tile_cap,
+                       (long long)KORG_NOALLOC_SCRATCH_BYTES);
+    TRY(rc);
     // device time: the host's sizing of the tile lists (ev[5] -> ev[6]) is left out
     const float gap = ctx->last_sized ? elapsed(ctx, 5, 6) : 0.0f;
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
@@ -3586,6 +3697,7 @@ int run_vjp(korg_ctx* ctx, const void* G, int g_dtype, const korg_grid* g, const
     const int L = ly->n_layers;
     hipStream_t s = ctx->stream;
     ctx->last_n_lines = N;
+    ctx->last_fixed = false;
     ctx->last_precision = KORG_PRECISION_FP64;
     ctx->last_n_lblocks = 0;
     ctx->last_sized = false;
@@ -3876,6 +3988,7 @@ int run_fit(korg_ctx* ctx, std::vector<StarHost>& hs, const korg_species* sp, in
     TRY(plan_batch(ctx, ctx->tuning, (int)hs.size(), hs.data(), sp, L, alpha_dtype, precision, ws,
                    s, b));
     ctx->last_n_lines = b.N;
+    ctx->last_fixed = false;
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
     ctx->last_nb = b.NB;
@@ -4279,6 +4392,12 @@ int korg_create(korg_ctx** out, int device) {
             return KORG_ERR_HIP;
         }
     }
+    // set-up of the allocation-free entry point: the fixed scratch's address on this device, and
+    // host-side room for a one-star table, so that its calls allocate nothing
+    resolve_scratch(ctx);
+    ctx->ws_fixed.stars_host.reserve(1);
+    ctx->last_grid_off.reserve(1);
+    ctx->last_line_off.reserve(1);
     *out = ctx;
     return KORG_OK;
 }
@@ -4349,6 +4468,35 @@ int korg_line_absorption_device(korg_ctx* ctx, void* alpha, int alpha_dtype, con
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     TRY(run_device(ctx, alpha, alpha_dtype, grid, lines, species, layers, cntm, cutoff_threshold,
                    precision));
+    if (synchronize) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_stats(ctx));
+    }
+    return KORG_OK;
+}
+
+int korg_line_absorption_device_noalloc(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                        const korg_grid* grid, const korg_lines* lines,
+                                        const korg_species* species, const korg_layers* layers,
+                                        const korg_continuum* cntm, double cutoff_threshold,
+                                        int precision, int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(validate(ctx, grid, lines, species, layers, cntm, alpha_dtype, precision));
+    if (!alpha && lines->n_lines > 0) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha is null");
+    if (!ctx->scratch_ptr)
+        return set_err(ctx, KORG_ERR_HIP, "no fixed scratch on device %d: %s", ctx->device,
+                       ctx->scratch_err.c_str());
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const StarHost one = {alpha, *grid, *lines, *layers, *cntm, cutoff_threshold};
+    const int rc = run_batch(ctx, 1, &one, species, layers->n_layers, alpha_dtype, precision, true);
+    if (rc == KORG_ERR_CAPACITY)
+        return set_err(ctx, KORG_ERR_CAPACITY,
+                       "%lld lines x %d layers on %lld points need more than the fixed scratch of "
+                       "%lld bytes (%s); use korg_line_absorption_device",
+                       (long long)lines->n_lines, layers->n_layers, (long long)grid->n_wl,
+                       (long long)KORG_NOALLOC_SCRATCH_BYTES, ctx->err.c_str());
+    TRY(rc);
     if (synchronize) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         TRY(finish_stats(ctx));
@@ -4482,7 +4630,8 @@ int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {
     if (n_lines == 0) return KORG_OK;
     std::vector<LineMeta> m(n_lines);
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipMemcpy(m.data(), ctx->ws.meta.p, sizeof(LineMeta) * n_lines,
+    const Workspace& ws = ctx->last_fixed ? ctx->ws_fixed : ctx->ws;
+    HIP_TRY(ctx, hipMemcpy(m.data(), ws.meta.p, sizeof(LineMeta) * n_lines,
                            hipMemcpyDeviceToHost));
     // grid indices are global over the call's stars: make them 1-based within each line's star
     const auto& lo = ctx->last_line_off;

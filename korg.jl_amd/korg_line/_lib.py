@@ -17,6 +17,7 @@ KORG_ERR_OUT_OF_MEMORY = 3
 KORG_ERR_NO_DEVICE = 4
 KORG_ERR_UNSUPPORTED = 5
 KORG_ERR_CAPACITY = 6
+NOALLOC_SCRATCH_BYTES = 64 << 20  # KORG_NOALLOC_SCRATCH_BYTES
 
 PRECISION_FP64 = 0
 PRECISION_FP32_FIXUP = 1
@@ -118,7 +119,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 9  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 10  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -128,7 +129,7 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_plan_launch", "korg_plan_wait", "korg_plan_get_info", "korg_plan_destroy",
             "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit",
             "korg_continuum_batched", "korg_flux_batched", "korg_set_hydrogen_tables",
-            "korg_hydrogen_line_absorption"]
+            "korg_hydrogen_line_absorption", "korg_line_absorption_device_noalloc"]
 
 _lib = None
 
@@ -164,6 +165,8 @@ def load():
     L.korg_line_absorption.restype = i32
     L.korg_line_absorption_device.argtypes = common + [i32]
     L.korg_line_absorption_device.restype = i32
+    L.korg_line_absorption_device_noalloc.argtypes = common + [i32]
+    L.korg_line_absorption_device_noalloc.restype = i32
     L.korg_set_stream.argtypes = [vp, vp]
     L.korg_set_stream.restype = i32
     L.korg_synchronize.argtypes = [vp]

@@ -259,6 +259,21 @@ class Context:
             1 if synchronize else 0)
         _lib.check(self._h, rc)
 
+    def line_absorption_device_noalloc(self, alpha_dev, dprob, precision="fp32",
+                                       synchronize=True):
+        """``line_absorption_device`` through the entry point that allocates no device memory
+        (its scratch is the library's fixed per-GPU array of ``_lib.NOALLOC_SCRATCH_BYTES``)."""
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        P = dprob.problem
+        if tuple(alpha_dev.shape) != (P.n_wl, P.n_layers) or alpha_dev.dtype not in dt:
+            raise ValueError("alpha_dev must be a (n_wl, n_layers) float32/float64 device array")
+        rc = self._L.korg_line_absorption_device_noalloc(
+            self._h, ctypes.c_void_p(alpha_dev.ptr), dt[alpha_dev.dtype], ctypes.byref(dprob.grid),
+            ctypes.byref(dprob.lines), ctypes.byref(dprob.species), ctypes.byref(dprob.layers),
+            ctypes.byref(dprob.cntm), P.cutoff_threshold, PRECISIONS[precision],
+            1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
     @staticmethod
     def _stars(alphas, dprobs):
         """The korg_star array of a batch, and its alpha dtype code."""
