@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 10
+#define KORG_LINE_ABI_VERSION 11
 
 enum korg_status {
     KORG_OK = 0,
@@ -76,6 +76,14 @@ typedef struct {
     const double* vdW1;        /* gamma_vdW at 10^4 K if vdW2 == -1, else ABO sigma */
     const double* vdW2;        /* -1, or ABO alpha */
     const int32_t* species;    /* index into korg_species */
+    /* Optional frozen windows, 1-based as korg_get_windows returns them (lb > ub skips the line):
+     * when both are given they replace the windows the call would compute
+     * (src/line_absorption.jl:108-119), so that alpha is a smooth function of the per-layer
+     * inputs -- as under ForwardDiff, whose duals carry no derivative through the integer
+     * window -- for finite-difference checks and fits that hold the windows of an iterate.
+     * Both NULL: computed, as the reference does. */
+    const int64_t* window_lb;
+    const int64_t* window_ub;
 } korg_lines;
 
 /* Species of the lines: mass (get_mass, src/species.jl:175) and ismolecule (:168). */
@@ -280,6 +288,45 @@ int korg_line_absorption_vjp_device(korg_ctx* ctx, const void* grad_alpha, int g
                                     const korg_species* species, const korg_layers* layers,
                                     const korg_continuum* cntm, double cutoff_threshold,
                                     korg_layer_grads* grads, int synchronize);
+
+/*
+ * Tangents of the per-layer inputs: one direction of a forward-mode derivative.  Members as
+ * korg_layer_grads (n_div_U [n_species * n_layers]); a NULL member is a zero tangent.
+ */
+typedef struct {
+    const double* T;
+    const double* n_e;
+    const double* xi;
+    const double* n_HI;
+    const double* n_HeI;
+    const double* n_H2;
+    const double* n_div_U;
+} korg_layer_tangents;
+
+/*
+ * Jacobian-vector products of line_absorption! (src/line_absorption.jl:33-129): for each of
+ * n_tangents directions x_k of the per-layer inputs, ADDS J x_k -- the derivative of alpha along
+ * x_k -- into dalpha[k] (column-major [n_layers x n_wl], dtype korg_dtype).  This is what a
+ * ForwardDiff.Dual alpha receives in its partials when Korg's Fit differentiates synthesize
+ * (src/Fit/fit_via_synthesis.jl:345-355 with autodiff=:forwarddiff; src/synthesize.jl:242-247,
+ * 305-306): one direction per partial.  As ForwardDiff does, the windows are held at the values'
+ * windows (their dependence on the inputs is integer) and the branch of voigt_hjerting in use is
+ * differentiated exactly.  Float64 throughout; deterministic.  A zero tangent component adds
+ * nothing.  The host form takes host pointers (dalpha: a host array of n_tangents host
+ * pointers) and is synchronous; the _device form takes device pointers (dalpha: a host array of
+ * device pointers; tangents: a host array of structs of device pointers).
+ */
+int korg_line_absorption_jvp(korg_ctx* ctx, void* const* dalpha, int dalpha_dtype,
+                             const korg_grid* grid, const korg_lines* lines,
+                             const korg_species* species, const korg_layers* layers,
+                             const korg_continuum* cntm, double cutoff_threshold,
+                             int64_t n_tangents, const korg_layer_tangents* tangents);
+int korg_line_absorption_jvp_device(korg_ctx* ctx, void* const* dalpha, int dalpha_dtype,
+                                    const korg_grid* grid, const korg_lines* lines,
+                                    const korg_species* species, const korg_layers* layers,
+                                    const korg_continuum* cntm, double cutoff_threshold,
+                                    int64_t n_tangents, const korg_layer_tangents* tangents,
+                                    int synchronize);
 
 /*
  * Plans: a batched forward call (korg_line_absorption_batched) captured once in a HIP graph for

@@ -226,6 +226,7 @@ struct StarDev {
     void* alpha;  // column-major [L x n_wl]
     const double *wl, *log_gf, *E_lower, *gamma_rad, *gamma_stark, *vdW1, *vdW2;
     const int32_t* species;
+    const int64_t *win_lb, *win_ub;  // frozen 1-based windows (null: computed)
     const double *temps, *n_e, *n_div_U, *n_HI, *n_HeI, *n_H2, *xi;
     int32_t cntm_mode, pad0;
     int64_t n_knots;
@@ -546,9 +547,15 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int64_t r = wave_bounds(S.wls, n, gi.g0, gi.ginv, x, upper, lane < nsearch && !past, lane);
     int64_t res = r;
     if (lane < nsearch && past) res = n;  // lower bound n; upper bound n (last index n - 1)
-    const int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
-    const int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
+    int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
+    int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
     const int64_t jc = __builtin_amdgcn_readlane((int)res, 2);
+    if (S.win_lb) {  // frozen windows (1-based), clipped to the grid; zones are clipped to them
+        lb = S.win_lb[li] - 1;
+        ub = S.win_ub[li] - 1;
+        lb = lb < 0 ? 0 : lb;
+        ub = ub > n - 1 ? n - 1 : ub;
+    }
     int64_t clo = 1, chi = 0;
     if (lb <= ub && reach >= 0.0) {
         clo = __builtin_amdgcn_readlane((int)res, 3);
@@ -1788,6 +1795,81 @@ __global__ void k_vjp_segs(int n_stars, const StarDev* __restrict__ stars,
     seg_cnt[s] = (int)((n + VJP_SEG - 1) / VJP_SEG);
 }
 
+// Partial derivatives of one (line, layer)'s profile parameters -- line_profile is
+// scaling * H(a, v) with inv = 1/(sigma sqrt 2), a = gamma inv, v = |lambda - lambda0| inv and
+// scaling = inv/sqrt(pi) * amplitude (src/line_absorption.jl:245-249) -- with respect to the
+// layer's inputs x = (T, n_e, xi, n_HI, n_HeI, n_H2, n/U of the line's species): da[q] = da/dx_q,
+// dli[q] = d ln(inv)/dx_q, ds[q] = dscaling/dx_q (src/line_absorption.jl:86-106, 190).
+struct LayerPartials {
+    double da[7], dli[7], ds[7];
+};
+
+struct LinePartialsIn {  // the line's own constants
+    double mass, lam0, E_lower, E_up, gf, sl, gscale, gamma_stark, vdW1, vdW2;
+    bool mol;
+};
+
+__device__ inline LayerPartials layer_partials(const LinePartialsIn& ln, const LayerTab& tb,
+                                               double T, double xi, double n_e, double nU,
+                                               double inv, double a, double scaling) {
+    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
+    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;  // as k_layers
+    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
+    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+    // d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 (src/line_absorption.jl:190)
+    const double q = kboltz_cgs * T / ln.mass + xi * xi / 2.0;
+    const double dli_dT = -(kboltz_cgs / ln.mass) / (2.0 * q);
+    const double dli_dxi = -xi / (2.0 * q);
+    // dGamma/dx (src/line_absorption.jl:90-99)
+    double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
+    if (!ln.mol) {
+        const double stk = ln.gamma_stark * tb.t16;
+        vdw = (ln.vdW2 == -1.0) ? ln.vdW1 * tb.t03 : scaled_vdW_abo(ln.vdW1, ln.vdW2, ln.mass, T);
+        const double dvdw_dT = (ln.vdW2 == -1.0 ? 0.3 : (1.0 - ln.vdW2) / 2.0) * vdw / T;
+        dG_dne = stk;
+        dG_dT = n_e * stk / (6.0 * T) + tb.n_eff * dvdw_dT;
+    }
+    const double ig = inv * ln.gscale;
+    // scaling = inv / sqrt(pi) * gf * sigma_line * levels * n/U (src/line_absorption.jl:101-106)
+    const double el = exp(-tb.beta * ln.E_lower), eu = exp(-tb.beta * ln.E_up);
+    const double dlev_dT = (tb.beta / T) * (ln.E_lower * el - ln.E_up * eu);
+    const double base = inv * INV_SQRT_PI * ln.gf * ln.sl;
+    LayerPartials o;
+    o.da[0] = ig * dG_dT + a * dli_dT;
+    o.da[1] = ig * dG_dne;
+    o.da[2] = a * dli_dxi;
+    o.da[3] = ig * vdw;
+    o.da[4] = ig * c_He * vdw;
+    o.da[5] = ig * c_H2 * vdw;
+    o.da[6] = 0.0;
+    for (int k = 0; k < 7; ++k) o.dli[k] = 0.0;
+    o.dli[0] = dli_dT;
+    o.dli[2] = dli_dxi;
+    for (int k = 0; k < 7; ++k) o.ds[k] = 0.0;
+    o.ds[0] = scaling * dli_dT + base * nU * dlev_dT;
+    o.ds[2] = scaling * dli_dxi;
+    o.ds[6] = base * (el - eu);
+    return o;
+}
+
+__device__ inline LinePartialsIn line_partials_in(const KCONST StarDev& S, int64_t li, double lam0,
+                                                  const double* mass, const uint8_t* is_mol) {
+    LinePartialsIn ln;
+    const int32_t s = S.species[li];
+    ln.mass = mass[s];
+    ln.mol = is_mol[s] != 0;
+    ln.lam0 = lam0;
+    ln.E_lower = S.E_lower[li];
+    ln.E_up = ln.E_lower + c_cgs * hplanck_eV / lam0;  // src/line_absorption.jl:101
+    ln.gf = exp10(S.log_gf[li]);
+    ln.sl = sigma_line(lam0);
+    ln.gscale = (lam0 * lam0) / (c_cgs * (4.0 * PI));
+    ln.gamma_stark = S.gamma_stark[li];
+    ln.vdW1 = S.vdW1[li];
+    ln.vdW2 = S.vdW2[li];
+    return ln;
+}
+
 struct VjpArgs {
     int n_stars;
     int L, Ls, n_species;
@@ -1820,10 +1902,6 @@ void k_vjp(VjpArgs A) {
     const int NQ = 6 + A.n_species;
     if (A.counters[3] != 0) return;  // failed setup, or the segments do not fit
     const int64_t n_segs = A.seg_start[A.n_stars];
-    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
-    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;  // as k_layers
-    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
-    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
     for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
         // the star of this segment and its item range
         int st = 0;
@@ -1866,13 +1944,7 @@ void k_vjp(VjpArgs A) {
             const bool far = SERIES && first && !far_zone_empty(m, z);
             const double lam0 = m.lam0;
             const int32_t s = S.species[li];
-            const double mass = A.mass[s];
-            const bool mol = A.is_mol[s] != 0;
-            const double E_lower = S.E_lower[li];
-            const double E_up = E_lower + c_cgs * hplanck_eV / lam0;  // src/line_absorption.jl:101
-            const double gf = exp10(S.log_gf[li]);
-            const double sl = sigma_line(lam0);
-            const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * PI));
+            const LinePartialsIn lp = line_partials_in(S, li, lam0, A.mass, A.is_mol);
             for (int l = lane; l < L; l += WAVE) {
                 const double4 r = A.rec64[i * A.Ls + l];
                 const double inv = r.x, a = r.y, scaling = r.z;
@@ -1936,38 +2008,16 @@ void k_vjp(VjpArgs A) {
                     Sv += a * INV_SQRT_PI * (-2.0 * Q1 - 4.0 * e2 * Q2 - 6.0 * e3 * Q3 - 8.0 * e4 * Q4);
                 }
                 const LayerTab tb = A.tab[(int64_t)st * L + l];
-                const double T = S.temps[l], xi = S.xi[l];
-                // d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 (src/line_absorption.jl:190)
-                const double q = kboltz_cgs * T / mass + xi * xi / 2.0;
-                const double dli_dT = -(kboltz_cgs / mass) / (2.0 * q);
-                const double dli_dxi = -xi / (2.0 * q);
-                // dGamma/dx (src/line_absorption.jl:90-99)
-                double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
-                if (!mol) {
-                    const double stk = S.gamma_stark[li] * tb.t16;
-                    const double v1 = S.vdW1[li], v2 = S.vdW2[li];
-                    vdw = (v2 == -1.0) ? v1 * tb.t03 : scaled_vdW_abo(v1, v2, mass, T);
-                    const double dvdw_dT = (v2 == -1.0 ? 0.3 : (1.0 - v2) / 2.0) * vdw / T;
-                    dG_dne = stk;
-                    dG_dT = S.n_e[l] * stk / (6.0 * T) + tb.n_eff * dvdw_dT;
-                }
-                const double ig = inv * gscale;
-                const double da_dT = ig * dG_dT + a * dli_dT;
-                const double da_dxi = a * dli_dxi;
-                // scaling = inv / sqrt(pi) * gf * sigma_line * levels * n/U (src/line_absorption.jl:101-106)
-                const double nU = S.n_div_U[(int64_t)s * L + l];
-                const double el = exp(-tb.beta * E_lower), eu = exp(-tb.beta * E_up);
-                const double dlev_dT = (tb.beta / T) * (E_lower * el - E_up * eu);
-                const double base = inv * INV_SQRT_PI * gf * sl;
-                const double ds_dT = scaling * dli_dT + base * nU * dlev_dT;
-                const double ds_dxi = scaling * dli_dxi;
-                part[0 * L + l] += scaling * (da_dT * Sa + dli_dT * Sv) + ds_dT * SH;
-                part[1 * L + l] += scaling * (ig * dG_dne * Sa);
-                part[2 * L + l] += scaling * (da_dxi * Sa + dli_dxi * Sv) + ds_dxi * SH;
-                part[3 * L + l] += scaling * (ig * vdw * Sa);
-                part[4 * L + l] += scaling * (ig * c_He * vdw * Sa);
-                part[5 * L + l] += scaling * (ig * c_H2 * vdw * Sa);
-                part[(6 + s) * L + l] += base * (el - eu) * SH;
+                const LayerPartials d = layer_partials(lp, tb, S.temps[l], S.xi[l], S.n_e[l],
+                                                       S.n_div_U[(int64_t)s * L + l], inv, a,
+                                                       scaling);
+                part[0 * L + l] += scaling * (d.da[0] * Sa + d.dli[0] * Sv) + d.ds[0] * SH;
+                part[1 * L + l] += scaling * (d.da[1] * Sa);
+                part[2 * L + l] += scaling * (d.da[2] * Sa + d.dli[2] * Sv) + d.ds[2] * SH;
+                part[3 * L + l] += scaling * (d.da[3] * Sa);
+                part[4 * L + l] += scaling * (d.da[4] * Sa);
+                part[5 * L + l] += scaling * (d.da[5] * Sa);
+                part[(6 + s) * L + l] += d.ds[6] * SH;
             }
         }
     }
@@ -2000,6 +2050,144 @@ __global__ __launch_bounds__(256) void k_vjp_reduce(int L, int n_species,
         const int q = b / L, l = b % L;
         double* dst = S.gq[q < 6 ? q : 6];
         if (dst) dst[q < 6 ? l : (int64_t)(q - 6) * L + l] = red[0];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward mode (Jacobian-vector products): for up to JVP_DIRS tangent directions x_k of the
+// per-layer inputs, d alpha_k = sum over lines of (ds.x_k) H + scaling ((da.x_k) dH/da +
+// (dli.x_k) v dH/dv) over each line's window, in Float64, windows held fixed.
+// ---------------------------------------------------------------------------------------------
+constexpr int JVP_DIRS = 4;
+
+struct JvpTangent {  // device arrays of one direction (null: zero tangent)
+    const double *T, *n_e, *xi, *n_HI, *n_HeI, *n_H2, *n_div_U;
+};
+
+struct JvpArgs {
+    int n_dirs;
+    int L, Ls;
+    const StarDev* stars;
+    const LineMeta* meta;
+    const double4* rec64;
+    const LayerTab* tab;
+    const double* mass;
+    const uint8_t* is_mol;
+    JvpTangent t[JVP_DIRS];
+    double* coef;  // [(line * Ls + l) * JVP_DIRS + k][3]: coefficients of H, dH/da, v dH/dv
+    // profile pass
+    int64_t n_wl;
+    int ntiles, tile_w, n_lblocks;
+    int64_t n_items;
+    const Cand* cands;
+    const int2* ranges;
+    void* out[JVP_DIRS];  // d alpha of each direction, column-major [L x n_wl]
+    int out_f32;
+    unsigned long long* counters;
+};
+
+// One wavefront per line, lane = layer: the coefficients (ds.x, scaling da.x, scaling dli.x) of
+// each direction.  A zero tangent component contributes nothing (so a NaN partial, e.g. of a
+// missing perturber density, only matters where its input moves).
+__global__ __launch_bounds__(256) void k_jvp_coef(int64_t n_lines, JvpArgs A) {
+    const int lane = threadIdx.x % WAVE;
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (i >= n_lines) return;
+    const KCONST StarDev& S = uniform_ptr(A.stars)[0];
+    const int L = A.L;
+    const int32_t s = S.species[i];
+    const bool ok = A.counters[3] == 0;  // a failed setup: the species index may be invalid
+    LinePartialsIn lp = {};
+    if (ok) lp = line_partials_in(S, i, A.meta[i].lam0, A.mass, A.is_mol);
+    for (int l = lane; l < A.Ls; l += WAVE) {
+        double* c = A.coef + ((i * A.Ls + l) * JVP_DIRS) * 3;
+        double cf[JVP_DIRS][3] = {};
+        if (ok && l < L) {
+            const double4 r = A.rec64[i * A.Ls + l];
+            const LayerPartials d = layer_partials(lp, A.tab[l], S.temps[l], S.xi[l], S.n_e[l],
+                                                   S.n_div_U[(int64_t)s * L + l], r.x, r.y, r.z);
+            for (int k = 0; k < A.n_dirs; ++k) {
+                const JvpTangent& t = A.t[k];
+                const double x[7] = {t.T ? t.T[l] : 0.0,       t.n_e ? t.n_e[l] : 0.0,
+                                     t.xi ? t.xi[l] : 0.0,     t.n_HI ? t.n_HI[l] : 0.0,
+                                     t.n_HeI ? t.n_HeI[l] : 0.0, t.n_H2 ? t.n_H2[l] : 0.0,
+                                     t.n_div_U ? t.n_div_U[(int64_t)s * L + l] : 0.0};
+                double dsx = 0.0, dax = 0.0, dlx = 0.0;
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    if (x[q] == 0.0) continue;
+                    dsx += d.ds[q] * x[q];
+                    dax += d.da[q] * x[q];
+                    dlx += d.dli[q] * x[q];
+                }
+                cf[k][0] = dsx;
+                cf[k][1] = r.z * dax;
+                cf[k][2] = r.z * dlx;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < JVP_DIRS; ++k)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) c[k * 3 + q] = cf[k][q];
+    }
+}
+
+// The profile pass: a work item is (64-point tile, group of G layers), the lines visited in the
+// Float64 form's fixed order (window classes, partition order), one evaluation of H and its two
+// partials per (line, layer, point) shared by the directions.
+template <int G>
+__global__ __launch_bounds__(WAVE) void k_jvp_profile(JvpArgs A) {
+    const int lane = threadIdx.x;
+    const int L = A.L;
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written
+    const KCONST StarDev& S = uniform_ptr(A.stars)[0];
+    for (;;) {
+        const int item = wave_take(&A.counters[0], lane);
+        if (item >= A.n_items) break;
+        const int tile = item / A.n_lblocks;
+        const int l0 = (item % A.n_lblocks) * G;
+        const int64_t j0 = (int64_t)tile * A.tile_w;
+        const int64_t j1 = min(j0 + A.tile_w, S.n_wl) - 1;
+        const int jj = (int)(j0 + lane);
+        const double lam = S.wls[min((int64_t)jj, j1)];  // points past j1 are never used
+        double acc[JVP_DIRS][G];
+#pragma unroll
+        for (int k = 0; k < JVP_DIRS; ++k)
+#pragma unroll
+            for (int l = 0; l < G; ++l) acc[k][l] = 0.0;
+        auto no_operand = [](int) { return 0; };
+        for_each_line_pf<int>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_WINDOW,
+                              no_operand, [&](const LineView& v, int) {
+            const bool inw = jj >= v.lb && jj <= v.ub;
+            const double dl = fabs(lam - v.lam0);
+            const KCONST double4* rp = uniform_ptr(A.rec64 + (int64_t)v.line * A.Ls + l0);
+            const KCONST double* cp =
+                uniform_ptr(A.coef + ((int64_t)v.line * A.Ls + l0) * JVP_DIRS * 3);
+#pragma unroll
+            for (int l = 0; l < G; ++l) {
+                const double4 r = rp[l];
+                const double vv = dl * r.x;
+                double Ha, Hv;
+                const double H = voigt_hjerting64_grad(r.y, vv, Ha, Hv);
+                const double vHv = vv * Hv;
+#pragma unroll
+                for (int k = 0; k < JVP_DIRS; ++k) {
+                    const double* f = cp + (l * JVP_DIRS + k) * 3;
+                    const double t = f[0] * H + f[1] * Ha + f[2] * vHv;
+                    acc[k][l] += inw ? t : 0.0;
+                }
+            }
+        });
+        if (jj > j1) continue;
+#pragma unroll
+        for (int k = 0; k < JVP_DIRS; ++k) {
+            if (k >= A.n_dirs) break;
+#pragma unroll
+            for (int l = 0; l < G; ++l)
+                if (l0 + l < L)
+                    add_alpha(A.out[k], A.out_f32, (int64_t)(l0 + l) + (int64_t)L * jj, acc[k][l]);
+        }
     }
 }
 
@@ -2898,7 +3086,8 @@ struct Workspace {
         tc_start, tile_lines, stars;
     // adjoint and fit step (adj_*, gexp, g32: the series form's moments)
     DevBuf vjp_cnt, vjp_start, seg_cnt, seg_start, vjp_partial, rt_tab, fit_G, chi2_part, adj_blk,
-        adj_start, adj_line, gexp, g32, lam_s, azone, adj_lead, adj_groups, adj_gstart, scan_part;
+        adj_start, adj_line, gexp, g32, lam_s, azone, adj_lead, adj_groups, adj_gstart, scan_part,
+        jvp_coef;
     int64_t tile_cap = 0;             // entries tile_lines holds
     int64_t seg_cap = 0;              // adjoint segments vjp_partial holds
     std::vector<StarDev> stars_host;  // the star table (host copy)
@@ -2920,7 +3109,7 @@ struct Workspace {
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
-                          &scan_part};
+                          &scan_part, &jvp_coef};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -2961,8 +3150,10 @@ struct korg_ctx {
     // adjoint outputs of the host-memory entry point, and its staged G
     DevBuf vjp_out, h_grad;
     // staging for the host-memory entry points
-    DevBuf h_alpha, h_wls, h_lines[8], h_species_mass, h_species_mol, h_layers[7], h_knots,
-        h_cntm;
+    DevBuf h_alpha, h_wls, h_lines[8], h_win[2], h_species_mass, h_species_mol, h_layers[7],
+        h_knots, h_cntm;
+    // staging of the host-memory forward-mode entry point: tangents and d alpha
+    DevBuf h_tan, h_dalpha;
     // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
     DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices, hl_hull;
     int hl_ntrans = 0, hl_max_dnu = 0;
@@ -3118,6 +3309,9 @@ int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const
         d.vdW1 = x.lines.vdW1;
         d.vdW2 = x.lines.vdW2;
         d.species = x.lines.species;
+        const bool frozen = x.lines.window_lb && x.lines.window_ub;
+        d.win_lb = frozen ? x.lines.window_lb : nullptr;
+        d.win_ub = frozen ? x.lines.window_ub : nullptr;
         d.temps = x.layers.temps;
         d.n_e = x.layers.n_e;
         d.n_div_U = x.layers.n_div_U;
@@ -3945,6 +4139,90 @@ int finish_vjp_stats(korg_ctx* ctx) {
     return KORG_OK;
 }
 
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0; }
+
+// The forward-mode pipeline of one star on device pointers: the Float64 form's setup and window
+// partition, then per group of up to JVP_DIRS directions the coefficient kernel and the profile
+// pass, which add J x_k into out[k].  `tan` and `out` are host arrays of K entries (device
+// pointers inside).
+int run_jvp(korg_ctx* ctx, void* const* out, int out_dtype, const korg_grid* g,
+            const korg_lines* ln, const korg_species* sp, const korg_layers* ly,
+            const korg_continuum* cn, double cutoff, int64_t K, const korg_layer_tangents* tan) {
+    hipStream_t s = ctx->stream;
+    Workspace& ws = ctx->ws;
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    ctx->last_n_lines = 0;
+    ctx->last_fixed = false;
+    ctx->last_sized = false;
+    korg_tuning t = ctx->tuning;  // the profile pass uses the Float64 form's 64-point tiles
+    t.points_per_lane = 1;
+    t.count_branches = 0;
+    const StarHost one = {nullptr, *g, *ln, *ly, *cn, cutoff};
+    Batch b;
+    TRY(plan_batch(ctx, t, 1, &one, sp, ly->n_layers, KORG_DTYPE_F64, KORG_PRECISION_FP64, ws, s,
+                   b));
+    ctx->last_n_lines = b.N;
+    ctx->last_precision = KORG_PRECISION_FP64;
+    ctx->last_n_lblocks = b.n_groups;
+    ctx->last_nb = 0;
+    if (b.N == 0 || K == 0) return KORG_OK;
+    constexpr int G = 4;  // layers per work item of the profile pass (divides the rec64 stride)
+    TRY(grow(ctx, ws.jvp_coef, sizeof(double) * 3 * JVP_DIRS * (size_t)b.N * b.Ls));
+    TRY(enqueue_front(ctx, ws, b, s, INT64_MAX, true));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+    JvpArgs A{};
+    A.L = b.L;
+    A.Ls = b.Ls;
+    A.stars = (const StarDev*)ws.stars.p;
+    A.meta = (const LineMeta*)ws.meta.p;
+    A.rec64 = (const double4*)ws.rec64.p;
+    A.tab = (const LayerTab*)ws.tab.p;
+    A.mass = sp->mass;
+    A.is_mol = sp->is_molecule;
+    A.coef = (double*)ws.jvp_coef.p;
+    A.n_wl = b.nwl;
+    A.ntiles = b.ntiles;
+    A.tile_w = b.tile_w;
+    A.n_lblocks = b.Ls / G;
+    A.n_items = (int64_t)b.ntiles * A.n_lblocks;
+    A.cands = (const Cand*)ws.cls_cand.p;
+    A.ranges = (const int2*)ws.ranges.p;
+    A.out_f32 = out_dtype == KORG_DTYPE_F32;
+    A.counters = (unsigned long long*)ws.counters.p;
+    int64_t nwaves = (int64_t)ctx->n_cu * b.wpc;
+    if (nwaves > A.n_items) nwaves = A.n_items;
+    for (int64_t k0 = 0; k0 < K; k0 += JVP_DIRS) {
+        A.n_dirs = (int)std::min<int64_t>(JVP_DIRS, K - k0);
+        for (int k = 0; k < JVP_DIRS; ++k) {
+            const bool on = k < A.n_dirs;
+            const korg_layer_tangents z{};
+            const korg_layer_tangents& x = on ? tan[k0 + k] : z;
+            A.t[k] = JvpTangent{x.T, x.n_e, x.xi, x.n_HI, x.n_HeI, x.n_H2, x.n_div_U};
+            A.out[k] = on ? out[k0 + k] : nullptr;
+        }
+        if (k0 > 0) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, A.counters);
+        hipLaunchKernelGGL(k_jvp_coef, dim3((unsigned)((b.N + 3) / 4)), dim3(256), 0, s, b.N, A);
+        hipLaunchKernelGGL(k_jvp_profile<G>, dim3((unsigned)nwaves), dim3(WAVE), 0, s, A);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    return KORG_OK;
+}
+
+int check_jvp_args(korg_ctx* ctx, void* const* dalpha, int64_t K, const korg_layer_tangents* t,
+                   int64_t n_lines) {
+    if (K < 0 || K > 4096) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "n_tangents %lld out of "
+                                          "[0, 4096]", (long long)K);
+    if (K > 0 && n_lines > 0) {
+        if (!dalpha || !t) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "null dalpha or tangents");
+        for (int64_t k = 0; k < K; ++k)
+            if (!dalpha[k]) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "dalpha[%lld] is null",
+                                           (long long)k);
+    }
+    return KORG_OK;
+}
+
 // Validates the fit targets of a batch and puts them (and the gradient outputs) into the stars.
 int collect_fit(korg_ctx* ctx, int64_t n_stars, const korg_fit_target* t,
                 std::vector<StarHost>& hs) {
@@ -4039,6 +4317,12 @@ int stage_inputs(korg_ctx* ctx, const korg_grid* grid, const korg_lines* lines,
     TRY(stage(ctx, ctx->h_lines[5], lines->vdW1, N, &ln->vdW1));
     TRY(stage(ctx, ctx->h_lines[6], lines->vdW2, N, &ln->vdW2));
     TRY(stage(ctx, ctx->h_lines[7], lines->species, N, &ln->species));
+    if (lines->window_lb && lines->window_ub) {
+        TRY(stage(ctx, ctx->h_win[0], lines->window_lb, N, &ln->window_lb));
+        TRY(stage(ctx, ctx->h_win[1], lines->window_ub, N, &ln->window_ub));
+    } else {
+        ln->window_lb = ln->window_ub = nullptr;
+    }
     TRY(stage(ctx, ctx->h_species_mass, species->mass, S, &sp->mass));
     TRY(stage(ctx, ctx->h_species_mol, species->is_molecule, S, &sp->is_molecule));
     TRY(stage(ctx, ctx->h_layers[0], layers->temps, L, &ly->temps));
@@ -4416,6 +4700,8 @@ void korg_destroy(korg_ctx* ctx) {
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
         if (b.p) (void)hipFree(b.p);
+    for (DevBuf* b : {&ctx->h_win[0], &ctx->h_win[1], &ctx->h_tan, &ctx->h_dalpha})
+        if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_layers)
         if (b.p) (void)hipFree(b.p);
     for (auto& e : ctx->ev)
@@ -4620,6 +4906,83 @@ int korg_line_absorption_vjp(korg_ctx* ctx, const void* grad_alpha, int grad_dty
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return finish_vjp_stats(ctx);
+}
+
+int korg_line_absorption_jvp_device(korg_ctx* ctx, void* const* dalpha, int dalpha_dtype,
+                                    const korg_grid* grid, const korg_lines* lines,
+                                    const korg_species* species, const korg_layers* layers,
+                                    const korg_continuum* cntm, double cutoff_threshold,
+                                    int64_t n_tangents, const korg_layer_tangents* tangents,
+                                    int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(validate(ctx, grid, lines, species, layers, cntm, dalpha_dtype, KORG_PRECISION_FP64));
+    TRY(check_jvp_args(ctx, dalpha, n_tangents, tangents, lines->n_lines));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_jvp(ctx, dalpha, dalpha_dtype, grid, lines, species, layers, cntm, cutoff_threshold,
+                n_tangents, tangents));
+    if (synchronize) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(finish_stats(ctx));
+    }
+    return KORG_OK;
+}
+
+int korg_line_absorption_jvp(korg_ctx* ctx, void* const* dalpha, int dalpha_dtype,
+                             const korg_grid* grid, const korg_lines* lines,
+                             const korg_species* species, const korg_layers* layers,
+                             const korg_continuum* cntm, double cutoff_threshold,
+                             int64_t n_tangents, const korg_layer_tangents* tangents) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(validate(ctx, grid, lines, species, layers, cntm, dalpha_dtype, KORG_PRECISION_FP64));
+    TRY(check_jvp_args(ctx, dalpha, n_tangents, tangents, lines->n_lines));
+    if (lines->n_lines == 0 || n_tangents == 0) {
+        ctx->last_n_lines = 0;
+        std::memset(&ctx->stats, 0, sizeof ctx->stats);
+        return KORG_OK;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int64_t L = layers->n_layers, S = species->n_species, K = n_tangents;
+    korg_grid g = *grid;
+    korg_lines ln = *lines;
+    korg_species sp = *species;
+    korg_layers ly = *layers;
+    korg_continuum cn = *cntm;
+    TRY(stage_inputs(ctx, grid, lines, species, layers, cntm, &g, &ln, &sp, &ly, &cn));
+    // tangents: per direction 6 per-layer arrays and the n/U block, packed; null stays null
+    const size_t per = (size_t)(6 * L + S * L);
+    TRY(grow(ctx, ctx->h_tan, sizeof(double) * per * (size_t)K));
+    std::vector<korg_layer_tangents> td((size_t)K);
+    for (int64_t k = 0; k < K; ++k) {
+        double* base = (double*)ctx->h_tan.p + per * (size_t)k;
+        const double* src[7] = {tangents[k].T, tangents[k].n_e, tangents[k].xi, tangents[k].n_HI,
+                                tangents[k].n_HeI, tangents[k].n_H2, tangents[k].n_div_U};
+        const double* dst[7] = {};
+        for (int q = 0; q < 7; ++q) {
+            if (!src[q]) continue;
+            double* d = base + (size_t)q * L;
+            const size_t cnt = q < 6 ? (size_t)L : (size_t)(S * L);
+            HIP_TRY(ctx, hipMemcpyAsync(d, src[q], sizeof(double) * cnt, hipMemcpyHostToDevice,
+                                        ctx->stream));
+            dst[q] = d;
+        }
+        td[k] = korg_layer_tangents{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6]};
+    }
+    const size_t esz = dalpha_dtype == KORG_DTYPE_F32 ? 4 : 8;
+    const size_t abytes = esz * (size_t)L * (size_t)grid->n_wl;
+    TRY(grow(ctx, ctx->h_dalpha, abytes * (size_t)K));
+    std::vector<void*> dd((size_t)K);
+    for (int64_t k = 0; k < K; ++k) {
+        dd[k] = (char*)ctx->h_dalpha.p + abytes * (size_t)k;
+        HIP_TRY(ctx, hipMemcpyAsync(dd[k], dalpha[k], abytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    TRY(run_jvp(ctx, dd.data(), dalpha_dtype, &g, &ln, &sp, &ly, &cn, cutoff_threshold, K,
+                td.data()));
+    for (int64_t k = 0; k < K; ++k)
+        HIP_TRY(ctx, hipMemcpyAsync(dalpha[k], dd[k], abytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return finish_stats(ctx);
 }
 
 int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {

@@ -32,7 +32,8 @@ _c_dp = ctypes.c_void_p  # all array pointers are passed as raw addresses
 class KorgLines(ctypes.Structure):
     _fields_ = [("n_lines", ctypes.c_int64), ("wl", _c_dp), ("log_gf", _c_dp),
                 ("E_lower", _c_dp), ("gamma_rad", _c_dp), ("gamma_stark", _c_dp),
-                ("vdW1", _c_dp), ("vdW2", _c_dp), ("species", _c_dp)]
+                ("vdW1", _c_dp), ("vdW2", _c_dp), ("species", _c_dp), ("window_lb", _c_dp),
+                ("window_ub", _c_dp)]
 
 
 class KorgSpecies(ctypes.Structure):
@@ -64,6 +65,11 @@ class KorgStats(ctypes.Structure):
 
 
 class KorgLayerGrads(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2",
+                                               "n_div_U")]
+
+
+class KorgLayerTangents(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2",
                                                "n_div_U")]
 
@@ -119,7 +125,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 10  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 11  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -129,7 +135,8 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_plan_launch", "korg_plan_wait", "korg_plan_get_info", "korg_plan_destroy",
             "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit",
             "korg_continuum_batched", "korg_flux_batched", "korg_set_hydrogen_tables",
-            "korg_hydrogen_line_absorption", "korg_line_absorption_device_noalloc"]
+            "korg_hydrogen_line_absorption", "korg_line_absorption_device_noalloc",
+            "korg_line_absorption_jvp", "korg_line_absorption_jvp_device"]
 
 _lib = None
 
@@ -184,6 +191,13 @@ def load():
     L.korg_line_absorption_vjp.restype = i32
     L.korg_line_absorption_vjp_device.argtypes = vjp + [i32]
     L.korg_line_absorption_vjp_device.restype = i32
+    jvp = [vp, ctypes.POINTER(vp), i32, ctypes.POINTER(KorgGrid), ctypes.POINTER(KorgLines),
+           ctypes.POINTER(KorgSpecies), ctypes.POINTER(KorgLayers), ctypes.POINTER(KorgContinuum),
+           d, i64, ctypes.POINTER(KorgLayerTangents)]
+    L.korg_line_absorption_jvp.argtypes = jvp
+    L.korg_line_absorption_jvp.restype = i32
+    L.korg_line_absorption_jvp_device.argtypes = jvp + [i32]
+    L.korg_line_absorption_jvp_device.restype = i32
     L.korg_line_absorption_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
                                                ctypes.POINTER(KorgSpecies), i32, i32, i32]
     L.korg_line_absorption_batched.restype = i32

@@ -29,13 +29,18 @@ def _host_structs(P):
     keep = [P.all_wls, ll.wl, ll.log_gf, ll.E_lower, ll.gamma_rad, ll.gamma_stark, ll.vdW1,
             ll.vdW2, ll.species_index, P.species_mass, P.species_is_mol, P.n_div_U, P.temps,
             P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi, P.cntm_values, P.cntm_knots]
+    frozen = P.window_lb is not None and P.window_ub is not None
+    if frozen:
+        keep += [P.window_lb, P.window_ub]
     for a in keep:
         assert a.flags.c_contiguous
     g = _lib.KorgGrid(P.n_wl, P.all_wls.ctypes.data)
     lines = _lib.KorgLines(len(ll), ll.wl.ctypes.data, ll.log_gf.ctypes.data,
                            ll.E_lower.ctypes.data, ll.gamma_rad.ctypes.data,
                            ll.gamma_stark.ctypes.data, ll.vdW1.ctypes.data, ll.vdW2.ctypes.data,
-                           ll.species_index.ctypes.data)
+                           ll.species_index.ctypes.data,
+                           P.window_lb.ctypes.data if frozen else None,
+                           P.window_ub.ctypes.data if frozen else None)
     sp = _lib.KorgSpecies(P.species_mass.size, P.species_mass.ctypes.data,
                           P.species_is_mol.ctypes.data)
     ly = _lib.KorgLayers(P.n_layers, P.temps.ctypes.data, P.n_e.ctypes.data,
@@ -59,8 +64,11 @@ class DeviceProblem:
         t = self.t
         d = lambda k: t[k].ptr
         self.grid = _lib.KorgGrid(P.n_wl, d("all_wls"))
+        frozen = "window_lb" in t
         self.lines = _lib.KorgLines(len(ll), d("wl"), d("log_gf"), d("E_lower"), d("gamma_rad"),
-                                    d("gamma_stark"), d("vdW1"), d("vdW2"), d("species"))
+                                    d("gamma_stark"), d("vdW1"), d("vdW2"), d("species"),
+                                    d("window_lb") if frozen else None,
+                                    d("window_ub") if frozen else None)
         self.species = _lib.KorgSpecies(P.species_mass.size, d("mass"), d("is_mol"))
         self.layers = _lib.KorgLayers(P.n_layers, d("temps"), d("n_e"), d("n_div_U"), d("n_HI"),
                                       d("n_HeI"), d("n_H2"), d("xi"))
@@ -69,12 +77,15 @@ class DeviceProblem:
 
     def _arrays(self, P):
         ll = P.lines
-        return {"all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
-                "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
-                "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
-                "is_mol": P.species_is_mol, "n_div_U": P.n_div_U, "temps": P.temps, "n_e": P.n_e,
-                "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
-                "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}
+        out = {"all_wls": P.all_wls, "wl": ll.wl, "log_gf": ll.log_gf, "E_lower": ll.E_lower,
+               "gamma_rad": ll.gamma_rad, "gamma_stark": ll.gamma_stark, "vdW1": ll.vdW1,
+               "vdW2": ll.vdW2, "species": ll.species_index, "mass": P.species_mass,
+               "is_mol": P.species_is_mol, "n_div_U": P.n_div_U, "temps": P.temps, "n_e": P.n_e,
+               "n_HI": P.n_HI, "n_HeI": P.n_HeI, "n_H2": P.n_H2, "xi": P.xi,
+               "cntm": P.cntm_values, "knots": P.cntm_knots if P.n_knots else np.zeros(1)}
+        if P.window_lb is not None and P.window_ub is not None:
+            out["window_lb"], out["window_ub"] = P.window_lb, P.window_ub
+        return out
 
     def update(self, problem):
         """Writes ``problem`` (same sizes and continuum mode) into this problem's device buffers,
@@ -85,6 +96,8 @@ class DeviceProblem:
                 self.problem.n_wl, self.problem.n_layers, len(self.problem.lines),
                 self.problem.cntm_mode, self.problem.n_knots):
             raise ValueError("update needs a problem of the same sizes and continuum mode")
+        if new.keys() != self.t.keys():
+            raise ValueError("update cannot freeze or unfreeze the windows")
         for k, v in new.items():
             if v.shape != self.t[k].shape or v.dtype != self.t[k].dtype:
                 raise ValueError(f"update: {k} changes shape or dtype")
@@ -441,6 +454,71 @@ class Context:
             self._h, ctypes.c_void_p(G_dev.ptr), dt[G_dev.dtype], ctypes.byref(dprob.grid),
             ctypes.byref(dprob.lines), ctypes.byref(dprob.species), ctypes.byref(dprob.layers),
             ctypes.byref(dprob.cntm), P.cutoff_threshold, ctypes.byref(grads),
+            1 if synchronize else 0)
+        _lib.check(self._h, rc)
+
+    def line_absorption_jvp(self, problem, tangents, dalpha=None, dtype=np.float64):
+        """Forward mode on host arrays: ``tangents`` is a list of K dicts of per-layer input
+        tangents (``T``, ``n_e``, ``xi``, ``n_HI``, ``n_HeI``, ``n_H2``: length n_layers;
+        ``n_div_U``: (n_species, n_layers); a missing key is a zero tangent).  Adds J x_k into
+        ``dalpha[k]`` (a list of K (n_layers, n_wl) Fortran-ordered arrays; zeros when not
+        given) and returns the list.  Windows held fixed; Float64; deterministic."""
+        P = problem
+        L, S, K = P.n_layers, P.species_mass.size, len(tangents)
+        if dalpha is None:
+            dalpha = [np.zeros((L, P.n_wl), dtype=dtype, order="F") for _ in range(K)]
+        if len(dalpha) != K:
+            raise ValueError("one dalpha per tangent")
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        for a in dalpha:
+            if a.shape != (L, P.n_wl) or not a.flags.f_contiguous or a.dtype != dalpha[0].dtype \
+                    or a.dtype not in dt:
+                raise ValueError("dalpha: (n_layers, n_wl) Fortran-ordered float64/float32 arrays "
+                                 "of one dtype")
+        names = ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2", "n_div_U")
+        keep = []
+
+        def arr(t, k):
+            if k not in t:
+                return None
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(t[k], dtype=np.float64),
+                                                     (S, L) if k == "n_div_U" else (L,)))
+            keep.append(a)
+            return a.ctypes.data
+        tv = (_lib.KorgLayerTangents * max(K, 1))(*[_lib.KorgLayerTangents(*[arr(t, k) for k in names])
+                                                    for t in tangents])
+        ptrs = (ctypes.c_void_p * max(K, 1))(*[a.ctypes.data for a in dalpha])
+        structs, keep2 = _host_structs(P)
+        g, lines, sp, ly, cn = structs
+        rc = self._L.korg_line_absorption_jvp(
+            self._h, ptrs, dt[dalpha[0].dtype] if K else _lib.DTYPE_F64, ctypes.byref(g),
+            ctypes.byref(lines), ctypes.byref(sp), ctypes.byref(ly), ctypes.byref(cn),
+            P.cutoff_threshold, K, tv)
+        del keep, keep2
+        _lib.check(self._h, rc)
+        return dalpha
+
+    def line_absorption_jvp_device(self, dprob, tangents_dev, dalpha_dev, synchronize=True):
+        """Forward mode on device arrays: ``tangents_dev`` a list of dicts of device arrays
+        (keys as ``line_absorption_jvp``), ``dalpha_dev`` a list of (n_wl, n_layers) device arrays
+        of one dtype (the layout of ``DeviceProblem.new_alpha``), added into."""
+        P = dprob.problem
+        K = len(tangents_dev)
+        dt = {np.dtype(np.float64): _lib.DTYPE_F64, np.dtype(np.float32): _lib.DTYPE_F32}
+        if len(dalpha_dev) != K:
+            raise ValueError("one dalpha per tangent")
+        for a in dalpha_dev:
+            if tuple(a.shape) != (P.n_wl, P.n_layers) or a.dtype != dalpha_dev[0].dtype:
+                raise ValueError("dalpha_dev: (n_wl, n_layers) device arrays of one dtype")
+        names = ("T", "n_e", "xi", "n_HI", "n_HeI", "n_H2", "n_div_U")
+        tv = (_lib.KorgLayerTangents * max(K, 1))(
+            *[_lib.KorgLayerTangents(*[t[k].ptr if k in t else None for k in names])
+              for t in tangents_dev])
+        ptrs = (ctypes.c_void_p * max(K, 1))(*[a.ptr for a in dalpha_dev])
+        rc = self._L.korg_line_absorption_jvp_device(
+            self._h, ptrs, dt[dalpha_dev[0].dtype] if K else _lib.DTYPE_F64,
+            ctypes.byref(dprob.grid), ctypes.byref(dprob.lines), ctypes.byref(dprob.species),
+            ctypes.byref(dprob.layers), ctypes.byref(dprob.cntm), P.cutoff_threshold, K, tv,
             1 if synchronize else 0)
         _lib.check(self._h, rc)
 

@@ -75,6 +75,21 @@ class Problem:
                             dtype=np.float64)
         self.n_knots = self.cntm_knots.size
         self.cutoff_threshold = float(cutoff_threshold)
+        # frozen windows (1-based int64, as Context.windows returns them), or None: computed
+        self.window_lb = None
+        self.window_ub = None
+
+    def with_windows(self, lb, ub):
+        """A copy of this problem whose line windows are frozen at ``lb``/``ub`` (1-based, one
+        per line; lb > ub skips the line) instead of computed from the inputs."""
+        import copy
+        lb = np.ascontiguousarray(lb, dtype=np.int64)
+        ub = np.ascontiguousarray(ub, dtype=np.int64)
+        if lb.shape != (len(self.lines),) or ub.shape != (len(self.lines),):
+            raise ValueError("one window per line")
+        P = copy.copy(self)
+        P.window_lb, P.window_ub = lb, ub
+        return P
 
     @property
     def n_layers(self):

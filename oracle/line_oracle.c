@@ -280,6 +280,8 @@ static void line_setup(const problem_t* P, int64_t i, double* sigma, double* gam
  *           1: cntm_values[l*n_knots + k] on knots[k] (linear interpolation).
  * lb_out/ub_out (optional): the 1-based windows, lb > ub for skipped lines.
  * n_evals_out (optional): sum over lines of L*(ub-lb+1) for lb <= ub.
+ * lb_in/ub_in (optional, both or neither): frozen 1-based windows used instead of the computed
+ *   ones (clipped to the grid), the windows a ForwardDiff evaluation holds constant.
  * Returns 0.
  */
 int oracle_line_absorption(double* alpha, int64_t L, int64_t nwl, const double* wls,
@@ -292,7 +294,8 @@ int oracle_line_absorption(double* alpha, int64_t L, int64_t nwl, const double* 
                            const double* n_HeI, const double* n_H2, const double* xi,
                            int cntm_mode, const double* cntm_values, int64_t n_knots,
                            const double* knots, double cutoff_threshold, int nthreads, int mode,
-                           int64_t* lb_out, int64_t* ub_out, double* n_evals_out) {
+                           int64_t* lb_out, int64_t* ub_out, double* n_evals_out,
+                           const int64_t* lb_in, const int64_t* ub_in) {
     if (nlines == 0) return 0; /* src/line_absorption.jl:35-37 */
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -331,6 +334,12 @@ int oracle_line_absorption(double* alpha, int64_t L, int64_t nwl, const double* 
             line_setup(&P, i, sig + i * L, gam + i * L, amp + i * L, rho, inv, lb + i, ub + i);
         free(rho);
         free(inv);
+    }
+    if (lb_in && ub_in) {
+        for (int64_t i = 0; i < nlines; ++i) {
+            lb[i] = lb_in[i] < 1 ? 1 : lb_in[i];
+            ub[i] = ub_in[i] > nwl ? nwl : ub_in[i];
+        }
     }
     double nev = 0.0;
     for (int64_t i = 0; i < nlines; ++i)
@@ -490,11 +499,88 @@ double oracle_voigt_hjerting_grad(double a, double v, double* dHda, double* dHdv
     }
 }
 
+/* Partial derivatives of one (line, layer)'s profile parameters with respect to the layer's
+ * inputs x = (T, n_e, xi, n_HI, n_HeI, n_H2, n/U of the line's species): the line_profile of
+ * src/line_absorption.jl:245-249 is scaling * H(a, v) with inv = 1/(sigma sqrt 2), a = gamma inv,
+ * v = |lambda - lambda0| inv, scaling = inv/sqrt(pi) * amplitude.  da[q] = da/dx_q,
+ * dli[q] = d ln(inv)/dx_q, ds[q] = dscaling/dx_q (src/line_absorption.jl:86-106, 190). */
+typedef struct {
+    double inv, a, scaling;
+    double da[7], dli[7], ds[7];
+} layer_partials_t;
+
+static void layer_partials(const problem_t* P, int64_t i, int64_t l, const double* sig,
+                           const double* gam, const double* amp, double c_He, double c_H2,
+                           layer_partials_t* o) {
+    const int32_t s = P->species_index[i];
+    const double m = P->species_mass[s];
+    const int mol = P->species_is_mol[s] != 0;
+    const double lam0 = P->wl[i];
+    const double E_lower = P->E_lower[i];
+    const double E_up = E_lower + c_cgs * hplanck_eV / lam0;
+    const double gf = pow(10.0, P->log_gf[i]);
+    const double sl = oracle_sigma_line(lam0);
+    const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * M_PI));
+    const double T = P->temps[l], xi = P->xi[l];
+    const double inv = 1.0 / (sig[l] * sqrt(2.0));
+    const double a = gam[l] * inv;
+    const double scaling = inv / sqrt(M_PI) * amp[l];
+    /* d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 */
+    const double q = kboltz_cgs * T / m + xi * xi / 2.0;
+    const double dli_dT = -(kboltz_cgs / m) / (2.0 * q);
+    const double dli_dxi = -xi / (2.0 * q);
+    /* dGamma / dx */
+    double dG_dT = 0.0, dG_dne = 0.0, vdw = 0.0;
+    if (!mol) {
+        const double st = oracle_scaled_stark(P->gamma_stark[i], T);
+        vdw = oracle_scaled_vdW(P->vdW1[i], P->vdW2[i], m, T);
+        const double dvdw_dT = (P->vdW2[i] == -1.0 ? 0.3 : (1.0 - P->vdW2[i]) / 2.0) * vdw / T;
+        dG_dne = st;
+        dG_dT = P->n_e[l] * st / (6.0 * T) + P->n_eff[l] * dvdw_dT;
+    }
+    /* da/dx = inv dgamma/dx + a dln(inv)/dx */
+    const double ig = inv * gscale;
+    /* scaling = inv / sqrt(pi) * gf * sigma_line * levels * nU */
+    const double nU = P->n_div_U[(int64_t)s * P->L + l];
+    const double bl = P->beta[l];
+    const double el = exp(-bl * E_lower), eu = exp(-bl * E_up);
+    const double dlev_dT = (bl / T) * (E_lower * el - E_up * eu);
+    const double base = inv / sqrt(M_PI) * gf * sl;
+    o->inv = inv;
+    o->a = a;
+    o->scaling = scaling;
+    const double da[7] = {ig * dG_dT + a * dli_dT, ig * dG_dne, a * dli_dxi, ig * vdw,
+                          ig * c_He * vdw, ig * c_H2 * vdw, 0.0};
+    const double dli[7] = {dli_dT, 0.0, dli_dxi, 0.0, 0.0, 0.0, 0.0};
+    const double ds[7] = {scaling * dli_dT + base * nU * dlev_dT, 0.0, scaling * dli_dxi, 0.0, 0.0,
+                          0.0, base * (el - eu)};
+    memcpy(o->da, da, sizeof da);
+    memcpy(o->dli, dli, sizeof dli);
+    memcpy(o->ds, ds, sizeof ds);
+}
+
+static void vdW_mix(double* c_He, double* c_H2) {
+    /* src/line_absorption.jl:54-60 */
+    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
+    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;
+    *c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
+    *c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+}
+
+/* The window of line i: computed (line_setup), or the frozen one when lb_in/ub_in are given. */
+static void line_window(const problem_t* P, int64_t i, const int64_t* lb_in, const int64_t* ub_in,
+                        int64_t* lb, int64_t* ub) {
+    if (lb_in && ub_in) {
+        *lb = lb_in[i] < 1 ? 1 : lb_in[i];
+        *ub = ub_in[i] > P->nwl ? P->nwl : ub_in[i];
+    }
+}
+
 /*
- * Vector-Jacobian product of oracle_line_absorption: given G = dL/dalpha (column-major
- * [L x nwl], G[l + L*j]), writes dL/d(input) per layer:
+ * Vector-Jacobian product: given G = dL/dalpha (column-major [L x nwl]), accumulates
  *   g_T[l], g_ne[l], g_xi[l], g_nHI[l], g_nHeI[l], g_nH2[l]  and  g_nU[s*L + l] (n_div_U).
- * Inputs as oracle_line_absorption.  Each line's contribution per layer is
+ * Inputs as oracle_line_absorption (lb_in/ub_in: optional frozen windows).  Each line's
+ * contribution per layer is
  *   scaling * (da/dx * S_a + dln(1/(sigma sqrt 2))/dx * S_v) + dscaling/dx * S_H
  * with S_H = sum_j G H, S_a = sum_j G dH/da, S_v = sum_j G v dH/dv over the line's window; the
  * contributions are summed in line order (deterministic).  Returns 0.
@@ -510,7 +596,8 @@ int oracle_line_absorption_vjp(const double* G, int64_t L, int64_t nwl, const do
                                const double* xi, int cntm_mode, const double* cntm_values,
                                int64_t n_knots, const double* knots, double cutoff_threshold,
                                int nthreads, double* g_T, double* g_ne, double* g_xi,
-                               double* g_nHI, double* g_nHeI, double* g_nH2, double* g_nU) {
+                               double* g_nHI, double* g_nHeI, double* g_nH2, double* g_nU,
+                               const int64_t* lb_in, const int64_t* ub_in) {
     for (int64_t l = 0; l < L; ++l) g_T[l] = g_ne[l] = g_xi[l] = g_nHI[l] = g_nHeI[l] = g_nH2[l] = 0.0;
     for (int64_t k = 0; k < n_species * L; ++k) g_nU[k] = 0.0;
     if (nlines == 0) return 0;
@@ -524,10 +611,8 @@ int oracle_line_absorption_vjp(const double* G, int64_t L, int64_t nwl, const do
                    cntm_mode, cntm_values, knots, n_knots, cutoff_threshold, NULL, NULL};
     double* beta = malloc(sizeof(double) * L);
     double* n_eff = malloc(sizeof(double) * L);
-    const double a_H = 4.50711, a_He = 1.38375, a_H2 = 5.503;
-    const double m_H = 1.008 * amu_cgs, m_He = 4.003 * amu_cgs;
-    const double c_He = pow(a_He / a_H, 0.4) * pow(m_He / m_H, -0.3);
-    const double c_H2 = pow(a_H2 / a_H, 0.4) * pow(2.0, -0.3);
+    double c_He, c_H2;
+    vdW_mix(&c_He, &c_H2);
     for (int64_t l = 0; l < L; ++l) {
         beta[l] = 1.0 / (kboltz_eV * temps[l]);
         n_eff[l] = n_HI[l] + c_He * n_HeI[l] + c_H2 * n_H2[l];
@@ -547,71 +632,26 @@ int oracle_line_absorption_vjp(const double* G, int64_t L, int64_t nwl, const do
         for (int64_t i = 0; i < nlines; ++i) {
             int64_t lb, ub;
             line_setup(&P, i, sig, gam, amp, rho, tmp, &lb, &ub);
+            line_window(&P, i, lb_in, ub_in, &lb, &ub);
             if (lb > ub) continue;
-            const int32_t s = species_index[i];
-            const double m = species_mass[s];
-            const int mol = species_is_mol[s] != 0;
             const double lam0 = wl[i];
-            const double E_up = E_lower[i] + c_cgs * hplanck_eV / lam0;
-            const double gf = pow(10.0, log_gf[i]);
-            const double sl = oracle_sigma_line(lam0);
-            const double gscale = (lam0 * lam0) / (c_cgs * (4.0 * M_PI));
             for (int64_t l = 0; l < L; ++l) {
-                const double T = temps[l];
-                const double inv = 1.0 / (sig[l] * sqrt(2.0));
-                const double a = gam[l] * inv;
-                const double scaling = inv / sqrt(M_PI) * amp[l];
+                layer_partials_t d;
+                layer_partials(&P, i, l, sig, gam, amp, c_He, c_H2, &d);
                 double SH = 0.0, Sa = 0.0, Sv = 0.0;
                 for (int64_t j = lb - 1; j < ub; ++j) {
                     const double g = G[j * L + l];
                     if (g == 0.0) continue;
-                    const double v = fabs(wls[j] - lam0) * inv;
+                    const double v = fabs(wls[j] - lam0) * d.inv;
                     double Ha, Hv;
-                    const double H = oracle_voigt_hjerting_grad(a, v, &Ha, &Hv);
+                    const double H = oracle_voigt_hjerting_grad(d.a, v, &Ha, &Hv);
                     SH += g * H;
                     Sa += g * Ha;
                     Sv += g * v * Hv;
                 }
-                /* d ln(1/(sigma sqrt 2)) = -d ln sigma, sigma^2 ~ kT/m + xi^2/2 */
-                const double q = kboltz_cgs * T / m + xi[l] * xi[l] / 2.0;
-                const double dlinv_dT = -(kboltz_cgs / m) / (2.0 * q);
-                const double dlinv_dxi = -xi[l] / (2.0 * q);
-                /* dGamma / dx */
-                double dG_dT = 0.0, dG_dne = 0.0, dG_dnHI = 0.0, dG_dnHeI = 0.0, dG_dnH2 = 0.0;
-                if (!mol) {
-                    const double st = oracle_scaled_stark(gamma_stark[i], T);
-                    const double vdw = oracle_scaled_vdW(vdW1[i], vdW2[i], m, T);
-                    const double dvdw_dT = (vdW2[i] == -1.0 ? 0.3 : (1.0 - vdW2[i]) / 2.0) * vdw / T;
-                    dG_dne = st;
-                    dG_dT = n_e[l] * st / (6.0 * T) + n_eff[l] * dvdw_dT;
-                    dG_dnHI = vdw;
-                    dG_dnHeI = c_He * vdw;
-                    dG_dnH2 = c_H2 * vdw;
-                }
-                /* da/dx = inv dgamma/dx + a dln(inv)/dx */
-                const double da_dT = inv * gscale * dG_dT + a * dlinv_dT;
-                const double da_dxi = a * dlinv_dxi;
-                const double da_dne = inv * gscale * dG_dne;
-                const double da_dnHI = inv * gscale * dG_dnHI;
-                const double da_dnHeI = inv * gscale * dG_dnHeI;
-                const double da_dnH2 = inv * gscale * dG_dnH2;
-                /* scaling = inv / sqrt(pi) * gf * sigma_line * levels * nU */
-                const double nU = n_div_U[(int64_t)s * L + l];
-                const double bl = beta[l];
-                const double el = exp(-bl * E_lower[i]), eu = exp(-bl * E_up);
-                const double dlev_dT = (bl / T) * (E_lower[i] * el - E_up * eu);
-                const double base = inv / sqrt(M_PI) * gf * sl;
-                const double ds_dT = scaling * dlinv_dT + base * nU * dlev_dT;
-                const double ds_dxi = scaling * dlinv_dxi;
-                const double ds_dnU = base * (el - eu);
                 double* c = contrib + (i * L + l) * NG;
-                c[0] = scaling * (da_dT * Sa + dlinv_dT * Sv) + ds_dT * SH;
-                c[1] = scaling * (da_dne * Sa);
-                c[2] = scaling * (da_dxi * Sa + dlinv_dxi * Sv) + ds_dxi * SH;
-                c[3] = scaling * (da_dnHI * Sa);
-                c[4] = scaling * (da_dnHeI * Sa);
-                c[5] = scaling * (da_dnH2 * Sa);
-                c[6] = ds_dnU * SH;
+                for (int q = 0; q < NG; ++q)
+                    c[q] = d.scaling * (d.da[q] * Sa + d.dli[q] * Sv) + d.ds[q] * SH;
             }
         }
         free(sig); free(gam); free(amp); free(rho); free(tmp);
@@ -632,5 +672,123 @@ int oracle_line_absorption_vjp(const double* G, int64_t L, int64_t nwl, const do
     free(contrib);
     free(beta);
     free(n_eff);
+    return 0;
+}
+
+/*
+ * Jacobian-vector products (forward mode, what a ForwardDiff.Dual alpha receives): for each of
+ * K tangent directions k of the per-layer inputs -- t_T[k*L + l], t_ne, t_xi, t_nHI, t_nHeI,
+ * t_nH2 (each [K x L]) and t_nU[(k*n_species + s)*L + l] -- ACCUMULATES into dalpha + k*L*nwl
+ * (column-major [L x nwl] each)
+ *   sum over lines of  (ds.x) H + scaling ((da.x) dH/da + (dli.x) v dH/dv)
+ * over each line's window (computed, or frozen by lb_in/ub_in), the windows held fixed and the
+ * branch of voigt_hjerting in use differentiated exactly, lines in list order per element.
+ * Returns 0.
+ */
+int oracle_line_absorption_jvp(double* dalpha, int64_t K, int64_t L, int64_t nwl,
+                               const double* wls, int64_t nlines, const double* wl,
+                               const double* log_gf, const double* E_lower,
+                               const double* gamma_rad, const double* gamma_stark,
+                               const double* vdW1, const double* vdW2,
+                               const int32_t* species_index, const double* species_mass,
+                               const uint8_t* species_is_mol, int64_t n_species,
+                               const double* n_div_U, const double* temps, const double* n_e,
+                               const double* n_HI, const double* n_HeI, const double* n_H2,
+                               const double* xi, int cntm_mode, const double* cntm_values,
+                               int64_t n_knots, const double* knots, double cutoff_threshold,
+                               int nthreads, const double* t_T, const double* t_ne,
+                               const double* t_xi, const double* t_nHI, const double* t_nHeI,
+                               const double* t_nH2, const double* t_nU, const int64_t* lb_in,
+                               const int64_t* ub_in) {
+    if (nlines == 0 || K == 0) return 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    problem_t P = {L, nwl, nlines, wls, wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2,
+                   species_index, species_mass, species_is_mol, n_div_U, temps, n_e, xi,
+                   cntm_mode, cntm_values, knots, n_knots, cutoff_threshold, NULL, NULL};
+    double* beta = malloc(sizeof(double) * L);
+    double* n_eff = malloc(sizeof(double) * L);
+    double c_He, c_H2;
+    vdW_mix(&c_He, &c_H2);
+    for (int64_t l = 0; l < L; ++l) {
+        beta[l] = 1.0 / (kboltz_eV * temps[l]);
+        n_eff[l] = n_HI[l] + c_He * n_HeI[l] + c_H2 * n_H2[l];
+    }
+    P.beta = beta;
+    P.n_eff = n_eff;
+    /* per (line, layer, direction): the coefficients of H, dH/da and v dH/dv */
+    double* cf = malloc(sizeof(double) * nlines * L * K * 3);
+    int64_t* lb = malloc(sizeof(int64_t) * nlines);
+    int64_t* ub = malloc(sizeof(int64_t) * nlines);
+    double* av = malloc(sizeof(double) * nlines * L * 2); /* (inv, a) */
+#pragma omp parallel num_threads(nthreads)
+    {
+        double* sig = malloc(sizeof(double) * L);
+        double* gam = malloc(sizeof(double) * L);
+        double* amp = malloc(sizeof(double) * L);
+        double* rho = malloc(sizeof(double) * L);
+        double* tmp = malloc(sizeof(double) * L);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < nlines; ++i) {
+            line_setup(&P, i, sig, gam, amp, rho, tmp, lb + i, ub + i);
+            line_window(&P, i, lb_in, ub_in, lb + i, ub + i);
+            const int32_t s = species_index[i];
+            for (int64_t l = 0; l < L; ++l) {
+                layer_partials_t d;
+                layer_partials(&P, i, l, sig, gam, amp, c_He, c_H2, &d);
+                av[(i * L + l) * 2] = d.inv;
+                av[(i * L + l) * 2 + 1] = d.a;
+                for (int64_t k = 0; k < K; ++k) {
+                    const double x[7] = {t_T ? t_T[k * L + l] : 0.0, t_ne ? t_ne[k * L + l] : 0.0,
+                                         t_xi ? t_xi[k * L + l] : 0.0,
+                                         t_nHI ? t_nHI[k * L + l] : 0.0,
+                                         t_nHeI ? t_nHeI[k * L + l] : 0.0,
+                                         t_nH2 ? t_nH2[k * L + l] : 0.0,
+                                         t_nU ? t_nU[(k * n_species + s) * L + l] : 0.0};
+                    double dsx = 0.0, dax = 0.0, dlx = 0.0;
+                    for (int q = 0; q < 7; ++q) {
+                        if (x[q] == 0.0) continue; /* a zero tangent adds nothing, NaN partials included */
+                        dsx += d.ds[q] * x[q];
+                        dax += d.da[q] * x[q];
+                        dlx += d.dli[q] * x[q];
+                    }
+                    double* c = cf + ((i * L + l) * K + k) * 3;
+                    c[0] = dsx;
+                    c[1] = d.scaling * dax;
+                    c[2] = d.scaling * dlx;
+                }
+            }
+        }
+        free(sig); free(gam); free(amp); free(rho); free(tmp);
+    }
+    /* each thread owns a block of wavelengths; lines in list order per element */
+    const int64_t chunk = 512;
+    const int64_t nchunks = (nwl + chunk - 1) / chunk;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+    for (int64_t c = 0; c < nchunks; ++c) {
+        const int64_t j0 = c * chunk;
+        const int64_t j1 = (j0 + chunk < nwl) ? j0 + chunk : nwl;
+        for (int64_t i = 0; i < nlines; ++i) {
+            if (lb[i] > ub[i]) continue;
+            const int64_t a0 = (lb[i] - 1 > j0) ? lb[i] - 1 : j0;
+            const int64_t b0 = (ub[i] < j1) ? ub[i] : j1;
+            for (int64_t j = a0; j < b0; ++j)
+                for (int64_t l = 0; l < L; ++l) {
+                    const double inv = av[(i * L + l) * 2], a = av[(i * L + l) * 2 + 1];
+                    const double v = fabs(wls[j] - wl[i]) * inv;
+                    double Ha, Hv;
+                    const double H = oracle_voigt_hjerting_grad(a, v, &Ha, &Hv);
+                    const double vHv = v * Hv;
+                    for (int64_t k = 0; k < K; ++k) {
+                        const double* f = cf + ((i * L + l) * K + k) * 3;
+                        dalpha[k * L * nwl + j * L + l] += f[0] * H + f[1] * Ha + f[2] * vHv;
+                    }
+                }
+        }
+    }
+    free(cf); free(lb); free(ub); free(av); free(beta); free(n_eff);
     return 0;
 }

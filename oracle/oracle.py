@@ -55,6 +55,7 @@ def lib():
         L.oracle_voigt_hjerting_grad.argtypes = [d, d, ctypes.POINTER(d), ctypes.POINTER(d)]
         L.oracle_voigt_hjerting_grad.restype = d
         L.oracle_line_absorption_vjp.restype = ctypes.c_int
+        L.oracle_line_absorption_jvp.restype = ctypes.c_int
         L.oracle_blackbody.argtypes = [d, d]
         L.oracle_blackbody.restype = d
         L.oracle_expint2.argtypes = [d, ctypes.POINTER(d)]
@@ -124,9 +125,18 @@ def searchsortedlast(all_wls, lam):
     return lib().oracle_searchsorted_last(_p(a), a.size, float(lam))
 
 
+def _frozen(P):
+    """The problem's frozen windows (1-based int64 arrays) or (None, None)."""
+    lb, ub = getattr(P, "window_lb", None), getattr(P, "window_ub", None)
+    if lb is None or ub is None:
+        return None, None
+    return (np.ascontiguousarray(lb, dtype=np.int64), np.ascontiguousarray(ub, dtype=np.int64))
+
+
 def line_absorption(alpha, problem, nthreads=0, mode=DETERMINISTIC, return_windows=False):
     """Accumulate the lines of ``problem`` (a ``korg_line.problem.Problem``) into ``alpha``
-    (float64, shape (L, n_wl), Fortran order == the reference's column-major layout).
+    (float64, shape (L, n_wl), Fortran order == the reference's column-major layout).  The
+    problem's frozen windows (``window_lb``/``window_ub``), when set, replace the computed ones.
 
     Returns ``(lb, ub, n_evals)`` (1-based windows) when ``return_windows``.
     """
@@ -145,7 +155,7 @@ def line_absorption(alpha, problem, nthreads=0, mode=DETERMINISTIC, return_windo
         _p(P.n_div_U), _p(P.temps), _p(P.n_e), _p(P.n_HI), _p(P.n_HeI), _p(P.n_H2), _p(P.xi),
         ctypes.c_int(P.cntm_mode), _p(P.cntm_values), ctypes.c_int64(P.n_knots),
         _p(P.cntm_knots), ctypes.c_double(P.cutoff_threshold), ctypes.c_int(nthreads),
-        ctypes.c_int(mode), _p(lb), _p(ub), ctypes.byref(nev))
+        ctypes.c_int(mode), _p(lb), _p(ub), ctypes.byref(nev), *map(_p, _frozen(P)))
     if rc != 0:
         raise RuntimeError(f"oracle_line_absorption returned {rc}")
     if return_windows:
@@ -185,10 +195,47 @@ def line_absorption_vjp(G, problem, nthreads=0):
         ctypes.c_int64(S), _p(P.n_div_U), _p(P.temps), _p(P.n_e), _p(P.n_HI), _p(P.n_HeI),
         _p(P.n_H2), _p(P.xi), ctypes.c_int(P.cntm_mode), _p(P.cntm_values),
         ctypes.c_int64(P.n_knots), _p(P.cntm_knots), ctypes.c_double(P.cutoff_threshold),
-        ctypes.c_int(nthreads), *[_p(out[k]) for k in GRAD_NAMES], _p(out["n_div_U"]))
+        ctypes.c_int(nthreads), *[_p(out[k]) for k in GRAD_NAMES], _p(out["n_div_U"]),
+        *map(_p, _frozen(P)))
     if rc != 0:
         raise RuntimeError(f"oracle_line_absorption_vjp returned {rc}")
     return out
+
+
+def line_absorption_jvp(problem, tangents, nthreads=0):
+    """Jacobian-vector products of the line opacity: ``tangents`` is a list of K dicts, each
+    giving tangents of the per-layer inputs (``T``, ``n_e``, ``xi``, ``n_HI``, ``n_HeI``,
+    ``n_H2``: length L; ``n_div_U``: (n_species, L); a missing key is a zero tangent).  Returns a
+    (K, L, n_wl) float64 array: dalpha_k = J x_k, windows held fixed (frozen ones if set)."""
+    P = problem
+    L, nwl, S = P.n_layers, P.n_wl, len(P.species_mass)
+    K = len(tangents)
+    n = len(P.lines)
+    # each direction's alpha is column-major (L, n_wl): a C-ordered (K, n_wl, L) array
+    out = np.zeros((K, nwl, L))
+    if K == 0:
+        return out.transpose(0, 2, 1)
+
+    def stack(name, shape):
+        if all(name not in t for t in tangents):
+            return None
+        return np.ascontiguousarray(np.stack([np.broadcast_to(np.asarray(t.get(name, 0.0),
+                                                                         dtype=np.float64), shape)
+                                              for t in tangents]))
+    tv = [stack(k, (L,)) for k in GRAD_NAMES] + [stack("n_div_U", (S, L))]
+    ll = P.lines
+    rc = lib().oracle_line_absorption_jvp(
+        _p(out), ctypes.c_int64(K), ctypes.c_int64(L), ctypes.c_int64(nwl), _p(P.all_wls),
+        ctypes.c_int64(n), _p(ll.wl), _p(ll.log_gf), _p(ll.E_lower), _p(ll.gamma_rad),
+        _p(ll.gamma_stark), _p(ll.vdW1), _p(ll.vdW2), _p(ll.species_index), _p(P.species_mass),
+        _p(P.species_is_mol), ctypes.c_int64(S), _p(P.n_div_U), _p(P.temps), _p(P.n_e),
+        _p(P.n_HI), _p(P.n_HeI), _p(P.n_H2), _p(P.xi), ctypes.c_int(P.cntm_mode),
+        _p(P.cntm_values), ctypes.c_int64(P.n_knots), _p(P.cntm_knots),
+        ctypes.c_double(P.cutoff_threshold), ctypes.c_int(nthreads), *map(_p, tv),
+        *map(_p, _frozen(P)))
+    if rc != 0:
+        raise RuntimeError(f"oracle_line_absorption_jvp returned {rc}")
+    return out.transpose(0, 2, 1)
 
 
 
