@@ -416,20 +416,12 @@ def bench_batched(args):
 def bench_fit(args):
     """The fit step of 64 stars as one graph replay per step (forward Float32 + adjoint)."""
     import numpy as np
-    from korg_line import Context, DeviceProblem, FitTarget, Wavelengths, workloads
+    from korg_line import Context, DeviceProblem, FitTarget, workloads
     from korg_line.hipmem import synchronize as device_sync
-    from korg_line.atmosphere import Atmosphere
 
     n_stars = 64 if args.stars == 256 else args.stars
     t = time.time()
-    u = lambda stream: workloads.uniform(7, stream, n_stars)
-    teff = 4000.0 + 2500.0 * u(101)
-    logg = 1.0 + 4.0 * u(102)
-    mh = -2.5 + 3.0 * u(103)
-    wls = Wavelengths((5000.0, 6000.0))
-    ll = workloads.synthetic_linelist(7, 100_000, 5000.0, 6000.0, (-4.0, 0.5), (0.0, 5.0))
-    atms = [Atmosphere.scaled(float(teff[k]), float(logg[k]), float(mh[k])) for k in range(n_stars)]
-    probs = [workloads.make_problem(a, ll, wls, 1.0) for a in atms]
+    probs, atms, ll, wls = workloads.fit_stars(n_stars)
     L = probs[0].n_layers
     log(f"{n_stars} stars built in {time.time() - t:.1f}s: {len(ll)} shared lines, "
         f"{probs[0].n_wl} points, {L} layers")
@@ -438,7 +430,7 @@ def bench_fit(args):
     dprobs = [DeviceProblem(P) for P in probs]
     a32 = [d.new_alpha("float32") for d in dprobs]
     tuning = ctx.autotune_batched(a32, dprobs, "fp32", cache_path=cache)
-    refs = [(10.0 ** a.lgTau5, a.continuum(np.array([5000e-8]))[:, 0]) for a in atms]
+    refs = [workloads.fit_reference(a) for a in atms]
     rng = np.random.default_rng(7)
     targets = [FitTarget(P, tr, ar, np.ones(P.n_wl), 0.01) for P, (tr, ar) in zip(probs, refs)]
     # data: the model flux with lines 10% deeper, plus noise -- an iterate away from the optimum,

@@ -1,9 +1,16 @@
 # KorgLineGPU.jl -- Julia binding of libkorgline.so (include/korg_line.h).
 #
-# Loading this module adds a more specific method of Korg.line_absorption! for Float64 alpha
-# matrices and Float64 lines, so that Korg's own calls inside `synthesize`
-# (src/synthesize.jl:266-276 and :305-306) dispatch to the GPU.  Other element types (e.g.
-# ForwardDiff.Dual during fitting) keep using Korg's CPU method.
+# Loading this module adds more specific methods of Korg.line_absorption! for Float64 lines, so
+# that every one of Korg's calls dispatches to the GPU:
+#   - Float64 alpha, as a Matrix or any view of one: synthesize's main call
+#     (src/synthesize.jl:305-306), its reference-wavelength call on view(α_ref, :, 1)
+#     (src/synthesize.jl:266-276) and the molecular cross-section precompute on view(α, i, :, :)
+#     (src/molecular_cross_sections.jl:75-76);
+#   - ForwardDiff.Dual alpha, as when Korg's Fit differentiates synthesize with
+#     autodiff=:forwarddiff (src/Fit/fit_via_synthesis.jl:345-355, src/synthesize.jl:242-247):
+#     the values through korg_line_absorption, the partials through korg_line_absorption_jvp
+#     (one tangent direction per partial, windows held at the values' windows as ForwardDiff's
+#     integer searchsorted holds them).
 #
 #     using Korg, KorgLineGPU
 #     KorgLineGPU.set_precision!(:fp32)   # or :fp64 (default)
@@ -27,6 +34,8 @@ struct KorgLines
     vdW1::Ptr{Float64}
     vdW2::Ptr{Float64}
     species::Ptr{Int32}
+    window_lb::Ptr{Int64}   # frozen windows (unused here: C_NULL, the windows are computed)
+    window_ub::Ptr{Int64}
 end
 
 struct KorgSpecies
@@ -102,6 +111,9 @@ function continuum_arrays(α_cntm, linelist, L)
     Int32(0), Float64[], vals
 end
 
+to_layers(x::Number, L) = fill(Float64(x), L)
+to_layers(x, L) = Float64.(collect(x))
+
 # The C-ABI view of one call: arrays (kept alive by the returned NamedTuple) and the structs that
 # point into them.  `ll` is sorted by wavelength; `specs` lists its species (index = table row).
 function flatten(ll, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α_cntm)
@@ -122,12 +134,13 @@ function flatten(ll, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α
     for (i, s) in enumerate(specs)
         n_div_U[:, i] .= n_densities[s] ./ partition_fns[s].(log.(temps))
     end
-    nan = fill(NaN, L)
-    nHI = Float64.(get(n_densities, Korg.species"H I", nan))
-    nHeI = Float64.(get(n_densities, Korg.species"He I", nan))
-    nH2 = Float64.(get(n_densities, Korg.species"H2", nan))
-    xi = ξ isa Number ? fill(Float64(ξ), L) : Float64.(ξ)
-    ne = Float64.(nₑ)
+    # densities may be per-layer vectors or one number for every layer (the molecular
+    # precompute passes scalars, src/molecular_cross_sections.jl:66-67)
+    nHI = to_layers(get(n_densities, Korg.species"H I", NaN), L)
+    nHeI = to_layers(get(n_densities, Korg.species"He I", NaN), L)
+    nH2 = to_layers(get(n_densities, Korg.species"H2", NaN), L)
+    xi = to_layers(ξ, L)
+    ne = to_layers(nₑ, L)
     T = Float64.(temps)
     mode, knots, cvals = continuum_arrays(α_cntm, ll, L)
     all_wls = λs.all_wls
@@ -135,7 +148,7 @@ function flatten(ll, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α
               xi, ne, T, knots, cvals, all_wls)
     grid = KorgGrid(length(all_wls), pointer(all_wls))
     lines = KorgLines(length(wl), pointer(wl), pointer(log_gf), pointer(El), pointer(grad),
-                      pointer(gstark), pointer(vdW1), pointer(vdW2), pointer(sp))
+                      pointer(gstark), pointer(vdW1), pointer(vdW2), pointer(sp), C_NULL, C_NULL)
     species = KorgSpecies(length(specs), pointer(mass), pointer(ismol))
     layers = KorgLayers(L, pointer(T), pointer(ne), pointer(n_div_U), pointer(nHI),
                         pointer(nHeI), pointer(nH2), pointer(xi))
@@ -144,30 +157,132 @@ function flatten(ll, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α
     (arrays=arrays, grid=grid, lines=lines, species=species, layers=layers, cntm=cntm)
 end
 
-function Korg.line_absorption!(α::Matrix{Float64},
-                               linelist::AbstractVector{<:Korg.Line{Float64,Float64,Float64,
-                                                                     Float64,Float64,Float64}},
-                               λs::Korg.Wavelengths, temps::AbstractVector{Float64}, nₑ,
-                               n_densities, partition_fns, ξ, α_cntm;
-                               cutoff_threshold=3e-4, tasks_per_thread=1)
-    isempty(linelist) && return nothing                       # src/line_absorption.jl:35-37
-    if !issorted(linelist; by=l -> l.wl)
-        linelist = sort(linelist; by=l -> l.wl)
-    end
-    specs = unique(l.species for l in linelist)
+const Float64Line = Korg.Line{Float64,Float64,Float64,Float64,Float64,Float64}
+
+# The sorted line list and its species table; logs Korg's error for H I (src/line_absorption.jl:45-47).
+function prepare_lines(linelist)
+    ll = issorted(linelist; by=l -> l.wl) ? linelist : sort(linelist; by=l -> l.wl)
+    specs = unique(l.species for l in ll)
     if Korg.species"H I" in specs
         @error "Atomic hydrogen should not be in the linelist. Korg has built-in hydrogen lines."
     end
-    args = flatten(linelist, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α_cntm)
-    GC.@preserve α args begin   # args references every array the structs point into
+    ll, specs
+end
+
+# alpha as the library's column-major n_layers x n_wl block: α itself when it is one (a Matrix,
+# or a view with unit stride down a layer column and columns n_layers apart, e.g.
+# view(α_ref, :, 1) of a vector), else a zeroed scratch matrix whose sum the caller adds back
+# (e.g. the strided view(α, i, :, :) of the molecular precompute).
+is_library_layout(α) = α isa StridedArray && stride(α, 1) == 1 &&
+                       (ndims(α) == 1 || stride(α, 2) == size(α, 1))
+
+function add_lines!(α::AbstractVecOrMat{Float64}, args, cutoff_threshold)
+    direct = is_library_layout(α)
+    buf = direct ? α : zeros(Float64, size(α, 1), size(α, 2))
+    GC.@preserve buf args begin   # args references every array the structs point into
         lock(LOCK) do
             rc = ccall((:korg_line_absorption, LIB), Cint,
                        (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{KorgGrid}, Ref{KorgLines},
                         Ref{KorgSpecies}, Ref{KorgLayers}, Ref{KorgContinuum}, Cdouble, Cint),
-                       context(), α, Cint(0), args.grid, args.lines, args.species, args.layers,
-                       args.cntm, Float64(cutoff_threshold), PRECISION[])
+                       context(), pointer(buf), Cint(0), args.grid, args.lines, args.species,
+                       args.layers, args.cntm, Float64(cutoff_threshold), PRECISION[])
             check(rc)
         end
+    end
+    direct || (α .+= buf)
+    nothing
+end
+
+function Korg.line_absorption!(α::AbstractVecOrMat{Float64},
+                               linelist::AbstractVector{<:Float64Line},
+                               λs::Korg.Wavelengths, temps::AbstractVector{Float64}, nₑ,
+                               n_densities, partition_fns, ξ, α_cntm;
+                               cutoff_threshold=3e-4, tasks_per_thread=1)
+    isempty(linelist) && return nothing                       # src/line_absorption.jl:35-37
+    size(α, 1) == length(temps) && size(α, 2) == length(λs.all_wls) ||
+        throw(DimensionMismatch("α must be n_layers × n_wavelengths"))
+    ll, specs = prepare_lines(linelist)
+    args = flatten(ll, specs, λs, temps, nₑ, n_densities, partition_fns, ξ, α_cntm)
+    add_lines!(α, args, cutoff_threshold)
+end
+
+# ---------------------------------------------------------------------------------------------
+# Forward mode: ForwardDiff.Dual alpha.  The values go through korg_line_absorption (as the
+# Float64 method), the partials through korg_line_absorption_jvp: partial k of alpha receives
+# J x_k, where x_k is partial k of each per-layer input (T, n_e, xi, the vdW perturbers, and n/U,
+# whose Dual arithmetic carries both the densities' and the partition functions' derivatives).
+
+struct KorgLayerTangents
+    T::Ptr{Float64}
+    n_e::Ptr{Float64}
+    xi::Ptr{Float64}
+    n_HI::Ptr{Float64}
+    n_HeI::Ptr{Float64}
+    n_H2::Ptr{Float64}
+    n_div_U::Ptr{Float64}
+end
+
+value_of(x) = ForwardDiff.value(x)          # Duals, possibly nested in arrays
+partial_of(x::ForwardDiff.Dual, k) = ForwardDiff.partials(x, k)
+partial_of(x::Real, k) = 0.0
+
+struct ValuedContinuum{F}   # a continuum callable's value (windows use values only)
+    f::F
+end
+(c::ValuedContinuum)(λ) = value_of(c.f(λ))
+
+function Korg.line_absorption!(α::AbstractVecOrMat{<:ForwardDiff.Dual{Tg,Float64,N}},
+                               linelist::AbstractVector{<:Float64Line},
+                               λs::Korg.Wavelengths, temps, nₑ, n_densities, partition_fns, ξ,
+                               α_cntm; cutoff_threshold=3e-4, tasks_per_thread=1) where {Tg,N}
+    isempty(linelist) && return nothing                       # src/line_absorption.jl:35-37
+    L, nwl = length(temps), length(λs.all_wls)
+    size(α, 1) == L && size(α, 2) == nwl ||
+        throw(DimensionMismatch("α must be n_layers × n_wavelengths"))
+    ll, specs = prepare_lines(linelist)
+    # per-layer inputs as Duals (plain numbers have zero partials)
+    nd = Dict(s => (v isa Number ? fill(v, L) : collect(v)) for (s, v) in n_densities)
+    nU = [nd[s] ./ partition_fns[s].(log.(temps)) for s in specs]   # Dual arithmetic
+    ξv = ξ isa Number ? fill(ξ, L) : collect(ξ)
+    get_nd(code) = haskey(nd, code) ? nd[code] : fill(NaN, L)
+    # values: the Float64 call's arguments
+    vals = Dict(s => Float64.(value_of.(v)) for (s, v) in nd)
+    pf_vals = Dict(s => (lnT -> value_of(partition_fns[s](lnT))) for s in specs)
+    cntm_vals = all(f -> f isa Interpolations.Extrapolation && eltype(f.itp.coefs) == Float64, α_cntm) ?
+                α_cntm : [ValuedContinuum(f) for f in α_cntm]
+    args = flatten(ll, specs, λs, Float64.(value_of.(temps)), Float64.(value_of.(nₑ)), vals,
+                   pf_vals, Float64.(value_of.(ξv)), cntm_vals)
+    αv = zeros(Float64, L, nwl)
+    add_lines!(αv, args, cutoff_threshold)
+    # tangents, one direction per partial; layout of korg_layer_tangents
+    S = length(specs)
+    tT = [Float64[partial_of(x, k) for x in temps] for k in 1:N]
+    tne = [Float64[partial_of(x, k) for x in nₑ] for k in 1:N]
+    txi = [Float64[partial_of(x, k) for x in ξv] for k in 1:N]
+    tHI = [Float64[partial_of(x, k) for x in get_nd(Korg.species"H I")] for k in 1:N]
+    tHeI = [Float64[partial_of(x, k) for x in get_nd(Korg.species"He I")] for k in 1:N]
+    tH2 = [Float64[partial_of(x, k) for x in get_nd(Korg.species"H2")] for k in 1:N]
+    tnU = [reduce(hcat, [Float64[partial_of(x, k) for x in nU[i]] for i in 1:S]) for k in 1:N]
+    dα = [zeros(Float64, L, nwl) for _ in 1:N]
+    GC.@preserve args tT tne txi tHI tHeI tH2 tnU dα begin
+        tangents = [KorgLayerTangents(pointer(tT[k]), pointer(tne[k]), pointer(txi[k]),
+                                      pointer(tHI[k]), pointer(tHeI[k]), pointer(tH2[k]),
+                                      pointer(tnU[k])) for k in 1:N]
+        ptrs = [pointer(d) for d in dα]
+        lock(LOCK) do
+            check(ccall((:korg_line_absorption_jvp, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{Ptr{Float64}}, Cint, Ref{KorgGrid}, Ref{KorgLines},
+                         Ref{KorgSpecies}, Ref{KorgLayers}, Ref{KorgContinuum}, Cdouble, Int64,
+                         Ptr{KorgLayerTangents}),
+                        context(), ptrs, Cint(0), args.grid, args.lines, args.species,
+                        args.layers, args.cntm, Float64(cutoff_threshold), Int64(N), tangents))
+        end
+    end
+    for j in 1:nwl, l in 1:L
+        a = α[l, j]
+        α[l, j] = ForwardDiff.Dual{Tg}(ForwardDiff.value(a) + αv[l, j],
+                                       ForwardDiff.Partials(ntuple(k -> ForwardDiff.partials(a, k) +
+                                                                        dα[k][l, j], N)))
     end
     nothing
 end

@@ -11,8 +11,9 @@ from .wavelengths import Wavelengths  # noqa: F401
 from .linelist import Line, LineList, approximate_gammas, approximate_radiative_gamma  # noqa: F401
 from .problem import Problem, ContinuumGrid  # noqa: F401
 from .line_absorption import (Context, DeviceProblem, FitTarget, FluxTarget, Plan,  # noqa: F401
-                             line_absorption_, line_absorption_vjp_)
+                             line_absorption_, line_absorption_vjp_, line_absorption_jvp_)
 
 __all__ = ["Species", "species", "Wavelengths", "Line", "LineList", "approximate_gammas",
            "approximate_radiative_gamma", "Problem", "ContinuumGrid", "Context", "DeviceProblem",
-           "Plan", "FitTarget", "FluxTarget", "line_absorption_", "line_absorption_vjp_"]
+           "Plan", "FitTarget", "FluxTarget", "line_absorption_", "line_absorption_vjp_",
+           "line_absorption_jvp_"]

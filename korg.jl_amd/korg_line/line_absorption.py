@@ -652,3 +652,50 @@ def line_absorption_vjp_(G, linelist, wls, temps, n_e, n_densities, partition_fn
     return {"temps": dT, "n_e": raw["n_e"], "xi": dxi, "n_densities": dens,
             "n_div_U": raw["n_div_U"]}
 
+
+
+def line_absorption_jvp_(dalphas, linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                         alpha_cntm, tangents, cutoff_threshold=3e-4, ctx=None):
+    """Forward-mode derivative of ``line_absorption_``: what the partials of a ForwardDiff.Dual
+    alpha receive (``src/synthesize.jl:242-247, 305-306`` under ``Fit``).
+
+    ``tangents`` is a list of K dicts, each a direction of the per-layer arguments: ``temps``,
+    ``n_e``, ``xi`` (arrays or scalars) and ``n_densities`` ({species: per-layer tangent});
+    missing entries are zero.  For each k, adds J x_k into ``dalphas[k]`` (column-major
+    (n_layers, n_wl) arrays).  n/U moves with both the densities and the partition functions:
+    d(n/U) = dn/U - n U'(ln T) dT / (U^2 T), U' by a central difference of the callable, as the
+    reverse mode.  Windows held fixed; Float64.
+    """
+    if len(linelist) == 0 or len(tangents) == 0:
+        return None  # src/line_absorption.jl:35-37
+    if not linelist.is_sorted():
+        linelist = linelist.sorted()
+    P = Problem.from_korg_args(linelist, wls, temps, n_e, n_densities, partition_fns, xi,
+                               alpha_cntm, cutoff_threshold)
+    from .species import Species
+    L = P.n_layers
+    temps = np.asarray(temps, dtype=np.float64)
+    lnT = np.log(temps)
+    used = np.unique(linelist.species_index)
+    lay = lambda x: np.broadcast_to(np.asarray(x, dtype=np.float64), (L,)).copy()
+    xs = []
+    for t in tangents:
+        dT = lay(t.get("temps", 0.0))
+        dn = t.get("n_densities", {})
+        dnU = np.zeros_like(P.n_div_U)
+        for s in used:
+            sp = linelist.species[s]
+            n = np.asarray(n_densities[sp], dtype=np.float64)
+            U = np.asarray(partition_fns[sp](lnT), dtype=np.float64)
+            h = 1e-6
+            dU = (np.asarray(partition_fns[sp](lnT + h)) -
+                  np.asarray(partition_fns[sp](lnT - h))) / (2 * h)
+            dnU[s] = lay(dn.get(sp, 0.0)) / U - n * dU * dT / (U * U * temps)
+        x = {"T": dT, "n_e": lay(t.get("n_e", 0.0)), "xi": lay(t.get("xi", 0.0)),
+             "n_div_U": dnU}
+        for code, key in (("H I", "n_HI"), ("He I", "n_HeI"), ("H2", "n_H2")):
+            if Species(code) in dn:
+                x[key] = lay(dn[Species(code)])
+        xs.append(x)
+    (ctx or default_context()).line_absorption_jvp(P, xs, dalphas)
+    return None

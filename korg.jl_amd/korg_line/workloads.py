@@ -137,3 +137,26 @@ def batched_stars(n_stars=256, seed=3, scale=1.0, wl_range=(4000.0, 7000.0)):
         out.append((make_problem(atm, ll, wls, 1.0), atm, wls))
     return out
 
+
+
+def fit_stars(n_stars=64, seed=7, n_lines=100_000, wl_range=(5000.0, 6000.0)):
+    """The batched fit workload: ``n_stars`` stars drawn as ``batched_stars`` draws them (Teff
+    ~ U[4000, 6500] K, log g ~ U[1, 5], [M/H] ~ U[-2.5, 0.5]), all on one grid (5000-6000 A at
+    0.01 A: 100,001 points) with one shared seeded list of ``n_lines`` lines from the solar_small
+    distribution.  Returns (problems, atmospheres, linelist, wavelengths)."""
+    u = lambda stream: uniform(seed, stream, n_stars)
+    teff = 4000.0 + 2500.0 * u(_S_TEFF)
+    logg = 1.0 + 4.0 * u(_S_LOGG)
+    mh = -2.5 + 3.0 * u(_S_MH)
+    wls = Wavelengths(wl_range)
+    ll = synthetic_linelist(seed, n_lines, float(wl_range[0]), float(wl_range[1]), (-4.0, 0.5),
+                            (0.0, 5.0))
+    atms = [Atmosphere.scaled(float(teff[k]), float(logg[k]), float(mh[k])) for k in range(n_stars)]
+    probs = [make_problem(a, ll, wls, 1.0) for a in atms]
+    return probs, atms, ll, wls
+
+
+def fit_reference(atm):
+    """The anchored scheme's reference optical depth and absorption (at 5000 A) of a stand-in
+    atmosphere, per layer."""
+    return 10.0 ** atm.lgTau5, atm.continuum(np.array([5000e-8]))[:, 0]

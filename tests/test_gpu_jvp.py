@@ -136,3 +136,55 @@ def test_jvp_empty_list_and_no_directions(ctx, prob):
     ctx.line_absorption_jvp(E, [_tangent(P, 1)], d)
     assert np.all(d[0] == 3.0)
     assert ctx.line_absorption_jvp(P, []) == []
+
+
+def test_jvp_mirror_with_partition_functions(ctx):
+    """line_absorption_jvp_ (the Dual path of the mirror: n/U moves with the densities and, through
+    U(T), with T) against central differences of the frozen-window device forward, and against
+    line_absorption_vjp_ by the adjoint identity."""
+    from korg_line import Species, Wavelengths, line_absorption_jvp_, line_absorption_vjp_
+    from korg_line.atmosphere import Atmosphere, continuum_grid
+    from korg_line.problem import ContinuumGrid
+    atm = Atmosphere.from_model("sun")
+    wls = Wavelengths((5000, 5030))
+    ll = workloads.synthetic_linelist(14, 150, 4995.0, 5035.0, (-4.0, 0.5), (0.0, 5.0))
+    cg = ContinuumGrid(*continuum_grid(atm, float(wls.all_wls[0]), float(wls.all_wls[-1])))
+    L, n = atm.n_layers, wls.all_wls.size
+    xi = 1e5 * np.ones(L)
+    rng = np.random.default_rng(15)
+    fe = Species("Fe I")
+    x = {"temps": rng.uniform(0.5, 1.5, L) * atm.temps, "n_e": rng.uniform(0.5, 1.5, L) * atm.n_e,
+         "xi": rng.uniform(0.5, 1.5, L) * xi,
+         "n_densities": {fe: rng.uniform(0.5, 1.5, L) * atm.n_densities[fe],
+                         Species("H I"): rng.uniform(0.5, 1.5, L) * atm.n_densities[Species("H I")]}}
+    J = [np.zeros((L, n), order="F")]
+    line_absorption_jvp_(J, ll, wls, atm.temps, atm.n_e, atm.n_densities, atm.partition_fns, xi,
+                         cg, [x], ctx=ctx)
+    J = J[0]
+    # central differences of the device forward with the windows of the unperturbed call
+    P0 = Problem.from_korg_args(ll, wls, atm.temps, atm.n_e, atm.n_densities, atm.partition_fns,
+                                xi, cg)
+    a0 = np.zeros((L, n), order="F")
+    ctx.line_absorption(a0, P0, "fp64")
+    lb, ub = ctx.windows(len(ll))
+
+    def forward(h):
+        nd = dict(atm.n_densities)
+        for sp, d in x["n_densities"].items():
+            nd[sp] = atm.n_densities[sp] + h * d
+        P = Problem.from_korg_args(ll, wls, atm.temps + h * x["temps"], atm.n_e + h * x["n_e"],
+                                   nd, atm.partition_fns, xi + h * x["xi"], cg).with_windows(lb, ub)
+        a = np.zeros((L, n), order="F")
+        ctx.line_absorption(a, P, "fp64")
+        return a
+    h = 1e-6
+    fd = (forward(h) - forward(-h)) / (2 * h)
+    assert np.linalg.norm(J - fd) <= 1e-6 * np.linalg.norm(J), np.linalg.norm(J - fd) / np.linalg.norm(J)
+    G = np.asfortranarray(rng.standard_normal((L, n)))
+    g = line_absorption_vjp_(G, ll, wls, atm.temps, atm.n_e, atm.n_densities, atm.partition_fns,
+                             xi, cg, ctx=ctx)
+    rhs = (float(np.sum(g["temps"] * x["temps"])) + float(np.sum(g["n_e"] * x["n_e"])) +
+           float(np.sum(g["xi"] * x["xi"])) +
+           sum(float(np.sum(g["n_densities"][sp] * d)) for sp, d in x["n_densities"].items()))
+    lhs = float(np.sum(G * J))
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs), (lhs, rhs)
