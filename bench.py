@@ -31,6 +31,11 @@ sys.path.insert(0, os.path.join(ROOT, "korg.jl_amd"))
 
 METRIC = "spectra/s + Voigt evals/s, FP32 vs FP64 GPU at ≤1e-4 flux err, vs CPU"
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X published dense bfloat16 matrix peak
+PEAK_FP32_TFLOPS = 157.3        # MI355X published FP32 vector peak
+DTYPE = "fp32+bf16x3"           # the arithmetic the Float32 form computes in
+PRECISION = ("fp32 exact Hunger evaluation in each line's near zone + far wing as a 4-term "
+             "series on the bfloat16 matrix cores (hi/lo splits of both operands, three products, "
+             "fp32 accumulate) + fp64 setup/window/zone/branch/offset fix-ups")
 
 # Far-wing kernel (k_wing_bf16): the wing profile of a line at layer l and point j is the
 # series sum_k c_k(l) R_j^k, R_j = 1/(lambda_j - lambda_0)^2, k = 1..4.  Algorithmic cost: 2 FMA =
@@ -56,8 +61,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-fraction", type=float, default=0.5,
-                    help="fraction of the lines timed on the CPU oracle (every k-th line)")
+    ap.add_argument("--cpu-fraction", type=float, default=1.0,
+                    help="fraction of the lines timed on the CPU oracle (every k-th line; 1: the "
+                         "whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flux", action="store_true")
     ap.add_argument("--no-hydrogen", action="store_true")
@@ -85,12 +91,13 @@ def main():
         return
 
     import numpy as np
-    from korg_line import Context, DeviceProblem, rt, workloads
+    from korg_line import Context, DeviceProblem, FluxTarget, workloads
     from korg_line.hipmem import synchronize as device_sync
 
     t = time.time()
     P, atm, wls = workloads.solar_wide()
-    log(f"workload built in {time.time() - t:.1f}s: {len(P.lines)} lines, {P.n_wl} points, "
+    t_flatten = time.time() - t
+    log(f"workload built in {t_flatten:.1f}s: {len(P.lines)} lines, {P.n_wl} points, "
         f"{P.n_layers} layers")
     ctx = Context(0)
     D = DeviceProblem(P)
@@ -160,16 +167,24 @@ def main():
     lines32 = z32.numpy().astype(np.float64).T
     m = np.abs(lines64) >= 1e-8 * np.abs(lines64).max()
     alpha_err = float(np.max(np.abs(lines32[m] - lines64[m]) / np.abs(lines64[m])))
+    del lines64, lines32
     flux_err = None
     if not args.no_flux:
-        cntm = np.array([np.interp(wls.all_wls, P.cntm_knots, P.cntm_values[l])
-                         for l in range(P.n_layers)])
-        aref = np.array([np.interp(5000e-8, P.cntm_knots, P.cntm_values[l])
-                         for l in range(P.n_layers)])
-        tau = 10.0 ** atm.lgTau5
-        f64 = rt.normalized_flux(cntm + lines64, cntm, aref, tau, atm.temps, wls.all_wls)
-        f32 = rt.normalized_flux(cntm + lines32, cntm, aref, tau, atm.temps, wls.all_wls)
-        flux_err = float(np.max(np.abs(f32 - f64)))
+        # the whole spectrum on the device: continuum on the grid, lines added, Korg's planar
+        # transfer (korg_continuum_batched, korg_flux_batched), once per form
+        tau, aref = workloads.fit_reference(atm)
+        fn = {}
+        for prec, dt, tune in (("fp64", "float64", tune64), ("fp32", "float32", tune32)):
+            use(tune)
+            tot = D.new_alpha(dt)
+            ctx.continuum_batched([tot], [D])
+            ctx.line_absorption_device(tot, D, prec)
+            ft = FluxTarget(P, tau, aref)
+            ctx.flux_batched([tot], [D], [ft])
+            fn[prec] = ft.flux() / ft.cntm()
+            del tot
+        use(tune32)
+        flux_err = float(np.max(np.abs(fn["fp32"] - fn["fp64"])))
     log(f"alpha max rel err {alpha_err:.3e}, flux max abs err {flux_err}")
 
     wing_ms = float(np.mean([s["profile_ms"] for s in st32]))
@@ -184,8 +199,37 @@ def main():
     achieved = wing_flops / (wing_ms * 1e-3) / 1e12
     issued = MFMA_FLOPS * n_mfma / (wing_ms * 1e-3) / 1e12
     ref_flops = sum(c * f for c, f in zip((nA, nB, nC, nD), REF_FLOPS))
+    # the near zone holds every B and C evaluation; the rest of it is A (counted) and D
+    near_A = sb["n_evals_near_A"]
+    near_D = sb["n_evals_near"] - nB - nC - near_A
+    near_flops = sum(c * f for c, f in zip((near_A, nB, nC, near_D), REF_FLOPS))
+    near_achieved = near_flops / (near_ms * 1e-3) / 1e12
+    traffic = load_traffic()
+    rooflines = {
+        "k_near": {
+            "bound": "valu", "achieved": near_achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": near_achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic.get("k_near"), "kernel_ms": near_ms, "flops_per_launch": near_flops,
+            "evals_by_branch": {"A": near_A, "B": nB, "C": nC, "D": near_D},
+            "note": "exact Hunger evaluations of the near zones, the reference's per-branch op "
+                    "counts (A 19, B 27, C 57, D 28) over the FP32 vector peak; lanes are layers, "
+                    "whose branches differ"},
+        "k_wing_bf16": {
+            "bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_MFMA_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_MFMA_TFLOPS,
+            "traffic": traffic.get("k_wing_bf16"), "kernel_ms": wing_ms,
+            "flops_per_launch": wing_flops, "issued_mfma_tflops": issued,
+            "issued_frac": issued / PEAK_BF16_MFMA_TFLOPS,
+            "note": "4 flops per far-zone evaluation (R, R^2) + 4 per tail evaluation (R^3, R^4) on "
+                    "the bfloat16 matrix cores with 3 split products per term; bound by the "
+                    "vector work that builds the operands"},
+    }
+    dominant = "k_near" if near_ms >= wing_ms else "k_wing_bf16"
+    log(f"k_near {near_ms:.3f} ms ({near_achieved:.1f} TFLOP/s), k_wing_bf16 {wing_ms:.3f} ms "
+        f"({achieved:.1f} TFLOP/s algorithmic)")
 
     hyd = None if args.no_hydrogen else hydrogen_lines_bench(ctx, D, atm)
+    e2e = None if args.no_cpu else end_to_end_host(ctx, P, t_flatten)
 
     cpu = None
     if not args.no_cpu:
@@ -202,12 +246,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
             "workload": "solar_wide: 300,000 seeded lines, 4000-7000 A at 0.01 A (300,001 "
                         "points), 56-layer solar stand-in atmosphere, FP32 alpha resident in HBM",
-            "precision": "fp32 profiles + fp64 setup/window/branch/offset fix-ups",
+            "precision": PRECISION,
             "parallelism": "single GPU",
             "tuning": tune32,
             "step": "one HIP graph replay of the whole pass (setup, binning, profiles)",
@@ -221,27 +265,19 @@ def main():
         "fp32_over_fp64": t64 / t32,
         "alpha_max_rel_err_fp32_vs_fp64": alpha_err,
         "flux_max_abs_err_fp32_vs_fp64": flux_err,
+        "flux_scheme": "normalized flux of continuum + lines by the device's Korg planar transfer "
+                       "(korg_flux_batched: anchored tau, linear source, Korg's E2), both forms",
         "evals_far_wing_mfma": e_far,
         "evals_near_zone_exact": e_near,
         "evals_far_with_tail_terms": e_tail,
         "kernel_ms": {"setup": setup_ms, "binning": bin_ms, "k_wing_bf16": wing_ms,
                       "k_near": near_ms},
         "reference_equivalent_tflops": ref_flops / t32 / 1e12,
-        "roofline": {
-            "bound": "mfma",
-            "kernel": "k_wing_bf16",
-            "achieved": achieved,
-            "peak": PEAK_BF16_MFMA_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_BF16_MFMA_TFLOPS,
-            "traffic": load_traffic(),
-            "flops_per_launch": wing_flops,
-            "issued_mfma_tflops": issued,
-            "issued_frac": issued / PEAK_BF16_MFMA_TFLOPS,
-            "note": "bfloat16 matrix cores with 3 split products per term; the kernel is bound "
-                    "by the vector work that builds the operands (R, its powers, their "
-                    "high/low splits), not by the matrix cores",
-        },
+        "roofline": {"kernel": dominant, **{k: v for k, v in rooflines[dominant].items()
+                                            if k in ("bound", "achieved", "peak", "unit", "frac",
+                                                     "traffic")}},
+        "rooflines": rooflines,
+        "end_to_end_host_f64": e2e,
         "cpu_baseline": cpu,
     }
     if hyd is not None:
@@ -279,7 +315,7 @@ TUNING_CACHE = os.path.join(ROOT, "korg.jl_amd", "korg_line", "tuning_mi355x.txt
 def bench_batched(args):
     """The batched forward over many stars as one graph replay per step."""
     import numpy as np
-    from korg_line import Context, DeviceProblem, FluxTarget, rt, workloads
+    from korg_line import Context, DeviceProblem, FluxTarget, workloads
     from korg_line.hipmem import synchronize as device_sync
 
     t = time.time()
@@ -335,18 +371,23 @@ def bench_batched(args):
     ctx.line_absorption_batched(z64, [dprobs[k] for k in sample], "fp64")
     alpha_err = flux_err = 0.0
     for k, b32, b64 in zip(sample, z32, z64):
-        P, atm, wls = stars[k]
         l64 = b64.numpy().T
         l32 = b32.numpy().astype(np.float64).T
         m = np.abs(l64) >= 1e-8 * np.abs(l64).max()
         if m.any():
             alpha_err = max(alpha_err, float(np.max(np.abs(l32[m] - l64[m]) / np.abs(l64[m]))))
-        cntm = np.array([np.interp(wls.all_wls, P.cntm_knots, P.cntm_values[l]) for l in range(L)])
-        aref = np.array([np.interp(5000e-8, P.cntm_knots, P.cntm_values[l]) for l in range(L)])
-        tau = 10.0 ** atm.lgTau5
-        f64 = rt.normalized_flux(cntm + l64, cntm, aref, tau, atm.temps, wls.all_wls)
-        f32 = rt.normalized_flux(cntm + l32, cntm, aref, tau, atm.temps, wls.all_wls)
-        flux_err = max(flux_err, float(np.max(np.abs(f32 - f64))))
+    # normalized flux of the sample on the device: continuum + lines, Korg's planar transfer
+    sd = [dprobs[k] for k in sample]
+    fts = {}
+    for prec, dt in (("fp64", "float64"), ("fp32", "float32")):
+        tot = [d.new_alpha(dt) for d in sd]
+        ctx.continuum_batched(tot, sd)
+        ctx.line_absorption_batched(tot, sd, prec)
+        fts[prec] = [FluxTarget(stars[k][0], *workloads.fit_reference(stars[k][1])) for k in sample]
+        ctx.flux_batched(tot, sd, fts[prec])
+        del tot
+    for a, b in zip(fts["fp32"], fts["fp64"]):
+        flux_err = max(flux_err, float(np.max(np.abs(a.flux() / a.cntm() - b.flux() / b.cntm()))))
     log(f"  sample of {len(sample)} stars: alpha max rel err {alpha_err:.3e}, flux {flux_err:.3e}")
 
     # the rows on either side of the line opacity, over the whole batch: the continuum on the grid
@@ -381,14 +422,14 @@ def bench_batched(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
             "workload": f"batched: {len(probs)} stars (Teff U[4000,6500], log g U[1,5], "
                         f"[M/H] U[-2.5,0.5]), windows U[50,1000] A in 4000-7000 A at 0.01 A, "
                         f"U[5k,150k] lines each ({n_lines} lines, {n_pts} points), 56 layers, "
                         "FP32 alpha in HBM, one HIP graph replay per step",
-            "precision": "fp32 profiles + fp64 setup/window/branch/offset fix-ups",
+            "precision": PRECISION,
             "parallelism": "single GPU",
             "tuning": tuning,
         },
@@ -489,14 +530,16 @@ def bench_fit(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
             "workload": f"fit step: {n_stars} stars (Teff U[4000,6500], log g U[1,5], [M/H] "
                         "U[-2.5,0.5]), 5000-6000 A at 0.01 A (100,001 points), 100,000 shared "
-                        "lines, 56 layers; per step: line opacity (fp32 + fp64 fix-ups), radiative "
-                        "transfer, chi^2, line-opacity adjoint (far zones: fp32 series moments "
-                        "on the matrix cores; near zones: fp64), one HIP graph replay",
+                        "lines, 56 layers; per step: line opacity, radiative transfer (fp64), "
+                        "chi^2, line-opacity adjoint, one HIP graph replay",
+            "precision": PRECISION + "; adjoint: far zones by fp32 series moments on the matrix "
+                         "cores (v_mfma_f32_16x16x4_f32, fp64 totals per 64 points), near zones "
+                         "by exact fp32 evaluation of H and its partials, fp64 chain rule",
             "parallelism": "single GPU",
             "tuning": tuning,
         },
@@ -582,13 +625,33 @@ def copy_calibration(mib, reps=5):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "traffic_k_wing_bf16_solar_wide.json")
-    if os.path.exists(p):
-        try:
-            return json.load(open(p))["bytes_per_launch"]
-        except Exception:
-            return None
-    return None
+    """HBM bytes per launch of the solar_wide kernels, from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary (scaled by a device copy's readings)."""
+    p = os.path.join(ROOT, "profiles", "traffic_solar_wide.json")
+    try:
+        return {k: v["bytes_per_launch"] for k, v in json.load(open(p))["kernels"].items()}
+    except (OSError, KeyError, ValueError):
+        return {}
+
+
+def end_to_end_host(ctx, P, t_flatten, reps=3):
+    """The call Korg makes (src/synthesize.jl:305-306 into korg_line_absorption): host Float64
+    alpha and host inputs, staged to the device, the Float32 form, alpha copied back -- the
+    PCIe-inclusive rate, next to the host-side flattening of the arguments."""
+    import numpy as np
+    a = np.zeros((P.n_layers, P.n_wl), order="F")
+    ctx.line_absorption(a, P, "fp32")  # warm-up (staging buffers grown)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.line_absorption(a, P, "fp32")
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    log(f"end to end (host f64 alpha, copies included): {ms:.1f} ms per call")
+    return {"ms": ms, "spectra_per_s": 1e3 / ms, "flatten_ms": t_flatten * 1e3,
+            "alpha_bytes_each_way": 8 * P.n_layers * P.n_wl,
+            "includes": "host->device copies of the inputs and of Float64 alpha, the Float32 "
+                        "pass, device->host copy of alpha (korg_line_absorption); flatten_ms: "
+                        "the host building of the flattened arguments (line list defaults, n/U, "
+                        "continuum knots), once"}
 
 
 def cpu_baseline(P, fraction):
@@ -601,9 +664,9 @@ def cpu_baseline(P, fraction):
 
     k = max(1, int(round(1 / fraction)))
     idx = np.arange(0, len(P.lines), k)
-    S = Problem(P.all_wls, P.lines.take(idx), P.species_mass, P.species_is_mol, P.n_div_U,
-                P.temps, P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi, P.cntm_mode, P.cntm_values,
-                P.cntm_knots, P.cutoff_threshold)
+    S = P if k == 1 else Problem(P.all_wls, P.lines.take(idx), P.species_mass, P.species_is_mol,
+                                 P.n_div_U, P.temps, P.n_e, P.n_HI, P.n_HeI, P.n_H2, P.xi,
+                                 P.cntm_mode, P.cntm_values, P.cntm_knots, P.cutoff_threshold)
     threads = min(16, os.cpu_count() or 1)
     a = np.zeros((P.n_layers, P.n_wl), order="F")
     lb, ub, e_full = oracle.line_absorption(a, P, mode=oracle.WINDOWS_ONLY, return_windows=True,
@@ -616,11 +679,13 @@ def cpu_baseline(P, fraction):
     t_full = dt * e_full / e_sample
     log(f"cpu baseline: {len(idx)} lines ({e_sample:.3g} evals) in {dt:.1f}s on {threads} "
         f"threads -> {t_full:.1f}s per spectrum")
+    sample = ("the whole workload (300,000 lines, full grid), Float64, reference thread "
+              "schedule" if k == 1 else
+              f"every {k}th line of the same 300,000-line list ({len(idx)} lines, "
+              f"{e_sample:.3g} of {e_full:.3g} evaluations), full grid, Float64, reference "
+              f"thread schedule; time scaled by the evaluation ratio")
     return {"value": 1.0 / t_full, "unit": "spectra/s", "cores": threads, "kind": "port",
-            "evals_per_s": e_sample / dt,
-            "sample": f"every {k}th line of the same 300,000-line list ({len(idx)} lines, "
-                      f"{e_sample:.3g} of {e_full:.3g} evaluations), full grid, Float64, "
-                      f"reference thread schedule; time scaled by the evaluation ratio"}
+            "evals_per_s": e_sample / dt, "seconds": dt, "sample": sample}
 
 
 if __name__ == "__main__":

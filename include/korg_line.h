@@ -143,6 +143,9 @@ typedef struct {
      * series terms exceed 1e-5 of the leading term (the bfloat16 far wing includes them
      * everywhere) */
     double n_evals_tail;
+    /* Float32 form with korg_tuning.count_branches: near-zone evaluations in Hunger branch A
+     * (the near zone holds every branch-B and -C evaluation; its remaining evaluations are D) */
+    double n_evals_near_A;
 } korg_stats;
 
 typedef struct korg_ctx korg_ctx;

@@ -201,7 +201,8 @@ struct LayerTab {
 // [2] lines used, [3] error flags, [4] near-zone reach in points, [5] near queue,
 // [6..9] evaluations by Hunger branch, [10] near-zone evaluations, [11] far-wing MFMAs per layer
 // block, [12] tail-zone evaluations, [13] entries of the per-tile line lists, [14] segments of the
-// adjoint, [16..23] the adjoint moments' queues (one per part of the group list).
+// adjoint, [15] branch-A evaluations inside near zones (with branch counting), [16..23] the
+// adjoint moments' queues (one per part of the group list).
 // Error flags: 1 lines not sorted, 2 species index out of range, 4 the per-tile line lists need
 // more entries than their buffer holds.  With any flag set no kernel writes alpha.
 constexpr int N_COUNTERS = 24;
@@ -603,6 +604,12 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         const bool small_a = my_a <= 0.2;
         atomicAdd(&A.counters[small_a ? 6 : 9], (unsigned long long)wing_n);
         if (core_n) atomicAdd(&A.counters[small_a ? 7 : 8], (unsigned long long)core_n);
+        // branch-A evaluations inside the near zone (which holds every core point): the rest of
+        // the near zone past the core of a layer with a <= 0.2
+        if (F32 && small_a && clo <= chi) {
+            const int64_t near_wing = (chi - clo + 1) - core_n;
+            if (near_wing > 0) atomicAdd(&A.counters[15], (unsigned long long)near_wing);
+        }
     }
     int cls = N_CLASSES;
     if (lb <= ub) {
@@ -3763,6 +3770,7 @@ int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, in
     ctx->stats.n_wing_mfma = !f32 ? 0.0 : WING_BF16 ? (double)c[13] * n_lblocks * 1.5 * nb
                                                      : (double)c[11] * n_lblocks;
     ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
+    ctx->stats.n_evals_near_A = f32 ? (double)c[15] : 0.0;
     return KORG_OK;
 }
 

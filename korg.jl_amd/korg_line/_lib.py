@@ -61,7 +61,7 @@ class KorgStats(ctypes.Structure):
                 ("profile_ms", ctypes.c_double), ("core_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4),
                 ("n_evals_near", ctypes.c_double), ("n_wing_mfma", ctypes.c_double),
-                ("n_evals_tail", ctypes.c_double)]
+                ("n_evals_tail", ctypes.c_double), ("n_evals_near_A", ctypes.c_double)]
 
 
 class KorgLayerGrads(ctypes.Structure):

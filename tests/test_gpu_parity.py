@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle
 from korg_line import Context, DeviceProblem, Line, LineList, Problem, Species, Wavelengths
-from korg_line import _lib, rt, workloads
+from korg_line import _lib, workloads
 from korg_line.atmosphere import Atmosphere, continuum_grid
 from korg_line.problem import ContinuumGrid
 
@@ -84,14 +84,13 @@ def test_solar_small_fp64_and_fp32(ctx):
     P, atm, wls = workloads.solar_small()
     ref = check_fp64(ctx, P)
     got, err = check_fp32(ctx, P, ref)
-    # normalized flux: continuum on the fine grid + lines, anchored at the 5000 A continuum
+    # normalized flux: continuum on the fine grid + lines, by the oracle's Korg planar transfer
     cntm = np.array([np.interp(wls.all_wls, P.cntm_knots, P.cntm_values[l])
                      for l in range(P.n_layers)])
-    aref = np.interp(5000e-8, P.cntm_knots, P.cntm_values[0]) * 0 + \
-        np.array([np.interp(5000e-8, P.cntm_knots, P.cntm_values[l]) for l in range(P.n_layers)])
-    tau = 10.0 ** atm.lgTau5
-    f_ref = rt.normalized_flux(cntm + ref, cntm, aref, tau, atm.temps, wls.all_wls)
-    f_got = rt.normalized_flux(cntm + got, cntm, aref, tau, atm.temps, wls.all_wls)
+    tau, aref = workloads.fit_reference(atm)
+    fc = oracle.rt_flux(cntm, wls.all_wls, atm.temps, tau, aref)
+    f_ref = oracle.rt_flux(cntm + ref, wls.all_wls, atm.temps, tau, aref) / fc
+    f_got = oracle.rt_flux(cntm + got, wls.all_wls, atm.temps, tau, aref) / fc
     assert np.max(np.abs(f_got - f_ref)) <= FLUX_ATOL
     assert f_ref.min() < 0.5  # strong lines present: the comparison is not vacuous
 
