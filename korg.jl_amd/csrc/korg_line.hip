@@ -1768,6 +1768,9 @@ constexpr int VJP_CHUNK = 2048;
 constexpr int VJP_SEG = 64;
 // Points whose loads are in flight together: 8 for the Float32 evaluation, 2 for the Float64
 // one (whose registers allow no more at two waves per SIMD).
+#ifndef KORG_ADJ_NEAR_F64
+#define KORG_ADJ_NEAR_F64 0
+#endif
 template <bool SERIES>
 constexpr int vjp_batch() { return SERIES ? 8 : 2; }
 
@@ -1973,7 +1976,7 @@ void k_vjp(VjpArgs A) {
                         if (jb + u >= j1) break;
                         const double g = gv[u];
                         const double v = fabs(lv[u] - lam0) * inv;
-                        if (SERIES) {
+                        if (SERIES && !KORG_ADJ_NEAR_F64) {
                             float Ha, Hv;
                             const float H = voigt_hjerting_grad_f32(a, v, Ha, Hv);
                             SH += g * H;

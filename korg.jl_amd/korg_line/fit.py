@@ -1,35 +1,40 @@
-"""Gradient of a spectral-fit objective through the device line opacity and its adjoint.
+"""Gradient of a spectral-fit objective through the device line opacity, transfer and adjoint.
 
 The chain of a fit: stellar parameters theta -> per-layer atmosphere inputs (T, n_e, n_s / U_s,
-xi, perturber densities) -> line opacity alpha_lines (the device) + continuum -> normalized flux
--> chi^2 = sum ((F - data) / sigma)^2.  ``StarModel.chi2_and_grad`` computes d chi^2 / d theta with
+xi, perturber densities) -> line opacity alpha_lines + continuum -> normalized flux (Korg's planar
+transfer) -> chi^2 = sum ((F - data) / sigma)^2.  ``StarModel.chi2_and_grad`` computes
+d chi^2 / d theta with
 
-* the line-opacity path by the device adjoint: G = d chi^2 / d alpha_lines -> dL/d(per-layer
-  inputs) (``korg_line_absorption_vjp``), then chained to theta through the atmosphere's
-  per-layer inputs, whose derivatives are central differences of the stand-in atmosphere (cheap:
-  no line opacity is evaluated);
-* everything else -- the continuum, the reference opacity and the Planck source of the flux,
-  with alpha_lines held fixed -- by central differences (no line opacity either).
+* the line-opacity path on the device: one fit step (``korg_fit_gradient_batched``) gives chi^2
+  and d chi^2 / d(per-layer inputs) through alpha_lines -- the transfer's reverse mode gives
+  G = d chi^2 / d alpha_lines and the line-opacity adjoint turns it into per-layer gradients --
+  chained to theta through the atmosphere's per-layer inputs, whose derivatives are central
+  differences of the stand-in atmosphere (cheap: no line opacity is evaluated);
+* everything else -- the continuum, the reference opacity and optical depth, and the Planck
+  source, with alpha_lines held fixed -- by central differences of the device transfer
+  (``korg_flux_batched``; no line opacity either).
 
-d F / d alpha (the radiative transfer, a host-side stand-in) is taken by central differences per
-layer, all wavelengths at once (F at a wavelength depends only on that wavelength's column).
-
-The reference obtains the same gradient with ForwardDiff through ``synthesize``
-(``test/autodiff.jl:23-45`` pattern); here the atmosphere and chemistry are the stand-ins of
-``korg_line.atmosphere``, so only the line-opacity part is the reference's arithmetic.
+Windows: the reference's windows are integer functions of the inputs, and ForwardDiff (which Korg's
+Fit uses, ``src/Fit/fit_via_synthesis.jl:345-355``) carries no derivative through them.  Passing
+``windows=(lb, ub)`` freezes them (``Problem.with_windows``), which makes chi^2 a smooth function
+of theta -- the function whose derivative the chain computes -- so that central differences of it
+check the gradient (``test/autodiff.jl:23-45`` pattern).  The atmosphere and chemistry are the
+stand-ins of ``korg_line.atmosphere``; only the line opacity and the transfer are the reference's
+arithmetic.
 """
 import numpy as np
 
-from . import rt
 from .atmosphere import Atmosphere, continuum_grid
+from .hipmem import DeviceArray
+from .line_absorption import DeviceProblem, FitTarget, FluxTarget
 from .problem import ContinuumGrid, Problem
-from .species import Species
 
 PARAMS = ("Teff", "logg", "M_H", "vmic", "Fe_H", "Ti_H", "Ni_H", "Ca_H")
-# relative (Teff, vmic) or absolute (log g and abundances, in dex) central-difference steps
-_STEP = {"Teff": ("rel", 1e-5), "logg": ("abs", 1e-5), "M_H": ("abs", 1e-5),
-         "vmic": ("rel", 1e-5), "Fe_H": ("abs", 1e-5), "Ti_H": ("abs", 1e-5),
-         "Ni_H": ("abs", 1e-5), "Ca_H": ("abs", 1e-5)}
+# relative (Teff, vmic) or absolute (log g and abundances, in dex) central-difference steps of the
+# stand-in atmosphere (smooth apart from its piecewise-linear partition functions)
+_STEP = {"Teff": ("rel", 1e-6), "logg": ("abs", 1e-6), "M_H": ("abs", 1e-6),
+         "vmic": ("rel", 1e-6), "Fe_H": ("abs", 1e-6), "Ti_H": ("abs", 1e-6),
+         "Ni_H": ("abs", 1e-6), "Ca_H": ("abs", 1e-6)}
 
 
 def _step(name, x):
@@ -46,6 +51,7 @@ class StarModel:
         self.wl = wls.all_wls
         self.ctx = ctx
         self.precision = precision
+        self.dtype = "float64" if precision == "fp64" else "float32"
         self.cutoff = cutoff_threshold
 
     # ------------------------------------------------------------------ inputs of the line opacity
@@ -54,88 +60,112 @@ class StarModel:
         ab = {"Fe": t["Fe_H"], "Ti": t["Ti_H"], "Ni": t["Ni_H"], "Ca": t["Ca_H"]}
         return Atmosphere.scaled(t["Teff"], t["logg"], t["M_H"], abundances=ab), t["vmic"] * 1e5
 
-    def problem(self, th):
+    def problem(self, th, windows=None):
         atm, xi = self.atmosphere(th)
         knots, vals = continuum_grid(atm, float(self.wl[0]), float(self.wl[-1]))
         P = Problem.from_korg_args(self.ll, self.wls, atm.temps, atm.n_e, atm.n_densities,
                                    atm.partition_fns, xi, ContinuumGrid(knots, vals),
                                    self.cutoff)
-        return atm, P, knots, vals
+        if windows is not None:
+            P = P.with_windows(*windows)
+        return atm, P
 
     @staticmethod
     def layer_inputs(P):
         """The per-layer inputs the adjoint differentiates, as one flat vector."""
         return np.concatenate([P.temps, P.n_e, P.xi, P.n_HI, P.n_HeI, P.n_H2, P.n_div_U.ravel()])
 
+    @staticmethod
+    def anchor(atm):
+        """The anchored scheme's reference optical depth and absorption (5000 A) per layer."""
+        return 10.0 ** atm.lgTau5, atm.continuum(np.array([5000e-8]))[:, 0]
+
     # ------------------------------------------------------------------ forward pieces
-    def line_opacity(self, P):
-        a = np.zeros((P.n_layers, P.n_wl), order="F")
-        self.ctx.line_absorption(a, P, self.precision)
-        return a
+    def windows(self, th):
+        """The windows (1-based lb, ub per line) the line opacity computes at theta."""
+        _, P = self.problem(th)
+        D = DeviceProblem(P)
+        self.ctx.line_absorption_device(D.new_alpha(self.dtype), D, self.precision)
+        return self.ctx.windows(len(self.ll))
 
-    def _rt_inputs(self, atm, knots, vals):
-        cntm = np.array([np.interp(self.wl, knots, vals[l]) for l in range(atm.n_layers)])
-        aref = atm.continuum(np.array([5000e-8]))[:, 0]
-        return cntm, aref, 10.0 ** atm.lgTau5, atm.temps
+    def _normalized_flux(self, atm, D, alpha):
+        ft = FluxTarget(D.problem, *self.anchor(atm))
+        self.ctx.flux_batched([alpha], [D], [ft])
+        return ft.flux() / ft.cntm()
 
-    def flux_from(self, alpha_lines, atm, knots, vals):
-        cntm, aref, tau, T = self._rt_inputs(atm, knots, vals)
-        return rt.normalized_flux(cntm + alpha_lines, cntm, aref, tau, T, self.wl)
+    def flux(self, th, windows=None):
+        """Normalized flux at theta: continuum on the grid + lines, Korg's planar transfer, all
+        on the device."""
+        atm, P = self.problem(th, windows)
+        D = DeviceProblem(P)
+        a = D.new_alpha(self.dtype)
+        self.ctx.continuum_batched([a], [D])
+        self.ctx.line_absorption_device(a, D, self.precision)
+        return self._normalized_flux(atm, D, a)
 
-    def flux(self, th):
-        atm, P, knots, vals = self.problem(th)
-        return self.flux_from(self.line_opacity(P), atm, knots, vals)
-
-    def chi2(self, th, data, sigma):
-        r = (self.flux(th) - data) / sigma
+    def chi2(self, th, data, sigma, windows=None):
+        r = (self.flux(th, windows) - data) / sigma
         return float(np.sum(r * r))
 
     # ------------------------------------------------------------------ gradient
-    def chi2_and_grad(self, th, data, sigma):
+    def chi2_and_grad(self, th, data, sigma, windows=None):
+        """chi^2 at theta and d chi^2 / d theta (the windows held at ``windows`` when given, else
+        at theta's own)."""
         th = np.asarray(th, dtype=np.float64)
-        atm, P, knots, vals = self.problem(th)
-        alpha_lines = self.line_opacity(P)
-        cntm, aref, tau, T = self._rt_inputs(atm, knots, vals)
-        Fc = rt.flux(cntm, aref, tau, T, self.wl)
-        Ft = rt.flux(cntm + alpha_lines, aref, tau, T, self.wl)
-        F = Ft / Fc
-        r = (F - data) / sigma
-        chi2 = float(np.sum(r * r))
-        dchi_dF = 2.0 * r / sigma
-        # G = d chi2 / d alpha_lines: dF/d alpha by central differences, one layer at a time
-        tot = cntm + alpha_lines
-        G = np.zeros_like(tot)
-        for l in range(tot.shape[0]):
-            h = 1e-6 * tot[l]
-            up, dn = tot.copy(), tot.copy()
-            up[l] += h
-            dn[l] -= h
-            dF = (rt.flux(up, aref, tau, T, self.wl) - rt.flux(dn, aref, tau, T, self.wl)) / (2 * h)
-            G[l] = dchi_dF * dF / Fc
-        # line-opacity path: the adjoint, then d(per-layer inputs)/d theta
-        raw = self.ctx.line_absorption_vjp(np.asfortranarray(G), P)
+        atm, P = self.problem(th, windows)
+        D = DeviceProblem(P)
+        a = D.new_alpha(self.dtype)
+        tg = FitTarget(P, *self.anchor(atm), data, sigma)
+        self.ctx.fit_gradient_batched([a], [D], [tg], self.precision)
+        chi2 = tg.chi2()
+        raw = tg.grads()
+        alpha_lines = a.numpy().astype(np.float64)  # (n_wl, n_layers): the device layout
         g_inputs = np.concatenate([raw["T"], raw["n_e"], raw["xi"], raw["n_HI"], raw["n_HeI"],
                                    raw["n_H2"], raw["n_div_U"].ravel()])
         grad = np.zeros(len(PARAMS))
         for k, name in enumerate(PARAMS):
             h = _step(name, th[k])
-            tp, tm = th.copy(), th.copy()
-            tp[k] += h
-            tm[k] -= h
-            _, Pp, kp, vp = self.problem(tp)
-            atm_p, _ = self.atmosphere(tp)
-            _, Pm, km, vm = self.problem(tm)
-            atm_m, _ = self.atmosphere(tm)
-            d_inputs = (self.layer_inputs(Pp) - self.layer_inputs(Pm)) / (2 * h)
+            # the line path: the adjoint's gradients along d(per-layer inputs)/d theta
+            plus, minus = th.copy(), th.copy()
+            plus[k] += h
+            minus[k] -= h
+            d_inputs = (self.layer_inputs(self.problem(plus, windows)[1]) -
+                        self.layer_inputs(self.problem(minus, windows)[1])) / (2 * h)
             ok = np.isfinite(d_inputs) & np.isfinite(g_inputs)
-            grad_lines = float(np.sum(g_inputs[ok] * d_inputs[ok]))
-            # the rest of the chain, alpha_lines held fixed
-            Fp = self.flux_from(alpha_lines, atm_p, kp, vp)
-            Fm = self.flux_from(alpha_lines, atm_m, km, vm)
-            c_p = float(np.sum(((Fp - data) / sigma) ** 2))
-            c_m = float(np.sum(((Fm - data) / sigma) ** 2))
-            grad[k] = grad_lines + (c_p - c_m) / (2 * h)
+            grad[k] = float(np.sum(g_inputs[ok] * d_inputs[ok])) + \
+                self._rest_derivative(th, k, h, windows, alpha_lines, data, sigma)
         return chi2, grad
+
+    def _chi2_lines_fixed(self, th, windows, alpha_lines, data, sigma):
+        """chi^2 with alpha_lines held fixed and the continuum, anchor and source at theta."""
+        atm, P = self.problem(th, windows)
+        D = DeviceProblem(P)
+        tot = D.new_alpha("float64")
+        self.ctx.continuum_batched([tot], [D])
+        tot = DeviceArray.from_numpy(tot.numpy() + alpha_lines)
+        F = self._normalized_flux(atm, D, tot)
+        return float(np.sum(((F - data) / sigma) ** 2))
+
+    def _rest_derivative(self, th, k, h, windows, alpha_lines, data, sigma, rtol=3e-7):
+        """d chi^2 / d theta_k with alpha_lines held fixed, by central differences.  Korg's E2
+        approximation is piecewise (src/RadiativeTransfer/RadiativeTransfer.jl:408-480) and jumps
+        by up to 1.6% at its piece boundaries, so a step that moves an optical depth across one
+        is off by the jump over the step; derivatives have no such term.  The step is divided by
+        3 until two successive differences agree to rtol: both are then free of crossings."""
+        def fd(step):
+            plus, minus = th.copy(), th.copy()
+            plus[k] += step
+            minus[k] -= step
+            return (self._chi2_lines_fixed(plus, windows, alpha_lines, data, sigma) -
+                    self._chi2_lines_fixed(minus, windows, alpha_lines, data, sigma)) / (2 * step)
+        prev = fd(h)
+        for _ in range(4):
+            h /= 3.0
+            cur = fd(h)
+            if abs(cur - prev) <= rtol * max(abs(cur), abs(prev)):
+                return cur
+            prev = cur
+        return prev
 
 
 def noisy_data(model, th, sigma=0.01, seed=1):

@@ -148,8 +148,25 @@ def test_fit_plan_equals_direct_and_reads_new_inputs(ctx):
                 assert np.array_equal(t2.grads()[q], t3.grads()[q]), q
 
 
-def test_fit_step_fp32_forward_close_to_fp64(ctx):
+@pytest.mark.parametrize("residuals,tol", [("systematic", 1e-5), ("noise", 1e-4)])
+def test_fit_step_fp32_forward_close_to_fp64(ctx, residuals, tol):
+    """The Float32 step against the Float64 step: flux within 1e-4, chi^2 within 1e-6, and each
+    quantity's per-layer gradient (2-norm) within ``tol``.
+
+    With systematic residuals (data: the lines 10% deeper, plus noise -- an iterate away from the
+    optimum) the gradient meets the 1e-5 of the north star.  With pure noise residuals (data: the
+    model itself with 2% noise, the gradient at the optimum) the gradient is a sum of terms of random
+    sign that cancel, and its relative error is set by the Float32 forward's flux error over sigma
+    (about 2e-7 / 1e-2) -- whatever the adjoint's precision (a Float64 near-zone adjoint or tighter
+    series radii leave it unchanged) -- so the bound stated for it is 1e-4."""
     P, atm, data, sigma = make_star(seed=10, n_lines=600)
+    if residuals == "systematic":
+        D = DeviceProblem(P)
+        t = target(P, atm, data, sigma)
+        ctx.fit_gradient_batched([D.new_alpha("float64")], [D], [t], "fp64")
+        rng = np.random.default_rng(10)
+        f = t.flux()
+        data = 1 - 1.1 * (1 - f) + sigma * rng.standard_normal(f.size)
     res = {}
     for prec, dt in (("fp64", "float64"), ("fp32", "float32")):
         D = DeviceProblem(P)
@@ -160,10 +177,11 @@ def test_fit_step_fp32_forward_close_to_fp64(ctx):
     c64, f64, g64 = res["fp64"]
     c32, f32, g32 = res["fp32"]
     assert np.max(np.abs(f32 - f64)) <= 1e-4  # normalized flux
-    assert c32 == pytest.approx(c64, rel=1e-3)
-    for q in ("T", "n_e", "xi", "n_div_U"):
-        scale = np.max(np.abs(g64[q]))
-        assert np.max(np.abs(g32[q] - g64[q])) <= 1e-3 * scale, q
+    assert c32 == pytest.approx(c64, rel=1e-6)
+    for q in ("T", "n_e", "xi", "n_HI", "n_div_U"):
+        err = np.linalg.norm(g32[q] - g64[q]) / np.linalg.norm(g64[q])
+        print(f"fit step fp32 vs fp64 ({residuals} residuals): {q} {err:.2e}")
+        assert err <= tol, (q, err)
 
 
 def test_fit_step_gradient_matches_central_differences(ctx):
