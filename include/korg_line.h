@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 11
+#define KORG_LINE_ABI_VERSION 12
 
 enum korg_status {
     KORG_OK = 0,
@@ -499,6 +499,42 @@ typedef struct {
 int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype,
                                   const korg_grid* grid, const korg_hlayers* layers,
                                   double window_size, int use_MHD, int synchronize);
+
+/*
+ * Precomputed molecular cross-sections (Korg's MolecularCrossSection,
+ * src/molecular_cross_sections.jl:3-8): the absorption cross-section of one molecular species on a
+ * (vmic, log10 T, wavelength) grid, built by the precompute (:48-83) from line_absorption! --
+ * every call of which this library can run (korg_line_absorption_batched over the vmic knots, the
+ * log T knots as layers).  sigma is laid out lambda-row by lambda-row within each vmic knot:
+ * sigma[(iv * n_wl + j) * n_logT + iT] (Korg's itp.coefs[iv, iT, j] permuted).
+ */
+typedef struct {
+    int32_t n_vmic, n_logT;   /* at least 2 knots each */
+    int64_t n_wl;             /* at least 2 */
+    const double* vmic;       /* [n_vmic] km/s, ascending */
+    const double* logT;       /* [n_logT] log10(T / K), ascending */
+    const double* wls;        /* [n_wl] cm, ascending: the table's wavelengths */
+    const double* sigma;      /* cm^2 */
+    const double* n_species;  /* [n_layers]: the species' number density in each layer */
+} korg_mol_xsection;
+
+/*
+ * interpolate_molecular_cross_sections! (src/molecular_cross_sections.jl:99-114): for every table
+ * and layer l, alpha[l, j] += sigma(vmic[l], log10 temps[l], lambda_j) * n_species[l], sigma
+ * linear in each gridded dimension and zero outside the table (extrapolate(..., 0.0), :76-78).
+ * vmic in km/s per layer.  alpha column-major [n_layers x n_wl], f64 or f32.  No tables: alpha
+ * untouched.  The host form takes host pointers and is synchronous; the _device form takes device
+ * pointers (tables: a host array of structs of device pointers).  Float64 arithmetic.
+ */
+int korg_molecular_cross_sections(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                  const korg_grid* grid, int32_t n_layers, const double* temps,
+                                  const double* vmic, int64_t n_tables,
+                                  const korg_mol_xsection* tables);
+int korg_molecular_cross_sections_device(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                         const korg_grid* grid, int32_t n_layers,
+                                         const double* temps, const double* vmic,
+                                         int64_t n_tables, const korg_mol_xsection* tables,
+                                         int synchronize);
 
 #ifdef __cplusplus
 }

@@ -3064,6 +3064,86 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------
+// Molecular cross-sections (src/molecular_cross_sections.jl:99-114): alpha[l, j] += sigma(vmic_l,
+// log10 T_l, lambda_j) n_l for each precomputed table, sigma interpolated linearly in each of the
+// three gridded dimensions and zero outside the table (extrapolate(..., 0.0), :76-78).  A block
+// covers MOL_TILE grid points x 64 layers; lane = layer.  The table's lambda interval of each
+// point is found once per block (lane = point) and shared through LDS.
+// ---------------------------------------------------------------------------------------------
+constexpr int MOL_TILE = 64;
+
+struct MolTable {  // device copy of korg_mol_xsection
+    int n_vmic, n_logT;
+    int64_t n_wl;
+    const double *vmic, *logT, *wls, *sigma, *n_species;
+};
+
+// Interval i (0 .. n-2) of ascending knots x holding v and its weight (v - x_i)/(x_{i+1} - x_i);
+// false when v is outside [x_0, x_{n-1}] (or NaN).
+__device__ inline bool gridded_cell(const double* x, int64_t n, double v, int64_t& i, double& w) {
+    if (!(v >= x[0] && v <= x[n - 1])) return false;
+    int64_t lo = 0, hi = n - 1;  // x[lo] <= v, and v < x[hi] unless v is the last knot
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (x[mid] <= v) lo = mid; else hi = mid;
+    }
+    i = lo;
+    w = (v - x[lo]) / (x[lo + 1] - x[lo]);
+    return true;
+}
+
+__global__ __launch_bounds__(64) void k_mol_interp(int L, int64_t n_wl, const double* __restrict__ wls,
+                                                   const double* __restrict__ temps,
+                                                   const double* __restrict__ vmic, int n_tables,
+                                                   const MolTable* __restrict__ tabs,
+                                                   void* __restrict__ alpha, int alpha_f32) {
+    __shared__ int64_t s_i[MOL_TILE];
+    __shared__ double s_w[MOL_TILE];
+    const int lane = threadIdx.x;
+    const int l = blockIdx.y * 64 + lane;
+    const int64_t j0 = (int64_t)blockIdx.x * MOL_TILE;
+    const int np = (int)min((int64_t)MOL_TILE, n_wl - j0);
+    for (int k = 0; k < n_tables; ++k) {
+        const MolTable tb = tabs[k];
+        __syncthreads();  // the previous table's cells are no longer read
+        {
+            int64_t i = -1;
+            double w = 0.0;
+            if (lane < np && !gridded_cell(tb.wls, tb.n_wl, wls[j0 + lane], i, w)) i = -1;
+            s_i[lane] = i;
+            s_w[lane] = w;
+        }
+        __syncthreads();
+        if (l >= L) continue;
+        int64_t iv = 0, iT = 0;
+        double wv = 0.0, wT = 0.0;
+        const bool in = gridded_cell(tb.vmic, tb.n_vmic, vmic[l], iv, wv) &&
+                        gridded_cell(tb.logT, tb.n_logT, log10(temps[l]), iT, wT);
+        if (!in) continue;
+        const double n = tb.n_species[l];
+        const int64_t nT = tb.n_logT;
+        // the four (vmic, log T) corners of this layer, as offsets into a lambda row
+        const double* r0 = tb.sigma + (iv * tb.n_wl) * nT + iT;        // vmic knot iv
+        const double* r1 = tb.sigma + ((iv + 1) * tb.n_wl) * nT + iT;  // vmic knot iv + 1
+        for (int t = 0; t < np; ++t) {
+            const int64_t i = s_i[t];
+            if (i < 0) continue;  // wavefront-uniform: the point is outside the table
+            const double w = s_w[t];
+            auto corner = [&](const double* r, int64_t jj) {  // linear in log T at lambda knot jj
+                const double* c = r + jj * nT;
+                return (1.0 - wT) * c[0] + wT * c[1];
+            };
+            const double a0 = (1.0 - w) * corner(r0, i) + w * corner(r0, i + 1);
+            const double a1 = (1.0 - w) * corner(r1, i) + w * corner(r1, i + 1);
+            const double v = ((1.0 - wv) * a0 + wv * a1) * n;
+            const int64_t idx = (int64_t)l + (int64_t)L * (j0 + t);
+            if (alpha_f32) ((float*)alpha)[idx] += (float)v;
+            else ((double*)alpha)[idx] += v;
+        }
+    }
+}
+
 // ============================================================================================
 // host side
 // ============================================================================================
@@ -3164,6 +3244,8 @@ struct korg_ctx {
         h_knots, h_cntm;
     // staging of the host-memory forward-mode entry point: tangents and d alpha
     DevBuf h_tan, h_dalpha;
+    // molecular cross-sections: the device table structs, and the host entry point's staging
+    DevBuf mol_tabs, h_mol;
     // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
     DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices, hl_hull;
     int hl_ntrans = 0, hl_max_dnu = 0;
@@ -4711,7 +4793,8 @@ void korg_destroy(korg_ctx* ctx) {
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
         if (b.p) (void)hipFree(b.p);
-    for (DevBuf* b : {&ctx->h_win[0], &ctx->h_win[1], &ctx->h_tan, &ctx->h_dalpha})
+    for (DevBuf* b : {&ctx->h_win[0], &ctx->h_win[1], &ctx->h_tan, &ctx->h_dalpha, &ctx->mol_tabs,
+                      &ctx->h_mol})
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_layers)
         if (b.p) (void)hipFree(b.p);
@@ -5257,6 +5340,125 @@ int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, c
                        ctx->stream, a);
     HIP_TRY(ctx, hipGetLastError());
     if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return KORG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_mol_args(korg_ctx* ctx, const void* alpha, int alpha_dtype, const korg_grid* grid,
+                   int32_t n_layers, const double* temps, const double* vmic, int64_t n_tables,
+                   const korg_mol_xsection* tables) {
+    if (!alpha || !grid || !grid->all_wls || grid->n_wl < 1)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha and a non-empty grid are required");
+    if (alpha_dtype != KORG_DTYPE_F64 && alpha_dtype != KORG_DTYPE_F32)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "alpha_dtype must be KORG_DTYPE_F64 or _F32");
+    if (n_layers < 1 || n_layers > 65535 * 64 || !temps || !vmic)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "molecular cross-sections: layers missing");
+    if (n_tables < 0 || n_tables > 4096 || (n_tables > 0 && !tables))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "n_tables out of [0, 4096] or tables null");
+    for (int64_t k = 0; k < n_tables; ++k) {
+        const korg_mol_xsection& t = tables[k];
+        if (t.n_vmic < 2 || t.n_logT < 2 || t.n_wl < 2 || !t.vmic || !t.logT || !t.wls ||
+            !t.sigma || !t.n_species)
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "table %lld: at least 2 knots per dimension and every array", (long long)k);
+    }
+    return KORG_OK;
+}
+
+// The interpolation on device pointers (tables: a host array of structs of device pointers).
+int run_mol(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* grid, int32_t L,
+            const double* temps, const double* vmic, int64_t n_tables,
+            const korg_mol_xsection* tables) {
+    if (n_tables == 0) return KORG_OK;  // src/molecular_cross_sections.jl:102-104
+    std::vector<MolTable> h((size_t)n_tables);
+    for (int64_t k = 0; k < n_tables; ++k) {
+        const korg_mol_xsection& t = tables[k];
+        h[k] = MolTable{t.n_vmic, t.n_logT, t.n_wl, t.vmic, t.logT, t.wls, t.sigma, t.n_species};
+    }
+    TRY(grow(ctx, ctx->mol_tabs, sizeof(MolTable) * h.size()));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->mol_tabs.p, h.data(), sizeof(MolTable) * h.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+    const dim3 g((unsigned)((grid->n_wl + MOL_TILE - 1) / MOL_TILE), (unsigned)((L + 63) / 64));
+    hipLaunchKernelGGL(k_mol_interp, g, dim3(64), 0, ctx->stream, (int)L, grid->n_wl,
+                       grid->all_wls, temps, vmic, (int)n_tables,
+                       (const MolTable*)ctx->mol_tabs.p, alpha, alpha_dtype == KORG_DTYPE_F32 ? 1 : 0);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int korg_molecular_cross_sections_device(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                         const korg_grid* grid, int32_t n_layers,
+                                         const double* temps, const double* vmic,
+                                         int64_t n_tables, const korg_mol_xsection* tables,
+                                         int synchronize) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(check_mol_args(ctx, alpha, alpha_dtype, grid, n_layers, temps, vmic, n_tables, tables));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(run_mol(ctx, alpha, alpha_dtype, grid, n_layers, temps, vmic, n_tables, tables));
+    if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return KORG_OK;
+}
+
+int korg_molecular_cross_sections(korg_ctx* ctx, void* alpha, int alpha_dtype,
+                                  const korg_grid* grid, int32_t n_layers, const double* temps,
+                                  const double* vmic, int64_t n_tables,
+                                  const korg_mol_xsection* tables) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    TRY(check_mol_args(ctx, alpha, alpha_dtype, grid, n_layers, temps, vmic, n_tables, tables));
+    if (n_tables == 0) return KORG_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // stage: grid, layers, alpha, then each table's arrays, packed into one buffer
+    const int64_t L = n_layers, n = grid->n_wl;
+    const size_t esz = alpha_dtype == KORG_DTYPE_F32 ? 4 : 8;
+    size_t total = (size_t)(n + 2 * L) * 8 + esz * (size_t)(L * n);
+    for (int64_t k = 0; k < n_tables; ++k) {
+        const korg_mol_xsection& t = tables[k];
+        total += 8 * (size_t)(t.n_vmic + t.n_logT + t.n_wl + L +
+                              (int64_t)t.n_vmic * t.n_logT * t.n_wl) + 64 * 5;
+    }
+    TRY(grow(ctx, ctx->h_mol, total));
+    char* base = (char*)ctx->h_mol.p;
+    size_t off = 0;
+    auto put = [&](const void* src, size_t bytes) -> void* {
+        void* d = base + off;
+        off += (bytes + 63) & ~(size_t)63;
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess
+                   ? d : nullptr;
+    };
+    korg_grid g{n, (const double*)put(grid->all_wls, 8 * (size_t)n)};
+    const double* dT = (const double*)put(temps, 8 * (size_t)L);
+    const double* dv = (const double*)put(vmic, 8 * (size_t)L);
+    void* da = put(alpha, esz * (size_t)(L * n));
+    std::vector<korg_mol_xsection> dt((size_t)n_tables);
+    bool ok = g.all_wls && dT && dv && da;
+    for (int64_t k = 0; k < n_tables && ok; ++k) {
+        const korg_mol_xsection& t = tables[k];
+        korg_mol_xsection& d = dt[k];
+        d = t;
+        d.vmic = (const double*)put(t.vmic, 8 * (size_t)t.n_vmic);
+        d.logT = (const double*)put(t.logT, 8 * (size_t)t.n_logT);
+        d.wls = (const double*)put(t.wls, 8 * (size_t)t.n_wl);
+        d.sigma = (const double*)put(t.sigma, 8 * (size_t)t.n_vmic * t.n_logT * t.n_wl);
+        d.n_species = (const double*)put(t.n_species, 8 * (size_t)L);
+        ok = d.vmic && d.logT && d.wls && d.sigma && d.n_species;
+    }
+    if (!ok) {
+        (void)hipGetLastError();
+        return set_err(ctx, KORG_ERR_HIP, "staging the molecular cross-sections failed");
+    }
+    TRY(run_mol(ctx, da, alpha_dtype, &g, n_layers, dT, dv, n_tables, dt.data()));
+    HIP_TRY(ctx, hipMemcpyAsync(alpha, da, esz * (size_t)(L * n), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;
 }
 

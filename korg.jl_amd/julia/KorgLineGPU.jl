@@ -6,6 +6,7 @@
 #     (src/synthesize.jl:305-306), its reference-wavelength call on view(α_ref, :, 1)
 #     (src/synthesize.jl:266-276) and the molecular cross-section precompute on view(α, i, :, :)
 #     (src/molecular_cross_sections.jl:75-76);
+#   - interpolate_molecular_cross_sections! for Float64 alpha (src/synthesize.jl:277-282, 307-308);
 #   - ForwardDiff.Dual alpha, as when Korg's Fit differentiates synthesize with
 #     autodiff=:forwarddiff (src/Fit/fit_via_synthesis.jl:345-355, src/synthesize.jl:242-247):
 #     the values through korg_line_absorption, the partials through korg_line_absorption_jvp
@@ -284,6 +285,60 @@ function Korg.line_absorption!(α::AbstractVecOrMat{<:ForwardDiff.Dual{Tg,Float6
                                        ForwardDiff.Partials(ntuple(k -> ForwardDiff.partials(a, k) +
                                                                         dα[k][l, j], N)))
     end
+    nothing
+end
+
+# ---------------------------------------------------------------------------------------------
+# Molecular cross-sections: Korg's interpolate_molecular_cross_sections!
+# (src/molecular_cross_sections.jl:99-114) for Float64 alpha through korg_molecular_cross_sections.
+# (The precompute, MolecularCrossSection(linelist, wls), calls line_absorption! on views of its
+# table and so already reaches the GPU through the methods above.)
+
+struct KorgMolXsection
+    n_vmic::Int32
+    n_logT::Int32
+    n_wl::Int64
+    vmic::Ptr{Float64}
+    logT::Ptr{Float64}
+    wls::Ptr{Float64}
+    sigma::Ptr{Float64}
+    n_species::Ptr{Float64}
+end
+
+function Korg.interpolate_molecular_cross_sections!(α::AbstractVecOrMat{Float64},
+                                                    molecular_cross_sections, λs::Korg.Wavelengths,
+                                                    Ts::AbstractVector{Float64}, vmic,
+                                                    number_densities)
+    isempty(molecular_cross_sections) && return nothing     # src/molecular_cross_sections.jl:102-104
+    L = length(Ts)
+    vm = to_layers(vmic, L)
+    T = Float64.(Ts)
+    all_wls = collect(Float64, λs)
+    keep = Any[vm, T, all_wls]
+    tabs = KorgMolXsection[]
+    for σ in molecular_cross_sections
+        knots = σ.itp.itp.knots
+        v, lt, w = collect(Float64, knots[1]), collect(Float64, knots[2]), collect(Float64, knots[3])
+        # Korg's coefs[iv, iT, j] -> the library's sigma[(iv * n_wl + j) * n_logT + iT]
+        coefs = permutedims(σ.itp.itp.coefs, (2, 3, 1))
+        n = to_layers(number_densities[σ.species], L)
+        push!(keep, v, lt, w, coefs, n)
+        push!(tabs, KorgMolXsection(length(v), length(lt), length(w), pointer(v), pointer(lt),
+                                    pointer(w), pointer(coefs), pointer(n)))
+    end
+    direct = is_library_layout(α)
+    buf = direct ? α : zeros(Float64, size(α, 1), size(α, 2))
+    grid = KorgGrid(length(all_wls), pointer(all_wls))
+    GC.@preserve buf keep tabs begin
+        lock(LOCK) do
+            check(ccall((:korg_molecular_cross_sections, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{KorgGrid}, Int32, Ptr{Float64},
+                         Ptr{Float64}, Int64, Ptr{KorgMolXsection}),
+                        context(), pointer(buf), Cint(0), grid, Int32(L), pointer(T), pointer(vm),
+                        Int64(length(tabs)), tabs))
+        end
+    end
+    direct || (α .+= buf)
     nothing
 end
 

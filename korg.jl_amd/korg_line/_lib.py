@@ -74,6 +74,12 @@ class KorgLayerTangents(ctypes.Structure):
                                                "n_div_U")]
 
 
+class KorgMolXsection(ctypes.Structure):
+    _fields_ = [("n_vmic", ctypes.c_int32), ("n_logT", ctypes.c_int32), ("n_wl", ctypes.c_int64),
+                ("vmic", _c_dp), ("logT", _c_dp), ("wls", _c_dp), ("sigma", _c_dp),
+                ("n_species", _c_dp)]
+
+
 class KorgStar(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_void_p), ("grid", KorgGrid), ("lines", KorgLines),
                 ("layers", KorgLayers), ("cntm", KorgContinuum),
@@ -125,7 +131,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 11  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 12  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -136,7 +142,8 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_autotune_batched", "korg_fit_gradient_batched", "korg_plan_fit",
             "korg_continuum_batched", "korg_flux_batched", "korg_set_hydrogen_tables",
             "korg_hydrogen_line_absorption", "korg_line_absorption_device_noalloc",
-            "korg_line_absorption_jvp", "korg_line_absorption_jvp_device"]
+            "korg_line_absorption_jvp", "korg_line_absorption_jvp_device",
+            "korg_molecular_cross_sections", "korg_molecular_cross_sections_device"]
 
 _lib = None
 
@@ -198,6 +205,12 @@ def load():
     L.korg_line_absorption_jvp.restype = i32
     L.korg_line_absorption_jvp_device.argtypes = jvp + [i32]
     L.korg_line_absorption_jvp_device.restype = i32
+    mol = [vp, vp, i32, ctypes.POINTER(KorgGrid), ctypes.c_int32, vp, vp, i64,
+           ctypes.POINTER(KorgMolXsection)]
+    L.korg_molecular_cross_sections.argtypes = mol
+    L.korg_molecular_cross_sections.restype = i32
+    L.korg_molecular_cross_sections_device.argtypes = mol + [i32]
+    L.korg_molecular_cross_sections_device.restype = i32
     L.korg_line_absorption_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
                                                ctypes.POINTER(KorgSpecies), i32, i32, i32]
     L.korg_line_absorption_batched.restype = i32

@@ -155,7 +155,7 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 11
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 12
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -169,7 +169,8 @@ def test_struct_layouts_match_header(tmp_path):
                "korg_star": _lib.KorgStar, "korg_plan_info": _lib.KorgPlanInfo,
                "korg_fit_target": _lib.KorgFitTarget, "korg_flux_target": _lib.KorgFluxTarget,
                "korg_htransition": _lib.KorgHTransition, "korg_htables": _lib.KorgHTables,
-               "korg_hlayers": _lib.KorgHLayers, "korg_layer_tangents": _lib.KorgLayerTangents}
+               "korg_hlayers": _lib.KorgHLayers, "korg_layer_tangents": _lib.KorgLayerTangents,
+               "korg_mol_xsection": _lib.KorgMolXsection}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "korg_line.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
