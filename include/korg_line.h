@@ -447,12 +447,14 @@ int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                       const korg_flux_target* targets, int alpha_dtype, int synchronize);
 
 /*
- * Hydrogen lines (src/hydrogen_line_absorption.jl:75-150): the Stehle & Hutcheon (1999) Stark
+ * Hydrogen lines (src/hydrogen_line_absorption.jl:75-169): the Stehle & Hutcheon (1999) Stark
  * profiles of the Lyman, Balmer and Paschen series to n = 30, plus the Barklem et al. (2000)
  * resonance profiles of H alpha, beta and gamma, with the MHD occupation probabilities
- * (src/statmech.jl:463-497).  The Brackett series (src/hydrogen_line_absorption.jl:152-169) is not
- * included: its lines lie beyond 14,434 A (the series limit less the 150 A window), and grids that
- * reach past that are rejected.
+ * (src/statmech.jl:463-497); and the Brackett series 4 -> m, m = 5 .. the tables' highest upper
+ * level, whose Griem electron-impact and quasistatic (Holtsmark) Stark profiles are convolved
+ * with each other and with the Doppler profile on 201 wavelengths per line and layer
+ * (:180-470), over windows of 5 max(Doppler width, Stark width) that can reach far from the
+ * lines (into the optical, for high m and n_e).
  *
  * korg_set_hydrogen_tables copies the tables to the device once per context, replacing Korg's
  * load-time _load_stark_profiles (src/hydrogen_line_absorption.jl:6-39).  Per transition, in

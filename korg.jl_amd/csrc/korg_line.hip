@@ -2873,6 +2873,19 @@ struct HArgs {
     int* hull;       // [transition][2]: the union of its windows over the layers (lb > ub: none)
     void* alpha;
     int alpha_f32;
+    // Brackett series (lines 4 -> m, m = 5 .. n_max)
+    int n_max, n_br;
+    struct BRec* brec;  // [layer * n_br + (m - 5)]
+    double* bprof;      // [layer * n_br + (m - 5)][BR_POINTS]
+    int* bhull;         // [m - 5][2]
+};
+
+// Brackett line m of one layer: its convolved Stark profile sampled at start + k step,
+// k < BR_POINTS, its amplitude and its window on the grid (0-based; lb > ub: nothing).
+constexpr int BR_POINTS = 201;
+struct BRec {
+    double start, step, amp;
+    int32_t lb, ub;
 };
 
 // src/statmech.jl:463-497 (without the Hubeny generalization)
@@ -3055,6 +3068,256 @@ __global__ __launch_bounds__(64) void k_hlines(HArgs A) {
         }
     }
     if (!lv || !touched) return;  // a tile no transition reaches leaves alpha as it is
+    for (int q = 0; q < jn; ++q) {
+        const int64_t k = (int64_t)l + (int64_t)A.L * (j0 + q);
+        if (A.alpha_f32) ((float*)A.alpha)[k] += (float)acc[q];
+        else ((double*)A.alpha)[k] += acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Brackett series (src/hydrogen_line_absorption.jl:152-169): per layer and line 4 -> m, Griem's
+// electron-impact and quasistatic (Holtsmark) Stark profiles on BR_POINTS wavelengths across
+// 5 max(Doppler width, Stark width) of the line centre (clipped to the grid), convolved with each
+// other and, when the Doppler width is at least a quarter of the Stark width, first the
+// quasistatic one with the Doppler Gaussian (bracket_line_interpolator, :211-249); the grid
+// points within the window then get the linear interpolation of the result times the amplitude.
+// ---------------------------------------------------------------------------------------------
+__constant__ double c_holt_prob7[15][5] = {
+    {0.005, 0.128, 0.260, 0.389, 0.504},     {0.004, 0.109, 0.220, 0.318, 0.389},
+    {-0.007, 0.079, 0.162, 0.222, 0.244},    {-0.018, 0.041, 0.089, 0.106, 0.080},
+    {-0.026, -0.003, 0.003, -0.023, -0.086}, {-0.025, -0.048, -0.087, -0.148, -0.234},
+    {-0.008, -0.085, -0.165, -0.251, -0.343}, {0.018, -0.111, -0.223, -0.321, -0.407},
+    {0.032, -0.130, -0.255, -0.354, -0.431}, {0.014, -0.148, -0.269, -0.359, -0.427},
+    {-0.005, -0.140, -0.243, -0.323, -0.386}, {0.005, -0.095, -0.178, -0.248, -0.307},
+    {-0.002, -0.068, -0.129, -0.187, -0.241}, {-0.007, -0.049, -0.094, -0.139, -0.186},
+    {-0.010, -0.036, -0.067, -0.103, -0.143}};
+__constant__ double c_holt_c7[5] = {511.318, 1.532, 4.044, 19.266, 41.812};
+__constant__ double c_holt_d7[5] = {-6.070, -4.528, -8.759, -14.984, -23.956};
+__constant__ double c_holt_pp[5] = {0.0, 0.2, 0.4, 0.6, 0.8};
+__constant__ double c_holt_beta[15] = {1.0,  1.259, 1.585, 1.995, 2.512, 3.162, 3.981, 5.012,
+                                       6.310, 7.943, 10.0,  12.59, 15.85, 19.95, 25.12};
+
+// src/line_absorption.jl:169-182
+__device__ inline double exponential_integral_1(double x) {
+#pragma clang fp contract(off)
+    if (x < 0) return 0.0;
+    if (x <= 0.01) return -log(x) - 0.577215 + x;
+    if (x <= 1.0)
+        return -log(x) - 0.57721566 +
+               x * (0.99999193 + x * (-0.24991055 + x * (0.05519968 + x * (-0.00976004 + x * 0.00107857))));
+    if (x <= 30.0) return (x * (x + 2.334733) + 0.25062) / (x * (x + 3.330657) + 1.681534) / x * exp(-x);
+    return 0.0;
+}
+
+// src/hydrogen_line_absorption.jl:397-445 (0-based table indices)
+__device__ inline double holtsmark_profile(double beta, double P) {
+#pragma clang fp contract(off)
+    if (beta > 500) return (1.5 / sqrt(beta) + 27 / (beta * beta)) / (beta * beta);
+    const int im = min((int)floor(5 * P + 1), 4) - 1, ip = im + 1;
+    const double WTPP = 5 * (P - c_holt_pp[im]), WTPM = 1 - WTPP;
+    if (beta <= 25.12) {
+        int jp = 0;
+        while (beta > c_holt_beta[jp]) ++jp;  // the first knot >= beta (beta <= 25.12: it exists)
+        jp = max(1, jp);
+        const int jm = jp - 1;
+        const double WTBP = (beta - c_holt_beta[jm]) / (c_holt_beta[jp] - c_holt_beta[jm]);
+        const double WTBM = 1 - WTBP;
+        const double CBP = c_holt_prob7[jp][ip] * WTPP + c_holt_prob7[jp][im] * WTPM;
+        const double CBM = c_holt_prob7[jm][ip] * WTPP + c_holt_prob7[jm][im] * WTPM;
+        const double CORR = 1 + CBP * WTBP + CBM * WTBM;
+        const double WT = fmax(fmin(0.5 * (10 - beta), 1.0), 0.0);
+        const double PR1 = beta <= 10 ? 8 / (83 + (2 + 0.95 * (beta * beta)) * beta) : 0.0;
+        const double PR2 = beta >= 8 ? (1.5 / sqrt(beta) + 27 / (beta * beta)) / (beta * beta) : 0.0;
+        return (PR1 * WT + PR2 * (1 - WT)) * CORR;
+    }
+    const double CC = c_holt_c7[ip] * WTPP + c_holt_c7[im] * WTPM;
+    const double DD = c_holt_d7[ip] * WTPP + c_holt_d7[im] * WTPM;
+    const double CORR = 1 + DD / (CC + beta * sqrt(beta));
+    return (1.5 / sqrt(beta) + 27 / (beta * beta)) / (beta * beta) * CORR;
+}
+
+// src/hydrogen_line_absorption.jl:370-389 (n = 4)
+__device__ inline double greim_1960_Knm_4(int m) {
+    if (m - 4 <= 3) return m == 5 ? 0.5820000 : (m == 6 ? 0.8660000 : 1.0200000);
+    const double md = m, m4 = (md * md) * (md * md);
+    return 5.5e-5 * 256.0 * m4 / (md * md - 16.0) / (1 + 0.13 / (md - 4));
+}
+
+// src/hydrogen_line_absorption.jl:180-191 (n = 4)
+__device__ inline double brackett_oscillator_strength(int m) {
+#pragma clang fp contract(off)
+    const double n = 4.0, md = m;
+    const double GINF = 0.2027 / pow(n, 0.71), GCA = 0.124 / n, FKN = 1.9603 * n;
+    const double WTC = 0.45 - 2.4 / (n * n * n) * (n - 1);
+    const double r = md / ((md - n) * (md + n));
+    const double FK = FKN * (r * r * r);
+    const double XMN12 = pow(md - n, 1.2);
+    const double WT = (XMN12 - 1) / (XMN12 + WTC);
+    return FK * (1 - WT * GINF - (0.222 + GCA / md) * (1 - WT));
+}
+
+// The central BR_POINTS values of the full convolution of a and b (both BR_POINTS long): point k
+// sums a[i] b[(BR_POINTS - 1)/2 + k - i] over the valid i, ascending.
+__device__ inline double conv_center(const double* a, const double* b, int k) {
+    const int s = (BR_POINTS - 1) / 2 + k;
+    const int i0 = max(0, s - (BR_POINTS - 1)), i1 = min(BR_POINTS - 1, s);
+    double acc = 0.0;
+    for (int i = i0; i <= i1; ++i) acc += a[i] * b[s - i];
+    return acc;
+}
+
+// One block of 256 threads per (layer, line); thread k < BR_POINTS owns sample k.
+__global__ __launch_bounds__(256) void k_bsetup(HArgs A) {
+#pragma clang fp contract(off)
+    __shared__ double s_imp[BR_POINTS], s_q[BR_POINTS], s_g[BR_POINTS];
+    const int p = blockIdx.x;
+    const int l = p / A.n_br, m = 5 + p % A.n_br;
+    const int k = threadIdx.x;
+    const double T = A.T[l], ne = A.ne[l], xi = A.xi[l];
+    const double lmin = A.wls[0], lmax = A.wls[A.n_wl - 1];
+    // :154-159: the line centre and amplitude
+    const double E = RydbergH_eV * (1.0 / 16.0 - 1.0 / (double)(m * m));
+    const double lam0 = hplanck_eV * c_cgs / E;
+    // :218-234: widths, window and the sampled range
+    const double F0 = 1.25e-9 * pow(ne, 2.0 / 3.0);
+    const double Knm = greim_1960_Knm_4(m);
+    const double stark_width = 1.6678E-18 * Knm * F0 * c_cgs;
+    const double sdop = doppler_width(lam0, T, H_MASS, xi);
+    const double window = 5.0 * fmax(sdop, stark_width);
+    const double start = fmax(lmin, lam0 - window), end = fmin(lam0 + window, lmax);
+    BRec rec = {0.0, 0.0, 0.0, 1, 0};
+    if (start > lmax || end < lmin || start == end) {  // block-uniform: no contribution
+        if (k == 0) A.brec[p] = rec;
+        return;
+    }
+    const double step = (end - start) / (BR_POINTS - 1);
+    // :272-351: the two Stark profiles at start + k step
+    const double nu0 = c_cgs / lam0;
+    double lam = 0.0, nu = 0.0, imp = 0.0, q = 0.0;
+    bool red = false;
+    if (k < BR_POINTS) {
+        lam = start + step * k;
+        nu = c_cgs / lam;
+        red = nu < nu0;
+        const double ne_1_6 = pow(ne, 1.0 / 6.0);
+        const double GCON1 = 0.2 + 0.09 * sqrt(T / 10000.0) / (1 + ne / 1.0e13);
+        const double GCON2 = 0.2 / (1 + ne / 1.0e15);
+        const double WTY1 = 1 / (1 + ne / (m - 4 <= 3 ? 1e14 : 1e13));
+        const double Y1B = 2 / (1 + 0.012 / T * sqrt(ne / T));
+        const double mm = (double)(m * m - 16);
+        const double C1CON = Knm / lam0 * (mm * mm) / (16.0 * (double)(m * m)) * 1e-8;
+        const double Y1SCAL = 320 * (pow(T / 10000.0, 0.3) / ne_1_6) * WTY1 + Y1B * (1 - WTY1);
+        const double C1 = F0 * 78940 / T * C1CON * Y1SCAL;
+        const double C2 = (F0 * F0) / (5.96E-23 * ne) * ((Knm / lam0) * (Knm / lam0)) * 1e-16;
+        const double beta = fabs(lam - lam0) / F0 / Knm * 1e8;
+        const double y1 = C1 * beta, y2 = C2 * (beta * beta);
+        const double G1 = 6.77 * sqrt(C1);
+        double width;
+        if (y2 <= 1e-4 && y1 <= 1e-5) {
+            width = G1 * fmax(0.0, 0.2114 + log(sqrt(C2) / C1)) * (1 - GCON1 - GCON2);
+        } else {
+            const double y1c = 90 * y1;
+            const double GAM = G1 *
+                               (0.5 * exp(-fmin(80.0, y1)) + exponential_integral_1(y1) -
+                                0.5 * exponential_integral_1(y2)) *
+                               (1 - GCON1 / (1 + y1c * y1c * y1c) - GCON2 / (1 + 2000 * y1));
+            width = GAM <= 1e-20 ? 0.0 : GAM;
+        }
+        imp = width > 0 ? width / (PI * (width * width + beta * beta)) : 0.0;
+        const double PP = ne_1_6 * 0.08989 / sqrt(T);
+        const double ps = (0.9 * y1) * (0.9 * y1);
+        const double qe = (ps + 0.03 * sqrt(y1)) / (ps + 1.0);
+        q = holtsmark_profile(beta, PP) * (1 + qe);
+    }
+    // every sample is rescaled by the red-wing Boltzmann factor once any sample is redward
+    red = __syncthreads_or(red);
+    if (k < BR_POINTS) {
+        const double dbeta_dlam = 1e8 / (Knm * F0);
+        const double sq = sqrt(lam / lam0);
+        const double bz = red ? exp((hplanck_cgs * (nu - nu0)) / kboltz_cgs / T) : 1.0;
+        imp = imp * sq;
+        q = q * sq;
+        if (red) {
+            imp = imp * bz;
+            q = q * bz;
+        }
+        s_imp[k] = imp * dbeta_dlam;
+        s_q[k] = q * dbeta_dlam;
+    }
+    if (sdop / stark_width > 0.25) {  // block-uniform
+        if (k < BR_POINTS) {
+            const double d = lam - lam0;
+            s_g[k] = exp(-0.5 * (d * d) / (sdop * sdop)) / sqrt(2 * PI) / sdop;
+        }
+        __syncthreads();
+        const double c = k < BR_POINTS ? conv_center(s_q, s_g, k) * step : 0.0;
+        __syncthreads();
+        if (k < BR_POINTS) s_q[k] = c;
+    }
+    __syncthreads();
+    if (k < BR_POINTS) A.bprof[(int64_t)p * BR_POINTS + k] = conv_center(s_imp, s_q, k) * step;
+    if (k == 0) {
+        const double beta_T = 1.0 / (kboltz_eV * T);
+        const double E_low = RydbergH_eV * (1.0 - 1.0 / 16.0);
+        const double w = A.use_MHD ? hummer_mihalas_w(T, (double)m, A.nH[l], A.nHe[l], ne) : 1.0;
+        const double levels = w * exp(-beta_T * E_low) * (1 - exp(-beta_T * E)) / A.U[l];
+        const double gf = 2.0 * 16.0 * brackett_oscillator_strength(m);
+        rec.start = start;
+        rec.step = step;
+        rec.amp = gf * A.nH[l] * sigma_line(lam0) * levels;
+        const int64_t lb = lower_bound_f64(A.wls, A.n_wl, lam0 - window);
+        const int64_t ub = upper_bound_f64(A.wls, A.n_wl, lam0 + window) - 1;
+        if (lb <= ub) {
+            rec.lb = (int32_t)lb;
+            rec.ub = (int32_t)ub;
+            atomicMin(&A.bhull[2 * (m - 5)], (int)lb);
+            atomicMax(&A.bhull[2 * (m - 5) + 1], (int)ub);
+        }
+        A.brec[p] = rec;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_binit(HArgs A) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t < A.n_br) {
+        A.bhull[2 * t] = INT_MAX;
+        A.bhull[2 * t + 1] = -1;
+    }
+}
+
+// One workgroup per HCT-point tile, lanes = layers: the Brackett lines whose window meets the
+// point, in order of m, summed and added into alpha once.
+__global__ __launch_bounds__(64) void k_blines(HArgs A) {
+#pragma clang fp contract(off)
+    const int l = blockIdx.y * 64 + threadIdx.x;
+    const bool lv = l < A.L;
+    const int64_t j0 = (int64_t)blockIdx.x * HCT;
+    const int jn = (int)min((int64_t)HCT, A.n_wl - j0);
+    double acc[HCT];
+#pragma unroll
+    for (int q = 0; q < HCT; ++q) acc[q] = 0.0;
+    bool touched = false;  // wavefront-uniform
+    for (int mi = 0; mi < A.n_br; ++mi) {
+        if (A.bhull[2 * mi] > j0 + jn - 1 || A.bhull[2 * mi + 1] < j0) continue;  // uniform
+        const int64_t p = (int64_t)(lv ? l : 0) * A.n_br + mi;
+        const BRec r = A.brec[p];
+        const bool ov = lv && r.lb <= r.ub && r.lb < j0 + jn && r.ub >= j0;
+        if (!ballot(ov)) continue;
+        touched = true;
+        const double* prof = A.bprof + p * BR_POINTS;
+#pragma unroll
+        for (int q = 0; q < HCT; ++q) {
+            const int64_t jj = j0 + q;
+            if (!(q < jn && ov && jj >= r.lb && jj <= r.ub)) continue;
+            const double t = (A.wls[jj] - r.start) / r.step;
+            int i = (int)floor(t);
+            i = i < 0 ? 0 : (i > BR_POINTS - 2 ? BR_POINTS - 2 : i);
+            const double w = t - i;
+            acc[q] += (prof[i] * (1 - w) + prof[i + 1] * w) * r.amp;
+        }
+    }
+    if (!lv || !touched) return;
     for (int q = 0; q < jn; ++q) {
         const int64_t k = (int64_t)l + (int64_t)A.L * (j0 + q);
         if (A.alpha_f32) ((float*)A.alpha)[k] += (float)acc[q];
@@ -3248,7 +3511,8 @@ struct korg_ctx {
     DevBuf mol_tabs, h_mol;
     // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
     DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices, hl_hull;
-    int hl_ntrans = 0, hl_max_dnu = 0;
+    DevBuf hl_brec, hl_bprof, hl_bhull;  // Brackett series
+    int hl_ntrans = 0, hl_max_dnu = 0, hl_nmax = 0;
 };
 
 namespace {
@@ -4788,7 +5052,7 @@ void korg_destroy(korg_ctx* ctx) {
                       &ctx->h_species_mass, &ctx->h_species_mol, &ctx->h_knots, &ctx->h_cntm,
                       &ctx->hl_trans, &ctx->hl_T,     &ctx->hl_ne,     &ctx->hl_dnu,
                       &ctx->hl_logP,  &ctx->hl_lam0,  &ctx->hl_pairs,  &ctx->hl_slices,
-                      &ctx->hl_hull};
+                      &ctx->hl_hull,  &ctx->hl_brec,  &ctx->hl_bprof,  &ctx->hl_bhull};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_lines)
@@ -5232,7 +5496,7 @@ int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* t) {
         !t->n_e || !t->log_dnu || !t->log_profile || !t->lambda0)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "hydrogen tables: missing arrays or no transitions");
     std::vector<HTrans> tr(t->n_transitions);
-    int max_dnu = 0;
+    int max_dnu = 0, n_max = 0;
     auto fits = [](int64_t off, int64_t len, int64_t n) { return off >= 0 && len >= 0 && off <= n - len; };
     auto ascending = [](const double* a, int64_t from, int64_t n) {
         for (int64_t i = from + 1; i < n; ++i)
@@ -5254,6 +5518,7 @@ int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* t) {
         tr[k] = {x.lower, x.upper, x.n_T, x.n_ne, x.n_dnu, 0, x.log_gf,
                  x.off_T, x.off_ne, x.off_dnu, x.off_profile, x.off_lambda0};
         max_dnu = std::max(max_dnu, x.n_dnu);
+        n_max = std::max(n_max, x.upper);
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     ctx->hl_ntrans = 0;  // until the upload completes: a failed upload leaves no tables set
@@ -5274,6 +5539,7 @@ int korg_set_hydrogen_tables(korg_ctx* ctx, const korg_htables* t) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->hl_ntrans = t->n_transitions;
     ctx->hl_max_dnu = max_dnu;
+    ctx->hl_nmax = n_max;  // the Brackett series runs to the tables' highest upper level
     return KORG_OK;
 }
 
@@ -5294,21 +5560,14 @@ int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, c
     if (!(window_size >= 0.0) || !std::isfinite(window_size))
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "window_size must be finite and >= 0");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    // the Brackett series (src/hydrogen_line_absorption.jl:152-169) is not included: refuse grids
-    // that reach the windows of its lines (all beyond its series limit, 14,589 A)
-    double last = 0.0;
-    HIP_TRY(ctx, hipMemcpyAsync(&last, grid->all_wls + grid->n_wl - 1, sizeof last,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    const double brackett_limit = hplanck_eV * c_cgs / (RydbergH_eV / 16.0);
-    if (!(last < brackett_limit - window_size))
-        return set_err(ctx, KORG_ERR_UNSUPPORTED,
-                       "the grid reaches %.1f A; hydrogen lines of the Brackett series (from %.1f A, "
-                       "less the window) are not included", last * 1e8, brackett_limit * 1e8);
     const int L = layers->n_layers, NT = ctx->hl_ntrans;
+    const int n_br = std::max(0, ctx->hl_nmax - 4);  // Brackett lines 4 -> 5 .. n_max
     TRY(grow(ctx, ctx->hl_pairs, (size_t)L * NT * sizeof(HPair)));
     TRY(grow(ctx, ctx->hl_slices, (size_t)L * NT * ctx->hl_max_dnu * sizeof(double)));
     TRY(grow(ctx, ctx->hl_hull, (size_t)NT * 2 * sizeof(int)));
+    TRY(grow(ctx, ctx->hl_brec, (size_t)L * n_br * sizeof(BRec)));
+    TRY(grow(ctx, ctx->hl_bprof, (size_t)L * n_br * BR_POINTS * sizeof(double)));
+    TRY(grow(ctx, ctx->hl_bhull, (size_t)std::max(n_br, 1) * 2 * sizeof(int)));
     HArgs a;
     a.L = L;
     a.n_trans = NT;
@@ -5334,10 +5593,20 @@ int korg_hydrogen_line_absorption(korg_ctx* ctx, void* alpha, int alpha_dtype, c
     a.hull = (int*)ctx->hl_hull.p;
     a.alpha = alpha;
     a.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
+    a.n_max = ctx->hl_nmax;
+    a.n_br = n_br;
+    a.brec = (BRec*)ctx->hl_brec.p;
+    a.bprof = (double*)ctx->hl_bprof.p;
+    a.bhull = (int*)ctx->hl_bhull.p;
+    const dim3 tiles((unsigned)((grid->n_wl + HCT - 1) / HCT), (L + 63) / 64);
     hipLaunchKernelGGL(k_hinit, dim3((NT + 63) / 64), dim3(64), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_hsetup, dim3((L * NT + 63) / 64), dim3(64), 0, ctx->stream, a);
-    hipLaunchKernelGGL(k_hlines, dim3((unsigned)((grid->n_wl + HCT - 1) / HCT), (L + 63) / 64), dim3(64), 0,
-                       ctx->stream, a);
+    hipLaunchKernelGGL(k_hlines, tiles, dim3(64), 0, ctx->stream, a);
+    if (n_br > 0) {  // src/hydrogen_line_absorption.jl:152-169, after the tabulated lines
+        hipLaunchKernelGGL(k_binit, dim3((n_br + 63) / 64), dim3(64), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_bsetup, dim3((unsigned)(L * n_br)), dim3(256), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_blines, tiles, dim3(64), 0, ctx->stream, a);
+    }
     HIP_TRY(ctx, hipGetLastError());
     if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;

@@ -1,14 +1,15 @@
 """Hydrogen lines on the device: synthesize's hydrogen-line loop (``src/synthesize.jl:293-301``)
-over ``hydrogen_line_absorption!`` (``src/hydrogen_line_absorption.jl:75-150``) -- the Stehle &
+over ``hydrogen_line_absorption!`` (``src/hydrogen_line_absorption.jl:75-169``) -- the Stehle &
 Hutcheon (1999) Stark profiles of the Lyman, Balmer and Paschen series to n = 30, the Barklem et al.
-(2000) resonance profiles of H alpha, beta and gamma, and the MHD occupation probabilities.
+(2000) resonance profiles of H alpha, beta and gamma, the MHD occupation probabilities, and the
+Brackett series to the tables' highest upper level (Griem's impact and Holtsmark's quasistatic
+profiles convolved, with the Doppler profile where it matters).
 
 ``StarkTables`` packs the tables of ``Stehle-Hutchson-hydrogen-profiles.h5`` (shipped in
 ``korg_line/data``) for ``korg_set_hydrogen_tables`` as Korg loads them
 (``src/hydrogen_line_absorption.jl:6-39``): the transitions in the file's (name) order, log of
 each profile with log 0 clipped to -700, log(delta_nu / F0) knots (the first, for delta_nu = 0,
-stands for -floatmax and is not read), line-centre tables in cm.  The Brackett series is not
-included (the library refuses grids that reach it).
+stands for -floatmax and is not read), line-centre tables in cm.
 """
 import ctypes
 import os

@@ -3,8 +3,11 @@
 Stehle & Hutcheon (1999) tables as Korg interpolates them (``src/hydrogen_line_absorption.jl:6-39``:
 trilinear in (T, n_e, log(dnu / F0)) with flat extrapolation, the first log-detuning knot at
 -floatmax, log 0 clipped to -700) and the MHD occupation probability ``hummer_mihalas_w``
-(``src/statmech.jl:463-497``).  The Brackett-series part (``:152-169``) is not restated: its lines
-lie beyond 1.44 um and reach no grid this checks.
+(``src/statmech.jl:463-497``), and the Brackett series (``:152-169``): its Stark profiles
+(``brackett_line_stark_profiles``, ``:259-351``: electron impact + quasistatic Holtsmark, ``:391-445``)
+convolved with each other and, when the Doppler width matters, with a Gaussian on a 201-point grid
+(``bracket_line_interpolator``, ``:211-249``), then interpolated linearly onto the grid.  The
+convolutions are direct sums (the reference uses DSP.conv; the two agree to rounding).
 
 Only the tests use this module, as the checker of the device library; the product never imports
 it.  Tables are passed in as plain arrays (``transitions``: a list of dicts with ``lower``,
@@ -17,6 +20,7 @@ import numpy as np
 
 # src/constants.jl:3-19
 kboltz_cgs = 1.380649e-16
+hplanck_cgs = 6.62607015e-27
 kboltz_eV = 8.617333262145e-5
 hplanck_eV = 4.135667696e-15
 c_cgs = 2.99792458e10
@@ -138,6 +142,195 @@ def searchsortedlast(wls, x):
     return int(np.searchsorted(wls, x, side="right"))
 
 
+def exponential_integral_1(x):
+    """``src/line_absorption.jl:169-182`` (piecewise approximation of E1)."""
+    if x < 0:
+        return 0.0
+    if x <= 0.01:
+        return -math.log(x) - 0.577215 + x
+    if x <= 1.0:
+        return -math.log(x) - 0.57721566 + \
+            x * (0.99999193 + x * (-0.24991055 + x * (0.05519968 + x * (-0.00976004 + x * 0.00107857))))
+    if x <= 30.0:
+        return (x * (x + 2.334733) + 0.25062) / (x * (x + 3.330657) + 1.681534) / x * math.exp(-x)
+    return 0.0
+
+
+def brackett_oscillator_strength(n, m):
+    """``src/hydrogen_line_absorption.jl:180-191``."""
+    GINF = 0.2027 / n ** 0.71
+    GCA = 0.124 / n
+    FKN = 1.9603 * n
+    WTC = 0.45 - 2.4 / n ** 3 * (n - 1)
+    FK = FKN * (m / ((m - n) * (m + n))) ** 3
+    XMN12 = (m - n) ** 1.2
+    WT = (XMN12 - 1) / (XMN12 + WTC)
+    return FK * (1 - WT * GINF - (0.222 + GCA / m) * (1 - WT))
+
+
+_GRIEM_KMN = [[0.0001716, 0.0090190, 0.1001000, 0.5820000],
+              [0.0005235, 0.0177200, 0.1710000, 0.8660000],
+              [0.0008912, 0.0250700, 0.2230000, 1.0200000]]
+
+
+def greim_1960_Knm(n, m):
+    """``src/hydrogen_line_absorption.jl:370-389``."""
+    if m - n <= 3 and n <= 4:
+        return _GRIEM_KMN[m - n - 1][n - 1]
+    return 5.5e-5 * n ** 4 * m ** 4 / (m ** 2 - n ** 2) / (1 + 0.13 / (m - n))
+
+
+_H_PROB7 = np.array([[0.005, 0.128, 0.260, 0.389, 0.504], [0.004, 0.109, 0.220, 0.318, 0.389],
+                     [-0.007, 0.079, 0.162, 0.222, 0.244], [-0.018, 0.041, 0.089, 0.106, 0.080],
+                     [-0.026, -0.003, 0.003, -0.023, -0.086], [-0.025, -0.048, -0.087, -0.148, -0.234],
+                     [-0.008, -0.085, -0.165, -0.251, -0.343], [0.018, -0.111, -0.223, -0.321, -0.407],
+                     [0.032, -0.130, -0.255, -0.354, -0.431], [0.014, -0.148, -0.269, -0.359, -0.427],
+                     [-0.005, -0.140, -0.243, -0.323, -0.386], [0.005, -0.095, -0.178, -0.248, -0.307],
+                     [-0.002, -0.068, -0.129, -0.187, -0.241], [-0.007, -0.049, -0.094, -0.139, -0.186],
+                     [-0.010, -0.036, -0.067, -0.103, -0.143]])
+_H_C7 = [511.318, 1.532, 4.044, 19.266, 41.812]
+_H_D7 = [-6.070, -4.528, -8.759, -14.984, -23.956]
+_H_PP = [0.0, 0.2, 0.4, 0.6, 0.8]
+_H_BETA = [1.0, 1.259, 1.585, 1.995, 2.512, 3.162, 3.981, 5.012, 6.310, 7.943, 10.0, 12.59,
+           15.85, 19.95, 25.12]
+
+
+def holtsmark_profile(beta, P):
+    """``src/hydrogen_line_absorption.jl:397-445`` (0-based table indices)."""
+    if beta > 500:
+        return (1.5 / math.sqrt(beta) + 27 / beta ** 2) / beta ** 2
+    im = min(int(math.floor(5 * P + 1)), 4) - 1
+    ip = im + 1
+    WTPP = 5 * (P - _H_PP[im])
+    WTPM = 1 - WTPP
+    if beta <= 25.12:
+        jp = max(1, next(i for i, k in enumerate(_H_BETA) if beta <= k))
+        jm = jp - 1
+        WTBP = (beta - _H_BETA[jm]) / (_H_BETA[jp] - _H_BETA[jm])
+        WTBM = 1 - WTBP
+        CBP = _H_PROB7[jp, ip] * WTPP + _H_PROB7[jp, im] * WTPM
+        CBM = _H_PROB7[jm, ip] * WTPP + _H_PROB7[jm, im] * WTPM
+        CORR = 1 + CBP * WTBP + CBM * WTBM
+        WT = max(min(0.5 * (10 - beta), 1), 0)
+        PR1 = 8 / (83 + (2 + 0.95 * beta ** 2) * beta) if beta <= 10 else 0.0
+        PR2 = (1.5 / math.sqrt(beta) + 27 / beta ** 2) / beta ** 2 if beta >= 8 else 0.0
+        return (PR1 * WT + PR2 * (1 - WT)) * CORR
+    CC = _H_C7[ip] * WTPP + _H_C7[im] * WTPM
+    DD = _H_D7[ip] * WTPP + _H_D7[im] * WTPM
+    CORR = 1 + DD / (CC + beta * math.sqrt(beta))
+    return (1.5 / math.sqrt(beta) + 27 / beta ** 2) / beta ** 2 * CORR
+
+
+def brackett_line_stark_profiles(m, lams, lam0, T, ne):
+    """``src/hydrogen_line_absorption.jl:272-351``: the electron-impact and quasistatic profiles
+    of the Brackett line to level m at the wavelengths ``lams`` (cm, ascending)."""
+    n = 4
+    nus = c_cgs / lams
+    nu0 = c_cgs / lam0
+    ne_1_6 = ne ** (1 / 6)
+    F0 = 1.25e-9 * ne ** (2 / 3)
+    GCON1 = 0.2 + 0.09 * math.sqrt(T / 10_000) / (1 + ne / 1.0e13)
+    GCON2 = 0.2 / (1 + ne / 1.0e15)
+    Knm = greim_1960_Knm(n, m)
+    Y1WHT = 1e14 if m - n <= 3 else 1e13
+    WTY1 = 1 / (1 + ne / Y1WHT)
+    Y1B = 2 / (1 + 0.012 / T * math.sqrt(ne / T))
+    C1CON = Knm / lam0 * (m ** 2 - n ** 2) ** 2 / (n ** 2 * m ** 2) * 1e-8
+    Y1NUM = 320
+    Y1SCAL = Y1NUM * ((T / 10_000) ** 0.3 / ne_1_6) * WTY1 + Y1B * (1 - WTY1)
+    C1 = F0 * 78940 / T * C1CON * Y1SCAL
+    C2 = F0 ** 2 / (5.96e-23 * ne) * (Knm / lam0) ** 2 * 1e-16
+    betas = np.abs(lams - lam0) / F0 / Knm * 1e8
+    G1 = 6.77 * math.sqrt(C1)
+    impact = np.zeros(lams.size)
+    quasi = np.zeros(lams.size)
+    PP = ne_1_6 * 0.08989 / math.sqrt(T)
+    for k, b in enumerate(betas):
+        y1 = C1 * b
+        y2 = C2 * b ** 2
+        if y2 <= 1e-4 and y1 <= 1e-5:
+            width = G1 * max(0.0, 0.2114 + math.log(math.sqrt(C2) / C1)) * (1 - GCON1 - GCON2)
+        else:
+            GAM = G1 * (0.5 * math.exp(-min(80.0, y1)) + exponential_integral_1(y1) -
+                        0.5 * exponential_integral_1(y2)) * \
+                (1 - GCON1 / (1 + (90 * y1) ** 3) - GCON2 / (1 + 2000 * y1))
+            width = 0.0 if GAM <= 1e-20 else GAM
+        impact[k] = width / (math.pi * (width ** 2 + b ** 2)) if width > 0 else 0.0
+        ps = (0.9 * y1) ** 2
+        qe = (ps + 0.03 * math.sqrt(y1)) / (ps + 1.0)
+        quasi[k] = holtsmark_profile(b, PP) * (1 + qe)
+    dbeta_dlam = 1e8 / (Knm * F0)
+    red = bool(np.any(nus < nu0))  # findlast(nus .< nu0) exists: every point is rescaled
+    for prof in (impact, quasi):
+        prof *= np.sqrt(lams / lam0)
+        if red:
+            prof *= np.exp((hplanck_cgs * (nus - nu0)) / kboltz_cgs / T)
+        prof *= dbeta_dlam
+    return impact, quasi
+
+
+def _conv_center(a, b):
+    """The central len(a) values of the full convolution (``conv(a, b)[start:start+n-1]``,
+    start = (n + 1) / 2, 1-based).  Direct sums: every term is non-negative, so any summation order
+    agrees to rounding."""
+    n = a.size
+    return np.convolve(a, b)[(n - 1) // 2:(n - 1) // 2 + n]
+
+
+def bracket_line_interpolator(m, lam0, T, ne, xi, lmin=0.0, lmax=math.inf, n_points=201,
+                              window_size=5, include_doppler_threshold=0.25):
+    """``src/hydrogen_line_absorption.jl:211-249``: (first wavelength, step, convolved profile,
+    window); the profile is interpolated linearly between its wavelengths start + k step.
+    (None, None, None, 0.0) when the window misses [lmin, lmax]."""
+    n = 4
+    F0 = 1.25e-9 * ne ** (2 / 3)
+    Knm = greim_1960_Knm(n, m)
+    stark_width = 1.6678e-18 * Knm * F0 * c_cgs
+    sdop = doppler_width(lam0, T, H_MASS, xi)
+    window = window_size * max(sdop, stark_width)
+    start = max(lmin, lam0 - window)
+    end = min(lam0 + window, lmax)
+    if start > lmax or end < lmin or start == end:
+        return None, None, None, 0.0
+    step = (end - start) / (n_points - 1)
+    lams = start + step * np.arange(n_points)
+    impact, quasi = brackett_line_stark_profiles(m, lams, lam0, T, ne)
+    if sdop / stark_width > include_doppler_threshold:
+        dop = np.exp(-0.5 * (lams - lam0) ** 2 / sdop ** 2) / math.sqrt(2 * math.pi) / sdop
+        quasi = _conv_center(quasi, dop) * step
+    return start, step, _conv_center(impact, quasi) * step, window
+
+
+def _interp_range(start, step, vals, x):
+    """Linear interpolation of vals at start + k step (``linear_interpolation(range, vals)``)."""
+    t = (x - start) / step
+    i = np.clip(np.floor(t).astype(np.int64), 0, vals.size - 2)
+    w = t - i
+    return vals[i] * (1 - w) + vals[i + 1] * w
+
+
+def brackett_lines(alpha_row, wls, T, ne, nH_I, UH_I, xi, ws, n_max):
+    """``src/hydrogen_line_absorption.jl:152-169``: adds the Brackett lines 4 -> m, m = 5 ..
+    n_max, to one layer; ``ws[k]`` is the occupation probability of level k + 1."""
+    n = 4
+    beta = 1 / (kboltz_eV * T)
+    E_low = RydbergH_eV * (1 - 1 / n ** 2)
+    for m in range(5, n_max + 1):
+        E = RydbergH_eV * (1 / n ** 2 - 1 / m ** 2)
+        lam0 = hplanck_eV * c_cgs / E
+        levels = ws[m - 1] * math.exp(-beta * E_low) * (1 - math.exp(-beta * E)) / UH_I
+        gf = 2 * n ** 2 * brackett_oscillator_strength(n, m)
+        amplitude = gf * nH_I * sigma_line(lam0) * levels
+        start, step, prof, window = bracket_line_interpolator(m, lam0, T, ne, xi, wls[0], wls[-1])
+        lb = searchsortedfirst(wls, lam0 - window)
+        ub = searchsortedlast(wls, lam0 + window)
+        if prof is None or lb > ub:
+            continue
+        sl = slice(lb - 1, ub)
+        alpha_row[sl] += _interp_range(start, step, prof, wls[sl]) * amplitude
+    return alpha_row
+
+
 def hydrogen_line_absorption(alpha_row, wls, T, ne, nH_I, nHe_I, UH_I, xi, window_size,
                              transitions, use_MHD=True):
     """Adds hydrogen-line opacity to ``alpha_row`` (the grid ``wls`` in cm), one layer."""
@@ -182,4 +375,5 @@ def hydrogen_line_absorption(alpha_row, wls, T, ne, nH_I, nHe_I, UH_I, xi, windo
             a = _bilinear(logP[k], temps, nes, T, ne)
             b = _bilinear(logP[k + 1], temps, nes, T, ne)
             alpha_row[j] += math.exp(a * (1 - w) + b * w) * dnudl[j] * amplitude
+    brackett_lines(alpha_row, wls, T, ne, nH_I, UH_I, xi, ws, n_max)
     return alpha_row

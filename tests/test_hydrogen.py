@@ -1,4 +1,4 @@
-"""Hydrogen lines (src/hydrogen_line_absorption.jl:75-150): the oracle (oracle/hydrogen_oracle.py)
+"""Hydrogen lines (src/hydrogen_line_absorption.jl:75-169): the oracle (oracle/hydrogen_oracle.py)
 and the device (korg_hydrogen_line_absorption, through the C ABI).
 
 The oracle is pinned by the reference's own test (test/line_absorption.jl:19-39): the Lyman
@@ -8,7 +8,8 @@ available offline, so it enters as the one constant fitted on the line cores, wh
 physical value (2 plus the excited levels' share, between 2.00001 and 2.0001).  Its helper
 functions are checked against the C oracle's, which the line-opacity tests pin.  The device is
 checked against the oracle at 1e-12 (Float64) on the golden case and on the solar atmosphere around
-H alpha, H beta and the Paschen lines, with MHD on.
+H alpha, H beta and the Paschen lines, with MHD on, and around the Brackett lines (their own pins
+are in test_brackett_oracle.py).
 """
 import ctypes
 import math
@@ -169,14 +170,39 @@ def test_device_solar_matches_oracle(ctx, transitions, rng_A):
 
 
 @pytest.mark.gpu
-def test_device_rejects_brackett_range_and_missing_tables(ctx):
+@pytest.mark.parametrize("rng_A,use_MHD", [((15000.0, 15300.0, 0.05), False),
+                                           ((21640.0, 21680.0, 0.02), False),
+                                           ((21655.0, 21668.0, 0.01), True)])
+def test_device_brackett_matches_oracle(ctx, transitions, rng_A, use_MHD):
+    """The Brackett series (src/hydrogen_line_absorption.jl:152-169) with the tabulated lines, 56
+    solar layers: the high-m lines near the series limit (m = 19 .. 30 reach 15,000-15,300 A, Stark
+    dominated in the deep layers) and Br gamma (its window clipped by a grid narrower than it, MHD
+    on), in both regimes of the Doppler convolution; Float64 alpha at 1e-12, Float32 within its
+    rounding."""
+    atm, lay = solar_layers()
+    wls = Wavelengths(rng_A).all_wls
+    got = device_hlines(ctx, wls, lay, 150e-8, use_MHD)
+    ref = np.zeros_like(got)
+    br = np.zeros_like(got)
+    for l in range(atm.n_layers):
+        H.hydrogen_line_absorption(ref[l], wls, lay["T"][l], lay["ne"][l], lay["nH"][l],
+                                   lay["nHe"][l], lay["U"][l], lay["xi"][l], 150e-8, transitions,
+                                   use_MHD=use_MHD)
+        ws = [H.hummer_mihalas_w(lay["T"][l], n, lay["nH"][l], lay["nHe"][l], lay["ne"][l])
+              if use_MHD else 1.0 for n in range(1, 31)]
+        H.brackett_lines(br[l], wls, lay["T"][l], lay["ne"][l], lay["nH"][l], lay["U"][l],
+                         lay["xi"][l], ws, 30)
+    # the Brackett lines are most of the hydrogen opacity here, and reach every layer
+    assert np.all(br.max(axis=1) > 0) and np.sum(br) > 0.5 * np.sum(ref)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+    got32 = device_hlines(ctx, wls, lay, 150e-8, use_MHD, dtype=np.float32)
+    np.testing.assert_allclose(got32, ref, rtol=1e-6, atol=0)
+
+
+@pytest.mark.gpu
+def test_device_rejects_missing_tables(ctx):
     from korg_line import Context
     atm, lay = solar_layers()
-    ok = Wavelengths((14400.0, 14410.0, 0.1)).all_wls  # ends 150 A short of 14,589 A: accepted
-    device_hlines(ctx, ok, lay, 150e-8, True)
-    wls = Wavelengths((14450.0, 14460.0, 0.1)).all_wls  # within 150 A of the Brackett limit
-    with pytest.raises(_lib.KorgError):
-        device_hlines(ctx, wls, lay, 150e-8, True)
     fresh = Context(0)
     fresh._htables = True  # skip the upload: the call must report the missing tables
     with pytest.raises(_lib.KorgError):
