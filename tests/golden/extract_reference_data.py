@@ -1,9 +1,9 @@
 """Extract the plain-text physical data this package needs from a Korg.jl checkout.
 
 Run once, in a container that has the reference checkout, to (re)generate the
-``*.tsv`` files next to this script:
+``*.tsv`` files of ``korg.jl_amd/korg_line/data``:
 
-    python korg.jl_amd/korg_line/data/extract_reference_data.py /root/reference
+    python tests/golden/extract_reference_data.py /root/reference
 
 Only data is extracted (numbers from Korg's data files), never code:
 
@@ -18,7 +18,8 @@ Only data is extracted (numbers from Korg's data files), never code:
 import os
 import sys
 
-HERE = os.path.dirname(os.path.abspath(__file__))
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "korg.jl_amd",
+                    "korg_line", "data")
 
 PARTITION_SPECIES = ["H_I", "H_II", "He_I", "He_II", "Ca_I", "Ca_II", "Ca_III", "Ti_I", "Ti_II",
                      "Ti_III", "Fe_I", "Fe_II", "Fe_III", "Ni_I", "Ni_II", "Ni_III"]
