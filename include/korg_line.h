@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 12
+#define KORG_LINE_ABI_VERSION 13
 
 enum korg_status {
     KORG_OK = 0,
@@ -537,6 +537,36 @@ int korg_molecular_cross_sections_device(korg_ctx* ctx, void* alpha, int alpha_d
                                          const double* temps, const double* vmic,
                                          int64_t n_tables, const korg_mol_xsection* tables,
                                          int synchronize);
+
+/*
+ * filter_linelist (src/synthesize.jl:331-350) on the device, for a batch of stars drawn from one
+ * master list in HBM.  Star k keeps, for each of its wavelength ranges r in order, the master lines
+ * searchsortedfirst(wl, first_r - line_buffer) .. searchsortedlast(wl, last_r + line_buffer), a
+ * range starting no earlier than one past the previous range's searchsortedlast (so no line is
+ * kept twice), in master order -- exactly the reference's selection.
+ *
+ * master: device arrays, sorted by wavelength (window_lb/ub must be NULL: frozen windows belong
+ * to one star's list).  stars: host array of n_stars; their ranges are host arrays.  out: host
+ * array of n_stars korg_lines, filled with device pointers and species indices of the master's
+ * species table, windows NULL.  A star whose kept lines form one contiguous run of the master
+ * (every single-range star) gets pointers into the master's arrays (no copy); the others get their
+ * lines gathered into `pool`, a caller-owned device buffer of KORG_FILTER_POOL_BYTES(pool_lines)
+ * bytes (7 Float64 columns wl, log_gf, E_lower, gamma_rad, gamma_stark, vdW1, vdW2, then the
+ * int32 species column, each pool_lines long).  *pool_lines_needed receives the pool size the call
+ * needs; when pool_lines is smaller the call returns KORG_ERR_CAPACITY and writes neither pool nor
+ * out (pool may be NULL then: a size query).  The outputs stay valid while master and pool do.
+ * Synchronous (the selection sizes come back to the host).
+ */
+typedef struct {
+    int32_t n_ranges;
+    const double* ranges;  /* host [2 * n_ranges]: first and last wavelength of each range, cm,
+                            * ascending (eachwindow(wls), src/wavelengths.jl:159) */
+    double line_buffer;    /* cm */
+} korg_filter_star;
+#define KORG_FILTER_POOL_BYTES(n) ((int64_t)(n) * 60)
+int korg_filter_linelist_batched(korg_ctx* ctx, const korg_lines* master, int64_t n_stars,
+                                 const korg_filter_star* stars, void* pool, int64_t pool_lines,
+                                 korg_lines* out, int64_t* pool_lines_needed);
 
 #ifdef __cplusplus
 }

@@ -3509,6 +3509,8 @@ struct korg_ctx {
     DevBuf h_tan, h_dalpha;
     // molecular cross-sections: the device table structs, and the host entry point's staging
     DevBuf mol_tabs, h_mol;
+    // filter_linelist: search keys, search results, gather runs
+    DevBuf flt_keys, flt_found, flt_runs;
     // hydrogen-line tables (korg_set_hydrogen_tables) and the per-call pairs and profile slices
     DevBuf hl_trans, hl_T, hl_ne, hl_dnu, hl_logP, hl_lam0, hl_pairs, hl_slices, hl_hull;
     DevBuf hl_brec, hl_bprof, hl_bhull;  // Brackett series
@@ -5058,7 +5060,7 @@ void korg_destroy(korg_ctx* ctx) {
     for (auto& b : ctx->h_lines)
         if (b.p) (void)hipFree(b.p);
     for (DevBuf* b : {&ctx->h_win[0], &ctx->h_win[1], &ctx->h_tan, &ctx->h_dalpha, &ctx->mol_tabs,
-                      &ctx->h_mol})
+                      &ctx->h_mol, &ctx->flt_keys, &ctx->flt_found, &ctx->flt_runs})
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->h_layers)
         if (b.p) (void)hipFree(b.p);
@@ -5728,6 +5730,177 @@ int korg_molecular_cross_sections(korg_ctx* ctx, void* alpha, int alpha_dtype,
     HIP_TRY(ctx, hipMemcpyAsync(alpha, da, esz * (size_t)(L * n), hipMemcpyDeviceToHost,
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return KORG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// filter_linelist (src/synthesize.jl:331-350) for stars drawn from one master list in HBM.
+// k_filter_search: one thread per (star, range) key pair, the two binary searches of the master's
+// wavelengths; the host then applies the reference's no-double-count rule in range order, merges
+// adjacent runs, and k_filter_gather copies the lines of stars with more than one run into the
+// caller's pool (one thread per pool line, its run found by binary search over the run table).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(256) void k_filter_search(const double* wl, int64_t n,
+                                                       const double* keys, int64_t n_pairs,
+                                                       int64_t* found) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_pairs) return;
+    // searchsortedfirst (1-based) of first - buffer, searchsortedlast of last + buffer
+    found[2 * t] = lower_bound_f64(wl, n, keys[2 * t]) + 1;
+    found[2 * t + 1] = upper_bound_f64(wl, n, keys[2 * t + 1]);
+}
+
+struct FilterRun {
+    int64_t src, dst;  // 0-based first master line and first pool line of a gathered run
+};
+
+__global__ __launch_bounds__(256) void k_filter_gather(korg_lines m, const FilterRun* runs,
+                                                       int64_t n_runs, int64_t total,
+                                                       int64_t cap, char* pool) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    int64_t lo = 0, hi = n_runs - 1;  // the last run with dst <= t (runs[0].dst == 0)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (runs[mid].dst <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t s = runs[lo].src + (t - runs[lo].dst);
+    double* col = (double*)pool;
+    const double* src[7] = {m.wl, m.log_gf, m.E_lower, m.gamma_rad, m.gamma_stark, m.vdW1, m.vdW2};
+#pragma unroll
+    for (int f = 0; f < 7; ++f) col[f * cap + t] = src[f][s];
+    ((int32_t*)(col + 7 * cap))[t] = m.species[s];
+}
+
+}  // namespace
+
+extern "C" {
+
+int korg_filter_linelist_batched(korg_ctx* ctx, const korg_lines* master, int64_t n_stars,
+                                 const korg_filter_star* stars, void* pool, int64_t pool_lines,
+                                 korg_lines* out, int64_t* pool_lines_needed) {
+    if (!ctx) return KORG_ERR_INVALID_ARGUMENT;
+    ctx->err.clear();
+    if (!master || master->n_lines < 0 || n_stars < 0 || (n_stars > 0 && (!stars || !out)) ||
+        pool_lines < 0)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "filter_linelist: master, stars and out are required");
+    const korg_lines& m = *master;
+    if (m.n_lines > 0 && (!m.wl || !m.log_gf || !m.E_lower || !m.gamma_rad || !m.gamma_stark ||
+                          !m.vdW1 || !m.vdW2 || !m.species))
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "filter_linelist: a master column is missing");
+    if (m.window_lb || m.window_ub)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                       "filter_linelist: the master list cannot carry frozen windows");
+    std::vector<double> keys;
+    std::vector<int64_t> first_pair((size_t)n_stars + 1, 0);
+    for (int64_t k = 0; k < n_stars; ++k) {
+        const korg_filter_star& s = stars[k];
+        if (s.n_ranges < 0 || (s.n_ranges > 0 && !s.ranges) || !std::isfinite(s.line_buffer))
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                           "filter_linelist: star %lld needs its ranges and a finite buffer", (long long)k);
+        for (int32_t r = 0; r < s.n_ranges; ++r) {
+            const double a = s.ranges[2 * r], b = s.ranges[2 * r + 1];
+            if (!(a <= b) || !std::isfinite(a) || !std::isfinite(b) ||
+                (r > 0 && !(s.ranges[2 * r - 1] <= a)))
+                return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
+                               "filter_linelist: star %lld: ranges must be finite and ascending",
+                               (long long)k);
+            keys.push_back(a - s.line_buffer);  // src/synthesize.jl:336
+            keys.push_back(b + s.line_buffer);  // :340
+        }
+        first_pair[k + 1] = first_pair[k] + s.n_ranges;
+    }
+    const int64_t n_pairs = first_pair[n_stars];
+    std::vector<int64_t> found((size_t)(2 * n_pairs));
+    if (n_pairs > 0 && m.n_lines > 0) {
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        TRY(grow(ctx, ctx->flt_keys, sizeof(double) * keys.size()));
+        TRY(grow(ctx, ctx->flt_found, sizeof(int64_t) * found.size()));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->flt_keys.p, keys.data(), sizeof(double) * keys.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_filter_search, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0,
+                           ctx->stream, m.wl, m.n_lines, (const double*)ctx->flt_keys.p, n_pairs,
+                           (int64_t*)ctx->flt_found.p);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipMemcpyAsync(found.data(), ctx->flt_found.p, sizeof(int64_t) * found.size(),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        for (int64_t i = 0; i < n_pairs; ++i) {  // an empty master: searchsortedfirst 1, last 0
+            found[2 * i] = 1;
+            found[2 * i + 1] = 0;
+        }
+    }
+    // the reference's selection, as 0-based runs [first, end) of the master per star
+    struct Run { int64_t first, end; };
+    std::vector<std::vector<Run>> runs((size_t)n_stars);
+    std::vector<FilterRun> gather;
+    int64_t need = 0;
+    for (int64_t k = 0; k < n_stars; ++k) {
+        int64_t last = 0;  // last_line_index, 1-based (src/synthesize.jl:334)
+        for (int64_t p = first_pair[k]; p < first_pair[k + 1]; ++p) {
+            const int64_t f = std::max(found[2 * p], last + 1);  // :338
+            last = found[2 * p + 1];                             // :340
+            if (f > last) continue;
+            std::vector<Run>& rk = runs[k];
+            if (!rk.empty() && rk.back().end == f - 1) rk.back().end = last;  // adjacent: one run
+            else rk.push_back({f - 1, last});
+        }
+        if (runs[k].size() > 1)
+            for (const Run& r : runs[k]) {
+                gather.push_back({r.first, need});
+                need += r.end - r.first;
+            }
+    }
+    if (pool_lines_needed) *pool_lines_needed = need;
+    if (need > pool_lines)
+        return set_err(ctx, KORG_ERR_CAPACITY,
+                       "filter_linelist: the pool holds %lld lines, %lld are needed",
+                       (long long)pool_lines, (long long)need);
+    if (need > 0 && !pool)
+        return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "filter_linelist: pool is NULL");
+    if (need > 0) {
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        TRY(grow(ctx, ctx->flt_runs, sizeof(FilterRun) * gather.size()));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->flt_runs.p, gather.data(), sizeof(FilterRun) * gather.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_filter_gather, dim3((unsigned)((need + 255) / 256)), dim3(256), 0,
+                           ctx->stream, m, (const FilterRun*)ctx->flt_runs.p,
+                           (int64_t)gather.size(), need, pool_lines, (char*)pool);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    const double* col = (const double*)pool;
+    int64_t g = 0;
+    for (int64_t k = 0; k < n_stars; ++k) {
+        korg_lines o = m;
+        o.window_lb = o.window_ub = nullptr;
+        const std::vector<Run>& rk = runs[k];
+        if (rk.size() <= 1) {  // a view into the master (nothing kept: zero lines at its start)
+            const int64_t f = rk.empty() ? 0 : rk[0].first;
+            o.n_lines = rk.empty() ? 0 : rk[0].end - f;
+            if (m.n_lines > 0) {
+                o.wl += f; o.log_gf += f; o.E_lower += f; o.gamma_rad += f; o.gamma_stark += f;
+                o.vdW1 += f; o.vdW2 += f; o.species += f;
+            }
+        } else {
+            const int64_t d = gather[g].dst;
+            o.n_lines = 0;
+            for (const Run& r : rk) o.n_lines += r.end - r.first;
+            g += (int64_t)rk.size();
+            const double* c[7];
+            for (int f = 0; f < 7; ++f) c[f] = col + f * pool_lines + d;
+            o.wl = c[0]; o.log_gf = c[1]; o.E_lower = c[2]; o.gamma_rad = c[3];
+            o.gamma_stark = c[4]; o.vdW1 = c[5]; o.vdW2 = c[6];
+            o.species = (const int32_t*)(col + 7 * pool_lines) + d;
+        }
+        out[k] = o;
+    }
     return KORG_OK;
 }
 

@@ -80,6 +80,15 @@ class KorgMolXsection(ctypes.Structure):
                 ("n_species", _c_dp)]
 
 
+class KorgFilterStar(ctypes.Structure):
+    _fields_ = [("n_ranges", ctypes.c_int32), ("ranges", _c_dp), ("line_buffer", ctypes.c_double)]
+
+
+def filter_pool_bytes(n):
+    """KORG_FILTER_POOL_BYTES: bytes of a filter pool of n lines."""
+    return 60 * int(n)
+
+
 class KorgStar(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_void_p), ("grid", KorgGrid), ("lines", KorgLines),
                 ("layers", KorgLayers), ("cntm", KorgContinuum),
@@ -131,7 +140,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 12  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 13  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",
@@ -143,7 +152,8 @@ EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version"
             "korg_continuum_batched", "korg_flux_batched", "korg_set_hydrogen_tables",
             "korg_hydrogen_line_absorption", "korg_line_absorption_device_noalloc",
             "korg_line_absorption_jvp", "korg_line_absorption_jvp_device",
-            "korg_molecular_cross_sections", "korg_molecular_cross_sections_device"]
+            "korg_molecular_cross_sections", "korg_molecular_cross_sections_device",
+            "korg_filter_linelist_batched"]
 
 _lib = None
 
@@ -211,6 +221,10 @@ def load():
     L.korg_molecular_cross_sections.restype = i32
     L.korg_molecular_cross_sections_device.argtypes = mol + [i32]
     L.korg_molecular_cross_sections_device.restype = i32
+    L.korg_filter_linelist_batched.argtypes = [vp, ctypes.POINTER(KorgLines), i64,
+                                               ctypes.POINTER(KorgFilterStar), vp, i64,
+                                               ctypes.POINTER(KorgLines), ctypes.POINTER(i64)]
+    L.korg_filter_linelist_batched.restype = i32
     L.korg_line_absorption_batched.argtypes = [vp, i64, ctypes.POINTER(KorgStar),
                                                ctypes.POINTER(KorgSpecies), i32, i32, i32]
     L.korg_line_absorption_batched.restype = i32

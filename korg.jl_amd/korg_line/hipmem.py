@@ -46,6 +46,15 @@ def synchronize():
     _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def read(ptr, n, dtype):
+    """``n`` elements of ``dtype`` at the device address ``ptr``, copied into a new numpy array."""
+    out = np.empty(int(n), dtype=dtype)
+    if out.nbytes:
+        _check(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr),
+                               out.nbytes, D2H), "hipMemcpy D2H")
+    return out
+
+
 class DeviceArray:
     """A device buffer holding an array of ``shape`` and ``dtype`` (C order)."""
 

@@ -52,23 +52,42 @@ def _host_structs(P):
     return (g, lines, sp, ly, cn), keep
 
 
-class DeviceProblem:
-    """A ``Problem`` uploaded once to device memory (HBM-resident inputs)."""
+_LINE_FIELDS = ("wl", "log_gf", "E_lower", "gamma_rad", "gamma_stark", "vdW1", "vdW2", "species")
 
-    def __init__(self, problem):
+
+class DeviceProblem:
+    """A ``Problem`` uploaded once to device memory (HBM-resident inputs).
+
+    ``lines``: optionally a ``korg_lines`` of device pointers already in HBM (one star's list from
+    ``Context.filter_linelist_batched``) used in place of the problem's own lines, which are then
+    not uploaded; the problem's line list only supplies the species table, which must be the one
+    those lines index, and its continuum must be given on knots (per-line values would belong to
+    the unfiltered list).  ``keep``: the object that owns that memory, held as long as this
+    problem."""
+
+    def __init__(self, problem, lines=None, keep=None):
         from .hipmem import DeviceArray
         P = problem
         ll = P.lines
         self.problem = P
-        self.t = {k: DeviceArray.from_numpy(v) for k, v in self._arrays(P).items()}
+        self._owner = keep
+        if lines is not None and P.cntm_mode != _lib.CNTM_GRID:
+            raise ValueError("device lines need a continuum on knots")
+        if lines is not None and P.window_lb is not None:
+            raise ValueError("device lines carry no frozen windows")
+        self.t = {k: DeviceArray.from_numpy(v) for k, v in self._arrays(P).items()
+                  if lines is None or k not in _LINE_FIELDS}
         t = self.t
         d = lambda k: t[k].ptr
         self.grid = _lib.KorgGrid(P.n_wl, d("all_wls"))
         frozen = "window_lb" in t
-        self.lines = _lib.KorgLines(len(ll), d("wl"), d("log_gf"), d("E_lower"), d("gamma_rad"),
-                                    d("gamma_stark"), d("vdW1"), d("vdW2"), d("species"),
-                                    d("window_lb") if frozen else None,
-                                    d("window_ub") if frozen else None)
+        if lines is not None:
+            self.lines = lines
+        else:
+            self.lines = _lib.KorgLines(len(ll), d("wl"), d("log_gf"), d("E_lower"),
+                                        d("gamma_rad"), d("gamma_stark"), d("vdW1"), d("vdW2"),
+                                        d("species"), d("window_lb") if frozen else None,
+                                        d("window_ub") if frozen else None)
         self.species = _lib.KorgSpecies(P.species_mass.size, d("mass"), d("is_mol"))
         self.layers = _lib.KorgLayers(P.n_layers, d("temps"), d("n_e"), d("n_div_U"), d("n_HI"),
                                       d("n_HeI"), d("n_H2"), d("xi"))
@@ -114,6 +133,55 @@ class DeviceProblem:
         if fill is None:
             return DeviceArray.zeros((P.n_wl, P.n_layers), dt)
         return DeviceArray.from_numpy(np.asarray(fill, dtype=dt).T)
+
+
+class DeviceLineList:
+    """A wavelength-sorted ``LineList`` in device memory: the master list of
+    ``Context.filter_linelist_batched``."""
+
+    def __init__(self, linelist):
+        from .hipmem import DeviceArray
+        ll = linelist
+        if not ll.is_sorted():
+            raise ValueError("the master list must be sorted by wavelength")
+        self.linelist = ll
+        cols = dict(wl=ll.wl, log_gf=ll.log_gf, E_lower=ll.E_lower, gamma_rad=ll.gamma_rad,
+                    gamma_stark=ll.gamma_stark, vdW1=ll.vdW1, vdW2=ll.vdW2,
+                    species=ll.species_index.astype(np.int32))
+        self.t = {k: DeviceArray.from_numpy(np.ascontiguousarray(v)) for k, v in cols.items()}
+        self.lines = _lib.KorgLines(len(ll), *[self.t[k].ptr for k in _LINE_FIELDS], None, None)
+
+
+class FilteredLines:
+    """The per-star line lists of ``Context.filter_linelist_batched``: ``lines[k]`` (a
+    ``korg_lines`` of device pointers into the master list or ``pool``), valid while this object
+    lives (it holds the master and the pool)."""
+
+    def __init__(self, master, lines, pool):
+        self.master = master
+        self.lines = lines
+        self.pool = pool
+
+    def __len__(self):
+        return len(self.lines)
+
+    def n_lines(self, k):
+        return int(self.lines[k].n_lines)
+
+    def to_host(self, k):
+        """Star k's list copied back to the host, as a ``LineList`` over the master's species."""
+        from .hipmem import read
+        from .linelist import LineList
+        L = self.lines[k]
+        n = L.n_lines
+        f = [read(getattr(L, name), n, np.float64) for name in _LINE_FIELDS[:-1]]
+        sp = read(L.species, n, np.int32)
+        return LineList(*f, sp, self.master.linelist.species)
+
+    def problem(self, k, problem):
+        """A ``DeviceProblem`` of ``problem`` (built over the master list's species table, its
+        continuum on knots) with star k's filtered lines."""
+        return DeviceProblem(problem, lines=self.lines[k], keep=self)
 
 
 class FitTarget:
@@ -309,6 +377,40 @@ class Context:
             stars[k] = _lib.KorgStar(a.ptr, d.grid, d.lines, d.layers, d.cntm,
                                      d.problem.cutoff_threshold)
         return stars, dt[next(iter(dtypes))]
+
+    def filter_linelist_batched(self, master, wls_list, line_buffer):
+        """``filter_linelist`` (``src/synthesize.jl:331-350``) on the device for many stars drawn
+        from one master list (a ``DeviceLineList``): star k keeps the master's lines within
+        ``line_buffer`` (cm; a scalar or one per star) of each range of ``wls_list[k]`` (a
+        ``Wavelengths``), each line once, in order.  Returns a ``FilteredLines``.  Warns, as the
+        reference does, for a star that keeps nothing of a non-empty master."""
+        import warnings
+        from .hipmem import DeviceArray
+        n = len(wls_list)
+        buf = np.broadcast_to(np.asarray(line_buffer, dtype=np.float64), (n,))
+        ranges = []
+        stars = (_lib.KorgFilterStar * max(n, 1))()
+        for k, w in enumerate(wls_list):
+            r = np.ascontiguousarray(np.array(w.eachwindow(), dtype=np.float64).ravel())
+            ranges.append(r)
+            stars[k] = _lib.KorgFilterStar(r.size // 2, r.ctypes.data, float(buf[k]))
+        out = (_lib.KorgLines * max(n, 1))()
+        need = ctypes.c_int64(0)
+        rc = self._L.korg_filter_linelist_batched(self._h, ctypes.byref(master.lines), n, stars,
+                                                  None, 0, out, ctypes.byref(need))
+        pool = None
+        if rc == _lib.KORG_ERR_CAPACITY:
+            pool = DeviceArray((_lib.filter_pool_bytes(need.value),), np.uint8)
+            rc = self._L.korg_filter_linelist_batched(self._h, ctypes.byref(master.lines), n,
+                                                      stars, ctypes.c_void_p(pool.ptr),
+                                                      need.value, out, ctypes.byref(need))
+        _lib.check(self._h, rc)
+        del ranges
+        lines = [out[k] for k in range(n)]
+        if len(master.linelist) and any(L.n_lines == 0 for L in lines):
+            warnings.warn("The provided linelist was not empty, but none of the lines were "
+                          "within the provided wavelength range.")
+        return FilteredLines(master, lines, pool)
 
     def line_absorption_batched(self, alphas, dprobs, precision="fp32", synchronize=True):
         """Many stars in one pipeline: ``alphas[k]`` (a ``DeviceProblem.new_alpha`` array) receives

@@ -354,10 +354,8 @@ def filter_linelist(linelist, wls, line_buffer):
     list) within ``line_buffer`` cm of each of the grid's ranges, each line at most once, in order.
     """
     wl = linelist.wl
-    idx, last, start = [], 0, 0
-    for _, _, n in wls.wl_ranges:
-        lo, hi = wls.all_wls[start], wls.all_wls[start + n - 1]
-        start += n
+    idx, last = [], 0
+    for lo, hi in wls.eachwindow():
         first = int(np.searchsorted(wl, lo - line_buffer, side="left")) + 1  # 1-based
         first = max(first, last + 1)
         last = int(np.searchsorted(wl, hi + line_buffer, side="right"))

@@ -130,6 +130,14 @@ class Wavelengths:
         lam = lam * 1e-8 if lam >= 1 else lam
         return int(np.searchsorted(self.all_wls, lam, side="right"))
 
+    def eachwindow(self):
+        """(first, last) wavelength of each range, cm (``src/wavelengths.jl:159``)."""
+        out, i = [], 0
+        for _, _, n in self.wl_ranges:
+            out.append((float(self.all_wls[i]), float(self.all_wls[i + n - 1])))
+            i += n
+        return out
+
     def subspectrum_indices(self):
         out, i = [], 1
         for _, _, n in self.wl_ranges:

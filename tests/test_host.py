@@ -155,7 +155,7 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 12
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 13
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -170,12 +170,14 @@ def test_struct_layouts_match_header(tmp_path):
                "korg_fit_target": _lib.KorgFitTarget, "korg_flux_target": _lib.KorgFluxTarget,
                "korg_htransition": _lib.KorgHTransition, "korg_htables": _lib.KorgHTables,
                "korg_hlayers": _lib.KorgHLayers, "korg_layer_tangents": _lib.KorgLayerTangents,
-               "korg_mol_xsection": _lib.KorgMolXsection}
+               "korg_mol_xsection": _lib.KorgMolXsection,
+               "korg_filter_star": _lib.KorgFilterStar}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "korg_line.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for fname, _ in cls._fields_:
             lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append('printf("pool_bytes 1000 %lld\\n", (long long)KORG_FILTER_POOL_BYTES(1000));')
     lines.append("return 0; }")
     src = tmp_path / "layout.c"
     src.write_text("\n".join(lines))
@@ -186,6 +188,7 @@ def test_struct_layouts_match_header(tmp_path):
     for row in subprocess.check_output([str(exe)], text=True).splitlines():
         cname, what, val = row.split()
         got[(cname, what)] = int(val)
+    assert got[("pool_bytes", "1000")] == _lib.filter_pool_bytes(1000)
     for cname, cls in structs.items():
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
