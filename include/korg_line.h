@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 13
+#define KORG_LINE_ABI_VERSION 14
 
 enum korg_status {
     KORG_OK = 0,
@@ -252,6 +252,8 @@ typedef struct {
     int32_t tile_points;       /* Float32 far wing: wavelength points per tile, 32, 64 or 128
                                   (default 64) */
     int32_t near_waves_per_cu; /* Float32 near zone: persistent waves per compute unit (default 16) */
+    int32_t near_layout;       /* Float32 near zone: 1 = lanes over 64 layers, one point at a time;
+                                  2 = lanes over 8 layers x 8 points (default); same alpha */
 } korg_tuning;
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
 int korg_get_tuning(const korg_ctx* ctx, korg_tuning* out);

@@ -1745,6 +1745,94 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
     }
 }
 
+// The same near zone with lanes = 8 layers x 8 points: lane (r, c) evaluates layer 8g + r of the
+// block at point 8p + c of the tile for each layer group g and point group p.  Layers of one
+// group have nearly the same damping and Doppler width, so the Hunger branch (and the Harris
+// piece) is uniform over a group's 8 x 8 evaluations away from the few boundaries, where the
+// one-point-at-a-time form above runs every branch that any of its 64 layers takes.  The line's
+// per-layer records come in as one coalesced load per lane (its own layer, as above) and are
+// shared through LDS.  Each (layer, point) accumulates the same terms in the same (line) order as
+// k_near: the two forms give identical alpha.
+constexpr int NEAR_GROUPS = WAVE / 8;  // layer groups per 64-layer block
+constexpr int NEAR_PGROUPS = CT / 8;   // point groups per tile
+
+__global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
+    __shared__ double4 s_q[WAVE];
+    const int lane = threadIdx.x;
+    const int r = lane & 7, c = lane >> 3;
+    const int L = A.L;
+    if (A.counters[3] != 0) return;
+    for (;;) {
+        const int item = wave_take(&A.counters[5], lane);
+        if (item >= A.n_items) break;
+        const int tile = item / A.n_lblocks;
+        const int lbase = (item % A.n_lblocks) * WAVE;
+        const int nl = min(WAVE, L - lbase);  // layers of this block
+        const int l_own = lbase + lane;       // the layer whose record this lane loads
+        const bool own_v = lane < nl;
+        const int64_t j0 = (int64_t)tile * CT;
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        const int64_t j1 = min(j0 + CT, go + S.n_wl) - 1;
+        if (j1 < j0) continue;  // padding between two stars' grids: no points
+        double lam[NEAR_PGROUPS];  // this lane's points 8p + c (clamped into the tile: in bounds)
+        const double* wt = S.wls + (j0 - go);
+#pragma unroll
+        for (int p = 0; p < NEAR_PGROUPS; ++p) lam[p] = wt[min((int64_t)(8 * p + c), j1 - j0)];
+        float acc[NEAR_GROUPS][NEAR_PGROUPS];
+#pragma unroll
+        for (int g = 0; g < NEAR_GROUPS; ++g)
+#pragma unroll
+            for (int p = 0; p < NEAR_PGROUPS; ++p) acc[g][p] = 0.0f;
+        const double4* __restrict__ rq = A.rec64 + (own_v ? l_own : 0);
+        auto load_q = [&](int line) {
+            return own_v ? rq[(int64_t)line * A.Ls] : make_double4(0.0, 0.0, 0.0, 0.0);
+        };
+        const int j0s = (int)j0;
+        for_each_line_pf<double4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_NEAR,
+                                  load_q, [&](const LineView& v, const double4& q) {
+            // LDS operations of one wave complete in order: the reads below see this write, and
+            // the previous line's reads were issued before it
+            s_q[lane] = q;
+            const int tlo = v.clo - j0s, thi = v.chi - j0s;
+#pragma unroll
+            for (int g = 0; g < NEAR_GROUPS; ++g) {
+                if (8 * g >= nl) break;  // wavefront-uniform
+                const double4 qq = s_q[8 * g + r];
+                const bool gv = 8 * g + r < nl;
+                const float a32 = (float)qq.y, sc = (float)qq.z;
+#pragma unroll
+                for (int p = 0; p < NEAR_PGROUPS; ++p) {
+                    if (8 * p + 7 < tlo || 8 * p > thi) continue;  // wavefront-uniform
+                    const int t = 8 * p + c;
+                    if (!gv || t < tlo || t > thi) continue;
+                    const double v64 = fabs(lam[p] - v.lam0) * qq.x;
+                    const int br = hunger_branch(qq.y, v64);
+                    float h;
+                    if (br == 1 || br == 2) {
+                        const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
+                        h = voigt_core_f32(a32, (float)v64, piece, br == 2);
+                    } else {
+                        h = voigt_far_f32(a32, (float)v64, br == 3);
+                    }
+                    acc[g][p] += h * sc;
+                }
+            }
+        });
+#pragma unroll
+        for (int g = 0; g < NEAR_GROUPS; ++g) {
+            const int l = lbase + 8 * g + r;
+            if (8 * g + r >= nl) continue;
+#pragma unroll
+            for (int p = 0; p < NEAR_PGROUPS; ++p) {
+                const int64_t j = j0 + 8 * p + c;
+                if (j <= j1) add_alpha(S.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * (j - go), acc[g][p]);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Adjoint (vector-Jacobian product) of the line opacity, in Float64, over a batch of stars.  Given
 // G = dL/dalpha per star, an item is (line, chunk of up to VJP_CHUNK window points); lanes =
@@ -3713,7 +3801,7 @@ struct Batch {
     int64_t N = 0, nwl = 0;
     bool f32 = false, alpha_f32 = false;
     bool adj = false;  // Float32 fit step: the adjoint's series form (near zones, moments)
-    int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, count_branches = 0;
+    int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, near_layout = 2, count_branches = 0;
     int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
     korg_species sp{};
 };
@@ -3727,7 +3815,7 @@ int check_tuning(korg_ctx* ctx, const korg_tuning* t) {
     const int tp = t->tile_points, wt = t->waves_per_tile;
     if (t->points_per_lane < 0 || t->points_per_lane > 2 || t->waves_per_cu < 0 ||
         t->waves_per_cu > 64 || t->near_waves_per_cu < 0 || t->near_waves_per_cu > 64 ||
-        t->count_branches < 0 || t->count_branches > 1 ||
+        t->count_branches < 0 || t->count_branches > 1 || t->near_layout < 0 || t->near_layout > 2 ||
         !(tp == 0 || tp == 32 || tp == 64 || tp == 128) || !(wt == 0 || wt == 1 || wt == 2 || wt == 4))
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "tuning out of range");
     if (!wing_variant_ok(tp ? tp / 32 : 2, wt ? wt : 4))
@@ -3771,6 +3859,7 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.P64 = t.points_per_lane == 2 ? 2 : 1;
     b.wpc = t.waves_per_cu > 0 ? t.waves_per_cu : 12;
     b.near_wpc = t.near_waves_per_cu > 0 ? t.near_waves_per_cu : 16;
+    b.near_layout = t.near_layout > 0 ? t.near_layout : 2;
     b.count_branches = t.count_branches ? 1 : 0;
     TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
     b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
@@ -3983,7 +4072,8 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         na.counters = counters;
         int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
         if (cwaves > na.n_items) cwaves = na.n_items;
-        hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        if (b.near_layout == 1) hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else hipLaunchKernelGGL(k_near2, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));

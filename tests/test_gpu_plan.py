@@ -5,7 +5,8 @@ A replay must add exactly what the direct batched call adds for the same buffers
 (bitwise): the graph holds the same kernels in the same order.  Replays are checked one by one,
 with direct calls and host synchronisation between them.  The tile-shape variants change the
 summation order of the far wing only, so each is checked against the Float64 form at the
-Float32 tolerance of the parity tests (1e-4 relative).
+Float32 tolerance of the parity tests (1e-4 relative).  The near zone's two lane layouts add the
+same terms in the same order, so they are checked for identical alpha.
 """
 import numpy as np
 import pytest
@@ -25,6 +26,16 @@ def _eager(ctx, probs, precision, dtype):
     alphas = [d.new_alpha(dtype) for d in dprobs]
     ctx.line_absorption_batched(alphas, dprobs, precision)
     return [a.numpy() for a in alphas]
+
+
+def _layers(P, n):
+    """P on n layers resampled from its own (repeats when n exceeds them)."""
+    idx = np.round(np.linspace(0, P.n_layers - 1, n)).astype(int)
+    c = np.ascontiguousarray
+    return Problem(P.all_wls, P.lines, P.species_mass, P.species_is_mol, c(P.n_div_U[:, idx]),
+                   c(P.temps[idx]), c(P.n_e[idx]), c(P.n_HI[idx]), c(P.n_HeI[idx]), c(P.n_H2[idx]),
+                   c(P.xi[idx]), P.cntm_mode, c(P.cntm_values[idx]), P.cntm_knots,
+                   P.cutoff_threshold)
 
 
 def _zero(alphas):
@@ -154,11 +165,28 @@ def test_tuning_variants_match_fp64(tile_points, waves_per_tile):
         assert np.array_equal(a.numpy(), got[1])
 
 
+@pytest.mark.parametrize("n_layers", [56, 20, 70, 130])
+def test_near_layouts_identical(n_layers):
+    """The near zone's two lane layouts (lanes over 64 layers, one point at a time; lanes over
+    8 layers x 8 points) accumulate the same terms in the same order: identical alpha, for layer
+    counts below, above and at several 64-layer blocks, on a batch of stars."""
+    probs = _batch(n_stars=3, seed=28, scale=0.03)
+    if n_layers != probs[0].n_layers:
+        probs = [_layers(P, n_layers) for P in probs]
+    with Context(0) as c:
+        out = []
+        for layout in (1, 2):
+            c.set_tuning(near_layout=layout)
+            out.append(_eager(c, probs, "fp32", "float32"))
+        for k, (x, y) in enumerate(zip(*out)):
+            assert np.any(x) and np.array_equal(x, y), k
+
+
 def test_invalid_tuning_rejected():
     from korg_line._lib import KorgError
     with Context(0) as c:
         for kw in ({"tile_points": 128, "waves_per_tile": 4}, {"tile_points": 48},
-                   {"waves_per_tile": 3}, {"near_waves_per_cu": 65}):
+                   {"waves_per_tile": 3}, {"near_waves_per_cu": 65}, {"near_layout": 3}):
             with pytest.raises(KorgError, match="tuning"):
                 c.set_tuning(**kw)
 
