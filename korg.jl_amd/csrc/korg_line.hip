@@ -1073,6 +1073,46 @@ __device__ inline void for_each_line_pf(const int2* __restrict__ ranges, const C
     }
 }
 
+// for_each_line_pf for the near zone with a one-line-deep pipeline (the operand of the next
+// visited line is loaded before body() runs for the current one): fewer operand registers than
+// the two-deep form, for a kernel whose occupancy is register-bound.  Same visiting order.
+template <class T, class Load, class Body>
+__device__ inline void for_each_near_line_pf1(const int2* __restrict__ ranges,
+                                              const Cand* __restrict__ cands, int ntiles, int tile,
+                                              int64_t j0, int64_t j1, int lane, Load&& load,
+                                              Body&& body) {
+    for (int c = 0; c < N_CLASSES; ++c) {
+        const int2 r = ranges[(int64_t)c * ntiles + tile];
+        if (r.x >= r.y) continue;
+        for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
+            const Cand cc = cands[min(k0 + lane, r.y - 1)];
+            const bool ovl = k0 + lane < r.y && cc.clo <= cc.chi && cc.clo <= j1 && cc.chi >= j0;
+            uint64_t mask = ballot(ovl);
+            int b0 = pop_lane(mask);
+            if (b0 < 0) continue;
+            T x0 = load(readlane_i32(cc.line, b0));
+            for (;;) {
+                const int b1 = pop_lane(mask);
+                // past the last line, this batch's first line again as a stand-in load
+                const T x1 = load(readlane_i32(cc.line, b1 >= 0 ? b1 : b0));
+                LineView v;
+                v.line = readlane_i32(cc.line, b0);
+                v.lam0 = readlane_f64(cc.lam0, b0);
+                v.lb = v.ub = v.flo = v.fhi = 0;
+                v.clo = readlane_i32(cc.clo, b0);
+                v.chi = readlane_i32(cc.chi, b0);
+                body(v, x0);
+                if (b1 < 0) break;
+                x0 = x1;
+                b0 = b1;
+            }
+            // retire the stand-in load, so that no load into the operand registers is pending
+            // when the next batch enters the loop
+            __builtin_amdgcn_s_waitcnt(VMCNT0);
+        }
+    }
+}
+
 __device__ inline int wave_take(unsigned long long* counter, int lane) {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(counter, 1ull);
@@ -1753,11 +1793,15 @@ __global__ __launch_bounds__(WAVE) void k_near(NearArgs A) {
 // per-layer records come in as one coalesced load per lane (its own layer, as above) and are
 // shared through LDS.  Each (layer, point) accumulates the same terms in the same (line) order as
 // k_near: the two forms give identical alpha.
-constexpr int NEAR_GROUPS = WAVE / 8;  // layer groups per 64-layer block
-constexpr int NEAR_PGROUPS = CT / 8;   // point groups per tile
+constexpr int NEAR_PGROUPS = CT / 8;  // point groups per tile
 
+// NG: layer groups per 64-layer block (7 when no block has more than 56 layers).  The line's
+// records go through LDS as (1/(sigma sqrt 2), a) and the scaling; one line's record load is in
+// flight while the previous line is evaluated (register-bound: four waves per SIMD).
+template <int NG>
 __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
-    __shared__ double4 s_q[WAVE];
+    __shared__ double2 s_q[WAVE];
+    __shared__ float s_sc[WAVE];
     const int lane = threadIdx.x;
     const int r = lane & 7, c = lane >> 3;
     const int L = A.L;
@@ -1780,9 +1824,9 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
         const double* wt = S.wls + (j0 - go);
 #pragma unroll
         for (int p = 0; p < NEAR_PGROUPS; ++p) lam[p] = wt[min((int64_t)(8 * p + c), j1 - j0)];
-        float acc[NEAR_GROUPS][NEAR_PGROUPS];
+        float acc[NG][NEAR_PGROUPS];
 #pragma unroll
-        for (int g = 0; g < NEAR_GROUPS; ++g)
+        for (int g = 0; g < NG; ++g)
 #pragma unroll
             for (int p = 0; p < NEAR_PGROUPS; ++p) acc[g][p] = 0.0f;
         const double4* __restrict__ rq = A.rec64 + (own_v ? l_own : 0);
@@ -1790,18 +1834,20 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
             return own_v ? rq[(int64_t)line * A.Ls] : make_double4(0.0, 0.0, 0.0, 0.0);
         };
         const int j0s = (int)j0;
-        for_each_line_pf<double4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, KEY_NEAR,
-                                  load_q, [&](const LineView& v, const double4& q) {
-            // LDS operations of one wave complete in order: the reads below see this write, and
-            // the previous line's reads were issued before it
-            s_q[lane] = q;
+        for_each_near_line_pf1<double4>(A.ranges, A.cands, A.ntiles, tile, j0, j1, lane, load_q,
+                                        [&](const LineView& v, const double4& q) {
+            // LDS operations of one wave complete in order: the reads below see these writes,
+            // and the previous line's reads were issued before them
+            s_q[lane] = make_double2(q.x, q.y);
+            s_sc[lane] = (float)q.z;
             const int tlo = v.clo - j0s, thi = v.chi - j0s;
 #pragma unroll
-            for (int g = 0; g < NEAR_GROUPS; ++g) {
+            for (int g = 0; g < NG; ++g) {
                 if (8 * g >= nl) break;  // wavefront-uniform
-                const double4 qq = s_q[8 * g + r];
+                const double2 qq = s_q[8 * g + r];
+                const float sc = s_sc[8 * g + r];
                 const bool gv = 8 * g + r < nl;
-                const float a32 = (float)qq.y, sc = (float)qq.z;
+                const float a32 = (float)qq.y;
 #pragma unroll
                 for (int p = 0; p < NEAR_PGROUPS; ++p) {
                     if (8 * p + 7 < tlo || 8 * p > thi) continue;  // wavefront-uniform
@@ -1821,7 +1867,7 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
             }
         });
 #pragma unroll
-        for (int g = 0; g < NEAR_GROUPS; ++g) {
+        for (int g = 0; g < NG; ++g) {
             const int l = lbase + 8 * g + r;
             if (8 * g + r >= nl) continue;
 #pragma unroll
@@ -4073,7 +4119,9 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
         if (cwaves > na.n_items) cwaves = na.n_items;
         if (b.near_layout == 1) hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else hipLaunchKernelGGL(k_near2, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else if (std::min(b.L, WAVE) <= 56)
+            hipLaunchKernelGGL(k_near2<7>, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else hipLaunchKernelGGL(k_near2<8>, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
