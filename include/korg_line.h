@@ -207,8 +207,11 @@ int korg_line_absorption_device_noalloc(korg_ctx* ctx, void* alpha, int alpha_dt
 
 /*
  * One star -- one line_absorption! call -- of a batch (device pointers throughout).  Its line
- * list is sorted by wavelength and indexes the batch's shared species table; every star has the
- * same number of layers.
+ * list is sorted by wavelength and indexes the batch's shared species table.  Stars may have
+ * different layer counts (Korg's interpolate_marcs drops layers, src/atmosphere.jl:461-479): a
+ * batch of several counts runs as one pipeline per count, in order of first appearance, each
+ * synchronized before the next (so such a call returns synchronized whatever `synchronize` asks);
+ * plans and the autotuner take one count.
  */
 typedef struct {
     void* alpha;              /* column-major [n_layers x grid.n_wl], the batch's alpha dtype */

@@ -3633,6 +3633,10 @@ struct korg_ctx {
     std::string scratch_err;
     Arena* arena = nullptr;
     bool last_fixed = false;
+    // a batch of several layer counts runs as one pipeline per count: the windows of all its
+    // lines, in the caller's star order (korg_get_windows reads them when last_ragged is set)
+    bool last_ragged = false;
+    std::vector<int64_t> ragged_lb, ragged_ub;
     std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
     // adjoint outputs of the host-memory entry point, and its staged G
     DevBuf vjp_out, h_grad;
@@ -3747,7 +3751,8 @@ struct StarHost {
     double* flux_c = nullptr;  // flux-only transfer: the continuum's flux
 };
 
-// Validates the stars of a batched call (one layer count, one species table for all).
+// Validates the stars of a batched call (one species table for all; *L_out: star 0's layer
+// count -- layer_groups splits a batch of several counts).
 int collect_stars(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                   const korg_species* species, int alpha_dtype, int precision,
                   std::vector<StarHost>& hs, int* L_out) {
@@ -3763,16 +3768,52 @@ int collect_stars(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
             ctx->err = "star " + std::to_string(k) + ": " + ctx->err;
             return rc;
         }
-        if (x.layers.n_layers != L)
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                           "star %lld has %d layers, star 0 has %d: one layer count per batch",
-                           (long long)k, x.layers.n_layers, L);
         if (!x.alpha && x.lines.n_lines > 0)
             return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: alpha is null", (long long)k);
         hs[k] = StarHost{x.alpha, x.grid, x.lines, x.layers, x.cntm, x.cutoff_threshold};
     }
     *L_out = L;
     return KORG_OK;
+}
+
+// The stars of a batch grouped by layer count, in order of first appearance (the stars of a group
+// in batch order): every pipeline runs over one count.
+std::vector<std::vector<int>> layer_groups(const std::vector<StarHost>& hs) {
+    std::vector<std::vector<int>> g;
+    std::vector<int> count;
+    for (int k = 0; k < (int)hs.size(); ++k) {
+        const int L = hs[k].layers.n_layers;
+        size_t i = 0;
+        while (i < count.size() && count[i] != L) ++i;
+        if (i == count.size()) {
+            count.push_back(L);
+            g.emplace_back();
+        }
+        g[i].push_back(k);
+    }
+    return g;
+}
+
+std::vector<StarHost> subset(const std::vector<StarHost>& hs, const std::vector<int>& idx) {
+    std::vector<StarHost> out;
+    out.reserve(idx.size());
+    for (int k : idx) out.push_back(hs[k]);
+    return out;
+}
+
+void add_stats(korg_stats& a, const korg_stats& b) {
+    a.n_evals += b.n_evals;
+    a.n_lines_used += b.n_lines_used;
+    a.setup_ms += b.setup_ms;
+    a.bin_ms += b.bin_ms;
+    a.profile_ms += b.profile_ms;
+    a.core_ms += b.core_ms;
+    a.total_ms += b.total_ms;
+    for (int k = 0; k < 4; ++k) a.n_evals_branch[k] += b.n_evals_branch[k];
+    a.n_evals_near += b.n_evals_near;
+    a.n_wing_mfma += b.n_wing_mfma;
+    a.n_evals_tail += b.n_evals_tail;
+    a.n_evals_near_A += b.n_evals_near_A;
 }
 
 // Lays the stars out in the global index spaces (grids padded to STAR_ALIGN points, lines
@@ -4200,6 +4241,7 @@ int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_spec
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
     ctx->last_n_lines = 0;
     ctx->last_fixed = fixed;
+    ctx->last_ragged = false;
     if (!fixed) {
         TRY(plan_batch(ctx, ctx->tuning, n_stars, stars, sp, L, alpha_dtype, precision, ws, s, b));
     } else {
@@ -4572,10 +4614,9 @@ int collect_rt_stars(korg_ctx* ctx, int64_t n_stars, const korg_star* stars, int
     hs.resize((size_t)n_stars);
     for (int64_t k = 0; k < n_stars; ++k) {
         const korg_star& x = stars[k];
-        if (x.layers.n_layers != L || L < (transfer ? 2 : 1))
-            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
-                           "star %lld: %d layers (one count per batch, at least %d)", (long long)k,
-                           x.layers.n_layers, transfer ? 2 : 1);
+        if (x.layers.n_layers < (transfer ? 2 : 1))
+            return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: %d layers (at least %d)",
+                           (long long)k, x.layers.n_layers, transfer ? 2 : 1);
         if (x.grid.n_wl < 1 || !x.grid.all_wls || !x.alpha)
             return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "star %lld: empty grid or null alpha",
                            (long long)k);
@@ -4758,6 +4799,7 @@ int run_fit(korg_ctx* ctx, std::vector<StarHost>& hs, const korg_species* sp, in
     hipStream_t s = ctx->stream;
     Workspace& ws = ctx->ws;
     std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    ctx->last_ragged = false;
     TRY(place_fit_G(ctx, ws, hs, L));
     Batch b;
     TRY(plan_batch(ctx, ctx->tuning, (int)hs.size(), hs.data(), sp, L, alpha_dtype, precision, ws,
@@ -4783,6 +4825,62 @@ int finish_fit_stats(korg_ctx* ctx) {
     TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
                             ctx->last_n_lblocks, ctx->last_nb));
     ctx->stats.total_ms = elapsed(ctx, 0, 3);
+    return KORG_OK;
+}
+
+// Windows of the last pipeline's lines, 1-based within each line's star (korg_get_windows).
+int fetch_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {
+    std::vector<LineMeta> m(n_lines);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const Workspace& ws = ctx->last_fixed ? ctx->ws_fixed : ctx->ws;
+    HIP_TRY(ctx, hipMemcpy(m.data(), ws.meta.p, sizeof(LineMeta) * n_lines, hipMemcpyDeviceToHost));
+    // grid indices are global over the call's stars: make them 1-based within each line's star
+    const auto& lo = ctx->last_line_off;
+    const auto& go = ctx->last_grid_off;
+    size_t k = 0;
+    for (int64_t i = 0; i < n_lines; ++i) {
+        while (k + 1 < lo.size() && lo[k + 1] <= i) ++k;
+        const int64_t off = k < go.size() ? go[k] : 0;
+        lb[i] = (int64_t)m[i].lb - off + 1;
+        ub[i] = (int64_t)m[i].ub - off + 1;
+    }
+    return KORG_OK;
+}
+
+// A batch of several layer counts: one pipeline per count (run(stars, L) enqueues it), each
+// synchronized and finished (fin()), the statistics summed and the windows kept in the caller's
+// star order.  Each star's results are those of its count's pipeline, i.e. of a batch of those
+// stars alone.
+template <class Run, class Fin>
+int run_ragged(korg_ctx* ctx, const std::vector<StarHost>& hs, Run&& run, Fin&& fin) {
+    const std::vector<std::vector<int>> groups = layer_groups(hs);
+    std::vector<int64_t> loff(hs.size() + 1, 0);
+    for (size_t k = 0; k < hs.size(); ++k) loff[k + 1] = loff[k] + hs[k].lines.n_lines;
+    std::vector<int64_t> lb((size_t)loff.back(), 1), ub((size_t)loff.back(), 0);
+    korg_stats tot{};
+    for (const std::vector<int>& g : groups) {
+        std::vector<StarHost> sub = subset(hs, g);
+        TRY(run(sub, sub[0].layers.n_layers));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        TRY(fin());
+        add_stats(tot, ctx->stats);
+        const int64_t n = ctx->last_n_lines;
+        if (n > 0) {
+            std::vector<int64_t> glb((size_t)n), gub((size_t)n);
+            TRY(fetch_windows(ctx, glb.data(), gub.data(), n));
+            int64_t q = 0;
+            for (int k : g)
+                for (int64_t i = 0; i < hs[k].lines.n_lines; ++i, ++q) {
+                    lb[loff[k] + i] = glb[q];
+                    ub[loff[k] + i] = gub[q];
+                }
+        }
+    }
+    ctx->stats = tot;
+    ctx->last_n_lines = loff.back();
+    ctx->ragged_lb.swap(lb);
+    ctx->ragged_ub.swap(ub);
+    ctx->last_ragged = true;
     return KORG_OK;
 }
 
@@ -5297,6 +5395,13 @@ int korg_line_absorption_batched(korg_ctx* ctx, int64_t n_stars, const korg_star
     int L = 0;
     TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (layer_groups(hs).size() > 1)
+        return run_ragged(
+            ctx, hs,
+            [&](std::vector<StarHost>& sub, int Lg) {
+                return run_batch(ctx, (int)sub.size(), sub.data(), species, Lg, alpha_dtype, precision);
+            },
+            [&] { return finish_stats(ctx); });
     TRY(run_batch(ctx, (int)n_stars, hs.data(), species, L, alpha_dtype, precision));
     if (synchronize) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -5489,22 +5594,12 @@ int korg_get_windows(korg_ctx* ctx, int64_t* lb, int64_t* ub, int64_t n_lines) {
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "n_lines %lld != last call's %lld",
                        (long long)n_lines, (long long)ctx->last_n_lines);
     if (n_lines == 0) return KORG_OK;
-    std::vector<LineMeta> m(n_lines);
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    const Workspace& ws = ctx->last_fixed ? ctx->ws_fixed : ctx->ws;
-    HIP_TRY(ctx, hipMemcpy(m.data(), ws.meta.p, sizeof(LineMeta) * n_lines,
-                           hipMemcpyDeviceToHost));
-    // grid indices are global over the call's stars: make them 1-based within each line's star
-    const auto& lo = ctx->last_line_off;
-    const auto& go = ctx->last_grid_off;
-    size_t k = 0;
-    for (int64_t i = 0; i < n_lines; ++i) {
-        while (k + 1 < lo.size() && lo[k + 1] <= i) ++k;
-        const int64_t off = k < go.size() ? go[k] : 0;
-        lb[i] = (int64_t)m[i].lb - off + 1;
-        ub[i] = (int64_t)m[i].ub - off + 1;
+    if (ctx->last_ragged) {
+        std::copy(ctx->ragged_lb.begin(), ctx->ragged_lb.end(), lb);
+        std::copy(ctx->ragged_ub.begin(), ctx->ragged_ub.end(), ub);
+        return KORG_OK;
     }
-    return KORG_OK;
+    return fetch_windows(ctx, lb, ub, n_lines);
 }
 
 }  // extern "C"
@@ -5523,6 +5618,9 @@ int plan_create(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
     korg_plan* p = new korg_plan();
     p->ctx = ctx;
     int rc = collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, p->stars, &p->L);
+    if (rc == KORG_OK && layer_groups(p->stars).size() > 1)
+        rc = set_err(ctx, KORG_ERR_UNSUPPORTED,
+                     "a plan needs one layer count for all its stars (capture one plan per count)");
     if (rc == KORG_OK && targets) {
         p->fit = true;
         rc = collect_fit(ctx, n_stars, targets, p->stars);
@@ -5584,6 +5682,13 @@ int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* s
     TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
     TRY(collect_fit(ctx, n_stars, targets, hs));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (layer_groups(hs).size() > 1)
+        return run_ragged(
+            ctx, hs,
+            [&](std::vector<StarHost>& sub, int Lg) {
+                return run_fit(ctx, sub, species, Lg, alpha_dtype, precision);
+            },
+            [&] { return finish_fit_stats(ctx); });
     TRY(run_fit(ctx, hs, species, L, alpha_dtype, precision));
     if (synchronize) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -5600,7 +5705,12 @@ int korg_continuum_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* star
     int L = 0;
     TRY(collect_rt_stars(ctx, n_stars, stars, alpha_dtype, false, hs, &L));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    TRY(run_continuum(ctx, hs, L, alpha_dtype));
+    const std::vector<std::vector<int>> groups = layer_groups(hs);
+    for (size_t i = 0; i < groups.size(); ++i) {  // one pass per layer count
+        if (i > 0) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the star table is reused
+        std::vector<StarHost> sub = subset(hs, groups[i]);
+        TRY(run_continuum(ctx, sub, sub[0].layers.n_layers, alpha_dtype));
+    }
     if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;
 }
@@ -5624,7 +5734,12 @@ int korg_flux_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
         hs[k].flux_c = t.cntm;
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    TRY(run_flux(ctx, hs, L, alpha_dtype));
+    const std::vector<std::vector<int>> groups = layer_groups(hs);
+    for (size_t i = 0; i < groups.size(); ++i) {
+        if (i > 0) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<StarHost> sub = subset(hs, groups[i]);
+        TRY(run_flux(ctx, sub, sub[0].layers.n_layers, alpha_dtype));
+    }
     if (synchronize) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return KORG_OK;
 }
@@ -6088,6 +6203,9 @@ int korg_autotune_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* stars
     std::vector<StarHost> hs;
     int L = 0;
     TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
+    if (layer_groups(hs).size() > 1)
+        return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                       "autotuning needs one layer count for all the stars (tune per count)");
     int64_t N = 0, nwl = 0;
     for (const StarHost& x : hs) {
         N += x.lines.n_lines;

@@ -1,7 +1,8 @@
 """Batched line opacity (korg_line_absorption_batched): one pipeline over many stars with uneven
 line counts and wavelength ranges.  Each star's alpha must equal, bit for bit, what the single-star
 entry point adds for that star alone (a star's lines keep their order in every tile, and tiles
-never mix stars).  Needs a GPU.
+never mix stars).  Batches of several layer counts (one pipeline per count) are held to the same
+bar, with their windows, statistics, continuum and flux.  Needs a GPU.
 """
 import numpy as np
 import pytest
@@ -102,15 +103,70 @@ def test_batched_fp32_random_atmospheres_match_oracle(ctx):
         assert rel_err(got, ref) <= FP32_RTOL, (k, rel_err(got, ref))
 
 
-def test_batched_rejects_mismatched_layers(ctx):
-    stars = workloads.batched_stars(n_stars=2, seed=12, scale=0.02)
-    P1 = stars[1][0]
-    short = Problem(P1.all_wls, P1.lines, P1.species_mass, P1.species_is_mol,
-                    P1.n_div_U[:, :10], P1.temps[:10], P1.n_e[:10], P1.n_HI[:10],
-                    P1.n_HeI[:10], P1.n_H2[:10], P1.xi[:10], P1.cntm_mode,
-                    P1.cntm_values[:10], P1.cntm_knots, P1.cutoff_threshold)
-    d0, d1 = DeviceProblem(stars[0][0]), DeviceProblem(short)
+def _layers(P, n):
+    """P on its first n layers (Korg's interpolate_marcs drops layers: counts differ by star)."""
+    c = np.ascontiguousarray
+    return Problem(P.all_wls, P.lines, P.species_mass, P.species_is_mol, c(P.n_div_U[:, :n]),
+                   c(P.temps[:n]), c(P.n_e[:n]), c(P.n_HI[:n]), c(P.n_HeI[:n]), c(P.n_H2[:n]),
+                   c(P.xi[:n]), P.cntm_mode, c(P.cntm_values[:n]), P.cntm_knots,
+                   P.cutoff_threshold)
+
+
+@pytest.mark.parametrize("precision,dtype", [("fp32", "float32"), ("fp64", "float64")])
+def test_batched_ragged_layer_counts(ctx, precision, dtype):
+    """Stars of 56, 40, 56, 21 and 40 layers in one call: each star's alpha is what the single-star
+    entry point adds for it (bitwise), the windows come back in the call's star order, and the
+    statistics cover every star."""
+    stars = workloads.batched_stars(n_stars=5, seed=12, scale=0.03)
+    counts = [56, 40, 56, 21, 40]
+    probs = [_layers(P, n) for (P, _, _), n in zip(stars, counts)]
+    dprobs = [DeviceProblem(P) for P in probs]
+    ref, ref_lb, ref_ub, n_evals = [], [], [], 0.0
+    for d in dprobs:
+        a = d.new_alpha(dtype)
+        ctx.line_absorption_device(a, d, precision)
+        ref.append(a.numpy())
+        lb, ub = ctx.windows(len(d.problem.lines))
+        ref_lb.append(lb)
+        ref_ub.append(ub)
+        n_evals += ctx.stats()["n_evals"]
+    alphas = [d.new_alpha(dtype) for d in dprobs]
+    ctx.line_absorption_batched(alphas, dprobs, precision)
+    for k, (a, r) in enumerate(zip(alphas, ref)):
+        assert a.shape == (probs[k].n_wl, counts[k])
+        assert np.any(r) and np.array_equal(a.numpy(), r), k
+    lb, ub = ctx.windows(sum(len(P.lines) for P in probs))
+    assert np.array_equal(lb, np.concatenate(ref_lb)) and np.array_equal(ub, np.concatenate(ref_ub))
+    assert ctx.stats()["n_evals"] == n_evals
+
+
+def test_ragged_continuum_flux_and_plans(ctx):
+    """Continuum and flux of a ragged batch equal the per-star calls; a plan refuses one."""
+    from korg_line import FluxTarget
     from korg_line._lib import KorgError
-    with pytest.raises(KorgError, match="layers"):
-        ctx.line_absorption_batched([d0.new_alpha("float32"), d1.new_alpha("float32")],
-                                    [d0, d1], "fp32")
+    stars = workloads.batched_stars(n_stars=3, seed=13, scale=0.02)
+    counts = [56, 30, 56]
+    probs = [_layers(P, n) for (P, _, _), n in zip(stars, counts)]
+    atms = [a for _, a, _ in stars]
+    dprobs = [DeviceProblem(P) for P in probs]
+
+    # per-layer references cut to each star's layers
+    refs = [workloads.fit_reference(a) for a in atms]
+    out = []
+    for d, (tr, ar), n in zip(dprobs, refs, counts):
+        a = d.new_alpha("float64")
+        ctx.continuum_batched([a], [d])
+        ctx.line_absorption_batched([a], [d], "fp64")
+        t = FluxTarget(d.problem, tr[:n], ar[:n])
+        ctx.flux_batched([a], [d], [t])
+        out.append((a.numpy(), t.flux()))
+    alphas = [d.new_alpha("float64") for d in dprobs]
+    ctx.continuum_batched(alphas, dprobs)
+    ctx.line_absorption_batched(alphas, dprobs, "fp64")
+    tg = [FluxTarget(d.problem, tr[:n], ar[:n]) for d, (tr, ar), n in zip(dprobs, refs, counts)]
+    ctx.flux_batched(alphas, dprobs, tg)
+    for k in range(3):
+        assert np.array_equal(alphas[k].numpy(), out[k][0]), k
+        assert np.array_equal(tg[k].flux(), out[k][1]), k
+    with pytest.raises(KorgError, match="layer count"):
+        ctx.plan_batched([d.new_alpha("float32") for d in dprobs], dprobs, "fp32")

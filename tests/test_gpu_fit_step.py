@@ -111,6 +111,31 @@ def test_fit_step_batched_equals_single_stars(ctx):
             assert np.array_equal(t.grads()[q], g[q]), (k, q)
 
 
+def test_fit_step_ragged_layer_counts(ctx):
+    """Stars of 56, 40 and 56 layers in one fit step: chi^2, flux and gradients of each equal the
+    single-star step's (bitwise)."""
+    from test_gpu_batched import _layers
+    stars = [make_star(seed=s, Teff=T) for s, T in ((5, 5200.0), (6, 6100.0), (7, 4600.0))]
+    cut = []
+    for (P, atm, data, sigma), n in zip(stars, (56, 40, 56)):
+        tau_ref, alpha_ref = _reference(atm)
+        cut.append((_layers(P, n), tau_ref[:n], alpha_ref[:n], data, sigma))
+    single = []
+    for Q, tr, ar, data, sigma in cut:
+        D = DeviceProblem(Q)
+        t = FitTarget(Q, tr, ar, data, sigma)
+        ctx.fit_gradient_batched([D.new_alpha("float32")], [D], [t], "fp32")
+        single.append((t.chi2(), t.flux(), t.grads()))
+    Ds = [DeviceProblem(c[0]) for c in cut]
+    Ts = [FitTarget(*c) for c in cut]
+    ctx.fit_gradient_batched([d.new_alpha("float32") for d in Ds], Ds, Ts, "fp32")
+    for k, (t, (c, f, g)) in enumerate(zip(Ts, single)):
+        assert t.chi2() == c, k
+        assert np.array_equal(t.flux(), f), k
+        for q in NAMES:
+            assert np.array_equal(t.grads()[q], g[q]), (k, q)
+
+
 def test_fit_plan_equals_direct_and_reads_new_inputs(ctx):
     stars = [make_star(seed=s, Teff=T) for s, T in ((8, 5500.0), (9, 6300.0))]
     Ds = [DeviceProblem(P) for P, _, _, _ in stars]
