@@ -32,6 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "korg.jl_amd"))
 METRIC = "spectra/s + Voigt evals/s, FP32 vs FP64 GPU at ≤1e-4 flux err, vs CPU"
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X published dense bfloat16 matrix peak
 PEAK_FP32_TFLOPS = 157.3        # MI355X published FP32 vector peak
+PEAK_FP64_MFMA_TFLOPS = 78.6    # MI355X published dense FP64 matrix peak
 DTYPE = "fp32+bf16x3"           # the arithmetic the Float32 form computes in
 PRECISION = ("fp32 exact Hunger evaluation in each line's near zone + far wing as a 4-term "
              "series on the bfloat16 matrix cores (hi/lo splits of both operands, three products, "
@@ -46,6 +47,7 @@ PRECISION = ("fp32 exact Hunger evaluation in each line's near zone + far wing a
 WING_FLOPS_PER_EVAL = 4.0
 WING_FLOPS_PER_TAIL_EVAL = 4.0
 MFMA_FLOPS = 32768.0
+F64_MFMA_FLOPS = 2048.0  # v_mfma_f64_16x16x4_f64
 # The reference's line_profile + voigt_hjerting per evaluation (src/line_absorption.jl:245-310,
 # division/exp/sqrt counted as one op): A 19, B 27, C 57, D 28.
 REF_FLOPS = (19.0, 27.0, 57.0, 28.0)
@@ -146,6 +148,14 @@ def main():
     st64 = direct(a64, "fp64", 1, 1)
     t64 = replay(a64, "fp64", args.fp64_steps, 1)
     log(f"  {t64 * 1e3:.3f} ms/step")
+    # the Float32 form's algorithm in Float64 (series far wing on the Float64 matrix cores, exact
+    # near zones): the like-for-like counterpart of the Float32 form
+    log("timing the Float64 series form ...")
+    ctx.set_tuning()
+    as64 = D.new_alpha("float64")
+    sts = direct(as64, "fp64_series", 1, 1)
+    ts64 = replay(as64, "fp64_series", max(args.fp64_steps, 3), 1)
+    log(f"  {ts64 * 1e3:.3f} ms/step")
     use(tune32)
 
     # branch mix (untimed) and accuracy of one fresh pass of each form
@@ -167,7 +177,14 @@ def main():
     lines32 = z32.numpy().astype(np.float64).T
     m = np.abs(lines64) >= 1e-8 * np.abs(lines64).max()
     alpha_err = float(np.max(np.abs(lines32[m] - lines64[m]) / np.abs(lines64[m])))
-    del lines64, lines32
+    zs = D.new_alpha("float64")
+    ctx.set_tuning()
+    ctx.line_absorption_device(zs, D, "fp64_series")
+    use(tune32)
+    liness = zs.numpy().T
+    alpha_err_series = float(np.max(np.abs(liness[m] - lines64[m]) / np.abs(lines64[m])))
+    alpha_err_32_series = float(np.max(np.abs(lines32[m] - liness[m]) / np.abs(liness[m])))
+    del lines64, lines32, liness, zs
     flux_err = None
     if not args.no_flux:
         # the whole spectrum on the device: continuum on the grid, lines added, Korg's planar
@@ -211,9 +228,9 @@ def main():
             "unit": "TFLOP/s", "frac": near_achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic.get("k_near"), "kernel_ms": near_ms, "flops_per_launch": near_flops,
             "evals_by_branch": {"A": near_A, "B": nB, "C": nC, "D": near_D},
-            "note": "exact Hunger evaluations of the near zones, the reference's per-branch op "
-                    "counts (A 19, B 27, C 57, D 28) over the FP32 vector peak; lanes are layers, "
-                    "whose branches differ"},
+            "note": "exact Hunger evaluations of the near zones (k_near2: lanes over 8 layers x 8 "
+                    "points), the reference's per-branch op counts (A 19, B 27, C 57, D 28) over "
+                    "the FP32 vector peak (which counts packed FP32; this build issues none)"},
         "k_wing_bf16": {
             "bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_MFMA_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_MFMA_TFLOPS,
@@ -224,6 +241,14 @@ def main():
                     "the bfloat16 matrix cores with 3 split products per term; bound by the "
                     "vector work that builds the operands"},
     }
+    s_mfma = sts[-1]["n_wing_mfma"]
+    s_wing_ms = sts[-1]["profile_ms"]
+    s_ach = s_mfma * F64_MFMA_FLOPS / (s_wing_ms * 1e-3) / 1e12
+    rooflines["k_wing_f64"] = {
+        "bound": "mfma", "achieved": s_ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+        "frac": s_ach / PEAK_FP64_MFMA_TFLOPS, "traffic": None, "kernel_ms": s_wing_ms,
+        "note": "the Float64 series form's far wing: v_mfma_f64_16x16x4_f64 (2048 flops each, 8 "
+                "per line and 32-point tile) over the FP64 matrix peak"}
     dominant = "k_near" if near_ms >= wing_ms else "k_wing_bf16"
     log(f"k_near {near_ms:.3f} ms ({near_achieved:.1f} TFLOP/s), k_wing_bf16 {wing_ms:.3f} ms "
         f"({achieved:.1f} TFLOP/s algorithmic)")
@@ -263,6 +288,12 @@ def main():
         "fp64_spectra_per_s": 1.0 / t64,
         "fp64_ms_per_step": t64 * 1e3,
         "fp32_over_fp64": t64 / t32,
+        "fp64_series_ms_per_step": ts64 * 1e3,
+        "fp32_over_fp64_series": ts64 / t32,
+        "alpha_max_rel_err_fp64_series_vs_fp64": alpha_err_series,
+        "alpha_max_rel_err_fp32_vs_fp64_series": alpha_err_32_series,
+        "fp64_series_kernels_ms": {"setup": sts[-1]["setup_ms"], "wing_f64_mfma": sts[-1]["profile_ms"],
+                                   "near_f64": sts[-1]["core_ms"]},
         "alpha_max_rel_err_fp32_vs_fp64": alpha_err,
         "flux_max_abs_err_fp32_vs_fp64": flux_err,
         "flux_scheme": "normalized flux of continuum + lines by the device's Korg planar transfer "
@@ -271,7 +302,7 @@ def main():
         "evals_near_zone_exact": e_near,
         "evals_far_with_tail_terms": e_tail,
         "kernel_ms": {"setup": setup_ms, "binning": bin_ms, "k_wing_bf16": wing_ms,
-                      "k_near": near_ms},
+                      "k_near2": near_ms},
         "reference_equivalent_tflops": ref_flops / t32 / 1e12,
         "roofline": {"kernel": dominant, **{k: v for k, v in rooflines[dominant].items()
                                             if k in ("bound", "achieved", "peak", "unit", "frac",

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 14
+#define KORG_LINE_ABI_VERSION 15
 
 enum korg_status {
     KORG_OK = 0,
@@ -52,6 +52,13 @@ enum korg_precision {
     /* Float32 profile evaluation and accumulation with Float64 fix-ups: per-(line, layer)
      * setup, window, branch selection and the wavelength offset lambda - lambda0 stay Float64. */
     KORG_PRECISION_FP32_FIXUP = 1,
+    /* No counterpart in the reference: the Float32 form's algorithm in Float64, for a
+     * like-for-like comparison of the two precisions.  The same windows, near zones and
+     * per-tile line lists; the far zone as the four-term series on the Float64 matrix cores
+     * (v_mfma_f64_16x16x4_f64), the near zone with the reference's own Float64 expressions.  Its
+     * difference from KORG_PRECISION_FP64 is the series' truncation.  Forward entry points only
+     * (the adjoint and the fit step take FP64 or FP32_FIXUP). */
+    KORG_PRECISION_FP64_SERIES = 2,
 };
 
 enum korg_dtype { KORG_DTYPE_F64 = 0, KORG_DTYPE_F32 = 1 };

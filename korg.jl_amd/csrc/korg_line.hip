@@ -43,6 +43,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "korg_line.h"
@@ -316,6 +317,7 @@ struct SetupArgs {
     LineMeta* meta;
     double4* rec64;  // [line * Ls + l]: (1/(sigma sqrt 2), a, scaling, 0)
     float4* recM;    // [(line * n_lblocks + b) * 64 + lane]: far-wing coefficients, MFMA A layout
+    double4* recM64;  // the Float64 series form: the same coefficients in Float64 (recM unused)
     unsigned long long* counters;
     int2* azone;     // Float32 form, when given: each line's adjoint near zone (global indices)
 };
@@ -479,7 +481,9 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
                 for (int k = 0; k < 4; ++k) f32_bad |= !(fabs(c[k]) <= F32_COEF_MAX);  // or NaN
             }
         }
-        if (F32 && WING_BF16) {
+        if (F32 && A.recM64) {
+            A.recM64[(i * A.n_lblocks + l / WAVE) * WAVE + l % WAVE] = make_double4(c[0], c[1], c[2], c[3]);
+        } else if (F32 && WING_BF16) {
             // bfloat16 form: per (line, layer) the four coefficients' high parts, then their low
             // parts (c - high, rounded again): 16 B at [(line * n_lblocks + block) * 64 + l % 64]
             uint32_t hw[2], lw[2];
@@ -512,7 +516,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     // Float32 range: c_k R^k stays small where the series is used, but neither factor is
     // representable (and inf * 0 is NaN).  Such a line skips the series: its coefficients are
     // zero and the near-zone kernel evaluates its whole window exactly.
-    const bool series_off = F32 && ballot(f32_bad) != 0;
+    const bool series_off = F32 && !A.recM64 && ballot(f32_bad) != 0;
     if (series_off) {
         // (either layout: every coefficient slot of the line's blocks is zeroed)
         for (int l = lane; l < Lm; l += WAVE)
@@ -965,6 +969,7 @@ struct ProfileArgs {
     int n_stars;
     const double4* rec64;
     const float4* recM;
+    const double4* recM64;  // the Float64 series form's coefficients (recM's layout, Float64)
     const Cand* cands;   // lines in window-class partition order
     const int2* ranges;
     const LineMeta* meta;
@@ -1688,6 +1693,110 @@ void k_wing_bf16(ProfileArgs A) {
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// The Float64 series form's far wing: k_wing_bf16's work items (tile of 32 points, block of 64
+// layers) and per-tile line lists, one wave per item, with v_mfma_f64_16x16x4_f64 taking one line
+// per instruction, K = its 4 powers: A = the line's Float64 coefficients of 16 layers, B = (R,
+// R^2, R^3, R^4) at 16 points (R = 1 / (lambda - lambda0)^2 in DELTA_UNIT, divided in Float64;
+// zero outside the window and inside the near zone).  Lane i holds A(layer i % 16, power i / 16)
+// and B(power i / 16, point i % 16); register r of accumulator (row block, column block) holds
+// layer i / 16 + 4 r, point i % 16 of the block (the Float64 instruction interleaves its rows over
+// the registers; the Float32 16x16x4 one keeps four consecutive rows per lane).  8 instructions
+// per line: 4 row blocks x 2 column blocks.
+// ---------------------------------------------------------------------------------------------
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <class AlphaT>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_wing_f64(ProfileArgs A) {
+    const int lane = threadIdx.x;
+    const int col = lane & 15, kq = lane >> 4;
+    const int L = A.L;
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written
+    for (;;) {
+        const int item = wave_take(&A.counters[0], lane);
+        if (item >= A.n_items) break;
+        const int tile = item / A.n_lblocks;
+        const int lblk = item % A.n_lblocks;
+        const int64_t j0 = (int64_t)tile * A.tile_w;
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[star_of_point(stars, A.n_stars, j0)];
+        const int64_t go = S.grid_off;
+        const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;  // below j0: padding, no lines
+        int jj[2];
+        double lam[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            jj[cb] = (int)(j0 + 16 * cb + col);
+            lam[cb] = S.wls[min((int64_t)jj[cb], j1) - go] * (1.0 / DELTA_UNIT);  // in bounds
+        }
+        doublex4 acc[4][2];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = doublex4{0.0, 0.0, 0.0, 0.0};
+        // coefficient kq of layer 16 rb + col of the item's block, for line `line`
+        const double* __restrict__ recA =
+            (const double*)A.recM64 + ((int64_t)lblk * WAVE + col) * 4 + kq;
+        const int64_t rstride = (int64_t)A.n_lblocks * WAVE * 4;
+        const int64_t t0 = A.tile_start[(int64_t)tile * N_CLASSES];
+        const int64_t t1 = A.tile_start[(int64_t)(tile + 1) * N_CLASSES];
+        for (int64_t kb = t0; kb < t1; kb += WAVE) {
+            const int nb = (int)min((int64_t)WAVE, t1 - kb);
+            const int idx = A.tile_lines[min(kb + lane, t1 - 1)];  // clamped: in bounds
+            const LineMeta mm = A.meta[idx];
+            // the next line's coefficients are loaded while this one's products run
+            double an[4];
+            {
+                const int line0 = readlane_i32(idx, 0);
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) an[rb] = recA[(int64_t)line0 * rstride + rb * 64];
+            }
+            for (int b = 0; b < nb; ++b) {
+                const double lam0s = readlane_f64(mm.lam0, b) * (1.0 / DELTA_UNIT);
+                const int lb = readlane_i32(mm.lb, b), ub = readlane_i32(mm.ub, b);
+                const int clo = readlane_i32(mm.clo, b), chi = readlane_i32(mm.chi, b);
+                double a[4];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+                const int next = readlane_i32(idx, min(b + 1, nb - 1));  // past the last: itself
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) an[rb] = recA[(int64_t)next * rstride + rb * 64];
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    const int j = jj[cb];
+                    const bool far = j <= j1 && j >= lb && j <= ub && (j < clo || j > chi);
+                    const double d = lam[cb] - lam0s;
+                    // 1 / d^2: the hardware reciprocal refined by two Newton steps (to an ulp or
+                    // two; the series' truncation is 1e-5)
+                    const double d2 = d * d;
+                    double y = __builtin_amdgcn_rcp(d2);
+                    y = fma(y, fma(-d2, y, 1.0), y);
+                    y = fma(y, fma(-d2, y, 1.0), y);
+                    const double R = far ? y : 0.0;
+                    const double R2 = R * R;
+                    const double bv = kq == 0 ? R : (kq == 1 ? R2 : (kq == 2 ? R2 * R : R2 * R2));
+#pragma unroll
+                    for (int rb = 0; rb < 4; ++rb)
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bv, acc[rb][cb], 0, 0, 0);
+                }
+            }
+        }
+        AlphaT* __restrict__ alpha = (AlphaT*)S.alpha;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            if (jj[cb] > j1) continue;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int layer = lblk * WAVE + 16 * rb + kq + 4 * r;
+                    if (layer < L) alpha[(int64_t)layer + (int64_t)L * (jj[cb] - go)] += (AlphaT)acc[rb][cb][r];
+                }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Float32 near zone, lanes = layers.  A work item is (tile of CT points, block of 64 layers);
 // for every line whose near zone [clo, chi] meets the tile (in line order), each lane evaluates
@@ -1798,10 +1907,13 @@ constexpr int NEAR_PGROUPS = CT / 8;  // point groups per tile
 // NG: layer groups per 64-layer block (7 when no block has more than 56 layers).  The line's
 // records go through LDS as (1/(sigma sqrt 2), a) and the scaling; one line's record load is in
 // flight while the previous line is evaluated (register-bound: four waves per SIMD).
-template <int NG>
+// F64 (the Float64 series form): the evaluation is the reference's line_profile in Float64, the
+// expression of k_profile_f64, with Float64 sums.
+template <int NG, bool F64>
 __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
+    using AccT = typename std::conditional<F64, double, float>::type;
     __shared__ double2 s_q[WAVE];
-    __shared__ float s_sc[WAVE];
+    __shared__ AccT s_sc[WAVE];
     const int lane = threadIdx.x;
     const int r = lane & 7, c = lane >> 3;
     const int L = A.L;
@@ -1824,11 +1936,11 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
         const double* wt = S.wls + (j0 - go);
 #pragma unroll
         for (int p = 0; p < NEAR_PGROUPS; ++p) lam[p] = wt[min((int64_t)(8 * p + c), j1 - j0)];
-        float acc[NG][NEAR_PGROUPS];
+        AccT acc[NG][NEAR_PGROUPS];
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
-            for (int p = 0; p < NEAR_PGROUPS; ++p) acc[g][p] = 0.0f;
+            for (int p = 0; p < NEAR_PGROUPS; ++p) acc[g][p] = (AccT)0;
         const double4* __restrict__ rq = A.rec64 + (own_v ? l_own : 0);
         auto load_q = [&](int line) {
             return own_v ? rq[(int64_t)line * A.Ls] : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1839,13 +1951,13 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
             // LDS operations of one wave complete in order: the reads below see these writes,
             // and the previous line's reads were issued before them
             s_q[lane] = make_double2(q.x, q.y);
-            s_sc[lane] = (float)q.z;
+            s_sc[lane] = (AccT)q.z;
             const int tlo = v.clo - j0s, thi = v.chi - j0s;
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
                 if (8 * g >= nl) break;  // wavefront-uniform
                 const double2 qq = s_q[8 * g + r];
-                const float sc = s_sc[8 * g + r];
+                const AccT sc = s_sc[8 * g + r];
                 const bool gv = 8 * g + r < nl;
                 const float a32 = (float)qq.y;
 #pragma unroll
@@ -1854,6 +1966,10 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
                     const int t = 8 * p + c;
                     if (!gv || t < tlo || t > thi) continue;
                     const double v64 = fabs(lam[p] - v.lam0) * qq.x;
+                    if constexpr (F64) {
+                        acc[g][p] += voigt_hjerting64(qq.y, v64) * sc;
+                        continue;
+                    }
                     const int br = hunger_branch(qq.y, v64);
                     float h;
                     if (br == 1 || br == 2) {
@@ -3724,7 +3840,8 @@ int validate(korg_ctx* ctx, const korg_grid* g, const korg_lines* ln, const korg
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no species");
     if (alpha_dtype != KORG_DTYPE_F64 && alpha_dtype != KORG_DTYPE_F32)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "bad alpha dtype %d", alpha_dtype);
-    if (precision != KORG_PRECISION_FP64 && precision != KORG_PRECISION_FP32_FIXUP)
+    if (precision != KORG_PRECISION_FP64 && precision != KORG_PRECISION_FP32_FIXUP &&
+        precision != KORG_PRECISION_FP64_SERIES)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "bad precision %d", precision);
     if (cn->mode != KORG_CNTM_AT_LINES && cn->mode != KORG_CNTM_GRID)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "bad continuum mode %d", cn->mode);
@@ -3887,6 +4004,7 @@ struct Batch {
     int n_stars = 0, L = 0;
     int64_t N = 0, nwl = 0;
     bool f32 = false, alpha_f32 = false;
+    bool s64 = false;  // the Float64 series form (f32 is set too: the same zones and lists)
     bool adj = false;  // Float32 fit step: the adjoint's series form (near zones, moments)
     int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, near_layout = 2, count_branches = 0;
     int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
@@ -3939,9 +4057,10 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.n_stars = n_stars;
     b.L = L;
     b.sp = *sp;
-    b.f32 = precision == KORG_PRECISION_FP32_FIXUP;
+    b.s64 = precision == KORG_PRECISION_FP64_SERIES;
+    b.f32 = precision == KORG_PRECISION_FP32_FIXUP || b.s64;
     b.alpha_f32 = alpha_dtype == KORG_DTYPE_F32;
-    b.NB = t.tile_points ? t.tile_points / 32 : 2;
+    b.NB = b.s64 ? 1 : (t.tile_points ? t.tile_points / 32 : 2);  // the Float64 wing: 32 points
     b.WW = t.waves_per_tile ? t.waves_per_tile : 4;
     b.P64 = t.points_per_lane == 2 ? 2 : 1;
     b.wpc = t.waves_per_cu > 0 ? t.waves_per_cu : 12;
@@ -3964,7 +4083,8 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
     TRY(grow(ctx, ws.meta, sizeof(LineMeta) * N));
     TRY(grow(ctx, ws.rec64, sizeof(double4) * N * b.Ls));
-    if (b.f32) TRY(grow(ctx, ws.recM, sizeof(float4) * (size_t)N * b.n_lblocks * WAVE));
+    if (b.f32)
+        TRY(grow(ctx, ws.recM, (b.s64 ? sizeof(double4) : sizeof(float4)) * (size_t)N * b.n_lblocks * WAVE));
     TRY(grow(ctx, ws.tab, sizeof(LayerTab) * (size_t)n_stars * L));
     TRY(grow(ctx, ws.counters, sizeof(unsigned long long) * N_COUNTERS));
     TRY(grow(ctx, ws.blk, sizeof(int) * (size_t)b.nblocks * N_CLASSES));
@@ -4011,6 +4131,7 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.meta = (LineMeta*)ws.meta.p;
     sa.rec64 = (double4*)ws.rec64.p;
     sa.recM = (float4*)ws.recM.p;
+    sa.recM64 = b.s64 ? (double4*)ws.recM.p : nullptr;
     sa.counters = counters;
     sa.azone = b.adj ? (int2*)ws.azone.p : nullptr;
     const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);
@@ -4108,6 +4229,7 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
     pa.n_stars = b.n_stars;
     pa.rec64 = (const double4*)ws.rec64.p;
     pa.recM = (const float4*)ws.recM.p;
+    pa.recM64 = b.s64 ? (const double4*)ws.recM.p : nullptr;
     pa.cands = (const Cand*)ws.cls_cand.p;
     pa.meta = (const LineMeta*)ws.meta.p;
     pa.tile_start = (const int64_t*)ws.tc_start.p;
@@ -4127,7 +4249,12 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         int64_t nwg = (int64_t)ctx->n_cu * ((b.wpc + b.WW - 1) / b.WW);
         if (nwg > pa.n_items) nwg = pa.n_items;
         const dim3 wgrid((unsigned)nwg);
-        switch (b.NB * 8 + b.WW) {
+        if (b.s64) {
+            int64_t nw = (int64_t)ctx->n_cu * 16;  // four waves per SIMD
+            if (nw > pa.n_items) nw = pa.n_items;
+            if (pa.alpha_f32) hipLaunchKernelGGL(k_wing_f64<float>, dim3((unsigned)nw), dim3(WAVE), 0, s, pa);
+            else hipLaunchKernelGGL(k_wing_f64<double>, dim3((unsigned)nw), dim3(WAVE), 0, s, pa);
+        } else switch (b.NB * 8 + b.WW) {
             case 1 * 8 + 1: launch_wing<1, 1>(pa, wgrid, s); break;
             case 1 * 8 + 2: launch_wing<1, 2>(pa, wgrid, s); break;
             case 1 * 8 + 4: launch_wing<1, 4>(pa, wgrid, s); break;
@@ -4159,10 +4286,15 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         na.counters = counters;
         int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
         if (cwaves > na.n_items) cwaves = na.n_items;
-        if (b.near_layout == 1) hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        if (b.near_layout == 1 && !b.s64)
+            hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else if (b.s64 && std::min(b.L, WAVE) <= 56)
+            hipLaunchKernelGGL((k_near2<7, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else if (b.s64)
+            hipLaunchKernelGGL((k_near2<8, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
         else if (std::min(b.L, WAVE) <= 56)
-            hipLaunchKernelGGL(k_near2<7>, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else hipLaunchKernelGGL(k_near2<8>, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+            hipLaunchKernelGGL((k_near2<7, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        else hipLaunchKernelGGL((k_near2<8, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
@@ -4281,7 +4413,8 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
 }
 
 // Counters of a finished forward call into ctx->stats; an error for a raised flag.
-int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks, int nb) {
+int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks, int nb,
+                        bool s64 = false) {
     if (c[3] & FLAG_SPECIES)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
                        "a line's species index is outside [0, n_species); alpha was left unchanged");
@@ -4297,9 +4430,11 @@ int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, in
     ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
     // far-wing MFMAs: bfloat16, three per (group of 4 lines, 32-point block, 32-layer half),
     // counted per line-tile entry of the lists (groups rounded up are not counted); Float32, two
-    // per 32-point block and K step
-    ctx->stats.n_wing_mfma = !f32 ? 0.0 : WING_BF16 ? (double)c[13] * n_lblocks * 1.5 * nb
-                                                     : (double)c[11] * n_lblocks;
+    // per 32-point block and K step; the Float64 series form, 8 per (line, 32-point tile, 64-layer
+    // block)
+    ctx->stats.n_wing_mfma = !f32 ? 0.0 : s64 ? (double)c[13] * n_lblocks * 8.0
+                                     : WING_BF16 ? (double)c[13] * n_lblocks * 1.5 * nb
+                                                 : (double)c[11] * n_lblocks;
     ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
     ctx->stats.n_evals_near_A = f32 ? (double)c[15] : 0.0;
     return KORG_OK;
@@ -4320,8 +4455,9 @@ int finish_stats(korg_ctx* ctx) {
     unsigned long long c[N_COUNTERS];
     const Workspace& ws = ctx->last_fixed ? ctx->ws_fixed : ctx->ws;
     HIP_TRY(ctx, hipMemcpy(c, ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
-    const int rc = stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
-                                       ctx->last_n_lblocks, ctx->last_nb);
+    const int rc = stats_from_counters(ctx, c, ctx->last_precision != KORG_PRECISION_FP64,
+                                       ctx->last_n_lblocks, ctx->last_nb,
+                                       ctx->last_precision == KORG_PRECISION_FP64_SERIES);
     if (rc == KORG_ERR_CAPACITY && ctx->last_fixed)
         return set_err(ctx, KORG_ERR_CAPACITY,
                        "the per-tile line lists need %llu entries, more than the %lld the fixed "
@@ -4763,8 +4899,11 @@ int check_jvp_args(korg_ctx* ctx, void* const* dalpha, int64_t K, const korg_lay
 
 // Validates the fit targets of a batch and puts them (and the gradient outputs) into the stars.
 int collect_fit(korg_ctx* ctx, int64_t n_stars, const korg_fit_target* t,
-                std::vector<StarHost>& hs) {
+                std::vector<StarHost>& hs, int precision) {
     if (!t) return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "no fit targets");
+    if (precision == KORG_PRECISION_FP64_SERIES)
+        return set_err(ctx, KORG_ERR_UNSUPPORTED,
+                       "the fit step takes KORG_PRECISION_FP64 or KORG_PRECISION_FP32_FIXUP");
     if (n_stars > 65535)
         return set_err(ctx, KORG_ERR_UNSUPPORTED, "at most 65535 stars per fit step");
     for (int64_t k = 0; k < n_stars; ++k) {
@@ -5623,7 +5762,7 @@ int plan_create(korg_ctx* ctx, int64_t n_stars, const korg_star* stars,
                      "a plan needs one layer count for all its stars (capture one plan per count)");
     if (rc == KORG_OK && targets) {
         p->fit = true;
-        rc = collect_fit(ctx, n_stars, targets, p->stars);
+        rc = collect_fit(ctx, n_stars, targets, p->stars, precision);
     }
     if (rc != KORG_OK) {
         delete p;
@@ -5680,7 +5819,7 @@ int korg_fit_gradient_batched(korg_ctx* ctx, int64_t n_stars, const korg_star* s
     std::vector<StarHost> hs;
     int L = 0;
     TRY(collect_stars(ctx, n_stars, stars, species, alpha_dtype, precision, hs, &L));
-    TRY(collect_fit(ctx, n_stars, targets, hs));
+    TRY(collect_fit(ctx, n_stars, targets, hs, precision));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (layer_groups(hs).size() > 1)
         return run_ragged(

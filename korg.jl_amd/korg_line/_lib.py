@@ -21,6 +21,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20  # KORG_NOALLOC_SCRATCH_BYTES
 
 PRECISION_FP64 = 0
 PRECISION_FP32_FIXUP = 1
+PRECISION_FP64_SERIES = 2
 DTYPE_F64 = 0
 DTYPE_F32 = 1
 CNTM_AT_LINES = 0
@@ -141,7 +142,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 14  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 15  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",

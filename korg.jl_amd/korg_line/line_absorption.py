@@ -17,7 +17,7 @@ from . import _lib
 from .problem import Problem
 
 PRECISIONS = {"fp64": _lib.PRECISION_FP64, "fp32": _lib.PRECISION_FP32_FIXUP,
-              "fp32_fixup": _lib.PRECISION_FP32_FIXUP}
+              "fp32_fixup": _lib.PRECISION_FP32_FIXUP, "fp64_series": _lib.PRECISION_FP64_SERIES}
 
 
 def _ptr(a):
