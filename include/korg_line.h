@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 15
+#define KORG_LINE_ABI_VERSION 16
 
 enum korg_status {
     KORG_OK = 0,
@@ -264,6 +264,10 @@ typedef struct {
     int32_t near_waves_per_cu; /* Float32 near zone: persistent waves per compute unit (default 16) */
     int32_t near_layout;       /* Float32 near zone: 1 = lanes over 64 layers, one point at a time;
                                   2 = lanes over 8 layers x 8 points (default); same alpha */
+    int32_t near_concurrent;   /* Float32 near zone: 1 = after the far wing on the call's stream;
+                                  2 (default) = beside it on a second stream of the context, into
+                                  a buffer of n_layers x points floats added afterwards; same
+                                  alpha (the allocation-free entry point always runs 1) */
 } korg_tuning;
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
 int korg_get_tuning(const korg_ctx* ctx, korg_tuning* out);

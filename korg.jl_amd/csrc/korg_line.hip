@@ -1817,6 +1817,7 @@ struct NearArgs {
     const double4* rec64;
     int alpha_f32;
     unsigned long long* counters;
+    float* nbuf;  // when set: the near-zone sums are stored here ([global point][L]), not added
 };
 
 __device__ inline float voigt_far_f32(float a, float v, bool branch_d) {
@@ -1989,10 +1990,29 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
 #pragma unroll
             for (int p = 0; p < NEAR_PGROUPS; ++p) {
                 const int64_t j = j0 + 8 * p + c;
-                if (j <= j1) add_alpha(S.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * (j - go), acc[g][p]);
+                if (j > j1) continue;
+                if (!F64 && A.nbuf) A.nbuf[(int64_t)l + (int64_t)L * j] = (float)acc[g][p];
+                else add_alpha(S.alpha, A.alpha_f32, (int64_t)l + (int64_t)L * (j - go), acc[g][p]);
             }
         }
     }
+}
+
+// alpha += the near-zone sums stored by k_near2 (the same addition it makes itself when it runs
+// alone, so the two schedules give identical alpha).  One thread per (point, layer) of the
+// stars' grids; padding points are skipped.
+__global__ __launch_bounds__(256) void k_add_near(int n_stars, int L, int64_t n_total,
+                                                  const StarDev* __restrict__ stars,
+                                                  const float* __restrict__ nbuf, int alpha_f32,
+                                                  const unsigned long long* __restrict__ counters) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // a raised flag: k_near2 wrote nothing (alpha is left unchanged)
+    if (e >= n_total * L || counters[3] != 0) return;
+    const int64_t j = e / L;
+    const int l = (int)(e - j * L);
+    const StarDev& S = stars[star_of_point(stars, n_stars, j)];
+    if (j >= S.grid_off + S.n_wl) return;  // padding between two stars' grids
+    add_alpha(S.alpha, alpha_f32, (int64_t)l + (int64_t)L * (j - S.grid_off), nbuf[e]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3684,6 +3704,7 @@ struct Arena {
 // every plan owns its own, so the pointers a captured graph holds stay valid whatever later calls
 // on the context allocate.
 struct Workspace {
+    DevBuf nbuf;  // the near zone's sums when it runs beside the far wing
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -3699,7 +3720,8 @@ struct Workspace {
         DevBuf* bufs[] = {&meta,     &rec64,     &recM,      &tab,        &counters,  &blk,
                           &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
-                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part};
+                          &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
+                          &nbuf};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -3712,7 +3734,7 @@ struct Workspace {
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
-                          &scan_part, &jvp_coef};
+                          &scan_part, &jvp_coef, &nbuf};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -3731,7 +3753,10 @@ struct korg_ctx {
     hipStream_t stream = nullptr;
     // [0] start, [1] setup done, [2] partition done, [3] end, [4] far wing done,
     // [5] tile-list total known, [6] tile-list fill launched (the host sizes the lists between)
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[9] = {};
+    // the Float32 form's near zone runs on `side`, beside the far wing (fork/join events)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::string err;
     korg_stats stats{};
     korg_tuning tuning{};
@@ -3752,6 +3777,7 @@ struct korg_ctx {
     // a batch of several layer counts runs as one pipeline per count: the windows of all its
     // lines, in the caller's star order (korg_get_windows reads them when last_ragged is set)
     bool last_ragged = false;
+    bool last_overlap = false;  // the last call's near zone ran beside the wing (its own events)
     std::vector<int64_t> ragged_lb, ragged_ub;
     std::vector<int64_t> last_grid_off, last_line_off;  // per star of the last call
     // adjoint outputs of the host-memory entry point, and its staged G
@@ -4005,6 +4031,7 @@ struct Batch {
     int64_t N = 0, nwl = 0;
     bool f32 = false, alpha_f32 = false;
     bool s64 = false;  // the Float64 series form (f32 is set too: the same zones and lists)
+    bool overlap = false;  // Float32 form: the near zone beside the far wing (second stream)
     bool adj = false;  // Float32 fit step: the adjoint's series form (near zones, moments)
     int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, near_layout = 2, count_branches = 0;
     int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
@@ -4021,6 +4048,7 @@ int check_tuning(korg_ctx* ctx, const korg_tuning* t) {
     if (t->points_per_lane < 0 || t->points_per_lane > 2 || t->waves_per_cu < 0 ||
         t->waves_per_cu > 64 || t->near_waves_per_cu < 0 || t->near_waves_per_cu > 64 ||
         t->count_branches < 0 || t->count_branches > 1 || t->near_layout < 0 || t->near_layout > 2 ||
+        t->near_concurrent < 0 || t->near_concurrent > 2 ||
         !(tp == 0 || tp == 32 || tp == 64 || tp == 128) || !(wt == 0 || wt == 1 || wt == 2 || wt == 4))
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "tuning out of range");
     if (!wing_variant_ok(tp ? tp / 32 : 2, wt ? wt : 4))
@@ -4066,6 +4094,8 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.wpc = t.waves_per_cu > 0 ? t.waves_per_cu : 12;
     b.near_wpc = t.near_waves_per_cu > 0 ? t.near_waves_per_cu : 16;
     b.near_layout = t.near_layout > 0 ? t.near_layout : 2;
+    // not in the fixed-scratch path, whose capacity the near buffer would take
+    b.overlap = b.f32 && !b.s64 && b.near_layout != 1 && t.near_concurrent != 1 && !ctx->arena;
     b.count_branches = t.count_branches ? 1 : 0;
     TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
     b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
@@ -4085,6 +4115,7 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     TRY(grow(ctx, ws.rec64, sizeof(double4) * N * b.Ls));
     if (b.f32)
         TRY(grow(ctx, ws.recM, (b.s64 ? sizeof(double4) : sizeof(float4)) * (size_t)N * b.n_lblocks * WAVE));
+    if (b.overlap) TRY(grow(ctx, ws.nbuf, sizeof(float) * (size_t)L * (size_t)b.nwl));
     TRY(grow(ctx, ws.tab, sizeof(LayerTab) * (size_t)n_stars * L));
     TRY(grow(ctx, ws.counters, sizeof(unsigned long long) * N_COUNTERS));
     TRY(grow(ctx, ws.blk, sizeof(int) * (size_t)b.nblocks * N_CLASSES));
@@ -4203,6 +4234,40 @@ void launch_wing(const ProfileArgs& pa, dim3 grid, hipStream_t s) {
     else hipLaunchKernelGGL((k_wing_mfma<NB, WW, double>), grid, dim3(WAVE * WW), 0, s, pa);
 }
 
+// The Float32 (or Float64 series) near-zone kernel on `s`; nbuf: store its sums there instead of
+// adding them into alpha.
+int launch_near(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, float* nbuf) {
+    NearArgs na;
+    na.n_wl = b.nwl;
+    na.n_lines = b.N;
+    na.L = b.L;
+    na.Ls = b.Ls;
+    na.n_lblocks = b.n_lblocks;
+    na.ntiles = b.nt_near;
+    na.n_items = (int64_t)na.ntiles * b.n_lblocks;
+    na.stars = (const StarDev*)ws.stars.p;
+    na.n_stars = b.n_stars;
+    na.cands = (const Cand*)ws.near_cand.p;
+    na.ranges = (const int2*)ws.near_ranges.p;
+    na.rec64 = (const double4*)ws.rec64.p;
+    na.alpha_f32 = b.alpha_f32;
+    na.counters = (unsigned long long*)ws.counters.p;
+    na.nbuf = nbuf;
+    int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
+    if (cwaves > na.n_items) cwaves = na.n_items;
+    if (b.near_layout == 1 && !b.s64)
+        hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+    else if (b.s64 && std::min(b.L, WAVE) <= 56)
+        hipLaunchKernelGGL((k_near2<7, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+    else if (b.s64)
+        hipLaunchKernelGGL((k_near2<8, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+    else if (std::min(b.L, WAVE) <= 56)
+        hipLaunchKernelGGL((k_near2<7, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+    else hipLaunchKernelGGL((k_near2<8, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+    HIP_TRY(ctx, hipGetLastError());
+    return KORG_OK;
+}
+
 // Second half on `s`: (Float32 form) the per-tile list fill, then the profile kernels that add
 // into alpha.  `timed` records ev[6], ev[2], ev[4], ev[3].
 int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed) {
@@ -4246,6 +4311,15 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         HIP_TRY(ctx, hipGetLastError());
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     } else {
+        if (b.overlap) {
+            // the near zone on the side stream, into ws.nbuf, while the far wing runs on s
+            HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, s));
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+            if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], ctx->side));
+            TRY(launch_near(ctx, ws, b, ctx->side, (float*)ws.nbuf.p));
+            if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[8], ctx->side));
+            HIP_TRY(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+        }
         int64_t nwg = (int64_t)ctx->n_cu * ((b.wpc + b.WW - 1) / b.WW);
         if (nwg > pa.n_items) nwg = pa.n_items;
         const dim3 wgrid((unsigned)nwg);
@@ -4269,32 +4343,17 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         }
         HIP_TRY(ctx, hipGetLastError());
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-        NearArgs na;
-        na.n_wl = b.nwl;
-        na.n_lines = b.N;
-        na.L = b.L;
-        na.Ls = b.Ls;
-        na.n_lblocks = b.n_lblocks;
-        na.ntiles = b.nt_near;
-        na.n_items = (int64_t)na.ntiles * b.n_lblocks;
-        na.stars = (const StarDev*)ws.stars.p;
-        na.n_stars = b.n_stars;
-        na.cands = (const Cand*)ws.near_cand.p;
-        na.ranges = (const int2*)ws.near_ranges.p;
-        na.rec64 = (const double4*)ws.rec64.p;
-        na.alpha_f32 = b.alpha_f32;
-        na.counters = counters;
-        int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
-        if (cwaves > na.n_items) cwaves = na.n_items;
-        if (b.near_layout == 1 && !b.s64)
-            hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else if (b.s64 && std::min(b.L, WAVE) <= 56)
-            hipLaunchKernelGGL((k_near2<7, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else if (b.s64)
-            hipLaunchKernelGGL((k_near2<8, true>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else if (std::min(b.L, WAVE) <= 56)
-            hipLaunchKernelGGL((k_near2<7, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
-        else hipLaunchKernelGGL((k_near2<8, false>), dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
+        if (b.overlap) {
+            // the near zone's sums (launched beside the wing, below) are added now
+            HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
+            const int64_t ne = (int64_t)b.L * b.nwl;
+            hipLaunchKernelGGL(k_add_near, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                               b.n_stars, b.L, b.nwl, (const StarDev*)ws.stars.p,
+                               (const float*)ws.nbuf.p, (int)b.alpha_f32,
+                               (const unsigned long long*)counters);
+        } else {
+            TRY(launch_near(ctx, ws, b, s, nullptr));
+        }
         HIP_TRY(ctx, hipGetLastError());
     }
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
@@ -4395,6 +4454,7 @@ int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_spec
     ctx->last_n_lblocks = b.n_lblocks;
     ctx->last_nb = b.NB;
     ctx->last_sized = b.f32 && !fixed;
+    ctx->last_overlap = b.overlap;
     if (b.N == 0) return KORG_OK;
     TRY(enqueue_front(ctx, ws, b, s, fixed ? ws.tile_cap : INT64_MAX, true));
     if (b.f32 && !fixed) {
@@ -4470,7 +4530,7 @@ int finish_stats(korg_ctx* ctx) {
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
     ctx->stats.bin_ms = elapsed(ctx, 1, 2) - gap;
     ctx->stats.profile_ms = elapsed(ctx, 2, 4);
-    ctx->stats.core_ms = elapsed(ctx, 4, 3);
+    ctx->stats.core_ms = ctx->last_overlap ? elapsed(ctx, 7, 8) : elapsed(ctx, 4, 3);
     ctx->stats.total_ms = elapsed(ctx, 0, 3) - gap;
     return KORG_OK;
 }
@@ -4808,7 +4868,7 @@ int finish_vjp_stats(korg_ctx* ctx) {
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
     ctx->stats.profile_ms = elapsed(ctx, 2, 4);
-    ctx->stats.core_ms = elapsed(ctx, 4, 3);
+    ctx->stats.core_ms = ctx->last_overlap ? elapsed(ctx, 7, 8) : elapsed(ctx, 4, 3);
     ctx->stats.total_ms = elapsed(ctx, 0, 3);
     return KORG_OK;
 }
@@ -5410,6 +5470,12 @@ int korg_create(korg_ctx** out, int device) {
             return KORG_ERR_HIP;
         }
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+        korg_destroy(ctx);
+        return KORG_ERR_HIP;
+    }
     // set-up of the allocation-free entry point: the fixed scratch's address on this device, and
     // host-side room for a one-star table, so that its calls allocate nothing
     resolve_scratch(ctx);
@@ -5441,6 +5507,9 @@ void korg_destroy(korg_ctx* ctx) {
         if (b.p) (void)hipFree(b.p);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

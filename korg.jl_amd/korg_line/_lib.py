@@ -100,7 +100,7 @@ class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32),
                 ("tile_points", ctypes.c_int32), ("near_waves_per_cu", ctypes.c_int32),
-                ("near_layout", ctypes.c_int32)]
+                ("near_layout", ctypes.c_int32), ("near_concurrent", ctypes.c_int32)]
 
 
 class KorgFitTarget(ctypes.Structure):
@@ -142,7 +142,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 15  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 16  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",

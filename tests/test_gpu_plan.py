@@ -168,25 +168,27 @@ def test_tuning_variants_match_fp64(tile_points, waves_per_tile):
 @pytest.mark.parametrize("n_layers", [56, 20, 70, 130])
 def test_near_layouts_identical(n_layers):
     """The near zone's two lane layouts (lanes over 64 layers, one point at a time; lanes over
-    8 layers x 8 points) accumulate the same terms in the same order: identical alpha, for layer
-    counts below, above and at several 64-layer blocks, on a batch of stars."""
+    8 layers x 8 points), run after the far wing or beside it on a second stream (its sums added
+    afterwards), accumulate the same terms in the same order: identical alpha, for layer counts
+    below, above and at several 64-layer blocks, on a batch of stars."""
     probs = _batch(n_stars=3, seed=28, scale=0.03)
     if n_layers != probs[0].n_layers:
         probs = [_layers(P, n_layers) for P in probs]
     with Context(0) as c:
         out = []
-        for layout in (1, 2):
-            c.set_tuning(near_layout=layout)
+        for layout, conc in ((1, 1), (2, 1), (2, 2)):
+            c.set_tuning(near_layout=layout, near_concurrent=conc)
             out.append(_eager(c, probs, "fp32", "float32"))
-        for k, (x, y) in enumerate(zip(*out)):
-            assert np.any(x) and np.array_equal(x, y), k
+        for k, (x, y, z) in enumerate(zip(*out)):
+            assert np.any(x) and np.array_equal(x, y) and np.array_equal(x, z), k
 
 
 def test_invalid_tuning_rejected():
     from korg_line._lib import KorgError
     with Context(0) as c:
         for kw in ({"tile_points": 128, "waves_per_tile": 4}, {"tile_points": 48},
-                   {"waves_per_tile": 3}, {"near_waves_per_cu": 65}, {"near_layout": 3}):
+                   {"waves_per_tile": 3}, {"near_waves_per_cu": 65}, {"near_layout": 3},
+                   {"near_concurrent": 3}):
             with pytest.raises(KorgError, match="tuning"):
                 c.set_tuning(**kw)
 
