@@ -88,6 +88,9 @@ __device__ inline double readlane_f64(double x, int lane) {
     return __hiloint2double(hi, lo);
 }
 __device__ inline int readlane_i32(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+__device__ inline float readlane_f32(float x, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+}
 
 // Wavefront-uniform reads of data that no kernel in flight writes: through the constant address
 // space they become scalar (SMEM) loads into SGPRs.
@@ -1550,13 +1553,18 @@ void k_wing_bf16(ProfileArgs A) {
         // the star's last point at most; below j0 for padding between two stars' grids (such a
         // tile has no lines and writes nothing)
         const int64_t j1 = min(j0 + A.tile_w, go + S.n_wl) - 1;
-        double lam[NB];
+        // wavelengths in DELTA_UNIT relative to the tile's first point, in Float32: the offsets
+        // lambda - lambda0 below are then one Float32 subtraction.  A far-zone offset is at least
+        // the near-zone radius (3.2 sigma sqrt 2, ten units and more) while the tile spans 32 NB
+        // units, so its relative rounding stays below 1e-6 (R's, 2e-6: below the bfloat16 split's)
+        const double lamt0 = S.wls[min(j0, j1) - go] * (1.0 / DELTA_UNIT);  // (padding: j1)
+        float lam[NB];
         int jj[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             jj[b] = (int)(j0 + 32 * b + n);
-            // in units of DELTA_UNIT (points past j1 are never used)
-            lam[b] = S.wls[min((int64_t)jj[b], j1) - go] * (1.0 / DELTA_UNIT);
+            // (points past j1 are never used)
+            lam[b] = (float)(S.wls[min((int64_t)jj[b], j1) - go] * (1.0 / DELTA_UNIT) - lamt0);
         }
         floatx16 acc[2][NB];
 #pragma unroll
@@ -1576,9 +1584,9 @@ void k_wing_bf16(ProfileArgs A) {
         const int j0s = (int)j0, j1s = (int)j1;
         (void)j1s;
         uint32_t mk[NB];
-        double lam0s = 0.0;
+        float lam0s = 0.0f;
         auto describe = [&](const LineMeta& m) {
-            lam0s = m.lam0 * (1.0 / DELTA_UNIT);
+            lam0s = (float)(m.lam0 * (1.0 / DELTA_UNIT) - lamt0);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int base = j0s + 32 * b;
@@ -1609,10 +1617,10 @@ void k_wing_bf16(ProfileArgs A) {
                     ua[mb] = recB[la * rstride + 32 * mb];
                     ub[mb] = recB[lb2 * rstride + 32 * mb];
                 }
-                const double c0 = readlane_f64(lam0s, q0), c1 = readlane_f64(lam0s, q1);
-                const double c2 = readlane_f64(lam0s, q2), c3 = readlane_f64(lam0s, q3);
-                const double l0a = hi_half ? c2 : c0;
-                const double l0b = hi_half ? c3 : c1;
+                const float c0 = readlane_f32(lam0s, q0), c1 = readlane_f32(lam0s, q1);
+                const float c2 = readlane_f32(lam0s, q2), c3 = readlane_f32(lam0s, q3);
+                const float l0a = hi_half ? c2 : c0;
+                const float l0b = hi_half ? c3 : c1;
                 bf16x8 bh[NB], bl[NB];
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
@@ -1622,7 +1630,7 @@ void k_wing_bf16(ProfileArgs A) {
                     const uint32_t m3 = g + 3 < nb ? (uint32_t)readlane_i32((int)mk[b], q3) : 0u;
                     const bool ua_ = __builtin_amdgcn_inverse_ballot_w64(m0 | ((uint64_t)m2 << 32));
                     const bool ub_ = __builtin_amdgcn_inverse_ballot_w64(m1 | ((uint64_t)m3 << 32));
-                    const float da = (float)(lam[b] - l0a), db = (float)(lam[b] - l0b);
+                    const float da = lam[b] - l0a, db = lam[b] - l0b;
                     float Ra = __builtin_amdgcn_rcpf(da * da), Rb = __builtin_amdgcn_rcpf(db * db);
                     Ra = ua_ ? Ra : 0.0f;
                     Rb = ub_ ? Rb : 0.0f;
