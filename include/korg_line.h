@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KORG_LINE_ABI_VERSION 16
+#define KORG_LINE_ABI_VERSION 17
 
 enum korg_status {
     KORG_OK = 0,
@@ -153,6 +153,10 @@ typedef struct {
     /* Float32 form with korg_tuning.count_branches: near-zone evaluations in Hunger branch A
      * (the near zone holds every branch-B and -C evaluation; its remaining evaluations are D) */
     double n_evals_near_A;
+    /* Float32 form with the far field: device time of the far-field kernel (included in
+     * profile_ms), and the (line, 512-point super-tile) entries of its line lists */
+    double far_ms;
+    double n_far_entries;
 } korg_stats;
 
 typedef struct korg_ctx korg_ctx;
@@ -268,6 +272,12 @@ typedef struct {
                                   2 (default) = beside it on a second stream of the context, into
                                   a buffer of n_layers x points floats added afterwards; same
                                   alpha (the allocation-free entry point always runs 1) */
+    int32_t far_field;         /* Float32 far wing: 1 = every (line, tile) pair term by term;
+                                  2 (default) = the far field, where a 64-point sub-tile lies
+                                  inside a line's window, outside its near zone and more than 8
+                                  half-widths from its centre, as a 12-term expansion about the
+                                  sub-tile's centre (truncation below 2e-10 of the leading term);
+                                  grids that are not evenly spaced take no far field */
 } korg_tuning;
 int korg_set_tuning(korg_ctx* ctx, const korg_tuning* tuning);
 int korg_get_tuning(const korg_ctx* ctx, korg_tuning* out);
@@ -369,7 +379,7 @@ typedef struct korg_plan korg_plan;
 typedef struct {
     int64_t n_stars;
     int64_t n_lines;        /* lines of all stars */
-    int64_t n_points;       /* grid points of all stars, each star padded to a multiple of 128 */
+    int64_t n_points;       /* grid points of all stars, each star padded to a multiple of 512 */
     int64_t tile_capacity;  /* entries allocated for the per-tile line lists (Float32 form) */
     int64_t tile_entries;   /* entries the last finished replay needed */
     int32_t n_captures;     /* graph captures so far (1 + regrowths) */

@@ -13,8 +13,12 @@
 //   Float64 form:
 //     k_profile_f64  line_profile/voigt_hjerting per evaluation (the reference's expressions)
 //   Float32 form with Float64 fix-ups:
+//     k_wing_ff      the far field: each line's far-wing series where a 64-point sub-tile lies
+//                    well away from its centre, as 12-term expansions about the sub-tiles'
+//                    centres summed over lines on the matrix cores
 //     k_wing_bf16    far wing of every line as a rank-4 update on the matrix cores (bfloat16
-//                    MFMA on split high/low operands; k_wing_mfma, FP32 MFMA, when WING_BF16 = 0)
+//                    MFMA on split high/low operands; k_wing_mfma, FP32 MFMA, when WING_BF16 = 0),
+//                    each tile starting from its sub-tiles' far-field sums
 //     k_near         the "near zone" of each line (Voigt cores, and where the damped-wing series
 //                    is not yet converged): exact branch-by-branch evaluation
 // Each profile work item owns its block of alpha outright; lines are added in a fixed order, and
@@ -206,10 +210,11 @@ struct LayerTab {
 // [6..9] evaluations by Hunger branch, [10] near-zone evaluations, [11] far-wing MFMAs per layer
 // block, [12] tail-zone evaluations, [13] entries of the per-tile line lists, [14] segments of the
 // adjoint, [15] branch-A evaluations inside near zones (with branch counting), [16..23] the
-// adjoint moments' queues (one per part of the group list).
+// adjoint moments' queues (one per part of the group list), [24] the far-field queue, [25]
+// entries of the far-field line lists.
 // Error flags: 1 lines not sorted, 2 species index out of range, 4 the per-tile line lists need
 // more entries than their buffer holds.  With any flag set no kernel writes alpha.
-constexpr int N_COUNTERS = 24;
+constexpr int N_COUNTERS = 26;
 constexpr unsigned long long FLAG_UNSORTED = 1, FLAG_SPECIES = 2, FLAG_CAPACITY = 4;
 // Largest far-wing series coefficient kept in Float32 (2^120: headroom below FLT_MAX ~ 2^128).
 constexpr double F32_COEF_MAX = 1.329227995784916e36;
@@ -217,11 +222,12 @@ constexpr double F32_COEF_MAX = 1.329227995784916e36;
 struct GridInfo {
     double g0, ginv;  // first point and inverse mean spacing of the wavelength grid
     double k0, kinv;  // the same for the continuum knots
+    double step;      // mean spacing of the grid (0 for a single point)
 };
 
 // One star -- one line_absorption! call -- of a batch.  Grid indices in every kernel are global:
 // the star's points occupy [grid_off, grid_off + n_wl) of the concatenation of all stars' grids,
-// each padded to a multiple of STAR_ALIGN points so that no tile of any kernel straddles two
+// each padded to a multiple of STAR_PAD points so that no tile of any kernel straddles two
 // stars; its lines are [line_off, line_off + n_lines) of the concatenated line lists.
 constexpr int STAR_ALIGN = 128;
 struct StarDev {
@@ -233,7 +239,8 @@ struct StarDev {
     const int32_t* species;
     const int64_t *win_lb, *win_ub;  // frozen 1-based windows (null: computed)
     const double *temps, *n_e, *n_div_U, *n_HI, *n_HeI, *n_H2, *xi;
-    int32_t cntm_mode, pad0;
+    int32_t cntm_mode;
+    int32_t ff_irregular;  // far field: nonzero when the grid is not evenly spaced (k_ff_geo)
     int64_t n_knots;
     const double *knots, *cntm;
     double cutoff;
@@ -283,6 +290,8 @@ __global__ void k_layers(int n_stars, int L, StarDev* __restrict__ stars, LayerT
         const int64_t n_knots = S.cntm_mode == KORG_CNTM_GRID ? S.n_knots : 0;
         GridInfo g;
         g.g0 = S.wls[0];
+        g.step = S.n_wl > 1 ? (S.wls[S.n_wl - 1] - S.wls[0]) / (double)(S.n_wl - 1) : 0.0;
+        S.ff_irregular = 0;
         g.ginv = S.n_wl > 1 ? (double)(S.n_wl - 1) / (S.wls[S.n_wl - 1] - S.wls[0]) : 0.0;
         g.k0 = n_knots > 1 ? S.knots[0] : 0.0;
         g.kinv = n_knots > 1 ? (double)(n_knots - 1) / (S.knots[n_knots - 1] - S.knots[0]) : 0.0;
@@ -301,6 +310,99 @@ __global__ void k_layers(int n_stars, int L, StarDev* __restrict__ stars, LayerT
         t.t03 = pow(T / 10000.0, 0.3);
         tab[(int64_t)st * L + l] = t;
     }
+}
+
+// --------------------------------------------------------------------------------------------
+// The far field of the Float32 far wing.  Cut the grid into sub-tiles of FF_SUB points with
+// centre c and half-width h (of their points' wavelengths).  Where a sub-tile lies inside a
+// line's window, outside its near zone and at least FF_RATIO half-widths from its centre, the
+// line's far-wing series sum_k c_k (d + t)^(-2k) (d = c - lam0, t = lambda - c, |t| <= h) is the
+// convergent expansion
+//   sum_m tau^m sum_k c_k W_km,   tau = t / h,   W_km = binom(-2k, m) (h/d)^m d^(-2k),
+// truncated after FF_TERMS terms: at |h/d| <= 1/FF_RATIO the truncation is below 2e-10 of the
+// leading (R) term's value at every point, and below 1.5e-6 of the R^4 term's (itself at most
+// 1e-5 of the leading one beyond the tail radius).  The sums over lines, T[layer, m] = sum_(line, k) c_k W_km, are a matrix product
+// per sub-tile (k_wing_ff, a super-tile of FF_NS sub-tiles per work item, so each line's
+// coefficients are read once per FF_NS sub-tiles); the far-wing kernel starts each tile's
+// accumulators from T times the powers tau^m of its points and adds only the lines whose
+// far field does not cover the tile.  On the 300,000-line workload 93% of the far-wing
+// (line, point) pairs fall in far fields, with 12 terms per 64 points in place of the points.
+// k_setup decides once per line which sub-tiles take its far field: two runs of whole sub-tiles
+// inside the window and clear of the near zone, the lower one starting at the window's first
+// whole sub-tile, the upper one ending at its last (disjoint).  Every later kernel tests
+// membership of these runs only, so all agree.  The runs are found in closed form on an evenly
+// spaced grid (the grids of synthesize); a star whose grid is not (several ranges with gaps)
+// takes no far field and is computed term by term.
+// --------------------------------------------------------------------------------------------
+constexpr int FF_SUB = 64;        // points per sub-tile
+constexpr int FF_NS = 8;          // sub-tiles per work item of k_wing_ff
+constexpr int FF_TERMS = 12;      // terms of the expansion in tau
+constexpr double FF_RATIO = 8.0;  // distance from the line centre, in sub-tile half-widths
+constexpr int FF_CB = FF_NS * FF_TERMS / 32;  // 32-column blocks of a work item's T
+static_assert(FF_NS * FF_TERMS % 32 == 0, "whole column blocks");
+// Stars' grids are padded to whole far-field super-tiles: then every tile and super-tile lies in
+// one star and holds, in the same order, what it holds when the star is alone, so a batched star's
+// alpha is bitwise its single-star alpha.  (STAR_ALIGN-point blocks divide it.)
+constexpr int STAR_PAD = FF_NS * FF_SUB;
+static_assert(STAR_PAD % STAR_ALIGN == 0, "padding in whole blocks");
+
+struct FfGeo {
+    double c_cm, h_cm;  // centre and half-width of the sub-tile's points (cm)
+    double c_u;         // the centre in DELTA_UNIT
+    float h_u;          // the half-width in DELTA_UNIT
+    int first, last;    // first and last grid point of the sub-tile (global; first > last: none)
+    int pad;
+};
+static_assert(sizeof(FfGeo) == 40, "FfGeo layout");
+
+// Whether global sub-tile g lies in a line's far-field runs (x..y and z..w).
+__device__ inline bool ff_in(const int4& r, int64_t g) {
+    return (g >= r.x && g <= r.y) || (g >= r.z && g <= r.w);
+}
+// The list builders' key of a line: window (global grid indices), the lengths of its two runs
+// packed in 16 bits each, and the line's index.  The lower run starts at the window's first whole
+// sub-tile and the upper one ends at its last, so the window and the lengths give both runs (a
+// run longer than 65,535 sub-tiles is shortened: the rest is computed term by term).
+__device__ inline int ff_first_whole(int lb) { return (lb + FF_SUB - 1) / FF_SUB; }
+__device__ inline int ff_last_whole(int ub) { return (ub + 1) / FF_SUB - 1; }
+__device__ inline int4 ff_key(int lb, int ub, const int4& r, int line) {
+    const int lo = min(max(r.y - r.x + 1, 0), 0xFFFF), hi = min(max(r.w - r.z + 1, 0), 0xFFFF);
+    return make_int4(lb, ub, (lo << 16) | hi, line);
+}
+// The key's runs, lower then upper (empty when the end is before the start).
+__device__ inline int4 ff_key_runs(const int4& k) {
+    const int a = ff_first_whole(k.x), z = ff_last_whole(k.y);
+    return make_int4(a, a + (int)((unsigned)k.z >> 16) - 1, z - (k.z & 0xFFFF) + 1, z);
+}
+// The sub-tiles' geometry, one thread per sub-tile.  Sub-tiles never straddle two stars (stars are
+// aligned to a multiple of FF_SUB points); one wholly in a star's padding has no points.
+__global__ __launch_bounds__(256) void k_ff_geo(int64_t n_sub, int n_stars,
+                                                StarDev* __restrict__ stars,
+                                                FfGeo* __restrict__ geo) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_sub) return;
+    const int64_t j0 = g * FF_SUB;
+    StarDev& S = stars[star_of_point(stars, n_stars, j0)];
+    const int64_t first = max(j0, S.grid_off);
+    const int64_t last = min(j0 + FF_SUB - 1, S.grid_off + S.n_wl - 1);
+    FfGeo o = {0.0, 0.0, 0.0, 0.0f, 1, 0, 0};
+    if (first <= last) {
+        // the star's grid must be evenly spaced for the setup's far-field runs: every point
+        // within 1e-3 of a step of the line through its first point with the mean spacing
+        const double w0 = S.wls[0], step = S.gi.step;
+        bool bad = !(step > 0.0);
+        for (int64_t j = first; j <= last && !bad; ++j)
+            bad = !(fabs(S.wls[j - S.grid_off] - (w0 + (double)(j - S.grid_off) * step)) <= 1e-3 * step);
+        if (bad) atomicOr(&S.ff_irregular, 1);
+        const double a = S.wls[first - S.grid_off], b = S.wls[last - S.grid_off];
+        o.c_cm = 0.5 * (a + b);
+        o.h_cm = 0.5 * (b - a);
+        o.c_u = o.c_cm * (1.0 / DELTA_UNIT);
+        o.h_u = (float)(o.h_cm * (1.0 / DELTA_UNIT));
+        o.first = (int)first;
+        o.last = (int)last;
+    }
+    geo[g] = o;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -323,6 +425,11 @@ struct SetupArgs {
     double4* recM64;  // the Float64 series form: the same coefficients in Float64 (recM unused)
     unsigned long long* counters;
     int2* azone;     // Float32 form, when given: each line's adjoint near zone (global indices)
+    // far field (null ffrun: off): sub-tile geometry, and each line's two runs of far-field
+    // sub-tiles (global sub-tile indices, x..y and z..w; empty when x > y, z > w)
+    const FfGeo* geo;
+    int64_t n_sub;
+    int4* ffrun;
 };
 
 constexpr double INV_SQRT_PI = 0.5641895835477563;
@@ -618,6 +725,32 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             if (near_wing > 0) atomicAdd(&A.counters[15], (unsigned long long)near_wing);
         }
     }
+    int4 run = make_int4(1, 0, 1, 0);
+    if (A.ffrun && lb <= ub && S.ff_irregular == 0 && gi.step > 0.0) {
+        // The far-field runs, in local sub-tile indices q (points 64q .. 64q + 63, whole
+        // sub-tiles only): inside the window, clear of the near zone, and with the sub-tile's
+        // centre 64q + 31.5 at least 8 * 31.5 + 1 points from the line centre x0 (the grid is
+        // evenly spaced to 1e-3 of a step, so the expansion's ratio stays above 8 * 252/253).
+        // The lower run starts at the first whole sub-tile of the window, the upper one ends at
+        // the last; an empty run has its end before its start.
+        const double x0 = (wl - S.wls[0]) / gi.step;  // the line centre in points
+        const int64_t q0 = (lb + FF_SUB - 1) / FF_SUB;                 // first whole sub-tile
+        const int64_t q1 = (ub + 1) / FF_SUB - 1;                      // last one
+        const double reach = FF_RATIO * 0.5 * (FF_SUB - 1) + 1.0;      // points, centre to centre
+        int64_t lo_end = (int64_t)floor((x0 - reach - 0.5 * (FF_SUB - 1)) / FF_SUB);
+        int64_t hi_start = (int64_t)ceil((x0 + reach - 0.5 * (FF_SUB - 1)) / FF_SUB);
+        if (clo <= chi) {
+            lo_end = min(lo_end, (int64_t)floor((double)(clo - FF_SUB) / FF_SUB));  // 64q + 63 < clo
+            hi_start = max(hi_start, chi / FF_SUB + 1);                          // 64q > chi
+        }
+        lo_end = min(lo_end, q1);
+        hi_start = max(hi_start, max(q0, lo_end + 1));
+        // runs of at most 65,535 sub-tiles (the list keys hold their lengths in 16 bits)
+        lo_end = min(lo_end, q0 + 0xFFFE);
+        hi_start = max(hi_start, q1 - 0xFFFE);
+        const int64_t gq = S.grid_off / FF_SUB;
+        run = make_int4((int)(q0 + gq), (int)(lo_end + gq), (int)(hi_start + gq), (int)(q1 + gq));
+    }
     int cls = N_CLASSES;
     if (lb <= ub) {
         const int64_t W = ub - lb + 1;
@@ -639,6 +772,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         mt.fhi = (int32_t)(fhi + go);
         A.meta[i] = mt;
         if (F32 && A.azone) A.azone[i] = make_int2((int)(alo + go), (int)(ahi + go));
+        if (A.ffrun) A.ffrun[i] = run;
     }
 }
 
@@ -767,7 +901,9 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
                                                         Cand* __restrict__ cands,
                                                         int* __restrict__ cls_jc,
                                                         int2* __restrict__ keyrng,
-                                                        int* __restrict__ part_line, int key) {
+                                                        int* __restrict__ part_line, int key,
+                                                        const int4* __restrict__ ffrun,
+                                                        int4* __restrict__ ffkey) {
     __shared__ int wave_cnt[PART_BLOCK / WAVE][N_CLASSES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
@@ -799,6 +935,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_scatter(int64_t n, const LineMet
         // the key's grid-index range and the line index, compact, for the tile-list passes
         keyrng[pos] = key == KEY_WINDOW ? make_int2(m.lb, m.ub) : make_int2(m.clo, m.chi);
         part_line[pos] = (int)i;
+        if (ffrun) ffkey[pos] = ff_key(m.lb, m.ub, ffrun[i], (int)i);
     }
 }
 
@@ -829,14 +966,21 @@ __global__ void k_ranges(int ntiles, int tile_w, int64_t n_wl, const int* __rest
 }
 
 // Exact per-tile line lists.  One wavefront per (tile, class) scans the class's candidates for
-// the tile (the range from k_ranges) and keeps those whose key range meets the tile.  The count
-// pass writes the number per (tile, class); after an exclusive scan over (tile, class) in
-// tile-major order, the fill pass writes the line indices: for every tile, class by class and in
-// partition order within a class -- the same fixed order as the candidate scans.
+// the tile (the range from k_ranges) and keeps those whose key range meets the tile -- with the
+// far field on (ffkey given), except where the line's far-field runs cover the tile.
+// The count pass writes the number per (tile, class); after an exclusive scan over (tile, class)
+// in tile-major order, the fill pass writes the line indices: for every tile, class by class and
+// in partition order within a class -- the same fixed order as the candidate scans.
+// With the far field the same scan also builds the far-field lists, in slots ntiles * N_CLASSES
+// + (tile, class): a line joins the list of the super-tile (FF_NS sub-tiles) holding this tile
+// from the tile holding the first sub-tile of the super-tile that its runs cover -- once per
+// (line, super-tile).  A super-tile's list is then its tiles' slots, in tile, class and
+// partition order.
 __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int64_t n_wl,
                                                      const int2* __restrict__ ranges,
                                                      const int2* __restrict__ keyrng,
                                                      const int* __restrict__ part_line,
+                                                     const int4* __restrict__ ffkey,
                                                      const int64_t* __restrict__ tc_start,
                                                      int* __restrict__ tc_count,
                                                      int* __restrict__ lists,
@@ -849,20 +993,63 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
     const int64_t j1 = min(j0 + tile_w, n_wl) - 1;
     const int2 r = ranges[t];
     const int64_t slot = (int64_t)tile * N_CLASSES + c;
+    const int64_t fslot = (int64_t)ntiles * N_CLASSES + slot;
     const int64_t base = tc_start ? tc_start[slot] : 0;
-    int64_t cnt = 0;
-    for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
-        const int k = k0 + lane;
-        bool ovl = false;
-        if (k < r.y) {
-            const int2 kr = keyrng[k];
-            ovl = kr.x <= kr.y && kr.x <= j1 && kr.y >= j0;
+    const int64_t fbase = tc_start && ffkey ? tc_start[fslot] : 0;
+    // 32-bit indices from here (grids hold fewer than 2^31 points): the tile's points and
+    // sub-tiles, and the sub-tiles of its super-tile
+    const int i0 = (int)j0, i1 = (int)j1;
+    const int ga = i0 / FF_SUB, gz = i1 / FF_SUB;
+    const int s0 = ga / FF_NS * FF_NS, s1 = s0 + FF_NS - 1;
+    int64_t cnt = 0, fcnt = 0;
+    const uint64_t below = (1ull << lane) - 1;
+    if (ffkey) {
+        // software-pipelined: the next batch's key (and, filling, line index) is loaded while
+        // this batch is tested
+        const int4 none = make_int4(1, 0, 0, 0);
+        int4 key_n = r.x + lane < r.y ? ffkey[r.x + lane] : none;
+        for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
+            const int4 key = key_n;
+            const int line = key.w;
+            const int kn = k0 + WAVE + lane;
+            key_n = kn < r.y ? ffkey[kn] : none;  // past the end: no window
+            bool ovl = false, far = false;
+            if (key.x <= key.y && key.x <= i1 && key.y >= i0) {  // the window meets the tile
+                const int4 run = ff_key_runs(key);
+                ovl = !((ga >= run.x && gz <= run.y) || (ga >= run.z && gz <= run.w));
+                // the first sub-tile of the super-tile in a run; the line joins the far-field
+                // list from the tile holding that sub-tile's first point
+                int f = max(run.x, s0);
+                if (f > min(run.y, s1)) {
+                    f = max(run.z, s0);
+                    if (f > min(run.w, s1)) f = INT_MAX;
+                }
+                far = f != INT_MAX && f * FF_SUB >= i0 && f * FF_SUB <= i1;
+            }
+            const uint64_t b = ballot(ovl);
+            const uint64_t fb = ballot(far);
+            if (lists && ovl) lists[base + cnt + __popcll(b & below)] = line;
+            if (lists && far) lists[fbase + fcnt + __popcll(fb & below)] = line;
+            cnt += __popcll(b);
+            fcnt += __popcll(fb);
         }
-        const uint64_t b = ballot(ovl);
-        if (lists && ovl) lists[base + cnt + __popcll(b & ((1ull << lane) - 1))] = part_line[k];
-        cnt += __popcll(b);
+    } else {
+        for (int k0 = r.x; k0 < r.y; k0 += WAVE) {
+            const int k = k0 + lane;
+            bool ovl = false;
+            if (k < r.y) {
+                const int2 kr = keyrng[k];
+                ovl = kr.x <= kr.y && kr.x <= j1 && kr.y >= j0;
+            }
+            const uint64_t b = ballot(ovl);
+            if (lists && ovl) lists[base + cnt + __popcll(b & below)] = part_line[k];
+            cnt += __popcll(b);
+        }
     }
-    if (!tc_start && lane == 0) tc_count[slot] = (int)cnt;
+    if (!tc_start && lane == 0) {
+        tc_count[slot] = (int)cnt;
+        if (ffkey) tc_count[fslot] = (int)fcnt;
+    }
 }
 
 // One block: exclusive scan of n counts into start[0..n] (start[n] = total; start may be count
@@ -978,6 +1165,11 @@ struct ProfileArgs {
     const LineMeta* meta;
     const int64_t* tile_start;  // [tile * N_CLASSES]: first entry of the tile's list (far wing)
     const int* tile_lines;      // per tile, the lines whose window meets it (far wing)
+    // far field (null fft: off): sub-tile geometry and the far-field sums T of every sub-tile
+    const FfGeo* geo;
+    const float* fft;
+    const int4* ffrun;  // each line's far-field runs
+    int64_t n_sub;
     int alpha_f32;
     unsigned long long* counters;
 };
@@ -1577,6 +1769,53 @@ void k_wing_bf16(ProfileArgs A) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) asm volatile("" : "+a"(acc[0][b]), "+a"(acc[1][b]));
         };
+        // far field: the sub-tile of each 32-point block (wavefront-uniform)
+        FfGeo gb[NB];
+        if (A.fft) {
+            const KCONST FfGeo* geo = uniform_ptr(A.geo);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) gb[b] = geo[min((j0 + 32 * b) / FF_SUB, A.n_sub - 1)];
+        }
+        if (A.fft && w == 0) {
+            // wave 0 starts its blocks from the far-field sums: acc = T P with A = T (64 layers x
+            // the terms, padded to 16 with zeros) and B = tau^m of the block's points
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int64_t g = (j0 + 32 * b) / FF_SUB;
+                if (g >= A.n_sub || gb[b].first > gb[b].last) continue;  // padding: no points
+                const int64_t jp = min(max((int64_t)jj[b], (int64_t)gb[b].first), (int64_t)gb[b].last);
+                const float tau = gb[b].h_u > 0.0f
+                    ? (float)(S.wls[jp - go] * (1.0 / DELTA_UNIT) - gb[b].c_u) / gb[b].h_u : 0.0f;
+                float pw[8];
+                float p = 1.0f;
+                if (hi_half) {
+                    const float t2 = tau * tau, t4 = t2 * t2;
+                    p = t4 * t4;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    pw[e] = 8 * hf + e < FF_TERMS ? p : 0.0f;
+                    p *= tau;
+                }
+                bf16x8 ph, pl;
+                split_bf16(pw, sel, ph, pl);
+                const float* __restrict__ Tg =
+                    A.fft + (((g / FF_NS) * A.n_lblocks + lblk) * FF_NS + g % FF_NS) * (WAVE * FF_TERMS);
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    const float* row = Tg + (32 * mb + n) * FF_TERMS + 8 * hf;
+                    const float4 v0 = *(const float4*)row;
+                    float4 v1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (8 * hf + 4 < FF_TERMS) v1 = *(const float4*)(row + 4);
+                    const float tv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                    bf16x8 th, tl;
+                    split_bf16(tv, sel, th, tl);
+                    acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, ph, acc[mb][b], 0, 0, 0);
+                    acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, pl, acc[mb][b], 0, 0, 0);
+                    acc[mb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tl, ph, acc[mb][b], 0, 0, 0);
+                }
+            }
+        }
         // coefficients of (line, layer 32 mb + n): 16 B, high parts then low parts
         const uint4* __restrict__ recB = (const uint4*)A.recM + (int64_t)lblk * WAVE + n;
         const int64_t rstride = (int64_t)A.n_lblocks * WAVE;
@@ -1585,12 +1824,15 @@ void k_wing_bf16(ProfileArgs A) {
         (void)j1s;
         uint32_t mk[NB];
         float lam0s = 0.0f;
-        auto describe = [&](const LineMeta& m) {
+        auto describe = [&](const LineMeta& m, const int4& run) {
             lam0s = (float)(m.lam0 * (1.0 / DELTA_UNIT) - lamt0);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int base = j0s + 32 * b;
                 mk[b] = span_bits(m.lb, m.ub, base) & ~span_bits(m.clo, m.chi, base);
+                // a block whose sub-tile takes this line's far field has it already (tiles of
+                // at most one sub-tile list no such line)
+                if (NB * 32 > FF_SUB && A.fft && ff_in(run, (j0 + 32 * b) / FF_SUB)) mk[b] = 0u;
             }
         };
         const int64_t t0 = A.tile_start[(int64_t)tile * N_CLASSES];
@@ -1601,7 +1843,7 @@ void k_wing_bf16(ProfileArgs A) {
         for (int64_t kb = s0; kb < s1; kb += WAVE) {
             const int nb = (int)min((int64_t)WAVE, s1 - kb);
             const int idx = tl[min(kb + lane, s1 - 1)];  // clamped: a stand-in past the end
-            describe(A.meta[idx]);
+            describe(A.meta[idx], NB * 32 > FF_SUB && A.fft ? A.ffrun[idx] : make_int4(1, 0, 1, 0));
             for (int g = 0; g < nb; g += 4) {
                 // the group's batch positions g .. g + 3; past nb: stand-ins with no points
                 const int q0 = min(g, nb - 1), q1 = min(g + 1, nb - 1);
@@ -1701,6 +1943,226 @@ void k_wing_bf16(ProfileArgs A) {
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// k_wing_ff: the far-field sums T of one super-tile (FF_NS sub-tiles) and 64-layer block per
+// work item, one wave each:  T[layer, (s, m)] = sum over the super-tile's far-field list of
+// sum_k c_k(line, layer) W_km(line, s)  on v_mfma_f32_32x32x16_bf16, K = 4 lines x 4 powers:
+// A = the lines' coefficients exactly as k_wing_bf16 reads them, B = their W at 32 (sub-tile,
+// term) columns (zero for a sub-tile whose far field the line does not take), both split into
+// bfloat16 high and low parts (three products).  Column 32 cb + n of an item is sub-tile
+// (32 cb + n) / FF_TERMS, term (32 cb + n) % FF_TERMS.  W = binom(-2k, m) u^m R^k with
+// R = 1/d^2 and u = h/d (|u| <= 1/FF_RATIO), d = c - lam0 in DELTA_UNIT (a Float64 difference,
+// then Float32).  T goes to fft as [super-tile][layer block][sub-tile][64 layers][FF_TERMS].
+// ---------------------------------------------------------------------------------------------
+struct FarArgs {
+    int n_lblocks;
+    int64_t n_items;    // super-tiles x layer blocks
+    int64_t n_sub;      // sub-tiles of the call
+    int64_t base_slot;  // first far-field slot of tile_start
+    int ntiles;         // wing tiles of the call
+    int per;            // wing tiles per super-tile
+    const FfGeo* geo;
+    const LineMeta* meta;
+    const int4* ffrun;  // each line's far-field runs
+    const float4* recM;
+    const int64_t* tile_start;
+    const int* tile_lines;
+    float* fft;
+    unsigned long long* counters;
+};
+
+// u^m for 0 <= m < 16, by squaring.
+__device__ inline float pow_small(float u, int m) {
+    const float u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
+    float p = (m & 1) ? u : 1.0f;
+    p *= (m & 2) ? u2 : 1.0f;
+    p *= (m & 4) ? u4 : 1.0f;
+    p *= (m & 8) ? u8 : 1.0f;
+    return p;
+}
+
+// WW waves share a work item, each taking a contiguous 1/WW of its list; their partial sums are
+// added in wave order through LDS (deterministic).  96 accumulator registers and about 150 vector
+// registers: two waves per SIMD (the occupancy hint is left at one, so that the compiler need not
+// split the register file evenly and spill).
+template <int WW>
+__global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(WW > 1 ? 2 : 1))) void k_wing_ff(FarArgs A) {
+    __shared__ FfGeo sg[FF_NS];
+    __shared__ float4 s_acc[WW > 1 ? WW - 1 : 1][2 * FF_CB][4][WAVE];
+    __shared__ int s_item;
+    const int lane = threadIdx.x % WAVE;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int n = lane & 31, hf = lane >> 5;
+    const bool hi_half = hf != 0;
+    const SplitSel sel = split_selectors();
+    if (A.counters[3] != 0) return;  // failed setup: nothing is written
+    // the lane's sub-tile and term in each column block, and binom(-2k, m) for k = 1..4
+    int scol[FF_CB], mcol[FF_CB];
+    float bk[FF_CB][4];
+#pragma unroll
+    for (int cb = 0; cb < FF_CB; ++cb) {
+        const int col = 32 * cb + n;
+        scol[cb] = col / FF_TERMS;
+        mcol[cb] = col % FF_TERMS;
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) {
+            double b = 1.0;
+            for (int i = 1; i <= mcol[cb]; ++i) b *= -(double)(2 * k + i - 1) / (double)i;
+            bk[cb][k - 1] = (float)b;
+        }
+    }
+    const int64_t rstride = (int64_t)A.n_lblocks * WAVE;
+    const int* __restrict__ tl = A.tile_lines;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&A.counters[24], 1ull);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= A.n_items) break;  // the same for the whole workgroup
+        const int st = item / A.n_lblocks, lblk = item % A.n_lblocks;
+        if (item == 0 && threadIdx.x == 0)  // the far-field list entries, for the statistics
+            A.counters[25] = (unsigned long long)(A.tile_start[A.base_slot + (int64_t)A.ntiles * N_CLASSES] -
+                                                  A.tile_start[A.base_slot]);
+        if (threadIdx.x < FF_NS) {
+            const int64_t g = (int64_t)st * FF_NS + threadIdx.x;
+            const FfGeo none = {0.0, 0.0, 0.0, 0.0f, 1, 0, 0};
+            sg[threadIdx.x] = g < A.n_sub ? A.geo[g] : none;
+        }
+        __syncthreads();
+        double cu[FF_CB];
+        float hu[FF_CB];
+#pragma unroll
+        for (int cb = 0; cb < FF_CB; ++cb) {
+            cu[cb] = sg[scol[cb]].c_u;
+            hu[cb] = sg[scol[cb]].h_u;
+        }
+        floatx16 acc[2][FF_CB];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int cb = 0; cb < FF_CB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[mb][cb][r] = 0.0f;
+        const uint4* __restrict__ recB = (const uint4*)A.recM + (int64_t)lblk * WAVE + n;
+        // the super-tile's list: the far-field slots of its wing tiles
+        const int64_t e0 = A.tile_start[A.base_slot + (int64_t)st * A.per * N_CLASSES];
+        const int64_t e1 =
+            A.tile_start[A.base_slot + (int64_t)min((st + 1) * A.per, A.ntiles) * N_CLASSES];
+        const int64_t t0 = e0 + ((e1 - e0) * w) / WW;
+        const int64_t t1 = e0 + ((e1 - e0) * (w + 1)) / WW;
+        for (int64_t kb = t0; kb < t1; kb += WAVE) {
+            const int nb = (int)min((int64_t)WAVE, t1 - kb);
+            const int idx = tl[min(kb + lane, t1 - 1)];  // clamped: a stand-in past the end
+            // each lane describes its own line: centre in DELTA_UNIT and far-field sub-tiles
+            const double lam0u = A.meta[idx].lam0 * (1.0 / DELTA_UNIT);
+            const int4 run = A.ffrun[idx];
+            uint32_t emask = 0;
+#pragma unroll
+            for (int s = 0; s < FF_NS; ++s)
+                if (ff_in(run, (int64_t)st * FF_NS + s)) emask |= 1u << s;
+            for (int g = 0; g < nb; g += 4) {
+                const int q0 = min(g, nb - 1), q1 = min(g + 1, nb - 1);
+                const int q2 = min(g + 2, nb - 1), q3 = min(g + 3, nb - 1);
+                const int i0 = readlane_i32(idx, q0), i1 = readlane_i32(idx, q1);
+                const int i2 = readlane_i32(idx, q2), i3 = readlane_i32(idx, q3);
+                const int la = hi_half ? i2 : i0;
+                const int lb2 = hi_half ? i3 : i1;
+                uint4 ua[2], ub[2];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    ua[mb] = recB[la * rstride + 32 * mb];
+                    ub[mb] = recB[lb2 * rstride + 32 * mb];
+                }
+                const double l0 = readlane_f64(lam0u, q0), l1 = readlane_f64(lam0u, q1);
+                const double l2 = readlane_f64(lam0u, q2), l3 = readlane_f64(lam0u, q3);
+                const double lA = hi_half ? l2 : l0, lB = hi_half ? l3 : l1;
+                // stand-ins past nb take no far field
+                const uint32_t e0 = (uint32_t)readlane_i32((int)emask, q0);
+                const uint32_t e1 = g + 1 < nb ? (uint32_t)readlane_i32((int)emask, q1) : 0u;
+                const uint32_t e2 = g + 2 < nb ? (uint32_t)readlane_i32((int)emask, q2) : 0u;
+                const uint32_t e3 = g + 3 < nb ? (uint32_t)readlane_i32((int)emask, q3) : 0u;
+                const uint32_t eA = hi_half ? e2 : e0, eB = hi_half ? e3 : e1;
+                bf16x8 ah[2], al[2];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    ah[mb] = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].x, ua[mb].y, ub[mb].x, ub[mb].y));
+                    al[mb] = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].z, ua[mb].w, ub[mb].z, ub[mb].w));
+                }
+#pragma unroll
+                for (int cb = 0; cb < FF_CB; ++cb) {
+                    // one column block at a time: interleaving them costs registers, not time
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int sc = scol[cb];
+                    float xA = __builtin_amdgcn_rcpf((float)(cu[cb] - lA));
+                    float xB = __builtin_amdgcn_rcpf((float)(cu[cb] - lB));
+                    xA = ((eA >> sc) & 1u) ? xA : 0.0f;
+                    xB = ((eB >> sc) & 1u) ? xB : 0.0f;
+                    const float RA = xA * xA, RB = xB * xB;
+                    const float pA = pow_small(hu[cb] * xA, mcol[cb]);
+                    const float pB = pow_small(hu[cb] * xB, mcol[cb]);
+                    const float a1 = pA * RA, a2 = a1 * RA, a3 = a2 * RA, a4 = a3 * RA;
+                    const float b1 = pB * RB, b2 = b1 * RB, b3 = b2 * RB, b4 = b3 * RB;
+                    const float x[8] = {bk[cb][0] * a1, bk[cb][1] * a2, bk[cb][2] * a3, bk[cb][3] * a4,
+                                        bk[cb][0] * b1, bk[cb][1] * b2, bk[cb][2] * b3, bk[cb][3] * b4};
+                    bf16x8 bh, bl;
+                    split_bf16(x, sel, bh, bl);
+#pragma unroll
+                    for (int mb = 0; mb < 2; ++mb) {
+                        acc[mb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bh, acc[mb][cb], 0, 0, 0);
+                        acc[mb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bl, acc[mb][cb], 0, 0, 0);
+                        acc[mb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mb], bh, acc[mb][cb], 0, 0, 0);
+                    }
+                }
+                // the accumulators stay in AGPRs (the vector registers hold the operands)
+#pragma unroll
+                for (int cb = 0; cb < FF_CB; ++cb) asm volatile("" : "+a"(acc[0][cb]), "+a"(acc[1][cb]));
+            }
+        }
+        // the waves' partial sums, added in wave order by wave 0
+        if (w > 0) {
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int cb = 0; cb < FF_CB; ++cb)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        s_acc[w - 1][mb * FF_CB + cb][q][lane] =
+                            make_float4(acc[mb][cb][4 * q], acc[mb][cb][4 * q + 1],
+                                        acc[mb][cb][4 * q + 2], acc[mb][cb][4 * q + 3]);
+        }
+        __syncthreads();
+        if (w == 0) {
+            // accumulator layout of the 32x32 MFMAs: lane i, register r holds row
+            // 8*(r/4) + 4*(i/32) + r%4, column i%32
+            float* __restrict__ Tst =
+                A.fft + ((int64_t)st * A.n_lblocks + lblk) * (FF_NS * WAVE * FF_TERMS);
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int cb = 0; cb < FF_CB; ++cb)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        __builtin_amdgcn_sched_barrier(0);  // a few loads in flight at a time
+                        float v[4] = {acc[mb][cb][4 * q], acc[mb][cb][4 * q + 1],
+                                      acc[mb][cb][4 * q + 2], acc[mb][cb][4 * q + 3]};
+#pragma unroll
+                        for (int ww = 0; ww < WW - 1; ++ww) {
+                            const float4 p = s_acc[ww][mb * FF_CB + cb][q][lane];
+                            v[0] += p.x;
+                            v[1] += p.y;
+                            v[2] += p.z;
+                            v[3] += p.w;
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int layer = 32 * mb + 8 * q + 4 * hf + e;
+                            Tst[(scol[cb] * WAVE + layer) * FF_TERMS + mcol[cb]] = v[e];
+                        }
+                    }
+        }
+        __syncthreads();  // sg, s_acc and s_item are reused by the next item
+    }
+}
 
 // ---------------------------------------------------------------------------------------------
 // The Float64 series form's far wing: k_wing_bf16's work items (tile of 32 points, block of 64
@@ -2586,7 +3048,7 @@ __global__ void k_zero_alpha(int n_stars, int L, int64_t n_total, const StarDev*
 // interpolation of the knots' values at every grid point -- the alpha that synthesize starts the
 // lines from (src/synthesize.jl:225-226, 248-250), with the interpolation the window search uses
 // (cntm_lerp; outside the knots, the end intervals extended).  A wavefront per 64 points (no chunk
-// straddles two stars: the grids are padded to 128 points): lane p finds point p's knot interval
+// straddles two stars: the grids are padded to 512 points): lane p finds point p's knot interval
 // and weight, then the chunk's alpha -- one contiguous run of 64 L elements -- is written with
 // consecutive lanes on consecutive elements (coalesced stores), each element taking its point's
 // interval from the lane that found it.
@@ -2658,7 +3120,7 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     const int L = A.L;
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.n_total) return;
-    // stars' grids are padded to 128 points: the 64 points of a wavefront share one star
+    // stars' grids are padded to 512 points: the 64 points of a wavefront share one star
     const KCONST StarDev* stars = uniform_ptr(A.stars);
     const int st = star_of_point(stars, A.n_stars, j);
     const KCONST StarDev& S = stars[st];
@@ -2901,7 +3363,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_lead_scatter(int64_t n, const in
 
 // Per (star, layer): the binary exponent e of max_j |G[l, j]| (max |G| 2^-e in [0.5, 1)), by an
 // order-free atomicMax over wavefronts.  A wavefront per 128 points (one star: the stars' grids are
-// padded to 128 points); lanes = layers.
+// padded to 512 points); lanes = layers.
 __global__ __launch_bounds__(256) void k_g_exp(int n_stars, int L, int64_t n_total,
                                                const StarDev* __restrict__ stars, int* __restrict__ gexp) {
     const int lane = threadIdx.x % WAVE;
@@ -3713,6 +4175,8 @@ struct Arena {
 // on the context allocate.
 struct Workspace {
     DevBuf nbuf;  // the near zone's sums when it runs beside the far wing
+    DevBuf ffgeo, fft;  // far field: sub-tile geometry, the sums T of every sub-tile
+    DevBuf ffrun, ffkey;  // far field: each line's runs (line order), list keys (partition order)
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -3729,7 +4193,7 @@ struct Workspace {
                           &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
-                          &nbuf};
+                          &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -3742,7 +4206,8 @@ struct Workspace {
                           &vjp_start, &seg_cnt,  &seg_start, &vjp_partial, &rt_tab,   &fit_G,
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
-                          &scan_part, &jvp_coef, &nbuf};
+                          &scan_part, &jvp_coef, &nbuf,      &ffgeo,     &fft,       &ffrun,
+                          &ffkey};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -3761,7 +4226,7 @@ struct korg_ctx {
     hipStream_t stream = nullptr;
     // [0] start, [1] setup done, [2] partition done, [3] end, [4] far wing done,
     // [5] tile-list total known, [6] tile-list fill launched (the host sizes the lists between)
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[10] = {};
     // the Float32 form's near zone runs on `side`, beside the far wing (fork/join events)
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
@@ -3772,6 +4237,8 @@ struct korg_ctx {
     int last_precision = KORG_PRECISION_FP64;
     int last_n_lblocks = 0;
     int last_nb = 0;  // 32-point blocks per far-wing tile of the last call
+    bool last_ff = false;  // the last forward call ran the far field (ev[9] marks its end)
+    int last_ntiles = 0;   // far-wing tiles of the last call
     bool last_sized = false;  // the last forward call sized the tile lists on the host (ev[5], ev[6])
     Workspace ws;
     // the allocation-free entry point: its workspace (carved from the device's fixed scratch at
@@ -3967,7 +4434,7 @@ void add_stats(korg_stats& a, const korg_stats& b) {
     a.n_evals_near_A += b.n_evals_near_A;
 }
 
-// Lays the stars out in the global index spaces (grids padded to STAR_ALIGN points, lines
+// Lays the stars out in the global index spaces (grids padded to STAR_PAD points, lines
 // concatenated) and uploads the star table to ws.stars on `s`.  Returns the totals.
 int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const StarHost* st,
                  int64_t* n_lines_total, int64_t* n_wl_total) {
@@ -4019,7 +4486,7 @@ int upload_stars(korg_ctx* ctx, Workspace& ws, hipStream_t s, int n_stars, const
         d.flux_c = x.flux_c;
         ctx->last_grid_off[k] = go;
         ctx->last_line_off[k] = lo;
-        go += (x.grid.n_wl + STAR_ALIGN - 1) / STAR_ALIGN * STAR_ALIGN;
+        go += (x.grid.n_wl + STAR_PAD - 1) / STAR_PAD * STAR_PAD;
         lo += x.lines.n_lines;
     }
     if (go > (int64_t)INT32_MAX || lo > (int64_t)INT32_MAX)
@@ -4041,6 +4508,9 @@ struct Batch {
     bool s64 = false;  // the Float64 series form (f32 is set too: the same zones and lists)
     bool overlap = false;  // Float32 form: the near zone beside the far wing (second stream)
     bool adj = false;  // Float32 fit step: the adjoint's series form (near zones, moments)
+    bool ff = false;   // Float32 form: the far wing's far field (k_wing_ff)
+    int64_t n_sub = 0;  // far-field sub-tiles (FF_SUB points) and super-tiles (FF_NS sub-tiles)
+    int nsuper = 0;
     int NB = 2, WW = 4, P64 = 1, wpc = 12, near_wpc = 16, near_layout = 2, count_branches = 0;
     int tile_w = 0, ntiles = 0, nt_near = 0, n_groups = 0, Ls = 0, n_lblocks = 0, nblocks = 0;
     korg_species sp{};
@@ -4056,7 +4526,7 @@ int check_tuning(korg_ctx* ctx, const korg_tuning* t) {
     if (t->points_per_lane < 0 || t->points_per_lane > 2 || t->waves_per_cu < 0 ||
         t->waves_per_cu > 64 || t->near_waves_per_cu < 0 || t->near_waves_per_cu > 64 ||
         t->count_branches < 0 || t->count_branches > 1 || t->near_layout < 0 || t->near_layout > 2 ||
-        t->near_concurrent < 0 || t->near_concurrent > 2 ||
+        t->near_concurrent < 0 || t->near_concurrent > 2 || t->far_field < 0 || t->far_field > 2 ||
         !(tp == 0 || tp == 32 || tp == 64 || tp == 128) || !(wt == 0 || wt == 1 || wt == 2 || wt == 4))
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT, "tuning out of range");
     if (!wing_variant_ok(tp ? tp / 32 : 2, wt ? wt : 4))
@@ -4105,12 +4575,17 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     // not in the fixed-scratch path, whose capacity the near buffer would take
     b.overlap = b.f32 && !b.s64 && b.near_layout != 1 && t.near_concurrent != 1 && !ctx->arena;
     b.count_branches = t.count_branches ? 1 : 0;
+    b.ff = b.f32 && !b.s64 && WING_BF16 && t.far_field != 1;
     TRY(upload_stars(ctx, ws, s, n_stars, stars, &b.N, &b.nwl));
     b.tile_w = b.f32 ? 32 * b.NB : WAVE * b.P64;
     b.ntiles = (int)((b.nwl + b.tile_w - 1) / b.tile_w);
+    if (b.ff) {
+        b.n_sub = (b.nwl + FF_SUB - 1) / FF_SUB;
+        b.nsuper = (int)((b.n_sub + FF_NS - 1) / FF_NS);
+    }
     // scratch of the multi-block scans (tile lists, adjoint items and segments)
     TRY(grow(ctx, ws.scan_part, sizeof(int64_t) *
-                                    (size_t)(scan_blocks(std::max<int64_t>({(int64_t)b.ntiles * N_CLASSES, b.N,
+                                    (size_t)(scan_blocks(std::max<int64_t>({(int64_t)b.ntiles * N_CLASSES * 2, b.N,
                                                                              (int64_t)n_stars})) + 1)));
     if (b.N == 0) return KORG_OK;  // src/line_absorption.jl:35-37
     const int64_t N = b.N;
@@ -4135,8 +4610,16 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     TRY(grow(ctx, ws.cls_jc, sizeof(int) * N));
     TRY(grow(ctx, ws.ranges, sizeof(int2) * (size_t)b.ntiles * N_CLASSES));
     if (b.f32) {
-        TRY(grow(ctx, ws.tc_count, sizeof(int) * (size_t)b.ntiles * N_CLASSES));
-        TRY(grow(ctx, ws.tc_start, sizeof(int64_t) * ((size_t)b.ntiles * N_CLASSES + 1)));
+        // the wing tiles' lists, then (far field) the super-tiles' lists by their tiles' slots
+        const size_t nslots = (size_t)b.ntiles * N_CLASSES * (b.ff ? 2 : 1);
+        TRY(grow(ctx, ws.tc_count, sizeof(int) * nslots));
+        TRY(grow(ctx, ws.tc_start, sizeof(int64_t) * (nslots + 1)));
+        if (b.ff) {
+            TRY(grow(ctx, ws.ffrun, sizeof(int4) * (size_t)N));
+            TRY(grow(ctx, ws.ffkey, sizeof(int4) * (size_t)N));
+            TRY(grow(ctx, ws.ffgeo, sizeof(FfGeo) * (size_t)b.n_sub));
+            TRY(grow(ctx, ws.fft, sizeof(float) * (size_t)b.nsuper * b.n_lblocks * FF_NS * WAVE * FF_TERMS));
+        }
         TRY(grow(ctx, ws.ncstart, sizeof(int) * (N_CLASSES + 1)));
         TRY(grow(ctx, ws.near_cand, sizeof(Cand) * N));
         TRY(grow(ctx, ws.near_rng, sizeof(int2) * N));
@@ -4173,6 +4656,12 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.recM64 = b.s64 ? (double4*)ws.recM.p : nullptr;
     sa.counters = counters;
     sa.azone = b.adj ? (int2*)ws.azone.p : nullptr;
+    sa.geo = b.ff ? (const FfGeo*)ws.ffgeo.p : nullptr;
+    sa.n_sub = b.n_sub;
+    sa.ffrun = b.ff ? (int4*)ws.ffrun.p : nullptr;
+    if (b.ff)  // the sub-tiles' geometry, which the setup's far-field runs read
+        hipLaunchKernelGGL(k_ff_geo, dim3((unsigned)((b.n_sub + 255) / 256)), dim3(256), 0, s,
+                           b.n_sub, b.n_stars, (StarDev*)ws.stars.p, (FfGeo*)ws.ffgeo.p);
     const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);
     if (b.f32) {
         if (b.count_branches) hipLaunchKernelGGL((k_setup<true, true>), sgrid, sblock, 0, s, sa);
@@ -4202,7 +4691,8 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
     hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
                        (const LineMeta*)ws.meta.p, (const int*)ws.blk.p, (const int*)ws.cstart.p,
                        (Cand*)ws.cls_cand.p, (int*)ws.cls_jc.p, (int2*)ws.cls_rng.p,
-                       (int*)ws.cls_line.p, (int)KEY_WINDOW);
+                       (int*)ws.cls_line.p, (int)KEY_WINDOW,
+                       b.ff ? (const int4*)ws.ffrun.p : nullptr, (int4*)ws.ffkey.p);
     hipLaunchKernelGGL(k_ranges, dim3((b.ntiles * N_CLASSES + 255) / 256), dim3(256), 0, s,
                        b.ntiles, b.tile_w, b.nwl, (const int*)ws.cstart.p, (const int*)ws.cls_jc.p,
                        (int2*)ws.ranges.p);
@@ -4213,18 +4703,21 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
         hipLaunchKernelGGL(k_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, N,
                            (const LineMeta*)ws.meta.p, (const int*)ws.nblk.p,
                            (const int*)ws.ncstart.p, (Cand*)ws.near_cand.p, (int*)ws.near_jc.p,
-                           (int2*)ws.near_rng.p, (int*)ws.near_line.p, (int)KEY_NEAR);
+                           (int2*)ws.near_rng.p, (int*)ws.near_line.p, (int)KEY_NEAR,
+                           (const int4*)nullptr, (int4*)nullptr);
         hipLaunchKernelGGL(k_ranges, dim3((b.nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
                            b.nt_near, CT, b.nwl, (const int*)ws.ncstart.p,
                            (const int*)ws.near_jc.p, (int2*)ws.near_ranges.p);
-        // exact per-tile line lists of the far-wing kernel: count and scan
+        // exact per-tile line lists of the far-wing kernels (the wing tiles', then with the far
+        // field the super-tiles'): count and one scan over both
         const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+        const int4* ffkey = b.ff ? (const int4*)ws.ffkey.p : nullptr;
         hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, b.ntiles, b.tile_w,
                            b.nwl, (const int2*)ws.ranges.p, (const int2*)ws.cls_rng.p,
-                           (const int*)ws.cls_line.p, (const int64_t*)nullptr, (int*)ws.tc_count.p,
-                           (int*)nullptr, (const unsigned long long*)counters);
-        enqueue_scan(s, ntc, (const int*)ws.tc_count.p, (int64_t*)ws.tc_start.p, tile_cap, counters,
-                     13, (int64_t*)ws.scan_part.p);
+                           (const int*)ws.cls_line.p, ffkey, (const int64_t*)nullptr,
+                           (int*)ws.tc_count.p, (int*)nullptr, counters);
+        enqueue_scan(s, ntc * (b.ff ? 2 : 1), (const int*)ws.tc_count.p, (int64_t*)ws.tc_start.p,
+                     tile_cap, counters, 13, (int64_t*)ws.scan_part.p);
     }
     HIP_TRY(ctx, hipGetLastError());
     if (timed && b.f32) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
@@ -4283,10 +4776,11 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
     if (b.f32) {
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], s));
         const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+        const int4* ffkey = b.ff ? (const int4*)ws.ffkey.p : nullptr;
         hipLaunchKernelGGL(k_tile_lists, dim3((unsigned)ntc), dim3(WAVE), 0, s, b.ntiles, b.tile_w,
                            b.nwl, (const int2*)ws.ranges.p, (const int2*)ws.cls_rng.p,
-                           (const int*)ws.cls_line.p, (const int64_t*)ws.tc_start.p,
-                           (int*)nullptr, (int*)ws.tile_lines.p, (const unsigned long long*)counters);
+                           (const int*)ws.cls_line.p, ffkey, (const int64_t*)ws.tc_start.p,
+                           (int*)nullptr, (int*)ws.tile_lines.p, counters);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
@@ -4308,6 +4802,10 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
     pa.tile_start = (const int64_t*)ws.tc_start.p;
     pa.tile_lines = (const int*)ws.tile_lines.p;
     pa.ranges = (const int2*)ws.ranges.p;
+    pa.geo = b.ff ? (const FfGeo*)ws.ffgeo.p : nullptr;
+    pa.ffrun = b.ff ? (const int4*)ws.ffrun.p : nullptr;
+    pa.fft = b.ff ? (const float*)ws.fft.p : nullptr;
+    pa.n_sub = b.n_sub;
     pa.alpha_f32 = b.alpha_f32;
     pa.counters = counters;
     if (!b.f32) {
@@ -4327,6 +4825,35 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
             TRY(launch_near(ctx, ws, b, ctx->side, (float*)ws.nbuf.p));
             if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[8], ctx->side));
             HIP_TRY(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+        }
+        if (b.ff) {
+            // the far-field sums of every super-tile, which the wing's tiles start from
+            FarArgs fa;
+            fa.n_lblocks = b.n_lblocks;
+            fa.n_items = (int64_t)b.nsuper * b.n_lblocks;
+            fa.n_sub = b.n_sub;
+            fa.base_slot = (int64_t)b.ntiles * N_CLASSES;
+            fa.ntiles = b.ntiles;
+            fa.per = FF_NS * FF_SUB / b.tile_w;
+            fa.geo = (const FfGeo*)ws.ffgeo.p;
+            fa.meta = (const LineMeta*)ws.meta.p;
+            fa.ffrun = (const int4*)ws.ffrun.p;
+            fa.recM = (const float4*)ws.recM.p;
+            fa.tile_start = (const int64_t*)ws.tc_start.p;
+            fa.tile_lines = (const int*)ws.tile_lines.p;
+            fa.fft = (float*)ws.fft.p;
+            fa.counters = counters;
+            // two waves per SIMD; when the items are too few to occupy them, four waves share
+            // each item
+            const int64_t slots = (int64_t)ctx->n_cu * 8;
+            if (fa.n_items < 2 * slots) {
+                const int64_t nwg = std::min<int64_t>(slots / 4, fa.n_items);
+                hipLaunchKernelGGL(k_wing_ff<4>, dim3((unsigned)nwg), dim3(4 * WAVE), 0, s, fa);
+            } else {
+                hipLaunchKernelGGL(k_wing_ff<1>, dim3((unsigned)slots), dim3(WAVE), 0, s, fa);
+            }
+            HIP_TRY(ctx, hipGetLastError());
+            if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[9], s));
         }
         int64_t nwg = (int64_t)ctx->n_cu * ((b.wpc + b.WW - 1) / b.WW);
         if (nwg > pa.n_items) nwg = pa.n_items;
@@ -4372,7 +4899,7 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
 // to `headroom` times it.
 int size_tile_lists(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, double headroom,
                     int64_t* total_out) {
-    const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
+    const int64_t ntc = (int64_t)b.ntiles * N_CLASSES * (b.ff ? 2 : 1);
     int64_t total = 0;
     HIP_TRY(ctx, hipMemcpyAsync(&total, (int64_t*)ws.tc_start.p + ntc, sizeof total,
                                 hipMemcpyDeviceToHost, s));
@@ -4461,6 +4988,8 @@ int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_spec
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
     ctx->last_nb = b.NB;
+    ctx->last_ff = b.ff;
+    ctx->last_ntiles = b.ntiles;
     ctx->last_sized = b.f32 && !fixed;
     ctx->last_overlap = b.overlap;
     if (b.N == 0) return KORG_OK;
@@ -4482,7 +5011,7 @@ int run_device(korg_ctx* ctx, void* alpha, int alpha_dtype, const korg_grid* g,
 
 // Counters of a finished forward call into ctx->stats; an error for a raised flag.
 int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, int n_lblocks, int nb,
-                        bool s64 = false) {
+                        bool s64 = false, bool ff = false, int ntiles = 0) {
     if (c[3] & FLAG_SPECIES)
         return set_err(ctx, KORG_ERR_INVALID_ARGUMENT,
                        "a line's species index is outside [0, n_species); alpha was left unchanged");
@@ -4497,12 +5026,18 @@ int stats_from_counters(korg_ctx* ctx, const unsigned long long* c, bool f32, in
     for (int k = 0; k < 4; ++k) ctx->stats.n_evals_branch[k] = (double)c[6 + k];
     ctx->stats.n_evals_near = f32 ? (double)c[10] : 0.0;
     // far-wing MFMAs: bfloat16, three per (group of 4 lines, 32-point block, 32-layer half),
-    // counted per line-tile entry of the lists (groups rounded up are not counted); Float32, two
-    // per 32-point block and K step; the Float64 series form, 8 per (line, 32-point tile, 64-layer
-    // block)
+    // counted per line-tile entry of the lists (groups rounded up are not counted); with the far
+    // field, three per (group of 4 lines, 32-column block, 32-layer half) of a super-tile's list
+    // (counters[25] of the c[13] entries) and three per (32-point block, 32-layer half) of every
+    // tile for its start from T; Float32, two per 32-point block and K step; the Float64 series
+    // form, 8 per (line, 32-point tile, 64-layer block)
+    const double far_entries = ff ? (double)c[25] : 0.0;
     ctx->stats.n_wing_mfma = !f32 ? 0.0 : s64 ? (double)c[13] * n_lblocks * 8.0
-                                     : WING_BF16 ? (double)c[13] * n_lblocks * 1.5 * nb
-                                                 : (double)c[11] * n_lblocks;
+                             : WING_BF16 ? ((double)c[13] - far_entries) * n_lblocks * 1.5 * nb +
+                                               far_entries * n_lblocks * 1.5 * FF_CB +
+                                               (ff ? (double)ntiles * n_lblocks * 6.0 * nb : 0.0)
+                                         : (double)c[11] * n_lblocks;
+    ctx->stats.n_far_entries = far_entries;
     ctx->stats.n_evals_tail = f32 ? (double)c[12] : 0.0;
     ctx->stats.n_evals_near_A = f32 ? (double)c[15] : 0.0;
     return KORG_OK;
@@ -4525,7 +5060,8 @@ int finish_stats(korg_ctx* ctx) {
     HIP_TRY(ctx, hipMemcpy(c, ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
     const int rc = stats_from_counters(ctx, c, ctx->last_precision != KORG_PRECISION_FP64,
                                        ctx->last_n_lblocks, ctx->last_nb,
-                                       ctx->last_precision == KORG_PRECISION_FP64_SERIES);
+                                       ctx->last_precision == KORG_PRECISION_FP64_SERIES,
+                                       ctx->last_ff, ctx->last_ntiles);
     if (rc == KORG_ERR_CAPACITY && ctx->last_fixed)
         return set_err(ctx, KORG_ERR_CAPACITY,
                        "the per-tile line lists need %llu entries, more than the %lld the fixed "
@@ -4538,6 +5074,7 @@ int finish_stats(korg_ctx* ctx) {
     ctx->stats.setup_ms = elapsed(ctx, 0, 1);
     ctx->stats.bin_ms = elapsed(ctx, 1, 2) - gap;
     ctx->stats.profile_ms = elapsed(ctx, 2, 4);
+    ctx->stats.far_ms = ctx->last_ff ? elapsed(ctx, 2, 9) : 0.0;
     ctx->stats.core_ms = ctx->last_overlap ? elapsed(ctx, 7, 8) : elapsed(ctx, 4, 3);
     ctx->stats.total_ms = elapsed(ctx, 0, 3) - gap;
     return KORG_OK;
@@ -5016,6 +5553,8 @@ int run_fit(korg_ctx* ctx, std::vector<StarHost>& hs, const korg_species* sp, in
     ctx->last_precision = precision;
     ctx->last_n_lblocks = b.n_lblocks;
     ctx->last_nb = b.NB;
+    ctx->last_ff = b.ff;
+    ctx->last_ntiles = b.ntiles;
     ctx->last_sized = false;
     TRY(prepare_fit(ctx, ws, b));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
@@ -5030,7 +5569,8 @@ int finish_fit_stats(korg_ctx* ctx) {
     unsigned long long c[N_COUNTERS];
     HIP_TRY(ctx, hipMemcpy(c, ctx->ws.counters.p, sizeof c, hipMemcpyDeviceToHost));
     TRY(stats_from_counters(ctx, c, ctx->last_precision == KORG_PRECISION_FP32_FIXUP,
-                            ctx->last_n_lblocks, ctx->last_nb));
+                            ctx->last_n_lblocks, ctx->last_nb, false, ctx->last_ff,
+                            ctx->last_ntiles));
     ctx->stats.total_ms = elapsed(ctx, 0, 3);
     return KORG_OK;
 }
@@ -5300,7 +5840,8 @@ int plan_wait(korg_plan* p) {
         HIP_TRY(ctx, hipEventSynchronize(p->ev1));
         p->pending = false;
     }
-    TRY(stats_from_counters(ctx, p->h_counters, p->b.f32, p->b.n_lblocks, p->b.NB));
+    TRY(stats_from_counters(ctx, p->h_counters, p->b.f32, p->b.n_lblocks, p->b.NB, false, p->b.ff,
+                            p->b.ntiles));
     float t = 0;
     if (hipEventElapsedTime(&t, p->ev0, p->ev1) == hipSuccess) ctx->stats.total_ms = t;
     else (void)hipGetLastError();

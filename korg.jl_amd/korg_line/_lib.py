@@ -62,7 +62,8 @@ class KorgStats(ctypes.Structure):
                 ("profile_ms", ctypes.c_double), ("core_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("n_evals_branch", ctypes.c_double * 4),
                 ("n_evals_near", ctypes.c_double), ("n_wing_mfma", ctypes.c_double),
-                ("n_evals_tail", ctypes.c_double), ("n_evals_near_A", ctypes.c_double)]
+                ("n_evals_tail", ctypes.c_double), ("n_evals_near_A", ctypes.c_double),
+                ("far_ms", ctypes.c_double), ("n_far_entries", ctypes.c_double)]
 
 
 class KorgLayerGrads(ctypes.Structure):
@@ -100,7 +101,8 @@ class KorgTuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int32), ("waves_per_tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("count_branches", ctypes.c_int32),
                 ("tile_points", ctypes.c_int32), ("near_waves_per_cu", ctypes.c_int32),
-                ("near_layout", ctypes.c_int32), ("near_concurrent", ctypes.c_int32)]
+                ("near_layout", ctypes.c_int32), ("near_concurrent", ctypes.c_int32),
+                ("far_field", ctypes.c_int32)]
 
 
 class KorgFitTarget(ctypes.Structure):
@@ -142,7 +144,7 @@ class KorgHLayers(ctypes.Structure):
                 ("n_HeI", _c_dp), ("U_HI", _c_dp), ("xi", _c_dp)]
 
 
-ABI_VERSION = 16  # KORG_LINE_ABI_VERSION of include/korg_line.h
+ABI_VERSION = 17  # KORG_LINE_ABI_VERSION of include/korg_line.h
 
 EXPORTED = ["korg_create", "korg_destroy", "korg_last_error", "korg_abi_version",
             "korg_line_absorption", "korg_line_absorption_device", "korg_set_stream",

@@ -279,10 +279,12 @@ class Context:
         self.close()
 
     def set_tuning(self, points_per_lane=0, waves_per_cu=0, count_branches=0, tile_points=0,
-                   waves_per_tile=0, near_waves_per_cu=0, near_layout=0, near_concurrent=0):
+                   waves_per_tile=0, near_waves_per_cu=0, near_layout=0, near_concurrent=0,
+                   far_field=0):
         """Kernel tuning (0 = library default); see ``korg_tuning`` in include/korg_line.h."""
         t = _lib.KorgTuning(points_per_lane, waves_per_tile, waves_per_cu, count_branches,
-                            tile_points, near_waves_per_cu, near_layout, near_concurrent)
+                            tile_points, near_waves_per_cu, near_layout, near_concurrent,
+                            far_field)
         _lib.check(self._h, self._L.korg_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self):

@@ -279,29 +279,45 @@ def test_solar_wide_full_size_properties(ctx):
 
 def test_work_counts(ctx):
     """Near-zone, tail and far-wing counts of the Float32 form: the near zone is part of the
-    windows, tail evaluations are part of the far zone, and the far-wing bfloat16 MFMAs (three
-    split products per group of 4 lines, 32-point block and 32-layer half: 1.5 NB per line-tile
-    entry and 64-layer block, for tiles of 32 NB points) cover every far-zone evaluation, each
-    MFMA triple covering 4 lines x 32 points x 32 layers.  The Float64 form reports none of them."""
+    windows, tail evaluations are part of the far zone.  With the far field off every far-zone
+    evaluation is a term of the far-wing bfloat16 MFMAs (three split products per group of 4 lines,
+    32-point block and 32-layer half: 1.5 NB per line-tile entry and 64-layer block, for tiles of
+    32 NB points), each MFMA triple covering 4 lines x 32 points x 32 layers.  With it on (the
+    default) part of the pairs become far-field list entries, far fewer MFMAs are issued, and the
+    evaluation counts are the same.  The Float64 form reports none of them."""
     P, _, _ = workloads.solar_small()
-    gpu_alpha(ctx, P, "fp32")
-    s = ctx.stats()
-    far = s["n_evals"] - s["n_evals_near"]
-    assert 0 < s["n_evals_near"] < s["n_evals"]
-    assert s["n_evals_near"] % P.n_layers == 0
-    assert 0 < s["n_evals_tail"] <= far and s["n_evals_tail"] % P.n_layers == 0
-    n_lblocks = -(-P.n_layers // 64)
-    entries_nb = 2 * s["n_wing_mfma"] / (3 * n_lblocks)  # line-tile entries x NB
-    assert entries_nb == round(entries_nb) and entries_nb > 0
-    assert 4096 * s["n_wing_mfma"] / 3 >= far
-    # a line with a W-point window meets at most W/(32 NB) + 2 tiles, each costing 1.5 NB
-    # instructions per 64-layer block; NB <= 4
-    lb, ub = ctx.windows(len(P.lines))
-    w = (ub - lb + 1)[ub >= lb]
-    assert s["n_wing_mfma"] <= 1.5 * n_lblocks * np.sum(w / 32 + 8)
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(**{**saved, "far_field": 1})
+        a_off = gpu_alpha(ctx, P, "fp32")
+        s = ctx.stats()
+        far = s["n_evals"] - s["n_evals_near"]
+        assert 0 < s["n_evals_near"] < s["n_evals"]
+        assert s["n_evals_near"] % P.n_layers == 0
+        assert 0 < s["n_evals_tail"] <= far and s["n_evals_tail"] % P.n_layers == 0
+        assert s["n_far_entries"] == 0
+        n_lblocks = -(-P.n_layers // 64)
+        entries_nb = 2 * s["n_wing_mfma"] / (3 * n_lblocks)  # line-tile entries x NB
+        assert entries_nb == round(entries_nb) and entries_nb > 0
+        assert 4096 * s["n_wing_mfma"] / 3 >= far
+        # a line with a W-point window meets at most W/(32 NB) + 2 tiles, each costing 1.5 NB
+        # instructions per 64-layer block; NB <= 4
+        lb, ub = ctx.windows(len(P.lines))
+        w = (ub - lb + 1)[ub >= lb]
+        assert s["n_wing_mfma"] <= 1.5 * n_lblocks * np.sum(w / 32 + 8)
+        ctx.set_tuning(**{**saved, "far_field": 2})
+        a_on = gpu_alpha(ctx, P, "fp32")
+        s2 = ctx.stats()
+        assert s2["n_far_entries"] > 0 and s2["n_wing_mfma"] < s["n_wing_mfma"]
+        for k in ("n_evals", "n_evals_near", "n_evals_tail"):
+            assert s2[k] == s[k], k
+        assert rel_err(a_on, a_off) <= FP32_RTOL
+    finally:
+        ctx.set_tuning(**saved)
     gpu_alpha(ctx, P, "fp64")
     s = ctx.stats()
     assert s["n_evals_near"] == 0 and s["n_wing_mfma"] == 0 and s["n_evals_tail"] == 0
+    assert s["n_far_entries"] == 0
 
 
 def test_invalid_species_index_rejected(ctx):

@@ -155,7 +155,7 @@ def test_library_exports_declared_symbols():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.korg_abi_version() == _lib.ABI_VERSION == 16
+    assert L.korg_abi_version() == _lib.ABI_VERSION == 17
 
 
 def test_struct_layouts_match_header(tmp_path):
