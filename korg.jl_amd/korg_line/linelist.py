@@ -11,6 +11,7 @@ the large synthetic lists are built.
 """
 import math
 import os
+import warnings
 
 import numpy as np
 
@@ -470,14 +471,22 @@ def _or_zero(s):
     return float(s) if s else 0.0
 
 
-def read_kurucz_linelist(path, vacuum=False):
+_KURUCZ_COLUMN = object()  # the default of read_kurucz_linelist's isotopic_abundances
+
+
+def read_kurucz_linelist(path, vacuum=False, isotopic_abundances=_KURUCZ_COLUMN):
     """An atomic Kurucz linelist (160-column rows) as ``read_linelist(...; format="kurucz")`` reads
     it (``parse_kurucz_linelist``, ``src/linelist.jl:451-511``, then the common filter and sort):
     wavelengths in nm (air, converted to vacuum unless ``vacuum``), E_lower the smaller of the two
     levels' |wavenumber|, log gf plus the hyperfine column, gamma_rad and gamma_stark as log10 and
-    vdW as given (0 = missing).  No isotopic-abundance table is part of this library, so Kurucz's
-    own isotope column is added to log gf (Korg's ``isotopic_abundances = nothing`` path; with its
-    default table Korg recomputes that adjustment for lines that name an isotope).
+    vdW as given (0 = missing).
+
+    Isotopes, as Korg's ``isotopic_abundances`` argument: ``None`` adds Kurucz's own isotope
+    column to log gf (Korg's ``isotopic_abundances = nothing``); a table ``{Z: {A: fraction}}``
+    adds log10 of the fraction for rows that name an isotope (columns 107-109, else 116-118) found
+    in it and nothing for the others (Korg's rule with its table, whose NIST values this library
+    does not ship).  Left at its default the reader adds Kurucz's column and warns when a row names
+    an isotope, since Korg's default read differs there.
     Molecular Kurucz files (first row of 100 columns or fewer) are not supported.
     """
     with open(path) as f:
@@ -486,6 +495,7 @@ def read_kurucz_linelist(path, vacuum=False):
     if len(first) <= 100:
         raise ValueError(f"{path}: molecular Kurucz linelists are not supported")
     wl, loggf, E, grad, gstark, vdw, names = [], [], [], [], [], [], []
+    n_tagged = 0
     for row in rows:
         if row == "":
             continue
@@ -496,7 +506,16 @@ def read_kurucz_linelist(path, vacuum=False):
         e2 = abs(float(row[52:64])) * c_cgs * hplanck_eV
         E.append(min(e1, e2))
         names.append(row[18:24])
-        loggf.append(float(row[11:18]) + _or_zero(row[109:115]) + _or_zero(row[118:124]))
+        gf = float(row[11:18]) + _or_zero(row[109:115])
+        iso = int(_or_zero(row[106:109])) or int(_or_zero(row[115:118]))
+        if isotopic_abundances is None or isotopic_abundances is _KURUCZ_COLUMN:
+            gf += _or_zero(row[118:124])
+            n_tagged += iso != 0
+        elif iso != 0:
+            frac = isotopic_abundances.get(_atomic_number(row[18:24]), {}).get(iso)
+            if frac is not None:
+                gf += math.log10(frac)
+        loggf.append(gf)
         lam = float(row[0:11]) * 1e-7
         wl.append(lam if vacuum else air_to_vacuum(lam))
         grad.append(_or_zero(row[80:86]))
@@ -513,4 +532,17 @@ def read_kurucz_linelist(path, vacuum=False):
     ll = LineList.from_arrays(np.array(wl), np.array(loggf), species, sidx, np.array(E),
                               gamma_rad=_tentothe_or_missing(grad),
                               gamma_stark=_tentothe_or_missing(gstark), vdW=_id_or_missing(vdw))
+    if n_tagged and isotopic_abundances is _KURUCZ_COLUMN:
+        warnings.warn(f"{path}: {n_tagged} rows name an isotope; their log gf includes Kurucz's "
+                      "isotope column (Korg's isotopic_abundances = nothing), where Korg's default "
+                      "read uses its isotopic-abundance table.  Pass isotopic_abundances=None to "
+                      "accept this, or a table {Z: {A: fraction}}.", stacklevel=2)
     return _finish_text_linelist(ll)
+
+
+def _atomic_number(code):
+    """The atomic number of an atomic species code such as ``"26.01"``."""
+    atoms = Species(code).atoms
+    if len(atoms) != 1:
+        raise ValueError(f"{code!r} is not an atomic species")
+    return int(atoms[0])

@@ -245,9 +245,11 @@ KURUCZ = os.path.join(HERE, "golden", "kurucz")  # the reference's test files (t
 
 @pytest.mark.parametrize("fname", ["gfallvac08oct17.stub.dat", "gfallvac08oct17-missing-col.stub.dat"])
 def test_kurucz_linelist_parsing(fname):
-    """test/linelist.jl:154-166, value by value."""
+    """test/linelist.jl:154-166, value by value (the file's isotope-tagged rows draw the
+    default read's warning; its first line names no isotope)."""
     from korg_line.linelist import read_kurucz_linelist
-    ll = read_kurucz_linelist(os.path.join(KURUCZ, fname))
+    with pytest.warns(UserWarning, match="name an isotope"):
+        ll = read_kurucz_linelist(os.path.join(KURUCZ, fname))
     assert ll.is_sorted() and len(ll) == 987
     L0 = ll.to_lines()[0]
     assert L0.wl == pytest.approx(0.0007234041763337705, rel=1e-14)
@@ -261,10 +263,37 @@ def test_kurucz_linelist_parsing(fname):
 def test_kurucz_short_lines_and_molecular():
     """test/linelist.jl:168-189: stripped rows parse; a molecular list is refused."""
     from korg_line.linelist import read_kurucz_linelist
-    ll = read_kurucz_linelist(os.path.join(KURUCZ, "gfallvac08oct17-short-lines.stub.dat"))
+    ll = read_kurucz_linelist(os.path.join(KURUCZ, "gfallvac08oct17-short-lines.stub.dat"),
+                              isotopic_abundances=None)
     assert len(ll) >= 2
     L0 = ll.to_lines()[0]
     assert L0.wl == pytest.approx(0.0007234041763337705, rel=1e-14)
     assert L0.log_gf == -0.826 and L0.species == Species("Be_II")
     with pytest.raises(ValueError):
         read_kurucz_linelist(os.path.join(KURUCZ, "kurucz_cn.txt"))
+
+
+def test_kurucz_isotope_column_and_table(tmp_path):
+    """src/linelist.jl:478-495: with isotopic_abundances = nothing Kurucz's isotope column is added
+    to log gf; with a table, log10 of the named isotope's fraction (nothing when the table lacks
+    it).  The default adds the column and warns when a row names an isotope."""
+    import warnings
+    from korg_line.linelist import read_kurucz_linelist
+    row = open(os.path.join(KURUCZ, "gfallvac08oct17.stub.dat")).readline().rstrip("\n")
+    base = float(row[11:18])  # no hyperfine adjustment in this row
+    r = list(row.ljust(160))
+    r[106:109] = " 9"[:3].rjust(3)   # isotope 9 of Be
+    r[118:124] = "-0.100"           # Kurucz's isotope adjustment
+    path = tmp_path / "iso.dat"
+    path.write_text("".join(r) + "\n")
+    with pytest.warns(UserWarning, match="name an isotope"):
+        ll = read_kurucz_linelist(str(path))
+    assert ll.to_lines()[0].log_gf == pytest.approx(base - 0.1, abs=1e-12)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        ll = read_kurucz_linelist(str(path), isotopic_abundances=None)
+        assert ll.to_lines()[0].log_gf == pytest.approx(base - 0.1, abs=1e-12)
+        ll = read_kurucz_linelist(str(path), isotopic_abundances={4: {9: 0.5}})
+        assert ll.to_lines()[0].log_gf == pytest.approx(base + np.log10(0.5), abs=1e-12)
+        ll = read_kurucz_linelist(str(path), isotopic_abundances={4: {7: 0.5}})
+        assert ll.to_lines()[0].log_gf == pytest.approx(base, abs=1e-12)
