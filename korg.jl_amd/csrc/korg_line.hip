@@ -430,6 +430,7 @@ struct SetupArgs {
     const FfGeo* geo;
     int64_t n_sub;
     int4* ffrun;
+    int4* ffrun_adj;  // Float32 fit step: the runs clear of the adjoint's near zone (null: none)
 };
 
 constexpr double INV_SQRT_PI = 0.5641895835477563;
@@ -725,23 +726,24 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             if (near_wing > 0) atomicAdd(&A.counters[15], (unsigned long long)near_wing);
         }
     }
-    int4 run = make_int4(1, 0, 1, 0);
-    if (A.ffrun && lb <= ub && S.ff_irregular == 0 && gi.step > 0.0) {
-        // The far-field runs, in local sub-tile indices q (points 64q .. 64q + 63, whole
-        // sub-tiles only): inside the window, clear of the near zone, and with the sub-tile's
-        // centre 64q + 31.5 at least 8 * 31.5 + 1 points from the line centre x0 (the grid is
-        // evenly spaced to 1e-3 of a step, so the expansion's ratio stays above 8 * 252/253).
-        // The lower run starts at the first whole sub-tile of the window, the upper one ends at
-        // the last; an empty run has its end before its start.
+    // The far-field runs, in local sub-tile indices q (points 64q .. 64q + 63, whole sub-tiles
+    // only): inside the window, clear of the near zone [zlo, zhi], and with the sub-tile's centre
+    // 64q + 31.5 at least 8 * 31.5 + 1 points from the line centre x0 (the grid is evenly spaced
+    // to 1e-3 of a step, so the expansion's ratio stays above 8 * 252/253).  The lower run starts
+    // at the first whole sub-tile of the window, the upper one ends at the last; an empty run has
+    // its end before its start.  The forward's runs avoid its near zone, the adjoint's (Float32
+    // fit step) the adjoint's wider one.
+    const bool ff_ok = lb <= ub && S.ff_irregular == 0 && gi.step > 0.0;
+    auto ff_runs = [&](int64_t zlo, int64_t zhi) {
         const double x0 = (wl - S.wls[0]) / gi.step;  // the line centre in points
         const int64_t q0 = (lb + FF_SUB - 1) / FF_SUB;                 // first whole sub-tile
         const int64_t q1 = (ub + 1) / FF_SUB - 1;                      // last one
         const double reach = FF_RATIO * 0.5 * (FF_SUB - 1) + 1.0;      // points, centre to centre
         int64_t lo_end = (int64_t)floor((x0 - reach - 0.5 * (FF_SUB - 1)) / FF_SUB);
         int64_t hi_start = (int64_t)ceil((x0 + reach - 0.5 * (FF_SUB - 1)) / FF_SUB);
-        if (clo <= chi) {
-            lo_end = min(lo_end, (int64_t)floor((double)(clo - FF_SUB) / FF_SUB));  // 64q + 63 < clo
-            hi_start = max(hi_start, chi / FF_SUB + 1);                          // 64q > chi
+        if (zlo <= zhi) {
+            lo_end = min(lo_end, (int64_t)floor((double)(zlo - FF_SUB) / FF_SUB));  // 64q + 63 < zlo
+            hi_start = max(hi_start, zhi / FF_SUB + 1);                          // 64q > zhi
         }
         lo_end = min(lo_end, q1);
         hi_start = max(hi_start, max(q0, lo_end + 1));
@@ -749,8 +751,10 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         lo_end = min(lo_end, q0 + 0xFFFE);
         hi_start = max(hi_start, q1 - 0xFFFE);
         const int64_t gq = S.grid_off / FF_SUB;
-        run = make_int4((int)(q0 + gq), (int)(lo_end + gq), (int)(hi_start + gq), (int)(q1 + gq));
-    }
+        return make_int4((int)(q0 + gq), (int)(lo_end + gq), (int)(hi_start + gq), (int)(q1 + gq));
+    };
+    const int4 run = A.ffrun && ff_ok ? ff_runs(clo, chi) : make_int4(1, 0, 1, 0);
+    const int4 run_adj = F32 && A.ffrun_adj && ff_ok ? ff_runs(alo, ahi) : make_int4(1, 0, 1, 0);
     int cls = N_CLASSES;
     if (lb <= ub) {
         const int64_t W = ub - lb + 1;
@@ -773,6 +777,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         A.meta[i] = mt;
         if (F32 && A.azone) A.azone[i] = make_int2((int)(alo + go), (int)(ahi + go));
         if (A.ffrun) A.ffrun[i] = run;
+        if (F32 && A.ffrun_adj) A.ffrun_adj[i] = run_adj;
     }
 }
 
@@ -3391,6 +3396,7 @@ constexpr int MOM_PARTS = 8;
 constexpr int MOM_BLOCK = 4;
 constexpr int MOM_CHUNK = 16;
 constexpr int MOM_PAD = 4 * MOM_BLOCK;
+static_assert(4 * MOM_CHUNK == FF_SUB, "a chunk of the moments is one far-field sub-tile");
 
 // G in Float32, scaled by 2^-e per (star, layer), as [layer block][global point][64 layers] with
 // MOM_PAD points after the last (layers past L and points outside every star zero): the moments'
@@ -3426,6 +3432,50 @@ __global__ __launch_bounds__(256) void k_g_pack(int n_stars, int L, int n_lblock
     }
 }
 
+// The far field's moments of G: per sub-tile g and layer, g_m = sum over the sub-tile's points of
+// G[layer, j] 2^-e tau_j^m (tau_j = (lambda_j - c) / h, m < FF_TERMS, e the layer's scale exponent
+// of k_g_exp), summed in Float64 and stored in Float32 as [layer block][sub-tile][m][64 layers]:
+// the A operand of the moments' far-field steps (over a sub-tile in a line's far field,
+// sum_j G_j R_j^k = sum_m W_km g_m).  One wavefront per (sub-tile, layer block), lanes = layers;
+// sub-tiles with no points (padding) get zeros.
+__global__ __launch_bounds__(WAVE) void k_g_ffmom(int n_stars, int L, int n_lblocks, int64_t n_sub,
+                                                  const StarDev* __restrict__ stars,
+                                                  const FfGeo* __restrict__ geo,
+                                                  const int* __restrict__ gexp,
+                                                  float* __restrict__ gff) {
+    const int lane = threadIdx.x;
+    const int64_t item = blockIdx.x;
+    const int64_t g = item / n_lblocks;
+    const int lblk = (int)(item % n_lblocks);
+    if (g >= n_sub) return;
+    const FfGeo G = geo[g];
+    const int l = lblk * WAVE + lane;
+    double acc[FF_TERMS];
+#pragma unroll
+    for (int m = 0; m < FF_TERMS; ++m) acc[m] = 0.0;
+    int e = 0;
+    if (G.first <= G.last) {
+        const int st = star_of_point(stars, n_stars, G.first);
+        const StarDev& S = stars[st];
+        const double* __restrict__ Gs = (const double*)S.G;
+        e = l < L ? gexp[(int64_t)st * L + l] : 0;
+        const double h = (double)G.h_u;
+        for (int64_t j = G.first; j <= G.last; ++j) {
+            const int64_t jl = j - S.grid_off;
+            const double tau = h > 0.0 ? (S.wls[jl] * (1.0 / DELTA_UNIT) - G.c_u) / h : 0.0;
+            double p = l < L ? Gs[l + (int64_t)L * jl] : 0.0;
+#pragma unroll
+            for (int m = 0; m < FF_TERMS; ++m) {
+                acc[m] += p;
+                p *= tau;
+            }
+        }
+    }
+    float* __restrict__ dst = gff + ((int64_t)lblk * n_sub + g) * FF_TERMS * WAVE + lane;
+#pragma unroll
+    for (int m = 0; m < FF_TERMS; ++m) dst[m * WAVE] = l < L ? (float)ldexp(acc[m], -e) : 0.0f;
+}
+
 struct MomArgs {
     int n_stars, L, n_lblocks;
     int64_t n_total;           // global grid points
@@ -3440,6 +3490,12 @@ struct MomArgs {
     const double* lam_s;
     float4* mom;               // [(line * n_lblocks + b) * 64 + l % 64]: (M1, M2, M3, M4) / 2^e
     unsigned long long* counters;
+    // far field (null ffrun: none): each line's runs clear of its adjoint near zone, the sub-tile
+    // geometry and the sub-tiles' moments of G (k_g_ffmom)
+    const int4* ffrun;
+    const FfGeo* geo;
+    const float* gff;
+    int64_t n_sub;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -3447,15 +3503,24 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // One wavefront per (group of ADJ_GROUP lines, 64-layer block), widest classes first.  Lane i of a
 // K step: A = scaled G of layers 4 (i % 16) + m (m = 0..3, one MFMA each) at point j0 + i / 16;
 // B = R^(p+1) of line slot (i % 16) / 4 at that point, p = i % 4 (zero outside the line's far
-// zone: its window less its adjoint near zone).  The lines of a group are handled star by star (a group can straddle two stars), each
-// pass over the union of its lines' windows.  The wavefront's 52 VGPRs and 16 AGPRs allow 8 waves
-// per SIMD, which hide the L2 latency of the G and wavelength loads.
+// zone: its window less its adjoint near zone).  The lines of a group are handled star by star (a
+// group can straddle two stars), each pass over the union of its lines' windows, then (far field)
+// over the union of their far-field runs.  About 105 VGPRs and 20 AGPRs: four waves per SIMD.
 __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
     const int lane = threadIdx.x;
     if (A.counters[3] != 0) return;  // failed setup, or a capacity shortfall
     const int64_t n_items = (int64_t)A.n_groups[0] * A.n_lblocks;
     const int col = lane & 15, kk = lane >> 4;
     const int slot = col >> 2, pw = col & 3;
+    // far-field steps: binom(-2k, m) of the lane's power k = pw + 1 and terms m = 4t + kk
+    float bc[FF_TERMS / 4];
+#pragma unroll
+    for (int t = 0; t < FF_TERMS / 4; ++t) {
+        double b = 1.0;
+        for (int i = 1; i <= 4 * t + kk; ++i) b *= -(double)(2 * (pw + 1) + i - 1) / (double)i;
+        bc[t] = (float)b;
+    }
+    const KCONST FfGeo* geo = uniform_ptr(A.geo);
     // The item list in MOM_PARTS contiguous parts, part x first taken by the waves of
     // workgroups x, x + MOM_PARTS, ... (workgroups go to the XCDs round robin, so the waves of
     // one XCD share one L2 and work on nearby points); a wave whose part is used up moves on to
@@ -3479,11 +3544,13 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
         int line = -1, st = -1;
         LineMeta m = {0.0, 1, 0, 1, 0, 0, ADJ_NC, 1, 0};
         int2 z = make_int2(1, 0);
+        int4 fr = make_int4(1, 0, 1, 0);  // the line's far-field runs (empty: none)
         if (slot < nl) {
             line = A.adj_line[p0 + slot];
             m = A.meta[line];
             z = A.azone[line];
             st = star_of_line(A.stars, A.n_stars, line);
+            if (A.ffrun) fr = A.ffrun[line];
         }
         uint32_t todo = (1u << nl) - 1;  // line slots not yet handled
         while (todo) {
@@ -3503,10 +3570,10 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
             // (the lane's point is j0 + kk); not in this pass: no points
             const int lbr = (act ? m.lb : 1) - kk, ubr = (act ? m.ub : 0) - kk;
             const int clor = z.x - kk, chir = z.y - kk;
-            // Float32 sums over chunks of MOM_CHUNK K steps, added into Float64 totals: a far
-            // zone spans thousands of points and its moments enter differences (core against
-            // wing) in the gradients of the damping.  Chunks end at fixed points (multiples of
-            // 4 MOM_CHUNK), so a line's sums do not depend on its group either.
+            // Float32 sums over chunks of MOM_CHUNK K steps (one 64-point sub-tile), added into
+            // Float64 totals: a far zone spans thousands of points and its moments enter
+            // differences (core against wing) in the gradients of the damping.  Chunks end at fixed
+            // points, so a line's sums do not depend on its group either.
             floatx4 acc[4];
             double tot[4][4];
 #pragma unroll
@@ -3515,40 +3582,109 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) tot[q][r] = 0.0;
             }
+            // the pass's lines' far-field runs, wavefront-uniform (lane 4k holds slot k; lines not
+            // in the pass have none here)
+            int rx[ADJ_GROUP], ry[ADJ_GROUP], rz[ADJ_GROUP], rw[ADJ_GROUP];
+#pragma unroll
+            for (int k = 0; k < ADJ_GROUP; ++k) {
+                const bool in = k < nl && ((pass >> k) & 1);
+                rx[k] = in ? readlane_i32(fr.x, 4 * k) : 1;
+                ry[k] = in ? readlane_i32(fr.y, 4 * k) : 0;
+                rz[k] = in ? readlane_i32(fr.z, 4 * k) : 1;
+                rw[k] = in ? readlane_i32(fr.w, 4 * k) : 0;
+            }
+            const int4 lfr = act ? fr : make_int4(1, 0, 1, 0);
             // K steps start at multiples of 4 (a star's grid starts at one): every point falls in
             // the same step whatever lines share the group, and steps outside a line's far zone add
             // exact zeros, so a line's moments do not depend on its group.  Steps go in blocks of
-            // MOM_BLOCK, the block's loads issued before its MFMAs; the last block reads at most
-            // 4 MOM_BLOCK - 1 points past the union (B = 0 there; zeros past the grid).
+            // MOM_BLOCK, the block's loads issued before its MFMAs, and blocks in chunks of one
+            // 64-point sub-tile, whose Float32 sums are added into the Float64 totals at its end
+            // (a block reads at most 4 MOM_BLOCK - 1 points past the union: B = 0 there).  Points in
+            // a sub-tile of a line's far-field runs are left to the far-field steps below; a
+            // sub-tile in the runs of every line of the pass is skipped (it adds only zeros).
             const int js = u0 & ~(4 * MOM_BLOCK - 1);
-            const float4* __restrict__ gp =
-                (const float4*)(A.g32 + ((int64_t)lblk * (A.n_total + MOM_PAD) + js) * WAVE);
-            const double* __restrict__ lp = A.lam_s + js;
-            for (int jb = js; jb <= u1; jb += 4 * MOM_BLOCK, gp += MOM_BLOCK * WAVE, lp += 4 * MOM_BLOCK) {
-                // A: layers 4 col .. 4 col + 3 at point j0 + kk; the wavelength of that point
-                float4 a[MOM_BLOCK];
-                double lam[MOM_BLOCK];
+            for (int gs = js / FF_SUB; gs <= u1 / FF_SUB; ++gs) {
+                bool all = A.ffrun != nullptr;
 #pragma unroll
-                for (int u = 0; u < MOM_BLOCK; ++u) {
-                    a[u] = gp[u * WAVE + lane];
-                    lam[u] = lp[4 * u + kk];
+                for (int k = 0; k < ADJ_GROUP; ++k)
+                    if (k < nl && ((pass >> k) & 1))
+                        all = all && ((gs >= rx[k] && gs <= ry[k]) || (gs >= rz[k] && gs <= rw[k]));
+                if (all) continue;  // wavefront-uniform
+                const bool lane_ff = ff_in(lfr, gs);
+                const int jhi = min(gs * FF_SUB + FF_SUB - 4 * MOM_BLOCK, u1);
+                for (int jb = max(gs * FF_SUB, js); jb <= jhi; jb += 4 * MOM_BLOCK) {
+                    const float4* __restrict__ gp =
+                        (const float4*)(A.g32 + ((int64_t)lblk * (A.n_total + MOM_PAD) + jb) * WAVE);
+                    const double* __restrict__ lp = A.lam_s + jb;
+                    // A: layers 4 col .. 4 col + 3 at point j0 + kk; the wavelength of that point
+                    float4 a[MOM_BLOCK];
+                    double lam[MOM_BLOCK];
+#pragma unroll
+                    for (int u = 0; u < MOM_BLOCK; ++u) {
+                        a[u] = gp[u * WAVE + lane];
+                        lam[u] = lp[4 * u + kk];
+                    }
+#pragma unroll
+                    for (int u = 0; u < MOM_BLOCK; ++u) {
+                        const int j0 = jb + 4 * u;
+                        const float d = (float)(lam[u] - lam0s);
+                        float R = __builtin_amdgcn_rcpf(d * d);
+                        const bool far = j0 >= lbr && j0 <= ubr && (j0 < clor || j0 > chir) && !lane_ff;
+                        R = far ? R : 0.0f;
+                        const float R2 = R * R;
+                        const float b = ((pw & 1) ? R2 : R) * ((pw & 2) ? R2 : 1.0f);
+                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b, acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b, acc[1], 0, 0, 0);
+                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, b, acc[2], 0, 0, 0);
+                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, b, acc[3], 0, 0, 0);
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < MOM_BLOCK; ++u) {
-                    const int j0 = jb + 4 * u;
-                    const float d = (float)(lam[u] - lam0s);
-                    float R = __builtin_amdgcn_rcpf(d * d);
-                    const bool far = j0 >= lbr && j0 <= ubr && (j0 < clor || j0 > chir);
-                    R = far ? R : 0.0f;
-                    const float R2 = R * R;
-                    const float b = ((pw & 1) ? R2 : R) * ((pw & 2) ? R2 : 1.0f);
-                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b, acc[0], 0, 0, 0);
-                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b, acc[1], 0, 0, 0);
-                    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, b, acc[2], 0, 0, 0);
-                    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, b, acc[3], 0, 0, 0);
+                for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[q][r] += (double)acc[q][r];
+                    acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
-                // chunk ends (MOM_CHUNK steps) fall on block ends
-                if (((jb >> 2) & (MOM_CHUNK - 1)) == MOM_CHUNK - MOM_BLOCK) {
+            }
+            if (A.ffrun) {
+                // The far-field steps, sub-tile by sub-tile in ascending order (after every direct
+                // chunk: a line's Float64 totals add the same values in the same order in any
+                // group): over a sub-tile g in the line's runs, sum_j G_j R_j^k = sum_m W_km g_m with
+                // W_km = binom(-2k, m) u^m R0^k, R0 = 1/d^2, u = h/d, d = c - lam0; K = 4 terms
+                // per step, A = the sub-tile's moments of G (k_g_ffmom), three steps per sub-tile.
+                int glo = INT_MAX, ghi = INT_MIN;
+#pragma unroll
+                for (int k = 0; k < ADJ_GROUP; ++k) {
+                    if (rx[k] <= ry[k]) { glo = min(glo, rx[k]); ghi = max(ghi, ry[k]); }
+                    if (rz[k] <= rw[k]) { glo = min(glo, rz[k]); ghi = max(ghi, rw[k]); }
+                }
+                for (int gs = glo; gs <= ghi; ++gs) {
+                    bool any = false;
+#pragma unroll
+                    for (int k = 0; k < ADJ_GROUP; ++k)
+                        any = any || (gs >= rx[k] && gs <= ry[k]) || (gs >= rz[k] && gs <= rw[k]);
+                    if (!any) continue;  // wavefront-uniform
+                    const double cu = geo[gs].c_u;
+                    const float hu = geo[gs].h_u;
+                    float x = __builtin_amdgcn_rcpf((float)(cu - lam0s));
+                    x = ff_in(lfr, gs) ? x : 0.0f;
+                    const float R0 = x * x, R2 = R0 * R0;
+                    const float Rk = ((pw & 1) ? R2 : R0) * ((pw & 2) ? R2 : 1.0f);
+                    const float u = hu * x, u2 = u * u, u4 = u2 * u2;
+                    float w = Rk * ((kk & 1) ? u : 1.0f) * ((kk & 2) ? u2 : 1.0f);
+                    const float4* __restrict__ gq =
+                        (const float4*)(A.gff + ((int64_t)lblk * A.n_sub + gs) * FF_TERMS * WAVE);
+                    // (acc is zero here: every direct chunk ends by clearing it)
+#pragma unroll
+                    for (int t = 0; t < FF_TERMS / 4; ++t) {
+                        const float4 a4 = gq[t * WAVE + lane];  // terms 4t + kk, layers 4 col ..
+                        const float bw = bc[t] * w;
+                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, bw, acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y, bw, acc[1], 0, 0, 0);
+                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z, bw, acc[2], 0, 0, 0);
+                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, bw, acc[3], 0, 0, 0);
+                        w *= u4;
+                    }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
 #pragma unroll
@@ -3557,10 +3693,6 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
                     }
                 }
             }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tot[q][r] += (double)acc[q][r];
             if (act) {
                 // accumulator layout of v_mfma_f32_16x16x4_f32: lane i, register r holds row
                 // 4 (i / 16) + r, column i % 16; row i of MFMA m is layer 4 i + m of the block
@@ -4177,6 +4309,7 @@ struct Workspace {
     DevBuf nbuf;  // the near zone's sums when it runs beside the far wing
     DevBuf ffgeo, fft;  // far field: sub-tile geometry, the sums T of every sub-tile
     DevBuf ffrun, ffkey;  // far field: each line's runs (line order), list keys (partition order)
+    DevBuf ffrun_adj, gff;  // the adjoint's far field: runs clear of its near zone, moments of G
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -4193,7 +4326,8 @@ struct Workspace {
                           &cstart,   &cls_cand,  &cls_jc,    &ranges,     &nblk,      &ncstart,
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
-                          &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey};
+                          &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey,
+                          &ffrun_adj, &gff};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -4207,7 +4341,7 @@ struct Workspace {
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
                           &scan_part, &jvp_coef, &nbuf,      &ffgeo,     &fft,       &ffrun,
-                          &ffkey};
+                          &ffkey,     &ffrun_adj, &gff};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -4659,6 +4793,7 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.geo = b.ff ? (const FfGeo*)ws.ffgeo.p : nullptr;
     sa.n_sub = b.n_sub;
     sa.ffrun = b.ff ? (int4*)ws.ffrun.p : nullptr;
+    sa.ffrun_adj = b.adj && b.ff ? (int4*)ws.ffrun_adj.p : nullptr;
     if (b.ff)  // the sub-tiles' geometry, which the setup's far-field runs read
         hipLaunchKernelGGL(k_ff_geo, dim3((unsigned)((b.n_sub + 255) / 256)), dim3(256), 0, s,
                            b.n_sub, b.n_stars, (StarDev*)ws.stars.p, (FfGeo*)ws.ffgeo.p);
@@ -5246,6 +5381,11 @@ int prepare_fit(korg_ctx* ctx, Workspace& ws, Batch& b) {
         TRY(grow(ctx, ws.adj_gstart, sizeof(int) * 2));
         TRY(grow(ctx, ws.g32, sizeof(float) * WAVE * (size_t)b.n_lblocks * (size_t)(b.nwl + MOM_PAD)));
         TRY(grow(ctx, ws.lam_s, sizeof(double) * (size_t)(b.nwl + MOM_PAD)));
+        if (b.ff) {
+            TRY(grow(ctx, ws.ffrun_adj, sizeof(int4) * (size_t)(b.N > 0 ? b.N : 1)));
+            TRY(grow(ctx, ws.gff, sizeof(float) * WAVE * FF_TERMS * (size_t)b.n_lblocks *
+                                      (size_t)(b.n_sub > 0 ? b.n_sub : 1)));
+        }
     }
     return KORG_OK;
 }
@@ -5260,6 +5400,10 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     hipLaunchKernelGGL(k_g_pack, dim3(gblocks), dim3(256), 0, s, b.n_stars, b.L, b.n_lblocks, b.nwl,
                        (const StarDev*)ws.stars.p, (const int*)ws.gexp.p, (float*)ws.g32.p,
                        (double*)ws.lam_s.p);
+    if (b.ff && b.n_sub > 0)  // the sub-tiles' moments of G for the far-field steps
+        hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.n_sub * b.n_lblocks)), dim3(WAVE), 0, s,
+                           b.n_stars, b.L, b.n_lblocks, b.n_sub, (const StarDev*)ws.stars.p,
+                           (const FfGeo*)ws.ffgeo.p, (const int*)ws.gexp.p, (float*)ws.gff.p);
     hipLaunchKernelGGL(k_adj_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
                        (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (int*)ws.adj_blk.p);
     hipLaunchKernelGGL(k_scan<ADJ_NC>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.adj_blk.p,
@@ -5289,6 +5433,10 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     ma.lam_s = (const double*)ws.lam_s.p;
     ma.mom = (float4*)ws.recM.p;
     ma.counters = (unsigned long long*)ws.counters.p;
+    ma.ffrun = b.ff ? (const int4*)ws.ffrun_adj.p : nullptr;
+    ma.geo = (const FfGeo*)ws.ffgeo.p;
+    ma.gff = (const float*)ws.gff.p;
+    ma.n_sub = b.n_sub;
     hipLaunchKernelGGL(k_vjp_moments, dim3((unsigned)(ctx->n_cu * 32)), dim3(WAVE), 0, s, ma);
     HIP_TRY(ctx, hipGetLastError());
     return KORG_OK;
