@@ -4813,8 +4813,26 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
 // First half of the pipeline on `s`: setup, the class partitions, and (Float32 form) the per-tile
 // list counts and their scan, which flags FLAG_CAPACITY when the lists exceed tile_cap entries.
 // `timed` records ev[0], ev[1], ev[5].
+int launch_near(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, float* nbuf);
+
+// The Float32 near zone on the context's second stream, into ws.nbuf (b.overlap): forked from s
+// once its partition is on s, joined by enqueue_back before its sums are added.  `timed` records
+// ev[7], ev[8].
+int fork_near(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed) {
+    HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, s));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], ctx->side));
+    TRY(launch_near(ctx, ws, b, ctx->side, (float*)ws.nbuf.p));
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[8], ctx->side));
+    HIP_TRY(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+    return KORG_OK;
+}
+
+// early_near: with b.overlap, the near zone is forked as soon as its partition is done (beside the
+// far wing's list building as well as the far wing); the caller then runs enqueue_back with the
+// same flag, which joins it.
 int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int64_t tile_cap,
-                  bool timed) {
+                  bool timed, bool early_near = false) {
     TRY(enqueue_setup(ctx, ws, b, s, timed));
     const int64_t N = b.N;
     auto* counters = (unsigned long long*)ws.counters.p;
@@ -4843,6 +4861,8 @@ int enqueue_front(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, i
         hipLaunchKernelGGL(k_ranges, dim3((b.nt_near * N_CLASSES + 255) / 256), dim3(256), 0, s,
                            b.nt_near, CT, b.nwl, (const int*)ws.ncstart.p,
                            (const int*)ws.near_jc.p, (int2*)ws.near_ranges.p);
+        HIP_TRY(ctx, hipGetLastError());
+        if (early_near && b.overlap) TRY(fork_near(ctx, ws, b, s, timed));
         // exact per-tile line lists of the far-wing kernels (the wing tiles', then with the far
         // field the super-tiles'): count and one scan over both
         const int64_t ntc = (int64_t)b.ntiles * N_CLASSES;
@@ -4906,7 +4926,8 @@ int launch_near(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, flo
 
 // Second half on `s`: (Float32 form) the per-tile list fill, then the profile kernels that add
 // into alpha.  `timed` records ev[6], ev[2], ev[4], ev[3].
-int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed) {
+int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bool timed,
+                 bool early_near = false) {
     auto* counters = (unsigned long long*)ws.counters.p;
     if (b.f32) {
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], s));
@@ -4952,15 +4973,9 @@ int enqueue_back(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, bo
         HIP_TRY(ctx, hipGetLastError());
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     } else {
-        if (b.overlap) {
-            // the near zone on the side stream, into ws.nbuf, while the far wing runs on s
-            HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, s));
-            HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-            if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], ctx->side));
-            TRY(launch_near(ctx, ws, b, ctx->side, (float*)ws.nbuf.p));
-            if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[8], ctx->side));
-            HIP_TRY(ctx, hipEventRecord(ctx->join_ev, ctx->side));
-        }
+        // the near zone on the side stream, into ws.nbuf, while the far wing runs on s (unless
+        // enqueue_front forked it already)
+        if (b.overlap && !early_near) TRY(fork_near(ctx, ws, b, s, timed));
         if (b.ff) {
             // the far-field sums of every super-tile, which the wing's tiles start from
             FarArgs fa;
@@ -5128,12 +5143,12 @@ int run_batch(korg_ctx* ctx, int n_stars, const StarHost* stars, const korg_spec
     ctx->last_sized = b.f32 && !fixed;
     ctx->last_overlap = b.overlap;
     if (b.N == 0) return KORG_OK;
-    TRY(enqueue_front(ctx, ws, b, s, fixed ? ws.tile_cap : INT64_MAX, true));
+    TRY(enqueue_front(ctx, ws, b, s, fixed ? ws.tile_cap : INT64_MAX, true, true));
     if (b.f32 && !fixed) {
         int64_t total = 0;
         TRY(size_tile_lists(ctx, ws, b, s, 1.0, &total));
     }
-    return enqueue_back(ctx, ws, b, s, true);
+    return enqueue_back(ctx, ws, b, s, true, true);
 }
 
 // One line_absorption! call on device pointers: a batch of one star.
@@ -5473,12 +5488,12 @@ int enqueue_fit(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, boo
                        b.alpha_f32 ? 1 : 0);
     HIP_TRY(ctx, hipGetLastError());
     if (b.N > 0) {
-        TRY(enqueue_front(ctx, ws, b, s, sizing ? INT64_MAX : ws.tile_cap, false));
+        TRY(enqueue_front(ctx, ws, b, s, sizing ? INT64_MAX : ws.tile_cap, false, true));
         if (sizing && b.f32) {
             int64_t total = 0;
             TRY(size_tile_lists(ctx, ws, b, s, headroom, &total));
         }
-        TRY(enqueue_back(ctx, ws, b, s, false));
+        TRY(enqueue_back(ctx, ws, b, s, false, true));
     } else {
         hipLaunchKernelGGL(k_layers, dim3(b.n_stars), dim3(64), 0, s, b.n_stars, b.L,
                            (StarDev*)ws.stars.p, (LayerTab*)ws.tab.p,
@@ -5887,8 +5902,8 @@ int plan_capture(korg_plan* p) {
     if (p->fit) {
         rc = enqueue_fit(ctx, p->ws, p->b, p->cap_stream, false, 0.0);
     } else {
-        rc = enqueue_front(ctx, p->ws, p->b, p->cap_stream, p->ws.tile_cap, false);
-        if (rc == KORG_OK) rc = enqueue_back(ctx, p->ws, p->b, p->cap_stream, false);
+        rc = enqueue_front(ctx, p->ws, p->b, p->cap_stream, p->ws.tile_cap, false, true);
+        if (rc == KORG_OK) rc = enqueue_back(ctx, p->ws, p->b, p->cap_stream, false, true);
     }
     if (rc == KORG_OK) {
         const hipError_t e = hipMemcpyAsync(p->h_counters, p->ws.counters.p,
