@@ -1956,7 +1956,7 @@ void k_wing_bf16(ProfileArgs A) {
 // A = the lines' coefficients exactly as k_wing_bf16 reads them, B = their W at 32 (sub-tile,
 // term) columns (zero for a sub-tile whose far field the line does not take), both split into
 // bfloat16 high and low parts (three products).  Column 32 cb + n of an item is sub-tile
-// (32 cb + n) / FF_TERMS, term (32 cb + n) % FF_TERMS.  W = binom(-2k, m) u^m R^k with
+// n % FF_NS, term 4 cb + n / FF_NS.  W = binom(-2k, m) u^m R^k with
 // R = 1/d^2 and u = h/d (|u| <= 1/FF_RATIO), d = c - lam0 in DELTA_UNIT (a Float64 difference,
 // then Float32).  T goes to fft as [super-tile][layer block][sub-tile][64 layers][FF_TERMS].
 // ---------------------------------------------------------------------------------------------
@@ -2002,18 +2002,18 @@ __global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(WW > 
     const bool hi_half = hf != 0;
     const SplitSel sel = split_selectors();
     if (A.counters[3] != 0) return;  // failed setup: nothing is written
-    // the lane's sub-tile and term in each column block, and binom(-2k, m) for k = 1..4
-    int scol[FF_CB], mcol[FF_CB];
+    // Column 32 cb + n of an item is sub-tile n % FF_NS and term 4 cb + n / FF_NS: a lane keeps
+    // one sub-tile in every column block, so its line distances and powers are formed once per
+    // line, and the term steps up by 4 from one block to the next.  binom(-2k, m) for k = 1..4:
+    const int sub = n % FF_NS, m0 = n / FF_NS;
+    static_assert(FF_NS * 4 == 32 && FF_CB * 4 == FF_TERMS, "columns: 8 sub-tiles x 4 terms per block");
     float bk[FF_CB][4];
 #pragma unroll
     for (int cb = 0; cb < FF_CB; ++cb) {
-        const int col = 32 * cb + n;
-        scol[cb] = col / FF_TERMS;
-        mcol[cb] = col % FF_TERMS;
 #pragma unroll
         for (int k = 1; k <= 4; ++k) {
             double b = 1.0;
-            for (int i = 1; i <= mcol[cb]; ++i) b *= -(double)(2 * k + i - 1) / (double)i;
+            for (int i = 1; i <= 4 * cb + m0; ++i) b *= -(double)(2 * k + i - 1) / (double)i;
             bk[cb][k - 1] = (float)b;
         }
     }
@@ -2034,13 +2034,8 @@ __global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(WW > 
             sg[threadIdx.x] = g < A.n_sub ? A.geo[g] : none;
         }
         __syncthreads();
-        double cu[FF_CB];
-        float hu[FF_CB];
-#pragma unroll
-        for (int cb = 0; cb < FF_CB; ++cb) {
-            cu[cb] = sg[scol[cb]].c_u;
-            hu[cb] = sg[scol[cb]].h_u;
-        }
+        const double cu = sg[sub].c_u;  // the lane's sub-tile
+        const float hu = sg[sub].h_u;
         floatx16 acc[2][FF_CB];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -2093,22 +2088,32 @@ __global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(WW > 
                     ah[mb] = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].x, ua[mb].y, ub[mb].x, ub[mb].y));
                     al[mb] = __builtin_bit_cast(bf16x8, make_uint4(ua[mb].z, ua[mb].w, ub[mb].z, ub[mb].w));
                 }
+                // the two lines' 1/d, R^1..R^4 and u^m0 at the lane's sub-tile (zero where the
+                // sub-tile is not in the line's far field), once for all column blocks
+                float xA = __builtin_amdgcn_rcpf((float)(cu - lA));
+                float xB = __builtin_amdgcn_rcpf((float)(cu - lB));
+                xA = ((eA >> sub) & 1u) ? xA : 0.0f;
+                xB = ((eB >> sub) & 1u) ? xB : 0.0f;
+                const float RA = xA * xA, RB = xB * xB;
+                float rA[4] = {RA, RA * RA, 0.0f, 0.0f}, rB[4] = {RB, RB * RB, 0.0f, 0.0f};
+                rA[2] = rA[1] * RA;
+                rA[3] = rA[1] * rA[1];
+                rB[2] = rB[1] * RB;
+                rB[3] = rB[1] * rB[1];
+                const float uA = hu * xA, uB = hu * xB;
+                float pA = pow_small(uA, m0), pB = pow_small(uB, m0);  // u^m, m = 4 cb + m0
+                const float uA2 = uA * uA, uB2 = uB * uB;
+                const float uA4 = uA2 * uA2, uB4 = uB2 * uB2;
 #pragma unroll
                 for (int cb = 0; cb < FF_CB; ++cb) {
                     // one column block at a time: interleaving them costs registers, not time
                     __builtin_amdgcn_sched_barrier(0);
-                    const int sc = scol[cb];
-                    float xA = __builtin_amdgcn_rcpf((float)(cu[cb] - lA));
-                    float xB = __builtin_amdgcn_rcpf((float)(cu[cb] - lB));
-                    xA = ((eA >> sc) & 1u) ? xA : 0.0f;
-                    xB = ((eB >> sc) & 1u) ? xB : 0.0f;
-                    const float RA = xA * xA, RB = xB * xB;
-                    const float pA = pow_small(hu[cb] * xA, mcol[cb]);
-                    const float pB = pow_small(hu[cb] * xB, mcol[cb]);
-                    const float a1 = pA * RA, a2 = a1 * RA, a3 = a2 * RA, a4 = a3 * RA;
-                    const float b1 = pB * RB, b2 = b1 * RB, b3 = b2 * RB, b4 = b3 * RB;
-                    const float x[8] = {bk[cb][0] * a1, bk[cb][1] * a2, bk[cb][2] * a3, bk[cb][3] * a4,
-                                        bk[cb][0] * b1, bk[cb][1] * b2, bk[cb][2] * b3, bk[cb][3] * b4};
+                    const float x[8] = {bk[cb][0] * pA * rA[0], bk[cb][1] * pA * rA[1],
+                                        bk[cb][2] * pA * rA[2], bk[cb][3] * pA * rA[3],
+                                        bk[cb][0] * pB * rB[0], bk[cb][1] * pB * rB[1],
+                                        bk[cb][2] * pB * rB[2], bk[cb][3] * pB * rB[3]};
+                    pA *= uA4;
+                    pB *= uB4;
                     bf16x8 bh, bl;
                     split_bf16(x, sel, bh, bl);
 #pragma unroll
@@ -2161,7 +2166,7 @@ __global__ __launch_bounds__(WAVE * WW) __attribute__((amdgpu_waves_per_eu(WW > 
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             const int layer = 32 * mb + 8 * q + 4 * hf + e;
-                            Tst[(scol[cb] * WAVE + layer) * FF_TERMS + mcol[cb]] = v[e];
+                            Tst[(sub * WAVE + layer) * FF_TERMS + 4 * cb + m0] = v[e];
                         }
                     }
         }
