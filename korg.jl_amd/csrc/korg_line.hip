@@ -2783,6 +2783,154 @@ void k_vjp(VjpArgs A) {
     }
 }
 
+// The series form's near-zone items with lanes over 8 layers x 8 points (blocks of at most 64
+// layers): lane (r, c) = (lane % 8, lane / 8) evaluates layer 8 g + r of each layer group g at
+// points j0 + 8 p + c, so a wavefront's evaluations share their Hunger branch and Harris piece away
+// from the few boundaries (with lanes over layers every point runs every branch its layers take).
+// The line's records go through LDS; each lane sums its points in Float64, the 8 point lanes of a
+// layer are summed by shuffles (a fixed order), and the group's sums land in lane 8 g + r, which
+// then applies the far-zone moments and the chain rule exactly as k_vjp<true> does.
+template <int NG>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_vjp_near(VjpArgs A) {
+    __shared__ double2 s_q[WAVE];
+    const int lane = threadIdx.x;
+    const int r = lane & 7, c = lane >> 3;
+    const int L = A.L;
+    const int NQ = 6 + A.n_species;
+    if (A.counters[3] != 0) return;  // failed setup, or the segments do not fit
+    const int64_t n_segs = A.seg_start[A.n_stars];
+    for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
+        int st = 0;
+        {
+            int lo = 0, hi = A.n_stars;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (A.seg_start[mid] <= seg) lo = mid; else hi = mid;
+            }
+            st = lo;
+        }
+        const KCONST StarDev* stars = uniform_ptr(A.stars);
+        const KCONST StarDev& S = stars[st];
+        const int64_t go = S.grid_off;
+        const int64_t first = A.item_start[S.line_off];
+        const int64_t it0 = first + (seg - A.seg_start[st]) * VJP_SEG;
+        const int64_t it1 = min(it0 + VJP_SEG, A.item_start[S.line_off + S.n_lines]);
+        double* part = A.partial + seg * NQ * L;
+        for (int q = 0; q < NQ; ++q)
+            for (int l = lane; l < L; l += WAVE) part[q * L + l] = 0.0;
+        int64_t i = S.line_off;
+        {
+            int64_t lo = S.line_off, hi = S.line_off + S.n_lines;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) / 2;
+                if (A.item_start[mid] <= it0) lo = mid; else hi = mid;
+            }
+            i = lo;
+        }
+        for (int64_t item = it0; item < it1; ++item) {
+            while (A.item_start[i + 1] <= item) ++i;  // lines with no items are passed over
+            const int64_t li = i - S.line_off;
+            const LineMeta m = A.meta[i];
+            const bool first_item = item == A.item_start[i];
+            const int2 z = A.azone[i];
+            const int64_t j0 = z.x + (item - A.item_start[i]) * VJP_CHUNK;
+            const int64_t j1 = z.x > z.y ? j0 : min(j0 + VJP_CHUNK, (int64_t)z.y + 1);  // exclusive
+            const bool far = first_item && !far_zone_empty(m, z);
+            const double lam0 = m.lam0;
+            const int32_t s = S.species[li];
+            const LinePartialsIn lp = line_partials_in(S, li, lam0, A.mass, A.is_mol);
+            const double4 rown = lane < L ? A.rec64[i * A.Ls + lane] : make_double4(0, 0, 0, 0);
+            // (a wavefront's LDS operations complete in order: the reads below see these writes,
+            // and the previous item's reads were issued before them)
+            s_q[lane] = make_double2(rown.x, rown.y);
+            double mySH = 0.0, mySa = 0.0, mySv = 0.0;  // layer `lane`'s sums
+#pragma unroll 1
+            for (int g = 0; g < NG; ++g) {
+                if (8 * g >= L) break;  // wavefront-uniform
+                const int lg = 8 * g + r;
+                const bool gvld = lg < L;
+                const double2 qq = s_q[lg];
+                double SH = 0.0, Sa = 0.0, Sv = 0.0;
+                constexpr int NPB = 4;  // point steps whose loads are in flight together
+                for (int64_t jb = j0; jb < j1; jb += 8 * NPB) {
+                    double gv[NPB], lv[NPB];
+#pragma unroll
+                    for (int u = 0; u < NPB; ++u) {
+                        const int64_t jl = min(jb + 8 * u + c, j1 - 1) - go;  // clamped: in bounds
+                        const int64_t idx = (int64_t)(gvld ? lg : 0) + (int64_t)L * jl;
+                        gv[u] = A.g_f32 ? (double)((const float*)S.G)[idx] : ((const double*)S.G)[idx];
+                        lv[u] = S.wls[jl];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NPB; ++u) {
+                        if (!gvld || jb + 8 * u + c >= j1) continue;
+                        const double gg = gv[u];
+                        const double v = fabs(lv[u] - lam0) * qq.x;
+                        float Ha, Hv;
+                        const float H = voigt_hjerting_grad_f32(qq.y, v, Ha, Hv);
+                        SH += gg * H;
+                        Sa += gg * Ha;
+                        Sv += gg * (v * Hv);
+                    }
+                }
+                // the 8 point lanes of each layer (lanes r, r + 8, ..., r + 56), summed in a fixed
+                // order; lane 8 g + r keeps layer 8 g + r
+#pragma unroll
+                for (int off = 8; off < WAVE; off <<= 1) {
+                    SH += __shfl_xor(SH, off, WAVE);
+                    Sa += __shfl_xor(Sa, off, WAVE);
+                    Sv += __shfl_xor(Sv, off, WAVE);
+                }
+                if (c == g) {
+                    mySH = SH;
+                    mySa = Sa;
+                    mySv = Sv;
+                }
+            }
+            const int l = lane;
+            if (l < L) {
+                const double inv = rown.x, a = rown.y, scaling = rown.z;
+                double SH = mySH, Sa = mySa, Sv = mySv;
+                if (far) {
+                    // far-zone sums from the moments (as k_vjp<true>)
+                    const float4 mk = A.mom[(i * A.n_lblocks) * WAVE + l];
+                    const int e = A.gexp[(int64_t)st * L + l];
+                    const double w = 1.0 / (inv * DELTA_UNIT);
+                    const double q = w * w;
+                    const double Q1 = ldexp((double)mk.x, e) * q;
+                    const double Q2 = ldexp((double)mk.y, e) * (q * q);
+                    const double Q3 = ldexp((double)mk.z, e) * (q * q * q);
+                    const double Q4 = ldexp((double)mk.w, e) * ((q * q) * (q * q));
+                    double e2 = 1.5, e3 = 3.75, e4 = 0.0, f2 = 1.5, f3 = 3.75, f4 = 0.0;
+                    if (!(a <= 0.2)) {
+                        const double a2 = a * a;
+                        e2 = 1.5 - a2;
+                        e3 = 3.75 - 5.0 * a2 + a2 * a2;
+                        e4 = -26.25 * a2 + 10.5 * a2 * a2 - a2 * a2 * a2;
+                        f2 = 1.5 - 3.0 * a2;
+                        f3 = 3.75 - 15.0 * a2 + 5.0 * a2 * a2;
+                        f4 = -78.75 * a2 + 52.5 * a2 * a2 - 7.0 * a2 * a2 * a2;
+                    }
+                    SH += a * INV_SQRT_PI * (Q1 + e2 * Q2 + e3 * Q3 + e4 * Q4);
+                    Sa += INV_SQRT_PI * (Q1 + f2 * Q2 + f3 * Q3 + f4 * Q4);
+                    Sv += a * INV_SQRT_PI * (-2.0 * Q1 - 4.0 * e2 * Q2 - 6.0 * e3 * Q3 - 8.0 * e4 * Q4);
+                }
+                const LayerTab tb = A.tab[(int64_t)st * L + l];
+                const LayerPartials d = layer_partials(lp, tb, S.temps[l], S.xi[l], S.n_e[l],
+                                                       S.n_div_U[(int64_t)s * L + l], inv, a,
+                                                       scaling);
+                part[0 * L + l] += scaling * (d.da[0] * Sa + d.dli[0] * Sv) + d.ds[0] * SH;
+                part[1 * L + l] += scaling * (d.da[1] * Sa);
+                part[2 * L + l] += scaling * (d.da[2] * Sa + d.dli[2] * Sv) + d.ds[2] * SH;
+                part[3 * L + l] += scaling * (d.da[3] * Sa);
+                part[4 * L + l] += scaling * (d.da[4] * Sa);
+                part[5 * L + l] += scaling * (d.da[5] * Sa);
+                part[(6 + s) * L + l] += d.ds[6] * SH;
+            }
+        }
+    }
+}
+
 // One block per (star, output): output q < 6 is (quantity q, layer l), q = 6 + s the n_div_U of
 // species s at layer l.  The star's segments are summed in a fixed per-thread order and a fixed
 // tree (deterministic) and the sum goes to the star's output array of that quantity.
@@ -5304,7 +5452,11 @@ int enqueue_vjp(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int
     va.mom = (const float4*)ws.recM.p;
     va.gexp = (const int*)ws.gexp.p;
     const int64_t grid = (int64_t)ctx->n_cu * 16;
-    if (b.adj) hipLaunchKernelGGL(k_vjp<true>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    if (b.adj && b.L <= 56 && !KORG_ADJ_NEAR_F64)
+        hipLaunchKernelGGL(k_vjp_near<7>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    else if (b.adj && b.L <= WAVE && !KORG_ADJ_NEAR_F64)
+        hipLaunchKernelGGL(k_vjp_near<8>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
+    else if (b.adj) hipLaunchKernelGGL(k_vjp<true>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
     else hipLaunchKernelGGL(k_vjp<false>, dim3((unsigned)grid), dim3(WAVE), 0, s, va);
     HIP_TRY(ctx, hipGetLastError());
     const int NQ = 6 + b.sp.n_species;
