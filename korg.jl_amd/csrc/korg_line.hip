@@ -3747,25 +3747,49 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
                 rw[k] = in ? readlane_i32(fr.w, 4 * k) : 0;
             }
             const int4 lfr = act ? fr : make_int4(1, 0, 1, 0);
+            // the pass's lines' windows and adjoint near zones, wavefront-uniform
+            int wlo[ADJ_GROUP], whi[ADJ_GROUP], nlo[ADJ_GROUP], nhi[ADJ_GROUP];
+#pragma unroll
+            for (int k = 0; k < ADJ_GROUP; ++k) {
+                const bool in = k < nl && ((pass >> k) & 1);
+                wlo[k] = in ? readlane_i32(m.lb, 4 * k) : 1;
+                whi[k] = in ? readlane_i32(m.ub, 4 * k) : 0;
+                nlo[k] = readlane_i32(z.x, 4 * k);
+                nhi[k] = readlane_i32(z.y, 4 * k);
+            }
             // K steps start at multiples of 4 (a star's grid starts at one): every point falls in
             // the same step whatever lines share the group, and steps outside a line's far zone add
             // exact zeros, so a line's moments do not depend on its group.  Steps go in blocks of
             // MOM_BLOCK, the block's loads issued before its MFMAs, and blocks in chunks of one
             // 64-point sub-tile, whose Float32 sums are added into the Float64 totals at its end
             // (a block reads at most 4 MOM_BLOCK - 1 points past the union: B = 0 there).  Points in
-            // a sub-tile of a line's far-field runs are left to the far-field steps below; a
-            // sub-tile in the runs of every line of the pass is skipped (it adds only zeros).
+            // a sub-tile of a line's far-field runs are left to the far-field steps below.  Steps
+            // that would add only zeros are skipped: sub-tiles where no line of the pass has a point
+            // of its far zone outside its far-field runs (the union of a pass's windows is mostly
+            // such sub-tiles: far-field runs and near zones), and blocks with no such point for any
+            // lane; the sums are bitwise those of taking every step.
             const int js = u0 & ~(4 * MOM_BLOCK - 1);
             for (int gs = js / FF_SUB; gs <= u1 / FF_SUB; ++gs) {
-                bool all = A.ffrun != nullptr;
+                const int s0 = gs * FF_SUB, s1 = s0 + FF_SUB - 1;
+                bool need = false;
 #pragma unroll
                 for (int k = 0; k < ADJ_GROUP; ++k)
-                    if (k < nl && ((pass >> k) & 1))
-                        all = all && ((gs >= rx[k] && gs <= ry[k]) || (gs >= rz[k] && gs <= rw[k]));
-                if (all) continue;  // wavefront-uniform
+                    need = need || (wlo[k] <= s1 && whi[k] >= s0 && !(s0 >= nlo[k] && s1 <= nhi[k]) &&
+                                    !(A.ffrun && ((gs >= rx[k] && gs <= ry[k]) ||
+                                                  (gs >= rz[k] && gs <= rw[k]))));
+                if (!need) continue;  // wavefront-uniform
                 const bool lane_ff = ff_in(lfr, gs);
                 const int jhi = min(gs * FF_SUB + FF_SUB - 4 * MOM_BLOCK, u1);
                 for (int jb = max(gs * FF_SUB, js); jb <= jhi; jb += 4 * MOM_BLOCK) {
+                    bool farv[MOM_BLOCK];
+                    bool anyf = false;
+#pragma unroll
+                    for (int u = 0; u < MOM_BLOCK; ++u) {
+                        const int j0 = jb + 4 * u;
+                        farv[u] = j0 >= lbr && j0 <= ubr && (j0 < clor || j0 > chir) && !lane_ff;
+                        anyf = anyf || farv[u];
+                    }
+                    if (ballot(anyf) == 0) continue;  // wavefront-uniform
                     const float4* __restrict__ gp =
                         (const float4*)(A.g32 + ((int64_t)lblk * (A.n_total + MOM_PAD) + jb) * WAVE);
                     const double* __restrict__ lp = A.lam_s + jb;
@@ -3779,11 +3803,9 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
                     }
 #pragma unroll
                     for (int u = 0; u < MOM_BLOCK; ++u) {
-                        const int j0 = jb + 4 * u;
                         const float d = (float)(lam[u] - lam0s);
                         float R = __builtin_amdgcn_rcpf(d * d);
-                        const bool far = j0 >= lbr && j0 <= ubr && (j0 < clor || j0 > chir) && !lane_ff;
-                        R = far ? R : 0.0f;
+                        R = farv[u] ? R : 0.0f;
                         const float R2 = R * R;
                         const float b = ((pw & 1) ? R2 : R) * ((pw & 2) ? R2 : 1.0f);
                         acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b, acc[0], 0, 0, 0);
