@@ -213,24 +213,24 @@ __device__ inline float voigt_hjerting_grad_f32(double a64, double v64, float& d
         return a * dHda;
     }
     if (a64 <= 0.2 || (a64 <= 1.4 && a64 + v64 < 3.2)) {
-        const int piece = v64 < 1.3 ? 0 : (v64 < 2.4 ? 1 : 2);
+        // The three pieces of the Harris series' H1 as one form: a quartic Q(v) (the piece's
+        // polynomial; for v >= 2.4 the numerator of the rational piece) times iq (1, or
+        // 1/(v^2 - 1.5) for v >= 2.4), so that lanes in different pieces take one path:
+        // H1 = Q iq, dH1/dv = (Q' - 2 v H1 [v >= 2.4]) iq.
+        const bool p0 = v64 < 1.3, p2 = !(v64 < 2.4);
+        const float c0 = p0 ? -1.12470432f : (p2 ? 0.554153432f : -4.48480194f);
+        const float c1 = p0 ? -0.15516677f : (p2 ? 0.278711796f : 9.39456063f);
+        const float c2 = p0 ? 3.288675912f : (p2 ? -0.1883256872f : -6.61487486f);
+        const float c3 = p0 ? -2.34357915f : (p2 ? 0.042991293f : 1.98919585f);
+        const float c4 = p0 ? 0.42139162f : (p2 ? -0.003278278f : -0.22041650f);
+        const float Q = c0 + (c1 + (c2 + (c3 + c4 * v) * v) * v) * v;
+        const float dQ = c1 + (2.0f * c2 + (3.0f * c3 + 4.0f * c4 * v) * v) * v;
+        const float iq = p2 ? __builtin_amdgcn_rcpf(v2 - 1.5f) : 1.0f;
         const float H0 = __expf(-v2);
-        const float H1 = harris_h1_f32(v, v2, piece);
+        const float H1 = Q * iq;
+        const float dH1 = (p2 ? dQ - 2.0f * v * H1 : dQ) * iq;
         const float H2 = (1.0f - 2.0f * v2) * H0;
         const float dH0 = -2.0f * v * H0;
-        float dH1;
-        if (piece == 0) {
-            dH1 = -0.15516677f + (2.0f * 3.288675912f + (3.0f * -2.34357915f + 4.0f * 0.42139162f * v) * v) * v;
-        } else if (piece == 1) {
-            dH1 = 9.39456063f + (2.0f * -6.61487486f + (3.0f * 1.98919585f - 4.0f * 0.22041650f * v) * v) * v;
-        } else {
-            const float P = 0.554153432f +
-                            (0.278711796f + (-0.1883256872f + (0.042991293f - 0.003278278f * v) * v) * v) * v;
-            const float dP = 0.278711796f +
-                             (2.0f * -0.1883256872f + (3.0f * 0.042991293f - 4.0f * 0.003278278f * v) * v) * v;
-            const float iq = __builtin_amdgcn_rcpf(v2 - 1.5f);
-            dH1 = (dP - 2.0f * v * P * iq) * iq;
-        }
         const float dH2 = H0 * (-6.0f * v + 4.0f * v2 * v);
         if (a64 <= 0.2) {
             dHda = H1 + 2.0f * H2 * a;
