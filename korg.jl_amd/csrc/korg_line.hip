@@ -1057,96 +1057,86 @@ __global__ __launch_bounds__(WAVE) void k_tile_lists(int ntiles, int tile_w, int
     }
 }
 
-// One block: exclusive scan of n counts into start[0..n] (start[n] = total; start may be count
-// itself).  With counters given, the total is recorded in counters[slot] and FLAG_CAPACITY is
-// raised when it exceeds cap.
+// The scans are one wavefront per workgroup: they run beside the near zone's kernel on the other
+// stream, whose single-wave workgroups take every slot that frees up, so a scan block that needed
+// several waves' slots on one CU at once would wait for that whole kernel.
+
+// Inclusive scan across the wavefront.
+__device__ inline int64_t wave_incl_scan(int64_t x, int lane) {
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int64_t y = __shfl_up((long long)x, off, WAVE);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// One wavefront: exclusive scan of n counts into start[0..n] (start[n] = total; start may be
+// count itself), 64 counts per step in order, each step's total carried into the next.  With
+// counters given, the total is recorded in counters[slot] and FLAG_CAPACITY is raised when it
+// exceeds cap.
 template <class T>
-__global__ __launch_bounds__(1024) void k_scan_flat(int64_t n, const T* count, int64_t* start,
+__global__ __launch_bounds__(WAVE) void k_scan_flat(int64_t n, const T* count, int64_t* start,
                                                     int64_t cap, unsigned long long* __restrict__ counters,
                                                     int slot) {
-    __shared__ int64_t part[1024];
     const int t = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t a = t * chunk, b = min(a + chunk, n);
-    int64_t sum = 0;
-    for (int64_t i = a; i < b; ++i) sum += count[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const int64_t v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    int64_t run = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += WAVE) {
+        const int64_t i = c0 + t;
+        const int64_t v = i < n ? (int64_t)count[i] : 0;
+        const int64_t inc = wave_incl_scan(v, t);
+        if (i < n) start[i] = run + inc - v;
+        run += __shfl((long long)inc, WAVE - 1, WAVE);
     }
-    int64_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
-    for (int64_t i = a; i < b; ++i) {
-        const int64_t c = count[i];
-        start[i] = run;
-        run += c;
-    }
-    if (t == 1023) {
-        start[n] = part[1023];
+    if (t == 0) {
+        start[n] = run;
         if (counters) {
-            counters[slot] = (unsigned long long)part[1023];
-            if (part[1023] > cap) atomicOr(&counters[3], FLAG_CAPACITY);
+            counters[slot] = (unsigned long long)run;
+            if (run > cap) atomicOr(&counters[3], FLAG_CAPACITY);
         }
     }
 }
 
-// The same scan over many blocks for long inputs: block totals of SCAN_TILE counts
-// (k_scan_part), their scan in place by one block (k_scan_flat<int64_t>, which also raises the
-// capacity flag), and each block's scan from its offset (k_scan_fill).  Reads are coalesced.
-constexpr int SCAN_TILE = 8192;  // counts per block: 1024 threads x 8
+// The same scan over many workgroups for long inputs: block totals of SCAN_TILE counts
+// (k_scan_part), their scan in place (k_scan_flat<int64_t>, which also raises the capacity flag),
+// and each block's scan from its offset (k_scan_fill).  Reads are coalesced.
+constexpr int SCAN_TILE = 1024;  // counts per workgroup: 16 steps of 64
 
-__global__ __launch_bounds__(1024) void k_scan_part(int64_t n, const int* __restrict__ count,
+__global__ __launch_bounds__(WAVE) void k_scan_part(int64_t n, const int* __restrict__ count,
                                                     int64_t* __restrict__ part) {
-    __shared__ int64_t red[1024];
     const int t = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
     int64_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_TILE / 1024; ++k) {
-        const int64_t i = base + k * 1024 + t;
+    for (int k = 0; k < SCAN_TILE / WAVE; ++k) {
+        const int64_t i = base + k * WAVE + t;
         if (i < n) sum += count[i];
     }
-    red[t] = sum;
-    __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if (t < h) red[t] += red[t + h];
-        __syncthreads();
-    }
-    if (t == 0) part[blockIdx.x] = red[0];
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) sum += __shfl_xor((long long)sum, off, WAVE);
+    if (t == 0) part[blockIdx.x] = sum;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_fill(int64_t n, const int* __restrict__ count,
+__global__ __launch_bounds__(WAVE) void k_scan_fill(int64_t n, const int* __restrict__ count,
                                                     const int64_t* __restrict__ part,
                                                     int64_t* __restrict__ start) {
-    __shared__ int64_t sh[1024];
     const int t = threadIdx.x;
-    constexpr int PER = SCAN_TILE / 1024;
-    const int64_t i0 = (int64_t)blockIdx.x * SCAN_TILE + PER * t;  // this thread's counts
-    int v[PER];
-    int64_t sum = 0;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    constexpr int STEPS = SCAN_TILE / WAVE;
+    int v[STEPS];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        v[k] = i0 + k < n ? count[i0 + k] : 0;
-        sum += v[k];
+    for (int k = 0; k < STEPS; ++k) {
+        const int64_t i = base + k * WAVE + t;
+        v[k] = i < n ? count[i] : 0;
     }
-    sh[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const int64_t x = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += x;
-        __syncthreads();
-    }
-    int64_t run = part[blockIdx.x] + sh[t] - sum;
+    int64_t run = part[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < PER; ++k)
-        if (i0 + k < n) {
-            start[i0 + k] = run;
-            run += v[k];
-        }
+    for (int k = 0; k < STEPS; ++k) {
+        const int64_t i = base + k * WAVE + t;
+        const int64_t inc = wave_incl_scan((int64_t)v[k], t);
+        if (i < n) start[i] = run + inc - v[k];
+        run += __shfl((long long)inc, WAVE - 1, WAVE);
+    }
     if (blockIdx.x == 0 && t == 0) start[n] = part[gridDim.x];
 }
 
@@ -2298,7 +2288,19 @@ struct NearArgs {
     int alpha_f32;
     unsigned long long* counters;
     float* nbuf;  // when set: the near-zone sums are stored here ([global point][L]), not added
+    // nonzero: one item per wavefront (a grid of n_items, items grouped by XCD) instead of
+    // persistent wavefronts taking items from a counter, so that the slots of finished waves go
+    // to kernels on other streams as well (the near zone running beside the far wing)
+    int one_item;
 };
+
+// Item of workgroup b in a grid of n: workgroups go to the XCDs round robin (b % 8), and each
+// XCD takes a contiguous range of items, so the waves sharing an L2 work on neighbouring tiles.
+__device__ inline int64_t xcd_item(int64_t b, int64_t n) {
+    const int64_t q = n / 8, r = n % 8;
+    const int64_t x = b % 8, k = b / 8;
+    return x * q + (x < r ? x : r) + k;
+}
 
 __device__ inline float voigt_far_f32(float a, float v, bool branch_d) {
     const float isp = 0.5641895835477563f;  // 1/sqrt(pi)
@@ -2399,8 +2401,14 @@ __global__ __launch_bounds__(WAVE) void k_near2(NearArgs A) {
     const int r = lane & 7, c = lane >> 3;
     const int L = A.L;
     if (A.counters[3] != 0) return;
-    for (;;) {
-        const int item = wave_take(&A.counters[5], lane);
+    for (int pass = 0;; ++pass) {
+        int item;
+        if (A.one_item) {
+            if (pass > 0) break;
+            item = (int)xcd_item(blockIdx.x, gridDim.x);
+        } else {
+            item = wave_take(&A.counters[5], lane);
+        }
         if (item >= A.n_items) break;
         const int tile = item / A.n_lblocks;
         const int lbase = (item % A.n_lblocks) * WAVE;
@@ -4851,15 +4859,15 @@ int64_t scan_blocks(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 void enqueue_scan(hipStream_t s, int64_t n, const int* count, int64_t* start, int64_t cap,
                   unsigned long long* counters, int slot, int64_t* part) {
     if (n <= SCAN_TILE) {
-        hipLaunchKernelGGL(k_scan_flat<int>, dim3(1), dim3(1024), 0, s, n, count, start, cap,
+        hipLaunchKernelGGL(k_scan_flat<int>, dim3(1), dim3(WAVE), 0, s, n, count, start, cap,
                            counters, slot);
         return;
     }
     const int64_t nb = scan_blocks(n);
-    hipLaunchKernelGGL(k_scan_part, dim3((unsigned)nb), dim3(1024), 0, s, n, count, part);
-    hipLaunchKernelGGL(k_scan_flat<int64_t>, dim3(1), dim3(1024), 0, s, nb, (const int64_t*)part,
+    hipLaunchKernelGGL(k_scan_part, dim3((unsigned)nb), dim3(WAVE), 0, s, n, count, part);
+    hipLaunchKernelGGL(k_scan_flat<int64_t>, dim3(1), dim3(WAVE), 0, s, nb, (const int64_t*)part,
                        part, cap, counters, slot);
-    hipLaunchKernelGGL(k_scan_fill, dim3((unsigned)nb), dim3(1024), 0, s, n, count,
+    hipLaunchKernelGGL(k_scan_fill, dim3((unsigned)nb), dim3(WAVE), 0, s, n, count,
                        (const int64_t*)part, start);
 }
 
@@ -5084,7 +5092,9 @@ int launch_near(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, flo
     na.alpha_f32 = b.alpha_f32;
     na.counters = (unsigned long long*)ws.counters.p;
     na.nbuf = nbuf;
-    int64_t cwaves = (int64_t)ctx->n_cu * b.near_wpc;
+    // beside the far wing (nbuf set): one item per wavefront, in a grid of n_items
+    na.one_item = nbuf != nullptr && b.near_layout != 1 ? 1 : 0;
+    int64_t cwaves = na.one_item ? na.n_items : (int64_t)ctx->n_cu * b.near_wpc;
     if (cwaves > na.n_items) cwaves = na.n_items;
     if (b.near_layout == 1 && !b.s64)
         hipLaunchKernelGGL(k_near, dim3((unsigned)cwaves), dim3(WAVE), 0, s, na);
