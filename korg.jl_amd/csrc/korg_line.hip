@@ -120,42 +120,6 @@ __device__ inline double wave_max(double x) {
     return x;
 }
 
-// First index i in [0, n) with a[i] >= x (n if none): a 64-way search, every lane probes.
-// Requires a ascending and the whole wavefront active.
-template <class T>
-__device__ inline int64_t wave_lower_bound(const T* __restrict__ a, int64_t n, T x, int lane) {
-    int64_t lo = 0, hi = n;  // answer in [lo, hi]
-    while (hi - lo > WAVE) {
-        const int64_t step = (hi - lo + WAVE - 1) / WAVE;
-        const int64_t p = lo + (int64_t)(lane + 1) * step - 1;
-        const bool below = p < hi && a[p] < x;
-        const int k = __popcll(ballot(below));
-        const int64_t phi = lo + (int64_t)(k + 1) * step - 1;
-        hi = (k < WAVE && phi < hi) ? phi : hi;
-        lo = lo + (int64_t)k * step;
-    }
-    const int64_t p = lo + lane;
-    const bool below = p < hi && a[p] < x;
-    return lo + __popcll(ballot(below));
-}
-// First index i with a[i] > x (n if none).
-template <class T>
-__device__ inline int64_t wave_upper_bound(const T* __restrict__ a, int64_t n, T x, int lane) {
-    int64_t lo = 0, hi = n;
-    while (hi - lo > WAVE) {
-        const int64_t step = (hi - lo + WAVE - 1) / WAVE;
-        const int64_t p = lo + (int64_t)(lane + 1) * step - 1;
-        const bool le = p < hi && a[p] <= x;
-        const int k = __popcll(ballot(le));
-        const int64_t phi = lo + (int64_t)(k + 1) * step - 1;
-        hi = (k < WAVE && phi < hi) ? phi : hi;
-        lo = lo + (int64_t)k * step;
-    }
-    const int64_t p = lo + lane;
-    const bool le = p < hi && a[p] <= x;
-    return lo + __popcll(ballot(le));
-}
-
 // Per-lane search with a guess: on a (nearly) evenly spaced ascending array, the index estimated
 // from g0 + i * step is checked against its neighbours and moved at most 3 places.  upper: the
 // first index with a[i] > x, else the first with a[i] >= x.  Returns false when unresolved.
@@ -178,24 +142,19 @@ __device__ inline bool lane_bound_guess(const double* __restrict__ a, int64_t n,
     return false;
 }
 
-// Resolve, for every lane holding a query (active), lower/upper bounds of x on a: guesses first,
-// then the 64-way search for the lanes whose guess failed.  Whole wavefront must call.
-__device__ inline int64_t wave_bounds(const double* __restrict__ a, int64_t n, double g0,
-                                      double ginv, double x, bool upper, bool active, int lane) {
+// Lower (upper: upper) bound of x on an ascending array for one lane's own query: the guess,
+// else a binary search.
+__device__ inline int64_t lane_bound(const double* __restrict__ a, int64_t n, double g0, double ginv,
+                                     double x, bool upper) {
+    if (!(x == x)) return n;  // NaN: nothing compares; lower and upper bound are both n
     int64_t r = 0;
-    const bool ok = active ? lane_bound_guess(a, n, g0, ginv, x, upper, r) : true;
-    uint64_t bad = ballot(!ok);
-    while (bad) {
-        const int b = __ffsll((unsigned long long)bad) - 1;
-        bad &= bad - 1;
-        const double xb = readlane_f64(x, b);
-        const bool ub = __builtin_amdgcn_readlane((int)upper, b) != 0;
-        int64_t v;
-        if (!(xb == xb)) v = n;  // NaN: nothing compares; lower and upper bound are both n
-        else v = ub ? wave_upper_bound<double>(a, n, xb, lane) : wave_lower_bound<double>(a, n, xb, lane);
-        if (lane == b) r = v;
+    if (lane_bound_guess(a, n, g0, ginv, x, upper, r)) return r;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (upper ? a[mid] <= x : a[mid] < x) lo = mid + 1; else hi = mid;
     }
-    return r;
+    return lo;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -431,6 +390,23 @@ struct SetupArgs {
     int64_t n_sub;
     int4* ffrun;
     int4* ffrun_adj;  // Float32 fit step: the runs clear of the adjoint's near zone (null: none)
+    struct LinePre* pre;  // k_setup_pre -> k_setup: each line's own inputs
+    struct LineWin* win;  // k_setup -> k_setup_post: each line's window and zone reaches
+};
+
+// The setup runs as three kernels per call: k_setup_pre (one lane per line: star lookup, the
+// line's parameters, the continuum-knot interval), k_setup (one wavefront per line, lanes =
+// layers: the per-layer records and their maxima over layers) and k_setup_post (one lane per
+// line: the grid searches, zones, far-field runs, class and LineMeta).  The searches and loads
+// are chains of dependent memory reads, which one lane per line keeps many of in flight.
+struct LinePre {
+    double wl, E_lower, gf_sl, gamma_rad, gamma_stark, vdW1, vdW2, m;  // gf_sl = 10^log_gf sigma_line
+    int32_t st, species, kidx, flags;  // flags: 1 molecule, 2 invalid species (line skipped)
+};
+struct LineWin {
+    double window;                 // NaN when some layer's window is NaN
+    double reach, treach, areach;  // zone radii in wavelength (negative: none)
+    int32_t series_off, pad;
 };
 
 constexpr double INV_SQRT_PI = 0.5641895835477563;
@@ -453,13 +429,57 @@ __device__ inline double cntm_lerp(const double* knots, const double* vals, int6
     return (1.0 - t) * vals[i] + t * vals[i + 1];
 }
 
-// Three waves per SIMD (at most 168 registers, a few spilled): measured faster than two.  The
-// Float32 coefficient block (F32) and the branch counting (CB) are compile-time choices, so the
-// common variants carry neither's registers.
+// k_setup_pre: one lane per line.  Also flags lines out of order and species out of range (such
+// a line gets an empty LineMeta here and is skipped by the later kernels).
+__global__ __launch_bounds__(256) void k_setup_pre(SetupArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_lines) return;
+    const StarDev* __restrict__ stars = A.stars;
+    const int st = star_of_line(stars, A.n_stars, i);
+    const StarDev& S = stars[st];
+    const int64_t li = i - S.line_off;  // index in the star's own line list
+    LinePre P;
+    P.wl = S.wl[li];
+    P.st = st;
+    P.species = S.species[li];
+    P.kidx = 0;
+    P.flags = 0;
+    P.E_lower = P.gf_sl = P.gamma_rad = P.gamma_stark = P.vdW1 = P.vdW2 = P.m = 0.0;
+    const int32_t s = P.species;
+    if (s < 0 || s >= A.n_species) {
+        // invalid species index: flag the call as failed (nothing is added to alpha)
+        atomicOr(&A.counters[3], 2ull);
+        LineMeta mt = {P.wl, 1, 0, 1, 0, 0, N_CLASSES, 1, 0};
+        A.meta[i] = mt;
+        P.flags = 2;
+        A.pre[i] = P;
+        return;
+    }
+    if (li > 0 && S.wl[li - 1] > P.wl) atomicOr(&A.counters[3], 1ull);
+    P.m = A.mass[s];
+    P.flags = A.is_mol[s] != 0 ? 1 : 0;
+    P.gamma_rad = S.gamma_rad[li];
+    P.gamma_stark = S.gamma_stark[li];
+    P.vdW1 = S.vdW1[li];
+    P.vdW2 = S.vdW2[li];
+    P.E_lower = S.E_lower[li];
+    P.gf_sl = exp10(S.log_gf[li]) * sigma_line(P.wl);
+    if (S.cntm_mode == KORG_CNTM_GRID) {
+        // knot interval holding wl: clamp(searchsortedlast(knots, wl), 1, n_knots - 1) - 1
+        int64_t k = lane_bound(S.knots, S.n_knots, S.gi.k0, S.gi.kinv, P.wl, true) - 1;
+        k = k < 0 ? 0 : (k > S.n_knots - 2 ? S.n_knots - 2 : k);
+        P.kidx = (int32_t)k;
+    }
+    A.pre[i] = P;
+}
+
+// k_setup: one wavefront per line, lanes = layers (blocks of 64).  Three waves per SIMD (at most
+// 168 registers): measured faster than two.  The Float32 coefficient block (F32) is a
+// compile-time choice, so the Float64 variant carries none of its registers.
 #ifndef KORG_SETUP_ATTR
 #define KORG_SETUP_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 #endif
-template <bool F32, bool CB>
+template <bool F32>
 __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int lane = threadIdx.x % WAVE;
     // the wave's line, its star and the star's fields are wavefront-uniform: scalar registers
@@ -467,42 +487,26 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     if (i >= A.n_lines) return;  // whole wavefront
     const int L = A.L;
+    const KCONST LinePre* pre = uniform_ptr(A.pre);
+    const LinePre P = pre[i];
+    if (P.flags & 2) return;  // invalid species: LineMeta written by k_setup_pre
     const KCONST StarDev* stars = uniform_ptr(A.stars);
-    const int st = star_of_line(stars, A.n_stars, i);
+    const int st = P.st;
     const KCONST StarDev& S = stars[st];
-    const int64_t li = i - S.line_off;  // index in the star's own line list
-    const double wl = S.wl[li];
-    const int32_t s = S.species[li];
-    if (s < 0 || s >= A.n_species) {
-        // invalid species index: flag the call as failed (nothing is added to alpha)
-        if (lane == 0) {
-            atomicOr(&A.counters[3], 2ull);
-            LineMeta mt = {wl, 1, 0, 1, 0, 0, N_CLASSES, 1, 0};
-            A.meta[i] = mt;
-        }
-        return;
-    }
-    if (lane == 0 && li > 0 && S.wl[li - 1] > wl) atomicOr(&A.counters[3], 1ull);
-    const GridInfo gi = S.gi;
-    const double m = A.mass[s];
-    const bool mol = A.is_mol[s] != 0;
-    const double gamma_rad = S.gamma_rad[li], gamma_stark = S.gamma_stark[li];
-    const double vdW1 = S.vdW1[li], vdW2 = S.vdW2[li];
-    const double E_lower = S.E_lower[li];
+    const int64_t li = i - S.line_off;
+    const double wl = P.wl;
+    const int32_t s = P.species;
+    const double m = P.m;
+    const bool mol = (P.flags & 1) != 0;
+    const double gamma_rad = P.gamma_rad, gamma_stark = P.gamma_stark;
+    const double vdW1 = P.vdW1, vdW2 = P.vdW2;
+    const double E_lower = P.E_lower;
     const double E_upper = E_lower + c_cgs * hplanck_eV / wl;  // src/line_absorption.jl:101
-    const double gf = exp10(S.log_gf[li]);
-    const double sl = sigma_line(wl);
-    int64_t kidx = 0;
-    if (S.cntm_mode == KORG_CNTM_GRID) {
-        // knot interval holding wl: clamp(searchsortedlast(knots, wl), 1, n_knots - 1) - 1
-        int64_t ub = wave_bounds(S.knots, S.n_knots, gi.k0, gi.kinv, wl, true, lane == 0, lane);
-        kidx = __builtin_amdgcn_readfirstlane((int)ub) - 1;
-        kidx = kidx < 0 ? 0 : (kidx > S.n_knots - 2 ? S.n_knots - 2 : kidx);
-    }
+    const double gf_sl = P.gf_sl;
+    const int64_t kidx = P.kidx;
     double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0, amax = -1.0;
     bool any_nan = false;
     bool f32_bad = false;  // a far-wing coefficient of some layer is beyond Float32 range
-    double my_hw = -1.0, my_a = 0.0;  // layer `lane` (first 64 layers), for branch counting
     for (int l = L + lane; l < A.Ls; l += WAVE) A.rec64[i * A.Ls + l] = make_double4(0, 0, 0, 0);
     const int Lm = A.n_lblocks * WAVE;
     for (int l = lane; l < Lm; l += WAVE) {
@@ -520,7 +524,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             }
             const double gamma = G * (wl * wl) / (c_cgs * (4.0 * PI));
             const double levels = exp(-tb.beta * E_lower) - exp(-tb.beta * E_upper);
-            const double amp = gf * sl * levels * S.n_div_U[(int64_t)s * L + l];
+            const double amp = gf_sl * levels * S.n_div_U[(int64_t)s * L + l];
             // src/line_absorption.jl:108-112
             const double cn = (S.cntm_mode == KORG_CNTM_GRID)
                                   ? cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, kidx, wl)
@@ -569,10 +573,6 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             }
             const double rt = vt * s2;
             tmax = rt > tmax ? rt : tmax;
-            if (l == lane) {
-                my_hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
-                my_a = a;
-            }
             if (F32) {
                 const double sq = s2 * (1.0 / DELTA_UNIT);
                 const double q = sq * sq;  // 1 / (DELTA_UNIT inv)^2
@@ -641,31 +641,50 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double treach = wave_max(tmax);
     const double areach = (F32 && A.azone) ? wave_max(amax) : -1.0;
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
+    if (lane == 0) {
+        LineWin w;
+        w.window = window;
+        w.reach = reach;
+        w.treach = treach;
+        w.areach = areach;
+        w.series_off = series_off ? 1 : 0;
+        w.pad = 0;
+        A.win[i] = w;
+    }
+}
+
+// k_setup_post: one lane per line.  Lower/upper bounds on the grid of the window's edges, the
+// centre, and the near, tail and adjoint zones' edges (src/line_absorption.jl:109-115), clipped
+// as the zones nest; the far-field runs, the window class and the LineMeta.
+template <bool F32>
+__global__ __launch_bounds__(256) void k_setup_post(SetupArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_lines) return;
+    const LinePre P = A.pre[i];
+    if (P.flags & 2) return;  // invalid species: LineMeta written by k_setup_pre
+    const LineWin w = A.win[i];
+    const StarDev& S = A.stars[P.st];
+    const int64_t li = i - S.line_off;
+    const GridInfo gi = S.gi;
+    const double wl = P.wl;
+    const double window = w.window, reach = w.reach, treach = w.treach, areach = w.areach;
+    const bool series_off = w.series_off != 0;
     const int64_t n = S.n_wl;
     const double last = S.wls[n - 1];
-    // lanes 0..4 search for lb, ub, jc, near-zone start and end in parallel
-    double x = wl;
-    bool upper = false;
     const double h = reach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
     const double ht = treach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
     const double ha = areach * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
-    if (lane == 0) x = wl - window;
-    if (lane == 1) { x = wl + window; upper = true; }
-    if (lane == 3) x = wl - h;
-    if (lane == 4) { x = wl + h; upper = true; }
-    if (lane == 5) x = wl - ht;
-    if (lane == 6) { x = wl + ht; upper = true; }
-    if (lane == 7) x = wl - ha;
-    if (lane == 8) { x = wl + ha; upper = true; }
-    if (lane <= 1 && x >= 1.0) x = x * 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
-    const bool past = !(x <= last);           // beyond the grid (or NaN)
-    const int nsearch = (F32 && A.azone) ? 9 : 7;
-    const int64_t r = wave_bounds(S.wls, n, gi.g0, gi.ginv, x, upper, lane < nsearch && !past, lane);
-    int64_t res = r;
-    if (lane < nsearch && past) res = n;  // lower bound n; upper bound n (last index n - 1)
-    int64_t lb = __builtin_amdgcn_readlane((int)res, 0);
-    int64_t ub = (int64_t)__builtin_amdgcn_readlane((int)res, 1) - 1;
-    const int64_t jc = __builtin_amdgcn_readlane((int)res, 2);
+    // x beyond the grid (or NaN): lower bound n, upper bound n (last index n - 1)
+    auto bound = [&](double x, bool upper) -> int64_t {
+        if (!(x <= last)) return n;
+        return lane_bound(S.wls, n, gi.g0, gi.ginv, x, upper);
+    };
+    double xlo = wl - window, xhi = wl + window;
+    if (xlo >= 1.0) xlo *= 1e-8;  // src/wavelengths.jl:189,201: >= 1 is Angstrom
+    if (xhi >= 1.0) xhi *= 1e-8;
+    int64_t lb = bound(xlo, false);
+    int64_t ub = bound(xhi, true) - 1;
+    const int64_t jc = bound(wl, false);
     if (S.win_lb) {  // frozen windows (1-based), clipped to the grid; zones are clipped to them
         lb = S.win_lb[li] - 1;
         ub = S.win_ub[li] - 1;
@@ -674,22 +693,22 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     }
     int64_t clo = 1, chi = 0;
     if (lb <= ub && reach >= 0.0) {
-        clo = __builtin_amdgcn_readlane((int)res, 3);
-        chi = (int64_t)__builtin_amdgcn_readlane((int)res, 4) - 1;
+        clo = bound(wl - h, false);
+        chi = bound(wl + h, true) - 1;
         clo = clo < lb ? lb : clo;
         chi = chi > ub ? ub : chi;
     }
     int64_t flo = 1, fhi = 0;
     if (lb <= ub && treach >= 0.0) {
-        flo = __builtin_amdgcn_readlane((int)res, 5);
-        fhi = (int64_t)__builtin_amdgcn_readlane((int)res, 6) - 1;
+        flo = bound(wl - ht, false);
+        fhi = bound(wl + ht, true) - 1;
         flo = flo < lb ? lb : flo;
         fhi = fhi > ub ? ub : fhi;
     }
     int64_t alo = clo, ahi = chi;  // the adjoint's near zone contains the forward's
     if (F32 && A.azone && lb <= ub && areach >= 0.0) {
-        alo = min((int64_t)__builtin_amdgcn_readlane((int)res, 7), clo);
-        ahi = max((int64_t)__builtin_amdgcn_readlane((int)res, 8) - 1, chi);
+        alo = min(bound(wl - ha, false), clo);
+        ahi = max(bound(wl + ha, true) - 1, chi);
         alo = alo < lb ? lb : alo;
         ahi = ahi > ub ? ub : ahi;
     }
@@ -700,31 +719,6 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         fhi = 0;
         alo = lb;
         ahi = ub;
-    }
-    if (CB && lb <= ub && lane < L) {
-        // evaluations of layer `lane` by branch: core points lie within my_hw of lam0
-        int64_t core_n = 0;
-        if (my_hw >= 0.0) {
-            int64_t c0 = 0, c1 = 0;
-            if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, wl - my_hw, true, c0))
-                c0 = lb;  // counting only: fall back to the window edge
-            if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, wl + my_hw, false, c1))
-                c1 = ub + 1;
-            c1 -= 1;
-            c0 = c0 < lb ? lb : c0;
-            c1 = c1 > ub ? ub : c1;
-            core_n = c1 >= c0 ? c1 - c0 + 1 : 0;
-        }
-        const int64_t wing_n = (ub - lb + 1) - core_n;
-        const bool small_a = my_a <= 0.2;
-        atomicAdd(&A.counters[small_a ? 6 : 9], (unsigned long long)wing_n);
-        if (core_n) atomicAdd(&A.counters[small_a ? 7 : 8], (unsigned long long)core_n);
-        // branch-A evaluations inside the near zone (which holds every core point): the rest of
-        // the near zone past the core of a layer with a <= 0.2
-        if (F32 && small_a && clo <= chi) {
-            const int64_t near_wing = (chi - clo + 1) - core_n;
-            if (near_wing > 0) atomicAdd(&A.counters[15], (unsigned long long)near_wing);
-        }
     }
     // The far-field runs, in local sub-tile indices q (points 64q .. 64q + 63, whole sub-tiles
     // only): inside the window, clear of the near zone [zlo, zhi], and with the sub-tile's centre
@@ -761,7 +755,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         cls = 0;
         while (cls < N_CLASSES - 1 && W > class_wmax(cls)) ++cls;
     }
-    if (lane == 0) {
+    {
         // grid indices are stored global (offset by the star's place in the concatenated grids)
         const int64_t go = S.grid_off;
         LineMeta mt;
@@ -778,6 +772,51 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         if (F32 && A.azone) A.azone[i] = make_int2((int)(alo + go), (int)(ahi + go));
         if (A.ffrun) A.ffrun[i] = run;
         if (F32 && A.ffrun_adj) A.ffrun_adj[i] = run_adj;
+    }
+}
+
+// Evaluations by Hunger branch (the count_branches diagnostic): one wavefront per line, lanes =
+// layers; a layer's core points (branches B and C) lie within its half-width hw of the centre:
+// 5 Doppler units for a <= 0.2, 3.2 - a for a <= 1.4, none beyond.  Counted into counters[6..9]
+// (A, B, C, D) and [15] (branch-A evaluations inside the near zone).
+template <bool F32>
+__global__ __launch_bounds__(256) void k_branch_count(SetupArgs A) {
+    const int lane = threadIdx.x % WAVE;
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (i >= A.n_lines || lane >= A.L) return;
+    const LinePre P = A.pre[i];
+    if (P.flags & 2) return;
+    const StarDev& S = A.stars[P.st];
+    const LineMeta mt = A.meta[i];
+    const int64_t go = S.grid_off;
+    const int64_t lb = mt.lb - go, ub = mt.ub - go, clo = mt.clo - go, chi = mt.chi - go;
+    if (lb > ub) return;
+    const double4 r = A.rec64[i * A.Ls + lane];
+    const double inv = r.x, a = r.y;
+    const double hw = (a <= 0.2) ? 5.0 / inv : (a <= 1.4 ? (3.2 - a) / inv : -1.0);
+    const int64_t n = S.n_wl;
+    const GridInfo gi = S.gi;
+    int64_t core_n = 0;
+    if (hw >= 0.0) {
+        int64_t c0 = 0, c1 = 0;
+        if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, P.wl - hw, true, c0))
+            c0 = lb;  // counting only: fall back to the window edge
+        if (!lane_bound_guess(S.wls, n, gi.g0, gi.ginv, P.wl + hw, false, c1))
+            c1 = ub + 1;
+        c1 -= 1;
+        c0 = c0 < lb ? lb : c0;
+        c1 = c1 > ub ? ub : c1;
+        core_n = c1 >= c0 ? c1 - c0 + 1 : 0;
+    }
+    const int64_t wing_n = (ub - lb + 1) - core_n;
+    const bool small_a = a <= 0.2;
+    atomicAdd(&A.counters[small_a ? 6 : 9], (unsigned long long)wing_n);
+    if (core_n) atomicAdd(&A.counters[small_a ? 7 : 8], (unsigned long long)core_n);
+    // branch-A evaluations inside the near zone (which holds every core point): the rest of
+    // the near zone past the core of a layer with a <= 0.2
+    if (F32 && small_a && clo <= chi) {
+        const int64_t near_wing = (chi - clo + 1) - core_n;
+        if (near_wing > 0) atomicAdd(&A.counters[15], (unsigned long long)near_wing);
     }
 }
 
@@ -4493,6 +4532,7 @@ struct Workspace {
     DevBuf ffgeo, fft;  // far field: sub-tile geometry, the sums T of every sub-tile
     DevBuf ffrun, ffkey;  // far field: each line's runs (line order), list keys (partition order)
     DevBuf ffrun_adj, gff;  // the adjoint's far field: runs clear of its near zone, moments of G
+    DevBuf lpre, lwin;      // the setup's per-line records (k_setup_pre, k_setup)
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -4510,7 +4550,7 @@ struct Workspace {
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
                           &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey,
-                          &ffrun_adj, &gff};
+                          &ffrun_adj, &gff,      &lpre,      &lwin};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -4524,7 +4564,7 @@ struct Workspace {
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
                           &scan_part, &jvp_coef, &nbuf,      &ffgeo,     &fft,       &ffrun,
-                          &ffkey,     &ffrun_adj, &gff};
+                          &ffkey,     &ffrun_adj, &gff,      &lpre,      &lwin};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -4912,6 +4952,8 @@ int plan_batch(korg_ctx* ctx, const korg_tuning& t, int n_stars, const StarHost*
     b.n_lblocks = (L + WAVE - 1) / WAVE;  // 64-layer blocks (Float32 records)
     b.nblocks = (int)((N + PART_BLOCK - 1) / PART_BLOCK);
     TRY(grow(ctx, ws.meta, sizeof(LineMeta) * N));
+    TRY(grow(ctx, ws.lpre, sizeof(LinePre) * N));
+    TRY(grow(ctx, ws.lwin, sizeof(LineWin) * N));
     TRY(grow(ctx, ws.rec64, sizeof(double4) * N * b.Ls));
     if (b.f32)
         TRY(grow(ctx, ws.recM, (b.s64 ? sizeof(double4) : sizeof(float4)) * (size_t)N * b.n_lblocks * WAVE));
@@ -4980,13 +5022,19 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     if (b.ff)  // the sub-tiles' geometry, which the setup's far-field runs read
         hipLaunchKernelGGL(k_ff_geo, dim3((unsigned)((b.n_sub + 255) / 256)), dim3(256), 0, s,
                            b.n_sub, b.n_stars, (StarDev*)ws.stars.p, (FfGeo*)ws.ffgeo.p);
-    const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);
+    sa.pre = (LinePre*)ws.lpre.p;
+    sa.win = (LineWin*)ws.lwin.p;
+    const dim3 lgrid((unsigned)((N + 255) / 256));                  // one lane per line
+    const dim3 sgrid((unsigned)((N + 3) / 4)), sblock(256);         // one wavefront per line
+    hipLaunchKernelGGL(k_setup_pre, lgrid, sblock, 0, s, sa);
     if (b.f32) {
-        if (b.count_branches) hipLaunchKernelGGL((k_setup<true, true>), sgrid, sblock, 0, s, sa);
-        else hipLaunchKernelGGL((k_setup<true, false>), sgrid, sblock, 0, s, sa);
+        hipLaunchKernelGGL(k_setup<true>, sgrid, sblock, 0, s, sa);
+        hipLaunchKernelGGL(k_setup_post<true>, lgrid, sblock, 0, s, sa);
+        if (b.count_branches) hipLaunchKernelGGL(k_branch_count<true>, sgrid, sblock, 0, s, sa);
     } else {
-        if (b.count_branches) hipLaunchKernelGGL((k_setup<false, true>), sgrid, sblock, 0, s, sa);
-        else hipLaunchKernelGGL((k_setup<false, false>), sgrid, sblock, 0, s, sa);
+        hipLaunchKernelGGL(k_setup<false>, sgrid, sblock, 0, s, sa);
+        hipLaunchKernelGGL(k_setup_post<false>, lgrid, sblock, 0, s, sa);
+        if (b.count_branches) hipLaunchKernelGGL(k_branch_count<false>, sgrid, sblock, 0, s, sa);
     }
     HIP_TRY(ctx, hipGetLastError());
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
