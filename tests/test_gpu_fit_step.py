@@ -248,3 +248,44 @@ def test_fit_step_gradient_matches_central_differences(ctx):
         fd = (cp - cm) / (2 * h)
         pred = float(np.sum(g[gk].reshape(d.shape) * d))
         assert pred == pytest.approx(fd, rel=1e-5, abs=1e-9 * abs(fd)), (name, pred, fd)
+
+
+def _refine(P, tau_ref, alpha_ref, n):
+    """P's atmosphere resampled to n layers (temperatures and microturbulence linear, densities,
+    continuum and the reference depth scale geometric in the layer index)."""
+    x = np.linspace(0.0, P.n_layers - 1.0, n)
+    k = np.arange(P.n_layers)
+    lin = lambda v: np.interp(x, k, v)
+    geo = lambda v: np.exp(np.interp(x, k, np.log(np.maximum(v, 1e-300))))
+    c = np.ascontiguousarray
+    nd = c(np.array([geo(r) for r in P.n_div_U]))
+    cv = c(np.array([geo(P.cntm_values[:, j]) for j in range(P.cntm_values.shape[1])]).T)
+    Q = Problem(P.all_wls, P.lines, P.species_mass, P.species_is_mol, nd, c(lin(P.temps)),
+                c(geo(P.n_e)), c(geo(P.n_HI)), c(geo(P.n_HeI)), c(geo(P.n_H2)), c(lin(P.xi)),
+                P.cntm_mode, cv, P.cntm_knots, P.cutoff_threshold)
+    return Q, geo(tau_ref), geo(alpha_ref)
+
+
+@pytest.mark.parametrize("n_layers", [60, 64, 72])
+def test_fit_step_fp32_gradient_layer_counts(ctx, n_layers):
+    """The Float32 step's gradient against the Float64 step's at 60 and 64 layers (the near-zone
+    adjoint with 8 layer groups of 8) and at 72 (two 64-layer blocks: lanes over layers), with
+    systematic residuals: each quantity's per-layer gradient within 1e-5 (2-norm)."""
+    P0, atm, data, sigma = make_star(seed=14, n_lines=400)
+    tau_ref, alpha_ref = _reference(atm)
+    P, tr, ar = _refine(P0, tau_ref, alpha_ref, n_layers)
+    D = DeviceProblem(P)
+    t = FitTarget(P, tr, ar, data, sigma)
+    ctx.fit_gradient_batched([D.new_alpha("float64")], [D], [t], "fp64")
+    f = t.flux()
+    data = 1 - 1.1 * (1 - f) + sigma * np.random.default_rng(14).standard_normal(f.size)
+    res = {}
+    for prec, dt in (("fp64", "float64"), ("fp32", "float32")):
+        D = DeviceProblem(P)
+        t = FitTarget(P, tr, ar, data, sigma)
+        ctx.fit_gradient_batched([D.new_alpha(dt)], [D], [t], prec)
+        res[prec] = t.grads()
+    for q in ("T", "n_e", "xi", "n_HI", "n_div_U"):
+        err = np.linalg.norm(res["fp32"][q] - res["fp64"][q]) / np.linalg.norm(res["fp64"][q])
+        print(f"{n_layers} layers: {q} {err:.2e}")
+        assert err <= 1e-5, (n_layers, q, err)
