@@ -112,12 +112,31 @@ __device__ inline const KCONST T* uniform_ptr(const T* p) {
     return (const KCONST T*)(((uint64_t)hi << 32) | lo);
 }
 
+// x moved across lanes by a DPP control (both halves of the double); lanes with no source, or in
+// rows the row mask leaves out, keep their own x.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)u, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(u >> 32), (int)(uint32_t)(u >> 32), CTRL, ROWS,
+                                               0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// The largest x of the wavefront (no NaN among them), in every lane: prefix maxima within each
+// 16-lane row by row shifts, then rows 1 and 3 take row 0's and row 2's last lane (row_bcast:15)
+// and rows 2 and 3 lane 31 (row_bcast:31), so lane 63 holds the maximum.  Exact, as a max is.
 __device__ inline double wave_max(double x) {
-    for (int off = 32; off >= 1; off >>= 1) {
-        double y = __shfl_xor(x, off, WAVE);
-        x = y > x ? y : x;
-    }
-    return x;
+    x = fmax(x, dpp_f64<0x111, 0xF>(x));  // row_shr:1
+    x = fmax(x, dpp_f64<0x112, 0xF>(x));  // row_shr:2
+    x = fmax(x, dpp_f64<0x114, 0xF>(x));  // row_shr:4
+    x = fmax(x, dpp_f64<0x118, 0xF>(x));  // row_shr:8
+    x = fmax(x, dpp_f64<0x142, 0xA>(x));  // row_bcast:15
+    x = fmax(x, dpp_f64<0x143, 0xC>(x));  // row_bcast:31
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // Per-lane search with a guess: on a (nearly) evenly spaced ascending array, the index estimated
@@ -401,6 +420,8 @@ struct SetupArgs {
 // are chains of dependent memory reads, which one lane per line keeps many of in flight.
 struct LinePre {
     double wl, E_lower, gf_sl, gamma_rad, gamma_stark, vdW1, vdW2, m;  // gf_sl = 10^log_gf sigma_line
+    double E_upper;  // src/line_absorption.jl:101
+    double kt;       // continuum on knots: wl's weight in its knot interval kidx
     int32_t st, species, kidx, flags;  // flags: 1 molecule, 2 invalid species (line skipped)
 };
 struct LineWin {
@@ -445,6 +466,7 @@ __global__ __launch_bounds__(256) void k_setup_pre(SetupArgs A) {
     P.kidx = 0;
     P.flags = 0;
     P.E_lower = P.gf_sl = P.gamma_rad = P.gamma_stark = P.vdW1 = P.vdW2 = P.m = 0.0;
+    P.E_upper = P.kt = 0.0;
     const int32_t s = P.species;
     if (s < 0 || s >= A.n_species) {
         // invalid species index: flag the call as failed (nothing is added to alpha)
@@ -464,11 +486,13 @@ __global__ __launch_bounds__(256) void k_setup_pre(SetupArgs A) {
     P.vdW2 = S.vdW2[li];
     P.E_lower = S.E_lower[li];
     P.gf_sl = exp10(S.log_gf[li]) * sigma_line(P.wl);
+    P.E_upper = P.E_lower + c_cgs * hplanck_eV / P.wl;  // src/line_absorption.jl:101
     if (S.cntm_mode == KORG_CNTM_GRID) {
         // knot interval holding wl: clamp(searchsortedlast(knots, wl), 1, n_knots - 1) - 1
         int64_t k = lane_bound(S.knots, S.n_knots, S.gi.k0, S.gi.kinv, P.wl, true) - 1;
         k = k < 0 ? 0 : (k > S.n_knots - 2 ? S.n_knots - 2 : k);
         P.kidx = (int32_t)k;
+        P.kt = (P.wl - S.knots[k]) / (S.knots[k + 1] - S.knots[k]);
     }
     A.pre[i] = P;
 }
@@ -501,9 +525,10 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double gamma_rad = P.gamma_rad, gamma_stark = P.gamma_stark;
     const double vdW1 = P.vdW1, vdW2 = P.vdW2;
     const double E_lower = P.E_lower;
-    const double E_upper = E_lower + c_cgs * hplanck_eV / wl;  // src/line_absorption.jl:101
+    const double E_upper = P.E_upper;
     const double gf_sl = P.gf_sl;
     const int64_t kidx = P.kidx;
+    const double kt = P.kt;
     double dmax = 0.0, lmax = 0.0, rmax = -1.0, tmax = -1.0, amax = -1.0;
     bool any_nan = false;
     bool f32_bad = false;  // a far-wing coefficient of some layer is beyond Float32 range
@@ -526,8 +551,9 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             const double levels = exp(-tb.beta * E_lower) - exp(-tb.beta * E_upper);
             const double amp = gf_sl * levels * S.n_div_U[(int64_t)s * L + l];
             // src/line_absorption.jl:108-112
+            const double* cv = S.cntm + (int64_t)l * S.n_knots;
             const double cn = (S.cntm_mode == KORG_CNTM_GRID)
-                                  ? cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, kidx, wl)
+                                  ? (1.0 - kt) * cv[kidx] + kt * cv[kidx + 1]  // cntm_lerp, t = kt
                                   : S.cntm[li * L + l];
             const double rho = cn * S.cutoff / amp;
             const double dg = inverse_gaussian_density(rho, sigma);
