@@ -409,6 +409,7 @@ struct SetupArgs {
     int64_t n_sub;
     int4* ffrun;
     int4* ffrun_adj;  // Float32 fit step: the runs clear of the adjoint's near zone (null: none)
+    int4* azq;       // Float32 fit step, at most 64 layers: each 16-layer block's adjoint near zone
     struct LinePre* pre;  // k_setup_pre -> k_setup: each line's own inputs
     struct LineWin* win;  // k_setup -> k_setup_post: each line's window and zone reaches
 };
@@ -427,6 +428,7 @@ struct LinePre {
 struct LineWin {
     double window;                 // NaN when some layer's window is NaN
     double reach, treach, areach;  // zone radii in wavelength (negative: none)
+    double ablk[4];                // the adjoint radius of each 16-layer block (with azq)
     int32_t series_off, pad;
 };
 
@@ -666,6 +668,15 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
     const double reach = wave_max(rmax);
     const double treach = wave_max(tmax);
     const double areach = (F32 && A.azone) ? wave_max(amax) : -1.0;
+    // each 16-layer block's own adjoint radius (lanes 16q .. 16q + 15 hold block q's layers)
+    double ablk[4] = {areach, areach, areach, areach};
+    if (F32 && A.azq && A.n_lblocks == 1) {
+        double m16 = amax;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) m16 = fmax(m16, __shfl_xor(m16, off, WAVE));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ablk[q] = readlane_f64(m16, 16 * q);
+    }
     const double window = wnan ? __builtin_nan("") : sqrt(lwin * lwin + dwin * dwin);
     if (lane == 0) {
         LineWin w;
@@ -673,6 +684,8 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
         w.reach = reach;
         w.treach = treach;
         w.areach = areach;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w.ablk[q] = ablk[q];
         w.series_off = series_off ? 1 : 0;
         w.pad = 0;
         A.win[i] = w;
@@ -775,6 +788,24 @@ __global__ __launch_bounds__(256) void k_setup_post(SetupArgs A) {
     };
     const int4 run = A.ffrun && ff_ok ? ff_runs(clo, chi) : make_int4(1, 0, 1, 0);
     const int4 run_adj = F32 && A.ffrun_adj && ff_ok ? ff_runs(alo, ahi) : make_int4(1, 0, 1, 0);
+    // each 16-layer block's adjoint near zone inside [alo, ahi] (the whole of it where the block
+    // has no narrower radius): beyond it the block's layers take the moments' series
+    int4 azq = make_int4(0, 0, 0, 0);
+    if (F32 && A.azq && !series_off && lb <= ub && alo <= ahi) {
+        int e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[q] = 0;
+            if (w.ablk[q] >= 0.0 && w.ablk[q] < areach) {
+                const double hq = w.ablk[q] * (1.0 + 1e-6) + 1e-6 * DELTA_UNIT;
+                const int64_t qlo = max(bound(wl - hq, false), alo);
+                const int64_t qhi = min(bound(wl + hq, true) - 1, ahi);
+                e[q] = (int)((uint32_t)min(qlo - alo, (int64_t)0xFFFF) |
+                             ((uint32_t)min(ahi - qhi, (int64_t)0xFFFF) << 16));
+            }
+        }
+        azq = make_int4(e[0], e[1], e[2], e[3]);
+    }
     int cls = N_CLASSES;
     if (lb <= ub) {
         const int64_t W = ub - lb + 1;
@@ -796,6 +827,7 @@ __global__ __launch_bounds__(256) void k_setup_post(SetupArgs A) {
         mt.fhi = (int32_t)(fhi + go);
         A.meta[i] = mt;
         if (F32 && A.azone) A.azone[i] = make_int2((int)(alo + go), (int)(ahi + go));
+        if (F32 && A.azq) A.azq[i] = azq;
         if (A.ffrun) A.ffrun[i] = run;
         if (F32 && A.ffrun_adj) A.ffrun_adj[i] = run_adj;
     }
@@ -2587,6 +2619,22 @@ __device__ inline bool far_zone_empty(const LineMeta& m, int2 z) {
     return m.lb > m.ub || (z.x <= m.lb && z.y >= m.ub);
 }
 
+// The adjoint near zone of 16-layer block q of a line whose zone is z, from its azq entry e
+// (offsets from z.x in the low 16 bits, from z.y in the high 16 bits).
+__device__ inline int2 azone_blk(int2 z, int e) {
+    return make_int2(z.x + (e & 0xFFFF), z.y - (int)((uint32_t)e >> 16));
+}
+// The innermost of a line's block zones (all of them nest about the centre): outside it some
+// block's layers take the moments' series.  Without azq, the line's zone.
+__device__ inline int2 azone_in(int2 z, const int4* __restrict__ azq, int64_t i) {
+    if (!azq) return z;
+    const int4 e = azq[i];
+    const int lo = max(max(e.x & 0xFFFF, e.y & 0xFFFF), max(e.z & 0xFFFF, e.w & 0xFFFF));
+    const int hi = (int)max(max((uint32_t)e.x >> 16, (uint32_t)e.y >> 16),
+                            max((uint32_t)e.z >> 16, (uint32_t)e.w >> 16));
+    return make_int2(z.x + lo, z.y - hi);
+}
+
 constexpr int VJP_CHUNK = 2048;
 constexpr int VJP_SEG = 64;
 // Points whose loads are in flight together: 8 for the Float32 evaluation, 2 for the Float64
@@ -2723,6 +2771,7 @@ struct VjpArgs {
     // 2^gexp[star * L + l])
     int n_lblocks;
     const int2* azone;
+    const int4* azq;  // each 16-layer block's adjoint zone (null: the line's)
     const float4* mom;
     const int* gexp;
 };
@@ -2908,7 +2957,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
             const int2 z = A.azone[i];
             const int64_t j0 = z.x + (item - A.item_start[i]) * VJP_CHUNK;
             const int64_t j1 = z.x > z.y ? j0 : min(j0 + VJP_CHUNK, (int64_t)z.y + 1);  // exclusive
-            const bool far = first_item && !far_zone_empty(m, z);
+            const bool far = first_item && !far_zone_empty(m, azone_in(z, A.azq, i));
+            const int4 zq = A.azq ? A.azq[i] : make_int4(0, 0, 0, 0);
             const double lam0 = m.lam0;
             const int32_t s = S.species[li];
             const LinePartialsIn lp = line_partials_in(S, li, lam0, A.mass, A.is_mol);
@@ -2924,19 +2974,24 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 const bool gvld = lg < L;
                 const double2 qq = s_q[lg];
                 double SH = 0.0, Sa = 0.0, Sv = 0.0;
+                // the points of the chunk in the zone of the group's 16-layer block (beyond it the
+                // block's layers take the moments' series)
+                const int eq = (g >> 1) == 0 ? zq.x : (g >> 1) == 1 ? zq.y : (g >> 1) == 2 ? zq.z : zq.w;
+                const int2 zb = azone_blk(z, eq);
+                const int64_t ja = max(j0, (int64_t)zb.x), jz = min(j1, (int64_t)zb.y + 1);
                 constexpr int NPB = 4;  // point steps whose loads are in flight together
-                for (int64_t jb = j0; jb < j1; jb += 8 * NPB) {
+                for (int64_t jb = ja; jb < jz; jb += 8 * NPB) {
                     double gv[NPB], lv[NPB];
 #pragma unroll
                     for (int u = 0; u < NPB; ++u) {
-                        const int64_t jl = min(jb + 8 * u + c, j1 - 1) - go;  // clamped: in bounds
+                        const int64_t jl = min(jb + 8 * u + c, jz - 1) - go;  // clamped: in bounds
                         const int64_t idx = (int64_t)(gvld ? lg : 0) + (int64_t)L * jl;
                         gv[u] = A.g_f32 ? (double)((const float*)S.G)[idx] : ((const double*)S.G)[idx];
                         lv[u] = S.wls[jl];
                     }
 #pragma unroll
                     for (int u = 0; u < NPB; ++u) {
-                        if (!gvld || jb + 8 * u + c >= j1) continue;
+                        if (!gvld || jb + 8 * u + c >= jz) continue;
                         const double gg = gv[u];
                         const double v = fabs(lv[u] - lam0) * qq.x;
                         float Ha, Hv;
@@ -3515,13 +3570,14 @@ __device__ inline int adj_class(const LineMeta& m, int2 z) {
 // Class counts per block of lines (the partition's first pass).
 __global__ __launch_bounds__(PART_BLOCK) void k_adj_count(int64_t n, const LineMeta* __restrict__ meta,
                                                           const int2* __restrict__ azone,
+                                                          const int4* __restrict__ azq,
                                                           int* __restrict__ block_counts) {
     __shared__ int cnt[ADJ_NC];
     if (threadIdx.x < ADJ_NC) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
     if (i < n) {
-        const int c = adj_class(meta[i], azone[i]);
+        const int c = adj_class(meta[i], azone_in(azone[i], azq, i));
         if (c < ADJ_NC) atomicAdd(&cnt[c], 1);
     }
     __syncthreads();
@@ -3533,6 +3589,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_adj_count(int64_t n, const LineM
 // Also marks each group's leader: adj_lead[line] = the group's position in adj_line, else -1.
 __global__ __launch_bounds__(PART_BLOCK) void k_adj_scatter(int64_t n, const LineMeta* __restrict__ meta,
                                                             const int2* __restrict__ azone,
+                                                            const int4* __restrict__ azq,
                                                             const int* __restrict__ block_offsets,
                                                             const int* __restrict__ class_start,
                                                             int* __restrict__ adj_line,
@@ -3540,7 +3597,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_adj_scatter(int64_t n, const Lin
     __shared__ int wave_cnt[PART_BLOCK / WAVE][ADJ_NC];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t i = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
-    const int c = i < n ? adj_class(meta[i], azone[i]) : ADJ_NC;
+    const int c = i < n ? adj_class(meta[i], azone_in(azone[i], azq, i)) : ADJ_NC;
     int rank = 0;
     for (int k = 0; k < ADJ_NC; ++k) {
         const uint64_t b = ballot(c == k);
@@ -3624,9 +3681,14 @@ constexpr int MOM_CHUNK = 16;
 constexpr int MOM_PAD = 4 * MOM_BLOCK;
 static_assert(4 * MOM_CHUNK == FF_SUB, "a chunk of the moments is one far-field sub-tile");
 
-// G in Float32, scaled by 2^-e per (star, layer), as [layer block][global point][64 layers] with
-// MOM_PAD points after the last (layers past L and points outside every star zero): the moments'
-// A operand, 1 KB per 4 points.  Also the wavelengths in units of DELTA_UNIT on the same global
+// Position of layer l (of a 64-layer block) in the moments' A operands: 4 (l % 16) + l / 16, so
+// that the float4 of lane column c holds layers c, c + 16, c + 32, c + 48 and the moments' MFMA q
+// (the float4's component q) covers the contiguous layers 16q .. 16q + 15.
+__device__ inline int mom_pos(int l) { return 4 * (l & 15) + (l >> 4); }
+
+// G in Float32, scaled by 2^-e per (star, layer), as [layer block][global point][64 positions]
+// (mom_pos) with MOM_PAD points after the last (layers past L and points outside every star
+// zero): the moments' A operand, 1 KB per 4 points.  Also the wavelengths in units of DELTA_UNIT on the same global
 // points (zero outside the stars).
 __global__ __launch_bounds__(256) void k_g_pack(int n_stars, int L, int n_lblocks, int64_t n_total,
                                                 const StarDev* __restrict__ stars,
@@ -3646,7 +3708,7 @@ __global__ __launch_bounds__(256) void k_g_pack(int n_stars, int L, int n_lblock
     for (int b = 0; b < n_lblocks; ++b) {
         const int l = b * WAVE + lane;
         const int e = l < L ? gexp[(int64_t)st * L + l] : 0;
-        float* __restrict__ dst = g32 + (int64_t)b * stride + lane;
+        float* __restrict__ dst = g32 + (int64_t)b * stride + mom_pos(lane);
         for (int64_t p = 0; p < np; ++p) {
             const int64_t jl = jl0 + p;
             dst[(j0 + p) * WAVE] = l < L && jl < S.n_wl ? (float)ldexp(G[l + (int64_t)L * jl], -e) : 0.0f;
@@ -3697,7 +3759,7 @@ __global__ __launch_bounds__(WAVE) void k_g_ffmom(int n_stars, int L, int n_lblo
             }
         }
     }
-    float* __restrict__ dst = gff + ((int64_t)lblk * n_sub + g) * FF_TERMS * WAVE + lane;
+    float* __restrict__ dst = gff + ((int64_t)lblk * n_sub + g) * FF_TERMS * WAVE + mom_pos(lane);
 #pragma unroll
     for (int m = 0; m < FF_TERMS; ++m) dst[m * WAVE] = l < L ? (float)ldexp(acc[m], -e) : 0.0f;
 }
@@ -3708,6 +3770,7 @@ struct MomArgs {
     const StarDev* stars;
     const LineMeta* meta;
     const int2* azone;         // the adjoint's near zones
+    const int4* azq;           // each 16-layer block's (null: the line's)
     const int* adj_line;       // the lines with a far zone, by class (ascending width)
     const int* adj_start;      // [ADJ_NC + 1]; adj_start[ADJ_NC] = the number of such lines
     const int* groups;         // the groups' positions in adj_line, in line order
@@ -3727,12 +3790,15 @@ struct MomArgs {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // One wavefront per (group of ADJ_GROUP lines, 64-layer block), widest classes first.  Lane i of a
-// K step: A = scaled G of layers 4 (i % 16) + m (m = 0..3, one MFMA each) at point j0 + i / 16;
-// B = R^(p+1) of line slot (i % 16) / 4 at that point, p = i % 4 (zero outside the line's far
-// zone: its window less its adjoint near zone).  The lines of a group are handled star by star (a
-// group can straddle two stars), each pass over the union of its lines' windows, then (far field)
-// over the union of their far-field runs.  About 105 VGPRs and 20 AGPRs: four waves per SIMD.
-__global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
+// K step: A = scaled G of layers 16 m + i % 16 (m = 0..3, one MFMA each: the operands are packed
+// in mom_pos order) at point j0 + i / 16; B = R^(p+1) of line slot (i % 16) / 4 at that point,
+// p = i % 4, zero outside the line's far zone: its window less its adjoint near zone -- for MFMA m
+// the near zone of the line's 16-layer block m (azq), inside which the block's layers are
+// evaluated exactly, so that a block whose own radius is narrower than the line's takes the series
+// over more of the window.  The lines of a group are handled star by star (a group can straddle
+// two stars), each pass over the union of its lines' windows, then (far field) over the union of
+// their far-field runs.  Four waves per SIMD (128 VGPRs, a few spilled: faster than three).
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_vjp_moments(MomArgs A) {
     const int lane = threadIdx.x;
     if (A.counters[3] != 0) return;  // failed setup, or a capacity shortfall
     const int64_t n_items = (int64_t)A.n_groups[0] * A.n_lblocks;
@@ -3765,16 +3831,18 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
         const int64_t p0 = A.groups[g];
         // the group's lines: up to ADJ_GROUP from p0, within the leader's class
         const int l0 = A.adj_line[p0];
-        const int64_t cend = A.adj_start[adj_class(A.meta[l0], A.azone[l0]) + 1];
+        const int64_t cend = A.adj_start[adj_class(A.meta[l0], azone_in(A.azone[l0], A.azq, l0)) + 1];
         const int nl = (int)min((int64_t)ADJ_GROUP, cend - p0);
         int line = -1, st = -1;
         LineMeta m = {0.0, 1, 0, 1, 0, 0, ADJ_NC, 1, 0};
         int2 z = make_int2(1, 0);
+        int4 zq = make_int4(0, 0, 0, 0);  // the line's 16-layer blocks' zones (azq)
         int4 fr = make_int4(1, 0, 1, 0);  // the line's far-field runs (empty: none)
         if (slot < nl) {
             line = A.adj_line[p0 + slot];
             m = A.meta[line];
             z = A.azone[line];
+            if (A.azq) zq = A.azq[line];
             st = star_of_line(A.stars, A.n_stars, line);
             if (A.ffrun) fr = A.ffrun[line];
         }
@@ -3795,7 +3863,27 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
             // the lane's far zone [lb, clo) and (chi, ub] as bounds on the step's first point j0
             // (the lane's point is j0 + kk); not in this pass: no points
             const int lbr = (act ? m.lb : 1) - kk, ubr = (act ? m.ub : 0) - kk;
-            const int clor = z.x - kk, chir = z.y - kk;
+            // the innermost of the line's block zones (outside it some block takes the series), and
+            // each block's zone: MFMA q's rows are the block's 16 layers, whose B is zero inside it
+            int2 zb[4];
+            zb[0] = azone_blk(z, zq.x);
+            zb[1] = azone_blk(z, zq.y);
+            zb[2] = azone_blk(z, zq.z);
+            zb[3] = azone_blk(z, zq.w);
+            int ilo = z.x, ihi = z.y;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ilo = max(ilo, zb[q].x);
+                ihi = min(ihi, zb[q].y);
+            }
+            const int clor = ilo - kk, chir = ihi - kk;
+            int qlo[4], qhi[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                qlo[q] = zb[q].x - kk;
+                qhi[q] = zb[q].y - kk;
+            }
+            const bool blocks = A.azq != nullptr;
             // Float32 sums over chunks of MOM_CHUNK K steps (one 64-point sub-tile), added into
             // Float64 totals: a far zone spans thousands of points and its moments enter
             // differences (core against wing) in the gradients of the damping.  Chunks end at fixed
@@ -3827,8 +3915,8 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
                 const bool in = k < nl && ((pass >> k) & 1);
                 wlo[k] = in ? readlane_i32(m.lb, 4 * k) : 1;
                 whi[k] = in ? readlane_i32(m.ub, 4 * k) : 0;
-                nlo[k] = readlane_i32(z.x, 4 * k);
-                nhi[k] = readlane_i32(z.y, 4 * k);
+                nlo[k] = readlane_i32(ilo, 4 * k);
+                nhi[k] = readlane_i32(ihi, 4 * k);
             }
             // K steps start at multiples of 4 (a star's grid starts at one): every point falls in
             // the same step whatever lines share the group, and steps outside a line's far zone add
@@ -3881,10 +3969,16 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
                         R = farv[u] ? R : 0.0f;
                         const float R2 = R * R;
                         const float b = ((pw & 1) ? R2 : R) * ((pw & 2) ? R2 : 1.0f);
-                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b, acc[0], 0, 0, 0);
-                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b, acc[1], 0, 0, 0);
-                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, b, acc[2], 0, 0, 0);
-                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, b, acc[3], 0, 0, 0);
+                        float bq[4] = {b, b, b, b};
+                        if (blocks) {  // wavefront-uniform
+                            const int j0 = jb + 4 * u;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) bq[q] = (j0 < qlo[q] || j0 > qhi[q]) ? b : 0.0f;
+                        }
+                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, bq[0], acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, bq[1], acc[1], 0, 0, 0);
+                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, bq[2], acc[2], 0, 0, 0);
+                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, bq[3], acc[3], 0, 0, 0);
                     }
                 }
 #pragma unroll
@@ -3943,12 +4037,12 @@ __global__ __launch_bounds__(WAVE) void k_vjp_moments(MomArgs A) {
             }
             if (act) {
                 // accumulator layout of v_mfma_f32_16x16x4_f32: lane i, register r holds row
-                // 4 (i / 16) + r, column i % 16; row i of MFMA m is layer 4 i + m of the block
+                // 4 (i / 16) + r, column i % 16; row i of MFMA m is layer 16 m + i of the block
                 float* __restrict__ dst = (float*)(A.mom + ((int64_t)line * A.n_lblocks + lblk) * WAVE);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) dst[(4 * (4 * kk + r) + q) * 4 + pw] = (float)tot[q][r];
+                    for (int r = 0; r < 4; ++r) dst[(16 * q + 4 * kk + r) * 4 + pw] = (float)tot[q][r];
             }
         }
     }
@@ -4559,6 +4653,7 @@ struct Workspace {
     DevBuf ffrun, ffkey;  // far field: each line's runs (line order), list keys (partition order)
     DevBuf ffrun_adj, gff;  // the adjoint's far field: runs clear of its near zone, moments of G
     DevBuf lpre, lwin;      // the setup's per-line records (k_setup_pre, k_setup)
+    DevBuf azq;             // Float32 fit step: each 16-layer block's adjoint near zone
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -4576,7 +4671,7 @@ struct Workspace {
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
                           &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey,
-                          &ffrun_adj, &gff,      &lpre,      &lwin};
+                          &ffrun_adj, &gff,      &lpre,      &lwin,      &azq};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -4590,7 +4685,7 @@ struct Workspace {
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
                           &scan_part, &jvp_coef, &nbuf,      &ffgeo,     &fft,       &ffrun,
-                          &ffkey,     &ffrun_adj, &gff,      &lpre,      &lwin};
+                          &ffkey,     &ffrun_adj, &gff,      &lpre,      &lwin,      &azq};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -5041,6 +5136,7 @@ int enqueue_setup(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, b
     sa.recM64 = b.s64 ? (double4*)ws.recM.p : nullptr;
     sa.counters = counters;
     sa.azone = b.adj ? (int2*)ws.azone.p : nullptr;
+    sa.azq = b.adj && b.L <= WAVE ? (int4*)ws.azq.p : nullptr;
     sa.geo = b.ff ? (const FfGeo*)ws.ffgeo.p : nullptr;
     sa.n_sub = b.n_sub;
     sa.ffrun = b.ff ? (int4*)ws.ffrun.p : nullptr;
@@ -5555,6 +5651,7 @@ int enqueue_vjp(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s, int
     va.counters = (const unsigned long long*)ws.counters.p;
     va.n_lblocks = b.n_lblocks;
     va.azone = (const int2*)ws.azone.p;
+    va.azq = b.L <= WAVE ? (const int4*)ws.azq.p : nullptr;
     va.mom = (const float4*)ws.recM.p;
     va.gexp = (const int*)ws.gexp.p;
     const int64_t grid = (int64_t)ctx->n_cu * 16;
@@ -5654,6 +5751,7 @@ int prepare_fit(korg_ctx* ctx, Workspace& ws, Batch& b) {
         TRY(grow(ctx, ws.adj_start, sizeof(int) * (ADJ_NC + 1)));
         TRY(grow(ctx, ws.adj_line, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
         TRY(grow(ctx, ws.azone, sizeof(int2) * (size_t)(b.N > 0 ? b.N : 1)));
+        TRY(grow(ctx, ws.azq, sizeof(int4) * (size_t)(b.N > 0 ? b.N : 1)));
         TRY(grow(ctx, ws.adj_lead, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
         TRY(grow(ctx, ws.adj_groups, sizeof(int) * (size_t)(b.N > 0 ? b.N : 1)));
         TRY(grow(ctx, ws.adj_gstart, sizeof(int) * 2));
@@ -5682,12 +5780,13 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
         hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.n_sub * b.n_lblocks)), dim3(WAVE), 0, s,
                            b.n_stars, b.L, b.n_lblocks, b.n_sub, (const StarDev*)ws.stars.p,
                            (const FfGeo*)ws.ffgeo.p, (const int*)ws.gexp.p, (float*)ws.gff.p);
+    const int4* azq = b.L <= WAVE ? (const int4*)ws.azq.p : nullptr;
     hipLaunchKernelGGL(k_adj_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
-                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (int*)ws.adj_blk.p);
+                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, azq, (int*)ws.adj_blk.p);
     hipLaunchKernelGGL(k_scan<ADJ_NC>, dim3(1), dim3(1024), 0, s, b.nblocks, (int*)ws.adj_blk.p,
                        (int*)ws.adj_start.p);
     hipLaunchKernelGGL(k_adj_scatter, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
-                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, (const int*)ws.adj_blk.p,
+                       (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, azq, (const int*)ws.adj_blk.p,
                        (const int*)ws.adj_start.p, (int*)ws.adj_line.p, (int*)ws.adj_lead.p);
     hipLaunchKernelGGL(k_lead_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
                        (const int*)ws.adj_lead.p, (int*)ws.adj_blk.p);
@@ -5703,6 +5802,7 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     ma.stars = (const StarDev*)ws.stars.p;
     ma.meta = (const LineMeta*)ws.meta.p;
     ma.azone = (const int2*)ws.azone.p;
+    ma.azq = b.L <= WAVE ? (const int4*)ws.azq.p : nullptr;
     ma.adj_line = (const int*)ws.adj_line.p;
     ma.adj_start = (const int*)ws.adj_start.p;
     ma.groups = (const int*)ws.adj_groups.p;
