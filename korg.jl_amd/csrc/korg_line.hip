@@ -383,6 +383,35 @@ __global__ __launch_bounds__(256) void k_ff_geo(int64_t n_sub, int n_stars,
     geo[g] = o;
 }
 
+// The geometry of each super-tile (FF_NS whole sub-tiles of one star; first > last when the
+// super-tile is not whole: the grid's end, or padding), for the adjoint's far-field steps over
+// whole super-tiles.
+__global__ __launch_bounds__(256) void k_ff_sgeo(int64_t n_super, int64_t n_sub, int n_stars,
+                                                 const StarDev* __restrict__ stars,
+                                                 const FfGeo* __restrict__ geo,
+                                                 FfGeo* __restrict__ sgeo) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_super) return;
+    FfGeo o = {0.0, 0.0, 0.0, 0.0f, 1, 0, 0};
+    bool whole = (s + 1) * FF_NS <= n_sub;
+    for (int k = 0; k < FF_NS && whole; ++k) {
+        const FfGeo& g = geo[s * FF_NS + k];
+        whole = g.first <= g.last && g.last - g.first + 1 == FF_SUB;
+    }
+    if (whole) {
+        const int64_t first = geo[s * FF_NS].first, last = geo[s * FF_NS + FF_NS - 1].last;
+        const StarDev& S = stars[star_of_point(stars, n_stars, first)];
+        const double a = S.wls[first - S.grid_off], b = S.wls[last - S.grid_off];
+        o.c_cm = 0.5 * (a + b);
+        o.h_cm = 0.5 * (b - a);
+        o.c_u = o.c_cm * (1.0 / DELTA_UNIT);
+        o.h_u = (float)(o.h_cm * (1.0 / DELTA_UNIT));
+        o.first = (int)first;
+        o.last = (int)last;
+    }
+    sgeo[s] = o;
+}
+
 // --------------------------------------------------------------------------------------------
 // k_setup: one wavefront per line; lane l handles layers l, l+64, ...
 // --------------------------------------------------------------------------------------------
@@ -3785,6 +3814,11 @@ struct MomArgs {
     const FfGeo* geo;
     const float* gff;
     int64_t n_sub;
+    // the same over whole super-tiles (FF_NS sub-tiles), for lines at least 8 super-tile
+    // half-widths away
+    const FfGeo* sgeo;
+    const float* gffs;
+    int64_t n_super;
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -3989,34 +4023,31 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
                 }
             }
             if (A.ffrun) {
-                // The far-field steps, sub-tile by sub-tile in ascending order (after every direct
-                // chunk: a line's Float64 totals add the same values in the same order in any
-                // group): over a sub-tile g in the line's runs, sum_j G_j R_j^k = sum_m W_km g_m with
-                // W_km = binom(-2k, m) u^m R0^k, R0 = 1/d^2, u = h/d, d = c - lam0; K = 4 terms
-                // per step, A = the sub-tile's moments of G (k_g_ffmom), three steps per sub-tile.
+                // The far-field steps in ascending order (after every direct chunk: a line's
+                // Float64 totals add the same values in the same order in any group): over a span
+                // of points (a sub-tile, or a whole super-tile of FF_NS) in the line's runs,
+                // sum_j G_j R_j^k = sum_m W_km g_m with W_km = binom(-2k, m) u^m R0^k, R0 = 1/d^2,
+                // u = h/d, d = c - lam0; K = 4 terms per MFMA, A = the span's moments of G
+                // (k_g_ffmom), three MFMAs per layer quarter.  A line whose runs cover a whole
+                // super-tile at least FF_RATIO of its half-widths from the centre takes it in one
+                // step (the same truncation bound as a sub-tile's) and adds zeros in the sub-tile
+                // steps there; the others take the sub-tile steps.
                 int glo = INT_MAX, ghi = INT_MIN;
 #pragma unroll
                 for (int k = 0; k < ADJ_GROUP; ++k) {
                     if (rx[k] <= ry[k]) { glo = min(glo, rx[k]); ghi = max(ghi, ry[k]); }
                     if (rz[k] <= rw[k]) { glo = min(glo, rz[k]); ghi = max(ghi, rw[k]); }
                 }
-                for (int gs = glo; gs <= ghi; ++gs) {
-                    bool any = false;
-#pragma unroll
-                    for (int k = 0; k < ADJ_GROUP; ++k)
-                        any = any || (gs >= rx[k] && gs <= ry[k]) || (gs >= rz[k] && gs <= rw[k]);
-                    if (!any) continue;  // wavefront-uniform
-                    const double cu = geo[gs].c_u;
-                    const float hu = geo[gs].h_u;
+                // one step over a span with centre cu, half-width hu and moments gq (zeros for lanes
+                // that are not on), flushed into the totals; acc is zero on entry (every direct
+                // chunk ends by clearing it)
+                auto ff_step = [&](double cu, float hu, bool on, const float4* __restrict__ gq) {
                     float x = __builtin_amdgcn_rcpf((float)(cu - lam0s));
-                    x = ff_in(lfr, gs) ? x : 0.0f;
+                    x = on ? x : 0.0f;
                     const float R0 = x * x, R2 = R0 * R0;
                     const float Rk = ((pw & 1) ? R2 : R0) * ((pw & 2) ? R2 : 1.0f);
                     const float u = hu * x, u2 = u * u, u4 = u2 * u2;
                     float w = Rk * ((kk & 1) ? u : 1.0f) * ((kk & 2) ? u2 : 1.0f);
-                    const float4* __restrict__ gq =
-                        (const float4*)(A.gff + ((int64_t)lblk * A.n_sub + gs) * FF_TERMS * WAVE);
-                    // (acc is zero here: every direct chunk ends by clearing it)
 #pragma unroll
                     for (int t = 0; t < FF_TERMS / 4; ++t) {
                         const float4 a4 = gq[t * WAVE + lane];  // terms 4t + kk, layers 4 col ..
@@ -4032,6 +4063,30 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
 #pragma unroll
                         for (int r = 0; r < 4; ++r) tot[q][r] += (double)acc[q][r];
                         acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+                };
+                const KCONST FfGeo* sgeo = uniform_ptr(A.sgeo);
+                for (int sp = glo / FF_NS; sp <= ghi / FF_NS; ++sp) {
+                    const int s0 = sp * FF_NS, s1 = s0 + FF_NS - 1;
+                    bool le = false;
+                    const double scu = sgeo[sp].c_u;
+                    const float shu = sgeo[sp].h_u;
+                    if (sgeo[sp].first <= sgeo[sp].last)  // wavefront-uniform: a whole super-tile
+                        le = ((s0 >= lfr.x && s1 <= lfr.y) || (s0 >= lfr.z && s1 <= lfr.w)) &&
+                             fabs(scu - lam0s) >= FF_RATIO * (double)shu;
+                    const uint64_t sb = ballot(le);  // lane 4k holds slot k
+                    if (sb)
+                        ff_step(scu, shu, le,
+                                (const float4*)(A.gffs + ((int64_t)lblk * A.n_super + sp) * FF_TERMS * WAVE));
+                    for (int gs = max(s0, glo); gs <= min(s1, ghi); ++gs) {
+                        bool any = false;
+#pragma unroll
+                        for (int k = 0; k < ADJ_GROUP; ++k)
+                            any = any || (!((sb >> (4 * k)) & 1) &&
+                                          ((gs >= rx[k] && gs <= ry[k]) || (gs >= rz[k] && gs <= rw[k])));
+                        if (!any) continue;  // wavefront-uniform
+                        ff_step(geo[gs].c_u, geo[gs].h_u, ff_in(lfr, gs) && !le,
+                                (const float4*)(A.gff + ((int64_t)lblk * A.n_sub + gs) * FF_TERMS * WAVE));
                     }
                 }
             }
@@ -4654,6 +4709,7 @@ struct Workspace {
     DevBuf ffrun_adj, gff;  // the adjoint's far field: runs clear of its near zone, moments of G
     DevBuf lpre, lwin;      // the setup's per-line records (k_setup_pre, k_setup)
     DevBuf azq;             // Float32 fit step: each 16-layer block's adjoint near zone
+    DevBuf gffs, ffsgeo;    // the adjoint's far field over whole super-tiles: G's moments, geometry
     DevBuf meta, rec64, recM, tab, counters, blk, cstart, cls_cand, cls_jc, ranges, nblk, ncstart,
         near_cand, near_jc, near_ranges, cls_rng, cls_line, near_rng, near_line, tc_count,
         tc_start, tile_lines, stars;
@@ -4671,7 +4727,8 @@ struct Workspace {
                           &near_cand, &near_jc,  &near_ranges, &cls_rng,  &cls_line,  &near_rng,
                           &near_line, &tc_count, &tc_start,  &tile_lines, &stars,     &scan_part,
                           &nbuf,      &ffgeo,    &fft,       &ffrun,      &ffkey,
-                          &ffrun_adj, &gff,      &lpre,      &lwin,      &azq};
+                          &ffrun_adj, &gff,      &lpre,      &lwin,      &azq,
+                          &gffs,      &ffsgeo};
         for (DevBuf* b : bufs) *b = DevBuf{};
         tile_cap = 0;
         seg_cap = 0;
@@ -4685,7 +4742,8 @@ struct Workspace {
                           &chi2_part, &adj_blk,  &adj_start, &adj_line,  &gexp,       &g32,
                           &lam_s,     &azone,    &adj_lead,  &adj_groups, &adj_gstart,
                           &scan_part, &jvp_coef, &nbuf,      &ffgeo,     &fft,       &ffrun,
-                          &ffkey,     &ffrun_adj, &gff,      &lpre,      &lwin,      &azq};
+                          &ffkey,     &ffrun_adj, &gff,      &lpre,      &lwin,      &azq,
+                          &gffs,      &ffsgeo};
         for (DevBuf* b : bufs) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -5761,6 +5819,9 @@ int prepare_fit(korg_ctx* ctx, Workspace& ws, Batch& b) {
             TRY(grow(ctx, ws.ffrun_adj, sizeof(int4) * (size_t)(b.N > 0 ? b.N : 1)));
             TRY(grow(ctx, ws.gff, sizeof(float) * WAVE * FF_TERMS * (size_t)b.n_lblocks *
                                       (size_t)(b.n_sub > 0 ? b.n_sub : 1)));
+            TRY(grow(ctx, ws.gffs, sizeof(float) * WAVE * FF_TERMS * (size_t)b.n_lblocks *
+                                       (size_t)(b.nsuper > 0 ? b.nsuper : 1)));
+            TRY(grow(ctx, ws.ffsgeo, sizeof(FfGeo) * (size_t)(b.nsuper > 0 ? b.nsuper : 1)));
         }
     }
     return KORG_OK;
@@ -5776,10 +5837,18 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     hipLaunchKernelGGL(k_g_pack, dim3(gblocks), dim3(256), 0, s, b.n_stars, b.L, b.n_lblocks, b.nwl,
                        (const StarDev*)ws.stars.p, (const int*)ws.gexp.p, (float*)ws.g32.p,
                        (double*)ws.lam_s.p);
-    if (b.ff && b.n_sub > 0)  // the sub-tiles' moments of G for the far-field steps
+    if (b.ff && b.n_sub > 0) {
+        // the sub-tiles' and the whole super-tiles' moments of G for the far-field steps
         hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.n_sub * b.n_lblocks)), dim3(WAVE), 0, s,
                            b.n_stars, b.L, b.n_lblocks, b.n_sub, (const StarDev*)ws.stars.p,
                            (const FfGeo*)ws.ffgeo.p, (const int*)ws.gexp.p, (float*)ws.gff.p);
+        hipLaunchKernelGGL(k_ff_sgeo, dim3((unsigned)((b.nsuper + 255) / 256)), dim3(256), 0, s,
+                           (int64_t)b.nsuper, b.n_sub, b.n_stars, (const StarDev*)ws.stars.p,
+                           (const FfGeo*)ws.ffgeo.p, (FfGeo*)ws.ffsgeo.p);
+        hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.nsuper * b.n_lblocks)), dim3(WAVE), 0, s,
+                           b.n_stars, b.L, b.n_lblocks, (int64_t)b.nsuper, (const StarDev*)ws.stars.p,
+                           (const FfGeo*)ws.ffsgeo.p, (const int*)ws.gexp.p, (float*)ws.gffs.p);
+    }
     const int4* azq = b.L <= WAVE ? (const int4*)ws.azq.p : nullptr;
     hipLaunchKernelGGL(k_adj_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
                        (const LineMeta*)ws.meta.p, (const int2*)ws.azone.p, azq, (int*)ws.adj_blk.p);
@@ -5815,6 +5884,9 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
     ma.geo = (const FfGeo*)ws.ffgeo.p;
     ma.gff = (const float*)ws.gff.p;
     ma.n_sub = b.n_sub;
+    ma.sgeo = (const FfGeo*)ws.ffsgeo.p;
+    ma.gffs = (const float*)ws.gffs.p;
+    ma.n_super = b.nsuper;
     hipLaunchKernelGGL(k_vjp_moments, dim3((unsigned)(ctx->n_cu * 32)), dim3(WAVE), 0, s, ma);
     HIP_TRY(ctx, hipGetLastError());
     return KORG_OK;
