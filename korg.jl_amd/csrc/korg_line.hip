@@ -474,12 +474,6 @@ __device__ inline double root_up(double x, float p) {
     return (double)__builtin_amdgcn_exp2f(lg * p) * 1.00001;
 }
 
-// Continuum at x for layer l on the coarse grid: linear interpolation between the bracketing
-// knots (index found once per line by the caller).
-__device__ inline double cntm_lerp(const double* knots, const double* vals, int64_t i, double x) {
-    const double t = (x - knots[i]) / (knots[i + 1] - knots[i]);
-    return (1.0 - t) * vals[i] + t * vals[i + 1];
-}
 
 // k_setup_pre: one lane per line.  Also flags lines out of order and species out of range (such
 // a line gets an empty LineMeta here and is skipped by the later kernels).
@@ -584,7 +578,7 @@ __global__ __launch_bounds__(256) KORG_SETUP_ATTR void k_setup(SetupArgs A) {
             // src/line_absorption.jl:108-112
             const double* cv = S.cntm + (int64_t)l * S.n_knots;
             const double cn = (S.cntm_mode == KORG_CNTM_GRID)
-                                  ? (1.0 - kt) * cv[kidx] + kt * cv[kidx + 1]  // cntm_lerp, t = kt
+                                  ? (1.0 - kt) * cv[kidx] + kt * cv[kidx + 1]  // linear between the knots, weight kt
                                   : S.cntm[li * L + l];
             const double rho = cn * S.cutoff / amp;
             const double dg = inverse_gaussian_density(rho, sigma);
@@ -3265,12 +3259,6 @@ __global__ __launch_bounds__(WAVE) void k_jvp_profile(JvpArgs A) {
 // normalized flux, chi^2 = sum ((F / F_c - data) / sigma)^2, and by a reverse sweep over the layers
 // G = dchi^2/dalpha_lines, which the line adjoint consumes.  Float64.
 // ---------------------------------------------------------------------------------------------
-__device__ inline double blackbody64(double T, double lam) {
-    const double h = hplanck_cgs, c = c_cgs, k = kboltz_cgs;
-    const double l2 = lam * lam;
-    return 2 * h * c * c / (l2 * l2 * lam) / expm1(h * c / lam / k / T);
-}
-
 // Korg's E2 pieces (src/RadiativeTransfer/RadiativeTransfer.jl:427-480): polynomials of degree 5
 // in y = x - n about n = 2..8, their coefficients as published there.
 __constant__ double E2_POLY[7][6] = {
@@ -3362,8 +3350,9 @@ __global__ void k_zero_alpha(int n_stars, int L, int64_t n_total, const StarDev*
 // Each star's alpha set to its continuum on the grid (KORG_CNTM_GRID): per layer, the linear
 // interpolation of the knots' values at every grid point -- the alpha that synthesize starts the
 // lines from (src/synthesize.jl:225-226, 248-250), with the interpolation the window search uses
-// (cntm_lerp; outside the knots, the end intervals extended).  A wavefront per 64 points (no chunk
-// straddles two stars: the grids are padded to 512 points): lane p finds point p's knot interval
+// (linear between the bracketing knots; outside the knots, the end intervals extended).  A
+// wavefront per 64 points (no chunk straddles two stars: the grids are padded to 512 points):
+// lane p finds point p's knot interval
 // and weight, then the chunk's alpha -- one contiguous run of 64 L elements -- is written with
 // consecutive lanes on consecutive elements (coalesced stores), each element taking its point's
 // interval from the lane that found it.
@@ -3377,7 +3366,7 @@ __global__ __launch_bounds__(256) void k_cntm_fill(int n_stars, int L, int64_t n
     const int np = (int)(min(jl0 + WAVE, S.n_wl) - jl0);
     if (np <= 0) return;  // padding between two stars' grids
     // lane p: point jl0 + p's knot interval (last knot <= lambda, within [0, n_knots - 2]) and
-    // its interpolation weight, as cntm_lerp computes it
+    // its interpolation weight (x - k_i) / (k_(i+1) - k_i)
     int kidx;
     double tw;
     {
@@ -3426,11 +3415,6 @@ struct RtArgs {
     int flux_only;
 };
 
-// alpha_cntm of layer l at the point (continuum interpolated on the star's knots)
-__device__ inline double cntm_at(const KCONST StarDev& S, int l, int64_t kidx, double x) {
-    return cntm_lerp(S.knots, S.cntm + (int64_t)l * S.n_knots, kidx, x);
-}
-
 __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     const int L = A.L;
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3456,6 +3440,17 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
         }
         kidx = lo > S.n_knots - 2 ? S.n_knots - 2 : lo;
     }
+    // the continuum's knot weight and Planck's factors, once per point (FP64 divisions are long
+    // sequences): B_lambda(T) = c1 / expm1(c2 / T) with c1 = 2 h c^2 / lam^5, c2 = h c / (k lam)
+    const double ckt = (lam - S.knots[kidx]) / (S.knots[kidx + 1] - S.knots[kidx]);
+    auto cntm_pt = [&](int l) {
+        const double* v = S.cntm + (int64_t)l * S.n_knots + kidx;
+        return (1.0 - ckt) * v[0] + ckt * v[1];
+    };
+    const double il = 1.0 / lam, il2 = il * il;
+    const double bb1 = (2 * hplanck_cgs * c_cgs * c_cgs) * (il2 * il2 * il);
+    const double bb2 = (hplanck_cgs * c_cgs / kboltz_cgs) * il;
+    auto planck = [&](int l) { return bb1 / expm1(bb2 / S.temps[l]); };
     auto alpha_lines = [&](int l) {
         const int64_t k = (int64_t)l + (int64_t)L * jl;
         return A.alpha_f32 ? (double)((const float*)S.alpha)[k] : ((const double*)S.alpha)[k];
@@ -3463,35 +3458,36 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     // the total absorption of layer l, given its continuum ac
     auto alpha_total = [&](int l, double ac) { return A.flux_only ? alpha_lines(l) : ac + alpha_lines(l); };
     // forward: flux of the total and of the continuum alone
+    // (F and Fc are 6 times the sums of the intervals' integrals until the end)
     double F = 0.0, Fc = 0.0;
     double tau0 = 0.0, tc0 = 0.0;
-    double ac0 = cntm_at(S, 0, kidx, lam);
+    double ac0 = cntm_pt(0);
     double u0 = alpha_total(0, ac0) * tb[0].f, uc0 = ac0 * tb[0].f;
-    double S0 = blackbody64(S.temps[0], lam);
+    double S0 = planck(0);
     double E0 = 1.0, x0 = 1.0, Ec0 = 1.0, xc0 = 1.0;  // E2 and e^{-tau} at tau = 0
     for (int i = 1; i < L; ++i) {
-        const double ac = cntm_at(S, i, kidx, lam);
+        const double ac = cntm_pt(i);
         const double u1 = alpha_total(i, ac) * tb[i].f, uc1 = ac * tb[i].f;
         const double dl = tb[i].ltau - tb[i - 1].ltau;
         const double tau1 = tau0 + 0.5 * (u1 + u0) * dl, tc1 = tc0 + 0.5 * (uc1 + uc0) * dl;
-        const double S1 = blackbody64(S.temps[i], lam);
+        const double S1 = planck(i);
         const double E1 = expint2_korg(tau1, nullptr), x1 = exp(-tau1);
         const double Ec1 = expint2_korg(tc1, nullptr), xc1 = exp(-tc1);
         {
             const double m = (S1 - S0) / (tau1 - tau0), b = S0 - m * tau0;
-            F += (tau1 * E1 * (3 * b + 2 * m * tau1) - x1 * (3 * b + 2 * m * (tau1 + 1))) / 6 -
-                 (tau0 * E0 * (3 * b + 2 * m * tau0) - x0 * (3 * b + 2 * m * (tau0 + 1))) / 6;
+            F += (tau1 * E1 * (3 * b + 2 * m * tau1) - x1 * (3 * b + 2 * m * (tau1 + 1))) -
+                 (tau0 * E0 * (3 * b + 2 * m * tau0) - x0 * (3 * b + 2 * m * (tau0 + 1)));
         }
         {
             const double m = (S1 - S0) / (tc1 - tc0), b = S0 - m * tc0;
-            Fc += (tc1 * Ec1 * (3 * b + 2 * m * tc1) - xc1 * (3 * b + 2 * m * (tc1 + 1))) / 6 -
-                  (tc0 * Ec0 * (3 * b + 2 * m * tc0) - xc0 * (3 * b + 2 * m * (tc0 + 1))) / 6;
+            Fc += (tc1 * Ec1 * (3 * b + 2 * m * tc1) - xc1 * (3 * b + 2 * m * (tc1 + 1))) -
+                  (tc0 * Ec0 * (3 * b + 2 * m * tc0) - xc0 * (3 * b + 2 * m * (tc0 + 1)));
         }
         tau0 = tau1; tc0 = tc1; u0 = u1; uc0 = uc1; S0 = S1;
         E0 = E1; x0 = x1; Ec0 = Ec1; xc0 = xc1;
     }
-    F *= 2 * PI;
-    Fc *= 2 * PI;
+    F *= 2 * PI / 6;
+    Fc *= 2 * PI / 6;
     if (A.flux_only) {
         // in erg/s/cm^4/A, as synthesize returns them (src/synthesize.jl:314-318)
         if (S.flux) S.flux[jl] = F * 1e-8;
@@ -3504,7 +3500,8 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     A.chi2_part[j] = r * r;
     if (S.flux) S.flux[jl] = Fn;
     // reverse: G[l] = dchi^2/dalpha_total[l] = dchi^2/dalpha_lines[l]
-    const double w = 2 * PI * (2 * r / sg / Fc);  // 2 pi dchi^2/dF
+    // 2 pi dchi^2/dF, over 6: the partials below are 6 times those of the integrals
+    const double w = (2 * PI / 6) * (2 * r / sg / Fc);
     double* G = (double*)S.G + (int64_t)L * jl;
     // walk the intervals from the deepest up, recomputing tau_i from tau_{i+1}
     double tau1 = tau0, u1 = u0, S1 = S0;
@@ -3513,28 +3510,28 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
     double pending = 0.0;  // contribution to gtau[i + 1] from interval i + 1 (already walked)
     double sfx = 0.0;      // sum of gtau[k] for k >= i + 2
     for (int i = L - 2; i >= 0; --i) {
-        const double ac = cntm_at(S, i, kidx, lam);
+        const double ac = cntm_pt(i);
         const double u_i = (ac + alpha_lines(i)) * tb[i].f;
         const double dl = tb[i + 1].ltau - tb[i].ltau;
         const double tau_i = i == 0 ? 0.0 : tau1 - 0.5 * (u1 + u_i) * dl;
-        const double S_i = blackbody64(S.temps[i], lam);
+        const double S_i = planck(i);
         double dE_i;
         const double E_i = expint2_korg(tau_i, i == 0 ? nullptr : &dE_i), x_i = exp(-tau_i);
         if (i == 0) dE_i = 0.0;  // tau_0 = 0 is fixed: its derivative is never used
-        const double dt = tau1 - tau_i;
-        const double m = (S1 - S_i) / dt, b = S_i - m * tau_i;
-        // partials of core(tau, m, b) = (tau E (3b + 2m tau) - e^{-tau} (3b + 2m (tau + 1))) / 6
+        const double idt = 1.0 / (tau1 - tau_i);
+        const double m = (S1 - S_i) * idt, b = S_i - m * tau_i;
+        // partials of 6 core(tau, m, b) = tau E (3b + 2m tau) - e^{-tau} (3b + 2m (tau + 1))
         auto partials = [&](double t, double E, double dE, double x, double& ct, double& cm, double& cb) {
             const double Aq = 3 * b + 2 * m * t, Bq = 3 * b + 2 * m * (t + 1);
-            ct = ((E + t * dE) * Aq + t * E * 2 * m + x * Bq - x * 2 * m) / 6;
-            cm = (t * E * 2 * t - x * 2 * (t + 1)) / 6;
-            cb = (t * E * 3 - x * 3) / 6;
+            ct = (E + t * dE) * Aq + t * E * 2 * m + x * Bq - x * 2 * m;
+            cm = t * E * 2 * t - x * 2 * (t + 1);
+            cb = t * E * 3 - x * 3;
         };
         double ct1, cm1, cb1, ct0, cm0, cb0;
         partials(tau1, E1, dE1, x1, ct1, cm1, cb1);
         partials(tau_i, E_i, dE_i, x_i, ct0, cm0, cb0);
         const double Tm = cm1 - cm0, Tb = cb1 - cb0;
-        const double dm_dt1 = -m / dt, dm_dt0 = m / dt;
+        const double dm_dt1 = -m * idt, dm_dt0 = m * idt;
         const double db_dt1 = -tau_i * dm_dt1, db_dt0 = -m - tau_i * dm_dt0;
         const double g_up = w * (ct1 + Tm * dm_dt1 + Tb * db_dt1) + pending;  // gtau[i + 1], final
         pending = w * (-ct0 + Tm * dm_dt0 + Tb * db_dt0);
