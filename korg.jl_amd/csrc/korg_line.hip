@@ -3516,7 +3516,7 @@ __global__ __launch_bounds__(256) void k_rt_fit(RtArgs A) {
         const double tau_i = i == 0 ? 0.0 : tau1 - 0.5 * (u1 + u_i) * dl;
         const double S_i = planck(i);
         double dE_i;
-        const double E_i = expint2_korg(tau_i, i == 0 ? nullptr : &dE_i), x_i = exp(-tau_i);
+        const double E_i = expint2_korg(tau_i, &dE_i), x_i = exp(-tau_i);
         if (i == 0) dE_i = 0.0;  // tau_0 = 0 is fixed: its derivative is never used
         const double idt = 1.0 / (tau1 - tau_i);
         const double m = (S1 - S_i) * idt, b = S_i - m * tau_i;
