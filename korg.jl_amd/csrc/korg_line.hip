@@ -3790,6 +3790,60 @@ __global__ __launch_bounds__(WAVE) void k_g_ffmom(int n_stars, int L, int n_lblo
     for (int m = 0; m < FF_TERMS; ++m) dst[m * WAVE] = l < L ? (float)ldexp(acc[m], -e) : 0.0f;
 }
 
+// The whole super-tiles' moments of G from their sub-tiles' (k_g_ffmom, same layout and scale):
+// tau_S = a tau_s + b with a = h_s / h_S and b = (c_s - c_S) / h_S, so the super-tile's moment m is
+// sum_s sum_i binom(m, i) b^(m - i) a^i g_s,i.  One wavefront per (super-tile, layer block), lanes
+// = the stored positions; super-tiles that are not whole get zeros.
+__global__ __launch_bounds__(WAVE) void k_g_ffmom_super(int n_lblocks, int64_t n_sub, int64_t n_super,
+                                                        const FfGeo* __restrict__ geo,
+                                                        const FfGeo* __restrict__ sgeo,
+                                                        const float* __restrict__ gff,
+                                                        float* __restrict__ gffs) {
+    const int lane = threadIdx.x;
+    const int64_t item = blockIdx.x;
+    const int64_t sp = item / n_lblocks;
+    const int lblk = (int)(item % n_lblocks);
+    if (sp >= n_super) return;
+    const FfGeo GS = sgeo[sp];
+    double acc[FF_TERMS];
+#pragma unroll
+    for (int m = 0; m < FF_TERMS; ++m) acc[m] = 0.0;
+    if (GS.first <= GS.last) {
+        const double ih = 1.0 / (double)GS.h_u;
+        for (int k = 0; k < FF_NS; ++k) {
+            const int64_t g = sp * FF_NS + k;
+            const double a = (double)geo[g].h_u * ih, b = (geo[g].c_u - GS.c_u) * ih;
+            const float* __restrict__ src = gff + ((int64_t)lblk * n_sub + g) * FF_TERMS * WAVE + lane;
+            double c[FF_TERMS];
+            double ai = 1.0;
+#pragma unroll
+            for (int i = 0; i < FF_TERMS; ++i) {
+                c[i] = ai * (double)src[i * WAVE];
+                ai *= a;
+            }
+            // v[i] = binom(m, i) b^(m - i), advanced m by m (v_(m+1)[i] = b v_m[i] + v_m[i - 1])
+            double v[FF_TERMS];
+#pragma unroll
+            for (int i = 0; i < FF_TERMS; ++i) v[i] = i == 0 ? 1.0 : 0.0;
+#pragma unroll
+            for (int m = 0; m < FF_TERMS; ++m) {
+                if (m > 0) {
+#pragma unroll
+                    for (int i = m; i > 0; --i) v[i] = b * v[i] + v[i - 1];
+                    v[0] *= b;
+                }
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i <= m; ++i) t += v[i] * c[i];
+                acc[m] += t;
+            }
+        }
+    }
+    float* __restrict__ dst = gffs + ((int64_t)lblk * n_super + sp) * FF_TERMS * WAVE + lane;
+#pragma unroll
+    for (int m = 0; m < FF_TERMS; ++m) dst[m * WAVE] = (float)acc[m];
+}
+
 struct MomArgs {
     int n_stars, L, n_lblocks;
     int64_t n_total;           // global grid points
@@ -5835,16 +5889,16 @@ int enqueue_moments(korg_ctx* ctx, Workspace& ws, const Batch& b, hipStream_t s)
                        (const StarDev*)ws.stars.p, (const int*)ws.gexp.p, (float*)ws.g32.p,
                        (double*)ws.lam_s.p);
     if (b.ff && b.n_sub > 0) {
-        // the sub-tiles' and the whole super-tiles' moments of G for the far-field steps
+        // the sub-tiles' moments of G for the far-field steps, and from them the whole super-tiles'
         hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.n_sub * b.n_lblocks)), dim3(WAVE), 0, s,
                            b.n_stars, b.L, b.n_lblocks, b.n_sub, (const StarDev*)ws.stars.p,
                            (const FfGeo*)ws.ffgeo.p, (const int*)ws.gexp.p, (float*)ws.gff.p);
         hipLaunchKernelGGL(k_ff_sgeo, dim3((unsigned)((b.nsuper + 255) / 256)), dim3(256), 0, s,
                            (int64_t)b.nsuper, b.n_sub, b.n_stars, (const StarDev*)ws.stars.p,
                            (const FfGeo*)ws.ffgeo.p, (FfGeo*)ws.ffsgeo.p);
-        hipLaunchKernelGGL(k_g_ffmom, dim3((unsigned)(b.nsuper * b.n_lblocks)), dim3(WAVE), 0, s,
-                           b.n_stars, b.L, b.n_lblocks, (int64_t)b.nsuper, (const StarDev*)ws.stars.p,
-                           (const FfGeo*)ws.ffsgeo.p, (const int*)ws.gexp.p, (float*)ws.gffs.p);
+        hipLaunchKernelGGL(k_g_ffmom_super, dim3((unsigned)(b.nsuper * b.n_lblocks)), dim3(WAVE), 0,
+                           s, b.n_lblocks, b.n_sub, (int64_t)b.nsuper, (const FfGeo*)ws.ffgeo.p,
+                           (const FfGeo*)ws.ffsgeo.p, (const float*)ws.gff.p, (float*)ws.gffs.p);
     }
     const int4* azq = b.L <= WAVE ? (const int4*)ws.azq.p : nullptr;
     hipLaunchKernelGGL(k_adj_count, dim3(b.nblocks), dim3(PART_BLOCK), 0, s, b.N,
